@@ -1,0 +1,151 @@
+/* lizard_amd: LIZARD's receive hot path on an MI355X (gfx950), behind liblizard's decode API.
+ *
+ * Two layers of C ABI, plain pointers and sizes only:
+ *   1. liblizard's own decode API (liblizard/include/lizard.h:125-155): the same symbols, struct layouts, pixel formats
+ *      and error codes, so a caller of the reference links against this library unchanged.
+ *   2. lizx_*: the reference's internal seams of the same path, which a GPU half plugs into (lizard_finish_ext,
+ *      liblizard/src/lizard_phy.h:236-240; ldpc_decode_stall, liblizard/src/ldpc.h:47-49), and their batched forms.
+ *
+ * Errors: a function returns LIZ_OK (0) or a count, or a negative LIZ_E_*; one returning a handle returns NULL. The
+ * calling thread's last failure is liz_last_error() / liz_last_error_code() (liblizard/api/api.cpp:16-36). No C++
+ * exception crosses this boundary. A handle is used from one thread at a time; different handles from different
+ * threads at once (liblizard/include/lizard.h:11-13). Each handle owns one HIP stream on its device.
+ */
+#ifndef LIZARD_AMD_H
+#define LIZARD_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LIZ_API __attribute__((visibility("default")))
+
+/* ---- liblizard's decode API (liblizard/include/lizard.h) -------------------------------------------------------- */
+
+enum {
+  LIZ_OK = 0,
+  LIZ_E_ARG = -1,
+  LIZ_E_NOMEM = -2,
+  LIZ_E_IO = -3,
+  LIZ_E_UNSUPPORTED = -4,
+  LIZ_E_FORMAT = -5,
+  LIZ_E_TOOBIG = -6,
+  LIZ_E_STATE = -7,
+  LIZ_E_INTERNAL = -8
+}; /* lizard.h:50-60 */
+
+enum { LIZ_BLOCK = 476, LIZ_ID_BYTES = 4, LIZ_PAYLOAD = 472, LIZ_MIN_BLOCKS = 4, LIZ_MAX_BLOCKS = 64 }; /* lizard.h:62-73 */
+
+typedef enum { LIZ_GREY8 = 1, LIZ_RGBX8 = 2, LIZ_RGBA8 = 3, LIZ_BGRA8 = 4 } liz_pixfmt; /* lizard.h:74-79 */
+
+/* lizard.h:132-148, field for field */
+typedef struct {
+  int found;
+  int ring;
+  int n;
+  int word;
+  int blocks;
+  int fps;
+  int held_used;
+  int total;
+  int verified;
+  float quad[8];
+  int pilot_blocks;
+  float pilot_r[2];
+  float pilot_sd[2];
+} liz_decoded;
+
+typedef struct liz_decoder liz_decoder;
+
+/* Replaces lizard.h:17 liz_version. */
+LIZ_API const char *liz_version(void);
+/* Replace lizard.h:41-43 (api/api.cpp:33-34): the calling thread's last failure. */
+LIZ_API const char *liz_last_error(void);
+LIZ_API int liz_last_error_code(void);
+/* Replaces lizard.h:127 (api/codec.cpp:101-114). nmax 384..1536, 0 = 1536. The handle owns a HIP stream and the
+   device buffers for the largest picture; NULL and LIZ_E_UNSUPPORTED where no GPU is usable. */
+LIZ_API liz_decoder *liz_decoder_new(int nmax);
+/* Replaces lizard.h:128. */
+LIZ_API void liz_decoder_free(liz_decoder *d);
+/* Replaces lizard.h:130 (api/codec.cpp:118). */
+LIZ_API int liz_decoder_max_blocks(const liz_decoder *d);
+/* Replaces lizard.h:154 (api/codec.cpp:121-160). The finder (registration estimation) is not part of this library:
+   a frame decodes through the registration held by lizx_decoder_hold_registration, the seam the reference's own
+   split decode uses (liblizard/src/lizard_phy.h:193-199, lizard_held_reg lizard_phy.c:1899-1900). With none held it
+   returns LIZ_E_STATE. */
+LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stride, liz_pixfmt fmt, int *held,
+                       uint8_t *verified, liz_decoded *out);
+
+/* ---- lizx: the path's seams ------------------------------------------------------------------------------------- */
+
+/* Per-block outcome codes, as the reference's lizard_block_stats (liblizard/src/lizard_phy.h:188-190): iterations
+   used (> 0), LIZX_ITS_CAP (-1, never converged), LIZX_ITS_DECLINED (-2, decline gate), LIZX_ITS_STALL (-3, stall
+   rule), 0 not tried. */
+enum { LIZX_ITS_CAP = -1, LIZX_ITS_DECLINED = -2, LIZX_ITS_STALL = -3 };
+
+/* Format facts for a version (4..64; LIZARD-16V): info[0..5] = n, subch, blocks, tiers, vblocks, slots used. Host
+   only, no device. Returns 0 or LIZ_E_FORMAT. Mirrors lizard_init's layout (liblizard/src/lizard_phy.c:485-578). */
+LIZ_API int lizx_format_info(int version, int *info);
+
+/* lizard_finish_ext with ext_llr (liblizard/src/lizard_phy.c:1566, :1753-1871): the frame's soft values made
+   elsewhere: llr = 16 * version * 640 int8 in frame slot order (whitened), est = one information estimate per
+   sub-channel. Decline gate, bit map, layered min-sum LDPC (30 iterations, stall 9 / 0.95), bit pack and CRC-32C on
+   the GPU. Host buffers in: blocks (blocks * 476 B, every block's decoded bytes), ok (1 where the CRC verified), its
+   (per block, codes above), blk_est (per block). Returns the verified count. */
+LIZ_API int lizx_finish_llr(liz_decoder *d, int version, const int8_t *llr, const float *est, uint8_t *blocks,
+                            uint8_t *ok, int8_t *its, float *blk_est);
+
+/* ---- lizx_ldpc: the LDPC decoder alone, batched ----------------------------------------------------------------- */
+
+typedef struct lizx_ldpc lizx_ldpc;
+
+/* Rate indices as liblizard/src/ldpc.h:31-33: 6 (7/8), 4 (3/4), 3 (2/3), 2 (1/2). dims (12 ints): n, k, m, z, mb,
+   kb, np, nt, edges, slots_total, message bytes, posterior bytes. Host only. Mirrors ldpc_tables (ldpc.c:1177). */
+LIZ_API int lizx_ldpc_dims(int rate, int *dims);
+/* A decoder on a HIP device (-1: the current one), with its own stream. NULL on failure. */
+LIZ_API lizx_ldpc *lizx_ldpc_new(int device);
+LIZ_API void lizx_ldpc_free(lizx_ldpc *h);
+
+/* Flags for the decode calls. */
+enum {
+  LIZX_LDPC_CRC = 1,     /* also pack bits 0..3807 MSB-first into 476 bytes and test CRC-32C against bits 3808..3839 */
+  LIZX_LDPC_TRACE = 2,   /* also write every codeword's final posteriors and messages (trace buffers) */
+  LIZX_LDPC_PLAIN = 4    /* one launch per codeword (the plain per-codeword path) instead of one persistent launch */
+};
+
+/* ldpc_decode_stall (liblizard/src/ldpc.h:49, ldpc.c:1080) over count codewords at once, host memory. Codeword i has
+   rate index rate[i]; its n int8 LLRs (codeword order, > 0 means bit 0, the unsent bits as the caller knows them)
+   start at llr + off[i]; its n hard decisions (0/1 a byte) go to bits + off[i]. its[i]: iterations, -1 or -3.
+   max_iter >= 1; stall_it 0 disables the stall rule. With LIZX_LDPC_CRC, bytes[i * 476] and crc_ok[i] are written.
+   With LIZX_LDPC_TRACE, post[off[i]] (int16, n per codeword, in the reference decoder's workspace order: data bits in
+   code order, parity as [block row][position]) and msg[i * 26880] (int8, slots_total * 128: [layer slot][position])
+   are written. Returns the count of codewords whose syndrome cleared, or a negative LIZ_E_*. */
+LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const int8_t *llr, const int64_t *off,
+                             uint8_t *bits, int8_t *its, int max_iter, int stall_it, float stall_ratio, int flags,
+                             uint8_t *bytes, uint8_t *crc_ok, int16_t *post, int8_t *msg);
+
+/* The same on device memory already resident in HBM, enqueued on the handle's stream (no host synchronisation):
+   every pointer is a device pointer. Timing of the last device call: lizx_ldpc_last_ms (HIP events around the
+   decode kernels, on the handle's stream). */
+LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr,
+                                    const int64_t *d_off, uint8_t *d_bits, int8_t *d_its, int max_iter, int stall_it,
+                                    float stall_ratio, int flags, uint8_t *d_bytes, uint8_t *d_crc_ok);
+LIZ_API int lizx_ldpc_sync(lizx_ldpc *h);
+LIZ_API float lizx_ldpc_last_ms(lizx_ldpc *h);
+
+/* Device memory helpers for callers that hold no HIP runtime of their own (tests, benches): allocate, free, copy
+   in/out (synchronous). */
+LIZ_API void *lizx_dev_alloc(size_t bytes);
+LIZ_API void lizx_dev_free(void *p);
+LIZ_API int lizx_dev_upload(void *dst, const void *src, size_t bytes);
+LIZ_API int lizx_dev_download(void *dst, const void *src, size_t bytes);
+/* Number of usable HIP devices (0 where none). */
+LIZ_API int lizx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,231 @@
+"""lizard_amd: LIZARD's receive hot path on MI355X, through its C ABI (include/lizard_amd.h).
+
+This module is a thin ctypes mirror of the C ABI for Python callers (tests, benches). Everything it computes runs in
+the native library lizard_amd/liblizard_amd.so; it raises if that library is missing rather than computing anything
+itself.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblizard_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "lizard_amd.h")
+
+LIZ_OK, LIZ_E_ARG, LIZ_E_NOMEM, LIZ_E_IO, LIZ_E_UNSUPPORTED, LIZ_E_FORMAT, LIZ_E_TOOBIG, LIZ_E_STATE, LIZ_E_INTERNAL = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8)
+LIZ_BLOCK = 476
+LDPC_CRC, LDPC_TRACE, LDPC_PLAIN = 1, 2, 4
+MSG_TRACE_STRIDE = 210 * 128
+RATE_INDEX = {"7/8": 6, "3/4": 4, "2/3": 3, "1/2": 2}
+
+_P, _I, _F, _SZ = C.c_void_p, C.c_int, C.c_float, C.c_size_t
+
+
+class LizError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d)" % (msg, code))
+        self.code = code
+
+
+class _Decoded(C.Structure):
+    _fields_ = [("found", C.c_int), ("ring", C.c_int), ("n", C.c_int), ("word", C.c_int), ("blocks", C.c_int),
+                ("fps", C.c_int), ("held_used", C.c_int), ("total", C.c_int), ("verified", C.c_int),
+                ("quad", C.c_float * 8), ("pilot_blocks", C.c_int), ("pilot_r", C.c_float * 2),
+                ("pilot_sd", C.c_float * 2)]
+
+
+_lib = None
+
+
+def lib():
+    """The native library; raises when it has not been built (python -m lizard_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("lizard_amd: %s is missing; build it with python -m lizard_amd.build" % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+
+    def sig(name, res, *args):
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = list(args)
+
+    sig("liz_version", C.c_char_p)
+    sig("liz_last_error", C.c_char_p)
+    sig("liz_last_error_code", _I)
+    sig("liz_decoder_new", _P, _I)
+    sig("liz_decoder_free", None, _P)
+    sig("liz_decoder_max_blocks", _I, _P)
+    sig("liz_decode", _I, _P, _P, _I, _I, _I, _I, _P, _P, _P)
+    sig("lizx_format_info", _I, _I, _P)
+    sig("lizx_finish_llr", _I, _P, _I, _P, _P, _P, _P, _P, _P)
+    sig("lizx_ldpc_dims", _I, _I, _P)
+    sig("lizx_ldpc_new", _P, _I)
+    sig("lizx_ldpc_free", None, _P)
+    sig("lizx_ldpc_decode", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P, _P, _P)
+    sig("lizx_ldpc_decode_device", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P)
+    sig("lizx_ldpc_sync", _I, _P)
+    sig("lizx_ldpc_last_ms", _F, _P)
+    sig("lizx_dev_alloc", _P, _SZ)
+    sig("lizx_dev_free", None, _P)
+    sig("lizx_dev_upload", _I, _P, _P, _SZ)
+    sig("lizx_dev_download", _I, _P, _P, _SZ)
+    sig("lizx_device_count", _I)
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def check(rc):
+    if rc is None or (isinstance(rc, int) and rc < 0):
+        L = lib()
+        raise LizError(L.liz_last_error_code(), L.liz_last_error().decode(errors="replace"))
+    return rc
+
+
+def device_count():
+    return lib().lizx_device_count()
+
+
+def format_info(version):
+    info = np.zeros(6, np.int32)
+    check(lib().lizx_format_info(version, _p(info)))
+    return dict(zip(("n", "subch", "blocks", "tiers", "vblocks", "used"), map(int, info)))
+
+
+def ldpc_dims(rate):
+    d = np.zeros(12, np.int32)
+    check(lib().lizx_ldpc_dims(rate, _p(d)))
+    return dict(zip(("n", "k", "m", "z", "mb", "kb", "np", "nt", "edges", "slots_total", "msg_bytes", "post_bytes"),
+                    map(int, d)))
+
+
+class DeviceArray:
+    """A device buffer owned through the library's allocator (no HIP runtime needed in Python)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().lizx_dev_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            check(-1)
+
+    @classmethod
+    def of(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        d = cls(arr.nbytes)
+        check(lib().lizx_dev_upload(d.ptr, _p(arr), arr.nbytes))
+        return d
+
+    def get(self, dtype, count):
+        out = np.empty(count, dtype)
+        check(lib().lizx_dev_download(_p(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().lizx_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class LdpcDecoder:
+    """Batched layered min-sum LDPC (lizx_ldpc_*): ldpc_decode_stall over many codewords of mixed rates."""
+
+    def __init__(self, device=-1):
+        self.h = lib().lizx_ldpc_new(device)
+        if not self.h:
+            check(-1)
+
+    def close(self):
+        if self.h:
+            lib().lizx_ldpc_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def decode(self, rates, llr, offsets, max_iter=30, stall_it=9, stall_ratio=0.95, crc=False, trace=False,
+               plain=False, bits=True):
+        rates = np.ascontiguousarray(rates, np.uint8)
+        llr = np.ascontiguousarray(llr, np.int8)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        count = len(rates)
+        its = np.zeros(count, np.int8)
+        out_bits = np.zeros(llr.size, np.uint8) if bits else None
+        flags = (LDPC_CRC if crc else 0) | (LDPC_TRACE if trace else 0) | (LDPC_PLAIN if plain else 0)
+        by = np.zeros(count * LIZ_BLOCK, np.uint8) if crc else None
+        ok = np.zeros(count, np.uint8) if crc else None
+        post = np.zeros(llr.size, np.int16) if trace else None
+        msg = np.zeros(count * MSG_TRACE_STRIDE, np.int8) if trace else None
+        check(lib().lizx_ldpc_decode(self.h, count, _p(rates), _p(llr), _p(offsets), _p(out_bits), _p(its), max_iter,
+                                     stall_it, stall_ratio, flags, _p(by), _p(ok), _p(post), _p(msg)))
+        res = {"its": its, "bits": out_bits}
+        if crc:
+            res["bytes"] = by.reshape(count, LIZ_BLOCK)
+            res["crc_ok"] = ok
+        if trace:
+            res["post"] = post
+            res["msg"] = msg.reshape(count, MSG_TRACE_STRIDE)
+        return res
+
+    def decode_device(self, count, d_rate, d_llr, d_off, d_bits, d_its, max_iter=20, stall_it=0, stall_ratio=0.95,
+                      crc=False, plain=False, d_bytes=None, d_crc_ok=None):
+        flags = (LDPC_CRC if crc else 0) | (LDPC_PLAIN if plain else 0)
+        ptr = lambda d: d.ptr if d is not None else None  # noqa: E731
+        check(lib().lizx_ldpc_decode_device(self.h, count, ptr(d_rate), ptr(d_llr), ptr(d_off), ptr(d_bits),
+                                            ptr(d_its), max_iter, stall_it, stall_ratio, flags, ptr(d_bytes),
+                                            ptr(d_crc_ok)))
+
+    def sync(self):
+        check(lib().lizx_ldpc_sync(self.h))
+        return float(lib().lizx_ldpc_last_ms(self.h))
+
+
+class Decoder:
+    """liblizard's decoder handle (liz_decoder_*) and the finish seams."""
+
+    def __init__(self, nmax=0):
+        self.h = lib().liz_decoder_new(nmax)
+        if not self.h:
+            check(-1)
+
+    def close(self):
+        if self.h:
+            lib().liz_decoder_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def max_blocks(self):
+        return check(lib().liz_decoder_max_blocks(self.h))
+
+    def finish_llr(self, version, llr, est):
+        """lizard_finish_ext with soft values made elsewhere: per-block verdicts, iterations, estimates, bytes."""
+        info = format_info(version)
+        B = info["blocks"]
+        llr = np.ascontiguousarray(llr, np.int8)
+        est = np.ascontiguousarray(est, np.float32)
+        blocks = np.zeros(B * LIZ_BLOCK, np.uint8)
+        ok = np.zeros(B, np.uint8)
+        its = np.zeros(B, np.int8)
+        be = np.zeros(B, np.float32)
+        good = check(lib().lizx_finish_llr(self.h, version, _p(llr), _p(est), _p(blocks), _p(ok), _p(its), _p(be)))
+        return {"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be}

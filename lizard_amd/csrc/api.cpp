@@ -1,0 +1,388 @@
+// The C ABI (include/lizard_amd.h): liblizard's decode API and the lizx_* seams, over the HIP stages.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "format.hpp"
+#include "ldpc.hpp"
+
+#define LIZARD_AMD_VERSION "lizard-amd 0.1 (gfx950)"
+
+namespace lz {
+
+// The calling thread's last failure (liblizard/api/api.cpp:16-30): a fixed array, so no thread_local destructor.
+static thread_local char last_error[1024];
+static thread_local int last_code = LIZ_OK;
+
+int fail(int code, const std::string &message) {
+  const size_t n = message.size() < sizeof last_error ? message.size() : sizeof last_error - 1;
+  std::memcpy(last_error, message.data(), n);
+  last_error[n] = '\0';
+  last_code = code;
+  return code;
+}
+void clear_error() {
+  last_error[0] = '\0';
+  last_code = LIZ_OK;
+}
+
+static int current_device(int want) {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw Error(LIZ_E_UNSUPPORTED, "no HIP device");
+  int dev = want;
+  if (dev < 0) hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev >= count) throw Error(LIZ_E_ARG, "no such device");
+  hip_check(hipSetDevice(dev), "hipSetDevice");
+  return dev;
+}
+
+// The most blocks a frame of a picture up to nmax carries (liblizard/api/codec.cpp:108-111 through cpu_dec_top).
+static int top_blocks(int nmax) {
+  int top = 0;
+  for (int v = 4; v <= 64; v++) {
+    const Format *f = format_for(v);
+    if (f && f->n <= nmax && f->blocks > top) top = f->blocks;
+  }
+  return top;
+}
+
+}  // namespace lz
+
+using namespace lz;
+
+struct lizx_ldpc {
+  std::unique_ptr<LdpcEngine> eng;
+  DevBuf tasks, rate_dev, llr, bits, its, bytes, crc_ok, post, msg, off;
+  std::vector<LdpcTask> host_tasks;
+};
+
+struct liz_decoder {
+  int nmax = 0, top = 0;
+  std::unique_ptr<LdpcEngine> eng;
+  DevBuf tasks, llr, est, its, blk_est, bytes, crc_ok;
+  std::vector<LdpcTask> host_tasks;
+  std::vector<uint8_t> blocks, ok;
+};
+
+// Builds the task list for count codewords in codeword mode.
+static void codeword_tasks(int count, const uint8_t *rate, const int64_t *off, std::vector<LdpcTask> &t) {
+  t.resize(count);
+  for (int i = 0; i < count; i++) {
+    const int ci = code_index(rate[i]);
+    if (ci < 0) throw Error(LIZ_E_ARG, "a rate index with no code (6, 4, 3 or 2)");
+    need(off[i] >= 0, "a negative offset");
+    t[i] = LdpcTask{off[i], off[i], ci, 0, 0, i};
+  }
+}
+
+static void check_params(int max_iter, int stall_it, float ratio) {
+  need(max_iter >= 1 && max_iter <= 127, "max_iter is 1 to 127");
+  need(stall_it >= 0 && stall_it <= 127, "stall_it is 0 to 127");
+  need(ratio == ratio, "stall_ratio is a number");
+}
+
+extern "C" {
+
+LIZ_API const char *liz_version(void) { return LIZARD_AMD_VERSION; }
+LIZ_API const char *liz_last_error(void) { return last_error; }
+LIZ_API int liz_last_error_code(void) { return last_code; }
+
+LIZ_API int lizx_device_count(void) {
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+LIZ_API void *lizx_dev_alloc(size_t bytes) {
+  void *p = nullptr;
+  guard([&] {
+    hip_check(hipMalloc(&p, bytes ? bytes : 1), "hipMalloc");
+    return 0;
+  });
+  return p;
+}
+LIZ_API void lizx_dev_free(void *p) {
+  if (p) (void)hipFree(p);
+}
+LIZ_API int lizx_dev_upload(void *dst, const void *src, size_t bytes) {
+  return guard([&] {
+    hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload");
+    return 0;
+  });
+}
+LIZ_API int lizx_dev_download(void *dst, const void *src, size_t bytes) {
+  return guard([&] {
+    hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "download");
+    return 0;
+  });
+}
+
+LIZ_API int lizx_format_info(int version, int *info) {
+  return guard([&] {
+    need(info, "a place for the info");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    info[0] = f->n;
+    info[1] = f->subch;
+    info[2] = f->blocks;
+    info[3] = f->tiers;
+    info[4] = f->vblocks;
+    info[5] = f->used;
+    return 0;
+  });
+}
+
+LIZ_API int lizx_ldpc_dims(int rate, int *dims) {
+  return guard([&] {
+    need(dims, "a place for the dims");
+    const int ci = code_index(rate);
+    if (ci < 0) throw Error(LIZ_E_ARG, "a rate index with no code (6, 4, 3 or 2)");
+    const Code &c = code(ci);
+    const int d[12] = {c.n, c.k, c.m, Z, c.mb, c.kb, c.np, c.nt, c.edges, c.slots_total, c.slots_total * Z,
+                       c.n * 2};
+    std::memcpy(dims, d, sizeof d);
+    return 0;
+  });
+}
+
+LIZ_API lizx_ldpc *lizx_ldpc_new(int device) {
+  return guard_new<lizx_ldpc>([&] {
+    const int dev = current_device(device);
+    auto h = std::make_unique<lizx_ldpc>();
+    h->eng = std::make_unique<LdpcEngine>(dev);
+    return h.release();
+  });
+}
+LIZ_API void lizx_ldpc_free(lizx_ldpc *h) { delete h; }
+
+LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr,
+                                    const int64_t *d_off, uint8_t *d_bits, int8_t *d_its, int max_iter, int stall_it,
+                                    float stall_ratio, int flags, uint8_t *d_bytes, uint8_t *d_crc_ok) {
+  return guard([&] {
+    need(h && count >= 0 && d_rate && d_llr && d_off && d_its, "a decoder and device arrays");
+    check_params(max_iter, stall_it, stall_ratio);
+    need(!(flags & LIZX_LDPC_CRC) || (d_bytes && d_crc_ok), "CRC outputs");
+    need(!(flags & LIZX_LDPC_TRACE), "traces are a host-call option");
+    // the task list is built on the host from the rates and offsets (a small download), then kept on the device
+    std::vector<uint8_t> rate(count);
+    std::vector<int64_t> off(count);
+    hip_check(hipSetDevice(h->eng->device), "hipSetDevice");
+    hip_check(hipMemcpy(rate.data(), d_rate, count, hipMemcpyDeviceToHost), "rates");
+    hip_check(hipMemcpy(off.data(), d_off, count * sizeof(int64_t), hipMemcpyDeviceToHost), "offsets");
+    codeword_tasks(count, rate.data(), off.data(), h->host_tasks);
+    h->tasks.reserve(count * sizeof(LdpcTask));
+    hip_check(hipMemcpyAsync(h->tasks.p, h->host_tasks.data(), count * sizeof(LdpcTask), hipMemcpyHostToDevice,
+                             h->eng->stream),
+              "tasks");
+    LdpcParams p{};
+    p.tasks = h->tasks.as<LdpcTask>();
+    p.count = count;
+    p.llr = d_llr;
+    p.bits = d_bits;
+    p.its = d_its;
+    p.bytes = d_bytes;
+    p.crc_ok = d_crc_ok;
+    p.crc_tab = h->eng->crc_tab.as<uint32_t>();
+    p.crc0 = h->eng->crc0;
+    p.max_iter = max_iter;
+    p.stall_it = stall_it;
+    p.stall_ratio = stall_ratio;
+    p.mode = MODE_CODEWORD;
+    p.flags = flags & LIZX_LDPC_CRC;
+    h->eng->launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_ldpc_sync(lizx_ldpc *h) {
+  return guard([&] {
+    need(h, "a decoder");
+    hip_check(hipStreamSynchronize(h->eng->stream), "decode");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, h->eng->ev0, h->eng->ev1) == hipSuccess) h->eng->last_ms = ms;
+    return 0;
+  });
+}
+
+LIZ_API float lizx_ldpc_last_ms(lizx_ldpc *h) { return h ? h->eng->last_ms : -1.0f; }
+
+LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const int8_t *llr, const int64_t *off,
+                             uint8_t *bits, int8_t *its, int max_iter, int stall_it, float stall_ratio, int flags,
+                             uint8_t *bytes, uint8_t *crc_ok, int16_t *post, int8_t *msg) {
+  return guard([&] {
+    need(h && count >= 0 && rate && llr && off && its, "a decoder, rates, LLRs, offsets and iteration outputs");
+    check_params(max_iter, stall_it, stall_ratio);
+    const bool crc = flags & LIZX_LDPC_CRC, trace = flags & LIZX_LDPC_TRACE;
+    need(!crc || (bytes && crc_ok), "CRC outputs");
+    need(!trace || (post && msg), "trace outputs");
+    if (!count) return 0;
+    codeword_tasks(count, rate, off, h->host_tasks);
+    int64_t span = 0;
+    for (int i = 0; i < count; i++) span = std::max<int64_t>(span, off[i] + code(h->host_tasks[i].code).n);
+    LdpcEngine &e = *h->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    h->tasks.reserve(count * sizeof(LdpcTask));
+    h->llr.reserve(span);
+    h->its.reserve(count);
+    if (bits) h->bits.reserve(span);
+    if (crc) {
+      h->bytes.reserve(static_cast<size_t>(count) * BLOCK_BYTES);
+      h->crc_ok.reserve(count);
+    }
+    if (trace) {
+      h->post.reserve(span * sizeof(int16_t));
+      h->msg.reserve(static_cast<size_t>(count) * MSG_TRACE_STRIDE);
+    }
+    hip_check(hipMemcpyAsync(h->tasks.p, h->host_tasks.data(), count * sizeof(LdpcTask), hipMemcpyHostToDevice,
+                             e.stream),
+              "tasks");
+    hip_check(hipMemcpyAsync(h->llr.p, llr, span, hipMemcpyHostToDevice, e.stream), "LLRs");
+    LdpcParams p{};
+    p.tasks = h->tasks.as<LdpcTask>();
+    p.count = count;
+    p.llr = h->llr.as<int8_t>();
+    p.bits = bits ? h->bits.as<uint8_t>() : nullptr;
+    p.its = h->its.as<int8_t>();
+    p.bytes = crc ? h->bytes.as<uint8_t>() : nullptr;
+    p.crc_ok = crc ? h->crc_ok.as<uint8_t>() : nullptr;
+    p.post = trace ? h->post.as<int16_t>() : nullptr;
+    p.msg = trace ? h->msg.as<int8_t>() : nullptr;
+    p.crc_tab = e.crc_tab.as<uint32_t>();
+    p.crc0 = e.crc0;
+    p.max_iter = max_iter;
+    p.stall_it = stall_it;
+    p.stall_ratio = stall_ratio;
+    p.mode = MODE_CODEWORD;
+    p.flags = (crc ? 1 : 0) | (trace ? 2 : 0);
+    e.launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true);
+    hip_check(hipStreamSynchronize(e.stream), "decode");
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, e.ev0, e.ev1) == hipSuccess) e.last_ms = ms;
+    hip_check(hipMemcpy(its, h->its.p, count, hipMemcpyDeviceToHost), "iterations");
+    if (bits) {
+      // only the codewords' own ranges: the caller's buffer between them is left as it was
+      std::vector<uint8_t> all(span);
+      hip_check(hipMemcpy(all.data(), h->bits.p, span, hipMemcpyDeviceToHost), "bits");
+      for (int i = 0; i < count; i++) std::memcpy(bits + off[i], all.data() + off[i], code(h->host_tasks[i].code).n);
+    }
+    if (crc) {
+      hip_check(hipMemcpy(bytes, h->bytes.p, static_cast<size_t>(count) * BLOCK_BYTES, hipMemcpyDeviceToHost), "bytes");
+      hip_check(hipMemcpy(crc_ok, h->crc_ok.p, count, hipMemcpyDeviceToHost), "CRC verdicts");
+    }
+    if (trace) {
+      std::vector<int16_t> all(span);
+      hip_check(hipMemcpy(all.data(), h->post.p, span * sizeof(int16_t), hipMemcpyDeviceToHost), "posteriors");
+      for (int i = 0; i < count; i++)
+        std::memcpy(post + off[i], all.data() + off[i], code(h->host_tasks[i].code).n * sizeof(int16_t));
+      hip_check(hipMemcpy(msg, h->msg.p, static_cast<size_t>(count) * MSG_TRACE_STRIDE, hipMemcpyDeviceToHost),
+                "messages");
+    }
+    int cleared = 0;
+    for (int i = 0; i < count; i++) cleared += its[i] > 0;
+    return cleared;
+  });
+}
+
+LIZ_API liz_decoder *liz_decoder_new(int nmax) {
+  return guard_new<liz_decoder>([&] {
+    if (!nmax) nmax = 1536;
+    need(nmax >= 384 && nmax <= 1536, "nmax is 384 to 1536 (or 0)");
+    const int dev = current_device(-1);
+    auto d = std::make_unique<liz_decoder>();
+    d->nmax = nmax;
+    d->top = top_blocks(nmax);
+    d->eng = std::make_unique<LdpcEngine>(dev);
+    return d.release();
+  });
+}
+
+LIZ_API void liz_decoder_free(liz_decoder *d) { delete d; }
+
+LIZ_API int liz_decoder_max_blocks(const liz_decoder *d) {
+  return guard([&] {
+    need(d, "a decoder");
+    return d->top;
+  });
+}
+
+LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stride, liz_pixfmt fmt, int *held,
+                       uint8_t *verified, liz_decoded *out) {
+  return guard([&]() -> int {
+    need(d && px && held && verified, "a decoder, an image, the held word and a place for the blocks");
+    need(w > 0 && h > 0, "an image of at least a pixel");
+    need(fmt == LIZ_GREY8 || fmt == LIZ_RGBX8 || fmt == LIZ_RGBA8 || fmt == LIZ_BGRA8, "a pixel format of lizard.h's");
+    need(stride >= w * (fmt == LIZ_GREY8 ? 1 : 4), "a stride of at least the image's width in bytes");
+    if (out) *out = liz_decoded{};
+    throw Error(LIZ_E_STATE, "no registration held: this library has no finder; hold one first");
+  });
+}
+
+LIZ_API int lizx_finish_llr(liz_decoder *d, int version, const int8_t *llr, const float *est, uint8_t *blocks,
+                            uint8_t *ok, int8_t *its, float *blk_est) {
+  return guard([&] {
+    need(d && llr && est && blocks && ok && its && blk_est, "a decoder, soft values, estimates and outputs");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    if (f->n > d->nmax) throw Error(LIZ_E_FORMAT, "the format's picture is larger than the decoder's nmax");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    e.ensure_frame_tables(static_cast<int>(f->white.size()));
+    const int S = f->subch * 640, B = f->blocks;
+    d->host_tasks.resize(B);
+    for (int b = 0; b < B; b++) d->host_tasks[b] = LdpcTask{f->blk_slot[b], -1, f->blk_code[b], f->blk_slot[b], 0, b};
+    std::vector<double> est_d(f->subch);
+    for (int j = 0; j < f->subch; j++) est_d[j] = est[j];
+    d->tasks.reserve(B * sizeof(LdpcTask));
+    d->llr.reserve(S);
+    d->est.reserve(f->subch * sizeof(double));
+    d->its.reserve(B);
+    d->blk_est.reserve(B * sizeof(float));
+    d->bytes.reserve(static_cast<size_t>(B) * BLOCK_BYTES);
+    d->crc_ok.reserve(B);
+    hip_check(hipMemcpyAsync(d->tasks.p, d->host_tasks.data(), B * sizeof(LdpcTask), hipMemcpyHostToDevice, e.stream),
+              "tasks");
+    hip_check(hipMemcpyAsync(d->llr.p, llr, S, hipMemcpyHostToDevice, e.stream), "soft values");
+    hip_check(hipMemcpyAsync(d->est.p, est_d.data(), f->subch * sizeof(double), hipMemcpyHostToDevice, e.stream),
+              "estimates");
+    LdpcParams p{};
+    p.tasks = d->tasks.as<LdpcTask>();
+    p.count = B;
+    p.llr = d->llr.as<int8_t>();
+    p.its = d->its.as<int8_t>();
+    p.blk_est = d->blk_est.as<float>();
+    p.bytes = d->bytes.as<uint8_t>();
+    p.crc_ok = d->crc_ok.as<uint8_t>();
+    p.white = e.white.as<uint8_t>();
+    for (int ci = 0; ci < NCODES; ci++) p.s2p[ci] = e.s2p[ci].as<int16_t>();
+    p.est = d->est.as<double>();
+    p.crc_tab = e.crc_tab.as<uint32_t>();
+    p.crc0 = e.crc0;
+    p.max_iter = 30;  // lizard_phy.c:1824 with LIZARD_STALL_IT, LIZARD_STALL_RATIO (:81-82)
+    p.stall_it = 9;
+    p.stall_ratio = 0.95f;
+    p.mode = MODE_FRAME;
+    p.flags = 1;
+    e.launch(p, false, true);
+    hip_check(hipStreamSynchronize(e.stream), "finish");
+    hip_check(hipMemcpy(its, d->its.p, B, hipMemcpyDeviceToHost), "iterations");
+    hip_check(hipMemcpy(blk_est, d->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost), "estimates");
+    hip_check(hipMemcpy(ok, d->crc_ok.p, B, hipMemcpyDeviceToHost), "verdicts");
+    d->blocks.resize(static_cast<size_t>(B) * BLOCK_BYTES);
+    hip_check(hipMemcpy(d->blocks.data(), d->bytes.p, d->blocks.size(), hipMemcpyDeviceToHost), "blocks");
+    int good = 0;
+    for (int b = 0; b < B; b++) {
+      if (its[b] == -2) ok[b] = 0;
+      if (ok[b]) {
+        std::memcpy(blocks + static_cast<size_t>(b) * BLOCK_BYTES, d->blocks.data() + static_cast<size_t>(b) * BLOCK_BYTES,
+                    BLOCK_BYTES);
+        good++;
+      }
+    }
+    return good;
+  });
+}
+
+}  // extern "C"

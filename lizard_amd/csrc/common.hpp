@@ -1,0 +1,84 @@
+// Host-side plumbing shared by the library's translation units: the error model of the C ABI (a thread's last
+// failure, no exception across the boundary; liblizard/api/api.hpp:15-44 and api.cpp:16-36 state the same contract)
+// and HIP status checks.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+
+#include "../../include/lizard_amd.h"
+
+namespace lz {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+int fail(int code, const std::string &message);
+void clear_error();
+
+inline void need(bool ok, const char *what) {
+  if (!ok) throw Error(LIZ_E_ARG, what);
+}
+
+inline void hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return;
+  const int code = e == hipErrorOutOfMemory ? LIZ_E_NOMEM
+                   : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? LIZ_E_UNSUPPORTED
+                                                                             : LIZ_E_INTERNAL;
+  throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guard(F &&f) {
+  try {
+    clear_error();
+    return f();
+  } catch (const Error &e) {
+    return fail(e.code, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(LIZ_E_NOMEM, "out of host memory");
+  } catch (const std::exception &e) {
+    return fail(LIZ_E_INTERNAL, e.what());
+  } catch (...) {
+    return fail(LIZ_E_INTERNAL, "an unknown failure");
+  }
+}
+
+template <class T, class F>
+T *guard_new(F &&f) {
+  T *out = nullptr;
+  guard([&] {
+    out = f();
+    return 0;
+  });
+  return out;
+}
+
+// A device buffer owned by RAII; grows, never shrinks.
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (p) hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    bytes = 0;
+    hip_check(hipMalloc(&p, n), "hipMalloc");
+    bytes = n;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+}  // namespace lz

@@ -1,0 +1,69 @@
+// The LDPC stage on the GPU: task descriptors, the per-device engine that owns the code tables, and the launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+#include "format.hpp"
+#include "ldpc_dev.hpp"
+
+namespace lz {
+
+// One codeword to decode. Codeword mode: llr is the offset of its n LLRs in codeword order. Frame mode: llr is the
+// offset of its block's first slot in the frame's soft values (slot order, whitened), slot0 that slot's index in the
+// frame (the whitening), est the index of the frame's first sub-channel estimate (the decline gate).
+struct LdpcTask {
+  int64_t llr;
+  int64_t bits;   // offset of n bytes of hard decisions (and, traced, of n int16 posteriors), or -1
+  int32_t code;   // code index 0..3 (7/8, 3/4, 2/3, 1/2)
+  int32_t slot0;
+  int32_t est;
+  int32_t out;    // output record: its[out], blk_est[out], crc_ok[out], bytes[out * 476]
+};
+
+enum { MODE_CODEWORD = 0, MODE_FRAME = 1 };
+
+struct LdpcParams {
+  const LdpcTask *tasks;
+  int count;
+  int *queue;              // persistent queue counter (zero before launch), or null: one task per workgroup
+  const int8_t *llr;
+  uint8_t *bits;
+  int8_t *its;
+  float *blk_est;
+  uint8_t *bytes;
+  uint8_t *crc_ok;
+  int16_t *post;
+  int8_t *msg;
+  const uint8_t *white;    // frame mode: whitening bit per frame slot
+  const int16_t *s2p[NCODES];  // frame mode: slot -> posterior index per code
+  const double *est;       // frame mode: sub-channel estimates
+  const uint32_t *crc_tab; // 3808 per-bit CRC words
+  uint32_t crc0;
+  int max_iter, stall_it;
+  float stall_ratio;
+  int mode, flags;
+};
+
+constexpr int LDPC_THREADS = 128;
+constexpr size_t MSG_TRACE_STRIDE = MAX_SLOTS_TOTAL * Z;  // 26880 bytes a codeword
+
+// Per-device state: the code tables in constant memory, the CRC table, frame-mode bit maps, the queue counter.
+struct LdpcEngine {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf queue, crc_tab, s2p[NCODES], white;
+  uint32_t crc0 = 0;
+  int cus = 0, wg_per_cu = 0;
+  float last_ms = 0;
+  explicit LdpcEngine(int device);
+  ~LdpcEngine();
+  // Enqueue a decode of count tasks already on the device. Plain: one launch per task.
+  void launch(const LdpcParams &p, bool plain, bool timed);
+  void ensure_frame_tables(int max_slots);
+};
+
+void fill_dev_code(const Code &c, DevCode &d);
+
+}  // namespace lz

@@ -1,0 +1,22 @@
+#!/bin/sh
+# Builds the reference decoder from its own sources, where they lie in the reference checkout, into oracle/_ref/:
+#   libref_scalar.so  gcc -DOB_SCALAR: the scalar form (the one research/SPEC.md states and wasm runs)
+#   libref_x86.so     ROCm clang -march=x86-64-v2: the native x86-64 SIMD form (liblizard/core/CMakeLists.txt:160-167)
+# Both with -ffp-contract=off as every reference target (liblizard/build.sh:15-17). Test infrastructure only.
+# Usage: oracle/ref/build_ref.sh [reference root, default /root/reference]
+set -e
+REF=${1:-/root/reference}
+SRC="$REF/liblizard/src"
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/../_ref"
+if [ ! -f "$SRC/lizard_phy.c" ]; then echo "build_ref: no reference sources at $SRC" >&2; exit 2; fi
+mkdir -p "$OUT"
+UNITS="$SRC/ldpc.c $SRC/layout.c $SRC/acquire.c $SRC/fft.c $SRC/fmt.c $SRC/rs.c"
+gcc -O2 -ffp-contract=off -DOB_SCALAR -fPIC -shared -I"$SRC" -o "$OUT/libref_scalar.so.tmp" "$HERE/ref_driver.c" $UNITS -lm
+mv "$OUT/libref_scalar.so.tmp" "$OUT/libref_scalar.so"
+CLANG=/opt/rocm/llvm/bin/clang
+if [ -x "$CLANG" ]; then
+  "$CLANG" -O2 -ffp-contract=off -march=x86-64-v2 -fPIC -shared -I"$SRC" -o "$OUT/libref_x86.so.tmp" "$HERE/ref_driver.c" $UNITS -lm
+  mv "$OUT/libref_x86.so.tmp" "$OUT/libref_x86.so"
+fi
+echo "build_ref: $(ls "$OUT")"
