@@ -1,0 +1,266 @@
+#!/usr/bin/env python3
+"""LIZARD receive hot path on one MI355X: the LDPC-only batched workload.
+
+Workload (one step = one decode of one batch): 4096 codewords with 3840 information bits each (476 random bytes and
+their CRC-32C), rates mixed 7/8 : 3/4 : 2/3 : 1/2 = 1597 : 1065 : 614 : 820 (39/26/15/20 %), BPSK on each QPSK axis
+over AWGN at Eb/N0 swept 1.5 to 4.0 dB in 0.5 dB steps (one batch a point, steps cycling through the points), LLR =
+2 y / sigma^2 quantised to int8 as round-half-even(8 LLR) clamped to +-80, the 7/8 code's 128 known zeros at +127;
+layered min-sum, at most 20 iterations with early termination, one persistent kernel per step, CRC-32C checked on
+the GPU. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line: LDPC Gbit/s (3840 x codewords / s) as the value, payload MB/s (472 x verified blocks / s),
+the plain per-codeword path's rate beside the batched one, the LDS roofline of the decode kernel and the reference
+CPU decoder timed on this machine's cores.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import threading
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import lizard_amd  # noqa: E402
+
+METRIC = "Decoded payload MB/s + LDPC Gbit/s at block-fail<1e-4; batched vs plain, 1 GPU"
+MIX = ((6, 1597), (4, 1065), (3, 614), (2, 820))
+EBN0 = (1.5, 2.0, 2.5, 3.0, 3.5, 4.0)
+MAX_ITER = 20
+# MI355X: 256 CUs at 2.4 GHz; LDS 128 bytes a clock per CU (the CDNA3-generation figure; CDNA4 documents doubled LDS
+# read bandwidth for some instructions, which would halve the fraction below). HBM3E 8 TB/s.
+LDS_PEAK_GBS = 256 * 128 * 2.4
+HBM_PEAK_GBS = 8000.0
+
+
+def build_batch(ebn0_db, seed):
+    """4096 codewords at one Eb/N0: rates in a seeded interleaved order; returns rates, int8 LLRs, offsets, payload
+    bytes and the sent codewords' information bits."""
+    rng = np.random.default_rng(seed)
+    rates = np.concatenate([np.full(c, r, np.uint8) for r, c in MIX])
+    rates = rates[rng.permutation(rates.size)]
+    dims = {r: lizard_amd.ldpc_dims(r) for r, _ in MIX}
+    n = np.array([dims[r]["n"] for r in rates])
+    offs = np.concatenate([[0], np.cumsum(n)[:-1]]).astype(np.int64)
+    llr = np.empty(int(n.sum()), np.int8)
+    payload = rng.integers(0, 256, (rates.size, 476), dtype=np.uint8)
+    info = np.zeros((rates.size, 3840), np.uint8)
+    for i in range(rates.size):
+        crc = lizard_amd.crc32c(payload[i])
+        info[i, :3808] = np.unpackbits(payload[i])
+        info[i, 3808:] = (crc >> (31 - np.arange(32))) & 1
+    ebn0 = 10.0 ** (ebn0_db / 10.0)
+    for r, _ in MIX:
+        d = dims[r]
+        idx = np.nonzero(rates == r)[0]
+        data = np.zeros((idx.size, d["k"]), np.uint8)
+        data[:, :3840] = info[idx]
+        cw = lizard_amd.ldpc_encode(r, data)
+        sigma = np.sqrt(1.0 / (2.0 * (3840.0 / d["nt"]) * ebn0))
+        y = (1.0 - 2.0 * cw) + sigma * rng.standard_normal(cw.shape)
+        q = np.clip(np.rint(8.0 * (2.0 * y / sigma ** 2).astype(np.float32).astype(np.float64)), -80, 80).astype(np.int8)
+        if d["k"] > 3840:
+            q[:, 3840:d["k"]] = 127
+        for j, i in enumerate(idx):
+            llr[offs[i]:offs[i] + d["n"]] = q[j]
+    return rates, llr, offs, payload, info
+
+
+def lds_bytes_per_iteration(rate):
+    # algorithmic LDS traffic of one layered iteration: posterior read + write (2 + 2 B) and message read + write
+    # (1 + 1 B) per edge of H
+    return 6 * lizard_amd.ldpc_dims(rate)["edges"]
+
+
+def effective_iterations(its):
+    its = its.astype(np.int64)
+    return np.where(its > 0, its, MAX_ITER)
+
+
+def cpu_baseline(batches, seconds):
+    """The reference decoder (compiled from its sources into oracle/_ref, native x86-64 form) on the host cores: one
+    decoder per thread as liblizard/core/cpu/pool.h runs them, over a bounded sample of the same codewords."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+    kind = "reference"
+    if not os.path.exists(ref_so):
+        return None
+    lib = C.CDLL(ref_so)
+    fn = lib.ref_ldpc_decode_many
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_float]
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = max(1, min(cores, os.cpu_count() or 1, 16))
+    # work items: 32 codewords of one rate from one Eb/N0 point, round robin over points and rates
+    items = []
+    for (rates, llr, offs, _, _) in batches:
+        for r, _ in MIX:
+            n = lizard_amd.ldpc_dims(r)["n"]
+            idx = np.nonzero(rates == r)[0]
+            for s in range(0, idx.size - 31, 32):
+                block = np.concatenate([llr[offs[i]:offs[i] + n] for i in idx[s:s + 32]])
+                items.append((r, n, block))
+    order = np.random.default_rng(0).permutation(len(items))
+    items = [items[i] for i in order]
+    lock = threading.Lock()
+    state = {"next": 0, "words": 0}
+    deadline = [0.0]
+
+    def worker():
+        out = np.zeros(32 * 7680, np.uint8)
+        its = np.zeros(32, np.int8)
+        while True:
+            with lock:
+                if time.perf_counter() > deadline[0]:
+                    return
+                r, n, block = items[state["next"] % len(items)]
+                state["next"] += 1
+            fn(r, 32, block.ctypes.data, out.ctypes.data, its.ctypes.data, MAX_ITER, 0, 0.95)
+            with lock:
+                state["words"] += 32
+
+    t0 = time.perf_counter()
+    deadline[0] = t0 + seconds
+    th = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    words = state["words"]
+    return {"value": round(3840.0 * words / wall / 1e9, 4), "unit": "Gbit/s", "cores": cores, "kind": kind,
+            "sample": "%d codewords (32-word runs of one rate from all six Eb/N0 batches, shuffled, cycled to the time bound), %.1f s wall on %d "
+                      "threads; liblizard ldpc_decode_stall (x86-64 SIMD build), 20 iterations, early termination"
+                      % (words, wall, cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24)
+    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the path shards by frame/codeword with no exchange (see "
+              "DESIGN.md), and no figure at several GPUs is produced", file=sys.stderr)
+        sys.exit(2)
+    if lizard_amd.device_count() < 1:
+        print("bench.py: no GPU", file=sys.stderr)
+        sys.exit(1)
+
+    batches = [build_batch(db, seed=1000 + i) for i, db in enumerate(EBN0)]
+    count = batches[0][0].size
+    D = lizard_amd.DeviceArray
+    dev = []
+    for rates, llr, offs, _, _ in batches:
+        dev.append((D.of(rates), D.of(llr), D.of(offs)))
+    span = max(b[1].size for b in batches)
+    d_bits = D(span)
+    d_its = D(count)
+    d_bytes = D(count * 476)
+    d_ok = D(count)
+    dec = lizard_amd.LdpcDecoder(0)
+
+    def step(s, plain=False, bits=False):
+        d_rt, d_llr, d_off = dev[s % len(dev)]
+        dec.decode_device(count, d_rt, d_llr, d_off, d_bits if bits else None, d_its, max_iter=MAX_ITER, stall_it=0,
+                          crc=True, plain=plain, d_bytes=d_bytes, d_crc_ok=d_ok)
+
+    # verification pass (untimed): per point, iterations, CRC verdicts and decisions against what was sent
+    points = []
+    its_all = []
+    verified_all = []
+    for p in range(len(EBN0)):
+        step(p, bits=True)
+        dec.sync()
+        its = d_its.get(np.int8, count)
+        ok = d_ok.get(np.uint8, count)
+        bits = d_bits.get(np.uint8, span)
+        rates, llr, offs, payload, info = batches[p]
+        right = np.array([bool(ok[i]) and np.array_equal(bits[offs[i]:offs[i] + 3840], info[i]) for i in range(count)])
+        fails = {}
+        for r, c in MIX:
+            m = rates == r
+            fails[{6: "7/8", 4: "3/4", 3: "2/3", 2: "1/2"}[r]] = round(float(1.0 - right[m].mean()), 6)
+        points.append({"ebn0_db": EBN0[p], "block_fail": round(float(1.0 - right.mean()), 6), "block_fail_by_rate": fails,
+                       "its_mean": round(float(effective_iterations(its).mean()), 3), "verified": int(ok.sum())})
+        its_all.append(its)
+        verified_all.append(int(ok.sum()))
+
+    for s in range(args.warmup):
+        step(s)
+    dec.sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(s)
+    ev_ms = dec.sync()
+    wall = time.perf_counter() - t0
+
+    codewords = count * args.steps
+    verified = sum(verified_all[s % len(EBN0)] for s in range(args.steps))
+    gbps = 3840.0 * codewords / wall / 1e9
+    payload_mbps = 472.0 * verified / wall / 1e6
+
+    # LDS roofline of the decode kernel: algorithmic bytes of the iterations actually run, per launch
+    lds_per_launch = []
+    for p in range(len(EBN0)):
+        rates = batches[p][0]
+        per_it = np.array([lds_bytes_per_iteration(int(r)) for r in rates], np.int64)
+        lds_per_launch.append(float((per_it * effective_iterations(its_all[p])).sum()))
+    lds_bytes = sum(lds_per_launch[s % len(EBN0)] for s in range(args.steps)) / args.steps
+    launch_ms = ev_ms / args.steps
+    lds_gbs = lds_bytes / (launch_ms * 1e-3) / 1e9
+    hbm_bytes = float(sum(b[1].size for b in batches) / len(batches) + count * (476 + 2))
+
+    plain = None
+    if not args.no_plain:
+        t0 = time.perf_counter()
+        for s in range(len(EBN0)):
+            step(s, plain=True)
+        dec.sync()
+        pw = time.perf_counter() - t0
+        plain = 3840.0 * count * len(EBN0) / pw / 1e9
+
+    # the practical HBM copy rate (the build's own stream-copy probe), for reference
+    nb = 1 << 30
+    a, b = D(nb), D(nb)
+    lizard_amd.copy_probe(b, a, nb)
+    copy_ms = min(lizard_amd.copy_probe(b, a, nb) for _ in range(3))
+    copy_gbs = 2.0 * nb / (copy_ms * 1e-3) / 1e9
+    a.free(), b.free()
+
+    cpu = None if args.no_cpu else cpu_baseline(batches, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC, "value": round(gbps, 3), "unit": "Gbit/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int8 messages / int16 posteriors", "data": "synthetic",
+        "config": {"workload": "LDPC only, batched: 4096 codewords x 3840 info bits, rates 7/8:3/4:2/3:1/2 = "
+                               "1597:1065:614:820, BPSK/AWGN Eb/N0 1.5-4.0 dB (one batch a point, steps cycling), "
+                               "int8 LLR = rhe(8 * 2y/sigma^2) clamped +-80, layered min-sum max 20 it, early "
+                               "termination, one persistent kernel, CRC-32C on the GPU",
+                   "codewords": count, "ebn0_db": list(EBN0), "max_iter": MAX_ITER},
+        "payload_MBps": round(payload_mbps, 2),
+        "plain_Gbps": round(plain, 4) if plain else None,
+        "batched_over_plain": round(gbps / plain, 2) if plain else None,
+        "points": points,
+        "roofline": {"bound": "lds", "kernel": "k_ldpc", "achieved": round(lds_gbs, 1), "peak": LDS_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "traffic": None,
+                     "launch_ms": round(launch_ms, 4), "lds_bytes_per_launch": lds_bytes,
+                     "iterations_per_s": round(sum(float(effective_iterations(its_all[s % len(EBN0)]).sum())
+                                                   for s in range(args.steps)) / (ev_ms * 1e-3), 1),
+                     "hbm": {"achieved": round(hbm_bytes / (launch_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
+                             "copy_probe_GBps": round(copy_gbs, 1), "algorithmic_bytes_per_launch": hbm_bytes}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

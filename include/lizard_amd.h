@@ -127,14 +127,27 @@ LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const
                              uint8_t *bits, int8_t *its, int max_iter, int stall_it, float stall_ratio, int flags,
                              uint8_t *bytes, uint8_t *crc_ok, int16_t *post, int8_t *msg);
 
-/* The same on device memory already resident in HBM, enqueued on the handle's stream (no host synchronisation):
-   every pointer is a device pointer. Timing of the last device call: lizx_ldpc_last_ms (HIP events around the
-   decode kernels, on the handle's stream). */
+/* The same on device memory already resident in HBM, enqueued on the handle's stream with no host synchronisation:
+   every pointer is a device pointer, and the task list is made on the device. A rate index with no code is not
+   decoded (its 0). lizx_ldpc_sync waits for the stream; lizx_ldpc_last_ms is then the time, by HIP events on the
+   handle's stream, from the first decode enqueued since the previous sync to the end of the last. */
 LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr,
                                     const int64_t *d_off, uint8_t *d_bits, int8_t *d_its, int max_iter, int stall_it,
                                     float stall_ratio, int flags, uint8_t *d_bytes, uint8_t *d_crc_ok);
 LIZ_API int lizx_ldpc_sync(lizx_ldpc *h);
 LIZ_API float lizx_ldpc_last_ms(lizx_ldpc *h);
+
+/* The systematic IRA encoder (liblizard/src/ldpc.h:44, ldpc.c:117-126), host: count codewords of one rate index;
+   data holds count * k bits (0/1 a byte), cw receives count * n. Returns 0 or LIZ_E_ARG. */
+LIZ_API int lizx_ldpc_encode(int rate, int count, const uint8_t *data, uint8_t *cw);
+
+/* CRC-32C (Castagnoli; research/SPEC.md 7.1, liblizard/src/layout.c:44-54 ob_crc32c), host. */
+LIZ_API uint32_t lizx_crc32c(const uint8_t *p, size_t n);
+
+/* A plain device copy of bytes (16-byte vectors, grid-stride), timed with HIP events: the practical HBM bandwidth the
+   rooflines are stated against, and the known-size kernel the HBM counters are calibrated with. *ms receives the
+   kernel's duration. */
+LIZ_API int lizx_copy_probe(void *d_dst, const void *d_src, size_t bytes, float *ms);
 
 /* Device memory helpers for callers that hold no HIP runtime of their own (tests, benches): allocate, free, copy
    in/out (synchronous). */

@@ -69,6 +69,9 @@ def lib():
     sig("lizx_ldpc_decode_device", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P)
     sig("lizx_ldpc_sync", _I, _P)
     sig("lizx_ldpc_last_ms", _F, _P)
+    sig("lizx_ldpc_encode", _I, _I, _I, _P, _P)
+    sig("lizx_crc32c", C.c_uint32, _P, _SZ)
+    sig("lizx_copy_probe", _I, _P, _P, _SZ, C.POINTER(C.c_float))
     sig("lizx_dev_alloc", _P, _SZ)
     sig("lizx_dev_free", None, _P)
     sig("lizx_dev_upload", _I, _P, _P, _SZ)
@@ -104,6 +107,26 @@ def ldpc_dims(rate):
     check(lib().lizx_ldpc_dims(rate, _p(d)))
     return dict(zip(("n", "k", "m", "z", "mb", "kb", "np", "nt", "edges", "slots_total", "msg_bytes", "post_bytes"),
                     map(int, d)))
+
+
+def ldpc_encode(rate, data):
+    """Encode count codewords: data is (count, k) bits (0/1); returns (count, n)."""
+    d = ldpc_dims(rate)
+    data = np.ascontiguousarray(data, np.uint8).reshape(-1, d["k"])
+    cw = np.zeros((data.shape[0], d["n"]), np.uint8)
+    check(lib().lizx_ldpc_encode(rate, data.shape[0], _p(data), _p(cw)))
+    return cw
+
+
+def crc32c(data):
+    data = np.ascontiguousarray(data, np.uint8)
+    return int(lib().lizx_crc32c(_p(data), data.size))
+
+
+def copy_probe(dst, src, nbytes):
+    ms = C.c_float(0)
+    check(lib().lizx_copy_probe(dst.ptr, src.ptr, nbytes, C.byref(ms)))
+    return ms.value
 
 
 class DeviceArray:

@@ -1,6 +1,7 @@
 // The C ABI (include/lizard_amd.h): liblizard's decode API and the lizx_* seams, over the HIP stages.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -76,6 +77,8 @@ static void codeword_tasks(int count, const uint8_t *rate, const int64_t *off, s
     need(off[i] >= 0, "a negative offset");
     t[i] = LdpcTask{off[i], off[i], ci, 0, 0, i};
   }
+  // the longest codewords (1/2) first, the shortest last: the persistent queue's tail stays short
+  std::stable_sort(t.begin(), t.end(), [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
 }
 
 static void check_params(int max_iter, int stall_it, float ratio) {
@@ -148,6 +151,39 @@ LIZ_API int lizx_ldpc_dims(int rate, int *dims) {
   });
 }
 
+// p_c = p_(c-1) ^ (xor of the data bits in check c), p_(-1) = 0 (research/SPEC.md 8.4): check mb*i + r holds data bit
+// 128 j + ((i - sh[r][j]) mod 128) for each circulant (j, sh) of block row r.
+LIZ_API int lizx_ldpc_encode(int rate, int count, const uint8_t *data, uint8_t *cw) {
+  return guard([&] {
+    const int ci = code_index(rate);
+    if (ci < 0) throw Error(LIZ_E_ARG, "a rate index with no code (6, 4, 3 or 2)");
+    need(count >= 0 && (count == 0 || (data && cw)), "data and codeword buffers");
+    const Code &c = code(ci);
+    for (int w = 0; w < count; w++) {
+      const uint8_t *d = data + static_cast<size_t>(w) * c.k;
+      uint8_t *o = cw + static_cast<size_t>(w) * c.n;
+      std::memcpy(o, d, c.k);
+      uint8_t p = 0;
+      for (int chk = 0; chk < c.m; chk++) {
+        const int i = chk / c.mb, r = chk % c.mb;
+        for (int e = c.lay_ptr[r]; e < c.lay_ptr[r + 1]; e++) p ^= d[c.lay_col[e] * Z + ((i - c.lay_shift[e]) & (Z - 1))];
+        o[c.k + chk] = p;
+      }
+    }
+    return 0;
+  });
+}
+
+LIZ_API uint32_t lizx_crc32c(const uint8_t *p, size_t n) { return p ? crc32c(p, static_cast<int>(n)) : 0u; }
+
+LIZ_API int lizx_copy_probe(void *d_dst, const void *d_src, size_t bytes, float *ms) {
+  return guard([&] {
+    need(d_dst && d_src && ms && bytes % 16 == 0, "device buffers and a size in 16-byte units");
+    *ms = copy_probe(d_dst, d_src, bytes);
+    return 0;
+  });
+}
+
 LIZ_API lizx_ldpc *lizx_ldpc_new(int device) {
   return guard_new<lizx_ldpc>([&] {
     const int dev = current_device(device);
@@ -166,17 +202,11 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
     check_params(max_iter, stall_it, stall_ratio);
     need(!(flags & LIZX_LDPC_CRC) || (d_bytes && d_crc_ok), "CRC outputs");
     need(!(flags & LIZX_LDPC_TRACE), "traces are a host-call option");
-    // the task list is built on the host from the rates and offsets (a small download), then kept on the device
-    std::vector<uint8_t> rate(count);
-    std::vector<int64_t> off(count);
+    if (!count) return 0;
+    // the task list is made on the device from the rates and offsets (which also zeroes the queue counter)
     hip_check(hipSetDevice(h->eng->device), "hipSetDevice");
-    hip_check(hipMemcpy(rate.data(), d_rate, count, hipMemcpyDeviceToHost), "rates");
-    hip_check(hipMemcpy(off.data(), d_off, count * sizeof(int64_t), hipMemcpyDeviceToHost), "offsets");
-    codeword_tasks(count, rate.data(), off.data(), h->host_tasks);
     h->tasks.reserve(count * sizeof(LdpcTask));
-    hip_check(hipMemcpyAsync(h->tasks.p, h->host_tasks.data(), count * sizeof(LdpcTask), hipMemcpyHostToDevice,
-                             h->eng->stream),
-              "tasks");
+    h->eng->make_tasks(h->tasks.as<LdpcTask>(), d_rate, d_off, count);
     LdpcParams p{};
     p.tasks = h->tasks.as<LdpcTask>();
     p.count = count;
@@ -192,7 +222,7 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
     p.stall_ratio = stall_ratio;
     p.mode = MODE_CODEWORD;
     p.flags = flags & LIZX_LDPC_CRC;
-    h->eng->launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true);
+    h->eng->launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true, true);
     return 0;
   });
 }
@@ -200,9 +230,7 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
 LIZ_API int lizx_ldpc_sync(lizx_ldpc *h) {
   return guard([&] {
     need(h, "a decoder");
-    hip_check(hipStreamSynchronize(h->eng->stream), "decode");
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, h->eng->ev0, h->eng->ev1) == hipSuccess) h->eng->last_ms = ms;
+    h->eng->sync();
     return 0;
   });
 }
@@ -220,8 +248,13 @@ LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const
     need(!trace || (post && msg), "trace outputs");
     if (!count) return 0;
     codeword_tasks(count, rate, off, h->host_tasks);
+    // per codeword (input order): its length; the device buffers cover every codeword's range
+    std::vector<int> len(count);
     int64_t span = 0;
-    for (int i = 0; i < count; i++) span = std::max<int64_t>(span, off[i] + code(h->host_tasks[i].code).n);
+    for (int i = 0; i < count; i++) {
+      len[i] = code(code_index(rate[i])).n;
+      span = std::max<int64_t>(span, off[i] + len[i]);
+    }
     LdpcEngine &e = *h->eng;
     hip_check(hipSetDevice(e.device), "hipSetDevice");
     h->tasks.reserve(count * sizeof(LdpcTask));
@@ -258,15 +291,13 @@ LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const
     p.mode = MODE_CODEWORD;
     p.flags = (crc ? 1 : 0) | (trace ? 2 : 0);
     e.launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true);
-    hip_check(hipStreamSynchronize(e.stream), "decode");
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, e.ev0, e.ev1) == hipSuccess) e.last_ms = ms;
+    e.sync();
     hip_check(hipMemcpy(its, h->its.p, count, hipMemcpyDeviceToHost), "iterations");
     if (bits) {
       // only the codewords' own ranges: the caller's buffer between them is left as it was
       std::vector<uint8_t> all(span);
       hip_check(hipMemcpy(all.data(), h->bits.p, span, hipMemcpyDeviceToHost), "bits");
-      for (int i = 0; i < count; i++) std::memcpy(bits + off[i], all.data() + off[i], code(h->host_tasks[i].code).n);
+      for (int i = 0; i < count; i++) std::memcpy(bits + off[i], all.data() + off[i], len[i]);
     }
     if (crc) {
       hip_check(hipMemcpy(bytes, h->bytes.p, static_cast<size_t>(count) * BLOCK_BYTES, hipMemcpyDeviceToHost), "bytes");
@@ -276,7 +307,7 @@ LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const
       std::vector<int16_t> all(span);
       hip_check(hipMemcpy(all.data(), h->post.p, span * sizeof(int16_t), hipMemcpyDeviceToHost), "posteriors");
       for (int i = 0; i < count; i++)
-        std::memcpy(post + off[i], all.data() + off[i], code(h->host_tasks[i].code).n * sizeof(int16_t));
+        std::memcpy(post + off[i], all.data() + off[i], len[i] * sizeof(int16_t));
       hip_check(hipMemcpy(msg, h->msg.p, static_cast<size_t>(count) * MSG_TRACE_STRIDE, hipMemcpyDeviceToHost),
                 "messages");
     }
@@ -333,6 +364,8 @@ LIZ_API int lizx_finish_llr(liz_decoder *d, int version, const int8_t *llr, cons
     const int S = f->subch * 640, B = f->blocks;
     d->host_tasks.resize(B);
     for (int b = 0; b < B; b++) d->host_tasks[b] = LdpcTask{f->blk_slot[b], -1, f->blk_code[b], f->blk_slot[b], 0, b};
+    std::stable_sort(d->host_tasks.begin(), d->host_tasks.end(),
+                     [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
     std::vector<double> est_d(f->subch);
     for (int j = 0; j < f->subch; j++) est_d[j] = est[j];
     d->tasks.reserve(B * sizeof(LdpcTask));
@@ -366,7 +399,7 @@ LIZ_API int lizx_finish_llr(liz_decoder *d, int version, const int8_t *llr, cons
     p.mode = MODE_FRAME;
     p.flags = 1;
     e.launch(p, false, true);
-    hip_check(hipStreamSynchronize(e.stream), "finish");
+    e.sync();
     hip_check(hipMemcpy(its, d->its.p, B, hipMemcpyDeviceToHost), "iterations");
     hip_check(hipMemcpy(blk_est, d->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost), "estimates");
     hip_check(hipMemcpy(ok, d->crc_ok.p, B, hipMemcpyDeviceToHost), "verdicts");

@@ -1,7 +1,7 @@
-// Layered min-sum LDPC + CRC-32C on gfx950: one codeword per 128-lane workgroup, a lane per check position of the
-// current block row, the codeword's posteriors (int16) and messages (int8) in LDS, a block-row barrier between
-// layers. A persistent grid pulls codewords off an atomic queue, so codewords of four rates with uneven iteration
-// counts share one launch; the plain path launches one workgroup per codeword instead.
+// Layered min-sum LDPC + CRC-32C on gfx950: one codeword per wave (64 lanes, each owning two check positions of the
+// current block row), the codeword's int16 posteriors in LDS and its int8 messages in VGPRs, the code's tables
+// compiled in (ldpc_tmpl.hpp). A persistent grid pulls codewords off an atomic queue ordered by code, so codewords of
+// four rates with uneven iteration counts share one launch; the plain path launches one workgroup per codeword.
 //
 // Follows liblizard/src/ldpc.c:1080-1172 (ldpc_decode_stall) and, for frame mode, liblizard/src/lizard_phy.c:1775-1870
 // (decline gate, bit map, known zeros, bit pack, CRC compare).
@@ -10,10 +10,9 @@
 #include <algorithm>
 
 #include "ldpc.hpp"
+#include "ldpc_tmpl.hpp"
 
 namespace lz {
-
-__constant__ DevCode c_codes[NCODES];
 
 __global__ void k_zero_int(int *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -29,153 +28,181 @@ __device__ inline int wave_sum(int v) {
   return v;
 }
 
+// Sum over the workgroup's two waves; slot is a 2-int LDS area no other lane uses until the next barrier.
+__device__ inline int group_sum(int v, int *slot, int lane) {
+  v = wave_sum(v);
+  if ((lane & 63) == 0) slot[lane >> 6] = v;
+  __syncthreads();
+  return slot[0] + slot[1];
+}
+__device__ inline uint32_t group_xor(uint32_t v, uint32_t *slot, int lane) {
+  v = wave_xor(v);
+  if ((lane & 63) == 0) slot[lane >> 6] = v;
+  __syncthreads();
+  return slot[0] ^ slot[1];
+}
+
+// Codeword-mode tasks from device arrays of rate indices and offsets, ordered by code (1/2 first: the longest
+// codewords lead the queue, the shortest fill its tail) by a counting sort in one workgroup; a rate with no code gets
+// code -1 (not decoded, its 0) at the end. Also zeroes the queue counter for the decode that follows on the stream.
+__global__ __launch_bounds__(1024) void k_make_tasks(LdpcTask *t, const uint8_t *rate, const int64_t *off, int count,
+                                                     int *queue) {
+  __shared__ int cnt[5], base[5];
+  const int tid = threadIdx.x;
+  if (tid < 5) cnt[tid] = 0;
+  if (tid == 0 && queue) *queue = 0;
+  __syncthreads();
+  auto slot_of = [](int r) { return r == 2 ? 0 : r == 3 ? 1 : r == 4 ? 2 : r == 6 ? 3 : 4; };
+  for (int i = tid; i < count; i += blockDim.x) atomicAdd(&cnt[slot_of(rate[i])], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int at = 0;
+    for (int q = 0; q < 5; q++) {
+      base[q] = at;
+      at += cnt[q];
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < count; i += blockDim.x) {
+    const int q = slot_of(rate[i]);
+    const int at = atomicAdd(&base[q], 1);
+    t[at] = LdpcTask{off[i], off[i], q < 4 ? 3 - q : -1, 0, 0, i};
+  }
+}
+
+// One codeword of code CI by one workgroup (LDPC_THREADS = 128 lanes, lane = check position), L the workgroup's
+// posteriors, red a 4-int LDS scratch for the reductions.
+template <int CI>
+__device__ __forceinline__ void decode_one(const LdpcParams &p, const LdpcTask &tk, int16_t *L, int *red, int lane) {
+  using S = tm::Shape<CI>;
+  constexpr int N = S::N, K = S::K, NP = S::cb.np * Z, NT = N - NP, MB = S::MB;
+  // ---- posteriors from the channel (ldpc.c:1107-1110; frame mode: map_in and the known zeros, lizard_phy.c:156-192,
+  // :1823)
+  if (p.mode == MODE_CODEWORD) {
+    const int8_t *src = p.llr + tk.llr;
+    for (int v = lane; v < K; v += LDPC_THREADS) L[v] = src[v];
+    for (int v = lane; v < N - K; v += LDPC_THREADS) L[K + (v % MB) * Z + v / MB] = src[K + v];
+  } else {
+    const int8_t *src = p.llr + tk.llr;
+    const uint8_t *wh = p.white + tk.slot0;
+    const int16_t *s2p = p.s2p[CI];
+    for (int v = K - NP + lane; v < K; v += LDPC_THREADS) L[v] = v >= INFO_BITS ? 127 : 0;
+    for (int s = lane; s < NT; s += LDPC_THREADS) {
+      const int x = src[s];
+      L[s2p[s]] = static_cast<int16_t>(static_cast<int8_t>(wh[s] ? -x : x));
+    }
+    for (int v = INFO_BITS + lane; v < K; v += LDPC_THREADS) L[v] = 127;
+  }
+  tm::Msgs<CI> M;
+  tm::clear_msgs(M);
+  __syncthreads();
+  // ---- iterations: block rows in order on even iterations, reversed on odd; syndrome after each (ldpc.c:1116-1149)
+  int used = -1, first = 0;
+  for (int it = 0; it < p.max_iter; it++) {
+    for (int step = 0; step < MB; step++) {
+      const int r = (it & 1) ? MB - 1 - step : step;
+      tm::layer_at<CI>(r, L, M, lane);
+      __syncthreads();
+    }
+    // two alternating reduction slots: a slot is rewritten only after a later barrier has passed every read of it
+    const int bad = group_sum(tm::unsat<CI>(L, lane), red + 2 * (it & 1), lane);
+    const int v = stop_rule(it, bad, p.stall_it, p.stall_ratio, &first);
+    if (v) {
+      used = v;
+      break;
+    }
+  }
+  // ---- outputs: hard decisions, CRC (lizard_phy.c:1828-1868), traces
+  if (p.bits && tk.bits >= 0) {
+    for (int v = lane; v < K; v += LDPC_THREADS) p.bits[tk.bits + v] = L[v] < 0;
+    for (int v = lane; v < N - K; v += LDPC_THREADS) p.bits[tk.bits + K + v] = L[K + (v % MB) * Z + v / MB] < 0;
+  }
+  if (p.flags & 1) {
+    uint32_t x = 0;
+    for (int j = lane; j < BLOCK_BYTES; j += LDPC_THREADS) {
+      int b = 0;
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int bit = L[8 * j + q] < 0;
+        b = (b << 1) | bit;
+        x ^= bit ? p.crc_tab[8 * j + q] : 0u;
+      }
+      p.bytes[static_cast<int64_t>(tk.out) * BLOCK_BYTES + j] = static_cast<uint8_t>(b);
+    }
+    x = group_xor(x, reinterpret_cast<uint32_t *>(red) + 4, lane);
+    if (lane == 0) {
+      uint32_t rx = 0;
+      for (int q = 0; q < 32; q++) rx = (rx << 1) | (L[8 * BLOCK_BYTES + q] < 0);
+      p.crc_ok[tk.out] = used > 0 && (p.crc0 ^ x) == rx;
+    }
+  }
+  if (p.flags & 2) {
+    for (int v = lane; v < N; v += LDPC_THREADS) p.post[tk.bits + v] = L[v];
+    tm::dump_msgs<CI>(M, p.msg + static_cast<int64_t>(tk.out) * (MAX_SLOTS_TOTAL * Z), lane);
+  }
+  if (lane == 0) p.its[tk.out] = static_cast<int8_t>(used);
+}
+
+// The frame-mode decline gate (lizard_phy.c:1782-1787, :1818): the block's estimate from its sub-channels', true
+// where the block is not to be decoded.
+template <int CI>
+__device__ __forceinline__ bool declined(const LdpcParams &p, const LdpcTask &tk, int *red, int lane) {
+  using S = tm::Shape<CI>;
+  constexpr int K = S::K, NT = S::N - S::cb.np * Z, NB = NT + (CI == 0 ? 16 : CI == 1 ? 32 : 0);
+  int d = 0;
+  if (lane == 0) {
+    const int s0 = tk.slot0;
+    double info = 0;
+    for (int j = s0 / 640; j * 640 < s0 + NB; j++) {
+      const int lo = s0 > 640 * j ? s0 : 640 * j, hi = s0 + NB < 640 * (j + 1) ? s0 + NB : 640 * (j + 1);
+      info += p.est[tk.est + j] * (hi - lo);
+    }
+    const float be = static_cast<float>(info / NB);
+    p.blk_est[tk.out] = be;
+    d = be < 0.2f * static_cast<float>(K) / static_cast<float>(NT);
+    if (d) {
+      p.its[tk.out] = -2;
+      if (p.crc_ok) p.crc_ok[tk.out] = 0;
+    }
+  }
+  if (lane == 0) red[6] = d;
+  __syncthreads();
+  return red[6] != 0;
+}
+
+template <int CI>
+__device__ __forceinline__ void run_task(const LdpcParams &p, const LdpcTask &tk, int16_t *L, int *red, int lane) {
+  if (p.mode == MODE_FRAME && declined<CI>(p, tk, red, lane)) return;
+  decode_one<CI>(p, tk, L, red, lane);
+}
+
 __global__ __launch_bounds__(LDPC_THREADS) void k_ldpc(LdpcParams p) {
   __shared__ int16_t L[MAX_N];
-  __shared__ int8_t R[MAX_SLOTS_TOTAL * Z];
-  __shared__ int s_task, s_bad[2], s_decl;
-  __shared__ uint32_t s_x[2];
-  const int i = threadIdx.x, lane = i & 63, wave = i >> 6;
-  if (i == 0) {
-    s_bad[0] = 0;
-    s_bad[1] = 0;
-  }
+  __shared__ int red[8];
+  __shared__ int s_task;
+  const int lane = threadIdx.x;
   for (int round = 0;; round++) {
-    if (i == 0) s_task = p.queue ? atomicAdd(p.queue, 1) : static_cast<int>(blockIdx.x);
+    if (lane == 0) s_task = p.queue ? atomicAdd(p.queue, 1) : static_cast<int>(blockIdx.x);
     __syncthreads();
     const int t = __builtin_amdgcn_readfirstlane(s_task);
     if (t >= p.count || (!p.queue && round > 0)) break;
     const LdpcTask tk = p.tasks[t];
-    const int ci = __builtin_amdgcn_readfirstlane(tk.code);
-    const DevCode &c = c_codes[ci];
-    const int n = c.n, k = c.k;
-
-    // ---- frame mode: the block's estimate and the decline gate (lizard_phy.c:1782-1787, :1818)
-    if (p.mode == MODE_FRAME) {
-      if (i == 0) {
-        const int nb = c.nt + (ci == 0 ? 16 : ci == 1 ? 32 : 0);
-        const int s0 = tk.slot0;
-        double info = 0;
-        for (int j = s0 / 640; j * 640 < s0 + nb; j++) {
-          const int lo = s0 > 640 * j ? s0 : 640 * j, hi = s0 + nb < 640 * (j + 1) ? s0 + nb : 640 * (j + 1);
-          info += p.est[tk.est + j] * (hi - lo);
-        }
-        const float be = static_cast<float>(info / nb);
-        p.blk_est[tk.out] = be;
-        s_decl = be < 0.2f * static_cast<float>(k) / static_cast<float>(c.nt);
-      }
-      __syncthreads();
-      if (s_decl) {
-        if (i == 0) {
-          p.its[tk.out] = -2;
-          if (p.crc_ok) p.crc_ok[tk.out] = 0;
-        }
-        __syncthreads();
-        if (!p.queue) break;
-        continue;
-      }
+    switch (__builtin_amdgcn_readfirstlane(tk.code)) {
+      case 0: run_task<0>(p, tk, L, red, lane); break;
+      case 1: run_task<1>(p, tk, L, red, lane); break;
+      case 2: run_task<2>(p, tk, L, red, lane); break;
+      case 3: run_task<3>(p, tk, L, red, lane); break;
+      default:
+        if (lane == 0) p.its[tk.out] = 0;
     }
-
-    // ---- posteriors from the channel, messages zero (ldpc.c:1107-1115; frame mode: map_in and the known zeros,
-    // lizard_phy.c:156-192, :1823)
-    if (p.mode == MODE_CODEWORD) {
-      const int8_t *src = p.llr + tk.llr;
-      for (int v = i; v < n; v += LDPC_THREADS) L[dev_post_index(c, v)] = src[v];
-    } else {
-      const int8_t *src = p.llr + tk.llr;
-      const uint8_t *wh = p.white + tk.slot0;
-      const int16_t *s2p = p.s2p[ci];
-      for (int v = k - c.np + i; v < k; v += LDPC_THREADS) L[v] = 0;
-      __syncthreads();
-      for (int s = i; s < c.nt; s += LDPC_THREADS) {
-        const int x = src[s];
-        L[s2p[s]] = static_cast<int16_t>(static_cast<int8_t>(wh[s] ? -x : x));
-      }
-      __syncthreads();
-      for (int v = INFO_BITS + i; v < k; v += LDPC_THREADS) L[v] = 127;
-    }
-    for (int s = i; s < c.slots_total * Z; s += LDPC_THREADS) R[s] = 0;
+    // every lane is done with this codeword (and has read s_task and red) before lane 0 takes the next
     __syncthreads();
-
-    // ---- iterations: block rows in order on even iterations, reversed on odd; syndrome after each
-    int used = -1, first = 0;
-    for (int it = 0; it < p.max_iter; it++) {
-      for (int step = 0; step < c.mb; step++) {
-        const int r = (it & 1) ? c.mb - 1 - step : step;
-        layer_update(c, L, R, r, i);
-        __syncthreads();
-      }
-      const int part = wave_sum(lane_unsat(c, L, i));
-      if (lane == 0) atomicAdd(&s_bad[it & 1], part);
-      __syncthreads();
-      const int bad = s_bad[it & 1];
-      if (i == 0) s_bad[(it + 1) & 1] = 0;
-      const int v = stop_rule(it, bad, p.stall_it, p.stall_ratio, &first);
-      if (v) {
-        used = v;
-        break;
-      }
-    }
-
-    // ---- outputs: hard decisions, CRC, trace
-    if (p.bits && tk.bits >= 0)
-      for (int v = i; v < n; v += LDPC_THREADS) p.bits[tk.bits + v] = L[dev_post_index(c, v)] < 0;
-    if (p.flags & 1) {
-      uint32_t x = 0;
-      for (int j = i; j < BLOCK_BYTES; j += LDPC_THREADS) {
-        int b = 0;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-          const int bit = L[8 * j + q] < 0;
-          b = (b << 1) | bit;
-          x ^= bit ? p.crc_tab[8 * j + q] : 0u;
-        }
-        p.bytes[static_cast<int64_t>(tk.out) * BLOCK_BYTES + j] = static_cast<uint8_t>(b);
-      }
-      x = wave_xor(x);
-      if (lane == 0) s_x[wave] = x;
-      __syncthreads();
-      if (i == 0) {
-        uint32_t rx = 0;
-        for (int q = 0; q < 32; q++) rx = (rx << 1) | (L[8 * BLOCK_BYTES + q] < 0);
-        p.crc_ok[tk.out] = used > 0 && (p.crc0 ^ s_x[0] ^ s_x[1]) == rx;
-      }
-    }
-    if (p.flags & 2) {
-      for (int v = i; v < n; v += LDPC_THREADS) p.post[tk.bits + v] = L[v];
-      int8_t *m = p.msg + static_cast<int64_t>(tk.out) * (MAX_SLOTS_TOTAL * Z);
-      for (int s = i; s < c.slots_total * Z; s += LDPC_THREADS) m[s] = R[s];
-    }
-    if (i == 0) p.its[tk.out] = static_cast<int8_t>(used);
-    __syncthreads();
-    // every lane has read the last count: the two count slots start the next codeword at zero
-    if (i == 0) {
-      s_bad[0] = 0;
-      s_bad[1] = 0;
-    }
     if (!p.queue) break;
   }
 }
 
-void fill_dev_code(const Code &c, DevCode &d) {
-  d = DevCode{};
-  d.n = c.n;
-  d.k = c.k;
-  d.m = c.m;
-  d.mb = c.mb;
-  d.kb = c.kb;
-  d.np = c.np;
-  d.nt = c.nt;
-  d.slots_total = c.slots_total;
-  for (int r = 0; r <= c.mb; r++) d.lay_ptr[r] = c.lay_ptr[r];
-  for (int r = 0; r < c.mb; r++) d.rbase[r] = c.lay_ptr[r] + 2 * r;
-  for (size_t e = 0; e < c.lay_col.size(); e++) d.ent[e] = static_cast<uint16_t>(c.lay_col[e] << 7 | c.lay_shift[e]);
-}
-
 LdpcEngine::LdpcEngine(int dev) : device(dev) {
   hip_check(hipSetDevice(device), "hipSetDevice");
-  DevCode dc[NCODES];
-  for (int ci = 0; ci < NCODES; ci++) fill_dev_code(code(ci), dc[ci]);
-  hip_check(hipMemcpyToSymbol(HIP_SYMBOL(c_codes), dc, sizeof dc), "code tables to the device");
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipEventCreate(&ev0), "hipEventCreate");
   hip_check(hipEventCreate(&ev1), "hipEventCreate");
@@ -210,10 +237,18 @@ void LdpcEngine::ensure_frame_tables(int max_slots) {
   }
 }
 
-void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed) {
+void LdpcEngine::make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count) {
+  hipLaunchKernelGGL(k_make_tasks, dim3(1), dim3(1024), 0, stream, d_tasks, d_rate, d_off, count, queue.as<int>());
+  hip_check(hipGetLastError(), "task launch");
+}
+
+void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed, bool queue_zeroed) {
   if (p0.count <= 0) return;
   LdpcParams p = p0;
-  if (timed) hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+  if (timed && !armed) {
+    hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+    armed = true;
+  }
   if (plain) {
     p.queue = nullptr;
     // one workgroup per codeword, one launch each: the per-codeword path the batched one is measured against
@@ -225,12 +260,55 @@ void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed) {
     }
   } else {
     p.queue = queue.as<int>();
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, p.queue, 1);
+    if (!queue_zeroed) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, p.queue, 1);
     const int grid = std::min(p.count, cus * wg_per_cu);
     hipLaunchKernelGGL(k_ldpc, dim3(grid), dim3(LDPC_THREADS), 0, stream, p);
   }
   hip_check(hipGetLastError(), "LDPC launch");
   if (timed) hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+}
+
+// Waits for the stream; the time from the first timed launch since the last sync to the last one.
+void LdpcEngine::sync() {
+  hip_check(hipStreamSynchronize(stream), "decode");
+  if (armed) {
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");
+    last_ms = ms;
+    armed = false;
+  }
+}
+
+}  // namespace lz
+
+namespace lz {
+
+__global__ __launch_bounds__(256) void k_copy16(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t n) {
+  const size_t stride = static_cast<size_t>(gridDim.x) * blockDim.x;
+  for (size_t i = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+float copy_probe(void *dst, const void *src, size_t bytes) {
+  hipEvent_t a, b;
+  hip_check(hipEventCreate(&a), "hipEventCreate");
+  hip_check(hipEventCreate(&b), "hipEventCreate");
+  const size_t n = bytes / 16;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  hip_check(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+  const unsigned grid = static_cast<unsigned>(prop.multiProcessorCount) * 8;
+  hip_check(hipEventRecord(a, nullptr), "hipEventRecord");
+  hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(256), 0, nullptr, static_cast<uint4 *>(dst),
+                     static_cast<const uint4 *>(src), n);
+  hip_check(hipGetLastError(), "copy launch");
+  hip_check(hipEventRecord(b, nullptr), "hipEventRecord");
+  hip_check(hipEventSynchronize(b), "copy");
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return ms;
 }
 
 }  // namespace lz

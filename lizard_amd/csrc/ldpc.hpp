@@ -57,13 +57,17 @@ struct LdpcEngine {
   uint32_t crc0 = 0;
   int cus = 0, wg_per_cu = 0;
   float last_ms = 0;
+  bool armed = false;
   explicit LdpcEngine(int device);
   ~LdpcEngine();
-  // Enqueue a decode of count tasks already on the device. Plain: one launch per task.
-  void launch(const LdpcParams &p, bool plain, bool timed);
+  // Enqueue a decode of count tasks already on the device. Plain: one launch per task. Timed: HIP events bracket the
+  // launches from the first since the last sync() to the last.
+  void launch(const LdpcParams &p, bool plain, bool timed, bool queue_zeroed = false);
+  void make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count);
+  void sync();
   void ensure_frame_tables(int max_slots);
 };
 
-void fill_dev_code(const Code &c, DevCode &d);
+float copy_probe(void *dst, const void *src, size_t bytes);
 
 }  // namespace lz

@@ -1,0 +1,166 @@
+// The layered min-sum decoder specialised per code at compile time: every circulant's column and shift is a constant,
+// so a block row's posterior reads are issued back to back with immediate offsets, and a check's int8 messages live
+// in VGPRs (four to a register, byte indices fixed per block row). A 128-lane workgroup (two waves) decodes one
+// codeword: lane i owns check position i of every block row. Posteriors (int16) stay in LDS.
+//
+// The arithmetic is the reference decoder's scalar form (liblizard/src/ldpc.c:171-191, :589-617, :681-705,
+// :1080-1172; research/SPEC.md 8.5)  Shared by the
+// gfx950 kernel and a host-side emulation (tests walk the 128 lanes on a CPU).
+#pragma once
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "ldpc_codes.h"
+
+#ifndef __HIPCC__
+#ifndef __host__
+#define __host__
+#define __device__
+#endif
+#define LZ_INLINE inline
+#else
+#define LZ_INLINE __forceinline__
+#endif
+
+namespace lz {
+namespace tm {
+
+template <int B, int E, class F>
+__host__ __device__ LZ_INLINE void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// Compile-time facts of code CI (0 = 7/8, 1 = 3/4, 2 = 2/3, 3 = 1/2).
+template <int CI>
+struct Shape {
+  static constexpr const CodeBase &cb = CODE_BASES[CI];
+  static constexpr int MB = cb.mb, KB = cb.kb, K = cb.kb * 128, N = (cb.kb + cb.mb) * 128;
+  static constexpr int SLOTS = cb.ptr[cb.mb] + 2 * cb.mb;   // message slots of a check position (all block rows)
+  static constexpr int WORDS = (SLOTS + 3) / 4;              // VGPRs a check position's messages take
+  static constexpr int d0(int r) { return cb.ptr[r]; }
+  static constexpr int nd(int r) { return cb.ptr[r + 1] - cb.ptr[r]; }
+  static constexpr int rbase(int r) { return cb.ptr[r] + 2 * r; }
+  static constexpr int col(int e) { return cb.col[e]; }
+  static constexpr int shift(int e) { return cb.shift[e]; }
+};
+
+// Messages of the check position a lane owns: slot s at byte s % 4 of word s / 4.
+template <int CI>
+struct Msgs {
+  uint32_t m[Shape<CI>::WORDS];
+};
+
+// Keeps the compiler from hoisting every block row's addresses out of the iteration loop (they depend only on the
+// lane, so all of them would stay live in VGPRs): the lane index is re-read, opaquely, once per block row.
+__host__ __device__ LZ_INLINE int fresh(int i) {
+#ifdef __HIP_DEVICE_COMPILE__
+  asm volatile("" : "+v"(i));
+#endif
+  return i;
+}
+
+template <int S>
+__host__ __device__ LZ_INLINE int get_msg(const uint32_t *w) {
+  return static_cast<int>(static_cast<int8_t>(static_cast<uint8_t>(w[S >> 2] >> ((S & 3) * 8))));
+}
+template <int S>
+__host__ __device__ LZ_INLINE void put_msg(uint32_t *w, int v) {
+  constexpr int sh = (S & 3) * 8;
+  w[S >> 2] = (w[S >> 2] & ~(0xFFu << sh)) | ((static_cast<uint32_t>(v) & 0xFFu) << sh);
+}
+
+// Block row R of code CI for the lane's check position i.
+template <int CI, int R>
+__host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
+  using S = Shape<CI>;
+  const int i = fresh(lane);
+  constexpr int D0 = S::d0(R), ND = S::nd(R), NS = ND + 2, RB = S::rbase(R), K = S::K;
+  // the check's slots: its circulants' data bits, the parity bit before it, its own (ldpc.c:589-602)
+  int adr[NS];
+  static_for<0, ND>([&](auto e) {
+    constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
+    adr[e] = col * 128 + ((i - sh) & 127);
+  });
+  adr[ND] = R > 0 ? K + (R - 1) * 128 + i : K + (S::MB - 1) * 128 + i - 1;
+  adr[ND + 1] = K + R * 128 + i;
+  // check 0 (block row 0, position 0) has no parity bit before it: read as a certain 0 of 8191, message zeroed
+  const bool absent = R == 0 && i == 0;
+  int q[NS];
+  int min1 = 32767, min2 = 32767, arg = -1, sgn = 0;
+  static_for<0, NS>([&](auto e) {
+    int post = L[adr[e]];
+    int msg = get_msg<RB + e>(M.m);
+    if (e == ND && absent) {
+      post = 8191;
+      msg = 0;
+    }
+    const int v = post - msg;
+    q[e] = v;
+    const int a = v < 0 ? -v : v;
+    sgn ^= v;
+    const bool lt = a < min1;
+    min2 = lt ? min1 : (a < min2 ? a : min2);
+    min1 = lt ? a : min1;
+    arg = lt ? static_cast<int>(e) : arg;
+  });
+  int m1 = (min1 * 13) >> 4, m2 = (min2 * 13) >> 4;
+  m1 = m1 > 127 ? 127 : m1;
+  m2 = m2 > 127 ? 127 : m2;
+  static_for<0, NS>([&](auto e) {
+    const int mag = arg == e ? m2 : m1;
+    const int s = (sgn ^ q[e]) >> 31;  // -1 where the other slots' signs make the message negative
+    const int msg = (mag ^ s) - s;
+    put_msg<RB + e>(M.m, msg);
+    int t = q[e] + msg;
+    t = t > 8191 ? 8191 : (t < -8191 ? -8191 : t);
+    if (!(e == ND && absent)) L[adr[e]] = static_cast<int16_t>(t);
+  });
+}
+
+// Block row r (runtime) dispatched to its compile-time form.
+template <int CI, int R = 0>
+__host__ __device__ LZ_INLINE void layer_at(int r, int16_t *L, Msgs<CI> &M, int lane) {
+  if constexpr (R < Shape<CI>::MB) {
+    if (r == R) layer<CI, R>(L, M, lane);
+    else layer_at<CI, R + 1>(r, L, M, lane);
+  }
+}
+
+// The lane's unsatisfied checks over every block row (the sign of the XOR of a check's posteriors).
+template <int CI>
+__host__ __device__ LZ_INLINE int unsat(const int16_t *L, int lane) {
+  using S = Shape<CI>;
+  int bad = 0;
+  static_for<0, S::MB>([&](auto r) {
+    constexpr int R = r, D0 = S::d0(R), ND = S::nd(R), K = S::K;
+    const int i = fresh(lane);
+    int x = L[K + R * 128 + i];
+    static_for<0, ND>([&](auto e) {
+      constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
+      x ^= L[col * 128 + ((i - sh) & 127)];
+    });
+    if (R > 0) x ^= L[K + (R - 1) * 128 + i];
+    else x ^= i > 0 ? L[K + (S::MB - 1) * 128 + i - 1] : 0;
+    bad += x < 0;
+  });
+  return bad;
+}
+
+template <int CI>
+__host__ __device__ LZ_INLINE void clear_msgs(Msgs<CI> &M) {
+#pragma unroll
+  for (int w = 0; w < Shape<CI>::WORDS; w++) M.m[w] = 0;
+}
+
+// The messages out in the reference decoder's layout: slot s of check position i at s * 128 + i.
+template <int CI>
+__host__ __device__ LZ_INLINE void dump_msgs(const Msgs<CI> &M, int8_t *out, int i) {
+  static_for<0, Shape<CI>::SLOTS>([&](auto s) { out[s * 128 + i] = static_cast<int8_t>(get_msg<s>(M.m)); });
+}
+
+}  // namespace tm
+}  // namespace lz
