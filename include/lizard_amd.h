@@ -98,6 +98,23 @@ LIZ_API int lizx_format_info(int version, int *info);
 LIZ_API int lizx_finish_llr(liz_decoder *d, int version, const int8_t *llr, const float *est, uint8_t *blocks,
                             uint8_t *ok, int8_t *its, float *blk_est);
 
+/* Optional outputs of the soft-value stages, for callers that check them (each pointer may be NULL). */
+typedef struct {
+  int8_t *llr;      /* 16 * version * 640 soft values, frame slot order (whitened), as lizard_phy.c's llrf */
+  double *sub_est;  /* per sub-channel information estimate (axis_info, lizard_phy.c:75) */
+  float *sub_rms;   /* per sub-channel rms axis value sqrt(m2 / 2) */
+  float *coef;      /* 16 * version * 320 coefficients (re, im), coefficient order, after the detrend, before the
+                       pilots' turn (lizard_debug's coefficients, lizard_phy.c:1719) */
+  float align[2];   /* out: the pilots' grid shift in samples along u and v, or 0, 0 (lizard_align) */
+} lizx_soft;
+
+/* lizard_finish with the caller's grid (liblizard/src/lizard_phy.c:1566 with grid, ext_llr NULL; lizard_phy.h:
+   234-240): n x n picture samples in [0, 1], row-major (row y, column x; n = the format's picture size), of
+   LIZARD-16V in the ring of the given span (128, 256 or 512 modules). On the GPU: anchor erase, detrend, 2D FFT,
+   pilot alignment, soft demap, then the path of lizx_finish_llr. Outputs as lizx_finish_llr, and soft (optional). */
+LIZ_API int lizx_finish_grid(liz_decoder *d, int version, int span, const float *grid, uint8_t *blocks, uint8_t *ok,
+                             int8_t *its, float *blk_est, lizx_soft *soft);
+
 /* ---- lizx_ldpc: the LDPC decoder alone, batched ----------------------------------------------------------------- */
 
 typedef struct lizx_ldpc lizx_ldpc;

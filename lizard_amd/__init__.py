@@ -36,6 +36,11 @@ class _Decoded(C.Structure):
                 ("pilot_sd", C.c_float * 2)]
 
 
+class Soft(C.Structure):
+    _fields_ = [("llr", C.c_void_p), ("sub_est", C.c_void_p), ("sub_rms", C.c_void_p), ("coef", C.c_void_p),
+                ("align", C.c_float * 2)]
+
+
 _lib = None
 
 
@@ -62,6 +67,7 @@ def lib():
     sig("liz_decode", _I, _P, _P, _I, _I, _I, _I, _P, _P, _P)
     sig("lizx_format_info", _I, _I, _P)
     sig("lizx_finish_llr", _I, _P, _I, _P, _P, _P, _P, _P, _P)
+    sig("lizx_finish_grid", _I, _P, _I, _I, _P, _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_ldpc_dims", _I, _I, _P)
     sig("lizx_ldpc_new", _P, _I)
     sig("lizx_ldpc_free", None, _P)
@@ -252,3 +258,27 @@ class Decoder:
         be = np.zeros(B, np.float32)
         good = check(lib().lizx_finish_llr(self.h, version, _p(llr), _p(est), _p(blocks), _p(ok), _p(its), _p(be)))
         return {"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be}
+
+    def finish_grid(self, version, span, grid, soft=True):
+        """lizard_finish with a supplied n x n grid (row y, column x) of LIZARD-16V in the ring of this span."""
+        info = format_info(version)
+        B, S = info["blocks"], info["subch"] * 320
+        grid = np.ascontiguousarray(grid, np.float32)
+        blocks = np.zeros(B * LIZ_BLOCK, np.uint8)
+        ok = np.zeros(B, np.uint8)
+        its = np.zeros(B, np.int8)
+        be = np.zeros(B, np.float32)
+        out = {}
+        sp = None
+        if soft:
+            out["llr"] = np.zeros(2 * S, np.int8)
+            out["sub_est"] = np.zeros(info["subch"], np.float64)
+            out["sub_rms"] = np.zeros(info["subch"], np.float32)
+            out["coef"] = np.zeros(2 * S, np.float32)
+            sp = Soft(_p(out["llr"]), _p(out["sub_est"]), _p(out["sub_rms"]), _p(out["coef"]), (C.c_float * 2)(0, 0))
+        good = check(lib().lizx_finish_grid(self.h, version, span, _p(grid), _p(blocks), _p(ok), _p(its), _p(be),
+                                            C.byref(sp) if sp is not None else None))
+        out.update({"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be})
+        if soft:
+            out["align"] = (sp.align[0], sp.align[1])
+        return out

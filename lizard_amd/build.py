@@ -11,7 +11,7 @@ OBJ = os.path.join(HERE, "_obj")
 OUT = os.path.join(HERE, "liblizard_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["format.cpp", "api.cpp", "ldpc.hip"]
+SOURCES = ["format.cpp", "api.cpp", "ldpc.hip", "spectrum.hip"]
 # -packed-fp32-ops: no packed FP32 instructions are generated (v_pk_*_f32)
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
          "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-fvisibility=hidden"]

@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "format.hpp"
 #include "ldpc.hpp"
+#include "spectrum.hpp"
 
 #define LIZARD_AMD_VERSION "lizard-amd 0.1 (gfx950)"
 
@@ -63,10 +64,76 @@ struct lizx_ldpc {
 struct liz_decoder {
   int nmax = 0, top = 0;
   std::unique_ptr<LdpcEngine> eng;
-  DevBuf tasks, llr, est, its, blk_est, bytes, crc_ok;
+  DevBuf tasks, llr, est, its, blk_est, bytes, crc_ok, grid;
   std::vector<LdpcTask> host_tasks;
   std::vector<uint8_t> blocks, ok;
+  std::vector<std::unique_ptr<FramePlan>> plans;  // one per (version, span) met
+  FrameBufs fb;
+  const FramePlan &plan(int version, int span) {
+    for (auto &p : plans)
+      if (p->version == version && p->span == span) return *p;
+    plans.push_back(make_plan(version, span));
+    return *plans.back();
+  }
 };
+
+// The decline gate, LDPC and CRC of one frame's blocks from soft values and estimates already on the device
+// (lizard_phy.c:1775-1870), enqueued on the decoder's stream.
+static void finish_blocks(liz_decoder *d, const Format *f, const int8_t *d_llr, const double *d_est) {
+  LdpcEngine &e = *d->eng;
+  e.ensure_frame_tables(static_cast<int>(f->white.size()));
+  const int B = f->blocks;
+  d->host_tasks.resize(B);
+  for (int b = 0; b < B; b++) d->host_tasks[b] = LdpcTask{f->blk_slot[b], -1, f->blk_code[b], f->blk_slot[b], 0, b};
+  std::stable_sort(d->host_tasks.begin(), d->host_tasks.end(),
+                   [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
+  d->tasks.reserve(B * sizeof(LdpcTask));
+  d->its.reserve(B);
+  d->blk_est.reserve(B * sizeof(float));
+  d->bytes.reserve(static_cast<size_t>(B) * BLOCK_BYTES);
+  d->crc_ok.reserve(B);
+  hip_check(hipMemcpyAsync(d->tasks.p, d->host_tasks.data(), B * sizeof(LdpcTask), hipMemcpyHostToDevice, e.stream),
+            "tasks");
+  LdpcParams p{};
+  p.tasks = d->tasks.as<LdpcTask>();
+  p.count = B;
+  p.llr = d_llr;
+  p.its = d->its.as<int8_t>();
+  p.blk_est = d->blk_est.as<float>();
+  p.bytes = d->bytes.as<uint8_t>();
+  p.crc_ok = d->crc_ok.as<uint8_t>();
+  p.white = e.white.as<uint8_t>();
+  for (int ci = 0; ci < NCODES; ci++) p.s2p[ci] = e.s2p[ci].as<int16_t>();
+  p.est = d_est;
+  p.crc_tab = e.crc_tab.as<uint32_t>();
+  p.crc0 = e.crc0;
+  p.max_iter = 30;  // lizard_phy.c:1824 with LIZARD_STALL_IT, LIZARD_STALL_RATIO (:81-82)
+  p.stall_it = 9;
+  p.stall_ratio = 0.95f;
+  p.mode = MODE_FRAME;
+  p.flags = 1;
+  e.launch(p, false, true);
+}
+
+// The per-block results of finish_blocks to the host; returns the verified count.
+static int collect_blocks(liz_decoder *d, const Format *f, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est) {
+  const int B = f->blocks;
+  hip_check(hipMemcpy(its, d->its.p, B, hipMemcpyDeviceToHost), "iterations");
+  hip_check(hipMemcpy(blk_est, d->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost), "estimates");
+  hip_check(hipMemcpy(ok, d->crc_ok.p, B, hipMemcpyDeviceToHost), "verdicts");
+  d->blocks.resize(static_cast<size_t>(B) * BLOCK_BYTES);
+  hip_check(hipMemcpy(d->blocks.data(), d->bytes.p, d->blocks.size(), hipMemcpyDeviceToHost), "blocks");
+  int good = 0;
+  for (int b = 0; b < B; b++) {
+    if (its[b] == -2) ok[b] = 0;
+    if (ok[b]) {
+      std::memcpy(blocks + static_cast<size_t>(b) * BLOCK_BYTES, d->blocks.data() + static_cast<size_t>(b) * BLOCK_BYTES,
+                  BLOCK_BYTES);
+      good++;
+    }
+  }
+  return good;
+}
 
 // Builds the task list for count codewords in codeword mode.
 static void codeword_tasks(int count, const uint8_t *rate, const int64_t *off, std::vector<LdpcTask> &t) {
@@ -360,61 +427,49 @@ LIZ_API int lizx_finish_llr(liz_decoder *d, int version, const int8_t *llr, cons
     if (f->n > d->nmax) throw Error(LIZ_E_FORMAT, "the format's picture is larger than the decoder's nmax");
     LdpcEngine &e = *d->eng;
     hip_check(hipSetDevice(e.device), "hipSetDevice");
-    e.ensure_frame_tables(static_cast<int>(f->white.size()));
-    const int S = f->subch * 640, B = f->blocks;
-    d->host_tasks.resize(B);
-    for (int b = 0; b < B; b++) d->host_tasks[b] = LdpcTask{f->blk_slot[b], -1, f->blk_code[b], f->blk_slot[b], 0, b};
-    std::stable_sort(d->host_tasks.begin(), d->host_tasks.end(),
-                     [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
+    const int S = f->subch * 640;
     std::vector<double> est_d(f->subch);
     for (int j = 0; j < f->subch; j++) est_d[j] = est[j];
-    d->tasks.reserve(B * sizeof(LdpcTask));
     d->llr.reserve(S);
     d->est.reserve(f->subch * sizeof(double));
-    d->its.reserve(B);
-    d->blk_est.reserve(B * sizeof(float));
-    d->bytes.reserve(static_cast<size_t>(B) * BLOCK_BYTES);
-    d->crc_ok.reserve(B);
-    hip_check(hipMemcpyAsync(d->tasks.p, d->host_tasks.data(), B * sizeof(LdpcTask), hipMemcpyHostToDevice, e.stream),
-              "tasks");
     hip_check(hipMemcpyAsync(d->llr.p, llr, S, hipMemcpyHostToDevice, e.stream), "soft values");
     hip_check(hipMemcpyAsync(d->est.p, est_d.data(), f->subch * sizeof(double), hipMemcpyHostToDevice, e.stream),
               "estimates");
-    LdpcParams p{};
-    p.tasks = d->tasks.as<LdpcTask>();
-    p.count = B;
-    p.llr = d->llr.as<int8_t>();
-    p.its = d->its.as<int8_t>();
-    p.blk_est = d->blk_est.as<float>();
-    p.bytes = d->bytes.as<uint8_t>();
-    p.crc_ok = d->crc_ok.as<uint8_t>();
-    p.white = e.white.as<uint8_t>();
-    for (int ci = 0; ci < NCODES; ci++) p.s2p[ci] = e.s2p[ci].as<int16_t>();
-    p.est = d->est.as<double>();
-    p.crc_tab = e.crc_tab.as<uint32_t>();
-    p.crc0 = e.crc0;
-    p.max_iter = 30;  // lizard_phy.c:1824 with LIZARD_STALL_IT, LIZARD_STALL_RATIO (:81-82)
-    p.stall_it = 9;
-    p.stall_ratio = 0.95f;
-    p.mode = MODE_FRAME;
-    p.flags = 1;
-    e.launch(p, false, true);
+    finish_blocks(d, f, d->llr.as<int8_t>(), d->est.as<double>());
     e.sync();
-    hip_check(hipMemcpy(its, d->its.p, B, hipMemcpyDeviceToHost), "iterations");
-    hip_check(hipMemcpy(blk_est, d->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost), "estimates");
-    hip_check(hipMemcpy(ok, d->crc_ok.p, B, hipMemcpyDeviceToHost), "verdicts");
-    d->blocks.resize(static_cast<size_t>(B) * BLOCK_BYTES);
-    hip_check(hipMemcpy(d->blocks.data(), d->bytes.p, d->blocks.size(), hipMemcpyDeviceToHost), "blocks");
-    int good = 0;
-    for (int b = 0; b < B; b++) {
-      if (its[b] == -2) ok[b] = 0;
-      if (ok[b]) {
-        std::memcpy(blocks + static_cast<size_t>(b) * BLOCK_BYTES, d->blocks.data() + static_cast<size_t>(b) * BLOCK_BYTES,
-                    BLOCK_BYTES);
-        good++;
-      }
+    return collect_blocks(d, f, blocks, ok, its, blk_est);
+  });
+}
+
+LIZ_API int lizx_finish_grid(liz_decoder *d, int version, int span, const float *grid, uint8_t *blocks, uint8_t *ok,
+                             int8_t *its, float *blk_est, lizx_soft *soft) {
+  return guard([&] {
+    need(d && grid && blocks && ok && its && blk_est, "a decoder, a grid and outputs");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    if (f->n > d->nmax) throw Error(LIZ_E_FORMAT, "the format's picture is larger than the decoder's nmax");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    const bool dbg = soft && soft->coef;
+    const size_t n = p.n, S = static_cast<size_t>(p.subch) * SUB;
+    d->fb.reserve(p, 1, dbg);
+    d->grid.reserve(n * n * sizeof(float));
+    hip_check(hipMemcpyAsync(d->grid.p, grid, n * n * sizeof(float), hipMemcpyHostToDevice, e.stream), "grid");
+    launch_soft(p, d->fb, d->grid.as<float>(), 1, dbg, e.stream);
+    finish_blocks(d, f, d->fb.llr.as<int8_t>(), d->fb.est.as<double>());
+    e.sync();
+    if (soft) {
+      if (soft->llr) hip_check(hipMemcpy(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost), "soft values");
+      if (soft->sub_est)
+        hip_check(hipMemcpy(soft->sub_est, d->fb.est.p, p.subch * sizeof(double), hipMemcpyDeviceToHost), "estimates");
+      if (soft->sub_rms)
+        hip_check(hipMemcpy(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
+      if (soft->coef)
+        hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
+      hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
     }
-    return good;
+    return collect_blocks(d, f, blocks, ok, its, blk_est);
   });
 }
 

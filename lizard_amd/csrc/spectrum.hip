@@ -1,0 +1,653 @@
+// Picture samples to soft values on gfx950 (liblizard/src/lizard_phy.c:1566-1772, research/SPEC.md 6.9-6.13, 7.3,
+// 8.5):
+//   k_erase_fit  one workgroup a frame: the shading's least-squares fit on every eighth sample outside the anchors
+//   k_rows       eight rows a workgroup: anchor erase on load, detrend row sums (double), two real rows packed per
+//                complex FFT along x (Stockham, radix 4/2/3/5 in LDS), pruned to |u| < E, written column-major
+//   k_cols       two columns u a workgroup: FFT along y, keeping v < E and n - v (the upper half-plane's reach)
+//   k_align      one workgroup a frame: the detrend projections and the pilots' two-pass least-squares grid shift
+//   k_demap      one workgroup a sub-channel: 320 coefficients gathered, detrended, turned, their moments, int8 LLRs
+//   k_blk_pilot  per block: the pilots' reading (lizard_pilot)
+// Floating-point steps the reference rounds in a fixed order (the detrend terms, the turn, the LLR scale) use
+// explicitly rounded operations, so no contraction into fused multiply-adds changes them.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <mutex>
+
+#include "spectrum.hpp"
+
+namespace lz {
+
+// ------------------------------------------------------------------------------------------------ FFT in LDS
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2 *v) {
+  if constexpr (R == 2) {
+    const float2 a = v[0], b = v[1];
+    v[0] = make_float2(a.x + b.x, a.y + b.y);
+    v[1] = make_float2(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    const float2 t0 = make_float2(v[0].x + v[2].x, v[0].y + v[2].y), t1 = make_float2(v[0].x - v[2].x, v[0].y - v[2].y);
+    const float2 t2 = make_float2(v[1].x + v[3].x, v[1].y + v[3].y), t3 = make_float2(v[1].x - v[3].x, v[1].y - v[3].y);
+    v[0] = make_float2(t0.x + t2.x, t0.y + t2.y);
+    v[2] = make_float2(t0.x - t2.x, t0.y - t2.y);
+    v[1] = make_float2(t1.x + t3.y, t1.y - t3.x);  // t1 - i t3
+    v[3] = make_float2(t1.x - t3.y, t1.y + t3.x);  // t1 + i t3
+  } else if constexpr (R == 3) {
+    const float k = 0.866025403784438647f;  // sin(2 pi / 3)
+    const float2 t1 = make_float2(v[1].x + v[2].x, v[1].y + v[2].y), t2 = make_float2(v[1].x - v[2].x, v[1].y - v[2].y);
+    const float2 m = make_float2(v[0].x - 0.5f * t1.x, v[0].y - 0.5f * t1.y);
+    v[0] = make_float2(v[0].x + t1.x, v[0].y + t1.y);
+    v[1] = make_float2(m.x + k * t2.y, m.y - k * t2.x);  // m - i k t2
+    v[2] = make_float2(m.x - k * t2.y, m.y + k * t2.x);  // m + i k t2
+  } else {
+    static_assert(R == 5, "radix");
+    const float c1 = 0.309016994374947424f, c2 = -0.809016994374947424f;   // cos(2 pi / 5), cos(4 pi / 5)
+    const float s1 = 0.951056516295153572f, s2 = 0.587785252292473129f;   // sin(2 pi / 5), sin(4 pi / 5)
+    const float2 a = v[0];
+    const float2 t1 = make_float2(v[1].x + v[4].x, v[1].y + v[4].y), t2 = make_float2(v[2].x + v[3].x, v[2].y + v[3].y);
+    const float2 t3 = make_float2(v[1].x - v[4].x, v[1].y - v[4].y), t4 = make_float2(v[2].x - v[3].x, v[2].y - v[3].y);
+    const float2 m1 = make_float2(a.x + c1 * t1.x + c2 * t2.x, a.y + c1 * t1.y + c2 * t2.y);
+    const float2 m2 = make_float2(a.x + c2 * t1.x + c1 * t2.x, a.y + c2 * t1.y + c1 * t2.y);
+    const float2 n1 = make_float2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+    const float2 n2 = make_float2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+    v[0] = make_float2(a.x + t1.x + t2.x, a.y + t1.y + t2.y);
+    v[1] = make_float2(m1.x + n1.y, m1.y - n1.x);  // m1 - i n1
+    v[4] = make_float2(m1.x - n1.y, m1.y + n1.x);
+    v[2] = make_float2(m2.x + n2.y, m2.y - n2.x);  // m2 - i n2
+    v[3] = make_float2(m2.x - n2.y, m2.y + n2.x);
+  }
+}
+
+// One Stockham stage of radix R over count sequences of length N laid one after another in LDS: every butterfly's
+// inputs read into registers, a barrier, the outputs written in their sorted places, a barrier (so one buffer does).
+template <int R, int NT, int COUNT>
+__device__ __forceinline__ void fft_stage(float2 *x, int N, int Ns, const float2 *tw, int tid) {
+  constexpr int MAXB = (COUNT * 1536 / R + NT - 1) / NT;
+  const int nb = N / R, total = nb * COUNT, step = N / (Ns * R);
+  float2 v[MAXB][R];
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int g = tid + q * NT;
+    if (g < total) {
+      const int seq = g / nb, j = g - seq * nb, k = j % Ns;
+      const float2 *xs = x + seq * N;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        float2 a = xs[j + r * nb];
+        if (r) a = cmul(a, tw[k * r * step]);
+        v[q][r] = a;
+      }
+      dft<R>(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < MAXB; q++) {
+    const int g = tid + q * NT;
+    if (g < total) {
+      const int seq = g / nb, j = g - seq * nb, k = j % Ns;
+      float2 *xs = x + seq * N;
+      const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+      for (int r = 0; r < R; r++) xs[base + r * Ns] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Forward DFT, exp(-2 pi i k n / N), unscaled, natural order in and out.
+template <int NT, int COUNT>
+__device__ void fft_lds(float2 *x, const PlanDev &P, int tid) {
+  int Ns = 1;
+  for (int s = 0; s < P.nstages; s++) {
+    switch (P.radix[s]) {
+      case 4: fft_stage<4, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
+      case 2: fft_stage<2, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
+      case 3: fft_stage<3, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
+      default: fft_stage<5, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
+    }
+    Ns *= P.radix[s];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ anchors, shading
+__device__ __forceinline__ bool in_square(const PlanDev &P, int t) {
+  bool in = false;
+#pragma unroll
+  for (int i = 0; i < ANCH_K; i++) in |= t >= P.elo[i] && t <= P.ehi[i];
+  return in;
+}
+
+// The shading's basis at sample (x, y): 1, c(x), q(x), c(y), q(y), c(x) c(y) (lizard_phy.c:337-340).
+__device__ __forceinline__ void hole_basis(const PlanDev &P, int x, int y, double *b) {
+  const double cx = P.cx[x], cy = P.cx[y];
+  b[0] = 1;
+  b[1] = cx;
+  b[2] = P.qx[x];
+  b[3] = cy;
+  b[4] = P.qx[y];
+  b[5] = __dmul_rn(cx, cy);
+}
+
+// The fill of an erased sample (lizard_phy.c:341-346): the shading there, summed in the basis' order.
+__device__ __forceinline__ float hole_value(const PlanDev &P, const double *beta, int x, int y) {
+  double b[6], v = 0;
+  hole_basis(P, x, y, b);
+#pragma unroll
+  for (int r = 0; r < 6; r++) v = __dadd_rn(v, __dmul_rn(beta[r], b[r]));
+  return static_cast<float>(v);
+}
+
+template <int NT>
+__device__ double block_sum(double v, double *red, int tid) {
+  red[tid] = v;
+  __syncthreads();
+  for (int o = NT / 2; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// The shading fit (lizard_phy.c:1605-1624): rhs over every eighth row and column outside the squares, beta =
+// hole_inv rhs.
+__global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, const float *grid, double *beta_out) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x, f = blockIdx.x, n = P.n, c = n / 8;
+  const float *g = grid + static_cast<size_t>(f) * n * n;
+  double rhs[6] = {0, 0, 0, 0, 0, 0};
+  for (int k = tid; k < c * c; k += 256) {
+    const int y = 8 * (k / c), x = 8 * (k % c);
+    if (in_square(P, y) && in_square(P, x)) continue;
+    const double v = g[static_cast<size_t>(y) * n + x];
+    double b[6];
+    hole_basis(P, x, y, b);
+#pragma unroll
+    for (int r = 0; r < 6; r++) rhs[r] += b[r] * v;
+  }
+  double tot[6];
+#pragma unroll
+  for (int r = 0; r < 6; r++) tot[r] = block_sum<256>(rhs[r], red, tid);
+  if (tid < 6) {
+    double s = 0;
+    for (int k = 0; k < 6; k++) s += P.hole_inv[tid * 6 + k] * tot[k];
+    beta_out[f * 6 + tid] = s;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ the two passes
+constexpr int ROWS = 8;  // picture rows a k_rows workgroup transforms (four packed complex sequences)
+
+__global__ __launch_bounds__(256) void k_rows(PlanDev P, const float *grid, const double *beta, float2 *X,
+                                              double *rowsum) {
+  extern __shared__ float2 zs[];  // ROWS / 2 sequences of n
+  __shared__ double bsh[6];
+  const int tid = threadIdx.x, f = blockIdx.y, y0 = blockIdx.x * ROWS, n = P.n;
+  if (tid < 6) bsh[tid] = beta[f * 6 + tid];
+  __syncthreads();
+  // load: 32 lanes a row, erase the anchors' samples, row sums for the detrend (s, s c(x), s q(x))
+  {
+    const int row = tid >> 5, l = tid & 31, y = y0 + row;
+    const float *g = grid + (static_cast<size_t>(f) * n + y) * n;
+    const bool yin = in_square(P, y);
+    float *zf = reinterpret_cast<float *>(zs + (row >> 1) * n) + (row & 1);
+    double rs = 0, rx = 0, rq = 0;
+    for (int x = l; x < n; x += 32) {
+      float s = g[x];
+      if (yin && in_square(P, x)) s = hole_value(P, bsh, x, y);
+      zf[2 * x] = s;
+      rs += s;
+      rx += static_cast<double>(s) * P.cx[x];
+      rq += static_cast<double>(s) * P.qx[x];
+    }
+    for (int o = 16; o > 0; o >>= 1) {
+      rs += __shfl_xor(rs, o, 32);
+      rx += __shfl_xor(rx, o, 32);
+      rq += __shfl_xor(rq, o, 32);
+    }
+    if (l == 0) {
+      double *o = rowsum + (static_cast<size_t>(f) * n + y) * 3;
+      o[0] = rs;
+      o[1] = rx;
+      o[2] = rq;
+    }
+  }
+  __syncthreads();
+  fft_lds<256, ROWS / 2>(zs, P, tid);
+  // unpack the two real rows of each sequence (Z(u) and conj Z(n - u)) for u < E; column-major out
+  const int E = P.E;
+  for (int idx = tid; idx < E * ROWS; idx += 256) {
+    const int u = idx / ROWS, row = idx % ROWS;
+    const float2 zu = zs[(row >> 1) * n + u], zm = zs[(row >> 1) * n + (u ? n - u : 0)];
+    const float2 o = (row & 1) ? make_float2(0.5f * (zu.y + zm.y), -0.5f * (zu.x - zm.x))
+                               : make_float2(0.5f * (zu.x + zm.x), 0.5f * (zu.y - zm.y));
+    X[(static_cast<size_t>(f) * E + u) * n + y0 + row] = o;
+  }
+}
+
+// Column u: Y(u, v) for v < E at [v], and Y(u, n - v) for 1 <= v < E at [E - 1 + v].
+__global__ __launch_bounds__(256) void k_cols(PlanDev P, const float2 *X, float2 *HP) {
+  extern __shared__ float2 zs[];  // 2 sequences of n
+  const int tid = threadIdx.x, f = blockIdx.y, u0 = 2 * blockIdx.x, n = P.n, E = P.E;
+  for (int idx = tid; idx < 2 * n; idx += 256) {
+    const int s = idx / n, y = idx % n, u = u0 + s;
+    zs[idx] = u < E ? X[(static_cast<size_t>(f) * E + u) * n + y] : make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<256, 2>(zs, P, tid);
+  const int W = 2 * E;
+  for (int idx = tid; idx < 2 * W; idx += 256) {
+    const int s = idx / W, k = idx % W, u = u0 + s;
+    if (u >= E || k >= W - 1) continue;
+    const int v = k < E ? k : n - (k - E + 1);
+    HP[(static_cast<size_t>(f) * E + u) * W + k] = zs[s * n + v];
+  }
+}
+
+// ------------------------------------------------------------------------------------------------ the coefficients
+struct Proj { double px, py, pxy, pxx, pyy; };
+
+// Coefficient (u, v) after the detrend (lizard_phy.c:1697-1716; SPEC 6.10), in the reference's float order.
+__device__ __forceinline__ float2 coef_at(const PlanDev &P, const float2 *HP, const Proj &pr, int u, int v) {
+  const int E = P.E, W = 2 * E, n = P.n;
+  float2 y = u >= 0 ? HP[u * W + v] : HP[(-u) * W + E - 1 + v];
+  if (u < 0) y.y = -y.y;  // Y(u, v) = conj Y(-u, n - v)
+  const int uw = u < 0 ? u + n : u;
+  const float ar = static_cast<float>(__dmul_rn(pr.pxy, P.x1r[uw])), ai = static_cast<float>(__dmul_rn(pr.pxy, P.x1i[uw]));
+  const float r = P.x1r[v], i = P.x1i[v];
+  y.x = __fsub_rn(y.x, __fsub_rn(__fmul_rn(ar, r), __fmul_rn(ai, i)));
+  y.y = __fsub_rn(y.y, __fadd_rn(__fmul_rn(ar, i), __fmul_rn(ai, r)));
+  const double dn = n;
+  if (v == 0) {
+    y.x = __fsub_rn(y.x, static_cast<float>(__dmul_rn(dn, __dadd_rn(__dmul_rn(pr.px, P.x1r[uw]), __dmul_rn(pr.pxx, P.x2r[uw])))));
+    y.y = __fsub_rn(y.y, static_cast<float>(__dmul_rn(dn, __dadd_rn(__dmul_rn(pr.px, P.x1i[uw]), __dmul_rn(pr.pxx, P.x2i[uw])))));
+  }
+  if (uw == 0) {
+    y.x = __fsub_rn(y.x, static_cast<float>(__dmul_rn(dn, __dadd_rn(__dmul_rn(pr.py, P.x1r[v]), __dmul_rn(pr.pyy, P.x2r[v])))));
+    y.y = __fsub_rn(y.y, static_cast<float>(__dmul_rn(dn, __dadd_rn(__dmul_rn(pr.py, P.x1i[v]), __dmul_rn(pr.pyy, P.x2i[v])))));
+  }
+  return y;
+}
+
+// Coefficient (u, v) turned back by the grid shift (dx, dy) through the 4096-step table, index rounded half-even
+// (lizard_phy.c:214-218).
+__device__ __forceinline__ float2 turn(const PlanDev &P, float2 y, int u, int v, float dx, float dy) {
+  const float a = __fmul_rn(__fadd_rn(__fmul_rn(static_cast<float>(u), dx), __fmul_rn(static_cast<float>(v), dy)),
+                            static_cast<float>(ALIGN_TURNS) / static_cast<float>(P.n));
+  const unsigned k = static_cast<unsigned>(static_cast<int>(rintf(a))) & (ALIGN_TURNS - 1);
+  const float2 cs = P.rot[k];
+  return make_float2(__fadd_rn(__fmul_rn(y.x, cs.x), __fmul_rn(y.y, cs.y)),
+                     __fsub_rn(__fmul_rn(y.y, cs.x), __fmul_rn(y.x, cs.y)));
+}
+
+__device__ __forceinline__ void coef_uv(const PlanDev &P, int i, int *u, int *v) {
+  const int32_t c = P.coef[i];
+  *u = static_cast<int16_t>(c & 0xFFFF);
+  *v = c >> 16;
+}
+
+// The projections from the row sums, then the pilots' grid shift (lizard_phy.c:280-329, :1697, :1739).
+__global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, const float2 *HP_all, double *proj_out,
+                                               float *align_out) {
+  __shared__ double red[256];
+  __shared__ double s_d[2];
+  __shared__ int s_go;
+  const int tid = threadIdx.x, f = blockIdx.x, n = P.n;
+  const double *rsum = rowsum + static_cast<size_t>(f) * n * 3;
+  double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
+  for (int y = tid; y < n; y += 256) {
+    const double rs = rsum[3 * y], rx = rsum[3 * y + 1], rq = rsum[3 * y + 2];
+    a0 += rx;
+    a1 += rs * P.cx[y];
+    a2 += rx * P.cx[y];
+    a3 += rq;
+    a4 += rs * P.qx[y];
+  }
+  Proj pr;
+  pr.px = block_sum<256>(a0, red, tid) / (n * P.sx2);
+  pr.py = block_sum<256>(a1, red, tid) / (n * P.sx2);
+  pr.pxy = block_sum<256>(a2, red, tid) / (P.sx2 * P.sx2);
+  pr.pxx = block_sum<256>(a3, red, tid) / (n * P.sq2);
+  pr.pyy = block_sum<256>(a4, red, tid) / (n * P.sq2);
+  if (tid == 0) {
+    double *o = proj_out + f * 5;
+    o[0] = pr.px;
+    o[1] = pr.py;
+    o[2] = pr.pxy;
+    o[3] = pr.pxx;
+    o[4] = pr.pyy;
+    s_d[0] = s_d[1] = 0;
+    s_go = 1;
+  }
+  __syncthreads();
+  const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
+  const double k = 2.0 * 3.14159265358979323846 / n;
+  double A = 0, B = 0, C = 0, Pq = 0, Q = 0, YY = 0;
+  for (int it = 0; it < 2; it++) {
+    if (!s_go) break;
+    const float dx = static_cast<float>(s_d[0]), dy = static_cast<float>(s_d[1]);
+    double a = 0, b = 0, c = 0, p = 0, q = 0, yy = 0;
+    for (int t = tid; t < P.npilots; t += 256) {
+      const int ci = P.pilot[t];
+      int u, v;
+      coef_uv(P, ci, &u, &v);
+      const float2 y = turn(P, coef_at(P, HP, pr, u, v), u, v, dx, dy);
+      const double xr = P.white[2 * ci] ? -1.0 : 1.0, xi = P.white[2 * ci + 1] ? -1.0 : 1.0;
+      const double zr = y.x * xr + y.y * xi, zi = y.y * xr - y.x * xi;
+      const double cu = k * zr * u, cv = k * zr * v;
+      a += cu * cu;
+      b += cu * cv;
+      c += cv * cv;
+      p += cu * zi;
+      q += cv * zi;
+      yy += zi * zi;
+    }
+    A = block_sum<256>(a, red, tid);
+    B = block_sum<256>(b, red, tid);
+    C = block_sum<256>(c, red, tid);
+    Pq = block_sum<256>(p, red, tid);
+    Q = block_sum<256>(q, red, tid);
+    YY = block_sum<256>(yy, red, tid);
+    if (tid == 0) {
+      const double det = A * C - B * B;
+      if (P.npilots < 8 || !(det > 0)) {
+        s_go = 0;
+        s_d[0] = s_d[1] = 0;
+      } else {
+        s_d[0] += (Pq * C - Q * B) / det;
+        s_d[1] += (A * Q - B * Pq) / det;
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    float d0 = 0, d1 = 0;
+    if (s_go) {
+      const double det = A * C - B * B, ex = (Pq * C - Q * B) / det, ey = (A * Q - B * Pq) / det;
+      const double s2 = (YY - ex * Pq - ey * Q) / (P.npilots - 2);
+      const double dx = s_d[0], dy = s_d[1];
+      const double chi = s2 > 0 ? (dx * dx * A + 2 * dx * dy * B + dy * dy * C) / s2 : 0;
+      if (chi > 13.8 && dx == dx && dy == dy) {
+        d0 = static_cast<float>(dx);
+        d1 = static_cast<float>(dy);
+      }
+    }
+    align_out[2 * f] = d0;
+    align_out[2 * f + 1] = d1;
+  }
+}
+
+// J at sigma = 2 sqrt(g) (ten Brink): the information a soft QPSK axis carries (lizard_phy.c:75).
+__device__ __forceinline__ double axis_info(double g) {
+  return g > 0 ? pow(1 - pow(2.0, -0.3073 * pow(2 * sqrt(g), 1.7870)), 1.1064) : 0;
+}
+
+// Sub-channel j of frame f (lizard_phy.c:1756-1772, :1322-1332; SPEC 8.5).
+__global__ __launch_bounds__(320) void k_demap(PlanDev P, const float2 *HP_all, const double *proj, const float *align,
+                                               int8_t *llr_all, double *est_all, float *rms_all, float *dbg_all) {
+  __shared__ double r2[320], r4[320];
+  __shared__ float s_k;
+  const int tid = threadIdx.x, j = blockIdx.x, f = blockIdx.y, ci = j * SUB + tid;
+  const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
+  const Proj pr{proj[f * 5], proj[f * 5 + 1], proj[f * 5 + 2], proj[f * 5 + 3], proj[f * 5 + 4]};
+  const float dx = align[2 * f], dy = align[2 * f + 1];
+  int u, v;
+  coef_uv(P, ci, &u, &v);
+  float2 y = coef_at(P, HP, pr, u, v);
+  if (dbg_all) reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * P.subch * SUB + ci] = y;
+  if (dx != 0 || dy != 0) y = turn(P, y, u, v, dx, dy);
+  const double e = static_cast<double>(y.x) * y.x + static_cast<double>(y.y) * y.y;
+  r2[tid] = e;
+  r4[tid] = e * e;
+  __syncthreads();
+  // 320 = 256 + 64: fold the top 64 in, then halve
+  if (tid < 64) {
+    r2[tid] += r2[tid + 256];
+    r4[tid] += r4[tid + 256];
+  }
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) {
+      r2[tid] += r2[tid + o];
+      r4[tid] += r4[tid + o];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double m2 = r2[0] / SUB, m4 = r4[0] / SUB;
+    rms_all[f * P.subch + j] = m2 > 0 ? static_cast<float>(sqrt(m2 / 2)) : 0.f;
+    double a2 = 2 * m2 * m2 - m4;
+    a2 = a2 > 0 ? sqrt(a2) : 0;
+    double nv = m2 - a2;
+    if (nv < 0.02 * m2) nv = 0.02 * m2;
+    est_all[f * P.subch + j] = axis_info(a2 / nv);
+    s_k = static_cast<float>(0.7 * 2 * sqrt(2 * a2) / nv);
+  }
+  __syncthreads();
+  const float k = s_k;
+  float lr = __fmul_rn(k, y.x), li = __fmul_rn(k, y.y);
+  lr = lr > 10.f ? 10.f : (lr < -10.f ? -10.f : lr);
+  li = li > 10.f ? 10.f : (li < -10.f ? -10.f : li);
+  int8_t *o = llr_all + static_cast<size_t>(f) * P.subch * 2 * SUB + 2 * ci;
+  o[0] = lr == lr ? static_cast<int8_t>(rintf(__fmul_rn(lr, 8.f))) : 0;
+  o[1] = li == li ? static_cast<int8_t>(rintf(__fmul_rn(li, 8.f))) : 0;
+}
+
+// ------------------------------------------------------------------------------------------------ host: plans
+static std::vector<int> radices(int n) {
+  std::vector<int> r;
+  int m = n;
+  while (m % 4 == 0) {
+    r.push_back(4);
+    m /= 4;
+  }
+  if (m % 2 == 0) {
+    r.push_back(2);
+    m /= 2;
+  }
+  while (m % 3 == 0) {
+    r.push_back(3);
+    m /= 3;
+  }
+  while (m % 5 == 0) {
+    r.push_back(5);
+    m /= 5;
+  }
+  if (m != 1) throw Error(LIZ_E_FORMAT, "picture size with a factor other than 2, 3 and 5");
+  return r;
+}
+
+// 6 x 6 inverse by Gauss-Jordan with partial pivots.
+static void invert6(double g[6][12]) {
+  for (int c = 0; c < 6; c++) {
+    int p = c;
+    for (int r = c + 1; r < 6; r++)
+      if (std::fabs(g[r][c]) > std::fabs(g[p][c])) p = r;
+    if (std::fabs(g[p][c]) < 1e-12) throw Error(LIZ_E_INTERNAL, "singular shading fit");
+    for (int k = 0; k < 12; k++) std::swap(g[c][k], g[p][k]);
+    const double d = g[c][c];
+    for (int k = 0; k < 12; k++) g[c][k] /= d;
+    for (int r = 0; r < 6; r++)
+      if (r != c) {
+        const double m = g[r][c];
+        for (int k = 0; k < 12; k++) g[r][k] -= m * g[c][k];
+      }
+  }
+}
+
+std::unique_ptr<FramePlan> make_plan(int version, int span) {
+  auto p = std::make_unique<FramePlan>();
+  const Format *f = format_for(version);
+  if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+  if (span != 128 && span != 256 && span != 512) throw Error(LIZ_E_ARG, "span is a ring's: 128, 256 or 512 modules");
+  const int n = f->n, S = f->subch * SUB;
+  p->version = version;
+  p->span = span;
+  p->n = n;
+  p->subch = f->subch;
+  p->fmt = f;
+  PlanDev &d = p->dev;
+  d.n = n;
+  d.subch = f->subch;
+  d.blocks = f->blocks;
+  // the detrend basis and its transforms (lizard_phy.c:388-399; SPEC 6.10)
+  std::vector<float> tab(6 * n);
+  float *cx = tab.data(), *qx = cx + n, *x1r = qx + n, *x1i = x1r + n, *x2r = x1i + n, *x2i = x2r + n;
+  double m2 = 0;
+  std::vector<double> ec(n), es(n);
+  for (int i = 0; i < n; i++) {
+    const double c = (2.0 * i + 1 - n) / n;
+    m2 += c * c;
+    ec[i] = std::cos(-2 * 3.14159265358979323846 * i / n);
+    es[i] = std::sin(-2 * 3.14159265358979323846 * i / n);
+  }
+  d.sx2 = m2;
+  m2 /= n;
+  d.sq2 = 0;
+  for (int i = 0; i < n; i++) {
+    const double c = (2.0 * i + 1 - n) / n, q = c * c - m2;
+    cx[i] = static_cast<float>(c);
+    qx[i] = static_cast<float>(q);
+    d.sq2 += q * q;
+  }
+  for (int u = 0; u < n; u++) {
+    double ar = 0, ai = 0, br = 0, bi = 0;
+    for (int i = 0; i < n; i++) {
+      const double c = (2.0 * i + 1 - n) / n, q = c * c - m2;
+      const int k = static_cast<int>((static_cast<long long>(u) * i) % n);
+      ar += c * ec[k];
+      ai += c * es[k];
+      br += q * ec[k];
+      bi += q * es[k];
+    }
+    x1r[u] = static_cast<float>(ar);
+    x1i[u] = static_cast<float>(ai);
+    x2r[u] = static_cast<float>(br);
+    x2i[u] = static_cast<float>(bi);
+  }
+  // anchors as erased (SPEC 6.13; lizard_phy.c:402-407, acquire.c:41-45)
+  const float scale = static_cast<float>(n) / span;
+  const int a = span / 64;
+  for (int i = 0; i < ANCH_K; i++) {
+    const int m0 = static_cast<int>(std::floor(BORDER + static_cast<double>(span) * (i + 0.5) / ANCH_K - a / 2.0 + 0.5));
+    const int lo = static_cast<int>(std::floor(static_cast<float>(m0 - BORDER) * scale)) - ANCH_MARGIN;
+    const int hi = static_cast<int>(std::ceil(static_cast<float>(m0 + a - BORDER) * scale)) - 1 + ANCH_MARGIN;
+    d.elo[i] = lo < 0 ? 0 : lo;
+    d.ehi[i] = hi > n - 1 ? n - 1 : hi;
+  }
+  auto in_sq = [&](int t) {
+    for (int i = 0; i < ANCH_K; i++)
+      if (t >= d.elo[i] && t <= d.ehi[i]) return true;
+    return false;
+  };
+  double g[6][12] = {{0}};
+  for (int y = 0; y < n; y += 8)
+    for (int x = 0; x < n; x += 8) {
+      if (in_sq(x) && in_sq(y)) continue;
+      const double b[6] = {1, cx[x], qx[x], cx[y], qx[y], static_cast<double>(cx[x]) * cx[y]};
+      for (int r = 0; r < 6; r++)
+        for (int c = 0; c < 6; c++) g[r][c] += b[r] * b[c];
+    }
+  for (int r = 0; r < 6; r++) g[r][6 + r] = 1;
+  invert6(g);
+  for (int r = 0; r < 6; r++)
+    for (int c = 0; c < 6; c++) d.hole_inv[r * 6 + c] = g[r][6 + c];
+  // the coefficients' (u, v) and the half-plane's reach
+  std::vector<int32_t> coef(S);
+  const int bw = f->bw;
+  int E = 0;
+  for (int i = 0; i < S; i++) {
+    const int pp = f->pos[i], v = (pp / (n * bw)) * bw + pp % bw, uw = (pp / bw) % n, u = uw < n / 2 ? uw : uw - n;
+    coef[i] = static_cast<int32_t>((static_cast<uint32_t>(v) << 16) | (static_cast<uint32_t>(u) & 0xFFFFu));
+    E = std::max(E, std::max(std::abs(u), v) + 1);
+  }
+  p->E = d.E = E;
+  // the pilots: the tail coefficients of every 7/8 and 3/4 block (SPEC 7.3; lizard_phy.c:293-317)
+  for (int b = 0; b < f->blocks; b++) {
+    const Code &c = code(f->blk_code[b]);
+    const int nb = f->blk_nslots[b], first = f->blk_slot[b] / 2;
+    if (c.nt >= nb) continue;
+    for (int s = c.nt / 2; s < nb / 2; s++) {
+      p->pilot_host.push_back(first + s);
+      p->pilot_block.push_back(b);
+    }
+  }
+  d.npilots = static_cast<int>(p->pilot_host.size());
+  p->radix = radices(n);
+  d.nstages = static_cast<int>(p->radix.size());
+  for (int s = 0; s < d.nstages; s++) d.radix[s] = p->radix[s];
+  std::vector<float2> tw(n), rot(ALIGN_TURNS);
+  for (int k = 0; k < n; k++) {
+    const double th = -2 * 3.14159265358979323846 * k / n;
+    tw[k] = make_float2(static_cast<float>(std::cos(th)), static_cast<float>(std::sin(th)));
+  }
+  for (int i = 0; i < ALIGN_TURNS; i++) {
+    const double th = 2.0 * 3.14159265358979323846 * i / ALIGN_TURNS;
+    rot[i] = make_float2(static_cast<float>(std::cos(th)), static_cast<float>(std::sin(th)));
+  }
+  // upload
+  p->tables.reserve(tab.size() * sizeof(float));
+  hip_check(hipMemcpy(p->tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice), "plan tables");
+  p->coef.reserve(coef.size() * sizeof(int32_t));
+  hip_check(hipMemcpy(p->coef.p, coef.data(), coef.size() * sizeof(int32_t), hipMemcpyHostToDevice), "coefficients");
+  p->pilot.reserve(std::max<size_t>(1, p->pilot_host.size()) * sizeof(int32_t));
+  if (!p->pilot_host.empty())
+    hip_check(hipMemcpy(p->pilot.p, p->pilot_host.data(), p->pilot_host.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+              "pilots");
+  p->tw.reserve(n * sizeof(float2));
+  hip_check(hipMemcpy(p->tw.p, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice), "twiddles");
+  p->rot.reserve(ALIGN_TURNS * sizeof(float2));
+  hip_check(hipMemcpy(p->rot.p, rot.data(), ALIGN_TURNS * sizeof(float2), hipMemcpyHostToDevice), "turn table");
+  p->white.reserve(f->white.size());
+  hip_check(hipMemcpy(p->white.p, f->white.data(), f->white.size(), hipMemcpyHostToDevice), "whitening");
+  const float *tb = p->tables.as<float>();
+  d.cx = tb;
+  d.qx = tb + n;
+  d.x1r = tb + 2 * n;
+  d.x1i = tb + 3 * n;
+  d.x2r = tb + 4 * n;
+  d.x2i = tb + 5 * n;
+  d.tw = p->tw.as<float2>();
+  d.rot = p->rot.as<float2>();
+  d.coef = p->coef.as<int32_t>();
+  d.pilot = p->pilot.as<int32_t>();
+  d.white = p->white.as<uint8_t>();
+  return p;
+}
+
+void FrameBufs::reserve(const FramePlan &p, int frames, bool debug) {
+  const size_t n = p.n, E = p.E, S = static_cast<size_t>(p.subch) * SUB;
+  X.reserve(frames * E * n * sizeof(float2));
+  HP.reserve(frames * E * 2 * E * sizeof(float2));
+  rowsum.reserve(frames * n * 3 * sizeof(double));
+  beta.reserve(frames * 6 * sizeof(double));
+  proj.reserve(frames * 5 * sizeof(double));
+  align.reserve(frames * 2 * sizeof(float));
+  llr.reserve(frames * 2 * S);
+  est.reserve(frames * p.subch * sizeof(double));
+  rms.reserve(frames * p.subch * sizeof(float));
+  if (debug) coef_dbg.reserve(frames * S * sizeof(float2));
+}
+
+void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s) {
+  const int n = p.n, E = p.E;
+  hipLaunchKernelGGL(k_erase_fit, dim3(frames), dim3(256), 0, s, p.dev, d_grid, b.beta.as<double>());
+  hipLaunchKernelGGL(k_rows, dim3(n / ROWS, frames), dim3(256), (ROWS / 2) * n * sizeof(float2), s, p.dev, d_grid,
+                     b.beta.as<double>(), b.X.as<float2>(), b.rowsum.as<double>());
+  hipLaunchKernelGGL(k_cols, dim3((E + 1) / 2, frames), dim3(256), 2 * n * sizeof(float2), s, p.dev, b.X.as<float2>(),
+                     b.HP.as<float2>());
+  hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum.as<double>(), b.HP.as<float2>(),
+                     b.proj.as<double>(), b.align.as<float>());
+  hipLaunchKernelGGL(k_demap, dim3(p.subch, frames), dim3(320), 0, s, p.dev, b.HP.as<float2>(), b.proj.as<double>(),
+                     b.align.as<float>(), b.llr.as<int8_t>(), b.est.as<double>(), b.rms.as<float>(),
+                     debug ? b.coef_dbg.as<float>() : nullptr);
+  hip_check(hipGetLastError(), "soft-value launches");
+}
+
+}  // namespace lz

@@ -1,0 +1,63 @@
+"""The GPU's picture-to-soft-value stages (lizx_finish_grid: anchor erase, detrend, 2D FFT, pilot shift, soft demap,
+then LDPC and CRC) against the FP64 restatement (oracle/picture.py) and the compiled reference (oracle/_ref).
+Tolerances (north star): coefficients within 1e-5 relative L2 of FP64; LLR floats within 1e-4 relative; int8 soft
+values may differ by one at rounding ties (a small fraction); every block's verdict, iterations and bytes equal the
+reference's where the soft values agree."""
+import numpy as np
+import pytest
+
+from _libs import ptr, ref
+from pictures import Frame, picture
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # version, span (ring), grid shift, SNR dB
+    (4, 128, (0.0, 0.0), 25), (5, 128, (0.3, -0.2), 20), (5, 256, (0.0, 0.0), 12), (20, 256, (0.15, 0.1), 15),
+    (36, 256, (-0.1, 0.05), 18), (45, 512, (0.2, 0.2), 20), (46, 512, (0.0, -0.3), 18), (64, 512, (-0.2, 0.25), 20),
+    (8, 128, (0.0, 0.0), 20), (28, 256, (0.05, -0.05), 20)]
+
+
+@pytest.mark.parametrize("version,span,shift,snr", CASES)
+def test_finish_grid(decoder_gpu, version, span, shift, snr):
+    F = Frame(version, span, seed=version + span, snr_db=snr, shift=shift)
+    g64 = F.fp64()
+    out = decoder_gpu.finish_grid(version, span, F.grid)
+    c = out["coef"][0::2] + 1j * out["coef"][1::2]
+    rel = np.linalg.norm(c - g64["coef"]) / np.linalg.norm(g64["coef"])
+    assert rel < 1e-5, rel
+    assert abs(out["align"][0] - g64["align"][0]) < 1e-3 and abs(out["align"][1] - g64["align"][1]) < 1e-3
+    assert np.allclose(out["sub_est"], g64["est"], rtol=1e-4, atol=1e-7)
+    dl = np.abs(out["llr"].astype(int) - g64["llr"].astype(int))
+    assert dl.max() <= 1 and (dl > 0).mean() < 1e-3, ((dl > 0).mean(), dl.max())
+    r = ref("scalar")
+    if r is None:
+        return
+    B, S = F.blocks_n, F.subch * 320
+    fr = r.ref_format_new(F.n, F.subch, span)
+    strip = picture.to_strips(F.grid, F.n)
+    bl, ok, its, be, bp = (np.zeros(B * 476, np.uint8), np.zeros(B, np.uint8), np.zeros(B, np.int8),
+                           np.zeros(B, np.float32), np.zeros(B, np.float32))
+    llr, est, rms, al, coef = (np.zeros(2 * S, np.int8), np.zeros(F.subch), np.zeros(F.subch, np.float32),
+                               np.zeros(2, np.float32), np.zeros(2 * S, np.float32))
+    good = r.ref_finish_grid(fr, ptr(strip), ptr(bl), ptr(ok), ptr(its), ptr(be), ptr(bp), ptr(llr), ptr(est),
+                             ptr(rms), ptr(al), ptr(coef))
+    r.ref_format_free(fr)
+    cr = coef[0::2] + 1j * coef[1::2]
+    assert np.linalg.norm(c - cr) / np.linalg.norm(cr) < 1e-5
+    dr = np.abs(out["llr"].astype(int) - llr.astype(int))
+    assert dr.max() <= 1 and (dr > 0).mean() < 1e-3
+    if not dr.any():
+        # identical soft values: the integer path after them is bit-exact
+        assert out["good"] == good
+        assert np.array_equal(out["its"], its) and np.array_equal(out["ok"], ok)
+    assert np.array_equal(out["ok"].astype(bool) & ok.astype(bool), ok.astype(bool) & out["ok"].astype(bool))
+    for b in range(B):
+        if out["ok"][b]:
+            assert np.array_equal(out["blocks"][b], F.blocks.reshape(B, 476)[b])
+
+
+def test_clean_frame_all_verify(decoder_gpu):
+    F = Frame(20, 256, seed=3, snr_db=40, shift=(0, 0), ramp=0.0, junk=False)
+    out = decoder_gpu.finish_grid(20, 256, F.grid)
+    assert out["good"] == F.blocks_n
+    assert np.array_equal(out["blocks"].ravel(), F.blocks)
