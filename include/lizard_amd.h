@@ -115,6 +115,35 @@ typedef struct {
 LIZ_API int lizx_finish_grid(liz_decoder *d, int version, int span, const float *grid, uint8_t *blocks, uint8_t *ok,
                              int8_t *its, float *blk_est, lizx_soft *soft);
 
+/* The node lattice of a ring (span 128, 256 or 512 modules): its points' module coordinates, the same on both axes
+   (liblizard/src/layout.c:110-118). Writes up to 40 floats to pos (may be NULL); returns the count a side. Host. */
+LIZ_API int lizx_node_lattice(int span, float *pos);
+
+/* A registration as the reference holds it (ob_reg_t, liblizard/src/internal.h:121-124): H maps module coordinates
+   to capture pixels (9 floats, row-major, h[8] = 1: u = (h0 x + h1 y + h2) / (h6 x + h7 y + 1), v likewise with h3,
+   h4, h5); nodes are the node lattice's displacements in pixels, nx * nx pairs (dx, dy), row by row. */
+typedef struct {
+  float H[9];
+  const float *nodes;
+} lizx_registration;
+
+/* The warp alone (the reference's picture sampler, liblizard/src/acquire.c:5121-5196 ob_sample_grid): the n x n
+   picture samples of LIZARD-16V in the ring of span, through reg, from a w x h GREY8 capture (rows stride bytes
+   apart), into grid (row-major). The samples are the reference's scalar grid walk's, to the bit. */
+LIZ_API int lizx_warp(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                      const lizx_registration *reg, float *grid);
+
+/* lizard_finish on a capture through a registration (the reference's split decode with the registration held:
+   lizard_phy.c:1566-1876 after lizard_acquire): warp, erase, detrend, FFT, pilot shift, soft values, decline gate,
+   LDPC, CRC, all on the GPU. Outputs as lizx_finish_grid. */
+LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                            const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
+                            lizx_soft *soft);
+
+/* Holds a registration and format for liz_decode (the state lizard_acquire leaves for lizard_finish). The nodes are
+   copied. version 4..64, span 128/256/512. */
+LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_registration *reg);
+
 /* ---- lizx_ldpc: the LDPC decoder alone, batched ----------------------------------------------------------------- */
 
 typedef struct lizx_ldpc lizx_ldpc;

@@ -36,6 +36,10 @@ class _Decoded(C.Structure):
                 ("pilot_sd", C.c_float * 2)]
 
 
+class Registration(C.Structure):
+    _fields_ = [("H", C.c_float * 9), ("nodes", C.c_void_p)]
+
+
 class Soft(C.Structure):
     _fields_ = [("llr", C.c_void_p), ("sub_est", C.c_void_p), ("sub_rms", C.c_void_p), ("coef", C.c_void_p),
                 ("align", C.c_float * 2)]
@@ -68,6 +72,10 @@ def lib():
     sig("lizx_format_info", _I, _I, _P)
     sig("lizx_finish_llr", _I, _P, _I, _P, _P, _P, _P, _P, _P)
     sig("lizx_finish_grid", _I, _P, _I, _I, _P, _P, _P, _P, _P, C.POINTER(Soft))
+    sig("lizx_node_lattice", _I, _I, _P)
+    sig("lizx_warp", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P)
+    sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
+    sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
     sig("lizx_ldpc_dims", _I, _I, _P)
     sig("lizx_ldpc_new", _P, _I)
     sig("lizx_ldpc_free", None, _P)
@@ -113,6 +121,20 @@ def ldpc_dims(rate):
     check(lib().lizx_ldpc_dims(rate, _p(d)))
     return dict(zip(("n", "k", "m", "z", "mb", "kb", "np", "nt", "edges", "slots_total", "msg_bytes", "post_bytes"),
                     map(int, d)))
+
+
+def node_lattice(span):
+    pos = np.zeros(40, np.float32)
+    n = check(lib().lizx_node_lattice(span, _p(pos)))
+    return pos[:n].copy()
+
+
+def registration(H, nodes):
+    """A lizx_registration (keeps the nodes array alive on the returned object)."""
+    nodes = np.ascontiguousarray(nodes, np.float32)
+    r = Registration((C.c_float * 9)(*[float(x) for x in np.asarray(H, np.float64).ravel()]), _p(nodes))
+    r._keep = nodes
+    return r
 
 
 def ldpc_encode(rate, data):
@@ -258,6 +280,52 @@ class Decoder:
         be = np.zeros(B, np.float32)
         good = check(lib().lizx_finish_llr(self.h, version, _p(llr), _p(est), _p(blocks), _p(ok), _p(its), _p(be)))
         return {"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be}
+
+    def warp(self, version, span, img, H, nodes):
+        img = np.ascontiguousarray(img, np.uint8)
+        n = format_info(version)["n"]
+        grid = np.zeros((n, n), np.float32)
+        reg = registration(H, nodes)
+        check(lib().lizx_warp(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
+                              C.byref(reg), _p(grid)))
+        return grid
+
+    def hold(self, version, span, H, nodes):
+        self._held = registration(H, nodes)
+        check(lib().lizx_decoder_hold(self.h, version, span, C.byref(self._held)))
+
+    def decode(self, img, fmt=1, held=0):
+        """liz_decode on a capture (through the held registration): verified blocks, count, liz_decoded."""
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape[:2]
+        stride = img.strides[0]
+        ver = np.zeros((self.max_blocks() * LIZ_BLOCK,), np.uint8)
+        held_c = C.c_int(held)
+        out = _Decoded()
+        k = check(lib().liz_decode(self.h, _p(img), w, h, stride, fmt, C.byref(held_c), _p(ver), C.byref(out)))
+        return k, ver[:k * LIZ_BLOCK].reshape(k, LIZ_BLOCK), out, held_c.value
+
+    def decode_reg(self, version, span, img, H, nodes, soft=True):
+        info = format_info(version)
+        B, S = info["blocks"], info["subch"] * 320
+        img = np.ascontiguousarray(img, np.uint8)
+        blocks, ok = np.zeros(B * LIZ_BLOCK, np.uint8), np.zeros(B, np.uint8)
+        its, be = np.zeros(B, np.int8), np.zeros(B, np.float32)
+        out, sp = {}, None
+        if soft:
+            out["llr"] = np.zeros(2 * S, np.int8)
+            out["sub_est"] = np.zeros(info["subch"], np.float64)
+            out["sub_rms"] = np.zeros(info["subch"], np.float32)
+            out["coef"] = np.zeros(2 * S, np.float32)
+            sp = Soft(_p(out["llr"]), _p(out["sub_est"]), _p(out["sub_rms"]), _p(out["coef"]), (C.c_float * 2)(0, 0))
+        reg = registration(H, nodes)
+        good = check(lib().lizx_decode_reg(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
+                                           C.byref(reg), _p(blocks), _p(ok), _p(its), _p(be),
+                                           C.byref(sp) if sp is not None else None))
+        out.update({"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be})
+        if soft:
+            out["align"] = (sp.align[0], sp.align[1])
+        return out
 
     def finish_grid(self, version, span, grid, soft=True):
         """lizard_finish with a supplied n x n grid (row y, column x) of LIZARD-16V in the ring of this span."""

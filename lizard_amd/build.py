@@ -12,9 +12,11 @@ OUT = os.path.join(HERE, "liblizard_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 SOURCES = ["format.cpp", "api.cpp", "ldpc.hip", "spectrum.hip"]
-# -packed-fp32-ops: no packed FP32 instructions are generated (v_pk_*_f32)
+# -packed-fp32-ops: no packed FP32 instructions are generated (v_pk_*_f32).
+# -ffp-contract=off: every float multiply and add rounds on its own, as in every reference build
+# (liblizard/build.sh:15-17); the warp, detrend, pilot turn and soft scale reproduce the reference's float order.
 FLAGS = ["-O3", "-fPIC", "-std=c++17", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang",
-         "-packed-fp32-ops", "-Wall", "-Wno-unused-result", "-fvisibility=hidden"]
+         "-packed-fp32-ops", "-ffp-contract=off", "-Wall", "-Wno-unused-result", "-fvisibility=hidden"]
 
 
 def _compile(src):

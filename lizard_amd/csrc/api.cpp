@@ -69,6 +69,13 @@ struct liz_decoder {
   std::vector<uint8_t> blocks, ok;
   std::vector<std::unique_ptr<FramePlan>> plans;  // one per (version, span) met
   FrameBufs fb;
+  DevBuf img, nodes, wframe;
+  WarpFrame wframe_host{};
+  std::vector<uint8_t> luma;
+  // the held registration for liz_decode
+  int held_version = 0, held_span = 0;
+  lizx_registration held_reg{};
+  std::vector<float> held_nodes;
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
       if (p->version == version && p->span == span) return *p;
@@ -155,6 +162,11 @@ static void check_params(int max_iter, int stall_it, float ratio) {
 }
 
 extern "C" {
+
+static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                          const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
+                          lizx_soft *soft);
+
 
 LIZ_API const char *liz_version(void) { return LIZARD_AMD_VERSION; }
 LIZ_API const char *liz_last_error(void) { return last_error; }
@@ -412,9 +424,167 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     need(d && px && held && verified, "a decoder, an image, the held word and a place for the blocks");
     need(w > 0 && h > 0, "an image of at least a pixel");
     need(fmt == LIZ_GREY8 || fmt == LIZ_RGBX8 || fmt == LIZ_RGBA8 || fmt == LIZ_BGRA8, "a pixel format of lizard.h's");
-    need(stride >= w * (fmt == LIZ_GREY8 ? 1 : 4), "a stride of at least the image's width in bytes");
+    const int bpp = fmt == LIZ_GREY8 ? 1 : 4;
+    need(stride >= w * bpp, "a stride of at least the image's width in bytes");
     if (out) *out = liz_decoded{};
-    throw Error(LIZ_E_STATE, "no registration held: this library has no finder; hold one first");
+    if (!d->held_version) throw Error(LIZ_E_STATE, "no registration held: this library has no finder; hold one first");
+    // tight luma as the reference's codec reads it (api/codec.cpp:128-140): BGRA/RGBA/RGBX weights 77/150/29
+    const uint8_t *img = px;
+    int istride = stride;
+    if (fmt != LIZ_GREY8) {
+      d->luma.resize(static_cast<size_t>(w) * h);
+      for (int y = 0; y < h; y++) {
+        const uint8_t *row = px + static_cast<size_t>(y) * stride;
+        uint8_t *o = d->luma.data() + static_cast<size_t>(y) * w;
+        for (int x = 0; x < w; x++) {
+          const int r = fmt == LIZ_BGRA8 ? row[4 * x + 2] : row[4 * x], g = row[4 * x + 1],
+                    b = fmt == LIZ_BGRA8 ? row[4 * x] : row[4 * x + 2];
+          o[x] = static_cast<uint8_t>((77 * r + 150 * g + 29 * b + 128) >> 8);
+        }
+      }
+      img = d->luma.data();
+      istride = w;
+    }
+    const Format *f = format_for(d->held_version);
+    const int B = f->blocks;
+    d->ok.resize(B);
+    std::vector<int8_t> its(B);
+    std::vector<float> est(B);
+    std::vector<uint8_t> all(static_cast<size_t>(B) * BLOCK_BYTES);
+    decode_capture(d, d->held_version, d->held_span, img, w, h, istride, &d->held_reg, all.data(), d->ok.data(),
+                   its.data(), est.data(), nullptr);
+    int k = 0;
+    const int total = std::min(B, d->top);
+    for (int b = 0; b < total; b++)
+      if (d->ok[b]) std::memcpy(verified + static_cast<size_t>(k++) * BLOCK_BYTES, all.data() + static_cast<size_t>(b) * BLOCK_BYTES, BLOCK_BYTES);
+    if (out) {
+      liz_decoded o{};
+      o.found = 1;
+      o.ring = d->held_span == 128 ? 0 : d->held_span == 256 ? 1 : 2;
+      o.n = f->n;
+      o.word = 0;
+      o.blocks = d->held_version;
+      o.held_used = 1;
+      o.total = B;
+      o.verified = k;
+      o.pilot_blocks = 0;
+      o.pilot_r[0] = o.pilot_r[1] = o.pilot_sd[0] = o.pilot_sd[1] = __builtin_nanf("");
+      *out = o;
+    }
+    if (!*held) *held = d->held_version;
+    return k;
+  });
+}
+
+// A capture and its registration onto the device as a WarpFrame (one frame), on the decoder's stream. Every entry
+// that stages a capture synchronises the stream before it returns, so the caller's buffers are read in time.
+static const WarpFrame *stage_capture(liz_decoder *d, const FramePlan &p, const uint8_t *img, int w, int h, int stride,
+                                      const lizx_registration *reg) {
+  need(img && reg && reg->nodes, "a capture and a registration with nodes");
+  need(w >= 2 && h >= 2 && stride >= w, "a capture of at least 2 x 2 pixels, rows stride >= w bytes apart");
+  LdpcEngine &e = *d->eng;
+  const size_t npix = static_cast<size_t>(w) * h, nn = static_cast<size_t>(p.dev.nx) * p.dev.ny;
+  d->img.reserve(npix);
+  if (stride == w) {
+    hip_check(hipMemcpyAsync(d->img.p, img, npix, hipMemcpyHostToDevice, e.stream), "capture");
+  } else {
+    hip_check(hipMemcpy2DAsync(d->img.p, w, img, stride, w, h, hipMemcpyHostToDevice, e.stream), "capture");
+  }
+  d->nodes.reserve(nn * sizeof(float2));
+  hip_check(hipMemcpyAsync(d->nodes.p, reg->nodes, nn * sizeof(float2), hipMemcpyHostToDevice, e.stream), "nodes");
+  WarpFrame wf{};
+  for (int k = 0; k < 9; k++) wf.H[k] = reg->H[k];
+  wf.nodes = d->nodes.as<float2>();
+  wf.img = d->img.as<uint8_t>();
+  wf.w = w;
+  wf.h = h;
+  d->wframe.reserve(sizeof(WarpFrame));
+  d->wframe_host = wf;
+  hip_check(hipMemcpyAsync(d->wframe.p, &d->wframe_host, sizeof wf, hipMemcpyHostToDevice, e.stream), "registration");
+  return d->wframe.as<WarpFrame>();
+}
+
+static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                          const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
+                          lizx_soft *soft) {
+  need(blocks && ok && its && blk_est, "outputs");
+  const Format *f = format_for(version);
+  if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+  if (f->n > d->nmax) throw Error(LIZ_E_FORMAT, "the format's picture is larger than the decoder's nmax");
+  LdpcEngine &e = *d->eng;
+  hip_check(hipSetDevice(e.device), "hipSetDevice");
+  const FramePlan &p = d->plan(version, span);
+  const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg);
+  const bool dbg = soft && soft->coef;
+  const size_t S = static_cast<size_t>(p.subch) * SUB;
+  d->fb.reserve(p, 1, dbg);
+  launch_soft_warp(p, d->fb, wf, 1, dbg, e.stream);
+  finish_blocks(d, f, d->fb.llr.as<int8_t>(), d->fb.est.as<double>());
+  e.sync();
+  if (soft) {
+    if (soft->llr) hip_check(hipMemcpy(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost), "soft values");
+    if (soft->sub_est)
+      hip_check(hipMemcpy(soft->sub_est, d->fb.est.p, p.subch * sizeof(double), hipMemcpyDeviceToHost), "estimates");
+    if (soft->sub_rms)
+      hip_check(hipMemcpy(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
+    if (soft->coef)
+      hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
+    hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
+  }
+  return collect_blocks(d, f, blocks, ok, its, blk_est);
+}
+
+LIZ_API int lizx_node_lattice(int span, float *pos) {
+  return guard([&] {
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    float tmp[MAX_NODES];
+    const int n = node_lattice(span, tmp);
+    if (pos) std::memcpy(pos, tmp, n * sizeof(float));
+    return n;
+  });
+}
+
+LIZ_API int lizx_warp(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                      const lizx_registration *reg, float *grid) {
+  return guard([&] {
+    need(d && grid, "a decoder and a place for the grid");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg);
+    const size_t n = p.n;
+    d->grid.reserve(n * n * sizeof(float));
+    launch_warp(p, wf, 1, d->grid.as<float>(), e.stream);
+    e.sync();
+    hip_check(hipMemcpy(grid, d->grid.p, n * n * sizeof(float), hipMemcpyDeviceToHost), "grid");
+    return 0;
+  });
+}
+
+LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                            const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
+                            lizx_soft *soft) {
+  return guard([&] {
+    need(d, "a decoder");
+    return decode_capture(d, version, span, img, w, h, stride, reg, blocks, ok, its, blk_est, soft);
+  });
+}
+
+LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_registration *reg) {
+  return guard([&] {
+    need(d && reg && reg->nodes, "a decoder and a registration with nodes");
+    if (!format_for(version)) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    float pos[MAX_NODES];
+    const int nx = node_lattice(span, pos);
+    d->held_nodes.assign(reg->nodes, reg->nodes + 2 * nx * nx);
+    d->held_reg = *reg;
+    d->held_reg.nodes = d->held_nodes.data();
+    d->held_version = version;
+    d->held_span = span;
+    return 0;
   });
 }
 

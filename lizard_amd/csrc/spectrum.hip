@@ -155,17 +155,80 @@ __device__ double block_sum(double v, double *red, int tid) {
   return r;
 }
 
+// ------------------------------------------------------------------------------------------------ the warp
+// A bilinear read of the uint8 capture at pixel coordinates (x, y) (pixel centres at half integers), clamped to the
+// frame, grey level i / 255 (liblizard/src/internal.h:11-23 at gamma 1). In bounds for any input (NaN reads pixel 0).
+__device__ __forceinline__ float sample(const WarpFrame &F, float x, float y) {
+  x = __fsub_rn(x, 0.5f);
+  y = __fsub_rn(y, 0.5f);
+  const float xm = __fsub_rn(static_cast<float>(F.w), 1.001f), ym = __fsub_rn(static_cast<float>(F.h), 1.001f);
+  if (x < 0) x = 0; else if (x > xm) x = xm;
+  if (y < 0) y = 0; else if (y > ym) y = ym;
+  const int xi = x >= 0 ? static_cast<int>(x) : 0, yi = y >= 0 ? static_cast<int>(y) : 0;
+  const float fx = __fsub_rn(x, static_cast<float>(xi)), fy = __fsub_rn(y, static_cast<float>(yi));
+  const uint8_t *p = F.img + static_cast<size_t>(yi) * F.w + xi;
+  const float a = __fdiv_rn(static_cast<float>(p[0]), 255.0f), b = __fdiv_rn(static_cast<float>(p[1]), 255.0f);
+  const float c = __fdiv_rn(static_cast<float>(p[F.w]), 255.0f), d = __fdiv_rn(static_cast<float>(p[F.w + 1]), 255.0f);
+  const float top = __fadd_rn(a, __fmul_rn(__fsub_rn(b, a), fx)), bot = __fadd_rn(c, __fmul_rn(__fsub_rn(d, c), fx));
+  return __fadd_rn(__fmul_rn(top, __fsub_rn(1.0f, fy)), __fmul_rn(bot, fy));
+}
+
+// Picture sample (x, y) of frame F: module coordinates (15 + (t + 1/2) span / n on either axis, SPEC 6.9), the
+// homography H plus the node lattice's bilinear displacement (acquire.c:5095-5103), read from the capture. The float
+// operations are the reference's scalar grid walk's, in its order (acquire.c:5121-5196 without OB_SIMD), so the
+// samples are that walk's to the bit.
+__device__ __forceinline__ float warp_sample(const PlanDev &P, const WarpFrame &F, int x, int y) {
+  const float mx = P.mcoord[x], my = P.mcoord[y];
+  const int i = P.mcell[x], j = P.mcell[y], nx = P.nx;
+  const float v = __fdiv_rn(__fsub_rn(my, P.node_y[j]), __fsub_rn(P.node_y[j + 1], P.node_y[j]));
+  const float *H = F.H;
+  const float kw = __fadd_rn(__fmul_rn(H[7], my), 1.0f), ku = __fadd_rn(__fmul_rn(H[1], my), H[2]);
+  const float kv = __fadd_rn(__fmul_rn(H[4], my), H[5]);
+  const float2 n00 = F.nodes[j * nx + i], n10 = F.nodes[j * nx + i + 1];
+  const float2 n01 = F.nodes[(j + 1) * nx + i], n11 = F.nodes[(j + 1) * nx + i + 1];
+  const float inv = __fdiv_rn(1.0f, __fsub_rn(P.node_x[i + 1], P.node_x[i])), omv = __fsub_rn(1.0f, v);
+  const float ex = __fmul_rn(__fadd_rn(__fmul_rn(__fsub_rn(n10.x, n00.x), omv), __fmul_rn(__fsub_rn(n11.x, n01.x), v)), inv);
+  const float cx = __fsub_rn(__fadd_rn(__fmul_rn(n00.x, omv), __fmul_rn(n01.x, v)), __fmul_rn(ex, P.node_x[i]));
+  const float ey = __fmul_rn(__fadd_rn(__fmul_rn(__fsub_rn(n10.y, n00.y), omv), __fmul_rn(__fsub_rn(n11.y, n01.y), v)), inv);
+  const float cy = __fsub_rn(__fadd_rn(__fmul_rn(n00.y, omv), __fmul_rn(n01.y, v)), __fmul_rn(ey, P.node_x[i]));
+  const float rw = __fdiv_rn(1.0f, __fadd_rn(__fmul_rn(H[6], mx), kw));
+  const float pu = __fmul_rn(__fadd_rn(__fmul_rn(H[0], mx), ku), rw), pv = __fmul_rn(__fadd_rn(__fmul_rn(H[3], mx), kv), rw);
+  const float dx = __fadd_rn(cx, __fmul_rn(ex, mx)), dy = __fadd_rn(cy, __fmul_rn(ey, mx));
+  return sample(F, __fadd_rn(pu, dx), __fadd_rn(pv, dy));
+}
+
+// Where the path's picture samples come from: a grid the caller supplied, or captures through their registrations.
+struct GridSource {
+  const float *grid;
+  __device__ __forceinline__ float operator()(const PlanDev &P, int f, int x, int y) const {
+    return grid[(static_cast<size_t>(f) * P.n + y) * P.n + x];
+  }
+};
+struct WarpSource {
+  const WarpFrame *frames;
+  __device__ __forceinline__ float operator()(const PlanDev &P, int f, int x, int y) const {
+    return warp_sample(P, frames[f], x, y);
+  }
+};
+
+__global__ __launch_bounds__(256) void k_warp(PlanDev P, const WarpFrame *frames, float *grid) {
+  const int f = blockIdx.y, n = P.n;
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n * n) return;
+  grid[static_cast<size_t>(f) * n * n + t] = warp_sample(P, frames[f], t % n, t / n);
+}
+
 // The shading fit (lizard_phy.c:1605-1624): rhs over every eighth row and column outside the squares, beta =
 // hole_inv rhs.
-__global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, const float *grid, double *beta_out) {
+template <class Src>
+__global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *beta_out) {
   __shared__ double red[256];
   const int tid = threadIdx.x, f = blockIdx.x, n = P.n, c = n / 8;
-  const float *g = grid + static_cast<size_t>(f) * n * n;
   double rhs[6] = {0, 0, 0, 0, 0, 0};
   for (int k = tid; k < c * c; k += 256) {
     const int y = 8 * (k / c), x = 8 * (k % c);
     if (in_square(P, y) && in_square(P, x)) continue;
-    const double v = g[static_cast<size_t>(y) * n + x];
+    const double v = src(P, f, x, y);
     double b[6];
     hole_basis(P, x, y, b);
 #pragma unroll
@@ -184,8 +247,8 @@ __global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, const float *grid,
 // ------------------------------------------------------------------------------------------------ the two passes
 constexpr int ROWS = 8;  // picture rows a k_rows workgroup transforms (four packed complex sequences)
 
-__global__ __launch_bounds__(256) void k_rows(PlanDev P, const float *grid, const double *beta, float2 *X,
-                                              double *rowsum) {
+template <class Src>
+__global__ __launch_bounds__(256) void k_rows(PlanDev P, Src src, const double *beta, float2 *X, double *rowsum) {
   extern __shared__ float2 zs[];  // ROWS / 2 sequences of n
   __shared__ double bsh[6];
   const int tid = threadIdx.x, f = blockIdx.y, y0 = blockIdx.x * ROWS, n = P.n;
@@ -194,12 +257,11 @@ __global__ __launch_bounds__(256) void k_rows(PlanDev P, const float *grid, cons
   // load: 32 lanes a row, erase the anchors' samples, row sums for the detrend (s, s c(x), s q(x))
   {
     const int row = tid >> 5, l = tid & 31, y = y0 + row;
-    const float *g = grid + (static_cast<size_t>(f) * n + y) * n;
     const bool yin = in_square(P, y);
     float *zf = reinterpret_cast<float *>(zs + (row >> 1) * n) + (row & 1);
     double rs = 0, rx = 0, rq = 0;
     for (int x = l; x < n; x += 32) {
-      float s = g[x];
+      float s = src(P, f, x, y);
       if (yin && in_square(P, x)) s = hole_value(P, bsh, x, y);
       zf[2 * x] = s;
       rs += s;
@@ -579,6 +641,23 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
     }
   }
   d.npilots = static_cast<int>(p->pilot_host.size());
+  // the warp's sample grid: sample t at module g0 + t step (lizard_phy.c:1592; SPEC 6.9), its node cell found as the
+  // reference's walk finds it (acquire.c:5111-5119)
+  d.nx = d.ny = node_lattice(span, d.node_x);
+  for (int k = 0; k < d.ny; k++) d.node_y[k] = d.node_x[k];
+  std::vector<float> mc(n);
+  std::vector<int32_t> cell(n);
+  {
+    const volatile float g0 = static_cast<float>(BORDER) + 0.5f / scale, step = 1.0f / scale;
+    for (int t = 0; t < n; t++) {
+      const volatile float prod = static_cast<float>(t) * step;
+      const float m = g0 + prod;
+      int i = 0;
+      while (i < d.nx - 2 && m >= d.node_x[i + 1]) i++;
+      mc[t] = m;
+      cell[t] = i;
+    }
+  }
   p->radix = radices(n);
   d.nstages = static_cast<int>(p->radix.size());
   for (int s = 0; s < d.nstages; s++) d.radix[s] = p->radix[s];
@@ -604,6 +683,13 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
   hip_check(hipMemcpy(p->tw.p, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice), "twiddles");
   p->rot.reserve(ALIGN_TURNS * sizeof(float2));
   hip_check(hipMemcpy(p->rot.p, rot.data(), ALIGN_TURNS * sizeof(float2), hipMemcpyHostToDevice), "turn table");
+  p->mtab.reserve(n * (sizeof(float) + sizeof(int32_t)));
+  hip_check(hipMemcpy(p->mtab.p, mc.data(), n * sizeof(float), hipMemcpyHostToDevice), "module coordinates");
+  hip_check(hipMemcpy(static_cast<char *>(p->mtab.p) + n * sizeof(float), cell.data(), n * sizeof(int32_t),
+                      hipMemcpyHostToDevice),
+            "node cells");
+  d.mcoord = p->mtab.as<float>();
+  d.mcell = reinterpret_cast<const int32_t *>(static_cast<char *>(p->mtab.p) + n * sizeof(float));
   p->white.reserve(f->white.size());
   hip_check(hipMemcpy(p->white.p, f->white.data(), f->white.size(), hipMemcpyHostToDevice), "whitening");
   const float *tb = p->tables.as<float>();
@@ -635,10 +721,11 @@ void FrameBufs::reserve(const FramePlan &p, int frames, bool debug) {
   if (debug) coef_dbg.reserve(frames * S * sizeof(float2));
 }
 
-void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s) {
+template <class Src>
+static void launch_soft_src(const FramePlan &p, FrameBufs &b, Src src, int frames, bool debug, hipStream_t s) {
   const int n = p.n, E = p.E;
-  hipLaunchKernelGGL(k_erase_fit, dim3(frames), dim3(256), 0, s, p.dev, d_grid, b.beta.as<double>());
-  hipLaunchKernelGGL(k_rows, dim3(n / ROWS, frames), dim3(256), (ROWS / 2) * n * sizeof(float2), s, p.dev, d_grid,
+  hipLaunchKernelGGL(k_erase_fit<Src>, dim3(frames), dim3(256), 0, s, p.dev, src, b.beta.as<double>());
+  hipLaunchKernelGGL(k_rows<Src>, dim3(n / ROWS, frames), dim3(256), (ROWS / 2) * n * sizeof(float2), s, p.dev, src,
                      b.beta.as<double>(), b.X.as<float2>(), b.rowsum.as<double>());
   hipLaunchKernelGGL(k_cols, dim3((E + 1) / 2, frames), dim3(256), 2 * n * sizeof(float2), s, p.dev, b.X.as<float2>(),
                      b.HP.as<float2>());
@@ -648,6 +735,31 @@ void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int fram
                      b.align.as<float>(), b.llr.as<int8_t>(), b.est.as<double>(), b.rms.as<float>(),
                      debug ? b.coef_dbg.as<float>() : nullptr);
   hip_check(hipGetLastError(), "soft-value launches");
+}
+
+void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s) {
+  launch_soft_src(p, b, GridSource{d_grid}, frames, debug, s);
+}
+
+void launch_soft_warp(const FramePlan &p, FrameBufs &b, const WarpFrame *d_frames, int frames, bool debug,
+                      hipStream_t s) {
+  launch_soft_src(p, b, WarpSource{d_frames}, frames, debug, s);
+}
+
+void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, float *d_grid, hipStream_t s) {
+  const int n = p.n;
+  hipLaunchKernelGGL(k_warp, dim3((n * n + 255) / 256, frames), dim3(256), 0, s, p.dev, d_frames, d_grid);
+  hip_check(hipGetLastError(), "warp launch");
+}
+
+// Node centres every 16 modules from 3.5 (the middle of the line-and-track band) to side - 3.5 (layout.c:110-118,
+// layout.h OB_NODE_LINE, OB_THIN_PITCH, OB_MAX_NODES).
+int node_lattice(int span, float *pos) {
+  const int len = span + 2 * BORDER;
+  int n = 0;
+  for (float c = 3.5f; c < static_cast<float>(len) - 3.5f - 8.0f && n < MAX_NODES - 1; c += 16.0f) pos[n++] = c;
+  pos[n++] = static_cast<float>(len) - 3.5f;
+  return n;
 }
 
 }  // namespace lz

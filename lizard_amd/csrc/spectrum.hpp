@@ -18,6 +18,16 @@ constexpr int ANCH_MARGIN = 2;      // samples erased round a square
 constexpr int BORDER = 15;          // modules between the symbol's edge and the picture (mark + guard)
 constexpr int ALIGN_TURNS = 4096;   // steps of the pilots' turn table
 constexpr int FFT_MAX_STAGES = 12;
+constexpr int MAX_NODES = 40;       // node lattice points a side (layout.h OB_MAX_NODES)
+
+// One frame's registration and capture for the warp: H maps module coordinates to capture pixels (h8 = 1), nodes
+// the node lattice's displacements in pixels (ny rows of nx), img a tight uint8 luma capture of w x h.
+struct WarpFrame {
+  float H[9];
+  const float2 *nodes;
+  const uint8_t *img;
+  int w, h;
+};
 
 // Device-side view of a plan (by value into kernels).
 struct PlanDev {
@@ -33,13 +43,18 @@ struct PlanDev {
   const int32_t *coef;                     // subch * 320: u (low 16 bits, signed) and v (high 16)
   const int32_t *pilot;                    // npilots: coefficient index | frame slot parity << 31 ... (see spectrum.hip)
   const uint8_t *white;                    // frame slot whitening
+  // the warp (picture sample x or y -> module coordinate and node cell; the same table on both axes)
+  int nx, ny;
+  float node_x[MAX_NODES], node_y[MAX_NODES];
+  const float *mcoord;                     // n: module coordinate of sample index t
+  const int32_t *mcell;                    // n: node cell of sample index t (the same on both axes)
 };
 
 struct FramePlan {
   int version = 0, span = 0, n = 0, subch = 0, E = 0;
   const Format *fmt = nullptr;
   PlanDev dev{};
-  DevBuf tables, coef, pilot, tw, rot, white;
+  DevBuf tables, coef, pilot, tw, rot, white, mtab;
   std::vector<int> radix;
   std::vector<int> pilot_host;  // coefficient indices of the pilots, in block order
   std::vector<int> pilot_block;
@@ -56,5 +71,12 @@ struct FrameBufs {
 
 // Enqueues grid (row-major n x n float per frame, frames back to back) -> soft values, estimates, pilot shift.
 void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s);
+// The same from captures through their registrations (warp fused into the erase fit and the row transform).
+void launch_soft_warp(const FramePlan &p, FrameBufs &b, const WarpFrame *d_frames, int frames, bool debug,
+                      hipStream_t s);
+// The warp alone: n x n picture samples a frame, row-major.
+void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, float *d_grid, hipStream_t s);
+// The node lattice of a ring's layout (module coordinates; layout.c:110-118).
+int node_lattice(int span, float *pos);
 
 }  // namespace lz

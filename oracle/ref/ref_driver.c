@@ -151,3 +151,23 @@ int ref_node_lattice(const void *p, int *dims, float *nx_out, float *ny_out) {
   if (ny_out) for (int i = 0; i < f->frame.ny; i++) ny_out[i] = f->frame.node_y[i];
   return 0;
 }
+
+/* lizard_finish on a capture through a given registration (what lizard_acquire would have held): the reference's
+   own sampler, erase, transform, soft values, LDPC and CRC. */
+int ref_decode_reg(const void *p, const uint8_t *img, int iw, int ih, const float *H, const float *nodes, uint8_t *blocks,
+                   uint8_t *ok, int8_t *its, float *blk_est, int8_t *llr_out, float *align) {
+  const lizard_t *f = p;
+  struct lizard_ws *w = f->ws;
+  ob_image_init(&w->hold_im, img, iw, ih, 1.0f);
+  memcpy(w->hold_reg.H.h, H, sizeof w->hold_reg.H.h);
+  const int nn = f->frame.nx * f->frame.ny;
+  w->hold_reg.nodes = calloc((size_t)nn, sizeof(node_t));
+  for (int i = 0; i < nn; i++) { w->hold_reg.nodes[i].dx = nodes[2 * i]; w->hold_reg.nodes[i].dy = nodes[2 * i + 1]; w->hold_reg.nodes[i].done = 1; }
+  w->hold_valid = 1;
+  ob_result_t res;
+  const int good = lizard_finish_ext(f, 0, 0, 0, blocks, ok, &res);
+  block_out(f, its, blk_est, 0);
+  if (llr_out) memcpy(llr_out, w->llrf, (size_t)f->subch * LIZARD_SUB * 2);
+  if (align) { align[0] = w->align[0]; align[1] = w->align[1]; }
+  return good;
+}

@@ -56,3 +56,81 @@ class Frame:
 
     def fp64(self):
         return picture.finish64(self.grid, self.n, self.span, self.u, self.v, self.white, self.pilots)
+
+
+def lattice(span):
+    side = span + 30
+    pos = []
+    c = np.float32(3.5)
+    while c < np.float32(side) - np.float32(3.5) - np.float32(8) and len(pos) < 39:
+        pos.append(float(c))
+        c = np.float32(c + 16)
+    pos.append(float(np.float32(side) - np.float32(3.5)))
+    return np.array(pos, np.float32)
+
+
+def homography(src, dst):
+    """The 3x3 projective map taking four src points to four dst points (h8 = 1), in FP64."""
+    A, b = [], []
+    for (x, y), (u, v) in zip(src, dst):
+        A.append([x, y, 1, 0, 0, 0, -u * x, -u * y]); b.append(u)
+        A.append([0, 0, 0, x, y, 1, -v * x, -v * y]); b.append(v)
+    h = np.linalg.solve(np.array(A, float), np.array(b, float))
+    return np.append(h, 1.0).reshape(3, 3)
+
+
+class Capture:
+    """A symbol painted by the reference encoder (oracle/_ref: lizard_encode + lizard_paint_grey), projected into a
+    camera frame by a homography G (grey-paint pixel edge coordinates -> capture pixel edge coordinates), blurred,
+    with AWGN, as uint8; with the registration the decoder needs (H = G composed with the module -> paint pixel map,
+    nodes zero, or a given node displacement field). Needs the compiled reference."""
+
+    def __init__(self, version, span, W=1280, Hh=720, rot_deg=5.0, persp=0.02, scale=0.8, blur=0.8, noise=4.0,
+                 seed=0, cnt=0, node_fn=None):
+        import scipy.ndimage as ndi
+        from _libs import ref
+        r = ref("scalar")
+        o = oracle()
+        self.version, self.span, self.subch = version, span, 16 * version
+        self.n = n = o.or_n_for(self.subch)
+        f = r.ref_format_new(n, self.subch, span)
+        info = np.zeros(10, np.int32)
+        r.ref_format_info(f, ptr(info))
+        self.blocks_n, pxm, px = int(info[2]), int(info[7]), int(info[8])
+        rng = np.random.default_rng(seed)
+        self.blocks = rng.integers(0, 256, self.blocks_n * 476).astype(np.uint8)
+        drive = np.zeros(px * px, np.float32)
+        r.ref_encode(f, ptr(self.blocks), cnt, ptr(drive), None)
+        Wg = px + 4 * pxm
+        grey = np.zeros(Wg * Wg, np.uint8)
+        r.ref_paint_grey(f, ptr(drive), ptr(grey))
+        r.ref_format_free(f)
+        grey = grey.reshape(Wg, Wg).astype(np.float64)
+        # G: the paint's square onto the frame, rotated, scaled to `scale` of the frame height, a little perspective
+        s = scale * Hh / Wg
+        th = np.deg2rad(rot_deg)
+        cxy = np.array([W / 2, Hh / 2])
+        corners = np.array([[0, 0], [Wg, 0], [Wg, Wg], [0, Wg]], float) - Wg / 2
+        R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+        dst = (corners @ R.T) * s + cxy
+        dst[1] += np.array([persp, -persp]) * Hh * 0.5
+        dst[2] -= np.array([persp, persp]) * Hh * 0.25
+        G = homography(corners + Wg / 2, dst)
+        Gi = np.linalg.inv(G)
+        Y, X = np.meshgrid(np.arange(Hh) + 0.5, np.arange(W) + 0.5, indexing="ij")
+        den = Gi[2, 0] * X + Gi[2, 1] * Y + Gi[2, 2]
+        qx = (Gi[0, 0] * X + Gi[0, 1] * Y + Gi[0, 2]) / den - 0.5
+        qy = (Gi[1, 0] * X + Gi[1, 1] * Y + Gi[1, 2]) / den - 0.5
+        img = ndi.map_coordinates(grey, [qy, qx], order=1, mode="constant", cval=255.0)
+        if blur > 0:
+            img = ndi.gaussian_filter(img, blur)
+        img = img + rng.standard_normal(img.shape) * noise
+        self.img = np.clip(np.rint(img), 0, 255).astype(np.uint8)
+        M = np.array([[pxm, 0, 2 * pxm], [0, pxm, 2 * pxm], [0, 0, 1]], float)
+        Hm = G @ M
+        self.H = (Hm / Hm[2, 2]).astype(np.float32).ravel()
+        self.node_x = lattice(span)
+        nx = len(self.node_x)
+        self.nodes = np.zeros((nx, nx, 2), np.float32)
+        if node_fn is not None:
+            self.nodes = node_fn(self.node_x).astype(np.float32)

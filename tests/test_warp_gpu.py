@@ -1,0 +1,106 @@
+"""The warp on the GPU (lizx_warp) against the reference's picture sampler compiled from its sources: the scalar
+grid walk to the bit, the x86-64 integer read within 1/200 of the grey range (liblizard/test/sample_grid.c:137); and
+the whole decode of a capture through its registration (lizx_decode_reg, liz_decode with a held registration)
+against the reference's lizard_finish on the same capture and registration."""
+import numpy as np
+import pytest
+
+import lizard_amd
+from _libs import ptr, ref
+from pictures import Capture
+
+pytestmark = pytest.mark.gpu
+
+
+def wavy(scale):
+    def f(node_x):
+        X, Y = np.meshgrid(node_x, node_x, indexing="xy")
+        return np.stack([scale * np.sin(X / 23.0) * np.cos(Y / 31.0), scale * np.cos(X / 17.0 + Y / 29.0)], -1)
+    return f
+
+
+WARPS = [(5, 128, 0.0), (5, 128, 1.5), (20, 256, 0.7), (64, 512, 1.0), (45, 512, 0.0), (8, 256, 2.0)]
+
+
+@pytest.mark.parametrize("version,span,wave", WARPS)
+def test_warp_matches_reference_sampler(decoder_gpu, version, span, wave):
+    C = Capture(version, span, seed=version, W=1920 if span == 512 else 1280, Hh=1080 if span == 512 else 720,
+                node_fn=wavy(wave) if wave else None)
+    grid = decoder_gpu.warp(version, span, C.img, C.H, C.nodes)
+    r = ref("scalar")
+    if r is None:
+        pytest.skip("oracle/_ref not built")
+    f = r.ref_format_new(C.n, C.subch, span)
+    want = np.zeros((C.n, C.n), np.float32)
+    r.ref_sample_grid(f, ptr(C.img), C.img.shape[1], C.img.shape[0], ptr(C.H), ptr(C.nodes), 0, ptr(want))
+    r.ref_format_free(f)
+    assert np.array_equal(grid, want), np.abs(grid - want).max()
+    rx = ref("x86")
+    if rx is not None:
+        f = rx.ref_format_new(C.n, C.subch, span)
+        lin = np.zeros((C.n, C.n), np.float32)
+        rx.ref_sample_grid(f, ptr(C.img), C.img.shape[1], C.img.shape[0], ptr(C.H), ptr(C.nodes), 1, ptr(lin))
+        rx.ref_format_free(f)
+        assert np.abs(grid - lin).max() <= 1.0 / 200
+
+
+def test_warp_clamps_outside_the_frame(decoder_gpu):
+    """Points mapped far outside the capture read its edge pixels (the clamp), never out of bounds."""
+    C = Capture(5, 128, seed=2)
+    H = C.H.copy()
+    H[2] += 5000.0  # the whole picture lands right of the frame
+    grid = decoder_gpu.warp(5, 128, C.img, H, C.nodes)
+    assert np.isfinite(grid).all() and grid.min() >= 0 and grid.max() <= 1
+
+
+DECODES = [(5, 128, 1, 0.8, 4.0, 0.0), (4, 128, 2, 0.5, 2.0, 0.5), (20, 256, 3, 1.0, 6.0, 0.7), (45, 512, 4, 0.8, 3.0, 0.0)]
+
+
+@pytest.mark.parametrize("version,span,seed,blur,noise,wave", DECODES)
+def test_decode_reg_matches_reference(decoder_gpu, version, span, seed, blur, noise, wave):
+    big = span == 512
+    C = Capture(version, span, seed=seed, W=1920 if big else 1280, Hh=1080 if big else 720, blur=blur, noise=noise,
+                node_fn=wavy(wave) if wave else None)
+    out = decoder_gpu.decode_reg(version, span, C.img, C.H, C.nodes)
+    B = C.blocks_n
+    for b in range(B):
+        if out["ok"][b]:
+            assert np.array_equal(out["blocks"][b], C.blocks.reshape(B, 476)[b])
+    r = ref("scalar")
+    if r is None:
+        return
+    f = r.ref_format_new(C.n, C.subch, span)
+    bl, ok, its, be = np.zeros(B * 476, np.uint8), np.zeros(B, np.uint8), np.zeros(B, np.int8), np.zeros(B, np.float32)
+    llr, al = np.zeros(C.subch * 640, np.int8), np.zeros(2, np.float32)
+    good = r.ref_decode_reg(f, ptr(C.img), C.img.shape[1], C.img.shape[0], ptr(C.H), ptr(C.nodes), ptr(bl), ptr(ok),
+                            ptr(its), ptr(be), ptr(llr), ptr(al))
+    r.ref_format_free(f)
+    d = np.abs(out["llr"].astype(int) - llr.astype(int))
+    assert d.max() <= 1 and (d > 0).mean() < 1e-3
+    assert abs(out["align"][0] - al[0]) < 1e-3 and abs(out["align"][1] - al[1]) < 1e-3
+    if not d.any():
+        assert out["good"] == good and np.array_equal(out["its"], its) and np.array_equal(out["ok"], ok)
+    else:
+        assert abs(out["good"] - good) <= 1
+
+
+@pytest.mark.parametrize("fmt", [1, 4])
+def test_liz_decode_held(fmt):
+    """liblizard's liz_decode, through a held registration, on GREY8 and BGRA8 captures: every block verified, packed
+    in block order (api/codec.cpp:143-147)."""
+    C = Capture(5, 128, seed=9, blur=0.6, noise=2.0)
+    dec = lizard_amd.Decoder(384)
+    dec.hold(5, 128, C.H, C.nodes)
+    img = C.img if fmt == 1 else np.repeat(C.img[:, :, None], 4, axis=2).copy()
+    k, blocks, out, held = dec.decode(img, fmt=fmt)
+    dec.close()
+    assert k == C.blocks_n and out.verified == k and out.total == C.blocks_n and held == 5
+    assert np.array_equal(blocks.ravel(), C.blocks)
+
+
+def test_liz_decode_without_registration_is_state_error():
+    dec = lizard_amd.Decoder(384)
+    with pytest.raises(lizard_amd.LizError) as e:
+        dec.decode(np.zeros((10, 10), np.uint8))
+    assert e.value.code == lizard_amd.LIZ_E_STATE
+    dec.close()
