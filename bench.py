@@ -10,7 +10,16 @@ the GPU. Inputs are resident in HBM before the timed region.
 
 Prints one JSON line: LDPC Gbit/s (3840 x codewords / s) as the value, payload MB/s (472 x verified blocks / s),
 the plain per-codeword path's rate beside the batched one, the LDS roofline of the decode kernel and the reference
-CPU decoder timed on this machine's cores.
+CPU decoder timed on this machine's cores. Two more workloads ride along as sub-objects:
+  "pipeline"   the full receive path, batched: 32 captured 1920x1080 uint8 frames, each an n = 1536 picture
+               (LIZARD-1024, 124 blocks) in the 256 ring through a random homography (rotation +-10 deg, scale 0.6-0.9
+               of the frame height, perspective up to 0.05), radial k1 in [-0.05, 0.05], blur sigma 0.5-1.5 px, AWGN
+               sigma 2-8 grey levels (lizard_amd.sim, rendered on the host); warp -> FFT -> demap -> LDPC -> CRC as one
+               HIP graph, against the same launches without the graph, frame by frame, and frame by frame with the
+               LDPC launched once a codeword; frames/s and payload MB/s; the reference decoder on the host cores.
+  "fft_demap"  64 picture grids with n drawn from {384, 768, 1152, 1536} (each at its top format), the sender's
+               picture plus AWGN at 20 dB SNR: erase, detrend, 2D FFT, pilot shift, soft demap as one HIP graph;
+               algorithmic HBM GB/s against the 8 TB/s peak.
 """
 import argparse
 import ctypes as C
@@ -137,6 +146,175 @@ def cpu_baseline(batches, seconds):
                       % (words, wall, cores)}
 
 
+def pipeline_frames(count, seed):
+    """count captures of LIZARD-1024 in the 256 ring through random cameras (rendered on the host: the library's
+    device memory is its own, so inputs travel through its allocator)."""
+    from lizard_amd import sim
+    rng = np.random.default_rng(seed)
+    caps = []
+    for k in range(count):
+        cam = sim.Camera.random(rng)
+        blocks = sim.random_blocks(64, rng)
+        img, H, nodes = sim.capture(sim.picture(64, blocks, cnt=k % 4), 512, cam, seed=seed + k)
+        caps.append(dict(img=img.numpy(), H=H, nodes=np.ascontiguousarray(nodes), blocks=blocks, cam=cam))
+    return caps
+
+
+def soft_bytes(version):
+    """Algorithmic HBM bytes of one frame's soft-value kernels from a grid: row pass n^2 4 read + E n 8 write, column
+    pass E n 8 read + E 2E 8 write, demap 320 subch 8 read + 640 subch write (plus sub-channel estimates)."""
+    info = lizard_amd.format_info(version)
+    n, subch = info["n"], info["subch"]
+    u, v = lizard_amd.format_coefs(version)
+    E = int(max(np.abs(u).max(), v.max())) + 1
+    return {"rows": n * n * 4 + E * n * 8, "cols": E * n * 8 + E * 2 * E * 8,
+            "demap": 320 * subch * 8 + 640 * subch + subch * 12, "n": n, "E": E}
+
+
+def run_batch(b, frames, mode, runs):
+    b.run(frames, mode)
+    b.sync()
+    t0 = time.perf_counter()
+    for _ in range(runs):
+        b.run(frames, mode)
+    ms = b.sync()
+    return (time.perf_counter() - t0) / runs, ms / runs
+
+
+def bench_pipeline(args, caps):
+    D = lizard_amd.DeviceArray
+    F = len(caps)
+    keep = []
+    items = []
+    for c in caps:
+        d_img, d_nodes = D.of(c["img"]), D.of(c["nodes"])
+        keep += [d_img, d_nodes]
+        items.append(dict(img=d_img, w=c["img"].shape[1], h=c["img"].shape[0], H=c["H"], nodes=d_nodes))
+    frames = lizard_amd.make_frames(items)
+    b = lizard_amd.Batch(0).plan([64] * F, [512] * F)
+    res = {}
+    outs = {}
+    for name, mode, runs in (("graph", lizard_amd.RUN_GRAPH, args.pipe_runs),
+                             ("stream", lizard_amd.RUN_STREAM, args.pipe_runs),
+                             ("frames", lizard_amd.RUN_FRAMES, max(1, args.pipe_runs // 2)),
+                             ("plain", lizard_amd.RUN_PLAIN, 1)):
+        wall, ev = run_batch(b, frames, mode, runs)
+        out = b.blocks_out()
+        outs[name] = out
+        res[name] = {"frames_per_s": round(F / wall, 2), "ms_per_batch": round(wall * 1e3, 3),
+                     "gpu_ms_per_batch": round(ev, 3), "payload_MBps": round(472.0 * out["good"] / wall / 1e6, 3),
+                     "runs": runs}
+    same = all(np.array_equal(outs[k]["ok"], outs["graph"]["ok"]) and np.array_equal(outs[k]["its"], outs["graph"]["its"])
+               and np.array_equal(outs[k]["blocks"][outs[k]["ok"] > 0], outs["graph"]["blocks"][outs["graph"]["ok"] > 0])
+               for k in outs)
+    g = outs["graph"]
+    sent = np.concatenate([c["blocks"].reshape(-1, 476) for c in caps])
+    right = (g["ok"] > 0) & np.all(g["blocks"] == sent, axis=1)
+    out = {
+        "workload": "full receive pipeline, batched: %d captured 1920x1080 uint8 frames, n=1536 (LIZARD-1024, 124 "
+                    "blocks, 256 ring), random homography (rot +-10 deg, scale 0.6-0.9, persp <= 0.05), k1 in "
+                    "[-0.05, 0.05], blur 0.5-1.5 px, AWGN 2-8 grey; warp+erase+row FFT fused, column FFT, pilot "
+                    "shift, demap, decline gate, 30-iteration stall-ruled LDPC, CRC-32C" % F,
+        "blocks": int(g["ok"].size), "verified": int(g["good"]), "block_fail": round(1.0 - float(right.mean()), 4),
+        "verified_bytes_are_sent": bool(right.sum() == g["good"]), "modes_bit_identical": bool(same),
+        "modes": res,
+        "graph_over_plain": round(res["graph"]["frames_per_s"] / res["plain"]["frames_per_s"], 2),
+        "graph_over_frames": round(res["graph"]["frames_per_s"] / res["frames"]["frames_per_s"], 2),
+    }
+    b.close()
+    for d in keep:
+        d.free()
+    return out
+
+
+def bench_fft_demap(args):
+    from lizard_amd import sim
+    rng = np.random.default_rng(77)
+    table = {384: (5, 128), 768: (20, 256), 1152: (45, 512), 1536: (64, 512)}
+    ns = rng.choice(sorted(table), size=args.fft_frames)
+    versions = [table[int(n)][0] for n in ns]
+    spans = [table[int(n)][1] for n in ns]
+    keep, items = [], []
+    total = 0
+    for v in versions:
+        pic = sim.picture(v, sim.random_blocks(v, rng)).numpy()
+        # AWGN at 20 dB SNR against the picture's own variance
+        grid = (pic + rng.standard_normal(pic.shape) * np.sqrt(pic.var() / 100.0)).astype(np.float32)
+        d = lizard_amd.DeviceArray.of(grid)
+        keep.append(d)
+        items.append(dict(grid=d))
+        sb = soft_bytes(v)
+        total += sb["rows"] + sb["cols"] + sb["demap"]
+    frames = lizard_amd.make_frames(items)
+    b = lizard_amd.Batch(0).plan(versions, spans, source=lizard_amd.SRC_GRID, stages=lizard_amd.STAGE_SOFT)
+    wall, ev = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
+    fwall, fev = run_batch(b, frames, lizard_amd.RUN_FRAMES, max(1, args.pipe_runs // 2))
+    gbs = total / (ev * 1e-3) / 1e9
+    b.close()
+    for d in keep:
+        d.free()
+    return {"workload": "%d picture grids, n drawn from {384, 768, 1152, 1536} (%s), top formats, sender picture + "
+                        "AWGN 20 dB; erase, detrend, 2D FFT, pilot shift, soft demap" %
+                        (len(ns), ", ".join("%d x %d" % (int((ns == n).sum()), n) for n in sorted(table))),
+            "graph_ms_per_batch": round(ev, 3), "frames_per_s": round(len(ns) / wall, 1),
+            "frame_by_frame_ms_per_batch": round(fev, 3),
+            "algorithmic_bytes_per_batch": int(total),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+
+
+def cpu_pipeline(caps_host, seconds):
+    """The reference decoder's finish path (sampler, erase, FFT, soft values, LDPC, CRC; x86-64 build) through the
+    same registrations, one format instance per thread."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+    if not os.path.exists(ref_so):
+        return None
+    lib = C.CDLL(ref_so)
+    lib.ref_format_new.restype = C.c_void_p
+    lib.ref_format_new.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.ref_format_free.argtypes = [C.c_void_p]
+    fn = lib.ref_decode_reg
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p] * 2 + [C.c_int] * 2 + [C.c_void_p] * 8
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = max(1, min(cores, os.cpu_count() or 1, 16))
+    lock = threading.Lock()
+    state = {"next": 0, "frames": 0, "good": 0}
+    deadline = [0.0]
+
+    def worker():
+        f = lib.ref_format_new(1536, 1024, 512)
+        out = np.zeros(124 * 476, np.uint8)
+        ok, its, be = np.zeros(124, np.uint8), np.zeros(124, np.int8), np.zeros(124, np.float32)
+        while True:
+            with lock:
+                if time.perf_counter() > deadline[0]:
+                    break
+                c = caps_host[state["next"] % len(caps_host)]
+                state["next"] += 1
+            good = fn(f, c["img"].ctypes.data, c["img"].shape[1], c["img"].shape[0], c["H"].ctypes.data,
+                      c["nodes"].ctypes.data, out.ctypes.data, ok.ctypes.data, its.ctypes.data, be.ctypes.data, None,
+                      None)
+            with lock:
+                state["frames"] += 1
+                state["good"] += good
+        lib.ref_format_free(f)
+
+    t0 = time.perf_counter()
+    deadline[0] = t0 + seconds
+    th = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    return {"value": round(472.0 * state["good"] / wall / 1e6, 4), "unit": "MB/s payload", "cores": cores,
+            "kind": "reference", "frames_per_s": round(state["frames"] / wall, 2),
+            "sample": "%d frames (the same %d captures cycled), %.1f s wall on %d threads; liblizard lizard_finish "
+                      "through the held registration (x86-64 SIMD build)" % (state["frames"], len(caps_host), wall,
+                                                                            cores)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +323,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--pipe-frames", type=int, default=32)
+    ap.add_argument("--pipe-runs", type=int, default=10)
+    ap.add_argument("--fft-frames", type=int, default=64)
+    ap.add_argument("--no-pipeline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by frame/codeword with no exchange (see "
@@ -237,6 +419,17 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(batches, args.cpu_seconds)
 
+    pipe = fft = None
+    if not args.no_pipeline:
+        caps = pipeline_frames(args.pipe_frames, 4242)
+        pipe = bench_pipeline(args, caps)
+        fft = bench_fft_demap(args)
+        if not args.no_cpu:
+            host = caps[:8]  # the reference decoder cycles over the first captures of the same workload
+            pipe["cpu_baseline"] = cpu_pipeline(host, args.cpu_seconds)
+            if pipe["cpu_baseline"] and pipe["cpu_baseline"]["value"] > 0:
+                pipe["graph_over_cpu"] = round(pipe["modes"]["graph"]["payload_MBps"] / pipe["cpu_baseline"]["value"], 2)
+
     line = {
         "metric": METRIC, "value": round(gbps, 3), "unit": "Gbit/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True,
@@ -258,6 +451,8 @@ def main():
                      "hbm": {"achieved": round(hbm_bytes / (launch_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,
                              "copy_probe_GBps": round(copy_gbs, 1), "algorithmic_bytes_per_launch": hbm_bytes}},
         "cpu_baseline": cpu,
+        "pipeline": pipe,
+        "fft_demap": fft,
     }
     print(json.dumps(line))
 

@@ -90,6 +90,13 @@ enum { LIZX_ITS_CAP = -1, LIZX_ITS_DECLINED = -2, LIZX_ITS_STALL = -3 };
    only, no device. Returns 0 or LIZ_E_FORMAT. Mirrors lizard_init's layout (liblizard/src/lizard_phy.c:485-578). */
 LIZ_API int lizx_format_info(int version, int *info);
 
+/* The encoder's bit side of a frame (liblizard/src/lizard_phy.c:1087-1150: CRC-32C, LDPC encode, the bit map and
+   whitening, the pilots' tails, the padding), host: blocks * 476 bytes in, subch * 640 frame slots (0/1) out, cnt the
+   painted pictures' count mod 4. Returns 0 or a negative LIZ_E_*. */
+LIZ_API int lizx_frame_slots(int version, const uint8_t *blocks, int cnt, uint8_t *slots);
+/* The coefficient order (research/SPEC.md 6.3) of LIZARD-16V: u and v of each of its subch * 320 coefficients. */
+LIZ_API int lizx_format_coefs(int version, int16_t *u, int16_t *v);
+
 /* lizard_finish_ext with ext_llr (liblizard/src/lizard_phy.c:1566, :1753-1871): the frame's soft values made
    elsewhere: llr = 16 * version * 640 int8 in frame slot order (whitened), est = one information estimate per
    sub-channel. Decline gate, bit map, layered min-sum LDPC (30 iterations, stall 9 / 0.95), bit pack and CRC-32C on
@@ -182,6 +189,45 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
                                     float stall_ratio, int flags, uint8_t *d_bytes, uint8_t *d_crc_ok);
 LIZ_API int lizx_ldpc_sync(lizx_ldpc *h);
 LIZ_API float lizx_ldpc_last_ms(lizx_ldpc *h);
+
+/* ---- Batches: many frames of mixed formats through one launch sequence (liblizard/include/lizard.h:285-287
+   liz_receiver_batch_cap, liblizard/core/rx/receiver.h:77-95 Receiver::push: the reference batches whole frames per
+   receiver; here a batch is one HIP graph: per format the fused warp + erase + row FFT, the column FFT, the pilot
+   shift and the soft values, then one persistent LDPC launch over every frame's blocks and their CRCs). */
+typedef struct lizx_batch lizx_batch;
+typedef struct {
+  const uint8_t *img;  /* device: tight uint8 luma capture, w x h (LIZX_SRC_CAPTURE) */
+  int w, h;
+  float H[9];          /* module coordinates -> capture pixels, h8 = 1 */
+  const float *nodes;  /* device: the node lattice's (dx, dy) pixels, ny rows of nx */
+  const float *grid;   /* device: n x n picture samples, row-major (LIZX_SRC_GRID) */
+} lizx_frame;
+enum { LIZX_SRC_CAPTURE = 0, LIZX_SRC_GRID = 1 };
+enum { LIZX_STAGE_SOFT = 0, LIZX_STAGE_BLOCKS = 1 };
+/* LIZX_RUN_GRAPH: the batch as one captured HIP graph (captured on the first run, replayed after); LIZX_RUN_STREAM:
+   the same launches without the graph; LIZX_RUN_FRAMES: frame by frame, each frame's own launches (one persistent
+   LDPC launch over its blocks), waiting for each frame before the next, as a decoder called once a frame runs;
+   LIZX_RUN_PLAIN: frame by frame as LIZX_RUN_FRAMES with the LDPC launched once a codeword. Every mode gives the
+   same bits. */
+enum { LIZX_RUN_GRAPH = 0, LIZX_RUN_STREAM = 1, LIZX_RUN_FRAMES = 2, LIZX_RUN_PLAIN = 3 };
+LIZ_API lizx_batch *lizx_batch_new(int device);
+LIZ_API void lizx_batch_free(lizx_batch *b);
+/* frames frames, frame i of LIZARD-16 version[i] in the ring of span span[i] (128, 256 or 512 modules); source
+   LIZX_SRC_*, stages LIZX_STAGE_*. Builds the plans and the device work areas once; a plan may be run many times. */
+LIZ_API int lizx_batch_plan(lizx_batch *b, int frames, const int *version, const int *span, int source, int stages);
+/* dims: frames, total blocks, total soft-value bytes, distinct formats. */
+LIZ_API int lizx_batch_dims(const lizx_batch *b, int *dims);
+/* Enqueues one run over frames[0 .. frames-1] (device-resident inputs) in a LIZX_RUN_* mode. Asynchronous except
+   LIZX_RUN_FRAMES and LIZX_RUN_PLAIN, which wait per frame. */
+LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode);
+/* Waits for the runs; *ms (optional) receives the GPU time from the first run's start since the last sync to the last
+   run's end. */
+LIZ_API int lizx_batch_sync(lizx_batch *b, float *ms);
+/* The last run's blocks, frame after frame in block order: total blocks * 476 bytes (only verified ones written),
+   verdicts, iterations (-2 declined, -1/-3 failed), block estimates. Returns the verified count. */
+LIZ_API int lizx_batch_blocks(lizx_batch *b, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est);
+/* Frame i's soft values (subch * 640), sub-channel estimates and rms, and pilot shift (each optional). */
+LIZ_API int lizx_batch_soft(lizx_batch *b, int frame, int8_t *llr, double *sub_est, float *sub_rms, float *align);
 
 /* The systematic IRA encoder (liblizard/src/ldpc.h:44, ldpc.c:117-126), host: count codewords of one rate index;
    data holds count * k bits (0/1 a byte), cw receives count * n. Returns 0 or LIZ_E_ARG. */

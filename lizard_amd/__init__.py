@@ -17,6 +17,9 @@ LIZ_OK, LIZ_E_ARG, LIZ_E_NOMEM, LIZ_E_IO, LIZ_E_UNSUPPORTED, LIZ_E_FORMAT, LIZ_E
     0, -1, -2, -3, -4, -5, -6, -7, -8)
 LIZ_BLOCK = 476
 LDPC_CRC, LDPC_TRACE, LDPC_PLAIN = 1, 2, 4
+SRC_CAPTURE, SRC_GRID = 0, 1
+STAGE_SOFT, STAGE_BLOCKS = 0, 1
+RUN_GRAPH, RUN_STREAM, RUN_FRAMES, RUN_PLAIN = 0, 1, 2, 3
 MSG_TRACE_STRIDE = 210 * 128
 RATE_INDEX = {"7/8": 6, "3/4": 4, "2/3": 3, "1/2": 2}
 
@@ -38,6 +41,12 @@ class _Decoded(C.Structure):
 
 class Registration(C.Structure):
     _fields_ = [("H", C.c_float * 9), ("nodes", C.c_void_p)]
+
+
+class Frame(C.Structure):
+    """lizx_frame: one frame of a batch (device pointers)."""
+    _fields_ = [("img", C.c_void_p), ("w", C.c_int), ("h", C.c_int), ("H", C.c_float * 9), ("nodes", C.c_void_p),
+                ("grid", C.c_void_p)]
 
 
 class Soft(C.Structure):
@@ -73,9 +82,19 @@ def lib():
     sig("lizx_finish_llr", _I, _P, _I, _P, _P, _P, _P, _P, _P)
     sig("lizx_finish_grid", _I, _P, _I, _I, _P, _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_node_lattice", _I, _I, _P)
+    sig("lizx_frame_slots", _I, _I, _P, _I, _P)
+    sig("lizx_format_coefs", _I, _I, _P, _P)
     sig("lizx_warp", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P)
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
+    sig("lizx_batch_new", _P, _I)
+    sig("lizx_batch_free", None, _P)
+    sig("lizx_batch_plan", _I, _P, _I, _P, _P, _I, _I)
+    sig("lizx_batch_dims", _I, _P, _P)
+    sig("lizx_batch_run", _I, _P, _P, _I)
+    sig("lizx_batch_sync", _I, _P, C.POINTER(C.c_float))
+    sig("lizx_batch_blocks", _I, _P, _P, _P, _P, _P)
+    sig("lizx_batch_soft", _I, _P, _I, _P, _P, _P, _P)
     sig("lizx_ldpc_dims", _I, _I, _P)
     sig("lizx_ldpc_new", _P, _I)
     sig("lizx_ldpc_free", None, _P)
@@ -121,6 +140,21 @@ def ldpc_dims(rate):
     check(lib().lizx_ldpc_dims(rate, _p(d)))
     return dict(zip(("n", "k", "m", "z", "mb", "kb", "np", "nt", "edges", "slots_total", "msg_bytes", "post_bytes"),
                     map(int, d)))
+
+
+def frame_slots(version, blocks, cnt=0):
+    info = format_info(version)
+    blocks = np.ascontiguousarray(blocks, np.uint8)
+    slots = np.zeros(info["subch"] * 640, np.uint8)
+    check(lib().lizx_frame_slots(version, _p(blocks), cnt, _p(slots)))
+    return slots
+
+
+def format_coefs(version):
+    S = format_info(version)["subch"] * 320
+    u, v = np.zeros(S, np.int16), np.zeros(S, np.int16)
+    check(lib().lizx_format_coefs(version, _p(u), _p(v)))
+    return u.astype(np.int64), v.astype(np.int64)
 
 
 def node_lattice(span):
@@ -350,3 +384,75 @@ class Decoder:
         if soft:
             out["align"] = (sp.align[0], sp.align[1])
         return out
+
+
+class Batch:
+    """lizx_batch_*: many frames of mixed formats through one launch sequence (a HIP graph once captured)."""
+
+    def __init__(self, device=-1):
+        self.h = lib().lizx_batch_new(device)
+        if not self.h:
+            check(-1)
+        self.frames = 0
+
+    def close(self):
+        if self.h:
+            lib().lizx_batch_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan(self, versions, spans, source=SRC_CAPTURE, stages=STAGE_BLOCKS):
+        v = np.ascontiguousarray(versions, np.int32)
+        s = np.ascontiguousarray(spans, np.int32)
+        check(lib().lizx_batch_plan(self.h, len(v), _p(v), _p(s), source, stages))
+        d = np.zeros(4, np.int32)
+        check(lib().lizx_batch_dims(self.h, _p(d)))
+        self.frames, self.blocks, self.soft_bytes, self.formats = map(int, d)
+        return self
+
+    def run(self, frames, mode=RUN_GRAPH):
+        """frames: a ctypes array of Frame (make_frames)."""
+        check(lib().lizx_batch_run(self.h, frames, mode))
+
+    def sync(self):
+        ms = C.c_float(0)
+        check(lib().lizx_batch_sync(self.h, C.byref(ms)))
+        return ms.value
+
+    def blocks_out(self):
+        B = self.blocks
+        blocks, ok = np.zeros(B * LIZ_BLOCK, np.uint8), np.zeros(B, np.uint8)
+        its, be = np.zeros(B, np.int8), np.zeros(B, np.float32)
+        good = check(lib().lizx_batch_blocks(self.h, _p(blocks), _p(ok), _p(its), _p(be)))
+        return {"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be}
+
+    def soft(self, frame, version):
+        info = format_info(version)
+        S = info["subch"] * 320
+        llr, est = np.zeros(2 * S, np.int8), np.zeros(info["subch"], np.float64)
+        rms, al = np.zeros(info["subch"], np.float32), np.zeros(2, np.float32)
+        check(lib().lizx_batch_soft(self.h, frame, _p(llr), _p(est), _p(rms), _p(al)))
+        return {"llr": llr, "sub_est": est, "sub_rms": rms, "align": al}
+
+
+def make_frames(items):
+    """A ctypes array of Frame from dicts with device arrays: img (DeviceArray), w, h, H (9), nodes (DeviceArray), or
+    grid (DeviceArray)."""
+    arr = (Frame * len(items))()
+    for i, it in enumerate(items):
+        f = arr[i]
+        if it.get("grid") is not None:
+            f.grid = it["grid"].ptr
+        if it.get("img") is not None:
+            f.img = it["img"].ptr
+            f.w, f.h = int(it["w"]), int(it["h"])
+            H = np.asarray(it["H"], np.float32).ravel()
+            for k in range(9):
+                f.H[k] = float(H[k])
+            f.nodes = it["nodes"].ptr
+    return arr

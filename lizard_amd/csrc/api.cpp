@@ -31,7 +31,7 @@ void clear_error() {
   last_code = LIZ_OK;
 }
 
-static int current_device(int want) {
+int current_device(int want) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) throw Error(LIZ_E_UNSUPPORTED, "no HIP device");
   int dev = want;
@@ -155,6 +155,18 @@ static void codeword_tasks(int count, const uint8_t *rate, const int64_t *off, s
   std::stable_sort(t.begin(), t.end(), [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
 }
 
+// The systematic QC-IRA encoder (liblizard/src/ldpc.c:117): data bits as they are, then parity bit m (check i of block row
+// r, m = i * mb + r) as the running XOR of every check's data sum in that order (the staircase's accumulator).
+static void encode_word(const Code &c, const uint8_t *d, uint8_t *o) {
+  std::memcpy(o, d, c.k);
+  uint8_t p = 0;
+  for (int chk = 0; chk < c.m; chk++) {
+    const int i = chk / c.mb, r = chk % c.mb;
+    for (int e = c.lay_ptr[r]; e < c.lay_ptr[r + 1]; e++) p ^= d[c.lay_col[e] * Z + ((i - c.lay_shift[e]) & (Z - 1))];
+    o[c.k + chk] = p;
+  }
+}
+
 static void check_params(int max_iter, int stall_it, float ratio) {
   need(max_iter >= 1 && max_iter <= 127, "max_iter is 1 to 127");
   need(stall_it >= 0 && stall_it <= 127, "stall_it is 0 to 127");
@@ -217,6 +229,56 @@ LIZ_API int lizx_format_info(int version, int *info) {
   });
 }
 
+LIZ_API int lizx_frame_slots(int version, const uint8_t *blocks, int cnt, uint8_t *slots) {
+  return guard([&] {
+    need(blocks && slots && cnt >= 0 && cnt <= 3, "blocks, slots and a count 0 to 3");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    std::memcpy(slots, f->white.data(), f->white.size());  // the padding: the whitening alone
+    std::vector<uint8_t> data(4096), cw(MAX_N);
+    for (int b = 0; b < f->blocks; b++) {
+      const Code &c = code(f->blk_code[b]);
+      const uint8_t *src = blocks + static_cast<size_t>(b) * BLOCK_BYTES;
+      std::fill(data.begin(), data.end(), 0);
+      for (int i = 0; i < 8 * BLOCK_BYTES; i++) data[i] = (src[i >> 3] >> (7 - (i & 7))) & 1;
+      const uint32_t crc = crc32c(src, BLOCK_BYTES);
+      for (int i = 0; i < 32; i++) data[8 * BLOCK_BYTES + i] = (crc >> (31 - i)) & 1;
+      encode_word(c, data.data(), cw.data());
+      // slot i: codeword bit at the bit map's position, or past nt the pilot bit (bit 0 of the count on even blocks,
+      // bit 1 on odd ones), whitened (SPEC 7.3)
+      const auto &s2p = f->slot_to_post[f->blk_code[b]];
+      const uint8_t q = static_cast<uint8_t>((cnt >> (b & 1)) & 1);
+      const int s0 = f->blk_slot[b], nb = f->blk_nslots[b];
+      for (int i = 0; i < nb; i++) {
+        uint8_t bit = q;
+        if (i < c.nt) {
+          // the map holds the posterior index of the codeword bit; back to the codeword index
+          const int pi = s2p[i];
+          const int v = pi < c.k ? pi : c.k + ((pi - c.k) % Z) * c.mb + (pi - c.k) / Z;
+          bit = cw[v];
+        }
+        slots[s0 + i] = static_cast<uint8_t>(bit ^ f->white[s0 + i]);
+      }
+    }
+    return 0;
+  });
+}
+
+LIZ_API int lizx_format_coefs(int version, int16_t *u, int16_t *v) {
+  return guard([&] {
+    need(u && v, "outputs");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    const int n = f->n, bw = f->bw;
+    for (size_t i = 0; i < f->pos.size(); i++) {
+      const int pp = f->pos[i], vv = (pp / (n * bw)) * bw + pp % bw, uw = (pp / bw) % n;
+      u[i] = static_cast<int16_t>(uw < n / 2 ? uw : uw - n);
+      v[i] = static_cast<int16_t>(vv);
+    }
+    return 0;
+  });
+}
+
 LIZ_API int lizx_ldpc_dims(int rate, int *dims) {
   return guard([&] {
     need(dims, "a place for the dims");
@@ -241,13 +303,7 @@ LIZ_API int lizx_ldpc_encode(int rate, int count, const uint8_t *data, uint8_t *
     for (int w = 0; w < count; w++) {
       const uint8_t *d = data + static_cast<size_t>(w) * c.k;
       uint8_t *o = cw + static_cast<size_t>(w) * c.n;
-      std::memcpy(o, d, c.k);
-      uint8_t p = 0;
-      for (int chk = 0; chk < c.m; chk++) {
-        const int i = chk / c.mb, r = chk % c.mb;
-        for (int e = c.lay_ptr[r]; e < c.lay_ptr[r + 1]; e++) p ^= d[c.lay_col[e] * Z + ((i - c.lay_shift[e]) & (Z - 1))];
-        o[c.k + chk] = p;
-      }
+      encode_word(c, d, o);
     }
     return 0;
   });

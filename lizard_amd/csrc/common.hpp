@@ -18,6 +18,8 @@ struct Error : std::runtime_error {
 
 int fail(int code, const std::string &message);
 void clear_error();
+// Makes device `want` (or, when negative, the current one) current; throws LIZ_E_UNSUPPORTED without a HIP device.
+int current_device(int want);
 
 inline void need(bool ok, const char *what) {
   if (!ok) throw Error(LIZ_E_ARG, what);

@@ -6,7 +6,6 @@
 //   k_cols       two columns u a workgroup: FFT along y, keeping v < E and n - v (the upper half-plane's reach)
 //   k_align      one workgroup a frame: the detrend projections and the pilots' two-pass least-squares grid shift
 //   k_demap      one workgroup a sub-channel: 320 coefficients gathered, detrended, turned, their moments, int8 LLRs
-//   k_blk_pilot  per block: the pilots' reading (lizard_pilot)
 // Floating-point steps the reference rounds in a fixed order (the detrend terms, the turn, the LLR scale) use
 // explicitly rounded operations, so no contraction into fused multiply-adds changes them.
 #include <hip/hip_runtime.h>
@@ -202,6 +201,12 @@ struct GridSource {
   const float *grid;
   __device__ __forceinline__ float operator()(const PlanDev &P, int f, int x, int y) const {
     return grid[(static_cast<size_t>(f) * P.n + y) * P.n + x];
+  }
+};
+struct GridListSource {
+  const float *const *grids;
+  __device__ __forceinline__ float operator()(const PlanDev &P, int f, int x, int y) const {
+    return grids[f][static_cast<size_t>(y) * P.n + x];
   }
 };
 struct WarpSource {
@@ -721,29 +726,60 @@ void FrameBufs::reserve(const FramePlan &p, int frames, bool debug) {
   if (debug) coef_dbg.reserve(frames * S * sizeof(float2));
 }
 
+SoftBufs FrameBufs::view(bool debug) {
+  SoftBufs v;
+  v.beta = beta.as<double>();
+  v.rowsum = rowsum.as<double>();
+  v.proj = proj.as<double>();
+  v.X = X.as<float2>();
+  v.HP = HP.as<float2>();
+  v.align = align.as<float>();
+  v.llr = llr.as<int8_t>();
+  v.est = est.as<double>();
+  v.rms = rms.as<float>();
+  v.coef_dbg = debug ? coef_dbg.as<float>() : nullptr;
+  return v;
+}
+
+SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
+  const size_t n = p.n, E = p.E, S = static_cast<size_t>(p.subch) * SUB, F = f;
+  SoftBufs v;
+  v.beta = beta + F * 6;
+  v.rowsum = rowsum + F * n * 3;
+  v.proj = proj + F * 5;
+  v.X = X + F * E * n;
+  v.HP = HP + F * E * 2 * E;
+  v.align = align + F * 2;
+  v.llr = llr + F * 2 * S;
+  v.est = est + F * p.subch;
+  v.rms = rms + F * p.subch;
+  v.coef_dbg = coef_dbg ? coef_dbg + F * 2 * S : nullptr;
+  return v;
+}
+
 template <class Src>
-static void launch_soft_src(const FramePlan &p, FrameBufs &b, Src src, int frames, bool debug, hipStream_t s) {
+static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s) {
   const int n = p.n, E = p.E;
-  hipLaunchKernelGGL(k_erase_fit<Src>, dim3(frames), dim3(256), 0, s, p.dev, src, b.beta.as<double>());
+  hipLaunchKernelGGL(k_erase_fit<Src>, dim3(frames), dim3(256), 0, s, p.dev, src, b.beta);
   hipLaunchKernelGGL(k_rows<Src>, dim3(n / ROWS, frames), dim3(256), (ROWS / 2) * n * sizeof(float2), s, p.dev, src,
-                     b.beta.as<double>(), b.X.as<float2>(), b.rowsum.as<double>());
-  hipLaunchKernelGGL(k_cols, dim3((E + 1) / 2, frames), dim3(256), 2 * n * sizeof(float2), s, p.dev, b.X.as<float2>(),
-                     b.HP.as<float2>());
-  hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum.as<double>(), b.HP.as<float2>(),
-                     b.proj.as<double>(), b.align.as<float>());
-  hipLaunchKernelGGL(k_demap, dim3(p.subch, frames), dim3(320), 0, s, p.dev, b.HP.as<float2>(), b.proj.as<double>(),
-                     b.align.as<float>(), b.llr.as<int8_t>(), b.est.as<double>(), b.rms.as<float>(),
-                     debug ? b.coef_dbg.as<float>() : nullptr);
+                     b.beta, b.X, b.rowsum);
+  hipLaunchKernelGGL(k_cols, dim3((E + 1) / 2, frames), dim3(256), 2 * n * sizeof(float2), s, p.dev, b.X, b.HP);
+  hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.HP, b.proj, b.align);
+  hipLaunchKernelGGL(k_demap, dim3(p.subch, frames), dim3(320), 0, s, p.dev, b.HP, b.proj, b.align, b.llr, b.est,
+                     b.rms, b.coef_dbg);
   hip_check(hipGetLastError(), "soft-value launches");
 }
 
-void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s) {
-  launch_soft_src(p, b, GridSource{d_grid}, frames, debug, s);
+void launch_soft(const FramePlan &p, const SoftBufs &b, const float *d_grid, int frames, hipStream_t s) {
+  launch_soft_src(p, b, GridSource{d_grid}, frames, s);
 }
 
-void launch_soft_warp(const FramePlan &p, FrameBufs &b, const WarpFrame *d_frames, int frames, bool debug,
-                      hipStream_t s) {
-  launch_soft_src(p, b, WarpSource{d_frames}, frames, debug, s);
+void launch_soft_grids(const FramePlan &p, const SoftBufs &b, const float *const *d_grids, int frames, hipStream_t s) {
+  launch_soft_src(p, b, GridListSource{d_grids}, frames, s);
+}
+
+void launch_soft_warp(const FramePlan &p, const SoftBufs &b, const WarpFrame *d_frames, int frames, hipStream_t s) {
+  launch_soft_src(p, b, WarpSource{d_frames}, frames, s);
 }
 
 void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, float *d_grid, hipStream_t s) {

@@ -63,17 +63,38 @@ struct FramePlan {
 // Builds (and uploads) the plan for LIZARD-16V in the ring of the given span (128, 256 or 512 modules).
 std::unique_ptr<FramePlan> make_plan(int version, int span);
 
+// Raw device pointers to one run's work areas and outputs; frame f of a run sits at f times each per-frame size.
+struct SoftBufs {
+  double *beta = nullptr, *rowsum = nullptr, *proj = nullptr;
+  float2 *X = nullptr, *HP = nullptr;
+  float *align = nullptr;
+  int8_t *llr = nullptr;
+  double *est = nullptr;
+  float *rms = nullptr;
+  float *coef_dbg = nullptr;  // null: no debug copy of the coefficients
+  SoftBufs at(const FramePlan &p, int f) const;  // frame f's slice
+};
+
 // Per-frame device work areas for one decode.
 struct FrameBufs {
   DevBuf grid, X, HP, rowsum, beta, proj, align, llr, est, rms, coef_dbg, pilot_r;
   void reserve(const FramePlan &p, int frames, bool debug);
+  SoftBufs view(bool debug);
 };
 
 // Enqueues grid (row-major n x n float per frame, frames back to back) -> soft values, estimates, pilot shift.
-void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s);
+void launch_soft(const FramePlan &p, const SoftBufs &b, const float *d_grid, int frames, hipStream_t s);
+// The same with each frame's grid at its own address (d_grids: frames device pointers, on the device).
+void launch_soft_grids(const FramePlan &p, const SoftBufs &b, const float *const *d_grids, int frames, hipStream_t s);
 // The same from captures through their registrations (warp fused into the erase fit and the row transform).
-void launch_soft_warp(const FramePlan &p, FrameBufs &b, const WarpFrame *d_frames, int frames, bool debug,
-                      hipStream_t s);
+void launch_soft_warp(const FramePlan &p, const SoftBufs &b, const WarpFrame *d_frames, int frames, hipStream_t s);
+inline void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s) {
+  launch_soft(p, b.view(debug), d_grid, frames, s);
+}
+inline void launch_soft_warp(const FramePlan &p, FrameBufs &b, const WarpFrame *d_frames, int frames, bool debug,
+                             hipStream_t s) {
+  launch_soft_warp(p, b.view(debug), d_frames, frames, s);
+}
 // The warp alone: n x n picture samples a frame, row-major.
 void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, float *d_grid, hipStream_t s);
 // The node lattice of a ring's layout (module coordinates; layout.c:110-118).

@@ -1,0 +1,174 @@
+"""Batches (lizx_batch_*): frames of mixed formats through one HIP graph, the same launches without it, frame by
+frame, and frame by frame with one LDPC launch a codeword. Every mode must give the single-frame path's bits
+(lizx_decode_reg / lizx_finish_grid, themselves held to the compiled reference in test_warp_gpu.py and
+test_soft_gpu.py), the graph on every replay, and the reference's own decoder's verdicts and bytes."""
+import numpy as np
+import pytest
+
+import lizard_amd as L
+from lizard_amd import sim
+from _libs import ptr, ref
+
+pytestmark = pytest.mark.gpu
+
+# (version, span, camera): small, decodable captures of three formats
+FRAMES = [
+    (5, 128, dict(W=1280, Hh=720, rot_deg=5.0, scale=0.8, persp=0.02, k1=0.0, blur=0.8, noise=4.0)),
+    (20, 256, dict(W=1920, Hh=1080, rot_deg=-4.0, scale=0.9, persp=0.02, k1=0.03, blur=0.7, noise=5.0)),
+    (5, 128, dict(W=1280, Hh=720, rot_deg=-8.0, scale=0.85, persp=0.03, k1=-0.04, blur=1.2, noise=12.0)),
+    (8, 256, dict(W=1280, Hh=720, rot_deg=2.0, scale=0.95, persp=0.01, k1=0.02, blur=0.6, noise=6.0)),
+    (20, 256, dict(W=1920, Hh=1080, rot_deg=9.0, scale=0.7, persp=0.04, k1=-0.02, blur=1.4, noise=8.0)),
+]
+
+
+def make_captures(frames, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k, (version, span, cam) in enumerate(frames):
+        blocks = sim.random_blocks(version, rng)
+        img, H, nodes = sim.capture(sim.picture(version, blocks, cnt=k % 4), span, sim.Camera(**cam), seed=seed + k)
+        out.append(dict(version=version, span=span, blocks=blocks, img=img.numpy(), H=H, nodes=nodes))
+    return out
+
+
+def device_frames(caps):
+    keep = []
+    items = []
+    for c in caps:
+        d_img, d_nodes = L.DeviceArray.of(c["img"]), L.DeviceArray.of(c["nodes"])
+        keep += [d_img, d_nodes]
+        items.append(dict(img=d_img, w=c["img"].shape[1], h=c["img"].shape[0], H=c["H"], nodes=d_nodes))
+    return L.make_frames(items), keep
+
+
+def single(decoder, caps):
+    outs = [decoder.decode_reg(c["version"], c["span"], c["img"], c["H"], c["nodes"], soft=True) for c in caps]
+    return {k: np.concatenate([o[k] for o in outs]) for k in ("ok", "its", "est", "blocks")}, outs
+
+
+def same(got, want):
+    assert np.array_equal(got["its"], want["its"])
+    assert np.array_equal(got["ok"], want["ok"])
+    assert np.array_equal(got["est"], want["est"])
+    v = want["ok"].astype(bool)
+    assert np.array_equal(got["blocks"][v], want["blocks"][v])
+
+
+@pytest.fixture(scope="module")
+def caps():
+    return make_captures(FRAMES, 11)
+
+
+def test_batch_modes_match_single_frames(decoder_gpu, caps):
+    want, outs = single(decoder_gpu, caps)
+    assert want["ok"].sum() > 0.5 * len(want["ok"])  # most of the channel decodes
+    b = L.Batch().plan([c["version"] for c in caps], [c["span"] for c in caps])
+    assert b.formats == 3 and b.blocks == len(want["ok"])
+    frames, keep = device_frames(caps)
+    for mode in (L.RUN_GRAPH, L.RUN_STREAM, L.RUN_FRAMES, L.RUN_PLAIN, L.RUN_GRAPH):
+        b.run(frames, mode)
+        b.sync()
+        same(b.blocks_out(), want)
+    # soft values of each frame equal the single-frame path's
+    for i, c in enumerate(caps):
+        s = b.soft(i, c["version"])
+        assert np.array_equal(s["llr"], outs[i]["llr"])
+        assert np.array_equal(s["sub_est"], outs[i]["sub_est"])
+        assert tuple(s["align"]) == tuple(np.float32(outs[i]["align"]))
+    b.close()
+
+
+def test_batch_verified_bytes_are_the_sent_blocks(caps):
+    b = L.Batch().plan([c["version"] for c in caps], [c["span"] for c in caps])
+    frames, keep = device_frames(caps)
+    b.run(frames, L.RUN_GRAPH)
+    b.sync()
+    got = b.blocks_out()
+    sent = np.concatenate([c["blocks"].reshape(-1, 476) for c in caps])
+    v = got["ok"].astype(bool)
+    assert np.array_equal(got["blocks"][v], sent[v])
+    r = ref("scalar")
+    if r is not None:  # the reference's own decoder on the same captures and registrations: the same verdicts
+        at = 0
+        for c in caps:
+            info = L.format_info(c["version"])
+            B = info["blocks"]
+            f = r.ref_format_new(info["n"], info["subch"], c["span"])
+            out, ok = np.zeros(B * 476, np.uint8), np.zeros(B, np.uint8)
+            its, be = np.zeros(B, np.int8), np.zeros(B, np.float32)
+            r.ref_decode_reg(f, ptr(c["img"]), c["img"].shape[1], c["img"].shape[0], ptr(c["H"]),
+                             ptr(np.ascontiguousarray(c["nodes"])), ptr(out), ptr(ok), ptr(its), ptr(be), None, None)
+            r.ref_format_free(f)
+            assert np.array_equal(got["ok"][at:at + B], ok)
+            assert np.array_equal(got["its"][at:at + B], its)
+            at += B
+    b.close()
+
+
+def test_graph_replays_with_other_work_between(decoder_gpu, caps):
+    """The captured graph replayed several times, with eager work of the library's own in between, gives the eager
+    result every time; new inputs on the same plan are taken up by the same graph."""
+    b = L.Batch().plan([c["version"] for c in caps], [c["span"] for c in caps])
+    frames, keep = device_frames(caps)
+    b.run(frames, L.RUN_STREAM)
+    b.sync()
+    eager = b.blocks_out()
+    for k in range(3):
+        b.run(frames, L.RUN_GRAPH)
+        decoder_gpu.decode_reg(caps[k]["version"], caps[k]["span"], caps[k]["img"], caps[k]["H"], caps[k]["nodes"])
+        b.sync()
+        same(b.blocks_out(), eager)
+    other = make_captures(FRAMES, 23)
+    want, _ = single(decoder_gpu, other)
+    frames2, keep2 = device_frames(other)
+    b.run(frames2, L.RUN_GRAPH)
+    b.sync()
+    same(b.blocks_out(), want)
+    b.run(frames, L.RUN_GRAPH)
+    b.sync()
+    same(b.blocks_out(), eager)
+    b.close()
+
+
+def test_grid_batch_of_mixed_sizes(decoder_gpu):
+    """Soft values only, from picture grids of four sizes (n = 384, 768, 1152, 1536) in one batch."""
+    rng = np.random.default_rng(5)
+    versions = [5, 20, 45, 64, 20, 5]
+    spans = [128, 256, 512, 512, 256, 128]
+    grids, keep, items = [], [], []
+    for v in versions:
+        pic = sim.picture(v, sim.random_blocks(v, rng)).numpy()
+        g = (pic + rng.standard_normal(pic.shape) * 0.02).astype(np.float32)
+        grids.append(g)
+        d = L.DeviceArray.of(g)
+        keep.append(d)
+        items.append(dict(grid=d))
+    frames = L.make_frames(items)
+    b = L.Batch().plan(versions, spans, source=L.SRC_GRID, stages=L.STAGE_SOFT)
+    assert b.formats == 4
+    for mode in (L.RUN_GRAPH, L.RUN_FRAMES, L.RUN_GRAPH):
+        b.run(frames, mode)
+        b.sync()
+        for i, (v, s) in enumerate(zip(versions, spans)):
+            want = decoder_gpu.finish_grid(v, s, grids[i])
+            got = b.soft(i, v)
+            assert np.array_equal(got["llr"], want["llr"])
+            assert np.array_equal(got["sub_est"], want["sub_est"])
+            assert np.array_equal(got["sub_rms"], want["sub_rms"])
+    b.close()
+
+
+def test_batch_argument_errors():
+    b = L.Batch()
+    with pytest.raises(L.LizError):
+        b.plan([5], [100])
+    with pytest.raises(L.LizError):
+        b.plan([3], [128])
+    with pytest.raises(L.LizError):
+        b.run(L.make_frames([{}]), L.RUN_GRAPH)  # not planned
+    b.plan([5], [128])
+    with pytest.raises(L.LizError):
+        b.run(L.make_frames([{}]), L.RUN_GRAPH)  # no capture
+    with pytest.raises(L.LizError):
+        b.run(L.make_frames([{}]), 9)
+    b.close()
