@@ -327,6 +327,8 @@ def main():
     ap.add_argument("--pipe-runs", type=int, default=10)
     ap.add_argument("--fft-frames", type=int, default=64)
     ap.add_argument("--no-pipeline", action="store_true")
+    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap"), default=None,
+                    help="run one workload alone (for per-kernel profiles); the JSON line then carries only it")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by frame/codeword with no exchange (see "
@@ -335,6 +337,14 @@ def main():
     if lizard_amd.device_count() < 1:
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
+
+    if args.only in ("pipeline", "fft_demap"):
+        out = (bench_pipeline(args, pipeline_frames(args.pipe_frames, 4242)) if args.only == "pipeline"
+               else bench_fft_demap(args))
+        print(json.dumps({"only": args.only, args.only: out}))
+        return
+    if args.only == "ldpc":
+        args.no_pipeline = True
 
     batches = [build_batch(db, seed=1000 + i) for i, db in enumerate(EBN0)]
     count = batches[0][0].size

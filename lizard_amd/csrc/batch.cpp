@@ -44,8 +44,28 @@ struct lizx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<uint8_t> out_bytes;
+  // one more stream a format beyond the first, so the formats' soft-value launches overlap (default priority)
+  std::vector<hipStream_t> side;
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
 
-  ~lizx_batch() { drop_graph(); }
+  ~lizx_batch() {
+    drop_graph();
+    for (auto e : join) (void)hipEventDestroy(e);
+    for (auto st : side) (void)hipStreamDestroy(st);
+    if (fork) (void)hipEventDestroy(fork);
+  }
+  void ensure_streams(int count) {
+    if (!fork) hip_check(hipEventCreateWithFlags(&fork, hipEventDisableTiming), "hipEventCreate");
+    while (static_cast<int>(side.size()) < count) {
+      hipStream_t st = nullptr;
+      hip_check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+      side.push_back(st);
+      hipEvent_t e = nullptr;
+      hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+      join.push_back(e);
+    }
+  }
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -74,11 +94,10 @@ struct lizx_batch {
     return p;
   }
 
-  // The soft-value launches of frames [i0, i0 + count) of group g (internal indices within the group).
-  void soft(const Group &g, int i0, int count) const {
+  // The soft-value launches of frames [i0, i0 + count) of group g (internal indices within the group) on stream s.
+  void soft(const Group &g, int i0, int count, hipStream_t s) const {
     const FramePlan &p = *g.plan;
     const SoftBufs v = g.view.at(p, i0);
-    const hipStream_t s = eng->stream;
     if (source == LIZX_SRC_GRID)
       launch_soft_grids(p, v, grids.as<const float *>() + g.first + i0, count, s);
     else
@@ -87,7 +106,19 @@ struct lizx_batch {
 
   // The whole batch's launches, in stream order (what the graph holds).
   void enqueue_batch(bool timed) const {
-    for (const auto &g : groups) soft(*g, 0, static_cast<int>(g->frames.size()));
+    const hipStream_t s = eng->stream;
+    if (groups.size() == 1) {
+      soft(*groups[0], 0, static_cast<int>(groups[0]->frames.size()), s);
+    } else {
+      // fork: each format on its own stream, joined back before the LDPC (a graph captures the same dependencies)
+      hip_check(hipEventRecord(fork, s), "hipEventRecord");
+      for (size_t g = 0; g < groups.size(); g++) {
+        hip_check(hipStreamWaitEvent(side[g], fork, 0), "hipStreamWaitEvent");
+        soft(*groups[g], 0, static_cast<int>(groups[g]->frames.size()), side[g]);
+        hip_check(hipEventRecord(join[g], side[g]), "hipEventRecord");
+        hip_check(hipStreamWaitEvent(s, join[g], 0), "hipStreamWaitEvent");
+      }
+    }
     if (stages == LIZX_STAGE_BLOCKS) {
       LdpcParams p = ldpc_params();
       p.tasks = tasks.as<LdpcTask>();
@@ -99,7 +130,7 @@ struct lizx_batch {
   // One frame's launches (caller index f).
   void enqueue_frame(int f, bool plain) const {
     const Group &g = *groups[group_of[f]];
-    soft(g, slot_of[f], 1);
+    soft(g, slot_of[f], 1, eng->stream);
     if (stages == LIZX_STAGE_BLOCKS) {
       LdpcParams p = ldpc_params();
       p.tasks = frame_tasks.as<LdpcTask>() + frame_task0[f];
@@ -191,6 +222,7 @@ LIZ_API int lizx_batch_plan(lizx_batch *b, int frames, const int *version, const
       g->view.llr = b->llr.as<int8_t>() + b->llr_off[g->frames[0]];
       g->view.est = b->est.as<double>() + b->est_off[g->frames[0]];
     }
+    if (b->groups.size() > 1) b->ensure_streams(static_cast<int>(b->groups.size()));
     b->wframes.reserve(frames * sizeof(WarpFrame));
     b->grids.reserve(frames * sizeof(float *));
     // the LDPC tasks: all blocks sorted by code (1/2 first) for the batch, and per frame for the frame modes

@@ -226,27 +226,49 @@ __global__ __launch_bounds__(256) void k_warp(PlanDev P, const WarpFrame *frames
 // The shading fit (lizard_phy.c:1605-1624): rhs over every eighth row and column outside the squares, beta =
 // hole_inv rhs.
 template <class Src>
-__global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *beta_out) {
-  __shared__ double red[256];
-  const int tid = threadIdx.x, f = blockIdx.x, n = P.n, c = n / 8;
+__global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *part_out) {
+  __shared__ double red[4][6];
+  const int tid = threadIdx.x, b = blockIdx.x, f = blockIdx.y, n = P.n, c = n / 8;
   double rhs[6] = {0, 0, 0, 0, 0, 0};
-  for (int k = tid; k < c * c; k += 256) {
-    const int y = 8 * (k / c), x = 8 * (k % c);
+  // this part's FIT_ROWS rows of the fit lattice
+  for (int k = tid; k < FIT_ROWS * c; k += 256) {
+    const int y = 8 * (b * FIT_ROWS + k / c), x = 8 * (k % c);
     if (in_square(P, y) && in_square(P, x)) continue;
     const double v = src(P, f, x, y);
-    double b[6];
-    hole_basis(P, x, y, b);
+    double bs[6];
+    hole_basis(P, x, y, bs);
 #pragma unroll
-    for (int r = 0; r < 6; r++) rhs[r] += b[r] * v;
+    for (int r = 0; r < 6; r++) rhs[r] += bs[r] * v;
   }
-  double tot[6];
+  // a fixed tree: the wave's lanes by shuffles, then the four waves in order
 #pragma unroll
-  for (int r = 0; r < 6; r++) tot[r] = block_sum<256>(rhs[r], red, tid);
+  for (int r = 0; r < 6; r++) {
+    for (int o = 32; o > 0; o >>= 1) rhs[r] += __shfl_xor(rhs[r], o, 64);
+  }
+  if ((tid & 63) == 0) {
+#pragma unroll
+    for (int r = 0; r < 6; r++) red[tid >> 6][r] = rhs[r];
+  }
+  __syncthreads();
+  if (tid < 6) part_out[static_cast<size_t>(f) * FIT_STRIDE + b * 6 + tid] =
+      (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
+}
+
+// The shading beta = hole_inv rhs (lizard_phy.c:1618-1624) from the fit's parts, summed in part order.
+__device__ __forceinline__ void fit_beta(const PlanDev &P, const double *part, double *rhs_sh, double *bsh, int tid) {
+  if (tid < 6) {
+    const int parts = P.n / 8 / FIT_ROWS;
+    double s = 0;
+    for (int b = 0; b < parts; b++) s += part[b * 6 + tid];
+    rhs_sh[tid] = s;
+  }
+  __syncthreads();
   if (tid < 6) {
     double s = 0;
-    for (int k = 0; k < 6; k++) s += P.hole_inv[tid * 6 + k] * tot[k];
-    beta_out[f * 6 + tid] = s;
+    for (int k = 0; k < 6; k++) s += P.hole_inv[tid * 6 + k] * rhs_sh[k];
+    bsh[tid] = s;
   }
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------------------ the two passes
@@ -255,10 +277,9 @@ constexpr int ROWS = 8;  // picture rows a k_rows workgroup transforms (four pac
 template <class Src>
 __global__ __launch_bounds__(256) void k_rows(PlanDev P, Src src, const double *beta, float2 *X, double *rowsum) {
   extern __shared__ float2 zs[];  // ROWS / 2 sequences of n
-  __shared__ double bsh[6];
+  __shared__ double bsh[6], rhs_sh[6];
   const int tid = threadIdx.x, f = blockIdx.y, y0 = blockIdx.x * ROWS, n = P.n;
-  if (tid < 6) bsh[tid] = beta[f * 6 + tid];
-  __syncthreads();
+  fit_beta(P, beta + static_cast<size_t>(f) * FIT_STRIDE, rhs_sh, bsh, tid);
   // load: 32 lanes a row, erase the anchors' samples, row sums for the detrend (s, s c(x), s q(x))
   {
     const int row = tid >> 5, l = tid & 31, y = y0 + row;
@@ -717,7 +738,7 @@ void FrameBufs::reserve(const FramePlan &p, int frames, bool debug) {
   X.reserve(frames * E * n * sizeof(float2));
   HP.reserve(frames * E * 2 * E * sizeof(float2));
   rowsum.reserve(frames * n * 3 * sizeof(double));
-  beta.reserve(frames * 6 * sizeof(double));
+  beta.reserve(frames * FIT_STRIDE * sizeof(double));
   proj.reserve(frames * 5 * sizeof(double));
   align.reserve(frames * 2 * sizeof(float));
   llr.reserve(frames * 2 * S);
@@ -744,7 +765,7 @@ SoftBufs FrameBufs::view(bool debug) {
 SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
   const size_t n = p.n, E = p.E, S = static_cast<size_t>(p.subch) * SUB, F = f;
   SoftBufs v;
-  v.beta = beta + F * 6;
+  v.beta = beta + F * FIT_STRIDE;
   v.rowsum = rowsum + F * n * 3;
   v.proj = proj + F * 5;
   v.X = X + F * E * n;
@@ -760,7 +781,7 @@ SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
 template <class Src>
 static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s) {
   const int n = p.n, E = p.E;
-  hipLaunchKernelGGL(k_erase_fit<Src>, dim3(frames), dim3(256), 0, s, p.dev, src, b.beta);
+  hipLaunchKernelGGL(k_erase_fit<Src>, dim3(n / 8 / FIT_ROWS, frames), dim3(256), 0, s, p.dev, src, b.beta);
   hipLaunchKernelGGL(k_rows<Src>, dim3(n / ROWS, frames), dim3(256), (ROWS / 2) * n * sizeof(float2), s, p.dev, src,
                      b.beta, b.X, b.rowsum);
   hipLaunchKernelGGL(k_cols, dim3((E + 1) / 2, frames), dim3(256), 2 * n * sizeof(float2), s, p.dev, b.X, b.HP);

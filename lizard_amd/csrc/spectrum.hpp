@@ -19,6 +19,8 @@ constexpr int BORDER = 15;          // modules between the symbol's edge and the
 constexpr int ALIGN_TURNS = 4096;   // steps of the pilots' turn table
 constexpr int FFT_MAX_STAGES = 12;
 constexpr int MAX_NODES = 40;       // node lattice points a side (layout.h OB_MAX_NODES)
+constexpr int FIT_ROWS = 4;         // rows of the shading fit's lattice (every eighth sample) a fit workgroup sums
+constexpr int FIT_STRIDE = 6 * 1536 / 8 / FIT_ROWS;  // doubles of fit partial sums a frame (6 per part, n <= 1536)
 
 // One frame's registration and capture for the warp: H maps module coordinates to capture pixels (h8 = 1), nodes
 // the node lattice's displacements in pixels (ny rows of nx), img a tight uint8 luma capture of w x h.
@@ -65,7 +67,8 @@ std::unique_ptr<FramePlan> make_plan(int version, int span);
 
 // Raw device pointers to one run's work areas and outputs; frame f of a run sits at f times each per-frame size.
 struct SoftBufs {
-  double *beta = nullptr, *rowsum = nullptr, *proj = nullptr;
+  double *beta = nullptr;  // the shading fit's partial sums: FIT_STRIDE doubles a frame
+  double *rowsum = nullptr, *proj = nullptr;
   float2 *X = nullptr, *HP = nullptr;
   float *align = nullptr;
   int8_t *llr = nullptr;
