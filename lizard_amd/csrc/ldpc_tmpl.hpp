@@ -73,26 +73,41 @@ __host__ __device__ LZ_INLINE void put_msg(uint32_t *w, int v) {
   w[S >> 2] = (w[S >> 2] & ~(0xFFu << sh)) | ((static_cast<uint32_t>(v) & 0xFFu) << sh);
 }
 
+// Byte offset of posterior (i - sh) mod 128 of a 128-entry block from 2 i: one add whose low byte is the answer (the
+// compiler folds the mask into the add's byte select).
+__host__ __device__ LZ_INLINE int rot_off(int i2, int sh) { return (i2 + (256 - 2 * sh)) & 0xFF; }
+
+__host__ __device__ LZ_INLINE int16_t &at_byte(int16_t *base, int off) {
+  return *reinterpret_cast<int16_t *>(reinterpret_cast<char *>(base) + off);
+}
+__host__ __device__ LZ_INLINE int16_t at_byte(const int16_t *base, int off) {
+  return *reinterpret_cast<const int16_t *>(reinterpret_cast<const char *>(base) + off);
+}
+
 // Block row R of code CI for the lane's check position i.
 template <int CI, int R>
 __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   using S = Shape<CI>;
-  const int i = fresh(lane);
+  const int i = fresh(lane), i2 = 2 * i;
   constexpr int D0 = S::d0(R), ND = S::nd(R), NS = ND + 2, RB = S::rbase(R), K = S::K;
-  // the check's slots: its circulants' data bits, the parity bit before it, its own (ldpc.c:589-602)
+  static_assert(NS <= 32, "edge indices fit the key's five bits");
+  // the check's slots: its circulants' data bits, the parity bit before it, its own (ldpc.c:589-602); byte offsets
+  // into L
   int adr[NS];
   static_for<0, ND>([&](auto e) {
     constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
-    adr[e] = col * 128 + ((i - sh) & 127);
+    adr[e] = col * 256 + rot_off(i2, sh);
   });
-  adr[ND] = R > 0 ? K + (R - 1) * 128 + i : K + (S::MB - 1) * 128 + i - 1;
-  adr[ND + 1] = K + R * 128 + i;
+  adr[ND] = R > 0 ? 2 * (K + (R - 1) * 128) + i2 : 2 * (K + (S::MB - 1) * 128 - 1) + i2;
+  adr[ND + 1] = 2 * (K + R * 128) + i2;
   // check 0 (block row 0, position 0) has no parity bit before it: read as a certain 0 of 8191, message zeroed
   const bool absent = R == 0 && i == 0;
   int q[NS];
-  int min1 = 32767, min2 = 32767, arg = -1, sgn = 0;
+  // magnitude and edge in one key, |q| << 5 | e: the smallest key is the first smallest magnitude (the reference's
+  // strict "<" keeps the first of equal minima), the second smallest is the next magnitude up or an equal later one
+  int key1 = 0x7FFFFFFF, key2 = 0x7FFFFFFF, sgn = 0;
   static_for<0, NS>([&](auto e) {
-    int post = L[adr[e]];
+    int post = at_byte(L, adr[e]);
     int msg = get_msg<RB + e>(M.m);
     if (e == ND && absent) {
       post = 8191;
@@ -102,12 +117,13 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
     q[e] = v;
     const int a = v < 0 ? -v : v;
     sgn ^= v;
-    const bool lt = a < min1;
-    min2 = lt ? min1 : (a < min2 ? a : min2);
-    min1 = lt ? a : min1;
-    arg = lt ? static_cast<int>(e) : arg;
+    const int key = (a << 5) | static_cast<int>(e);
+    const int lo = key1 < key ? key1 : key, hi = key1 < key ? key : key1;
+    key2 = hi < key2 ? hi : key2;  // the middle of key1 <= key2 and key
+    key1 = lo;
   });
-  int m1 = (min1 * 13) >> 4, m2 = (min2 * 13) >> 4;
+  const int arg = key1 & 31;
+  int m1 = ((key1 >> 5) * 13) >> 4, m2 = ((key2 >> 5) * 13) >> 4;
   m1 = m1 > 127 ? 127 : m1;
   m2 = m2 > 127 ? 127 : m2;
   static_for<0, NS>([&](auto e) {
@@ -117,7 +133,7 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
     put_msg<RB + e>(M.m, msg);
     int t = q[e] + msg;
     t = t > 8191 ? 8191 : (t < -8191 ? -8191 : t);
-    if (!(e == ND && absent)) L[adr[e]] = static_cast<int16_t>(t);
+    if (!(e == ND && absent)) at_byte(L, adr[e]) = static_cast<int16_t>(t);
   });
 }
 
@@ -137,11 +153,11 @@ __host__ __device__ LZ_INLINE int unsat(const int16_t *L, int lane) {
   int bad = 0;
   static_for<0, S::MB>([&](auto r) {
     constexpr int R = r, D0 = S::d0(R), ND = S::nd(R), K = S::K;
-    const int i = fresh(lane);
+    const int i = fresh(lane), i2 = 2 * i;
     int x = L[K + R * 128 + i];
     static_for<0, ND>([&](auto e) {
       constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
-      x ^= L[col * 128 + ((i - sh) & 127)];
+      x ^= at_byte(L, col * 256 + rot_off(i2, sh));
     });
     if (R > 0) x ^= L[K + (R - 1) * 128 + i];
     else x ^= i > 0 ? L[K + (S::MB - 1) * 128 + i - 1] : 0;

@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 
 #include "spectrum.hpp"
 
@@ -61,55 +62,162 @@ __device__ __forceinline__ void dft(float2 *v) {
   }
 }
 
-// One Stockham stage of radix R over count sequences of length N laid one after another in LDS: every butterfly's
-// inputs read into registers, a barrier, the outputs written in their sorted places, a barrier (so one buffer does).
-template <int R, int NT, int COUNT>
-__device__ __forceinline__ void fft_stage(float2 *x, int N, int Ns, const float2 *tw, int tid) {
-  constexpr int MAXB = (COUNT * 1536 / R + NT - 1) / NT;
-  const int nb = N / R, total = nb * COUNT, step = N / (Ns * R);
-  float2 v[MAXB][R];
-#pragma unroll
-  for (int q = 0; q < MAXB; q++) {
-    const int g = tid + q * NT;
-    if (g < total) {
-      const int seq = g / nb, j = g - seq * nb, k = j % Ns;
-      const float2 *xs = x + seq * N;
-#pragma unroll
-      for (int r = 0; r < R; r++) {
-        float2 a = xs[j + r * nb];
-        if (r) a = cmul(a, tw[k * r * step]);
-        v[q][r] = a;
-      }
-      dft<R>(v[q]);
-    }
+// f(integral_constant<int, i>) for i in [B, E): loops whose indices must be constants (register arrays, twiddles)
+template <int B, int E, class F>
+__device__ __forceinline__ void sfor(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    sfor<B + 1, E>(f);
   }
-  __syncthreads();
-#pragma unroll
-  for (int q = 0; q < MAXB; q++) {
-    const int g = tid + q * NT;
-    if (g < total) {
-      const int seq = g / nb, j = g - seq * nb, k = j % Ns;
-      float2 *xs = x + seq * N;
-      const int base = (j / Ns) * Ns * R + k;
-#pragma unroll
-      for (int r = 0; r < R; r++) xs[base + r * Ns] = v[q][r];
-    }
-  }
-  __syncthreads();
 }
 
-// Forward DFT, exp(-2 pi i k n / N), unscaled, natural order in and out.
-template <int NT, int COUNT>
-__device__ void fft_lds(float2 *x, const PlanDev &P, int tid) {
-  int Ns = 1;
-  for (int s = 0; s < P.nstages; s++) {
-    switch (P.radix[s]) {
-      case 4: fft_stage<4, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
-      case 2: fft_stage<2, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
-      case 3: fft_stage<3, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
-      default: fft_stage<5, NT, COUNT>(x, P.n, Ns, P.tw, tid); break;
+// ---- compile-time twiddles: cos and sin of 2 pi e / m by their Taylor series in double (|angle| <= pi after the
+// reduction below, where 30 terms are exact to double precision), rounded to float once
+constexpr double kPi = 3.14159265358979323846;
+constexpr double taylor_sin(double x) {
+  double term = x, sum = x;
+  for (int k = 1; k < 30; k++) {
+    term *= -x * x / ((2.0 * k) * (2.0 * k + 1.0));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double taylor_cos(double x) {
+  double term = 1, sum = 1;
+  for (int k = 1; k < 30; k++) {
+    term *= -x * x / ((2.0 * k - 1.0) * (2.0 * k));
+    sum += term;
+  }
+  return sum;
+}
+// exp(-2 pi i e / m) as (cos, sin) pairs, the angle reduced to (-pi, pi] exactly in integers first
+constexpr double angle_of(int e, int m) {
+  const int r = ((e % m) + m) % m;
+  const int s = 2 * r > m ? r - m : r;  // r or r - m, whichever lies in (-m/2, m/2]
+  return -2.0 * kPi * s / m;
+}
+
+template <int M, int E>
+__device__ __forceinline__ float2 twiddle(float2 x) {
+  constexpr int r = ((E % M) + M) % M;
+  if constexpr (r == 0) {
+    return x;
+  } else if constexpr (4 * r == M) {  // -i
+    return make_float2(x.y, -x.x);
+  } else if constexpr (2 * r == M) {  // -1
+    return make_float2(-x.x, -x.y);
+  } else if constexpr (4 * r == 3 * M) {  // +i
+    return make_float2(-x.y, x.x);
+  } else {
+    constexpr float c = static_cast<float>(taylor_cos(angle_of(E, M)));
+    constexpr float sn = static_cast<float>(taylor_sin(angle_of(E, M)));
+    return make_float2(x.x * c - x.y * sn, x.x * sn + x.y * c);
+  }
+}
+
+constexpr int first_factor(int m) { return m % 4 == 0 ? 4 : m % 2 == 0 ? 2 : m % 3 == 0 ? 3 : 5; }
+
+// In-register DFT of M points, natural order in and out (Cooley-Tukey over M = P Q, P the first factor; every
+// index and twiddle a compile-time constant).
+template <int M>
+__device__ __forceinline__ void dft_small(float2 *v) {
+  if constexpr (M == 1) {
+    return;
+  } else if constexpr (M <= 5) {
+    dft<M>(v);
+  } else {
+    constexpr int P = first_factor(M), Q = M / P;
+    float2 t[M];
+    sfor<0, Q>([&](auto q) {
+      float2 a[P];
+      sfor<0, P>([&](auto p) { a[p] = v[q + Q * p]; });
+      dft_small<P>(a);
+      sfor<0, P>([&](auto k1) { t[k1 * Q + q] = twiddle<M, q * k1>(a[k1]); });
+    });
+    sfor<0, P>([&](auto k1) {
+      dft_small<Q>(t + k1 * Q);
+      sfor<0, Q>([&](auto k2) { v[k1 + P * k2] = t[k1 * Q + k2]; });
+    });
+  }
+}
+
+// A wave's LDS operations are ordered: wait for this lane's writes, then let no access move across.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One 64-lane wave transforms one complex sequence of N = 64 M points (unscaled forward DFT): lane l holds
+// x[l + 64 m], m < M, in registers; an M-point DFT in registers, the twiddles w_N^(l k1), then the 64-point DFTs
+// across lanes as two radix-8 passes through the wave's own LDS area S (64 MP points, MP odd so that a lane stride
+// of MP spreads over the banks). Z(k) ends at S[pos(k)]. No workgroup barrier: waves run independently.
+template <int N>
+struct WaveFft {
+  static constexpr int M = N / 64;
+  static constexpr int MP = M % 2 == 0 ? M + 1 : M;
+  static constexpr int LDS = 64 * MP;               // float2 a wave
+  static constexpr int TASKS = 8 * M;               // radix-8 butterflies a pass
+  static constexpr int ROUNDS = (TASKS + 63) / 64;
+  static_assert(N % 64 == 0 && M >= 1 && M <= 24, "picture sizes 64 M, M <= 24");
+
+  // k = k1 + M (c + 8 d) sits at position (d + 8 c) MP + k1
+  __device__ static __forceinline__ int pos(int k) {
+    const int k1 = k % M, k2 = k / M;
+    return ((k2 >> 3) + 8 * (k2 & 7)) * MP + k1;
+  }
+
+  __device__ static __forceinline__ void run(float2 (&v)[M], float2 *S, const float2 *tw, int lane) {
+    dft_small<M>(v);
+#pragma unroll
+    for (int k1 = 0; k1 < M; k1++) S[lane * MP + k1] = k1 ? cmul(v[k1], tw[lane * k1]) : v[k1];
+    wave_sync();
+    // 64 = 8 x 8 on each k1: n2 = a + 8 b. Pass one: 8 points over b, twiddle w_64^(a c), in place at a + 8 c.
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+      const int t = lane + 64 * r;
+      if (TASKS % 64 == 0 || t < TASKS) {
+        const int k1 = t % M, a = t / M;
+        float2 u[8];
+#pragma unroll
+        for (int b = 0; b < 8; b++) u[b] = S[(a + 8 * b) * MP + k1];
+        dft_small<8>(u);
+#pragma unroll
+        for (int c = 0; c < 8; c++) S[(a + 8 * c) * MP + k1] = c ? cmul(u[c], tw[a * c * M]) : u[c];
+      }
     }
-    Ns *= P.radix[s];
+    wave_sync();
+    // pass two: 8 points over a, output d in place at d + 8 c
+#pragma unroll
+    for (int r = 0; r < ROUNDS; r++) {
+      const int t = lane + 64 * r;
+      if (TASKS % 64 == 0 || t < TASKS) {
+        const int k1 = t % M, c = t / M;
+        float2 u[8];
+#pragma unroll
+        for (int a = 0; a < 8; a++) u[a] = S[(a + 8 * c) * MP + k1];
+        dft_small<8>(u);
+#pragma unroll
+        for (int d = 0; d < 8; d++) S[(d + 8 * c) * MP + k1] = u[d];
+      }
+    }
+    wave_sync();
+  }
+};
+
+// Calls f with std::integral_constant<int, n> for every picture size LIZARD uses (64 M, M in 6, 8, 12, 16, 18, 20, 24).
+template <class F>
+void with_size(int n, F &&f) {
+  switch (n) {
+    case 384: f(std::integral_constant<int, 384>{}); break;
+    case 512: f(std::integral_constant<int, 512>{}); break;
+    case 768: f(std::integral_constant<int, 768>{}); break;
+    case 1024: f(std::integral_constant<int, 1024>{}); break;
+    case 1152: f(std::integral_constant<int, 1152>{}); break;
+    case 1280: f(std::integral_constant<int, 1280>{}); break;
+    case 1536: f(std::integral_constant<int, 1536>{}); break;
+    default: throw Error(LIZ_E_FORMAT, "a picture size outside 384, 512, 768, 1024, 1152, 1280, 1536");
   }
 }
 
@@ -272,70 +380,84 @@ __device__ __forceinline__ void fit_beta(const PlanDev &P, const double *part, d
 }
 
 // ------------------------------------------------------------------------------------------------ the two passes
-constexpr int ROWS = 8;  // picture rows a k_rows workgroup transforms (four packed complex sequences)
+constexpr int ROWS = 8;  // picture rows a k_rows workgroup transforms: two a wave, packed as one complex sequence
 
-template <class Src>
-__global__ __launch_bounds__(256) void k_rows(PlanDev P, Src src, const double *beta, float2 *X, double *rowsum) {
-  extern __shared__ float2 zs[];  // ROWS / 2 sequences of n
+// Rows y and y + 1 of wave w: load with the anchors' samples erased, the detrend's row sums (s, s c(x), s q(x)) in
+// double, one wave FFT of row y + i row y + 1, the two real rows' transforms separated (Z(u) and conj Z(n - u)) for
+// u < E and written column-major, X[frame][u][y].
+template <class Src, int N>
+__global__ __launch_bounds__(256) void k_rows(PlanDev P, Src src, const double *fitpart, float2 *X, double *rowsum) {
+  using F = WaveFft<N>;
+  constexpr int M = F::M;
+  extern __shared__ float2 zs[];  // 4 waves x F::LDS
   __shared__ double bsh[6], rhs_sh[6];
-  const int tid = threadIdx.x, f = blockIdx.y, y0 = blockIdx.x * ROWS, n = P.n;
-  fit_beta(P, beta + static_cast<size_t>(f) * FIT_STRIDE, rhs_sh, bsh, tid);
-  // load: 32 lanes a row, erase the anchors' samples, row sums for the detrend (s, s c(x), s q(x))
-  {
-    const int row = tid >> 5, l = tid & 31, y = y0 + row;
-    const bool yin = in_square(P, y);
-    float *zf = reinterpret_cast<float *>(zs + (row >> 1) * n) + (row & 1);
-    double rs = 0, rx = 0, rq = 0;
-    for (int x = l; x < n; x += 32) {
-      float s = src(P, f, x, y);
-      if (yin && in_square(P, x)) s = hole_value(P, bsh, x, y);
-      zf[2 * x] = s;
-      rs += s;
-      rx += static_cast<double>(s) * P.cx[x];
-      rq += static_cast<double>(s) * P.qx[x];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y;
+  const int y = blockIdx.x * ROWS + 2 * w;
+  fit_beta(P, fitpart + static_cast<size_t>(f) * FIT_STRIDE, rhs_sh, bsh, tid);
+  const bool yin0 = in_square(P, y), yin1 = in_square(P, y + 1);
+  float2 v[M];
+  double s0 = 0, x0 = 0, q0 = 0, s1 = 0, x1 = 0, q1 = 0;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int x = lane + 64 * m;
+    float a = src(P, f, x, y), b = src(P, f, x, y + 1);
+    if ((yin0 || yin1) && in_square(P, x)) {
+      if (yin0) a = hole_value(P, bsh, x, y);
+      if (yin1) b = hole_value(P, bsh, x, y + 1);
     }
-    for (int o = 16; o > 0; o >>= 1) {
-      rs += __shfl_xor(rs, o, 32);
-      rx += __shfl_xor(rx, o, 32);
-      rq += __shfl_xor(rq, o, 32);
-    }
-    if (l == 0) {
-      double *o = rowsum + (static_cast<size_t>(f) * n + y) * 3;
-      o[0] = rs;
-      o[1] = rx;
-      o[2] = rq;
-    }
+    v[m] = make_float2(a, b);
+    const double cx = P.cx[x], qx = P.qx[x];
+    s0 += a;
+    x0 += a * cx;
+    q0 += a * qx;
+    s1 += b;
+    x1 += b * cx;
+    q1 += b * qx;
   }
-  __syncthreads();
-  fft_lds<256, ROWS / 2>(zs, P, tid);
-  // unpack the two real rows of each sequence (Z(u) and conj Z(n - u)) for u < E; column-major out
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_xor(s0, o, 64);
+    x0 += __shfl_xor(x0, o, 64);
+    q0 += __shfl_xor(q0, o, 64);
+    s1 += __shfl_xor(s1, o, 64);
+    x1 += __shfl_xor(x1, o, 64);
+    q1 += __shfl_xor(q1, o, 64);
+  }
+  if (lane == 0) {
+    double *o = rowsum + (static_cast<size_t>(f) * N + y) * 3;
+    o[0] = s0;
+    o[1] = x0;
+    o[2] = q0;
+    o[3] = s1;
+    o[4] = x1;
+    o[5] = q1;
+  }
+  float2 *S = zs + w * F::LDS;
+  F::run(v, S, P.tw, lane);
   const int E = P.E;
-  for (int idx = tid; idx < E * ROWS; idx += 256) {
-    const int u = idx / ROWS, row = idx % ROWS;
-    const float2 zu = zs[(row >> 1) * n + u], zm = zs[(row >> 1) * n + (u ? n - u : 0)];
-    const float2 o = (row & 1) ? make_float2(0.5f * (zu.y + zm.y), -0.5f * (zu.x - zm.x))
-                               : make_float2(0.5f * (zu.x + zm.x), 0.5f * (zu.y - zm.y));
-    X[(static_cast<size_t>(f) * E + u) * n + y0 + row] = o;
+  for (int u = lane; u < E; u += 64) {
+    const float2 zu = S[F::pos(u)], zm = S[F::pos(u ? N - u : 0)];
+    const float4 o = make_float4(0.5f * (zu.x + zm.x), 0.5f * (zu.y - zm.y), 0.5f * (zu.y + zm.y), -0.5f * (zu.x - zm.x));
+    *reinterpret_cast<float4 *>(X + (static_cast<size_t>(f) * E + u) * N + y) = o;
   }
 }
 
-// Column u: Y(u, v) for v < E at [v], and Y(u, n - v) for 1 <= v < E at [E - 1 + v].
+// Column u (one wave): Y(u, v) for v < E at [v], and Y(u, n - v) for 1 <= v < E at [E - 1 + v].
+template <int N>
 __global__ __launch_bounds__(256) void k_cols(PlanDev P, const float2 *X, float2 *HP) {
-  extern __shared__ float2 zs[];  // 2 sequences of n
-  const int tid = threadIdx.x, f = blockIdx.y, u0 = 2 * blockIdx.x, n = P.n, E = P.E;
-  for (int idx = tid; idx < 2 * n; idx += 256) {
-    const int s = idx / n, y = idx % n, u = u0 + s;
-    zs[idx] = u < E ? X[(static_cast<size_t>(f) * E + u) * n + y] : make_float2(0.f, 0.f);
-  }
-  __syncthreads();
-  fft_lds<256, 2>(zs, P, tid);
-  const int W = 2 * E;
-  for (int idx = tid; idx < 2 * W; idx += 256) {
-    const int s = idx / W, k = idx % W, u = u0 + s;
-    if (u >= E || k >= W - 1) continue;
-    const int v = k < E ? k : n - (k - E + 1);
-    HP[(static_cast<size_t>(f) * E + u) * W + k] = zs[s * n + v];
-  }
+  using F = WaveFft<N>;
+  constexpr int M = F::M;
+  extern __shared__ float2 zs[];  // 4 waves x F::LDS
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E, W = 2 * E;
+  const int u = blockIdx.x * 4 + w;
+  if (u >= E) return;  // the whole wave; nothing below waits on the workgroup
+  const float2 *col = X + (static_cast<size_t>(f) * E + u) * N;
+  float2 v[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) v[m] = col[lane + 64 * m];
+  float2 *S = zs + w * F::LDS;
+  F::run(v, S, P.tw, lane);
+  float2 *out = HP + (static_cast<size_t>(f) * E + u) * W;
+  for (int k = lane; k < W - 1; k += 64) out[k] = S[F::pos(k < E ? k : N - (k - E + 1))];
 }
 
 // ------------------------------------------------------------------------------------------------ the coefficients
@@ -528,29 +650,6 @@ __global__ __launch_bounds__(320) void k_demap(PlanDev P, const float2 *HP_all, 
 }
 
 // ------------------------------------------------------------------------------------------------ host: plans
-static std::vector<int> radices(int n) {
-  std::vector<int> r;
-  int m = n;
-  while (m % 4 == 0) {
-    r.push_back(4);
-    m /= 4;
-  }
-  if (m % 2 == 0) {
-    r.push_back(2);
-    m /= 2;
-  }
-  while (m % 3 == 0) {
-    r.push_back(3);
-    m /= 3;
-  }
-  while (m % 5 == 0) {
-    r.push_back(5);
-    m /= 5;
-  }
-  if (m != 1) throw Error(LIZ_E_FORMAT, "picture size with a factor other than 2, 3 and 5");
-  return r;
-}
-
 // 6 x 6 inverse by Gauss-Jordan with partial pivots.
 static void invert6(double g[6][12]) {
   for (int c = 0; c < 6; c++) {
@@ -684,9 +783,7 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
       cell[t] = i;
     }
   }
-  p->radix = radices(n);
-  d.nstages = static_cast<int>(p->radix.size());
-  for (int s = 0; s < d.nstages; s++) d.radix[s] = p->radix[s];
+  with_size(n, [](auto) {});  // a size the transforms are built for, or LIZ_E_FORMAT
   std::vector<float2> tw(n), rot(ALIGN_TURNS);
   for (int k = 0; k < n; k++) {
     const double th = -2 * 3.14159265358979323846 * k / n;
@@ -782,9 +879,13 @@ template <class Src>
 static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s) {
   const int n = p.n, E = p.E;
   hipLaunchKernelGGL(k_erase_fit<Src>, dim3(n / 8 / FIT_ROWS, frames), dim3(256), 0, s, p.dev, src, b.beta);
-  hipLaunchKernelGGL(k_rows<Src>, dim3(n / ROWS, frames), dim3(256), (ROWS / 2) * n * sizeof(float2), s, p.dev, src,
-                     b.beta, b.X, b.rowsum);
-  hipLaunchKernelGGL(k_cols, dim3((E + 1) / 2, frames), dim3(256), 2 * n * sizeof(float2), s, p.dev, b.X, b.HP);
+  with_size(n, [&](auto N) {
+    constexpr int NN = decltype(N)::value;
+    const size_t lds = 4 * WaveFft<NN>::LDS * sizeof(float2);
+    hipLaunchKernelGGL((k_rows<Src, NN>), dim3(NN / ROWS, frames), dim3(256), lds, s, p.dev, src, b.beta, b.X,
+                       b.rowsum);
+    hipLaunchKernelGGL(k_cols<NN>, dim3((E + 3) / 4, frames), dim3(256), lds, s, p.dev, b.X, b.HP);
+  });
   hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.HP, b.proj, b.align);
   hipLaunchKernelGGL(k_demap, dim3(p.subch, frames), dim3(320), 0, s, p.dev, b.HP, b.proj, b.align, b.llr, b.est,
                      b.rms, b.coef_dbg);

@@ -17,7 +17,6 @@ constexpr int ANCH_K = 7;           // anchors per axis (SPEC 6.13)
 constexpr int ANCH_MARGIN = 2;      // samples erased round a square
 constexpr int BORDER = 15;          // modules between the symbol's edge and the picture (mark + guard)
 constexpr int ALIGN_TURNS = 4096;   // steps of the pilots' turn table
-constexpr int FFT_MAX_STAGES = 12;
 constexpr int MAX_NODES = 40;       // node lattice points a side (layout.h OB_MAX_NODES)
 constexpr int FIT_ROWS = 4;         // rows of the shading fit's lattice (every eighth sample) a fit workgroup sums
 constexpr int FIT_STRIDE = 6 * 1536 / 8 / FIT_ROWS;  // doubles of fit partial sums a frame (6 per part, n <= 1536)
@@ -33,8 +32,7 @@ struct WarpFrame {
 
 // Device-side view of a plan (by value into kernels).
 struct PlanDev {
-  int n, subch, E, blocks, npilots, nstages;
-  int radix[FFT_MAX_STAGES];
+  int n, subch, E, blocks, npilots;
   int elo[ANCH_K], ehi[ANCH_K];
   double hole_inv[36];
   double sx2, sq2;
@@ -57,7 +55,6 @@ struct FramePlan {
   const Format *fmt = nullptr;
   PlanDev dev{};
   DevBuf tables, coef, pilot, tw, rot, white, mtab;
-  std::vector<int> radix;
   std::vector<int> pilot_host;  // coefficient indices of the pilots, in block order
   std::vector<int> pilot_block;
 };

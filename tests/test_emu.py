@@ -37,3 +37,19 @@ def test_lanes_match_oracle(emu, rate):
             got = emu.emu_ldpc_decode(rate, ptr(llr), ptr(out), mi, st, 0.95, ptr(post), ptr(msg))
             assert got == its
             assert np.array_equal(out, bits) and np.array_equal(post, L) and np.array_equal(msg, R)
+
+
+@pytest.mark.parametrize("rate", RATES)
+def test_lanes_match_oracle_on_ties(emu, rate):
+    """LLRs of two magnitudes only: most checks see equal minima, so the first-minimum rule decides every message."""
+    c = code(rate)
+    rng = np.random.default_rng(90 + rate)
+    for w in range(6):
+        llr = (rng.choice([-16, -8, 8, 16], c.n, p=[0.1, 0.15, 0.45, 0.3])).astype(np.int8)
+        its, bits, L, R = c.decode(llr, 12, 0, 0.95, trace=True)
+        out = np.zeros(c.n, np.uint8)
+        post = np.zeros(c.n, np.int16)
+        msg = np.zeros(c.slots_total * 128, np.int8)
+        got = emu.emu_ldpc_decode(rate, ptr(llr), ptr(out), 12, 0, 0.95, ptr(post), ptr(msg))
+        assert got == its
+        assert np.array_equal(out, bits) and np.array_equal(post, L) and np.array_equal(msg, R)
