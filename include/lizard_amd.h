@@ -94,6 +94,9 @@ LIZ_API int lizx_format_info(int version, int *info);
    whitening, the pilots' tails, the padding), host: blocks * 476 bytes in, subch * 640 frame slots (0/1) out, cnt the
    painted pictures' count mod 4. Returns 0 or a negative LIZ_E_*. */
 LIZ_API int lizx_frame_slots(int version, const uint8_t *blocks, int cnt, uint8_t *slots);
+/* Block b of LIZARD-16V: its code's rate index, its first frame slot and its slot count (liblizard/src/lizard_phy.h
+   block_tier / block_slot, lizard_phy.c:485-637); returns the block count. */
+LIZ_API int lizx_format_blocks(int version, int *rate, int *slot0, int *nslots);
 /* The coefficient order (research/SPEC.md 6.3) of LIZARD-16V: u and v of each of its subch * 320 coefficients. */
 LIZ_API int lizx_format_coefs(int version, int16_t *u, int16_t *v);
 
@@ -189,6 +192,28 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
                                     float stall_ratio, int flags, uint8_t *d_bytes, uint8_t *d_crc_ok);
 LIZ_API int lizx_ldpc_sync(lizx_ldpc *h);
 LIZ_API float lizx_ldpc_last_ms(lizx_ldpc *h);
+
+/* ---- Registration refinement (no counterpart in the reference, whose registration is estimated on the CPU;
+   liblizard/src/acquire.c:1470-1512 lens model, :2196-2352 anchors' mesh). A registration's parameters, params[107]:
+   h0..h7 of the homography (module coordinates -> capture pixels, h8 = 1), k1 of a radial term k1 |q|^2 q D about the
+   frame's centre (q the offset over the farthest corner's distance D), and the 7 x 7 anchors' (dx, dy) pixels, row
+   after row. The node lattice's displacements are k1 times the lens field at each node's place under a reference
+   homography H0 (9 floats) plus the anchors' cubic Hermite mesh (interior nodes; 0 at the lattice's ends). */
+#define LIZX_REG_PARAMS 107
+/* The nodes (ny * nx (dx, dy) pairs, host) of these parameters in a w x h frame; returns nx. */
+LIZ_API int lizx_reg_nodes(int span, const float *H0, int w, int h, const float *params, float *nodes);
+/* The refinement loss sum (w - alpha r - beta)^2 over the picture's samples, w the capture warped through params, r
+   the reference picture (n x n floats, row-major, host: the blurred re-modulation of what is known), and its gradient
+   with respect to all LIZX_REG_PARAMS parameters (the hand-written adjoint of the warp: per-sample sums on the GPU,
+   the chain through the nodes on the host in FP64). */
+LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                               const float *H0, const float *params, const float *ref, float alpha, float beta,
+                               double *loss, double *grad);
+/* steps Gauss-Newton steps over h0..h7 on the GPU (k1 and the anchors held; alpha and beta refitted each pass; the
+   damped 8 x 8 normal equations solved in FP64 on the device); params' h0..h7 updated in place; trace (optional,
+   steps + 1 floats) receives the loss before each step and after the last. */
+LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                            const float *H0, float *params, const float *ref, int steps, float *trace);
 
 /* ---- Batches: many frames of mixed formats through one launch sequence (liblizard/include/lizard.h:285-287
    liz_receiver_batch_cap, liblizard/core/rx/receiver.h:77-95 Receiver::push: the reference batches whole frames per

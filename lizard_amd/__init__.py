@@ -84,6 +84,10 @@ def lib():
     sig("lizx_node_lattice", _I, _I, _P)
     sig("lizx_frame_slots", _I, _I, _P, _I, _P)
     sig("lizx_format_coefs", _I, _I, _P, _P)
+    sig("lizx_format_blocks", _I, _I, _P, _P, _P)
+    sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
+    sig("lizx_reg_loss_grad", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _F, _F, _P, _P)
+    sig("lizx_reg_refine", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_warp", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P)
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
@@ -155,6 +159,35 @@ def format_coefs(version):
     u, v = np.zeros(S, np.int16), np.zeros(S, np.int16)
     check(lib().lizx_format_coefs(version, _p(u), _p(v)))
     return u.astype(np.int64), v.astype(np.int64)
+
+
+def format_blocks(version):
+    B = format_info(version)["blocks"]
+    rate, s0, ns = np.zeros(B, np.int32), np.zeros(B, np.int32), np.zeros(B, np.int32)
+    check(lib().lizx_format_blocks(version, _p(rate), _p(s0), _p(ns)))
+    return rate, s0, ns
+
+
+REG_PARAMS = 107
+
+
+def reg_params(H, k1=0.0, anchors=None):
+    """The 107 registration parameters: h0..h7 of H (normalised to h8 = 1), k1, the 7 x 7 anchors' (dx, dy)."""
+    H = np.asarray(H, np.float64).reshape(3, 3)
+    p = np.zeros(REG_PARAMS, np.float32)
+    p[:8] = (H / H[2, 2]).ravel()[:8]
+    p[8] = k1
+    if anchors is not None:
+        p[9:] = np.asarray(anchors, np.float32).ravel()
+    return p
+
+
+def reg_nodes(span, H0, w, h, params):
+    nx = len(node_lattice(span))
+    out = np.zeros((nx, nx, 2), np.float32)
+    check(lib().lizx_reg_nodes(span, _p(np.ascontiguousarray(H0, np.float32).ravel()), w, h,
+                               _p(np.ascontiguousarray(params, np.float32)), _p(out)))
+    return out
 
 
 def node_lattice(span):
@@ -360,6 +393,27 @@ class Decoder:
         if soft:
             out["align"] = (sp.align[0], sp.align[1])
         return out
+
+    def reg_loss_grad(self, version, span, img, H0, params, ref, alpha=1.0, beta=0.0):
+        """Refinement loss and its gradient over the 107 registration parameters (lizx_reg_loss_grad)."""
+        img = np.ascontiguousarray(img, np.uint8)
+        loss, grad = C.c_double(0), np.zeros(REG_PARAMS, np.float64)
+        check(lib().lizx_reg_loss_grad(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
+                                       _p(np.ascontiguousarray(H0, np.float32).ravel()),
+                                       _p(np.ascontiguousarray(params, np.float32)),
+                                       _p(np.ascontiguousarray(ref, np.float32)), alpha, beta, C.byref(loss),
+                                       _p(grad)))
+        return loss.value, grad
+
+    def reg_refine(self, version, span, img, H0, params, ref, steps=10):
+        """Gauss-Newton refinement of h0..h7 (lizx_reg_refine): the new parameters and the loss trace."""
+        img = np.ascontiguousarray(img, np.uint8)
+        p = np.array(params, np.float32)
+        trace = np.zeros(steps + 1, np.float32)
+        check(lib().lizx_reg_refine(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
+                                    _p(np.ascontiguousarray(H0, np.float32).ravel()), _p(p),
+                                    _p(np.ascontiguousarray(ref, np.float32)), steps, _p(trace)))
+        return p, trace
 
     def finish_grid(self, version, span, grid, soft=True):
         """lizard_finish with a supplied n x n grid (row y, column x) of LIZARD-16V in the ring of this span."""

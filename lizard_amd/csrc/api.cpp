@@ -9,6 +9,7 @@
 #include "common.hpp"
 #include "format.hpp"
 #include "ldpc.hpp"
+#include "register.hpp"
 #include "spectrum.hpp"
 
 #define LIZARD_AMD_VERSION "lizard-amd 0.1 (gfx950)"
@@ -72,6 +73,10 @@ struct liz_decoder {
   DevBuf img, nodes, wframe;
   WarpFrame wframe_host{};
   std::vector<uint8_t> luma;
+  // registration refinement
+  DevBuf reg_ref, reg_part, reg_state, reg_trace;
+  std::vector<float> reg_nodes, reg_part_host;
+  RegState reg_state_host{};
   // the held registration for liz_decode
   int held_version = 0, held_span = 0;
   lizx_registration held_reg{};
@@ -261,6 +266,20 @@ LIZ_API int lizx_frame_slots(int version, const uint8_t *blocks, int cnt, uint8_
       }
     }
     return 0;
+  });
+}
+
+LIZ_API int lizx_format_blocks(int version, int *rate, int *slot0, int *nslots) {
+  return guard([&] {
+    need(rate && slot0 && nslots, "outputs");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    for (int b = 0; b < f->blocks; b++) {
+      rate[b] = code(f->blk_code[b]).rate;
+      slot0[b] = f->blk_slot[b];
+      nslots[b] = f->blk_nslots[b];
+    }
+    return f->blocks;
   });
 }
 
@@ -640,6 +659,116 @@ LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_
     d->held_reg.nodes = d->held_nodes.data();
     d->held_version = version;
     d->held_span = span;
+    return 0;
+  });
+}
+
+// The registration (H from params with h8 = 1, nodes from k1 and the anchors) of a refinement call, staged with its
+// capture and reference picture.
+static const WarpFrame *stage_refinement(liz_decoder *d, const FramePlan &p, const uint8_t *img, int w, int h, int stride,
+                                         const float *H0, const float *params, const float *ref, RegModel &model) {
+  need(H0 && params && ref, "a reference homography, parameters and a reference picture");
+  reg_model(p.span, H0, w, h, model);
+  reg_nodes(model, params, d->reg_nodes);
+  lizx_registration reg{};
+  for (int k = 0; k < 8; k++) reg.H[k] = params[k];
+  reg.H[8] = 1.0f;
+  reg.nodes = d->reg_nodes.data();
+  const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, &reg);
+  LdpcEngine &e = *d->eng;
+  const size_t nn = static_cast<size_t>(p.n) * p.n, cells = static_cast<size_t>(p.dev.nx - 1) * (p.dev.nx - 1);
+  d->reg_ref.reserve(nn * sizeof(float));
+  hip_check(hipMemcpyAsync(d->reg_ref.p, ref, nn * sizeof(float), hipMemcpyHostToDevice, e.stream), "reference picture");
+  d->reg_part.reserve(cells * REG_PART * sizeof(float));
+  d->reg_state.reserve(sizeof(RegState));
+  return wf;
+}
+
+LIZ_API int lizx_reg_nodes(int span, const float *H0, int w, int h, const float *params, float *nodes) {
+  return guard([&] {
+    need(H0 && params && nodes && w > 0 && h > 0, "a reference homography, parameters, a frame size and an output");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    RegModel m;
+    reg_model(span, H0, w, h, m);
+    std::vector<float> out;
+    reg_nodes(m, params, out);
+    std::copy(out.begin(), out.end(), nodes);
+    return m.nx;
+  });
+}
+
+LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                               const float *H0, const float *params, const float *ref, float alpha, float beta,
+                               double *loss, double *grad) {
+  return guard([&] {
+    need(d && loss && grad, "a decoder and outputs");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    RegModel model;
+    const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
+    RegState &st = d->reg_state_host;
+    for (int k = 0; k < 8; k++) st.H[k] = params[k];
+    st.H[8] = 1.0f;
+    st.alpha = alpha;
+    st.beta = beta;
+    hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
+    launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
+    const int nx = p.dev.nx, c = nx - 1;
+    d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_PART);
+    hip_check(hipMemcpyAsync(d->reg_part_host.data(), d->reg_part.p, d->reg_part_host.size() * sizeof(float),
+                             hipMemcpyDeviceToHost, e.stream), "partial sums");
+    hip_check(hipStreamSynchronize(e.stream), "refinement pass");
+    // the cells' sums in FP64, in cell order; the nodes' gradient from the corners of the cells about them
+    double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    std::vector<double> G(static_cast<size_t>(nx) * nx * 2, 0.0);
+    for (int cj = 0; cj < c; cj++)
+      for (int ci = 0; ci < c; ci++) {
+        const float *q = &d->reg_part_host[(static_cast<size_t>(cj) * c + ci) * REG_PART];
+        L += q[0];
+        for (int k = 0; k < 8; k++) gH[k] += q[1 + k];
+        const int corner[4] = {cj * nx + ci, cj * nx + ci + 1, (cj + 1) * nx + ci, (cj + 1) * nx + ci + 1};
+        for (int k = 0; k < 4; k++) {
+          G[2 * corner[k]] += 2.0 * q[50 + 2 * k];
+          G[2 * corner[k] + 1] += 2.0 * q[51 + 2 * k];
+        }
+      }
+    *loss = L;
+    for (int k = 0; k < 8; k++) grad[k] = 2.0 * gH[k];
+    reg_chain(model, G.data(), grad);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                            const float *H0, float *params, const float *ref, int steps, float *trace) {
+  return guard([&] {
+    need(d && steps >= 0 && steps <= 100, "a decoder and 0 to 100 steps");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    RegModel model;
+    const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
+    RegState &st = d->reg_state_host;
+    for (int k = 0; k < 8; k++) st.H[k] = params[k];
+    st.H[8] = 1.0f;
+    st.alpha = 1.0f;
+    st.beta = 0.0f;
+    hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
+    d->reg_trace.reserve((steps + 2) * sizeof(float));
+    // pass 0 fits the intensity model only; passes 1 .. steps move H; the last pass measures the result
+    for (int s = 0; s <= steps + 1; s++) {
+      launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
+      launch_gn_step(p, d->reg_part.as<float>(), d->reg_state.as<RegState>(), 1e-6f, s >= 1 && s <= steps,
+                     d->reg_trace.as<float>(), s, e.stream);
+    }
+    hip_check(hipMemcpyAsync(&st, d->reg_state.p, sizeof st, hipMemcpyDeviceToHost, e.stream), "state");
+    std::vector<float> tr(steps + 2);
+    hip_check(hipMemcpyAsync(tr.data(), d->reg_trace.p, tr.size() * sizeof(float), hipMemcpyDeviceToHost, e.stream),
+              "trace");
+    hip_check(hipStreamSynchronize(e.stream), "refinement");
+    for (int k = 0; k < 8; k++) params[k] = st.H[k];
+    if (trace) std::copy(tr.begin() + 1, tr.end(), trace);  // the loss before each step and after the last
     return 0;
   });
 }

@@ -1,0 +1,48 @@
+// The registration model beside the held {H, nodes} form: a radial lens term and the 7 x 7 anchors' mesh give the
+// node lattice's displacements, and its refinement on the GPU (the hand-written adjoint of the warp and the loss, and
+// Gauss-Newton over the homography).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "spectrum.hpp"
+
+namespace lz {
+
+constexpr int REG_ANCH = 7;                                 // anchors a side (SPEC 6.13)
+constexpr int REG_PARAMS = 8 + 1 + 2 * REG_ANCH * REG_ANCH; // homography, k1, anchors' (dx, dy)
+constexpr int REG_PART = 58;                                // floats of partial sums a node cell
+
+// The node lattice's displacements as a linear function of k1 and the anchors for one ring, frame size and reference
+// homography H0: nodes = k1 lens + sum over anchors of (dx, dy) phi.
+struct RegModel {
+  int nx = 0;
+  std::vector<double> node;  // nx lattice positions (modules)
+  std::vector<double> lens;  // nx * nx * 2: |q|^2 q D at node (i, j)'s place under H0 (liblizard/src/acquire.c:1506-1512)
+  std::vector<double> phi;   // nx * nx * 49: anchor a's weight at node (i, j) (acquire.c:2196-2210, :2319-2352)
+};
+void reg_model(int span, const float *H0, int w, int h, RegModel &m);
+void reg_nodes(const RegModel &m, const float *params, std::vector<float> &nodes);
+// grad_k1 and grad_anchors (98) from the gradient with respect to the nodes (nx * nx * 2).
+void reg_chain(const RegModel &m, const double *node_grad, double *grad);
+
+// Device state of a refinement: the homography and the intensity model w = alpha r + beta.
+struct RegState {
+  float H[9];
+  float alpha, beta;
+  float pad[5];
+};
+
+// Per node cell (grid (nx - 1) x (nx - 1)): sums of e^2, J^T e (8), J^T J (36, upper triangle), the intensity
+// statistics (sum w, sum r, sum w r, sum r r, count) and e (gx, gy) spread to the cell's four corners (8), e = w -
+// alpha r - beta the residual of the warped capture w against the reference picture r.
+void launch_reg_accum(const FramePlan &p, const WarpFrame *d_frame, const RegState *d_state, const float *d_ref,
+                      float *d_part, hipStream_t s);
+// Sums the cells' parts in FP64 and, when update, solves the damped 8 x 8 normal equations in FP64 and moves H; sets
+// alpha and beta to the least-squares intensity fit of this pass; trace[step] = the pass's loss.
+void launch_gn_step(const FramePlan &p, const float *d_part, RegState *d_state, float damping, int update, float *d_trace,
+                    int step, hipStream_t s);
+
+}  // namespace lz

@@ -782,6 +782,11 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
       mc[t] = m;
       cell[t] = i;
     }
+    for (int i = 0; i < d.nx; i++) {
+      int t = 0;
+      while (t < n && cell[t] < i) t++;
+      d.cell0[i] = i == d.nx - 1 ? n : t;
+    }
   }
   with_size(n, [](auto) {});  // a size the transforms are built for, or LIZ_E_FORMAT
   std::vector<float2> tw(n), rot(ALIGN_TURNS);

@@ -48,6 +48,7 @@ struct PlanDev {
   float node_x[MAX_NODES], node_y[MAX_NODES];
   const float *mcoord;                     // n: module coordinate of sample index t
   const int32_t *mcell;                    // n: node cell of sample index t (the same on both axes)
+  int cell0[MAX_NODES];                    // first sample index of node cell i (cell0[nx - 1] = n)
 };
 
 struct FramePlan {

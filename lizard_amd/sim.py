@@ -23,7 +23,7 @@ import math
 
 import numpy as np
 
-from . import format_coefs, format_info, frame_slots, node_lattice
+from . import format_blocks, format_coefs, format_info, frame_slots, node_lattice
 
 BORDER = 15  # modules between the symbol's edge and the picture
 
@@ -45,6 +45,32 @@ def picture(version, blocks, cnt=0, device="cpu"):
     p = torch.fft.ifft2(Y).real * (n * n)  # exp(+2 pi i (u x + v y) / n), unscaled
     L = 2.0 * math.sqrt(640.0 * subch)
     return (0.5 + torch.clamp(p, -L, L) / (2.0 * L)).to(torch.float32)
+
+
+def remodulate(version, blocks, known, cnt=0, blur=0.0):
+    """The picture re-modulated from what a receiver knows: the blocks whose CRC passed (known[b]) and the padding
+    past the last block (the whitening alone); every other coefficient 0; no clip (the partial sum is not the sent
+    picture's); blurred by a Gaussian of `blur` samples, applied as its transfer function on the coefficients (a
+    periodic blur). numpy float32, n x n."""
+    import torch
+    info = format_info(version)
+    n, subch = info["n"], info["subch"]
+    slots = frame_slots(version, blocks, cnt)
+    u, v = format_coefs(version)
+    keep = np.ones(subch * 320, np.float64)
+    _, s0, ns = format_blocks(version)
+    for b in range(len(s0)):
+        if not known[b]:
+            keep[s0[b] // 2:(s0[b] + ns[b]) // 2] = 0.0
+    otf = keep * np.exp(-2.0 * (math.pi * blur / n) ** 2 * (u.astype(np.float64) ** 2 + v.astype(np.float64) ** 2))
+    s = torch.as_tensor(slots.astype(np.float64))
+    H = torch.complex(1.0 - 2.0 * s[0::2], 1.0 - 2.0 * s[1::2]) / math.sqrt(2.0) * torch.as_tensor(otf)
+    Y = torch.zeros(n, n, dtype=torch.complex128)
+    uu, vv = torch.as_tensor(u), torch.as_tensor(v)
+    Y[vv % n, uu % n] = H
+    Y[(-vv) % n, (-uu) % n] = torch.conj(H)
+    p = torch.fft.ifft2(Y).real * (n * n)
+    return (0.5 + p / (4.0 * math.sqrt(640.0 * subch))).to(torch.float32).numpy()
 
 
 def homography(src, dst):
@@ -122,7 +148,7 @@ def capture(pic, span, cam, seed=0):
     # undo the lens by fixed-point iteration (a contraction for |k1| well below 1), then the homography
     a, D = cam.axis()
     px, py = X.clone(), Y.clone()
-    for _ in range(8):
+    for _ in range(6):
         qx, qy = (px - a[0]) / D, (py - a[1]) / D
         r2 = qx * qx + qy * qy
         px = X - cam.k1 * r2 * qx * D

@@ -11,7 +11,7 @@ HERE=$(cd "$(dirname "$0")" && pwd)
 OUT="$HERE/../_ref"
 if [ ! -f "$SRC/lizard_phy.c" ]; then echo "build_ref: no reference sources at $SRC" >&2; exit 2; fi
 mkdir -p "$OUT"
-UNITS="$SRC/ldpc.c $SRC/layout.c $SRC/acquire.c $SRC/fft.c $SRC/fmt.c $SRC/rs.c"
+UNITS="$SRC/ldpc.c $SRC/layout.c $HERE/ref_acquire.c $SRC/fft.c $SRC/fmt.c $SRC/rs.c"  # acquire.c through ref_acquire.c
 gcc -O2 -ffp-contract=off -DOB_SCALAR -fPIC -shared -I"$SRC" -o "$OUT/libref_scalar.so.tmp" "$HERE/ref_driver.c" $UNITS -lm
 mv "$OUT/libref_scalar.so.tmp" "$OUT/libref_scalar.so"
 CLANG=/opt/rocm/llvm/bin/clang

@@ -1,0 +1,114 @@
+"""The registration model and its refinement loss on the CPU: the FP64 restatement (oracle/register.py) pinned by the
+reference's own interpolation compiled from its sources, by the spline's exactness properties and by finite
+differences of its own loss; the product's node field (lizx_reg_nodes) against it; the synthetic channel's lens nodes
+against the model's."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import lizard_amd as L
+from lizard_amd import sim
+from _libs import ptr, ref
+
+import register as R  # oracle/register.py
+
+
+def ref_lib():
+    r = ref("scalar")
+    if r is None:
+        pytest.skip("oracle/_ref not built")
+    r.ref_anch_herm4.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+    r.ref_anchor_m0.restype = C.c_int
+    r.ref_anchor_m0.argtypes = [C.c_int, C.c_int]
+    return r
+
+
+def test_hermite_weights_match_reference():
+    r = ref_lib()
+    for span in (128, 256, 512):
+        t = R.anchor_knots(span)
+        for i in range(7):
+            assert R.anchor_m0(span, i) == r.ref_anchor_m0(span, i)
+        for x in np.concatenate([np.linspace(0, 1, 97), t, t[1:-1] - 1e-9]):
+            b = np.zeros(1, np.int32)
+            w = np.zeros(4)
+            r.ref_anch_herm4(ptr(t), len(t), float(x), ptr(b), ptr(w))
+            bb, ww = R.herm4(t, x)
+            assert bb == b[0]
+            assert np.allclose(ww, w, rtol=0, atol=1e-15)
+
+
+def test_hermite_interpolates_knots_and_lines():
+    t = R.anchor_knots(256)
+    v = np.random.default_rng(1).standard_normal(len(t))
+    lin = 0.3 + 2.5 * t
+    for k, x in enumerate(t):
+        b, w = R.herm4(t, x)
+        val = sum(w[a] * v[b + a] for a in range(4) if 0 <= b + a < len(t))
+        assert abs(val - v[k]) < 1e-12
+    for x in np.linspace(0, 1, 50):
+        b, w = R.herm4(t, x)
+        val = sum(w[a] * lin[b + a] for a in range(4) if 0 <= b + a < len(t))
+        assert abs(val - (0.3 + 2.5 * x)) < 1e-12  # slopes from neighbouring knots reproduce a line exactly
+
+
+def test_anchor_mesh_is_zero_on_the_lattice_edge_and_linear():
+    phi = R.anchor_basis(256)
+    assert np.all(phi[0] == 0) and np.all(phi[-1] == 0) and np.all(phi[:, 0] == 0) and np.all(phi[:, -1] == 0)
+    a = np.random.default_rng(2).standard_normal((7, 7, 2))
+    H0 = np.diag([3.0, 3.0, 1.0])
+    p1 = np.zeros(R.NPARAM); p1[:8] = H0.ravel()[:8]; p1[9:] = a.ravel()
+    p2 = p1.copy(); p2[9:] *= 2.5
+    n1, n2 = R.nodes(256, H0, 1920, 1080, p1), R.nodes(256, H0, 1920, 1080, p2)
+    assert np.allclose(n2, 2.5 * n1)
+
+
+@pytest.mark.parametrize("span", [128, 256, 512])
+def test_product_nodes_match_oracle(span):
+    rng = np.random.default_rng(span)
+    H = np.array([[2.9, 0.2, 300.0], [-0.15, 3.1, 120.0], [2e-5, -1e-5, 1.0]])
+    p = L.reg_params(H, rng.uniform(-0.05, 0.05), rng.normal(0, 0.4, 98))
+    got = L.reg_nodes(span, H.astype(np.float32), 1920, 1080, p)
+    want = R.nodes(span, H, 1920, 1080, p.astype(np.float64))
+    assert np.abs(got - want).max() < 1e-5 * max(1.0, np.abs(want).max())
+
+
+def test_channel_lens_nodes_are_the_models():
+    cam = sim.Camera(1920, 1080, rot_deg=7.0, scale=0.8, persp=0.03, k1=0.04)
+    Hm = cam.H(256)
+    want = sim.nodes_for(cam, 256, Hm)
+    p = np.zeros(R.NPARAM); p[:8] = (Hm / Hm[2, 2]).ravel()[:8]; p[8] = cam.k1
+    assert np.abs(R.nodes(256, Hm / Hm[2, 2], 1920, 1080, p) - want).max() < 1e-4
+
+
+@pytest.fixture(scope="module")
+def small_case():
+    version, span = 5, 128
+    rng = np.random.default_rng(8)
+    cam = sim.Camera(1280, 720, rot_deg=-4.0, scale=0.8, persp=0.02, k1=0.03, blur=0.8, noise=3.0)
+    blocks = sim.random_blocks(version, rng)
+    img, Hn, nodes = sim.capture(sim.picture(version, blocks), span, cam, seed=2)
+    n = L.format_info(version)["n"]
+    ref_pic = sim.remodulate(version, blocks, np.ones(9, bool), 0, 1.0).astype(np.float64)
+    H0 = Hn.reshape(3, 3).astype(np.float64)
+    p = np.zeros(R.NPARAM)
+    p[:8] = H0.ravel()[:8] + rng.normal(0, 1e-3, 8) * np.array([1, 1, 100, 1, 1, 100, 1e-4, 1e-4])
+    p[8] = cam.k1
+    p[9:] = rng.normal(0, 0.2, 98)
+    return dict(img=img.numpy(), n=n, span=span, version=version, H0=H0, p=p, ref=ref_pic)
+
+
+def test_oracle_gradient_matches_finite_differences(small_case):
+    c = small_case
+    f = lambda p: R.loss_grad(c["img"], c["n"], c["span"], c["H0"], p, c["ref"], 0.9, 0.05)["loss"]  # noqa: E731
+    g = R.loss_grad(c["img"], c["n"], c["span"], c["H0"], c["p"], c["ref"], 0.9, 0.05)["grad"]
+    # step per parameter in its own units: pixels for translation and the lens / anchors, 1e-4 for the linear part
+    steps = np.full(R.NPARAM, 1e-3)
+    steps[[0, 1, 3, 4]] = 1e-6
+    steps[[6, 7]] = 1e-9
+    steps[8] = 1e-5
+    for k in [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 57, 58, 105, 106]:
+        e = np.zeros(R.NPARAM); e[k] = steps[k]
+        fd = (f(c["p"] + e) - f(c["p"] - e)) / (2 * steps[k])
+        assert abs(fd - g[k]) <= 2e-3 * max(abs(g[k]), 1e-3 * np.abs(g).max()) + 1e-6, (k, fd, g[k])

@@ -1,0 +1,162 @@
+"""The hand-written adjoint of the warp and the refinement loss on the GPU (lizx_reg_loss_grad) against the FP64
+restatement (oracle/register.py), wrapped in a torch.autograd.Function and checked by torch's gradcheck against finite
+differences of the GPU loss; Gauss-Newton refinement (lizx_reg_refine) from a homography whose corners are 1.5 px off,
+and the blocks it recovers."""
+import numpy as np
+import pytest
+
+import lizard_amd as L
+from lizard_amd import sim
+
+import register as R  # oracle/register.py
+
+pytestmark = pytest.mark.gpu
+
+SIDE = {128: 158, 256: 286, 512: 542}
+
+
+def corners(span):
+    s = SIDE[span]
+    return np.array([[0, 0], [s, 0], [s, s], [0, s]], float)
+
+
+def perturbed(Ht, span, rng, px=1.5):
+    c = corners(span)
+    cp = np.stack(R.project(Ht, c[:, 0], c[:, 1]), -1)
+    return sim.homography(c, cp + rng.uniform(-px, px, (4, 2)))
+
+
+def corner_error(Ht, params, span):
+    H = np.append(np.asarray(params[:8], np.float64), 1.0).reshape(3, 3)
+    c = corners(span)
+    a = np.stack(R.project(Ht, c[:, 0], c[:, 1]), -1)
+    b = np.stack(R.project(H, c[:, 0], c[:, 1]), -1)
+    return float(np.abs(a - b).max())
+
+
+def make_case(version, span, seed, W=1920, Hh=1080, k1=0.03, known=None):
+    rng = np.random.default_rng(seed)
+    cam = sim.Camera(W, Hh, rot_deg=rng.uniform(-8, 8), scale=0.85, persp=0.03, k1=k1, blur=1.0, noise=4.0)
+    blocks = sim.random_blocks(version, rng)
+    img, Hn, nodes = sim.capture(sim.picture(version, blocks), span, cam, seed=seed)
+    Ht = cam.H(span)
+    Ht = Ht / Ht[2, 2]
+    n = L.format_info(version)["n"]
+    s = np.sqrt(abs(np.linalg.det(Ht[:2, :2]))) * span / n  # capture pixels a picture sample
+    blur = np.sqrt(cam.blur ** 2 + 1.0 / 3.0) / s
+    B = L.format_info(version)["blocks"]
+    ref_pic = sim.remodulate(version, blocks, np.ones(B, bool) if known is None else known, 0, blur)
+    H0 = perturbed(Ht, span, rng)
+    return dict(version=version, span=span, img=img.numpy(), Ht=Ht, H0=H0, k1=k1, blocks=blocks, ref=ref_pic, n=n,
+                nodes_true=nodes)
+
+
+@pytest.fixture(scope="module")
+def case384():
+    return make_case(5, 128, 3, W=1280, Hh=720)
+
+
+@pytest.mark.parametrize("which", ["384", "768"])
+def test_adjoint_matches_fp64(decoder_gpu, case384, which):
+    c = case384 if which == "384" else make_case(20, 256, 4)
+    rng = np.random.default_rng(11)
+    p = L.reg_params(c["H0"], c["k1"], rng.normal(0, 0.3, 98))
+    loss, grad = decoder_gpu.reg_loss_grad(c["version"], c["span"], c["img"], c["H0"], p, c["ref"], 0.9, 0.04)
+    want = R.loss_grad(c["img"], c["n"], c["span"], c["H0"].astype(np.float32).astype(np.float64),
+                       p.astype(np.float64), c["ref"].astype(np.float64), 0.9, 0.04)
+    assert abs(loss - want["loss"]) <= 1e-4 * want["loss"]
+    g, gw = grad, want["grad"]
+    # each group of parameters against its own scale (their units differ by orders of magnitude)
+    for sl in (slice(0, 2), slice(2, 3), slice(3, 5), slice(5, 6), slice(6, 8), slice(8, 9), slice(9, None)):
+        err = np.linalg.norm(g[sl] - gw[sl]) / max(np.linalg.norm(gw[sl]), 1e-30)
+        assert err < 2e-3, (sl, err)
+
+
+def test_autograd_gradcheck(decoder_gpu, case384):
+    """torch.autograd.gradcheck of the GPU loss (through the C ABI adjoint) against its own finite differences, over
+    a scaled perturbation of 12 parameters (one unit ~ one pixel of movement for each)."""
+    torch = pytest.importorskip("torch")
+    c = case384
+    p0 = L.reg_params(c["H0"], c["k1"], np.random.default_rng(5).normal(0, 0.3, 98)).astype(np.float64)
+    idx = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 58, 106])
+    scale = np.array([1 / 300, 1 / 300, 1.0, 1 / 300, 1 / 300, 1.0, 3e-6, 3e-6, 1 / 300, 1.0, 1.0, 1.0])
+
+    class RegLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, delta):
+            p = p0.copy()
+            p[idx] += delta.detach().cpu().numpy() * scale
+            loss, grad = decoder_gpu.reg_loss_grad(c["version"], c["span"], c["img"], c["H0"], p.astype(np.float32),
+                                                   c["ref"], 0.9, 0.04)
+            ctx.save_for_backward(torch.as_tensor(grad[idx] * scale, dtype=torch.float64))
+            return torch.tensor(loss, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, g):
+            (gr,) = ctx.saved_tensors
+            return g * gr
+
+    delta = torch.zeros(len(idx), dtype=torch.float64, requires_grad=True)
+    out = RegLoss.apply(delta)
+    out.backward()
+    assert torch.isfinite(delta.grad).all() and delta.grad.abs().max() > 0
+    assert torch.autograd.gradcheck(RegLoss.apply, (delta,), eps=0.02, atol=0.5, rtol=0.05, nondet_tol=1e-3)
+
+
+def test_refine_recovers_the_homography(decoder_gpu):
+    c = make_case(20, 256, 6)
+    p = L.reg_params(c["H0"], c["k1"])
+    e0 = corner_error(c["Ht"], p, c["span"])
+    assert e0 > 0.5
+    p1, trace = decoder_gpu.reg_refine(c["version"], c["span"], c["img"], c["H0"], p, c["ref"], steps=10)
+    assert corner_error(c["Ht"], p1, c["span"]) < 0.1
+    assert trace[-1] < 0.5 * trace[0] and np.all(np.isfinite(trace))
+    # the FP64 Gauss-Newton of the oracle lands on the same homography
+    q = p.astype(np.float64)
+    r = R.loss_grad(c["img"], c["n"], c["span"], c["H0"].astype(np.float32).astype(np.float64), q,
+                    c["ref"].astype(np.float64))
+    sw, sr, swr, srr, cnt = r["stats"]
+    al = (cnt * swr - sw * sr) / (cnt * srr - sr * sr)
+    be = (sw - al * sr) / cnt
+    for _ in range(4):
+        q, rr = R.gn_step(c["img"], c["n"], c["span"], c["H0"].astype(np.float32).astype(np.float64), q,
+                          c["ref"].astype(np.float64), al, be)
+        sw, sr, swr, srr, cnt = rr["stats"]
+        al = (cnt * swr - sw * sr) / (cnt * srr - sr * sr)
+        be = (sw - al * sr) / cnt
+    H1 = np.append(p1[:8], 1.0).reshape(3, 3)
+    Hq = np.append(q[:8], 1.0).reshape(3, 3)
+    cc = corners(c["span"])
+    d = np.stack(R.project(H1, cc[:, 0], cc[:, 1]), -1) - np.stack(R.project(Hq, cc[:, 0], cc[:, 1]), -1)
+    assert np.abs(d).max() < 0.05
+
+
+def test_refine_recovers_blocks(decoder_gpu):
+    """Decision-directed: decode through the perturbed registration, re-modulate the blocks whose CRC passed, refine,
+    decode again: no block lost, and the blocks the perturbation cost come back."""
+    c = make_case(20, 256, 9)
+    span, version = c["span"], c["version"]
+    p = L.reg_params(c["H0"], c["k1"])
+    nodes0 = L.reg_nodes(span, c["H0"].astype(np.float32), c["img"].shape[1], c["img"].shape[0], p)
+    before = decoder_gpu.decode_reg(version, span, c["img"], c["H0"], nodes0, soft=False)
+    info = L.format_info(version)
+    n = info["n"]
+    Ht = c["Ht"]
+    s = np.sqrt(abs(np.linalg.det(Ht[:2, :2]))) * span / n
+    ref_pic = sim.remodulate(version, c["blocks"], before["ok"].astype(bool), 0, np.sqrt(1.0 + 1.0 / 3.0) / s)
+    p1, trace = decoder_gpu.reg_refine(version, span, c["img"], c["H0"], p, ref_pic, steps=10)
+    H1 = np.append(p1[:8], 1.0).reshape(3, 3)
+    after = decoder_gpu.decode_reg(version, span, c["img"], H1, nodes0, soft=False)
+    assert after["good"] >= before["good"]
+    assert after["good"] == info["blocks"] or after["good"] > before["good"]
+    v = after["ok"].astype(bool)
+    assert np.array_equal(after["blocks"][v], c["blocks"].reshape(-1, 476)[v])
+
+
+def test_refine_argument_errors(decoder_gpu, case384):
+    c = case384
+    p = L.reg_params(c["H0"], c["k1"])
+    with pytest.raises(L.LizError):
+        decoder_gpu.reg_refine(c["version"], c["span"], c["img"], c["H0"], p, c["ref"], steps=101)
+    with pytest.raises(L.LizError):
+        decoder_gpu.reg_loss_grad(3, c["span"], c["img"], c["H0"], p, c["ref"])
