@@ -1,13 +1,17 @@
 // Picture samples to soft values on gfx950 (liblizard/src/lizard_phy.c:1566-1772, research/SPEC.md 6.9-6.13, 7.3,
 // 8.5):
-//   k_erase_fit  one workgroup a frame: the shading's least-squares fit on every eighth sample outside the anchors
-//   k_rows       eight rows a workgroup: anchor erase on load, detrend row sums (double), two real rows packed per
-//                complex FFT along x (Stockham, radix 4/2/3/5 in LDS), pruned to |u| < E, written column-major
-//   k_cols       two columns u a workgroup: FFT along y, keeping v < E and n - v (the upper half-plane's reach)
+//   k_erase_fit  n / 32 workgroups a frame: the shading's least-squares sums on every eighth sample outside the
+//                anchors, four fit rows a workgroup
+//   k_rows       eight rows a workgroup, two a wave packed as one complex sequence: the warp (or a supplied grid) read
+//                on load with the anchors' samples erased, the detrend row sums, the wave FFT along x, the two rows'
+//                transforms separated and pruned to u < E, written column-major
+//   k_cols       one column u a wave: the wave FFT along y, keeping v < E and n - v (the upper half-plane's reach)
 //   k_align      one workgroup a frame: the detrend projections and the pilots' two-pass least-squares grid shift
-//   k_demap      one workgroup a sub-channel: 320 coefficients gathered, detrended, turned, their moments, int8 LLRs
-// Floating-point steps the reference rounds in a fixed order (the detrend terms, the turn, the LLR scale) use
-// explicitly rounded operations, so no contraction into fused multiply-adds changes them.
+//   k_demap      one wave a sub-channel: 320 coefficients gathered, detrended, turned, their moments, int8 LLRs
+// The wave FFT (WaveFft): n = 64 M points, M in registers a lane, an M-point DFT in registers, then two radix-8 passes
+// through the wave's own LDS; every size compiled in. Floating-point steps the reference rounds in a fixed order (the
+// warp, the detrend terms, the turn, the LLR scale) use explicitly rounded operations, so no contraction into fused
+// multiply-adds changes them.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -263,6 +267,14 @@ __device__ double block_sum(double v, double *red, int tid) {
 }
 
 // ------------------------------------------------------------------------------------------------ the warp
+// Grey level g / 255, correctly rounded: the product by the rounded reciprocal, corrected by one fused multiply-add
+// (the residual g - 255 q is exact in it). Equal to the IEEE quotient for all 256 levels (checked exhaustively).
+__device__ __forceinline__ float grey(uint8_t g) {
+  constexpr float c = 1.0f / 255.0f;
+  const float f = static_cast<float>(g), q = __fmul_rn(f, c);
+  return __fmaf_rn(__fmaf_rn(-q, 255.0f, f), c, q);
+}
+
 // A bilinear read of the uint8 capture at pixel coordinates (x, y) (pixel centres at half integers), clamped to the
 // frame, grey level i / 255 (liblizard/src/internal.h:11-23 at gamma 1). In bounds for any input (NaN reads pixel 0).
 __device__ __forceinline__ float sample(const WarpFrame &F, float x, float y) {
@@ -274,54 +286,94 @@ __device__ __forceinline__ float sample(const WarpFrame &F, float x, float y) {
   const int xi = x >= 0 ? static_cast<int>(x) : 0, yi = y >= 0 ? static_cast<int>(y) : 0;
   const float fx = __fsub_rn(x, static_cast<float>(xi)), fy = __fsub_rn(y, static_cast<float>(yi));
   const uint8_t *p = F.img + static_cast<size_t>(yi) * F.w + xi;
-  const float a = __fdiv_rn(static_cast<float>(p[0]), 255.0f), b = __fdiv_rn(static_cast<float>(p[1]), 255.0f);
-  const float c = __fdiv_rn(static_cast<float>(p[F.w]), 255.0f), d = __fdiv_rn(static_cast<float>(p[F.w + 1]), 255.0f);
+  const float a = grey(p[0]), b = grey(p[1]), c = grey(p[F.w]), d = grey(p[F.w + 1]);
   const float top = __fadd_rn(a, __fmul_rn(__fsub_rn(b, a), fx)), bot = __fadd_rn(c, __fmul_rn(__fsub_rn(d, c), fx));
   return __fadd_rn(__fmul_rn(top, __fsub_rn(1.0f, fy)), __fmul_rn(bot, fy));
+}
+
+// The row part of a warp sample (everything that depends on y alone).
+struct WarpRow {
+  float my, v, omv, kw, ku, kv;
+  int j;
+};
+__device__ __forceinline__ WarpRow warp_row(const PlanDev &P, const WarpFrame &F, int y) {
+  WarpRow r;
+  r.my = P.mcoord[y];
+  r.j = P.mcell[y];
+  r.v = __fdiv_rn(__fsub_rn(r.my, P.node_y[r.j]), __fsub_rn(P.node_y[r.j + 1], P.node_y[r.j]));
+  r.omv = __fsub_rn(1.0f, r.v);
+  r.kw = __fadd_rn(__fmul_rn(F.H[7], r.my), 1.0f);
+  r.ku = __fadd_rn(__fmul_rn(F.H[1], r.my), F.H[2]);
+  r.kv = __fadd_rn(__fmul_rn(F.H[4], r.my), F.H[5]);
+  return r;
 }
 
 // Picture sample (x, y) of frame F: module coordinates (15 + (t + 1/2) span / n on either axis, SPEC 6.9), the
 // homography H plus the node lattice's bilinear displacement (acquire.c:5095-5103), read from the capture. The float
 // operations are the reference's scalar grid walk's, in its order (acquire.c:5121-5196 without OB_SIMD), so the
 // samples are that walk's to the bit.
-__device__ __forceinline__ float warp_sample(const PlanDev &P, const WarpFrame &F, int x, int y) {
-  const float mx = P.mcoord[x], my = P.mcoord[y];
-  const int i = P.mcell[x], j = P.mcell[y], nx = P.nx;
-  const float v = __fdiv_rn(__fsub_rn(my, P.node_y[j]), __fsub_rn(P.node_y[j + 1], P.node_y[j]));
+__device__ __forceinline__ float warp_at(const PlanDev &P, const WarpFrame &F, const WarpRow &r, int x) {
+  const float mx = P.mcoord[x];
+  const int i = P.mcell[x], nx = P.nx, j = r.j;
+  const float v = r.v, omv = r.omv;
   const float *H = F.H;
-  const float kw = __fadd_rn(__fmul_rn(H[7], my), 1.0f), ku = __fadd_rn(__fmul_rn(H[1], my), H[2]);
-  const float kv = __fadd_rn(__fmul_rn(H[4], my), H[5]);
   const float2 n00 = F.nodes[j * nx + i], n10 = F.nodes[j * nx + i + 1];
   const float2 n01 = F.nodes[(j + 1) * nx + i], n11 = F.nodes[(j + 1) * nx + i + 1];
-  const float inv = __fdiv_rn(1.0f, __fsub_rn(P.node_x[i + 1], P.node_x[i])), omv = __fsub_rn(1.0f, v);
+  const float inv = P.node_inv[i];
   const float ex = __fmul_rn(__fadd_rn(__fmul_rn(__fsub_rn(n10.x, n00.x), omv), __fmul_rn(__fsub_rn(n11.x, n01.x), v)), inv);
   const float cx = __fsub_rn(__fadd_rn(__fmul_rn(n00.x, omv), __fmul_rn(n01.x, v)), __fmul_rn(ex, P.node_x[i]));
   const float ey = __fmul_rn(__fadd_rn(__fmul_rn(__fsub_rn(n10.y, n00.y), omv), __fmul_rn(__fsub_rn(n11.y, n01.y), v)), inv);
   const float cy = __fsub_rn(__fadd_rn(__fmul_rn(n00.y, omv), __fmul_rn(n01.y, v)), __fmul_rn(ey, P.node_x[i]));
-  const float rw = __fdiv_rn(1.0f, __fadd_rn(__fmul_rn(H[6], mx), kw));
-  const float pu = __fmul_rn(__fadd_rn(__fmul_rn(H[0], mx), ku), rw), pv = __fmul_rn(__fadd_rn(__fmul_rn(H[3], mx), kv), rw);
+  const float rw = __fdiv_rn(1.0f, __fadd_rn(__fmul_rn(H[6], mx), r.kw));
+  const float pu = __fmul_rn(__fadd_rn(__fmul_rn(H[0], mx), r.ku), rw), pv = __fmul_rn(__fadd_rn(__fmul_rn(H[3], mx), r.kv), rw);
   const float dx = __fadd_rn(cx, __fmul_rn(ex, mx)), dy = __fadd_rn(cy, __fmul_rn(ey, mx));
   return sample(F, __fadd_rn(pu, dx), __fadd_rn(pv, dy));
 }
 
+__device__ __forceinline__ float warp_sample(const PlanDev &P, const WarpFrame &F, int x, int y) {
+  return warp_at(P, F, warp_row(P, F, y), x);
+}
+
 // Where the path's picture samples come from: a grid the caller supplied, or captures through their registrations.
+// operator() reads one sample; row() and at() read a row's samples with the row's part done once.
 struct GridSource {
   const float *grid;
   __device__ __forceinline__ float operator()(const PlanDev &P, int f, int x, int y) const {
     return grid[(static_cast<size_t>(f) * P.n + y) * P.n + x];
   }
+  using Row = const float *;
+  __device__ __forceinline__ Row row(const PlanDev &P, int f, int y) const {
+    return grid + (static_cast<size_t>(f) * P.n + y) * P.n;
+  }
+  __device__ __forceinline__ float at(const PlanDev &, Row r, int x) const { return r[x]; }
 };
 struct GridListSource {
   const float *const *grids;
   __device__ __forceinline__ float operator()(const PlanDev &P, int f, int x, int y) const {
     return grids[f][static_cast<size_t>(y) * P.n + x];
   }
+  using Row = const float *;
+  __device__ __forceinline__ Row row(const PlanDev &P, int f, int y) const {
+    return grids[f] + static_cast<size_t>(y) * P.n;
+  }
+  __device__ __forceinline__ float at(const PlanDev &, Row r, int x) const { return r[x]; }
 };
 struct WarpSource {
   const WarpFrame *frames;
   __device__ __forceinline__ float operator()(const PlanDev &P, int f, int x, int y) const {
     return warp_sample(P, frames[f], x, y);
   }
+  struct Row {
+    WarpFrame F;
+    WarpRow r;
+  };
+  __device__ __forceinline__ Row row(const PlanDev &P, int f, int y) const {
+    Row out;
+    out.F = frames[f];
+    out.r = warp_row(P, out.F, y);
+    return out;
+  }
+  __device__ __forceinline__ float at(const PlanDev &P, const Row &r, int x) const { return warp_at(P, r.F, r.r, x); }
 };
 
 __global__ __launch_bounds__(256) void k_warp(PlanDev P, const WarpFrame *frames, float *grid) {
@@ -362,21 +414,25 @@ __global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *p
       (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
-// The shading beta = hole_inv rhs (lizard_phy.c:1618-1624) from the fit's parts, summed in part order.
-__device__ __forceinline__ void fit_beta(const PlanDev &P, const double *part, double *rhs_sh, double *bsh, int tid) {
+// The shading beta = hole_inv rhs (lizard_phy.c:1618-1624) from the fit's parts, summed in part order; one wave a
+// frame, stored after the parts.
+__global__ __launch_bounds__(64) void k_fit_solve(PlanDev P, double *fit) {
+  __shared__ double part_sh[FIT_BETA], rhs_sh[6];
+  const int tid = threadIdx.x, f = blockIdx.x, parts = P.n / 8 / FIT_ROWS;
+  double *F = fit + static_cast<size_t>(f) * FIT_STRIDE;
+  for (int k = tid; k < parts * 6; k += 64) part_sh[k] = F[k];
+  __syncthreads();
   if (tid < 6) {
-    const int parts = P.n / 8 / FIT_ROWS;
     double s = 0;
-    for (int b = 0; b < parts; b++) s += part[b * 6 + tid];
+    for (int b = 0; b < parts; b++) s += part_sh[b * 6 + tid];
     rhs_sh[tid] = s;
   }
   __syncthreads();
   if (tid < 6) {
     double s = 0;
     for (int k = 0; k < 6; k++) s += P.hole_inv[tid * 6 + k] * rhs_sh[k];
-    bsh[tid] = s;
+    F[FIT_BETA + tid] = s;
   }
-  __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------------------ the two passes
@@ -390,30 +446,34 @@ __global__ __launch_bounds__(256) void k_rows(PlanDev P, Src src, const double *
   using F = WaveFft<N>;
   constexpr int M = F::M;
   extern __shared__ float2 zs[];  // 4 waves x F::LDS
-  __shared__ double bsh[6], rhs_sh[6];
+  __shared__ double bsh[6];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y;
   const int y = blockIdx.x * ROWS + 2 * w;
-  fit_beta(P, fitpart + static_cast<size_t>(f) * FIT_STRIDE, rhs_sh, bsh, tid);
+  if (tid < 6) bsh[tid] = fitpart[static_cast<size_t>(f) * FIT_STRIDE + FIT_BETA + tid];
+  __syncthreads();
   const bool yin0 = in_square(P, y), yin1 = in_square(P, y + 1);
+  const auto row0 = src.row(P, f, y), row1 = src.row(P, f, y + 1);
   float2 v[M];
-  double s0 = 0, x0 = 0, q0 = 0, s1 = 0, x1 = 0, q1 = 0;
+  // each lane's share of the row sums in float (as the reference's float chains), the lanes' in double
+  float s0f = 0, x0f = 0, q0f = 0, s1f = 0, x1f = 0, q1f = 0;
 #pragma unroll
   for (int m = 0; m < M; m++) {
     const int x = lane + 64 * m;
-    float a = src(P, f, x, y), b = src(P, f, x, y + 1);
+    float a = src.at(P, row0, x), b = src.at(P, row1, x);
     if ((yin0 || yin1) && in_square(P, x)) {
       if (yin0) a = hole_value(P, bsh, x, y);
       if (yin1) b = hole_value(P, bsh, x, y + 1);
     }
     v[m] = make_float2(a, b);
-    const double cx = P.cx[x], qx = P.qx[x];
-    s0 += a;
-    x0 += a * cx;
-    q0 += a * qx;
-    s1 += b;
-    x1 += b * cx;
-    q1 += b * qx;
+    const float cx = P.cx[x], qx = P.qx[x];
+    s0f += a;
+    x0f += a * cx;
+    q0f += a * qx;
+    s1f += b;
+    x1f += b * cx;
+    q1f += b * qx;
   }
+  double s0 = s0f, x0 = x0f, q0 = q0f, s1 = s1f, x1 = x1f, q1 = q1f;
   for (int o = 32; o > 0; o >>= 1) {
     s0 += __shfl_xor(s0, o, 64);
     x0 += __shfl_xor(x0, o, 64);
@@ -598,55 +658,56 @@ __device__ __forceinline__ double axis_info(double g) {
   return g > 0 ? pow(1 - pow(2.0, -0.3073 * pow(2 * sqrt(g), 1.7870)), 1.1064) : 0;
 }
 
-// Sub-channel j of frame f (lizard_phy.c:1756-1772, :1322-1332; SPEC 8.5).
-__global__ __launch_bounds__(320) void k_demap(PlanDev P, const float2 *HP_all, const double *proj, const float *align,
+// Sub-channel j of frame f (lizard_phy.c:1756-1772, :1322-1332; SPEC 8.5), one wave a sub-channel (four a
+// workgroup): each lane's five coefficients gathered, detrended and turned, the moments m2, m4 summed in double
+// across the wave by shuffles (every lane ends with the totals, so the scale needs no broadcast and no barrier).
+__global__ __launch_bounds__(256) void k_demap(PlanDev P, const float2 *HP_all, const double *proj, const float *align,
                                                int8_t *llr_all, double *est_all, float *rms_all, float *dbg_all) {
-  __shared__ double r2[320], r4[320];
-  __shared__ float s_k;
-  const int tid = threadIdx.x, j = blockIdx.x, f = blockIdx.y, ci = j * SUB + tid;
+  const int lane = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6), f = blockIdx.y;
+  if (j >= P.subch) return;  // the whole wave
   const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
   const Proj pr{proj[f * 5], proj[f * 5 + 1], proj[f * 5 + 2], proj[f * 5 + 3], proj[f * 5 + 4]};
   const float dx = align[2 * f], dy = align[2 * f + 1];
-  int u, v;
-  coef_uv(P, ci, &u, &v);
-  float2 y = coef_at(P, HP, pr, u, v);
-  if (dbg_all) reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * P.subch * SUB + ci] = y;
-  if (dx != 0 || dy != 0) y = turn(P, y, u, v, dx, dy);
-  const double e = static_cast<double>(y.x) * y.x + static_cast<double>(y.y) * y.y;
-  r2[tid] = e;
-  r4[tid] = e * e;
-  __syncthreads();
-  // 320 = 256 + 64: fold the top 64 in, then halve
-  if (tid < 64) {
-    r2[tid] += r2[tid + 256];
-    r4[tid] += r4[tid + 256];
+  constexpr int PER = SUB / 64;
+  float2 y[PER];
+  double s2 = 0, s4 = 0;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    const int ci = j * SUB + lane + 64 * q;
+    int u, v;
+    coef_uv(P, ci, &u, &v);
+    float2 c = coef_at(P, HP, pr, u, v);
+    if (dbg_all) reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * P.subch * SUB + ci] = c;
+    if (dx != 0 || dy != 0) c = turn(P, c, u, v, dx, dy);
+    y[q] = c;
+    const double e = static_cast<double>(c.x) * c.x + static_cast<double>(c.y) * c.y;
+    s2 += e;
+    s4 += e * e;
   }
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o) {
-      r2[tid] += r2[tid + o];
-      r4[tid] += r4[tid + o];
-    }
-    __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    s2 += __shfl_xor(s2, o, 64);
+    s4 += __shfl_xor(s4, o, 64);
   }
-  if (tid == 0) {
-    const double m2 = r2[0] / SUB, m4 = r4[0] / SUB;
+  const double m2 = s2 / SUB, m4 = s4 / SUB;
+  double a2 = 2 * m2 * m2 - m4;
+  a2 = a2 > 0 ? sqrt(a2) : 0;
+  double nv = m2 - a2;
+  if (nv < 0.02 * m2) nv = 0.02 * m2;
+  const float k = static_cast<float>(0.7 * 2 * sqrt(2 * a2) / nv);
+  if (lane == 0) {
     rms_all[f * P.subch + j] = m2 > 0 ? static_cast<float>(sqrt(m2 / 2)) : 0.f;
-    double a2 = 2 * m2 * m2 - m4;
-    a2 = a2 > 0 ? sqrt(a2) : 0;
-    double nv = m2 - a2;
-    if (nv < 0.02 * m2) nv = 0.02 * m2;
     est_all[f * P.subch + j] = axis_info(a2 / nv);
-    s_k = static_cast<float>(0.7 * 2 * sqrt(2 * a2) / nv);
   }
-  __syncthreads();
-  const float k = s_k;
-  float lr = __fmul_rn(k, y.x), li = __fmul_rn(k, y.y);
-  lr = lr > 10.f ? 10.f : (lr < -10.f ? -10.f : lr);
-  li = li > 10.f ? 10.f : (li < -10.f ? -10.f : li);
-  int8_t *o = llr_all + static_cast<size_t>(f) * P.subch * 2 * SUB + 2 * ci;
-  o[0] = lr == lr ? static_cast<int8_t>(rintf(__fmul_rn(lr, 8.f))) : 0;
-  o[1] = li == li ? static_cast<int8_t>(rintf(__fmul_rn(li, 8.f))) : 0;
+  int8_t *o = llr_all + static_cast<size_t>(f) * P.subch * 2 * SUB + 2 * j * SUB;
+#pragma unroll
+  for (int q = 0; q < PER; q++) {
+    float lr = __fmul_rn(k, y[q].x), li = __fmul_rn(k, y[q].y);
+    lr = lr > 10.f ? 10.f : (lr < -10.f ? -10.f : lr);
+    li = li > 10.f ? 10.f : (li < -10.f ? -10.f : li);
+    const int at = 2 * (lane + 64 * q);
+    o[at] = lr == lr ? static_cast<int8_t>(rintf(__fmul_rn(lr, 8.f))) : 0;
+    o[at + 1] = li == li ? static_cast<int8_t>(rintf(__fmul_rn(li, 8.f))) : 0;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------ host: plans
@@ -770,6 +831,10 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
   // reference's walk finds it (acquire.c:5111-5119)
   d.nx = d.ny = node_lattice(span, d.node_x);
   for (int k = 0; k < d.ny; k++) d.node_y[k] = d.node_x[k];
+  for (int k = 0; k + 1 < d.nx; k++) {
+    const volatile float span_k = d.node_x[k + 1] - d.node_x[k];
+    d.node_inv[k] = 1.0f / span_k;
+  }
   std::vector<float> mc(n);
   std::vector<int32_t> cell(n);
   {
@@ -884,6 +949,7 @@ template <class Src>
 static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s) {
   const int n = p.n, E = p.E;
   hipLaunchKernelGGL(k_erase_fit<Src>, dim3(n / 8 / FIT_ROWS, frames), dim3(256), 0, s, p.dev, src, b.beta);
+  hipLaunchKernelGGL(k_fit_solve, dim3(frames), dim3(64), 0, s, p.dev, b.beta);
   with_size(n, [&](auto N) {
     constexpr int NN = decltype(N)::value;
     const size_t lds = 4 * WaveFft<NN>::LDS * sizeof(float2);
@@ -892,8 +958,8 @@ static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int 
     hipLaunchKernelGGL(k_cols<NN>, dim3((E + 3) / 4, frames), dim3(256), lds, s, p.dev, b.X, b.HP);
   });
   hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.HP, b.proj, b.align);
-  hipLaunchKernelGGL(k_demap, dim3(p.subch, frames), dim3(320), 0, s, p.dev, b.HP, b.proj, b.align, b.llr, b.est,
-                     b.rms, b.coef_dbg);
+  hipLaunchKernelGGL(k_demap, dim3((p.subch + 3) / 4, frames), dim3(256), 0, s, p.dev, b.HP, b.proj, b.align, b.llr,
+                     b.est, b.rms, b.coef_dbg);
   hip_check(hipGetLastError(), "soft-value launches");
 }
 

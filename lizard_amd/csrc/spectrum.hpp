@@ -19,7 +19,8 @@ constexpr int BORDER = 15;          // modules between the symbol's edge and the
 constexpr int ALIGN_TURNS = 4096;   // steps of the pilots' turn table
 constexpr int MAX_NODES = 40;       // node lattice points a side (layout.h OB_MAX_NODES)
 constexpr int FIT_ROWS = 4;         // rows of the shading fit's lattice (every eighth sample) a fit workgroup sums
-constexpr int FIT_STRIDE = 6 * 1536 / 8 / FIT_ROWS;  // doubles of fit partial sums a frame (6 per part, n <= 1536)
+constexpr int FIT_BETA = 6 * 1536 / 8 / FIT_ROWS;    // the fit's partial sums a frame (6 per part, n <= 1536), then
+constexpr int FIT_STRIDE = FIT_BETA + 8;              // the frame's shading beta (6) at FIT_BETA
 
 // One frame's registration and capture for the warp: H maps module coordinates to capture pixels (h8 = 1), nodes
 // the node lattice's displacements in pixels (ny rows of nx), img a tight uint8 luma capture of w x h.
@@ -46,6 +47,7 @@ struct PlanDev {
   // the warp (picture sample x or y -> module coordinate and node cell; the same table on both axes)
   int nx, ny;
   float node_x[MAX_NODES], node_y[MAX_NODES];
+  float node_inv[MAX_NODES];               // 1 / (node_x[i + 1] - node_x[i]), rounded once (IEEE division)
   const float *mcoord;                     // n: module coordinate of sample index t
   const int32_t *mcell;                    // n: node cell of sample index t (the same on both axes)
   int cell0[MAX_NODES];                    // first sample index of node cell i (cell0[nx - 1] = n)
