@@ -85,6 +85,19 @@ def lds_bytes_per_iteration(rate):
     return 6 * lizard_amd.ldpc_dims(rate)["edges"]
 
 
+def measured_traffic():
+    """HBM bytes a k_ldpc launch moves, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel
+    calibrated); None where that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_ldpc_traffic_ldpc_batched_4096.txt")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def effective_iterations(its):
     its = its.astype(np.int64)
     return np.where(its > 0, its, MAX_ITER)
@@ -545,7 +558,7 @@ def main():
         "batched_over_plain": round(gbps / plain, 2) if plain else None,
         "points": points,
         "roofline": {"bound": "lds", "kernel": "k_ldpc", "achieved": round(lds_gbs, 1), "peak": LDS_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "launch_ms": round(launch_ms, 4), "lds_bytes_per_launch": lds_bytes,
                      "iterations_per_s": round(sum(float(effective_iterations(its_all[s % len(EBN0)]).sum())
                                                    for s in range(args.steps)) / (ev_ms * 1e-3), 1),

@@ -74,7 +74,7 @@ struct liz_decoder {
   WarpFrame wframe_host{};
   std::vector<uint8_t> luma;
   // registration refinement
-  DevBuf reg_ref, reg_part, reg_state, reg_trace;
+  DevBuf reg_ref, reg_part, reg_sum, reg_state, reg_trace;
   std::vector<float> reg_nodes, reg_part_host;
   RegState reg_state_host{};
   // the held registration for liz_decode
@@ -676,10 +676,12 @@ static const WarpFrame *stage_refinement(liz_decoder *d, const FramePlan &p, con
   reg.nodes = d->reg_nodes.data();
   const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, &reg);
   LdpcEngine &e = *d->eng;
-  const size_t nn = static_cast<size_t>(p.n) * p.n, cells = static_cast<size_t>(p.dev.nx - 1) * (p.dev.nx - 1);
+  const size_t nn = static_cast<size_t>(p.n) * p.n;
+  const size_t cells = static_cast<size_t>(p.dev.nx - 1) * (p.dev.nx - 1) * REG_SPLIT;
   d->reg_ref.reserve(nn * sizeof(float));
   hip_check(hipMemcpyAsync(d->reg_ref.p, ref, nn * sizeof(float), hipMemcpyHostToDevice, e.stream), "reference picture");
   d->reg_part.reserve(cells * REG_PART * sizeof(float));
+  d->reg_sum.reserve(REG_PART * sizeof(double));
   d->reg_state.reserve(sizeof(RegState));
   return wf;
 }
@@ -715,16 +717,17 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
     hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
     launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
     const int nx = p.dev.nx, c = nx - 1;
-    d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_PART);
+    d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_SPLIT * REG_PART);
     hip_check(hipMemcpyAsync(d->reg_part_host.data(), d->reg_part.p, d->reg_part_host.size() * sizeof(float),
                              hipMemcpyDeviceToHost, e.stream), "partial sums");
     hip_check(hipStreamSynchronize(e.stream), "refinement pass");
-    // the cells' sums in FP64, in cell order; the nodes' gradient from the corners of the cells about them
+    // the pieces' sums in FP64, in piece order; the nodes' gradient from the corners of the cells about them
     double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<double> G(static_cast<size_t>(nx) * nx * 2, 0.0);
-    for (int cj = 0; cj < c; cj++)
+    for (int row = 0; row < c * REG_SPLIT; row++)
       for (int ci = 0; ci < c; ci++) {
-        const float *q = &d->reg_part_host[(static_cast<size_t>(cj) * c + ci) * REG_PART];
+        const int cj = row / REG_SPLIT;
+        const float *q = &d->reg_part_host[(static_cast<size_t>(row) * c + ci) * REG_PART];
         L += q[0];
         for (int k = 0; k < 8; k++) gH[k] += q[1 + k];
         const int corner[4] = {cj * nx + ci, cj * nx + ci + 1, (cj + 1) * nx + ci, (cj + 1) * nx + ci + 1};
@@ -759,8 +762,8 @@ LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t
     // pass 0 fits the intensity model only; passes 1 .. steps move H; the last pass measures the result
     for (int s = 0; s <= steps + 1; s++) {
       launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
-      launch_gn_step(p, d->reg_part.as<float>(), d->reg_state.as<RegState>(), 1e-6f, s >= 1 && s <= steps,
-                     d->reg_trace.as<float>(), s, e.stream);
+      launch_gn_step(p, d->reg_part.as<float>(), d->reg_sum.as<double>(), d->reg_state.as<RegState>(), 1e-6f,
+                     s >= 1 && s <= steps, d->reg_trace.as<float>(), s, e.stream);
     }
     hip_check(hipMemcpyAsync(&st, d->reg_state.p, sizeof st, hipMemcpyDeviceToHost, e.stream), "state");
     std::vector<float> tr(steps + 2);

@@ -13,7 +13,8 @@ namespace lz {
 
 constexpr int REG_ANCH = 7;                                 // anchors a side (SPEC 6.13)
 constexpr int REG_PARAMS = 8 + 1 + 2 * REG_ANCH * REG_ANCH; // homography, k1, anchors' (dx, dy)
-constexpr int REG_PART = 58;                                // floats of partial sums a node cell
+constexpr int REG_PART = 58;                                // floats of partial sums a piece of a node cell
+constexpr int REG_SPLIT = 1;                                // pieces a node cell's rows are split into
 
 // The node lattice's displacements as a linear function of k1 and the anchors for one ring, frame size and reference
 // homography H0: nodes = k1 lens + sum over anchors of (dx, dy) phi.
@@ -35,14 +36,14 @@ struct RegState {
   float pad[5];
 };
 
-// Per node cell (grid (nx - 1) x (nx - 1)): sums of e^2, J^T e (8), J^T J (36, upper triangle), the intensity
+// Per piece of a node cell (grid (nx - 1) x (nx - 1) REG_SPLIT, pieces (cj REG_SPLIT + piece) (nx - 1) + ci): sums of e^2, J^T e (8), J^T J (36, upper triangle), the intensity
 // statistics (sum w, sum r, sum w r, sum r r, count) and e (gx, gy) spread to the cell's four corners (8), e = w -
 // alpha r - beta the residual of the warped capture w against the reference picture r.
 void launch_reg_accum(const FramePlan &p, const WarpFrame *d_frame, const RegState *d_state, const float *d_ref,
                       float *d_part, hipStream_t s);
 // Sums the cells' parts in FP64 and, when update, solves the damped 8 x 8 normal equations in FP64 and moves H; sets
 // alpha and beta to the least-squares intensity fit of this pass; trace[step] = the pass's loss.
-void launch_gn_step(const FramePlan &p, const float *d_part, RegState *d_state, float damping, int update, float *d_trace,
-                    int step, hipStream_t s);
+void launch_gn_step(const FramePlan &p, const float *d_part, double *d_sum, RegState *d_state, float damping, int update,
+                    float *d_trace, int step, hipStream_t s);
 
 }  // namespace lz
