@@ -4,7 +4,7 @@
 // codeword: lane i owns check position i of every block row. Posteriors (int16) stay in LDS.
 //
 // The arithmetic is the reference decoder's scalar form (liblizard/src/ldpc.c:171-191, :589-617, :681-705,
-// :1080-1172; research/SPEC.md 8.5)  Shared by the
+// :1080-1172; research/SPEC.md 8.5). Shared by the
 // gfx950 kernel and a host-side emulation (tests walk the 128 lanes on a CPU).
 #pragma once
 #include <stdint.h>
@@ -73,8 +73,8 @@ __host__ __device__ LZ_INLINE void put_msg(uint32_t *w, int v) {
   w[S >> 2] = (w[S >> 2] & ~(0xFFu << sh)) | ((static_cast<uint32_t>(v) & 0xFFu) << sh);
 }
 
-// Byte offset of posterior (i - sh) mod 128 of a 128-entry block from 2 i: one add whose low byte is the answer (the
-// compiler folds the mask into the add's byte select).
+// Byte offset of posterior (i - sh) mod 128 of a 128-entry block from 2 i: an add and a mask (the block's own offset
+// goes into the LDS instruction's immediate).
 __host__ __device__ LZ_INLINE int rot_off(int i2, int sh) { return (i2 + (256 - 2 * sh)) & 0xFF; }
 
 __host__ __device__ LZ_INLINE int16_t &at_byte(int16_t *base, int off) {
