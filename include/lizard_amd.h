@@ -116,6 +116,7 @@ typedef struct {
   float *coef;      /* 16 * version * 320 coefficients (re, im), coefficient order, after the detrend, before the
                        pilots' turn (lizard_debug's coefficients, lizard_phy.c:1719) */
   float align[2];   /* out: the pilots' grid shift in samples along u and v, or 0, 0 (lizard_align) */
+  float *blk_pilot; /* per block: the pilot tail's mean of +-y / rms (lizard_phy.c:1795-1809), NaN where none */
 } lizx_soft;
 
 /* lizard_finish with the caller's grid (liblizard/src/lizard_phy.c:1566 with grid, ext_llr NULL; lizard_phy.h:

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -526,8 +527,11 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     std::vector<int8_t> its(B);
     std::vector<float> est(B);
     std::vector<uint8_t> all(static_cast<size_t>(B) * BLOCK_BYTES);
+    std::vector<float> pil(B);
+    lizx_soft sp{};
+    sp.blk_pilot = pil.data();
     decode_capture(d, d->held_version, d->held_span, img, w, h, istride, &d->held_reg, all.data(), d->ok.data(),
-                   its.data(), est.data(), nullptr);
+                   its.data(), est.data(), &sp);
     int k = 0;
     const int total = std::min(B, d->top);
     for (int b = 0; b < total; b++)
@@ -542,8 +546,27 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
       o.held_used = 1;
       o.total = B;
       o.verified = k;
+      // the pilots over the even blocks and the odd (bits 0 and 1 of the picture's count): each parity's mean reading
+      // and its standard error, NaN where no block of it read (lizard_phy.c:451-472 lizard_pilot over every block)
       o.pilot_blocks = 0;
-      o.pilot_r[0] = o.pilot_r[1] = o.pilot_sd[0] = o.pilot_sd[1] = __builtin_nanf("");
+      for (int g = 0; g < 2; g++) {
+        double s1 = 0, s2 = 0;
+        int c = 0;
+        for (int b = g; b < B; b += 2)
+          if (pil[b] == pil[b]) {
+            s1 += pil[b];
+            s2 += static_cast<double>(pil[b]) * pil[b];
+            c++;
+          }
+        if (!c) {
+          o.pilot_r[g] = o.pilot_sd[g] = __builtin_nanf("");
+          continue;
+        }
+        const double mean = s1 / c, var = c > 1 ? (s2 - s1 * mean) / (c - 1) : 0;
+        o.pilot_r[g] = static_cast<float>(mean);
+        o.pilot_sd[g] = static_cast<float>(std::sqrt((var > 0 ? var : 0) / c));
+        o.pilot_blocks += c;
+      }
       *out = o;
     }
     if (!*held) *held = d->held_version;
@@ -605,6 +628,9 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
     if (soft->coef)
       hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
     hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
+    if (soft->blk_pilot)
+      hip_check(hipMemcpy(soft->blk_pilot, d->fb.pilot_r.p, p.dev.blocks * sizeof(float), hipMemcpyDeviceToHost),
+                "pilot readings");
   }
   return collect_blocks(d, f, blocks, ok, its, blk_est);
 }
@@ -826,6 +852,9 @@ LIZ_API int lizx_finish_grid(liz_decoder *d, int version, int span, const float 
       if (soft->coef)
         hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
       hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
+      if (soft->blk_pilot)
+        hip_check(hipMemcpy(soft->blk_pilot, d->fb.pilot_r.p, p.dev.blocks * sizeof(float), hipMemcpyDeviceToHost),
+                  "pilot readings");
     }
     return collect_blocks(d, f, blocks, ok, its, blk_est);
   });

@@ -219,6 +219,7 @@ LIZ_API int lizx_batch_plan(lizx_batch *b, int frames, const int *version, const
       const int cnt = static_cast<int>(g->frames.size());
       g->fb.reserve(*g->plan, cnt, false);
       g->view = g->fb.view(false);
+      g->view.blk_pilot = nullptr;  // a batch reports blocks, not pilot readings
       g->view.llr = b->llr.as<int8_t>() + b->llr_off[g->frames[0]];
       g->view.est = b->est.as<double>() + b->est_off[g->frames[0]];
     }

@@ -710,6 +710,38 @@ __global__ __launch_bounds__(256) void k_demap(PlanDev P, const float2 *HP_all, 
   }
 }
 
+// Block b's pilot reading (lizard_phy.c:1795-1809; SPEC 7.3): over the slots past its codeword, the axis value
+// against the whitening's sign in units of its sub-channel's rms axis value, averaged over the tail; NaN where the
+// block has no tail or no sub-channel of it read anything. One wave a block (tails are 16 or 32 slots).
+__global__ __launch_bounds__(64) void k_pilot_read(PlanDev P, const float2 *HP_all, const double *proj,
+                                                   const float *align, const float *rms_all, float *out) {
+  const int b = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+  const int4 bk = P.blk[b];
+  const int s0 = bk.x, nt = bk.y, nb = bk.z;
+  const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
+  const Proj pr{proj[f * 5], proj[f * 5 + 1], proj[f * 5 + 2], proj[f * 5 + 3], proj[f * 5 + 4]};
+  const float dx = align[2 * f], dy = align[2 * f + 1];
+  double cs = 0;
+  int got = 0;
+  for (int i = nt + lane; i < nb; i += 64) {
+    const int gs = s0 + i;
+    const float rms = rms_all[f * P.subch + gs / (2 * SUB)];
+    if (!(rms > 0)) continue;
+    int u, v;
+    coef_uv(P, gs >> 1, &u, &v);
+    float2 y = coef_at(P, HP, pr, u, v);
+    if (dx != 0 || dy != 0) y = turn(P, y, u, v, dx, dy);
+    const float a = (i & 1) ? y.y : y.x;
+    cs += (P.white[gs] ? -1.0 : 1.0) * a / rms;
+    got++;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    cs += __shfl_xor(cs, o, 64);
+    got += __shfl_xor(got, o, 64);
+  }
+  if (lane == 0) out[f * P.blocks + b] = got ? static_cast<float>(cs / (nb - nt)) : __builtin_nanf("");
+}
+
 // ------------------------------------------------------------------------------------------------ host: plans
 // 6 x 6 inverse by Gauss-Jordan with partial pivots.
 static void invert6(double g[6][12]) {
@@ -827,6 +859,12 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
     }
   }
   d.npilots = static_cast<int>(p->pilot_host.size());
+  std::vector<int4> blk(f->blocks);
+  for (int b = 0; b < f->blocks; b++)
+    blk[b] = make_int4(f->blk_slot[b], code(f->blk_code[b]).nt, f->blk_nslots[b], 0);
+  p->blk.reserve(blk.size() * sizeof(int4));
+  hip_check(hipMemcpy(p->blk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice), "blocks");
+  d.blk = p->blk.as<int4>();
   // the warp's sample grid: sample t at module g0 + t step (lizard_phy.c:1592; SPEC 6.9), its node cell found as the
   // reference's walk finds it (acquire.c:5111-5119)
   d.nx = d.ny = node_lattice(span, d.node_x);
@@ -912,6 +950,7 @@ void FrameBufs::reserve(const FramePlan &p, int frames, bool debug) {
   est.reserve(frames * p.subch * sizeof(double));
   rms.reserve(frames * p.subch * sizeof(float));
   if (debug) coef_dbg.reserve(frames * S * sizeof(float2));
+  pilot_r.reserve(frames * p.dev.blocks * sizeof(float));
 }
 
 SoftBufs FrameBufs::view(bool debug) {
@@ -926,6 +965,7 @@ SoftBufs FrameBufs::view(bool debug) {
   v.est = est.as<double>();
   v.rms = rms.as<float>();
   v.coef_dbg = debug ? coef_dbg.as<float>() : nullptr;
+  v.blk_pilot = pilot_r.p ? pilot_r.as<float>() : nullptr;
   return v;
 }
 
@@ -942,6 +982,7 @@ SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
   v.est = est + F * p.subch;
   v.rms = rms + F * p.subch;
   v.coef_dbg = coef_dbg ? coef_dbg + F * 2 * S : nullptr;
+  v.blk_pilot = blk_pilot ? blk_pilot + F * p.dev.blocks : nullptr;
   return v;
 }
 
@@ -960,6 +1001,9 @@ static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int 
   hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.HP, b.proj, b.align);
   hipLaunchKernelGGL(k_demap, dim3((p.subch + 3) / 4, frames), dim3(256), 0, s, p.dev, b.HP, b.proj, b.align, b.llr,
                      b.est, b.rms, b.coef_dbg);
+  if (b.blk_pilot)
+    hipLaunchKernelGGL(k_pilot_read, dim3(p.dev.blocks, frames), dim3(64), 0, s, p.dev, b.HP, b.proj, b.align, b.rms,
+                       b.blk_pilot);
   hip_check(hipGetLastError(), "soft-value launches");
 }
 

@@ -51,13 +51,14 @@ struct PlanDev {
   const float *mcoord;                     // n: module coordinate of sample index t
   const int32_t *mcell;                    // n: node cell of sample index t (the same on both axes)
   int cell0[MAX_NODES];                    // first sample index of node cell i (cell0[nx - 1] = n)
+  const int4 *blk;                         // per block: first slot, code's sent bits nt, slots nb, unused
 };
 
 struct FramePlan {
   int version = 0, span = 0, n = 0, subch = 0, E = 0;
   const Format *fmt = nullptr;
   PlanDev dev{};
-  DevBuf tables, coef, pilot, tw, rot, white, mtab;
+  DevBuf tables, coef, pilot, tw, rot, white, mtab, blk;
   std::vector<int> pilot_host;  // coefficient indices of the pilots, in block order
   std::vector<int> pilot_block;
 };
@@ -75,6 +76,7 @@ struct SoftBufs {
   double *est = nullptr;
   float *rms = nullptr;
   float *coef_dbg = nullptr;  // null: no debug copy of the coefficients
+  float *blk_pilot = nullptr; // null: no pilot readings
   SoftBufs at(const FramePlan &p, int f) const;  // frame f's slice
 };
 

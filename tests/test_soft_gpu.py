@@ -4,6 +4,8 @@ Tolerances (north star): coefficients within 1e-5 relative L2 of FP64; LLR float
 values may differ by one at rounding ties (a small fraction); every block's verdict, iterations and bytes equal the
 reference's where the soft values agree."""
 import numpy as np
+
+import lizard_amd as L
 import pytest
 
 from _libs import ptr, ref
@@ -42,6 +44,11 @@ def test_finish_grid(decoder_gpu, version, span, shift, snr):
     good = r.ref_finish_grid(fr, ptr(strip), ptr(bl), ptr(ok), ptr(its), ptr(be), ptr(bp), ptr(llr), ptr(est),
                              ptr(rms), ptr(al), ptr(coef))
     r.ref_format_free(fr)
+    # the blocks' pilot readings: NaN on the same blocks (no tail), the same values within float rounding
+    gp = out["blk_pilot"]
+    assert np.array_equal(np.isnan(gp), np.isnan(bp))
+    v = ~np.isnan(bp)
+    assert np.allclose(gp[v], bp[v], rtol=1e-4, atol=1e-4), np.abs(gp[v] - bp[v]).max()
     cr = coef[0::2] + 1j * coef[1::2]
     assert np.linalg.norm(c - cr) / np.linalg.norm(cr) < 1e-5
     dr = np.abs(out["llr"].astype(int) - llr.astype(int))
@@ -54,6 +61,21 @@ def test_finish_grid(decoder_gpu, version, span, shift, snr):
     for b in range(B):
         if out["ok"][b]:
             assert np.array_equal(out["blocks"][b], F.blocks.reshape(B, 476)[b])
+
+
+@pytest.mark.parametrize("cnt", [0, 1, 2, 3])
+def test_pilot_readings_carry_the_count(decoder_gpu, cnt):
+    """On a clean frame each block's pilot tail reads +1 where its bit of the painted count is 0 and -1 where it is 1
+    (bit 0 on even blocks, bit 1 on odd; SPEC 7.3); 2/3 and 1/2 blocks have no tail and read NaN."""
+    F = Frame(20, 256, seed=9 + cnt, snr_db=40, shift=(0.2, -0.1), ramp=0.0, junk=False, cnt=cnt)
+    out = decoder_gpu.finish_grid(20, 256, F.grid)
+    rate = L.format_blocks(20)[0]
+    for b, pr in enumerate(out["blk_pilot"]):
+        if rate[b] in (2, 3):
+            assert np.isnan(pr)
+        else:
+            want = -1.0 if (cnt >> (b & 1)) & 1 else 1.0
+            assert abs(pr - want) < 0.15, (b, pr, want)
 
 
 def test_clean_frame_all_verify(decoder_gpu):

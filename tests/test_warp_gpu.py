@@ -96,6 +96,11 @@ def test_liz_decode_held(fmt):
     dec.close()
     assert k == C.blocks_n and out.verified == k and out.total == C.blocks_n and held == 5
     assert np.array_equal(blocks.ravel(), C.blocks)
+    # the pilots (painted count 0): both parities read near +1 over the 7/8 and 3/4 blocks (lizard_pilot)
+    rates = lizard_amd.format_blocks(5)[0]
+    assert out.pilot_blocks == int(np.sum((rates == 6) | (rates == 4)))
+    assert out.pilot_r[0] > 0.5 and out.pilot_r[1] > 0.5
+    assert 0 <= out.pilot_sd[0] < 0.2 and 0 <= out.pilot_sd[1] < 0.2
 
 
 def test_liz_decode_without_registration_is_state_error():
