@@ -205,6 +205,9 @@ def bench_pipeline(args, caps):
         items.append(dict(img=d_img, w=c["img"].shape[1], h=c["img"].shape[0], H=c["H"], nodes=d_nodes))
     frames = lizard_amd.make_frames(items)
     b = lizard_amd.Batch(0).plan([64] * F, [512] * F)
+    # the default launch shape's graph time, then the autotuner's pick (the bits are the same whatever the shape)
+    _, default_ms = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
+    shape = b.tune(frames, reps=3)
     res = {}
     outs = {}
     for name, mode, runs in (("graph", lizard_amd.RUN_GRAPH, args.pipe_runs),
@@ -233,6 +236,9 @@ def bench_pipeline(args, caps):
         "modes": res,
         "graph_over_plain": round(res["graph"]["frames_per_s"] / res["plain"]["frames_per_s"], 2),
         "graph_over_frames": round(res["graph"]["frames_per_s"] / res["frames"]["frames_per_s"], 2),
+        "autotune": {"shape": {"rows_waves": shape[0], "cols_waves": shape[1], "demap_waves": shape[2],
+                               "ldpc_wg_per_cu": shape[3] or "max"},
+                     "default_shape_gpu_ms_per_batch": round(default_ms, 3)},
     }
     b.close()
     for d in keep:
@@ -260,6 +266,8 @@ def bench_fft_demap(args):
         total += sb["rows"] + sb["cols"] + sb["demap"]
     frames = lizard_amd.make_frames(items)
     b = lizard_amd.Batch(0).plan(versions, spans, source=lizard_amd.SRC_GRID, stages=lizard_amd.STAGE_SOFT)
+    _, default_ms = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
+    shape = b.tune(frames, reps=3)
     wall, ev = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
     fwall, fev = run_batch(b, frames, lizard_amd.RUN_FRAMES, max(1, args.pipe_runs // 2))
     gbs = total / (ev * 1e-3) / 1e9
@@ -270,6 +278,8 @@ def bench_fft_demap(args):
                         "AWGN 20 dB; erase, detrend, 2D FFT, pilot shift, soft demap" %
                         (len(ns), ", ".join("%d x %d" % (int((ns == n).sum()), n) for n in sorted(table))),
             "graph_ms_per_batch": round(ev, 3), "frames_per_s": round(len(ns) / wall, 1),
+            "autotune": {"shape": {"rows_waves": shape[0], "cols_waves": shape[1], "demap_waves": shape[2]},
+                         "default_shape_graph_ms_per_batch": round(default_ms, 3)},
             "frame_by_frame_ms_per_batch": round(fev, 3),
             "algorithmic_bytes_per_batch": int(total),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -246,6 +246,13 @@ LIZ_API int lizx_batch_dims(const lizx_batch *b, int *dims);
 /* Enqueues one run over frames[0 .. frames-1] (device-resident inputs) in a LIZX_RUN_* mode. Asynchronous except
    LIZX_RUN_FRAMES and LIZX_RUN_PLAIN, which wait per frame. */
 LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode);
+/* The autotuner: times each launch shape (waves a workgroup of the row FFT, column FFT and demap: 1, 2 or 4; the
+   persistent LDPC grid's workgroups a CU: as many as fit, 2, 4, 6 or 8) on these frames, reps runs a candidate, by
+   coordinate descent, and keeps the fastest. Shapes never change a result's bits. choice (optional, 4 ints) receives
+   the shape kept. The next LIZX_RUN_GRAPH run captures a new graph. */
+LIZ_API int lizx_batch_tune(lizx_batch *b, const lizx_frame *frames, int reps, int *choice);
+/* Sets a launch shape (4 ints, as lizx_batch_tune reports). */
+LIZ_API int lizx_batch_shape(lizx_batch *b, const int *choice);
 /* Waits for the runs; *ms (optional) receives the GPU time from the first run's start since the last sync to the last
    run's end. */
 LIZ_API int lizx_batch_sync(lizx_batch *b, float *ms);

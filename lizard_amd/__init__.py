@@ -97,6 +97,8 @@ def lib():
     sig("lizx_batch_dims", _I, _P, _P)
     sig("lizx_batch_run", _I, _P, _P, _I)
     sig("lizx_batch_sync", _I, _P, C.POINTER(C.c_float))
+    sig("lizx_batch_tune", _I, _P, _P, _I, _P)
+    sig("lizx_batch_shape", _I, _P, _P)
     sig("lizx_batch_blocks", _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_soft", _I, _P, _I, _P, _P, _P, _P)
     sig("lizx_ldpc_dims", _I, _I, _P)
@@ -481,6 +483,15 @@ class Batch:
         ms = C.c_float(0)
         check(lib().lizx_batch_sync(self.h, C.byref(ms)))
         return ms.value
+
+    def tune(self, frames, reps=3):
+        """The autotuner (lizx_batch_tune): the launch shape kept, as (rows, cols, demap waves, LDPC per CU)."""
+        ch = np.zeros(4, np.int32)
+        check(lib().lizx_batch_tune(self.h, frames, reps, _p(ch)))
+        return tuple(int(x) for x in ch)
+
+    def shape(self, choice):
+        check(lib().lizx_batch_shape(self.h, _p(np.ascontiguousarray(choice, np.int32))))
 
     def blocks_out(self):
         B = self.blocks

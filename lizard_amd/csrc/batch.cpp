@@ -44,6 +44,10 @@ struct lizx_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<uint8_t> out_bytes;
+  // launch shapes (the autotuner's choices): waves a workgroup of the row FFT, column FFT and demap; the persistent
+  // LDPC grid's workgroups a CU (0: as many as fit)
+  LaunchShape shape{};
+  int ldpc_per_cu = 0;
   // one more stream a format beyond the first, so the formats' soft-value launches overlap (default priority)
   std::vector<hipStream_t> side;
   std::vector<hipEvent_t> join;
@@ -99,9 +103,9 @@ struct lizx_batch {
     const FramePlan &p = *g.plan;
     const SoftBufs v = g.view.at(p, i0);
     if (source == LIZX_SRC_GRID)
-      launch_soft_grids(p, v, grids.as<const float *>() + g.first + i0, count, s);
+      launch_soft_grids(p, v, grids.as<const float *>() + g.first + i0, count, s, shape);
     else
-      launch_soft_warp(p, v, wframes.as<WarpFrame>() + g.first + i0, count, s);
+      launch_soft_warp(p, v, wframes.as<WarpFrame>() + g.first + i0, count, s, shape);
   }
 
   // The whole batch's launches, in stream order (what the graph holds).
@@ -123,7 +127,7 @@ struct lizx_batch {
       LdpcParams p = ldpc_params();
       p.tasks = tasks.as<LdpcTask>();
       p.count = blocks;
-      eng->launch(p, false, timed);
+      eng->launch(p, false, timed, false, ldpc_per_cu);
     }
   }
 
@@ -135,7 +139,7 @@ struct lizx_batch {
       LdpcParams p = ldpc_params();
       p.tasks = frame_tasks.as<LdpcTask>() + frame_task0[f];
       p.count = frame_task0[f + 1] - frame_task0[f];
-      eng->launch(p, plain, true);
+      eng->launch(p, plain, true, false, ldpc_per_cu);
     }
   }
 };
@@ -343,6 +347,82 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
       }
     }
     hip_check(hipEventRecord(e.ev1, e.stream), "hipEventRecord");
+    return 0;
+  });
+}
+
+// The autotuner: coordinate descent over the launch shapes, each candidate timed on the batch's own frames (the
+// runs' results are the same bits whatever the shape); the best shape is kept and the graph re-captured with it.
+LIZ_API int lizx_batch_tune(lizx_batch *b, const lizx_frame *frames, int reps, int *choice) {
+  return guard([&] {
+    need(b && b->frames > 0 && frames && reps > 0 && reps <= 100, "a planned batch, its frames and 1 to 100 repeats");
+    LdpcEngine &e = *b->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    auto time_it = [&]() {
+      // one run to settle, then reps timed ones, on the stream (no graph: the shape changes between candidates)
+      if (lizx_batch_run(b, frames, LIZX_RUN_STREAM)) throw Error(liz_last_error_code(), liz_last_error());
+      e.sync();
+      for (int r = 0; r < reps; r++)
+        if (lizx_batch_run(b, frames, LIZX_RUN_STREAM)) throw Error(liz_last_error_code(), liz_last_error());
+      e.sync();
+      return e.last_ms;
+    };
+    const int waves[3] = {1, 2, 4};
+    int *knobs[3] = {&b->shape.rows, &b->shape.cols, &b->shape.demap};
+    for (int k = 0; k < 3; k++) {
+      int best = *knobs[k];
+      float best_ms = time_it();
+      for (int w : waves) {
+        if (w == best) continue;
+        const int keep = *knobs[k];
+        *knobs[k] = w;
+        const float ms = time_it();
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = w;
+        }
+        *knobs[k] = keep;
+      }
+      *knobs[k] = best;
+    }
+    if (b->stages == LIZX_STAGE_BLOCKS) {
+      const int per_cu[5] = {0, 2, 4, 6, 8};
+      int best = b->ldpc_per_cu;
+      float best_ms = time_it();
+      for (int c : per_cu) {
+        if (c == best || (c > 0 && c > e.wg_per_cu)) continue;
+        const int keep = b->ldpc_per_cu;
+        b->ldpc_per_cu = c;
+        const float ms = time_it();
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = c;
+        }
+        b->ldpc_per_cu = keep;
+      }
+      b->ldpc_per_cu = best;
+    }
+    b->drop_graph();
+    if (choice) {
+      choice[0] = b->shape.rows;
+      choice[1] = b->shape.cols;
+      choice[2] = b->shape.demap;
+      choice[3] = b->ldpc_per_cu;
+    }
+    return 0;
+  });
+}
+
+LIZ_API int lizx_batch_shape(lizx_batch *b, const int *choice) {
+  return guard([&] {
+    need(b && choice, "a batch and a shape");
+    for (int k = 0; k < 3; k++) need(choice[k] == 1 || choice[k] == 2 || choice[k] == 4, "waves per workgroup 1, 2, 4");
+    need(choice[3] >= 0 && choice[3] <= 64, "LDPC workgroups a CU, 0 for as many as fit");
+    b->shape.rows = choice[0];
+    b->shape.cols = choice[1];
+    b->shape.demap = choice[2];
+    b->ldpc_per_cu = choice[3];
+    b->drop_graph();
     return 0;
   });
 }

@@ -242,7 +242,7 @@ void LdpcEngine::make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int6
   hip_check(hipGetLastError(), "task launch");
 }
 
-void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed, bool queue_zeroed) {
+void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed, bool queue_zeroed, int per_cu) {
   if (p0.count <= 0) return;
   LdpcParams p = p0;
   if (timed && !armed) {
@@ -261,7 +261,7 @@ void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed, bool queue
   } else {
     p.queue = queue.as<int>();
     if (!queue_zeroed) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, p.queue, 1);
-    const int grid = std::min(p.count, cus * wg_per_cu);
+    const int grid = std::min(p.count, cus * (per_cu > 0 ? std::min(per_cu, wg_per_cu) : wg_per_cu));
     hipLaunchKernelGGL(k_ldpc, dim3(grid), dim3(LDPC_THREADS), 0, stream, p);
   }
   hip_check(hipGetLastError(), "LDPC launch");

@@ -62,7 +62,8 @@ struct LdpcEngine {
   ~LdpcEngine();
   // Enqueue a decode of count tasks already on the device. Plain: one launch per task. Timed: HIP events bracket the
   // launches from the first since the last sync() to the last.
-  void launch(const LdpcParams &p, bool plain, bool timed, bool queue_zeroed = false);
+  // per_cu > 0 caps the persistent grid at that many workgroups a CU (a launch shape: the bits do not change).
+  void launch(const LdpcParams &p, bool plain, bool timed, bool queue_zeroed = false, int per_cu = 0);
   void make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count);
   void sync();
   void ensure_frame_tables(int max_slots);

@@ -436,19 +436,19 @@ __global__ __launch_bounds__(64) void k_fit_solve(PlanDev P, double *fit) {
 }
 
 // ------------------------------------------------------------------------------------------------ the two passes
-constexpr int ROWS = 8;  // picture rows a k_rows workgroup transforms: two a wave, packed as one complex sequence
+// k_rows: W waves a workgroup, two picture rows a wave packed as one complex sequence (2 W rows a workgroup)
 
 // Rows y and y + 1 of wave w: load with the anchors' samples erased, the detrend's row sums (s, s c(x), s q(x)) in
 // double, one wave FFT of row y + i row y + 1, the two real rows' transforms separated (Z(u) and conj Z(n - u)) for
 // u < E and written column-major, X[frame][u][y].
-template <class Src, int N>
-__global__ __launch_bounds__(256) void k_rows(PlanDev P, Src src, const double *fitpart, float2 *X, double *rowsum) {
+template <class Src, int N, int W>
+__global__ __launch_bounds__(64 * W) void k_rows(PlanDev P, Src src, const double *fitpart, float2 *X, double *rowsum) {
   using F = WaveFft<N>;
   constexpr int M = F::M;
-  extern __shared__ float2 zs[];  // 4 waves x F::LDS
+  extern __shared__ float2 zs[];  // W waves x F::LDS
   __shared__ double bsh[6];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y;
-  const int y = blockIdx.x * ROWS + 2 * w;
+  const int y = blockIdx.x * (2 * W) + 2 * w;
   if (tid < 6) bsh[tid] = fitpart[static_cast<size_t>(f) * FIT_STRIDE + FIT_BETA + tid];
   __syncthreads();
   const bool yin0 = in_square(P, y), yin1 = in_square(P, y + 1);
@@ -502,13 +502,13 @@ __global__ __launch_bounds__(256) void k_rows(PlanDev P, Src src, const double *
 }
 
 // Column u (one wave): Y(u, v) for v < E at [v], and Y(u, n - v) for 1 <= v < E at [E - 1 + v].
-template <int N>
-__global__ __launch_bounds__(256) void k_cols(PlanDev P, const float2 *X, float2 *HP) {
+template <int N, int WV>
+__global__ __launch_bounds__(64 * WV) void k_cols(PlanDev P, const float2 *X, float2 *HP) {
   using F = WaveFft<N>;
   constexpr int M = F::M;
-  extern __shared__ float2 zs[];  // 4 waves x F::LDS
+  extern __shared__ float2 zs[];  // WV waves x F::LDS
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E, W = 2 * E;
-  const int u = blockIdx.x * 4 + w;
+  const int u = blockIdx.x * WV + w;
   if (u >= E) return;  // the whole wave; nothing below waits on the workgroup
   const float2 *col = X + (static_cast<size_t>(f) * E + u) * N;
   float2 v[M];
@@ -658,12 +658,14 @@ __device__ __forceinline__ double axis_info(double g) {
   return g > 0 ? pow(1 - pow(2.0, -0.3073 * pow(2 * sqrt(g), 1.7870)), 1.1064) : 0;
 }
 
-// Sub-channel j of frame f (lizard_phy.c:1756-1772, :1322-1332; SPEC 8.5), one wave a sub-channel (four a
+// Sub-channel j of frame f (lizard_phy.c:1756-1772, :1322-1332; SPEC 8.5), one wave a sub-channel (WV a
 // workgroup): each lane's five coefficients gathered, detrended and turned, the moments m2, m4 summed in double
 // across the wave by shuffles (every lane ends with the totals, so the scale needs no broadcast and no barrier).
-__global__ __launch_bounds__(256) void k_demap(PlanDev P, const float2 *HP_all, const double *proj, const float *align,
-                                               int8_t *llr_all, double *est_all, float *rms_all, float *dbg_all) {
-  const int lane = threadIdx.x & 63, j = blockIdx.x * 4 + (threadIdx.x >> 6), f = blockIdx.y;
+template <int WV>
+__global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *HP_all, const double *proj,
+                                                   const float *align, int8_t *llr_all, double *est_all, float *rms_all,
+                                                   float *dbg_all) {
+  const int lane = threadIdx.x & 63, j = blockIdx.x * WV + (threadIdx.x >> 6), f = blockIdx.y;
   if (j >= P.subch) return;  // the whole wave
   const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
   const Proj pr{proj[f * 5], proj[f * 5 + 1], proj[f * 5 + 2], proj[f * 5 + 3], proj[f * 5 + 4]};
@@ -986,21 +988,42 @@ SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
   return v;
 }
 
+// Calls f with std::integral_constant<int, w> for the waves-per-workgroup choices the launches are built for.
+template <class F>
+void with_waves(int w, F &&f) {
+  switch (w) {
+    case 1: f(std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    default: throw Error(LIZ_E_ARG, "waves per workgroup: 1, 2 or 4");
+  }
+}
+
 template <class Src>
-static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s) {
+static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s,
+                            const LaunchShape &sh) {
   const int n = p.n, E = p.E;
   hipLaunchKernelGGL(k_erase_fit<Src>, dim3(n / 8 / FIT_ROWS, frames), dim3(256), 0, s, p.dev, src, b.beta);
   hipLaunchKernelGGL(k_fit_solve, dim3(frames), dim3(64), 0, s, p.dev, b.beta);
   with_size(n, [&](auto N) {
     constexpr int NN = decltype(N)::value;
-    const size_t lds = 4 * WaveFft<NN>::LDS * sizeof(float2);
-    hipLaunchKernelGGL((k_rows<Src, NN>), dim3(NN / ROWS, frames), dim3(256), lds, s, p.dev, src, b.beta, b.X,
-                       b.rowsum);
-    hipLaunchKernelGGL(k_cols<NN>, dim3((E + 3) / 4, frames), dim3(256), lds, s, p.dev, b.X, b.HP);
+    with_waves(sh.rows, [&](auto Wr) {
+      constexpr int W = decltype(Wr)::value;
+      hipLaunchKernelGGL((k_rows<Src, NN, W>), dim3(NN / (2 * W), frames), dim3(64 * W),
+                         W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, src, b.beta, b.X, b.rowsum);
+    });
+    with_waves(sh.cols, [&](auto Wc) {
+      constexpr int W = decltype(Wc)::value;
+      hipLaunchKernelGGL((k_cols<NN, W>), dim3((E + W - 1) / W, frames), dim3(64 * W),
+                         W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, b.X, b.HP);
+    });
   });
   hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.HP, b.proj, b.align);
-  hipLaunchKernelGGL(k_demap, dim3((p.subch + 3) / 4, frames), dim3(256), 0, s, p.dev, b.HP, b.proj, b.align, b.llr,
-                     b.est, b.rms, b.coef_dbg);
+  with_waves(sh.demap, [&](auto Wd) {
+    constexpr int W = decltype(Wd)::value;
+    hipLaunchKernelGGL(k_demap<W>, dim3((p.subch + W - 1) / W, frames), dim3(64 * W), 0, s, p.dev, b.HP, b.proj,
+                       b.align, b.llr, b.est, b.rms, b.coef_dbg);
+  });
   if (b.blk_pilot)
     hipLaunchKernelGGL(k_pilot_read, dim3(p.dev.blocks, frames), dim3(64), 0, s, p.dev, b.HP, b.proj, b.align, b.rms,
                        b.blk_pilot);
@@ -1008,15 +1031,17 @@ static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int 
 }
 
 void launch_soft(const FramePlan &p, const SoftBufs &b, const float *d_grid, int frames, hipStream_t s) {
-  launch_soft_src(p, b, GridSource{d_grid}, frames, s);
+  launch_soft_src(p, b, GridSource{d_grid}, frames, s, LaunchShape{});
 }
 
-void launch_soft_grids(const FramePlan &p, const SoftBufs &b, const float *const *d_grids, int frames, hipStream_t s) {
-  launch_soft_src(p, b, GridListSource{d_grids}, frames, s);
+void launch_soft_grids(const FramePlan &p, const SoftBufs &b, const float *const *d_grids, int frames, hipStream_t s,
+                       const LaunchShape &sh) {
+  launch_soft_src(p, b, GridListSource{d_grids}, frames, s, sh);
 }
 
-void launch_soft_warp(const FramePlan &p, const SoftBufs &b, const WarpFrame *d_frames, int frames, hipStream_t s) {
-  launch_soft_src(p, b, WarpSource{d_frames}, frames, s);
+void launch_soft_warp(const FramePlan &p, const SoftBufs &b, const WarpFrame *d_frames, int frames, hipStream_t s,
+                      const LaunchShape &sh) {
+  launch_soft_src(p, b, WarpSource{d_frames}, frames, s, sh);
 }
 
 void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, float *d_grid, hipStream_t s) {

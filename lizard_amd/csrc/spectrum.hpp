@@ -87,12 +87,20 @@ struct FrameBufs {
   SoftBufs view(bool debug);
 };
 
+// Waves a workgroup of the row FFT, the column FFT and the demap (1, 2 or 4): launch shapes that leave every
+// result's bits unchanged, so an autotuner may pick them by timing.
+struct LaunchShape {
+  int rows = 4, cols = 4, demap = 4;
+};
+
 // Enqueues grid (row-major n x n float per frame, frames back to back) -> soft values, estimates, pilot shift.
 void launch_soft(const FramePlan &p, const SoftBufs &b, const float *d_grid, int frames, hipStream_t s);
 // The same with each frame's grid at its own address (d_grids: frames device pointers, on the device).
-void launch_soft_grids(const FramePlan &p, const SoftBufs &b, const float *const *d_grids, int frames, hipStream_t s);
+void launch_soft_grids(const FramePlan &p, const SoftBufs &b, const float *const *d_grids, int frames, hipStream_t s,
+                       const LaunchShape &sh = LaunchShape{});
 // The same from captures through their registrations (warp fused into the erase fit and the row transform).
-void launch_soft_warp(const FramePlan &p, const SoftBufs &b, const WarpFrame *d_frames, int frames, hipStream_t s);
+void launch_soft_warp(const FramePlan &p, const SoftBufs &b, const WarpFrame *d_frames, int frames, hipStream_t s,
+                      const LaunchShape &sh = LaunchShape{});
 inline void launch_soft(const FramePlan &p, FrameBufs &b, const float *d_grid, int frames, bool debug, hipStream_t s) {
   launch_soft(p, b.view(debug), d_grid, frames, s);
 }

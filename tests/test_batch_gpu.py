@@ -172,3 +172,30 @@ def test_batch_argument_errors():
     with pytest.raises(L.LizError):
         b.run(L.make_frames([{}]), 9)
     b.close()
+
+
+def test_launch_shapes_leave_the_bits_alone(caps):
+    """Every launch shape the autotuner may pick gives the same bits; the tuner keeps one of them."""
+    b = L.Batch().plan([c["version"] for c in caps], [c["span"] for c in caps])
+    frames, keep = device_frames(caps)
+    b.run(frames, L.RUN_GRAPH)
+    b.sync()
+    want = b.blocks_out()
+    soft0 = [b.soft(i, c["version"]) for i, c in enumerate(caps)]
+    for shape in ([1, 4, 4, 0], [2, 4, 4, 0], [4, 1, 4, 0], [4, 2, 4, 0], [4, 4, 1, 0], [4, 4, 2, 0], [4, 4, 4, 2],
+                  [1, 1, 1, 4]):
+        b.shape(shape)
+        b.run(frames, L.RUN_GRAPH)
+        b.sync()
+        same(b.blocks_out(), want)
+        for i, c in enumerate(caps):
+            s = b.soft(i, c["version"])
+            assert np.array_equal(s["llr"], soft0[i]["llr"]) and np.array_equal(s["sub_est"], soft0[i]["sub_est"])
+    choice = b.tune(frames, reps=2)
+    assert choice[0] in (1, 2, 4) and choice[1] in (1, 2, 4) and choice[2] in (1, 2, 4) and choice[3] in (0, 2, 4, 6, 8)
+    b.run(frames, L.RUN_GRAPH)
+    b.sync()
+    same(b.blocks_out(), want)
+    with pytest.raises(L.LizError):
+        b.shape([3, 4, 4, 0])
+    b.close()
