@@ -307,6 +307,61 @@ def refine_frames(count, seed):
     return out
 
 
+def bench_small(args):
+    """One n = 384 capture (LIZARD-80, 64 ring) in a 1280x720 frame: homography (5 deg, 0.02 perspective), blur 0.8
+    px, AWGN 4 grey; the full decode through liz_decode (held registration, host buffers in and out) against the
+    reference's decoder on one host core, both on the same capture and registration."""
+    from lizard_amd import sim
+    rng = np.random.default_rng(384)
+    cam = sim.Camera(1280, 720, rot_deg=5.0, scale=0.8, persp=0.02, k1=0.0, blur=0.8, noise=4.0)
+    blocks = sim.random_blocks(5, rng)
+    img, H, nodes = sim.capture(sim.picture(5, blocks), 128, cam, seed=1)
+    img = img.numpy()
+    nodes = np.ascontiguousarray(nodes)
+    dec = lizard_amd.Decoder(384)
+    dec.hold(5, 128, H, nodes)
+    for _ in range(5):
+        dec.decode(img)
+    t0 = time.perf_counter()
+    reps = 50
+    for _ in range(reps):
+        k, got, out, _ = dec.decode(img)
+    gpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+    dec.close()
+    res = {"workload": "one n=384 LIZARD-80 picture (9 blocks, 64 ring) in a 1280x720 capture: homography 5 deg, "
+                       "persp 0.02, blur 0.8 px, AWGN 4 grey; liz_decode through the held registration",
+           "verified": int(k), "blocks": 9, "bytes_are_sent": bool(np.array_equal(got.ravel(), blocks[:k * 476])),
+           "gpu_ms_per_frame": round(gpu_ms, 3), "pilot_r": [round(out.pilot_r[0], 3), round(out.pilot_r[1], 3)]}
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+    if os.path.exists(ref_so) and not args.no_cpu:
+        lib = C.CDLL(ref_so)
+        lib.ref_format_new.restype = C.c_void_p
+        lib.ref_format_new.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.ref_format_free.argtypes = [C.c_void_p]
+        fn = lib.ref_decode_reg
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p] * 2 + [C.c_int] * 2 + [C.c_void_p] * 8
+        f = lib.ref_format_new(384, 80, 128)
+        o = np.zeros(9 * 476, np.uint8)
+        ok, its, be = np.zeros(9, np.uint8), np.zeros(9, np.int8), np.zeros(9, np.float32)
+        call = lambda: fn(f, img.ctypes.data, 1280, 720, H.ctypes.data, nodes.ctypes.data, o.ctypes.data,  # noqa: E731
+                          ok.ctypes.data, its.ctypes.data, be.ctypes.data, None, None)
+        for _ in range(3):
+            call()
+        t0 = time.perf_counter()
+        n = 0
+        while time.perf_counter() - t0 < 2.0:
+            good = call()
+            n += 1
+        cpu_ms = (time.perf_counter() - t0) * 1e3 / n
+        lib.ref_format_free(f)
+        res["cpu_baseline"] = {"value": round(cpu_ms, 3), "unit": "ms/frame", "cores": 1, "kind": "reference",
+                               "verified": int(good),
+                               "sample": "%d decodes of the same capture, 2 s, one thread; liblizard lizard_finish "
+                                         "through the held registration (x86-64 SIMD build)" % n}
+    return res
+
+
 def bench_refine(args):
     """Decision-directed registration refinement: decode through the perturbed registrations (one batch), re-modulate
     the CRC-passed blocks, 10 Gauss-Newton steps a frame on the GPU, decode again."""
@@ -440,7 +495,7 @@ def main():
     ap.add_argument("--fft-frames", type=int, default=64)
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--refine-frames", type=int, default=256)
-    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine"), default=None,
+    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small"), default=None,
                     help="run one workload alone (for per-kernel profiles); the JSON line then carries only it")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -451,9 +506,10 @@ def main():
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
 
-    if args.only in ("pipeline", "fft_demap", "refine"):
+    if args.only in ("pipeline", "fft_demap", "refine", "small"):
         out = (bench_pipeline(args, pipeline_frames(args.pipe_frames, 4242)) if args.only == "pipeline"
-               else bench_fft_demap(args) if args.only == "fft_demap" else bench_refine(args))
+               else bench_fft_demap(args) if args.only == "fft_demap" else bench_refine(args) if args.only == "refine"
+               else bench_small(args))
         print(json.dumps({"only": args.only, args.only: out}))
         return
     if args.only == "ldpc":
@@ -542,8 +598,9 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(batches, args.cpu_seconds)
 
-    pipe = fft = refine = None
+    pipe = fft = refine = small = None
     if not args.no_pipeline:
+        small = bench_small(args)
         refine = bench_refine(args)
         caps = pipeline_frames(args.pipe_frames, 4242)
         pipe = bench_pipeline(args, caps)
@@ -578,6 +635,7 @@ def main():
         "pipeline": pipe,
         "fft_demap": fft,
         "refine": refine,
+        "small": small,
     }
     print(json.dumps(line))
 

@@ -63,10 +63,47 @@ struct lizx_ldpc {
   std::vector<LdpcTask> host_tasks;
 };
 
+// A frame's per-block results in one device area (and one pinned host mirror), so they come back in one copy:
+// bytes (B x 476), iterations (B), verdicts (B), estimates (B floats), pilot readings (B floats).
+struct Results {
+  int B = 0;
+  size_t at_its = 0, at_ok = 0, at_est = 0, at_pilot = 0, total = 0;
+  void layout(int blocks) {
+    B = blocks;
+    at_its = static_cast<size_t>(B) * BLOCK_BYTES;
+    at_ok = at_its + B;
+    at_est = (at_ok + B + 3) & ~static_cast<size_t>(3);
+    at_pilot = at_est + 4 * static_cast<size_t>(B);
+    total = at_pilot + 4 * static_cast<size_t>(B);
+  }
+};
+
+struct PinnedBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (p) hip_check(hipHostFree(p), "hipHostFree");
+    p = nullptr;
+    bytes = 0;
+    hip_check(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc");
+    bytes = n;
+  }
+};
+
 struct liz_decoder {
   int nmax = 0, top = 0;
   std::unique_ptr<LdpcEngine> eng;
-  DevBuf tasks, llr, est, its, blk_est, bytes, crc_ok, grid;
+  DevBuf tasks, llr, est, grid, res;
+  PinnedBuf res_host;
+  Results lay;
+  int tasks_version = 0;  // the format whose frame-mode task list sits in tasks
   std::vector<LdpcTask> host_tasks;
   std::vector<uint8_t> blocks, ok;
   std::vector<std::unique_ptr<FramePlan>> plans;  // one per (version, span) met
@@ -96,25 +133,26 @@ static void finish_blocks(liz_decoder *d, const Format *f, const int8_t *d_llr, 
   LdpcEngine &e = *d->eng;
   e.ensure_frame_tables(static_cast<int>(f->white.size()));
   const int B = f->blocks;
-  d->host_tasks.resize(B);
-  for (int b = 0; b < B; b++) d->host_tasks[b] = LdpcTask{f->blk_slot[b], -1, f->blk_code[b], f->blk_slot[b], 0, b};
-  std::stable_sort(d->host_tasks.begin(), d->host_tasks.end(),
-                   [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
-  d->tasks.reserve(B * sizeof(LdpcTask));
-  d->its.reserve(B);
-  d->blk_est.reserve(B * sizeof(float));
-  d->bytes.reserve(static_cast<size_t>(B) * BLOCK_BYTES);
-  d->crc_ok.reserve(B);
-  hip_check(hipMemcpyAsync(d->tasks.p, d->host_tasks.data(), B * sizeof(LdpcTask), hipMemcpyHostToDevice, e.stream),
-            "tasks");
+  d->lay.layout(B);
+  d->res.reserve(d->lay.total);
+  if (d->tasks_version != f->version) {
+    d->host_tasks.resize(B);
+    for (int b = 0; b < B; b++) d->host_tasks[b] = LdpcTask{f->blk_slot[b], -1, f->blk_code[b], f->blk_slot[b], 0, b};
+    std::stable_sort(d->host_tasks.begin(), d->host_tasks.end(),
+                     [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
+    d->tasks.reserve(B * sizeof(LdpcTask));
+    hip_check(hipMemcpy(d->tasks.p, d->host_tasks.data(), B * sizeof(LdpcTask), hipMemcpyHostToDevice), "tasks");
+    d->tasks_version = f->version;
+  }
+  uint8_t *r = d->res.as<uint8_t>();
   LdpcParams p{};
   p.tasks = d->tasks.as<LdpcTask>();
   p.count = B;
   p.llr = d_llr;
-  p.its = d->its.as<int8_t>();
-  p.blk_est = d->blk_est.as<float>();
-  p.bytes = d->bytes.as<uint8_t>();
-  p.crc_ok = d->crc_ok.as<uint8_t>();
+  p.its = reinterpret_cast<int8_t *>(r + d->lay.at_its);
+  p.blk_est = reinterpret_cast<float *>(r + d->lay.at_est);
+  p.bytes = r;
+  p.crc_ok = r + d->lay.at_ok;
   p.white = e.white.as<uint8_t>();
   for (int ci = 0; ci < NCODES; ci++) p.s2p[ci] = e.s2p[ci].as<int16_t>();
   p.est = d_est;
@@ -128,20 +166,32 @@ static void finish_blocks(liz_decoder *d, const Format *f, const int8_t *d_llr, 
   e.launch(p, false, true);
 }
 
-// The per-block results of finish_blocks to the host; returns the verified count.
-static int collect_blocks(liz_decoder *d, const Format *f, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est) {
+// The frame's results area's pilot readings (where the picture stages write them).
+static float *pilot_area(liz_decoder *d, const Format *f) {
+  d->lay.layout(f->blocks);
+  d->res.reserve(d->lay.total);
+  return reinterpret_cast<float *>(d->res.as<uint8_t>() + d->lay.at_pilot);
+}
+
+// The per-block results of finish_blocks to the host in one copy (after the stream's work); returns the verified
+// count. pilot (optional) receives the pilot readings.
+static int collect_blocks(liz_decoder *d, const Format *f, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
+                          float *pilot = nullptr) {
+  LdpcEngine &e = *d->eng;
   const int B = f->blocks;
-  hip_check(hipMemcpy(its, d->its.p, B, hipMemcpyDeviceToHost), "iterations");
-  hip_check(hipMemcpy(blk_est, d->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost), "estimates");
-  hip_check(hipMemcpy(ok, d->crc_ok.p, B, hipMemcpyDeviceToHost), "verdicts");
-  d->blocks.resize(static_cast<size_t>(B) * BLOCK_BYTES);
-  hip_check(hipMemcpy(d->blocks.data(), d->bytes.p, d->blocks.size(), hipMemcpyDeviceToHost), "blocks");
+  d->res_host.reserve(d->lay.total);
+  hip_check(hipMemcpyAsync(d->res_host.p, d->res.p, d->lay.total, hipMemcpyDeviceToHost, e.stream), "results");
+  e.sync();
+  const uint8_t *h = static_cast<const uint8_t *>(d->res_host.p);
+  std::memcpy(its, h + d->lay.at_its, B);
+  std::memcpy(ok, h + d->lay.at_ok, B);
+  std::memcpy(blk_est, h + d->lay.at_est, B * sizeof(float));
+  if (pilot) std::memcpy(pilot, h + d->lay.at_pilot, B * sizeof(float));
   int good = 0;
   for (int b = 0; b < B; b++) {
     if (its[b] == -2) ok[b] = 0;
     if (ok[b]) {
-      std::memcpy(blocks + static_cast<size_t>(b) * BLOCK_BYTES, d->blocks.data() + static_cast<size_t>(b) * BLOCK_BYTES,
-                  BLOCK_BYTES);
+      std::memcpy(blocks + static_cast<size_t>(b) * BLOCK_BYTES, h + static_cast<size_t>(b) * BLOCK_BYTES, BLOCK_BYTES);
       good++;
     }
   }
@@ -602,6 +652,19 @@ static const WarpFrame *stage_capture(liz_decoder *d, const FramePlan &p, const 
   return d->wframe.as<WarpFrame>();
 }
 
+// The soft values, estimates, rms, coefficients and pilot shift of the last frame to the caller's lizx_soft.
+static void copy_soft(liz_decoder *d, const FramePlan &p, lizx_soft *soft, bool dbg) {
+  const size_t S = static_cast<size_t>(p.subch) * SUB;
+  if (soft->llr) hip_check(hipMemcpy(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost), "soft values");
+  if (soft->sub_est)
+    hip_check(hipMemcpy(soft->sub_est, d->fb.est.p, p.subch * sizeof(double), hipMemcpyDeviceToHost), "estimates");
+  if (soft->sub_rms)
+    hip_check(hipMemcpy(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
+  if (soft->coef && dbg)
+    hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
+  hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
+}
+
 static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
                           const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
                           lizx_soft *soft) {
@@ -614,25 +677,14 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
   const FramePlan &p = d->plan(version, span);
   const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg);
   const bool dbg = soft && soft->coef;
-  const size_t S = static_cast<size_t>(p.subch) * SUB;
   d->fb.reserve(p, 1, dbg);
-  launch_soft_warp(p, d->fb, wf, 1, dbg, e.stream);
-  finish_blocks(d, f, d->fb.llr.as<int8_t>(), d->fb.est.as<double>());
-  e.sync();
-  if (soft) {
-    if (soft->llr) hip_check(hipMemcpy(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost), "soft values");
-    if (soft->sub_est)
-      hip_check(hipMemcpy(soft->sub_est, d->fb.est.p, p.subch * sizeof(double), hipMemcpyDeviceToHost), "estimates");
-    if (soft->sub_rms)
-      hip_check(hipMemcpy(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
-    if (soft->coef)
-      hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
-    hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
-    if (soft->blk_pilot)
-      hip_check(hipMemcpy(soft->blk_pilot, d->fb.pilot_r.p, p.dev.blocks * sizeof(float), hipMemcpyDeviceToHost),
-                "pilot readings");
-  }
-  return collect_blocks(d, f, blocks, ok, its, blk_est);
+  SoftBufs v = d->fb.view(dbg);
+  v.blk_pilot = pilot_area(d, f);
+  launch_soft_warp(p, v, wf, 1, e.stream);
+  finish_blocks(d, f, v.llr, v.est);
+  const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr);
+  if (soft) copy_soft(d, p, soft, dbg);
+  return good;
 }
 
 LIZ_API int lizx_node_lattice(int span, float *pos) {
@@ -820,7 +872,6 @@ LIZ_API int lizx_finish_llr(liz_decoder *d, int version, const int8_t *llr, cons
     hip_check(hipMemcpyAsync(d->est.p, est_d.data(), f->subch * sizeof(double), hipMemcpyHostToDevice, e.stream),
               "estimates");
     finish_blocks(d, f, d->llr.as<int8_t>(), d->est.as<double>());
-    e.sync();
     return collect_blocks(d, f, blocks, ok, its, blk_est);
   });
 }
@@ -836,27 +887,17 @@ LIZ_API int lizx_finish_grid(liz_decoder *d, int version, int span, const float 
     hip_check(hipSetDevice(e.device), "hipSetDevice");
     const FramePlan &p = d->plan(version, span);
     const bool dbg = soft && soft->coef;
-    const size_t n = p.n, S = static_cast<size_t>(p.subch) * SUB;
+    const size_t n = p.n;
     d->fb.reserve(p, 1, dbg);
     d->grid.reserve(n * n * sizeof(float));
     hip_check(hipMemcpyAsync(d->grid.p, grid, n * n * sizeof(float), hipMemcpyHostToDevice, e.stream), "grid");
-    launch_soft(p, d->fb, d->grid.as<float>(), 1, dbg, e.stream);
-    finish_blocks(d, f, d->fb.llr.as<int8_t>(), d->fb.est.as<double>());
-    e.sync();
-    if (soft) {
-      if (soft->llr) hip_check(hipMemcpy(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost), "soft values");
-      if (soft->sub_est)
-        hip_check(hipMemcpy(soft->sub_est, d->fb.est.p, p.subch * sizeof(double), hipMemcpyDeviceToHost), "estimates");
-      if (soft->sub_rms)
-        hip_check(hipMemcpy(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
-      if (soft->coef)
-        hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
-      hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
-      if (soft->blk_pilot)
-        hip_check(hipMemcpy(soft->blk_pilot, d->fb.pilot_r.p, p.dev.blocks * sizeof(float), hipMemcpyDeviceToHost),
-                  "pilot readings");
-    }
-    return collect_blocks(d, f, blocks, ok, its, blk_est);
+    SoftBufs v = d->fb.view(dbg);
+    v.blk_pilot = pilot_area(d, f);
+    launch_soft(p, v, d->grid.as<float>(), 1, e.stream);
+    finish_blocks(d, f, v.llr, v.est);
+    const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr);
+    if (soft) copy_soft(d, p, soft, dbg);
+    return good;
   });
 }
 
