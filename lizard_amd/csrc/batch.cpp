@@ -23,7 +23,7 @@ struct Group {
   std::unique_ptr<FramePlan> plan;
   std::vector<int> frames;  // caller indices
   int first = 0;            // the group's first internal index
-  FrameBufs fb;             // X, HP, row sums, shading, projections, shift, rms for its frames
+  FrameBufs fb;             // X, coefficients, row sums, shading, projections, shift, rms for its frames
   SoftBufs view;            // fb's pointers with llr and est pointing into the batch-wide arrays
 };
 

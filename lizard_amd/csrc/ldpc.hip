@@ -105,7 +105,9 @@ __device__ __forceinline__ void decode_one(const LdpcParams &p, const LdpcTask &
       __syncthreads();
     }
     // two alternating reduction slots: a slot is rewritten only after a later barrier has passed every read of it
-    const int bad = group_sum(tm::unsat<CI>(L, lane), red + 2 * (it & 1), lane);
+    // the count where the stall rule reads it, else only whether every check holds (early out per block row)
+    const bool count = p.stall_it > 0 && (it == 0 || it == p.stall_it - 1);
+    const int bad = group_sum(count ? tm::unsat<CI>(L, lane) : tm::unsat_any<CI>(L, lane), red + 2 * (it & 1), lane);
     const int v = stop_rule(it, bad, p.stall_it, p.stall_ratio, &first);
     if (v) {
       used = v;

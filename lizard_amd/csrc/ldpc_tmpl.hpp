@@ -48,6 +48,9 @@ struct Shape {
   static constexpr int shift(int e) { return cb.shift[e]; }
 };
 
+// Two int16 lanes in one 32-bit register (packed 16-bit instructions on the GPU, element-wise on the host).
+typedef int16_t v2i16 __attribute__((vector_size(4)));
+
 // Messages of the check position a lane owns: slot s at byte s % 4 of word s / 4.
 template <int CI>
 struct Msgs {
@@ -67,6 +70,40 @@ template <int S>
 __host__ __device__ LZ_INLINE int get_msg(const uint32_t *w) {
   return static_cast<int>(static_cast<int8_t>(static_cast<uint8_t>(w[S >> 2] >> ((S & 3) * 8))));
 }
+// Byte permute of the 8-byte pool {hi, lo} (lo bytes 0-3, hi bytes 4-7): result byte i = pool byte sel[i]
+// (v_perm_b32 on the GPU).
+__host__ __device__ LZ_INLINE uint32_t perm_b32(uint32_t hi, uint32_t lo, uint32_t sel) {
+#ifdef __HIP_DEVICE_COMPILE__
+  return __builtin_amdgcn_perm(hi, lo, sel);
+#else
+  const uint64_t pool = (static_cast<uint64_t>(hi) << 32) | lo;
+  uint32_t out = 0;
+  for (int i = 0; i < 4; i++) out |= static_cast<uint32_t>((pool >> (8 * ((sel >> (8 * i)) & 7))) & 0xFF) << (8 * i);
+  return out;
+#endif
+}
+
+// Messages S and S + 1 from the two 16-bit halves of m (their low bytes), one byte permute a word.
+template <int S>
+__host__ __device__ LZ_INLINE void put_msg2(uint32_t *w, uint32_t m) {
+  constexpr int p = S & 3;
+  if constexpr (p < 3) {
+    // byte p <- m byte 0 (pool 4), byte p + 1 <- m byte 2 (pool 6), the rest kept
+    constexpr uint32_t sel = (p == 0 ? 0x03020604u : p == 1 ? 0x03060400u : 0x06040100u);
+    w[S >> 2] = perm_b32(m, w[S >> 2], sel);
+  } else {
+    w[S >> 2] = perm_b32(m, w[S >> 2], 0x04020100u);
+    w[(S >> 2) + 1] = perm_b32(m, w[(S >> 2) + 1], 0x03020106u);
+  }
+}
+// Message S from the low byte of m.
+template <int S>
+__host__ __device__ LZ_INLINE void put_msg1(uint32_t *w, uint32_t m) {
+  constexpr int p = S & 3;
+  constexpr uint32_t sel = (p == 0 ? 0x03020104u : p == 1 ? 0x03020400u : p == 2 ? 0x03040100u : 0x04020100u);
+  w[S >> 2] = perm_b32(m, w[S >> 2], sel);
+}
+
 template <int S>
 __host__ __device__ LZ_INLINE void put_msg(uint32_t *w, int v) {
   constexpr int sh = (S & 3) * 8;
@@ -126,14 +163,32 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   int m1 = ((key1 >> 5) * 13) >> 4, m2 = ((key2 >> 5) * 13) >> 4;
   m1 = m1 > 127 ? 127 : m1;
   m2 = m2 > 127 ? 127 : m2;
-  static_for<0, NS>([&](auto e) {
-    const int mag = arg == e ? m2 : m1;
-    const int s = (sgn ^ q[e]) >> 31;  // -1 where the other slots' signs make the message negative
-    const int msg = (mag ^ s) - s;
-    put_msg<RB + e>(M.m, msg);
-    int t = q[e] + msg;
-    t = t > 8191 ? 8191 : (t < -8191 ? -8191 : t);
-    if (!(e == ND && absent)) at_byte(L, adr[e]) = static_cast<int16_t>(t);
+  // the new messages and posteriors two edges at a time in 16-bit halves (every value fits: |q| <= 8191 + 127,
+  // |msg| <= 127): the pair holding the first minimum takes m2 in that half, the others m1 in both
+  const v2i16 m11 = {static_cast<int16_t>(m1), static_cast<int16_t>(m1)};
+  const v2i16 m11d = (arg & 1) ? v2i16{static_cast<int16_t>(m1), static_cast<int16_t>(m2)}
+                               : v2i16{static_cast<int16_t>(m2), static_cast<int16_t>(m1)};
+  const int argpair = arg >> 1;
+  const int16_t sg = static_cast<int16_t>(sgn >> 16);  // the check's sign in bit 15
+  const v2i16 sgn2 = {sg, sg};
+  const v2i16 hi = {8191, 8191}, lo = {-8191, -8191};
+  static_for<0, (NS + 1) / 2>([&](auto j) {
+    constexpr int e0 = 2 * j, e1 = 2 * j + 1;
+    const v2i16 q2 = {static_cast<int16_t>(q[e0]), static_cast<int16_t>(e1 < NS ? q[e1] : 0)};
+    const v2i16 mag = argpair == j ? m11d : m11;
+    const v2i16 s2 = (sgn2 ^ q2) >> 15;  // -1 where the other slots' signs make the message negative
+    const v2i16 msg = (mag ^ s2) - s2;
+    v2i16 t = q2 + msg;
+    t = t > hi ? hi : t;
+    t = t < lo ? lo : t;
+    uint32_t mw;
+    __builtin_memcpy(&mw, &msg, 4);
+    if constexpr (e1 < NS) put_msg2<RB + e0>(M.m, mw);
+    else put_msg1<RB + e0>(M.m, mw);
+    if (!(e0 == ND && absent)) at_byte(L, adr[e0]) = t[0];
+    if constexpr (e1 < NS) {
+      if (!(e1 == ND && absent)) at_byte(L, adr[e1]) = t[1];
+    }
   });
 }
 
@@ -164,6 +219,33 @@ __host__ __device__ LZ_INLINE int unsat(const int16_t *L, int lane) {
     bad += x < 0;
   });
   return bad;
+}
+
+// Whether any of the wave's checks is unsatisfied, block row by block row, stopping at the first row that has one
+// (the reference's early-out test, ldpc.c layers_clear, for the iterations where only "all satisfied" decides).
+// On the GPU the answer is the wave's (a ballot); on the host, the lane's.
+template <int CI>
+__host__ __device__ LZ_INLINE int unsat_any(const int16_t *L, int lane) {
+  using S = Shape<CI>;
+  int found = 0;
+  static_for<0, S::MB>([&](auto r) {
+    if (found) return;
+    constexpr int R = r, D0 = S::d0(R), ND = S::nd(R), K = S::K;
+    const int i = fresh(lane), i2 = 2 * i;
+    int x = L[K + R * 128 + i];
+    static_for<0, ND>([&](auto e) {
+      constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
+      x ^= at_byte(L, col * 256 + rot_off(i2, sh));
+    });
+    if (R > 0) x ^= L[K + (R - 1) * 128 + i];
+    else x ^= i > 0 ? L[K + (S::MB - 1) * 128 + i - 1] : 0;
+#ifdef __HIP_DEVICE_COMPILE__
+    found = __builtin_amdgcn_ballot_w64(x < 0) != 0;
+#else
+    found = x < 0;
+#endif
+  });
+  return found;
 }
 
 template <int CI>

@@ -501,13 +501,14 @@ __global__ __launch_bounds__(64 * W) void k_rows(PlanDev P, Src src, const doubl
   }
 }
 
-// Column u (one wave): Y(u, v) for v < E at [v], and Y(u, n - v) for 1 <= v < E at [E - 1 + v].
+// Column u (one wave): the FFT along y, then every coefficient on this column straight into coefficient order:
+// coefficient (u, v) is Y(u, v), coefficient (-u, v) is conj Y(u, n - v) (the picture is real).
 template <int N, int WV>
-__global__ __launch_bounds__(64 * WV) void k_cols(PlanDev P, const float2 *X, float2 *HP) {
+__global__ __launch_bounds__(64 * WV) void k_cols(PlanDev P, const float2 *X, float2 *C) {
   using F = WaveFft<N>;
   constexpr int M = F::M;
   extern __shared__ float2 zs[];  // WV waves x F::LDS
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E, W = 2 * E;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E;
   const int u = blockIdx.x * WV + w;
   if (u >= E) return;  // the whole wave; nothing below waits on the workgroup
   const float2 *col = X + (static_cast<size_t>(f) * E + u) * N;
@@ -516,18 +517,23 @@ __global__ __launch_bounds__(64 * WV) void k_cols(PlanDev P, const float2 *X, fl
   for (int m = 0; m < M; m++) v[m] = col[lane + 64 * m];
   float2 *S = zs + w * F::LDS;
   F::run(v, S, P.tw, lane);
-  float2 *out = HP + (static_cast<size_t>(f) * E + u) * W;
-  for (int k = lane; k < W - 1; k += 64) out[k] = S[F::pos(k < E ? k : N - (k - E + 1))];
+  float2 *out = C + static_cast<size_t>(f) * P.subch * SUB;
+  for (int e = P.col_off[u] + lane; e < P.col_off[u + 1]; e += 64) {
+    const uint32_t q = static_cast<uint32_t>(P.col_ent[e]);
+    float2 z = S[F::pos(static_cast<int>((q >> 19) & 0x7FF))];
+    if (q >> 30) z.y = -z.y;
+    out[q & 0x7FFFF] = z;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------ the coefficients
 struct Proj { double px, py, pxy, pxx, pyy; };
 
-// Coefficient (u, v) after the detrend (lizard_phy.c:1697-1716; SPEC 6.10), in the reference's float order.
-__device__ __forceinline__ float2 coef_at(const PlanDev &P, const float2 *HP, const Proj &pr, int u, int v) {
-  const int E = P.E, W = 2 * E, n = P.n;
-  float2 y = u >= 0 ? HP[u * W + v] : HP[(-u) * W + E - 1 + v];
-  if (u < 0) y.y = -y.y;  // Y(u, v) = conj Y(-u, n - v)
+// Coefficient ci = (u, v) after the detrend (lizard_phy.c:1697-1716; SPEC 6.10), in the reference's float order; C
+// holds the frame's raw coefficients in coefficient order.
+__device__ __forceinline__ float2 coef_at(const PlanDev &P, const float2 *C, const Proj &pr, int ci, int u, int v) {
+  const int n = P.n;
+  float2 y = C[ci];
   const int uw = u < 0 ? u + n : u;
   const float ar = static_cast<float>(__dmul_rn(pr.pxy, P.x1r[uw])), ai = static_cast<float>(__dmul_rn(pr.pxy, P.x1i[uw]));
   const float r = P.x1r[v], i = P.x1i[v];
@@ -563,7 +569,7 @@ __device__ __forceinline__ void coef_uv(const PlanDev &P, int i, int *u, int *v)
 }
 
 // The projections from the row sums, then the pilots' grid shift (lizard_phy.c:280-329, :1697, :1739).
-__global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, const float2 *HP_all, double *proj_out,
+__global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, const float2 *C_all, double *proj_out,
                                                float *align_out) {
   __shared__ double red[256];
   __shared__ double s_d[2];
@@ -596,7 +602,7 @@ __global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, 
     s_go = 1;
   }
   __syncthreads();
-  const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
+  const float2 *Cf = C_all + static_cast<size_t>(f) * P.subch * SUB;
   const double k = 2.0 * 3.14159265358979323846 / n;
   double A = 0, B = 0, C = 0, Pq = 0, Q = 0, YY = 0;
   for (int it = 0; it < 2; it++) {
@@ -607,7 +613,7 @@ __global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, 
       const int ci = P.pilot[t];
       int u, v;
       coef_uv(P, ci, &u, &v);
-      const float2 y = turn(P, coef_at(P, HP, pr, u, v), u, v, dx, dy);
+      const float2 y = turn(P, coef_at(P, Cf, pr, ci, u, v), u, v, dx, dy);
       const double xr = P.white[2 * ci] ? -1.0 : 1.0, xi = P.white[2 * ci + 1] ? -1.0 : 1.0;
       const double zr = y.x * xr + y.y * xi, zi = y.y * xr - y.x * xi;
       const double cu = k * zr * u, cv = k * zr * v;
@@ -662,12 +668,12 @@ __device__ __forceinline__ double axis_info(double g) {
 // workgroup): each lane's five coefficients gathered, detrended and turned, the moments m2, m4 summed in double
 // across the wave by shuffles (every lane ends with the totals, so the scale needs no broadcast and no barrier).
 template <int WV>
-__global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *HP_all, const double *proj,
+__global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_all, const double *proj,
                                                    const float *align, int8_t *llr_all, double *est_all, float *rms_all,
                                                    float *dbg_all) {
   const int lane = threadIdx.x & 63, j = blockIdx.x * WV + (threadIdx.x >> 6), f = blockIdx.y;
   if (j >= P.subch) return;  // the whole wave
-  const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
+  const float2 *C = C_all + static_cast<size_t>(f) * P.subch * SUB;
   const Proj pr{proj[f * 5], proj[f * 5 + 1], proj[f * 5 + 2], proj[f * 5 + 3], proj[f * 5 + 4]};
   const float dx = align[2 * f], dy = align[2 * f + 1];
   constexpr int PER = SUB / 64;
@@ -678,7 +684,7 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *HP_a
     const int ci = j * SUB + lane + 64 * q;
     int u, v;
     coef_uv(P, ci, &u, &v);
-    float2 c = coef_at(P, HP, pr, u, v);
+    float2 c = coef_at(P, C, pr, ci, u, v);
     if (dbg_all) reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * P.subch * SUB + ci] = c;
     if (dx != 0 || dy != 0) c = turn(P, c, u, v, dx, dy);
     y[q] = c;
@@ -715,12 +721,12 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *HP_a
 // Block b's pilot reading (lizard_phy.c:1795-1809; SPEC 7.3): over the slots past its codeword, the axis value
 // against the whitening's sign in units of its sub-channel's rms axis value, averaged over the tail; NaN where the
 // block has no tail or no sub-channel of it read anything. One wave a block (tails are 16 or 32 slots).
-__global__ __launch_bounds__(64) void k_pilot_read(PlanDev P, const float2 *HP_all, const double *proj,
+__global__ __launch_bounds__(64) void k_pilot_read(PlanDev P, const float2 *C_all, const double *proj,
                                                    const float *align, const float *rms_all, float *out) {
   const int b = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
   const int4 bk = P.blk[b];
   const int s0 = bk.x, nt = bk.y, nb = bk.z;
-  const float2 *HP = HP_all + static_cast<size_t>(f) * P.E * 2 * P.E;
+  const float2 *C = C_all + static_cast<size_t>(f) * P.subch * SUB;
   const Proj pr{proj[f * 5], proj[f * 5 + 1], proj[f * 5 + 2], proj[f * 5 + 3], proj[f * 5 + 4]};
   const float dx = align[2 * f], dy = align[2 * f + 1];
   double cs = 0;
@@ -731,7 +737,7 @@ __global__ __launch_bounds__(64) void k_pilot_read(PlanDev P, const float2 *HP_a
     if (!(rms > 0)) continue;
     int u, v;
     coef_uv(P, gs >> 1, &u, &v);
-    float2 y = coef_at(P, HP, pr, u, v);
+    float2 y = coef_at(P, C, pr, gs >> 1, u, v);
     if (dx != 0 || dy != 0) y = turn(P, y, u, v, dx, dy);
     const float a = (i & 1) ? y.y : y.x;
     cs += (P.white[gs] ? -1.0 : 1.0) * a / rms;
@@ -850,6 +856,30 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
     E = std::max(E, std::max(std::abs(u), v) + 1);
   }
   p->E = d.E = E;
+  // each column's coefficients for the column pass: coefficient (u, v) from column |u|, at v (u >= 0) or conjugated
+  // at n - v (u < 0); entry = ci | source index << 19 | conjugate << 30
+  {
+    std::vector<int32_t> off(E + 1, 0), ent(S);
+    for (int i = 0; i < S; i++) {
+      const int u = static_cast<int16_t>(coef[i] & 0xFFFF);
+      off[std::abs(u) + 1]++;
+    }
+    for (int c = 0; c < E; c++) off[c + 1] += off[c];
+    std::vector<int32_t> at(off.begin(), off.end() - 1);
+    for (int i = 0; i < S; i++) {
+      const int u = static_cast<int16_t>(coef[i] & 0xFFFF), v = coef[i] >> 16;
+      const int k = u >= 0 ? v : (n - v) % n;
+      ent[at[std::abs(u)]++] = static_cast<int32_t>(static_cast<uint32_t>(i) | (static_cast<uint32_t>(k) << 19) |
+                                                    (static_cast<uint32_t>(u < 0) << 30));
+    }
+    if (S >= (1 << 19) || n > 2048) throw Error(LIZ_E_INTERNAL, "coefficient table packing");
+    p->cols.reserve((off.size() + ent.size()) * sizeof(int32_t));
+    hip_check(hipMemcpy(p->cols.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice), "columns");
+    hip_check(hipMemcpy(static_cast<int32_t *>(p->cols.p) + off.size(), ent.data(), ent.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice), "columns");
+    d.col_off = p->cols.as<int32_t>();
+    d.col_ent = p->cols.as<int32_t>() + off.size();
+  }
   // the pilots: the tail coefficients of every 7/8 and 3/4 block (SPEC 7.3; lizard_phy.c:293-317)
   for (int b = 0; b < f->blocks; b++) {
     const Code &c = code(f->blk_code[b]);
@@ -943,7 +973,7 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
 void FrameBufs::reserve(const FramePlan &p, int frames, bool debug) {
   const size_t n = p.n, E = p.E, S = static_cast<size_t>(p.subch) * SUB;
   X.reserve(frames * E * n * sizeof(float2));
-  HP.reserve(frames * E * 2 * E * sizeof(float2));
+  C.reserve(frames * S * sizeof(float2));
   rowsum.reserve(frames * n * 3 * sizeof(double));
   beta.reserve(frames * FIT_STRIDE * sizeof(double));
   proj.reserve(frames * 5 * sizeof(double));
@@ -961,7 +991,7 @@ SoftBufs FrameBufs::view(bool debug) {
   v.rowsum = rowsum.as<double>();
   v.proj = proj.as<double>();
   v.X = X.as<float2>();
-  v.HP = HP.as<float2>();
+  v.C = C.as<float2>();
   v.align = align.as<float>();
   v.llr = llr.as<int8_t>();
   v.est = est.as<double>();
@@ -978,7 +1008,7 @@ SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
   v.rowsum = rowsum + F * n * 3;
   v.proj = proj + F * 5;
   v.X = X + F * E * n;
-  v.HP = HP + F * E * 2 * E;
+  v.C = C + F * S;
   v.align = align + F * 2;
   v.llr = llr + F * 2 * S;
   v.est = est + F * p.subch;
@@ -1015,17 +1045,17 @@ static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int 
     with_waves(sh.cols, [&](auto Wc) {
       constexpr int W = decltype(Wc)::value;
       hipLaunchKernelGGL((k_cols<NN, W>), dim3((E + W - 1) / W, frames), dim3(64 * W),
-                         W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, b.X, b.HP);
+                         W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, b.X, b.C);
     });
   });
-  hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.HP, b.proj, b.align);
+  hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.C, b.proj, b.align);
   with_waves(sh.demap, [&](auto Wd) {
     constexpr int W = decltype(Wd)::value;
-    hipLaunchKernelGGL(k_demap<W>, dim3((p.subch + W - 1) / W, frames), dim3(64 * W), 0, s, p.dev, b.HP, b.proj,
+    hipLaunchKernelGGL(k_demap<W>, dim3((p.subch + W - 1) / W, frames), dim3(64 * W), 0, s, p.dev, b.C, b.proj,
                        b.align, b.llr, b.est, b.rms, b.coef_dbg);
   });
   if (b.blk_pilot)
-    hipLaunchKernelGGL(k_pilot_read, dim3(p.dev.blocks, frames), dim3(64), 0, s, p.dev, b.HP, b.proj, b.align, b.rms,
+    hipLaunchKernelGGL(k_pilot_read, dim3(p.dev.blocks, frames), dim3(64), 0, s, p.dev, b.C, b.proj, b.align, b.rms,
                        b.blk_pilot);
   hip_check(hipGetLastError(), "soft-value launches");
 }

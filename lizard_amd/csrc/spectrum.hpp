@@ -52,13 +52,14 @@ struct PlanDev {
   const int32_t *mcell;                    // n: node cell of sample index t (the same on both axes)
   int cell0[MAX_NODES];                    // first sample index of node cell i (cell0[nx - 1] = n)
   const int4 *blk;                         // per block: first slot, code's sent bits nt, slots nb, unused
+  const int32_t *col_off, *col_ent;        // per column u < E: its coefficients (ci | source index << 19 | conj << 30)
 };
 
 struct FramePlan {
   int version = 0, span = 0, n = 0, subch = 0, E = 0;
   const Format *fmt = nullptr;
   PlanDev dev{};
-  DevBuf tables, coef, pilot, tw, rot, white, mtab, blk;
+  DevBuf tables, coef, pilot, tw, rot, white, mtab, blk, cols;
   std::vector<int> pilot_host;  // coefficient indices of the pilots, in block order
   std::vector<int> pilot_block;
 };
@@ -70,7 +71,7 @@ std::unique_ptr<FramePlan> make_plan(int version, int span);
 struct SoftBufs {
   double *beta = nullptr;  // the shading fit's partial sums: FIT_STRIDE doubles a frame
   double *rowsum = nullptr, *proj = nullptr;
-  float2 *X = nullptr, *HP = nullptr;
+  float2 *X = nullptr, *C = nullptr;  // row-pass output [u][y]; the coefficients in coefficient order
   float *align = nullptr;
   int8_t *llr = nullptr;
   double *est = nullptr;
@@ -82,7 +83,7 @@ struct SoftBufs {
 
 // Per-frame device work areas for one decode.
 struct FrameBufs {
-  DevBuf grid, X, HP, rowsum, beta, proj, align, llr, est, rms, coef_dbg, pilot_r;
+  DevBuf grid, X, C, rowsum, beta, proj, align, llr, est, rms, coef_dbg, pilot_r;
   void reserve(const FramePlan &p, int frames, bool debug);
   SoftBufs view(bool debug);
 };

@@ -21,8 +21,11 @@ static int emu(const int8_t *llr, uint8_t *out, int max_iter, int stall_it, floa
       const int r = (it & 1) ? S::MB - 1 - step : step;
       for (int lane = 0; lane < 128; lane++) lz::tm::layer_at<CI>(r, L.data(), M[lane], lane);
     }
+    // as the kernel: the count where the stall rule reads it, else only whether every check holds
+    const bool count = stall_it > 0 && (it == 0 || it == stall_it - 1);
     int bad = 0;
-    for (int lane = 0; lane < 128; lane++) bad += lz::tm::unsat<CI>(L.data(), lane);
+    for (int lane = 0; lane < 128; lane++)
+      bad += count ? lz::tm::unsat<CI>(L.data(), lane) : lz::tm::unsat_any<CI>(L.data(), lane);
     const int v = lz::stop_rule(it, bad, stall_it, ratio, &first);
     if (v) {
       used = v;
