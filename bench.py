@@ -554,6 +554,8 @@ def main():
         its_all.append(its)
         verified_all.append(int(ok.sum()))
 
+    # the autotuner picks the persistent grid's size on the first Eb/N0 point's batch (the bits do not depend on it)
+    tuned_per_cu = dec.tune(count, dev[0][0], dev[0][1], dev[0][2], d_its, max_iter=MAX_ITER, stall_it=0, reps=3)
     for s in range(args.warmup):
         step(s)
     dec.sync()
@@ -621,6 +623,7 @@ def main():
                                "termination, one persistent kernel, CRC-32C on the GPU",
                    "codewords": count, "ebn0_db": list(EBN0), "max_iter": MAX_ITER},
         "payload_MBps": round(payload_mbps, 2),
+        "autotune": {"ldpc_wg_per_cu": tuned_per_cu or "max"},
         "plain_Gbps": round(plain, 4) if plain else None,
         "batched_over_plain": round(gbps / plain, 2) if plain else None,
         "points": points,

@@ -192,6 +192,11 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
                                     const int64_t *d_off, uint8_t *d_bits, int8_t *d_its, int max_iter, int stall_it,
                                     float stall_ratio, int flags, uint8_t *d_bytes, uint8_t *d_crc_ok);
 LIZ_API int lizx_ldpc_sync(lizx_ldpc *h);
+/* The autotuner of the persistent grid: times these device-resident codewords (reps runs a candidate) at every grid
+   size from as many workgroups a CU as fit down to 2, keeps the fastest for this handle's later decodes (the bits do
+   not depend on it); *per_cu (optional) receives it (0: as many as fit). */
+LIZ_API int lizx_ldpc_tune(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr, const int64_t *d_off,
+                           int8_t *d_its, int max_iter, int stall_it, float stall_ratio, int reps, int *per_cu);
 LIZ_API float lizx_ldpc_last_ms(lizx_ldpc *h);
 
 /* ---- Registration refinement (no counterpart in the reference, whose registration is estimated on the CPU;

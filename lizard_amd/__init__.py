@@ -107,6 +107,7 @@ def lib():
     sig("lizx_ldpc_decode", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P, _P, _P)
     sig("lizx_ldpc_decode_device", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P)
     sig("lizx_ldpc_sync", _I, _P)
+    sig("lizx_ldpc_tune", _I, _P, _I, _P, _P, _P, _P, _I, _I, _F, _I, _P)
     sig("lizx_ldpc_last_ms", _F, _P)
     sig("lizx_ldpc_encode", _I, _I, _I, _P, _P)
     sig("lizx_crc32c", C.c_uint32, _P, _SZ)
@@ -313,6 +314,13 @@ class LdpcDecoder:
     def sync(self):
         check(lib().lizx_ldpc_sync(self.h))
         return float(lib().lizx_ldpc_last_ms(self.h))
+
+    def tune(self, count, d_rate, d_llr, d_off, d_its, max_iter=20, stall_it=0, stall_ratio=0.95, reps=3):
+        """The persistent grid's autotuner (lizx_ldpc_tune): workgroups a CU kept (0: as many as fit)."""
+        pc = C.c_int(0)
+        check(lib().lizx_ldpc_tune(self.h, count, d_rate.ptr, d_llr.ptr, d_off.ptr, d_its.ptr, max_iter, stall_it,
+                                   stall_ratio, reps, C.byref(pc)))
+        return pc.value
 
 
 class Decoder:

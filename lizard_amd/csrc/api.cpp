@@ -61,6 +61,7 @@ struct lizx_ldpc {
   std::unique_ptr<LdpcEngine> eng;
   DevBuf tasks, rate_dev, llr, bits, its, bytes, crc_ok, post, msg, off;
   std::vector<LdpcTask> host_tasks;
+  int per_cu = 0;  // the persistent grid's workgroups a CU (0: as many as fit); a launch shape the tuner picks
 };
 
 // A frame's per-block results in one device area (and one pinned host mirror), so they come back in one copy:
@@ -427,7 +428,36 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
     p.stall_ratio = stall_ratio;
     p.mode = MODE_CODEWORD;
     p.flags = flags & LIZX_LDPC_CRC;
-    h->eng->launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true, true);
+    h->eng->launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true, true, h->per_cu);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_ldpc_tune(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr, const int64_t *d_off,
+                           int8_t *d_its, int max_iter, int stall_it, float stall_ratio, int reps, int *per_cu) {
+  return guard([&] {
+    need(h && count > 0 && reps > 0 && reps <= 100, "a decoder, codewords and 1 to 100 repeats");
+    LdpcEngine &e = *h->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    int best = 0;
+    float best_ms = 0;
+    for (int c : {0, 2, 3, 4, 6, 8, 10, 12, 16}) {
+      if (c > e.wg_per_cu) continue;
+      h->per_cu = c;
+      for (int r = 0; r <= reps; r++) {
+        if (r == 1) e.sync();  // the first run settles; the next reps are timed
+        if (lizx_ldpc_decode_device(h, count, d_rate, d_llr, d_off, nullptr, d_its, max_iter, stall_it, stall_ratio, 0,
+                                    nullptr, nullptr))
+          throw Error(liz_last_error_code(), liz_last_error());
+      }
+      e.sync();
+      if (c == 0 || e.last_ms < best_ms) {
+        best_ms = e.last_ms;
+        best = c;
+      }
+    }
+    h->per_cu = best;
+    if (per_cu) *per_cu = best;
     return 0;
   });
 }

@@ -159,3 +159,32 @@ def test_device_entry(ldpc_gpu):
     for r, o in zip(rt, offs):
         n = code(int(r)).n
         assert np.array_equal(bits[o:o + n], a["bits"][o:o + n])
+
+
+def test_tuned_grid_same_bits():
+    """The autotuner's grid size (workgroups a CU) changes only the launch shape: the decode after tuning gives the
+    bits and iteration counts the untuned decoder gives."""
+    rates = [RATES[i % 4] for i in range(256)]
+    dbs = [DBS[i % len(DBS)] for i in range(256)]
+    rt, llr, offs, _ = batch(rates, dbs, seed=37)
+    dec = lizard_amd.LdpcDecoder()
+    try:
+        a = dec.decode(rt, llr, offs, max_iter=20, stall_it=0)
+        D = lizard_amd.DeviceArray
+        d_rt, d_llr, d_off = D.of(rt), D.of(llr), D.of(offs)
+        d_bits, d_its = D(llr.size), D(len(rt))
+        pc = dec.tune(len(rt), d_rt, d_llr, d_off, d_its, max_iter=20, stall_it=0, reps=1)
+        assert pc in (0, 2, 3, 4, 6, 8, 10, 12, 16)
+        with pytest.raises(lizard_amd.LizError):
+            dec.tune(len(rt), d_rt, d_llr, d_off, d_its, reps=0)
+        b = dec.decode(rt, llr, offs, max_iter=20, stall_it=0)
+        assert np.array_equal(a["bits"], b["bits"]) and np.array_equal(a["its"], b["its"])
+        dec.decode_device(len(rt), d_rt, d_llr, d_off, d_bits, d_its, max_iter=20, stall_it=0)
+        dec.sync()
+        assert np.array_equal(d_its.get(np.int8, len(rt)), a["its"])
+        bits = d_bits.get(np.uint8, llr.size)
+        for r, o in zip(rt, offs):
+            n = code(int(r)).n
+            assert np.array_equal(bits[o:o + n], a["bits"][o:o + n])
+    finally:
+        dec.close()
