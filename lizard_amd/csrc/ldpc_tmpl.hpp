@@ -46,10 +46,20 @@ struct Shape {
   static constexpr int rbase(int r) { return cb.ptr[r] + 2 * r; }
   static constexpr int col(int e) { return cb.col[e]; }
   static constexpr int shift(int e) { return cb.shift[e]; }
+  static constexpr int max_col_degree() {
+    int best = 0;
+    for (int j = 0; j < KB; j++) {
+      int d = 0;
+      for (int e = 0; e < cb.ptr[MB]; e++) d += cb.col[e] == j;
+      best = d > best ? d : best;
+    }
+    return best;
+  }
 };
 
 // Two int16 lanes in one 32-bit register (packed 16-bit instructions on the GPU, element-wise on the host).
 typedef int16_t v2i16 __attribute__((vector_size(4)));
+typedef uint16_t v2u16 __attribute__((vector_size(4)));
 
 // Messages of the check position a lane owns: slot s at byte s % 4 of word s / 4.
 template <int CI>
@@ -139,46 +149,71 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   adr[ND + 1] = 2 * (K + R * 128) + i2;
   // check 0 (block row 0, position 0) has no parity bit before it: read as a certain 0 of 8191, message zeroed
   const bool absent = R == 0 && i == 0;
-  int q[NS];
-  // magnitude and edge in one key, |q| << 5 | e: the smallest key is the first smallest magnitude (the reference's
-  // strict "<" keeps the first of equal minima), the second smallest is the next magnitude up or an equal later one
-  int key1 = 0x7FFFFFFF, key2 = 0x7FFFFFFF, sgn = 0;
-  static_for<0, NS>([&](auto e) {
-    int post = at_byte(L, adr[e]);
-    int msg = get_msg<RB + e>(M.m);
-    if (e == ND && absent) {
-      post = 8191;
-      msg = 0;
+  // pass 1, two edges at a time in 16-bit halves (edge 2j low, 2j + 1 high; an odd row's last high half is a phantom
+  // edge, q = 0, key above every real one): q = posterior - message, the sign product, and the key |q| << 5 | e. The
+  // smallest key is the first smallest magnitude (the reference's strict "<" keeps the first of equal minima), the
+  // second smallest the next magnitude up or an equal later one. A key fits 16 bits: a posterior is its channel value
+  // plus its column's messages (the 8191 clamp never acts), so |q| <= 128 + 127 (column degree - 1) < 2047.
+  static_assert(128 + 127 * (S::max_col_degree() - 1) < 2047, "keys fit 16 bits");
+  constexpr int NP = (NS + 1) / 2;
+  v2i16 q2[NP];
+  v2u16 k1 = {0xFFFF, 0xFFFF}, k2 = {0xFFFF, 0xFFFF};
+  uint32_t sx = 0;
+  static_for<0, NP>([&](auto j) {
+    constexpr int e0 = 2 * j, e1 = 2 * j + 1;
+    int p0 = at_byte(L, adr[e0]), m0 = get_msg<RB + e0>(M.m);
+    if (e0 == ND && absent) {
+      p0 = 8191;
+      m0 = 0;
     }
-    const int v = post - msg;
-    q[e] = v;
-    const int a = v < 0 ? -v : v;
-    sgn ^= v;
-    const int key = (a << 5) | static_cast<int>(e);
-    const int lo = key1 < key ? key1 : key, hi = key1 < key ? key : key1;
-    key2 = hi < key2 ? hi : key2;  // the middle of key1 <= key2 and key
-    key1 = lo;
+    int p1 = 0, m1 = 0;
+    if constexpr (e1 < NS) {
+      p1 = at_byte(L, adr[e1]);
+      m1 = get_msg<RB + e1>(M.m);
+      if (e1 == ND && absent) {
+        p1 = 8191;
+        m1 = 0;
+      }
+    }
+    const uint32_t w = static_cast<uint32_t>(static_cast<uint16_t>(p0 - m0)) | (static_cast<uint32_t>(p1 - m1) << 16);
+    sx ^= w;
+    v2i16 q;
+    __builtin_memcpy(&q, &w, 4);
+    q2[j] = q;
+    const v2i16 nq = -q;
+    const v2i16 a = q > nq ? q : nq;
+    v2u16 au;
+    __builtin_memcpy(&au, &a, 4);
+    v2u16 key = (au << 5) | v2u16{static_cast<uint16_t>(e0), static_cast<uint16_t>(e1)};
+    if constexpr (e1 >= NS) key[1] = 0xFFFF;
+    const v2u16 lo = k1 < key ? k1 : key, hi = k1 < key ? key : k1;
+    k2 = hi < k2 ? hi : k2;  // the middle of k1 <= k2 and key
+    k1 = lo;
   });
+  // the halves merged (keys are distinct: the edge index is in them)
+  const int a0 = k1[0], a1 = k1[1], b0 = k2[0], b1 = k2[1];
+  const int key1 = a0 < a1 ? a0 : a1, kmax = a0 < a1 ? a1 : a0, kb = b0 < b1 ? b0 : b1;
+  const int key2 = kmax < kb ? kmax : kb;
   const int arg = key1 & 31;
   int m1 = ((key1 >> 5) * 13) >> 4, m2 = ((key2 >> 5) * 13) >> 4;
   m1 = m1 > 127 ? 127 : m1;
   m2 = m2 > 127 ? 127 : m2;
-  // the new messages and posteriors two edges at a time in 16-bit halves (every value fits: |q| <= 8191 + 127,
-  // |msg| <= 127): the pair holding the first minimum takes m2 in that half, the others m1 in both
+  // pass 2, the new messages and posteriors in the same halves: the pair holding the first minimum takes m2 in that
+  // half, the others m1 in both
   const v2i16 m11 = {static_cast<int16_t>(m1), static_cast<int16_t>(m1)};
   const v2i16 m11d = (arg & 1) ? v2i16{static_cast<int16_t>(m1), static_cast<int16_t>(m2)}
                                : v2i16{static_cast<int16_t>(m2), static_cast<int16_t>(m1)};
   const int argpair = arg >> 1;
-  const int16_t sg = static_cast<int16_t>(sgn >> 16);  // the check's sign in bit 15
+  const int16_t sg = static_cast<int16_t>(sx ^ (sx >> 16));  // the check's sign in bit 15
   const v2i16 sgn2 = {sg, sg};
   const v2i16 hi = {8191, 8191}, lo = {-8191, -8191};
-  static_for<0, (NS + 1) / 2>([&](auto j) {
+  static_for<0, NP>([&](auto j) {
     constexpr int e0 = 2 * j, e1 = 2 * j + 1;
-    const v2i16 q2 = {static_cast<int16_t>(q[e0]), static_cast<int16_t>(e1 < NS ? q[e1] : 0)};
+    const v2i16 q = q2[j];
     const v2i16 mag = argpair == j ? m11d : m11;
-    const v2i16 s2 = (sgn2 ^ q2) >> 15;  // -1 where the other slots' signs make the message negative
+    const v2i16 s2 = (sgn2 ^ q) >> 15;  // -1 where the other slots' signs make the message negative
     const v2i16 msg = (mag ^ s2) - s2;
-    v2i16 t = q2 + msg;
+    v2i16 t = q + msg;
     t = t > hi ? hi : t;
     t = t < lo ? lo : t;
     uint32_t mw;

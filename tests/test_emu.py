@@ -53,3 +53,26 @@ def test_lanes_match_oracle_on_ties(emu, rate):
         got = emu.emu_ldpc_decode(rate, ptr(llr), ptr(out), 12, 0, 0.95, ptr(post), ptr(msg))
         assert got == its
         assert np.array_equal(out, bits) and np.array_equal(post, L) and np.array_equal(msg, R)
+
+
+@pytest.mark.parametrize("rate", RATES)
+def test_lanes_match_oracle_saturated(emu, rate):
+    """Channel values at +-80 with 0-3% of them wrong: most checks hold several magnitudes far above the message cap
+    (equal or not), and posteriors reach the largest sums of messages the column degrees allow."""
+    c = code(rate)
+    rng = np.random.default_rng(110 + rate)
+    for w in range(4):
+        if w < 2:
+            llr = rng.choice([-80, 80], c.n).astype(np.int8)  # no codeword near: every iteration runs
+        else:
+            _, llr = word(rate, 4.0, rng)
+            llr = (np.sign(llr.astype(np.int16)) * 80).astype(np.int8)
+            flip = rng.random(c.n) < 0.03 * (w - 1)
+            llr[flip] = -llr[flip]
+        its, bits, L, R = c.decode(llr, 20, 0, 0.95, trace=True)
+        out = np.zeros(c.n, np.uint8)
+        post = np.zeros(c.n, np.int16)
+        msg = np.zeros(c.slots_total * 128, np.int8)
+        got = emu.emu_ldpc_decode(rate, ptr(llr), ptr(out), 20, 0, 0.95, ptr(post), ptr(msg))
+        assert got == its
+        assert np.array_equal(out, bits) and np.array_equal(post, L) and np.array_equal(msg, R)
