@@ -120,9 +120,27 @@ __host__ __device__ LZ_INLINE void put_msg(uint32_t *w, int v) {
   w[S >> 2] = (w[S >> 2] & ~(0xFFu << sh)) | ((static_cast<uint32_t>(v) & 0xFFu) << sh);
 }
 
-// Byte offset of posterior (i - sh) mod 128 of a 128-entry block from 2 i: an add and a mask (the block's own offset
-// goes into the LDS instruction's immediate).
-__host__ __device__ LZ_INLINE int rot_off(int i2, int sh) { return (i2 + (256 - 2 * sh)) & 0xFF; }
+// Byte offset of posterior (i - SH) mod 128 of a 128-entry block from 2 i: (2 i + 256 - 2 SH) mod 256, the block's
+// own offset going into the LDS instruction's immediate. On the GPU one add whose result keeps only its low byte
+// (a sub-dword write, dst_sel BYTE_0) in place of an add and a mask; its constant is set in a scalar register within
+// the same statement, so no scalar register stays taken.
+template <int SH>
+__host__ __device__ LZ_INLINE int rot_off(int i2) {
+  if constexpr (SH == 0) {
+    return i2;
+  } else {
+#ifdef __HIP_DEVICE_COMPILE__
+    int r, c;
+    asm("s_movk_i32 %1, %2\n\tv_add_u32_sdwa %0, %1, %3 dst_sel:BYTE_0 dst_unused:UNUSED_PAD src0_sel:DWORD "
+        "src1_sel:DWORD"
+        : "=v"(r), "=&s"(c)
+        : "n"(256 - 2 * SH), "v"(i2));
+    return r;
+#else
+    return (i2 + (256 - 2 * SH)) & 0xFF;
+#endif
+  }
+}
 
 __host__ __device__ LZ_INLINE int16_t &at_byte(int16_t *base, int off) {
   return *reinterpret_cast<int16_t *>(reinterpret_cast<char *>(base) + off);
@@ -143,7 +161,7 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   int adr[NS];
   static_for<0, ND>([&](auto e) {
     constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
-    adr[e] = col * 256 + rot_off(i2, sh);
+    adr[e] = col * 256 + rot_off<sh>(i2);
   });
   adr[ND] = R > 0 ? 2 * (K + (R - 1) * 128) + i2 : 2 * (K + (S::MB - 1) * 128 - 1) + i2;
   adr[ND + 1] = 2 * (K + R * 128) + i2;
@@ -152,8 +170,11 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   // pass 1, two edges at a time in 16-bit halves (edge 2j low, 2j + 1 high; an odd row's last high half is a phantom
   // edge, q = 0, key above every real one): q = posterior - message, the sign product, and the key |q| << 5 | e. The
   // smallest key is the first smallest magnitude (the reference's strict "<" keeps the first of equal minima), the
-  // second smallest the next magnitude up or an equal later one. A key fits 16 bits: a posterior is its channel value
-  // plus its column's messages (the 8191 clamp never acts), so |q| <= 128 + 127 (column degree - 1) < 2047.
+  // second smallest the next magnitude up or an equal later one. A posterior is its channel value (|int8| <= 128)
+  // plus its column's messages (|m| <= 127), so it never reaches the reference's +-8191 clamp and |q| <= 128 + 127
+  // (column degree - 1) < 2047: a key fits 16 bits. The absent edge of check 0 (q = 8191) wraps to a key of at least
+  // (8191 << 5) mod 2^16 = 65504, above every real one.
+  static_assert(128 + 127 * S::max_col_degree() < 8191, "the posterior clamp never acts");
   static_assert(128 + 127 * (S::max_col_degree() - 1) < 2047, "keys fit 16 bits");
   constexpr int NP = (NS + 1) / 2;
   v2i16 q2[NP];
@@ -206,16 +227,13 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   const int argpair = arg >> 1;
   const int16_t sg = static_cast<int16_t>(sx ^ (sx >> 16));  // the check's sign in bit 15
   const v2i16 sgn2 = {sg, sg};
-  const v2i16 hi = {8191, 8191}, lo = {-8191, -8191};
   static_for<0, NP>([&](auto j) {
     constexpr int e0 = 2 * j, e1 = 2 * j + 1;
     const v2i16 q = q2[j];
     const v2i16 mag = argpair == j ? m11d : m11;
     const v2i16 s2 = (sgn2 ^ q) >> 15;  // -1 where the other slots' signs make the message negative
     const v2i16 msg = (mag ^ s2) - s2;
-    v2i16 t = q + msg;
-    t = t > hi ? hi : t;
-    t = t < lo ? lo : t;
+    const v2i16 t = q + msg;  // within +-8191 (see above): the reference's clamp is the identity here
     uint32_t mw;
     __builtin_memcpy(&mw, &msg, 4);
     if constexpr (e1 < NS) put_msg2<RB + e0>(M.m, mw);
@@ -247,7 +265,7 @@ __host__ __device__ LZ_INLINE int unsat(const int16_t *L, int lane) {
     int x = L[K + R * 128 + i];
     static_for<0, ND>([&](auto e) {
       constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
-      x ^= at_byte(L, col * 256 + rot_off(i2, sh));
+      x ^= at_byte(L, col * 256 + rot_off<sh>(i2));
     });
     if (R > 0) x ^= L[K + (R - 1) * 128 + i];
     else x ^= i > 0 ? L[K + (S::MB - 1) * 128 + i - 1] : 0;
@@ -270,7 +288,7 @@ __host__ __device__ LZ_INLINE int unsat_any(const int16_t *L, int lane) {
     int x = L[K + R * 128 + i];
     static_for<0, ND>([&](auto e) {
       constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
-      x ^= at_byte(L, col * 256 + rot_off(i2, sh));
+      x ^= at_byte(L, col * 256 + rot_off<sh>(i2));
     });
     if (R > 0) x ^= L[K + (R - 1) * 128 + i];
     else x ^= i > 0 ? L[K + (S::MB - 1) * 128 + i - 1] : 0;
