@@ -149,6 +149,25 @@ __host__ __device__ LZ_INLINE int16_t at_byte(const int16_t *base, int off) {
   return *reinterpret_cast<const int16_t *>(reinterpret_cast<const char *>(base) + off);
 }
 
+// Per 16-bit half of q: 16 |q| + J as max(16 q + J, J - 16 q) (|q| < 2048). On the GPU two packed multiply-adds
+// whose constants are inline (16, -16 and J broadcast to both halves) and a packed max.
+template <int J>
+__host__ __device__ LZ_INLINE v2i16 key16(v2i16 q) {
+#ifdef __HIP_DEVICE_COMPILE__
+  v2i16 x, y;
+  asm("v_pk_mad_i16 %0, %1, 16, %2 op_sel_hi:[1,0,0]" : "=v"(x) : "v"(q), "n"(J));
+  asm("v_pk_mad_i16 %0, %1, -16, %2 op_sel_hi:[1,0,0]" : "=v"(y) : "v"(q), "n"(J));
+  return x > y ? x : y;
+#else
+  v2i16 r;
+  for (int h = 0; h < 2; h++) {
+    const int a = q[h] < 0 ? -q[h] : q[h];
+    r[h] = static_cast<int16_t>(16 * a + J);
+  }
+  return r;
+#endif
+}
+
 // Block row R of code CI for the lane's check position i.
 template <int CI, int R>
 __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
@@ -165,26 +184,29 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   });
   adr[ND] = R > 0 ? 2 * (K + (R - 1) * 128) + i2 : 2 * (K + (S::MB - 1) * 128 - 1) + i2;
   adr[ND + 1] = 2 * (K + R * 128) + i2;
-  // check 0 (block row 0, position 0) has no parity bit before it: read as a certain 0 of 8191, message zeroed
+  // check 0 (block row 0, position 0) has no parity bit before it: the reference reads it as a certain 0 of 8191 with
+  // its message zeroed. Any value above every real |q| gives the same messages and posteriors (it is never a
+  // minimum, adds no sign and is not written back); 2000 keeps its key within 16 bits.
   const bool absent = R == 0 && i == 0;
   // pass 1, two edges at a time in 16-bit halves (edge 2j low, 2j + 1 high; an odd row's last high half is a phantom
-  // edge, q = 0, key above every real one): q = posterior - message, the sign product, and the key |q| << 5 | e. The
-  // smallest key is the first smallest magnitude (the reference's strict "<" keeps the first of equal minima), the
-  // second smallest the next magnitude up or an equal later one. A posterior is its channel value (|int8| <= 128)
-  // plus its column's messages (|m| <= 127), so it never reaches the reference's +-8191 clamp and |q| <= 128 + 127
-  // (column degree - 1) < 2047: a key fits 16 bits. The absent edge of check 0 (q = 8191) wraps to a key of at least
-  // (8191 << 5) mod 2^16 = 65504, above every real one.
+  // edge, q = 0, key above every real one): q = posterior - message, the sign product, and per half the key
+  // 16 |q| + j (j < 16). Doubled, plus the half, that is |q| << 5 | e: the smallest is the first smallest magnitude
+  // (the reference's strict "<" keeps the first of equal minima), the second smallest the next magnitude up or an
+  // equal later one. A posterior is its channel value (|int8| <= 128) plus its column's messages (|m| <= 127), so it
+  // never reaches the reference's +-8191 clamp and |q| <= 128 + 127 (column degree - 1) < 2047: 16 |q| + j fits a
+  // signed 16-bit half, computed as max(16 q + j, j - 16 q) (two packed multiply-adds and a packed max).
   static_assert(128 + 127 * S::max_col_degree() < 8191, "the posterior clamp never acts");
-  static_assert(128 + 127 * (S::max_col_degree() - 1) < 2047, "keys fit 16 bits");
+  static_assert(16 * 2000 + 15 < 32767 && 128 + 127 * (S::max_col_degree() - 1) < 2000, "keys fit 16 bits");
   constexpr int NP = (NS + 1) / 2;
+  static_assert(NP <= 16, "pair indices fit the key's four bits");
   v2i16 q2[NP];
-  v2u16 k1 = {0xFFFF, 0xFFFF}, k2 = {0xFFFF, 0xFFFF};
+  v2i16 k1 = {0x7FFF, 0x7FFF}, k2 = {0x7FFF, 0x7FFF};
   uint32_t sx = 0;
   static_for<0, NP>([&](auto j) {
     constexpr int e0 = 2 * j, e1 = 2 * j + 1;
     int p0 = at_byte(L, adr[e0]), m0 = get_msg<RB + e0>(M.m);
     if (e0 == ND && absent) {
-      p0 = 8191;
+      p0 = 2000;
       m0 = 0;
     }
     int p1 = 0, m1 = 0;
@@ -192,7 +214,7 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
       p1 = at_byte(L, adr[e1]);
       m1 = get_msg<RB + e1>(M.m);
       if (e1 == ND && absent) {
-        p1 = 8191;
+        p1 = 2000;
         m1 = 0;
       }
     }
@@ -201,18 +223,14 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
     v2i16 q;
     __builtin_memcpy(&q, &w, 4);
     q2[j] = q;
-    const v2i16 nq = -q;
-    const v2i16 a = q > nq ? q : nq;
-    v2u16 au;
-    __builtin_memcpy(&au, &a, 4);
-    v2u16 key = (au << 5) | v2u16{static_cast<uint16_t>(e0), static_cast<uint16_t>(e1)};
-    if constexpr (e1 >= NS) key[1] = 0xFFFF;
-    const v2u16 lo = k1 < key ? k1 : key, hi = k1 < key ? key : k1;
+    v2i16 key = key16<j>(q);
+    if constexpr (e1 >= NS) key[1] = 0x7FFF;
+    const v2i16 lo = k1 < key ? k1 : key, hi = k1 < key ? key : k1;
     k2 = hi < k2 ? hi : k2;  // the middle of k1 <= k2 and key
     k1 = lo;
   });
-  // the halves merged (keys are distinct: the edge index is in them)
-  const int a0 = k1[0], a1 = k1[1], b0 = k2[0], b1 = k2[1];
+  // the halves merged as |q| << 5 | e (distinct: the edge index is in them)
+  const int a0 = 2 * k1[0], a1 = 2 * k1[1] + 1, b0 = 2 * k2[0], b1 = 2 * k2[1] + 1;
   const int key1 = a0 < a1 ? a0 : a1, kmax = a0 < a1 ? a1 : a0, kb = b0 < b1 ? b0 : b1;
   const int key2 = kmax < kb ? kmax : kb;
   const int arg = key1 & 31;
