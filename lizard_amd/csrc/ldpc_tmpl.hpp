@@ -209,16 +209,27 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
       p0 = 2000;
       m0 = 0;
     }
-    int p1 = 0, m1 = 0;
+    uint32_t w = static_cast<uint16_t>(p0 - m0);
     if constexpr (e1 < NS) {
-      p1 = at_byte(L, adr[e1]);
-      m1 = get_msg<RB + e1>(M.m);
-      if (e1 == ND && absent) {
-        p1 = 2000;
-        m1 = 0;
+      int p1 = at_byte(L, adr[e1]);
+#ifdef __HIP_DEVICE_COMPILE__
+      if constexpr (!(R == 0 && e1 == ND)) {
+        // the high half written in place: posterior minus the sign-extended message byte, the low half kept
+        asm("v_sub_u16_sdwa %0, %1, sext(%2) dst_sel:WORD_1 dst_unused:UNUSED_PRESERVE src0_sel:WORD_0 "
+            "src1_sel:BYTE_%3"
+            : "+v"(w)
+            : "v"(p1), "v"(M.m[(RB + e1) >> 2]), "n"((RB + e1) & 3));
+      } else
+#endif
+      {
+        int m1 = get_msg<RB + e1>(M.m);
+        if (e1 == ND && absent) {
+          p1 = 2000;
+          m1 = 0;
+        }
+        w |= static_cast<uint32_t>(p1 - m1) << 16;
       }
     }
-    const uint32_t w = static_cast<uint32_t>(static_cast<uint16_t>(p0 - m0)) | (static_cast<uint32_t>(p1 - m1) << 16);
     sx ^= w;
     v2i16 q;
     __builtin_memcpy(&q, &w, 4);
