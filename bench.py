@@ -9,8 +9,9 @@ layered min-sum, at most 20 iterations with early termination, one persistent ke
 the GPU. Inputs are resident in HBM before the timed region.
 
 Prints one JSON line: LDPC Gbit/s (3840 x codewords / s) as the value, payload MB/s (472 x verified blocks / s),
-the plain per-codeword path's rate beside the batched one, the LDS roofline of the decode kernel and the reference
-CPU decoder timed on this machine's cores. Two more workloads ride along as sub-objects:
+the plain per-codeword path's rate beside the batched one, the decode kernel's roofline (integer operations against
+the vector issue peak, with the LDS and HBM figures beside it) and the reference CPU decoder timed on this machine's
+cores. Two more workloads ride along as sub-objects:
   "pipeline"   the full receive path, batched: 32 captured 1920x1080 uint8 frames, each an n = 1536 picture
                (LIZARD-1024, 124 blocks) in the 256 ring through a random homography (rotation +-10 deg, scale 0.6-0.9
                of the frame height, perspective up to 0.05), radial k1 in [-0.05, 0.05], blur sigma 0.5-1.5 px, AWGN
@@ -44,6 +45,11 @@ MAX_ITER = 20
 # read bandwidth for some instructions, which would halve the fraction below). HBM3E 8 TB/s.
 LDS_PEAK_GBS = 256 * 128 * 2.4
 HBM_PEAK_GBS = 8000.0
+# Vector integer issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, two 16-bit operations a lane (packed) a clock.
+VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 * 2 / 1e12
+# Integer operations of the reference's scalar layer kernel per edge per iteration (liblizard/src/ldpc.c:171-191):
+# subtract, magnitude, sign, first- and second-minimum updates; magnitude select, sign, add, clamp (two).
+OPS_PER_EDGE = 10
 
 
 def build_batch(ebn0_db, seed):
@@ -79,12 +85,6 @@ def build_batch(ebn0_db, seed):
     return rates, llr, offs, payload, info
 
 
-def lds_bytes_per_iteration(rate):
-    # algorithmic LDS traffic of one layered iteration: posterior read + write (2 + 2 B) and message read + write
-    # (1 + 1 B) per edge of H
-    return 6 * lizard_amd.ldpc_dims(rate)["edges"]
-
-
 def measured_traffic():
     """HBM bytes a k_ldpc launch moves, from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-kernel
     calibrated); None where that summary is absent."""
@@ -96,6 +96,21 @@ def measured_traffic():
     except OSError:
         pass
     return None
+
+
+def measured_issue():
+    """k_ldpc's vector-instruction issue over the timed launches from the committed rocprofv3 SQ counter summary:
+    {"valu_insts_per_codeword_iteration", "valu_busy"}; None where that summary is absent."""
+    path = os.path.join(ROOT, "profiles", "k_ldpc_issue_ldpc_batched_4096.txt")
+    out = {}
+    try:
+        for line in open(path):
+            for key in ("valu_insts_per_codeword_iteration", "valu_busy"):
+                if line.startswith(key + ":"):
+                    out[key] = float(line.split(":")[1].split()[0])
+    except OSError:
+        return None
+    return out or None
 
 
 def effective_iterations(its):
@@ -570,15 +585,21 @@ def main():
     gbps = 3840.0 * codewords / wall / 1e9
     payload_mbps = 472.0 * verified / wall / 1e6
 
-    # LDS roofline of the decode kernel: algorithmic bytes of the iterations actually run, per launch
-    lds_per_launch = []
+    # rooflines of the decode kernel over the iterations actually run, per launch: integer operations (the bound:
+    # the messages live in registers, so the kernel issues vector instructions rather than moving LDS bytes) and the
+    # algorithmic LDS bytes of the reference layout
+    edge_its, lds_per_launch = [], []
     for p in range(len(EBN0)):
         rates = batches[p][0]
-        per_it = np.array([lds_bytes_per_iteration(int(r)) for r in rates], np.int64)
-        lds_per_launch.append(float((per_it * effective_iterations(its_all[p])).sum()))
+        edges = np.array([lizard_amd.ldpc_dims(int(r))["edges"] for r in rates], np.int64)
+        eff = effective_iterations(its_all[p])
+        edge_its.append(float((edges * eff).sum()))
+        lds_per_launch.append(float((6 * edges * eff).sum()))
     lds_bytes = sum(lds_per_launch[s % len(EBN0)] for s in range(args.steps)) / args.steps
+    ops = OPS_PER_EDGE * sum(edge_its[s % len(EBN0)] for s in range(args.steps)) / args.steps
     launch_ms = ev_ms / args.steps
     lds_gbs = lds_bytes / (launch_ms * 1e-3) / 1e9
+    tops = ops / (launch_ms * 1e-3) / 1e12
     hbm_bytes = float(sum(b[1].size for b in batches) / len(batches) + count * (476 + 2))
 
     plain = None
@@ -627,9 +648,12 @@ def main():
         "plain_Gbps": round(plain, 4) if plain else None,
         "batched_over_plain": round(gbps / plain, 2) if plain else None,
         "points": points,
-        "roofline": {"bound": "lds", "kernel": "k_ldpc", "achieved": round(lds_gbs, 1), "peak": LDS_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "traffic": measured_traffic(),
-                     "launch_ms": round(launch_ms, 4), "lds_bytes_per_launch": lds_bytes,
+        "roofline": {"bound": "valu", "kernel": "k_ldpc", "achieved": round(tops, 2), "peak": VALU_PEAK_TOPS,
+                     "unit": "TOP/s", "frac": round(tops / VALU_PEAK_TOPS, 4), "traffic": measured_traffic(),
+                     "launch_ms": round(launch_ms, 4), "ops_per_launch": ops, "ops_per_edge": OPS_PER_EDGE,
+                     "issue": measured_issue(),
+                     "lds": {"achieved": round(lds_gbs, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(lds_gbs / LDS_PEAK_GBS, 4), "bytes_per_launch": lds_bytes},
                      "iterations_per_s": round(sum(float(effective_iterations(its_all[s % len(EBN0)]).sum())
                                                    for s in range(args.steps)) / (ev_ms * 1e-3), 1),
                      "hbm": {"achieved": round(hbm_bytes / (launch_ms * 1e-3) / 1e9, 2), "peak": HBM_PEAK_GBS,

@@ -68,7 +68,7 @@ struct Msgs {
 };
 
 // Keeps the compiler from hoisting every block row's addresses out of the iteration loop (they depend only on the
-// lane, so all of them would stay live in VGPRs): the lane index is re-read, opaquely, once per block row.
+// lane, so all of them would stay live in VGPRs): the lane's offset 2 i is re-read, opaquely, once per block row.
 __host__ __device__ LZ_INLINE int fresh(int i) {
 #ifdef __HIP_DEVICE_COMPILE__
   asm volatile("" : "+v"(i));
@@ -172,7 +172,7 @@ __host__ __device__ LZ_INLINE v2i16 key16(v2i16 q) {
 template <int CI, int R>
 __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   using S = Shape<CI>;
-  const int i = fresh(lane), i2 = 2 * i;
+  const int i2 = fresh(2 * lane);
   constexpr int D0 = S::d0(R), ND = S::nd(R), NS = ND + 2, RB = S::rbase(R), K = S::K;
   static_assert(NS <= 32, "edge indices fit the key's five bits");
   // the check's slots: its circulants' data bits, the parity bit before it, its own (ldpc.c:589-602); byte offsets
@@ -187,7 +187,7 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   // check 0 (block row 0, position 0) has no parity bit before it: the reference reads it as a certain 0 of 8191 with
   // its message zeroed. Any value above every real |q| gives the same messages and posteriors (it is never a
   // minimum, adds no sign and is not written back); 2000 keeps its key within 16 bits.
-  const bool absent = R == 0 && i == 0;
+  const bool absent = R == 0 && i2 == 0;
   // pass 1, two edges at a time in 16-bit halves (edge 2j low, 2j + 1 high; an odd row's last high half is a phantom
   // edge, q = 0, key above every real one): q = posterior - message, the sign product, and per half the key
   // 16 |q| + j (j < 16). Doubled, plus the half, that is |q| << 5 | e: the smallest is the first smallest magnitude
@@ -290,14 +290,14 @@ __host__ __device__ LZ_INLINE int unsat(const int16_t *L, int lane) {
   int bad = 0;
   static_for<0, S::MB>([&](auto r) {
     constexpr int R = r, D0 = S::d0(R), ND = S::nd(R), K = S::K;
-    const int i = fresh(lane), i2 = 2 * i;
-    int x = L[K + R * 128 + i];
+    const int i2 = fresh(2 * lane);
+    int x = at_byte(L, 2 * (K + R * 128) + i2);
     static_for<0, ND>([&](auto e) {
       constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
       x ^= at_byte(L, col * 256 + rot_off<sh>(i2));
     });
-    if (R > 0) x ^= L[K + (R - 1) * 128 + i];
-    else x ^= i > 0 ? L[K + (S::MB - 1) * 128 + i - 1] : 0;
+    if (R > 0) x ^= at_byte(L, 2 * (K + (R - 1) * 128) + i2);
+    else x ^= i2 > 0 ? at_byte(L, 2 * (K + (S::MB - 1) * 128 - 1) + i2) : 0;
     bad += x < 0;
   });
   return bad;
@@ -313,14 +313,14 @@ __host__ __device__ LZ_INLINE int unsat_any(const int16_t *L, int lane) {
   static_for<0, S::MB>([&](auto r) {
     if (found) return;
     constexpr int R = r, D0 = S::d0(R), ND = S::nd(R), K = S::K;
-    const int i = fresh(lane), i2 = 2 * i;
-    int x = L[K + R * 128 + i];
+    const int i2 = fresh(2 * lane);
+    int x = at_byte(L, 2 * (K + R * 128) + i2);
     static_for<0, ND>([&](auto e) {
       constexpr int col = S::col(D0 + e), sh = S::shift(D0 + e);
       x ^= at_byte(L, col * 256 + rot_off<sh>(i2));
     });
-    if (R > 0) x ^= L[K + (R - 1) * 128 + i];
-    else x ^= i > 0 ? L[K + (S::MB - 1) * 128 + i - 1] : 0;
+    if (R > 0) x ^= at_byte(L, 2 * (K + (R - 1) * 128) + i2);
+    else x ^= i2 > 0 ? at_byte(L, 2 * (K + (S::MB - 1) * 128 - 1) + i2) : 0;
 #ifdef __HIP_DEVICE_COMPILE__
     found = __builtin_amdgcn_ballot_w64(x < 0) != 0;
 #else
