@@ -5,7 +5,7 @@
 //   k_rows       eight rows a workgroup, two a wave packed as one complex sequence: the warp (or a supplied grid) read
 //                on load with the anchors' samples erased, the detrend row sums, the wave FFT along x, the two rows'
 //                transforms separated and pruned to u < E, written column-major
-//   k_cols       one column u a wave: the wave FFT along y, keeping v < E and n - v (the upper half-plane's reach)
+//   k_cols       one column u a wave: the wave FFT along y, the column's coefficients scattered into coefficient order
 //   k_align      one workgroup a frame: the detrend projections and the pilots' two-pass least-squares grid shift
 //   k_demap      one wave a sub-channel: 320 coefficients gathered, detrended, turned, their moments, int8 LLRs
 // The wave FFT (WaveFft): n = 64 M points, M in registers a lane, an M-point DFT in registers, then two radix-8 passes
