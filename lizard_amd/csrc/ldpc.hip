@@ -80,8 +80,42 @@ __device__ __forceinline__ void decode_one(const LdpcParams &p, const LdpcTask &
   // :1823)
   if (p.mode == MODE_CODEWORD) {
     const int8_t *src = p.llr + tk.llr;
-    for (int v = lane; v < K; v += LDPC_THREADS) L[v] = src[v];
-    for (int v = lane; v < N - K; v += LDPC_THREADS) L[K + (v % MB) * Z + v / MB] = src[K + v];
+    if ((reinterpret_cast<uintptr_t>(src) & 3) == 0) {
+      // four channel values a load, every load of the codeword in flight before the first store; K is a multiple of
+      // 4, so a word is all data bits or all parity bits
+      static_assert(K % 4 == 0 && N % 4 == 0, "whole words");
+      constexpr int NW = N / 4, R = (NW + LDPC_THREADS - 1) / LDPC_THREADS;
+      const uint32_t *s4 = reinterpret_cast<const uint32_t *>(src);
+      const int ln = tm::fresh(lane);  // lane-only addresses stay here, not hoisted out of the task loop
+      uint32_t w[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = ln + LDPC_THREADS * r;
+        w[r] = (NW % LDPC_THREADS == 0 || i < NW) ? s4[i] : 0u;
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = ln + LDPC_THREADS * r;
+        if (NW % LDPC_THREADS != 0 && i >= NW) break;
+        const int16_t x0 = static_cast<int8_t>(w[r]), x1 = static_cast<int8_t>(w[r] >> 8),
+                      x2 = static_cast<int8_t>(w[r] >> 16), x3 = static_cast<int8_t>(w[r] >> 24);
+        if (4 * i < K) {
+          L[4 * i] = x0;
+          L[4 * i + 1] = x1;
+          L[4 * i + 2] = x2;
+          L[4 * i + 3] = x3;
+        } else {
+          const int t = 4 * i - K;
+          L[K + (t % MB) * Z + t / MB] = x0;
+          L[K + ((t + 1) % MB) * Z + (t + 1) / MB] = x1;
+          L[K + ((t + 2) % MB) * Z + (t + 2) / MB] = x2;
+          L[K + ((t + 3) % MB) * Z + (t + 3) / MB] = x3;
+        }
+      }
+    } else {
+      for (int v = lane; v < K; v += LDPC_THREADS) L[v] = src[v];
+      for (int v = lane; v < N - K; v += LDPC_THREADS) L[K + (v % MB) * Z + v / MB] = src[K + v];
+    }
   } else {
     const int8_t *src = p.llr + tk.llr;
     const uint8_t *wh = p.white + tk.slot0;
