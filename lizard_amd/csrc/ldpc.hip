@@ -153,17 +153,35 @@ __device__ __forceinline__ void decode_one(const LdpcParams &p, const LdpcTask &
     for (int v = lane; v < K; v += LDPC_THREADS) p.bits[tk.bits + v] = L[v] < 0;
     for (int v = lane; v < N - K; v += LDPC_THREADS) p.bits[tk.bits + K + v] = L[K + (v % MB) * Z + v / MB] < 0;
   }
+  if (p.flags & 2) {
+    for (int v = lane; v < N; v += LDPC_THREADS) p.post[tk.bits + v] = L[v];
+    tm::dump_msgs<CI>(M, p.msg + static_cast<int64_t>(tk.out) * (MAX_SLOTS_TOTAL * Z), lane);
+  }
   if (p.flags & 1) {
+    // the CRC as the XOR of one table word per set bit: every table read of the lane in flight at once
+    constexpr int RB = (BLOCK_BYTES + LDPC_THREADS - 1) / LDPC_THREADS;
+    const int ln = tm::fresh(lane);
+    uint32_t tab[RB][8];
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int j = ln + LDPC_THREADS * r < BLOCK_BYTES ? ln + LDPC_THREADS * r : BLOCK_BYTES - 1;
+#pragma unroll
+      for (int q = 0; q < 8; q++) tab[r][q] = p.crc_tab[8 * j + q];
+    }
     uint32_t x = 0;
-    for (int j = lane; j < BLOCK_BYTES; j += LDPC_THREADS) {
+#pragma unroll
+    for (int r = 0; r < RB; r++) {
+      const int j = ln + LDPC_THREADS * r;
+      const bool in = j < BLOCK_BYTES;
+      const int jj = in ? j : BLOCK_BYTES - 1;
       int b = 0;
 #pragma unroll
       for (int q = 0; q < 8; q++) {
-        const int bit = L[8 * j + q] < 0;
+        const int bit = L[8 * jj + q] < 0;
         b = (b << 1) | bit;
-        x ^= bit ? p.crc_tab[8 * j + q] : 0u;
+        x ^= (in && bit) ? tab[r][q] : 0u;
       }
-      p.bytes[static_cast<int64_t>(tk.out) * BLOCK_BYTES + j] = static_cast<uint8_t>(b);
+      if (in) p.bytes[static_cast<int64_t>(tk.out) * BLOCK_BYTES + j] = static_cast<uint8_t>(b);
     }
     x = group_xor(x, reinterpret_cast<uint32_t *>(red) + 4, lane);
     if (lane == 0) {
@@ -171,10 +189,6 @@ __device__ __forceinline__ void decode_one(const LdpcParams &p, const LdpcTask &
       for (int q = 0; q < 32; q++) rx = (rx << 1) | (L[8 * BLOCK_BYTES + q] < 0);
       p.crc_ok[tk.out] = used > 0 && (p.crc0 ^ x) == rx;
     }
-  }
-  if (p.flags & 2) {
-    for (int v = lane; v < N; v += LDPC_THREADS) p.post[tk.bits + v] = L[v];
-    tm::dump_msgs<CI>(M, p.msg + static_cast<int64_t>(tk.out) * (MAX_SLOTS_TOTAL * Z), lane);
   }
   if (lane == 0) p.its[tk.out] = static_cast<int8_t>(used);
 }
