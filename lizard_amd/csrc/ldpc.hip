@@ -121,9 +121,38 @@ __device__ __forceinline__ void decode_one(const LdpcParams &p, const LdpcTask &
     const uint8_t *wh = p.white + tk.slot0;
     const int16_t *s2p = p.s2p[CI];
     for (int v = K - NP + lane; v < K; v += LDPC_THREADS) L[v] = v >= INFO_BITS ? 127 : 0;
-    for (int s = lane; s < NT; s += LDPC_THREADS) {
-      const int x = src[s];
-      L[s2p[s]] = static_cast<int16_t>(static_cast<int8_t>(wh[s] ? -x : x));
+    if (NT % 4 == 0 && ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(wh)) & 3) == 0 &&
+        (reinterpret_cast<uintptr_t>(s2p) & 7) == 0) {
+      // four slots a load from each of the soft values, the whitening and the bit map, all in flight at once
+      constexpr int NW = NT / 4, R = (NW + LDPC_THREADS - 1) / LDPC_THREADS;
+      const int ln = tm::fresh(lane);
+      uint32_t xs[R], ws[R];
+      uint2 ps[R];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = ln + LDPC_THREADS * r;
+        const bool in = NW % LDPC_THREADS == 0 || i < NW;
+        xs[r] = in ? reinterpret_cast<const uint32_t *>(src)[i] : 0u;
+        ws[r] = in ? reinterpret_cast<const uint32_t *>(wh)[i] : 0u;
+        ps[r] = in ? reinterpret_cast<const uint2 *>(s2p)[i] : make_uint2(0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        const int i = ln + LDPC_THREADS * r;
+        if (NW % LDPC_THREADS != 0 && i >= NW) break;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+          const int x = static_cast<int8_t>(xs[r] >> (8 * b));
+          const uint32_t pw = b < 2 ? ps[r].x : ps[r].y;
+          const int at = static_cast<int16_t>(pw >> (16 * (b & 1)));
+          L[at] = static_cast<int16_t>(static_cast<int8_t>(((ws[r] >> (8 * b)) & 0xFF) ? -x : x));
+        }
+      }
+    } else {
+      for (int s = lane; s < NT; s += LDPC_THREADS) {
+        const int x = src[s];
+        L[s2p[s]] = static_cast<int16_t>(static_cast<int8_t>(wh[s] ? -x : x));
+      }
     }
     for (int v = INFO_BITS + lane; v < K; v += LDPC_THREADS) L[v] = 127;
   }
