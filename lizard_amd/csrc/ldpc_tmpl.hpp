@@ -240,20 +240,25 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
     k2 = hi < k2 ? hi : k2;  // the middle of k1 <= k2 and key
     k1 = lo;
   });
-  // the halves merged as |q| << 5 | e (distinct: the edge index is in them)
-  const int a0 = 2 * k1[0], a1 = 2 * k1[1] + 1, b0 = 2 * k2[0], b1 = 2 * k2[1] + 1;
-  const int key1 = a0 < a1 ? a0 : a1, kmax = a0 < a1 ? a1 : a0, kb = b0 < b1 ? b0 : b1;
+  // the halves merged: the first minimum is the smaller half key, the low half's on equal keys (edge 2j comes before
+  // 2j + 1); the second the smaller of the other half's first and both halves' second (only its magnitude counts)
+  const int a0 = k1[0], a1 = k1[1], b0 = k2[0], b1 = k2[1];
+  const bool hi_first = a1 < a0;
+  const int key1 = hi_first ? a1 : a0, kmax = hi_first ? a0 : a1, kb = b0 < b1 ? b0 : b1;
   const int key2 = kmax < kb ? kmax : kb;
-  const int arg = key1 & 31;
-  int m1 = ((key1 >> 5) * 13) >> 4, m2 = ((key2 >> 5) * 13) >> 4;
-  m1 = m1 > 127 ? 127 : m1;
-  m2 = m2 > 127 ? 127 : m2;
+  // (m1, m2) = min((|q| 13) >> 4, 127) in the two halves at once, |q| = key >> 4 (|q| 13 < 2^16)
+  v2u16 mm = {static_cast<uint16_t>(key1), static_cast<uint16_t>(key2)};
+  mm = ((mm >> 4) * static_cast<uint16_t>(13)) >> 4;
+  const v2u16 cap = {127, 127};
+  mm = mm < cap ? mm : cap;
+  v2i16 m12, m21;
+  __builtin_memcpy(&m12, &mm, 4);
+  m21 = v2i16{m12[1], m12[0]};
   // pass 2, the new messages and posteriors in the same halves: the pair holding the first minimum takes m2 in that
   // half, the others m1 in both
-  const v2i16 m11 = {static_cast<int16_t>(m1), static_cast<int16_t>(m1)};
-  const v2i16 m11d = (arg & 1) ? v2i16{static_cast<int16_t>(m1), static_cast<int16_t>(m2)}
-                               : v2i16{static_cast<int16_t>(m2), static_cast<int16_t>(m1)};
-  const int argpair = arg >> 1;
+  const v2i16 m11 = {m12[0], m12[0]};
+  const v2i16 m11d = hi_first ? m12 : m21;
+  const int argpair = key1 & 15;
   const int16_t sg = static_cast<int16_t>(sx ^ (sx >> 16));  // the check's sign in bit 15
   const v2i16 sgn2 = {sg, sg};
   static_for<0, NP>([&](auto j) {
