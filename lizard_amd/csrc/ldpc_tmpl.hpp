@@ -251,22 +251,25 @@ __host__ __device__ LZ_INLINE void layer(int16_t *L, Msgs<CI> &M, int lane) {
   mm = ((mm >> 4) * static_cast<uint16_t>(13)) >> 4;
   const v2u16 cap = {127, 127};
   mm = mm < cap ? mm : cap;
-  v2i16 m12, m21;
+  // the magnitudes with the check's sign product (XOR of every q's sign) applied once: an edge's message is that,
+  // negated where its own q is negative (the product of the other edges' signs)
+  const int16_t sg = static_cast<int16_t>(sx ^ (sx >> 16));  // the check's sign in bit 15
+  const v2i16 cs = v2i16{sg, sg} >> 15;
+  v2i16 m12;
   __builtin_memcpy(&m12, &mm, 4);
-  m21 = v2i16{m12[1], m12[0]};
+  m12 = (m12 ^ cs) - cs;
+  const v2i16 m21 = {m12[1], m12[0]};
   // pass 2, the new messages and posteriors in the same halves: the pair holding the first minimum takes m2 in that
   // half, the others m1 in both
   const v2i16 m11 = {m12[0], m12[0]};
   const v2i16 m11d = hi_first ? m12 : m21;
   const int argpair = key1 & 15;
-  const int16_t sg = static_cast<int16_t>(sx ^ (sx >> 16));  // the check's sign in bit 15
-  const v2i16 sgn2 = {sg, sg};
   static_for<0, NP>([&](auto j) {
     constexpr int e0 = 2 * j, e1 = 2 * j + 1;
     const v2i16 q = q2[j];
     const v2i16 mag = argpair == j ? m11d : m11;
-    const v2i16 s2 = (sgn2 ^ q) >> 15;  // -1 where the other slots' signs make the message negative
-    const v2i16 msg = (mag ^ s2) - s2;
+    const v2i16 qs = q >> 15;  // -1 where the edge's own q is negative
+    const v2i16 msg = (mag ^ qs) - qs;
     const v2i16 t = q + msg;  // within +-8191 (see above): the reference's clamp is the identity here
     uint32_t mw;
     __builtin_memcpy(&mw, &msg, 4);
