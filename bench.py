@@ -268,12 +268,13 @@ def bench_fft_demap(args):
     ns = rng.choice(sorted(table), size=args.fft_frames)
     versions = [table[int(n)][0] for n in ns]
     spans = [table[int(n)][1] for n in ns]
-    keep, items = [], []
+    keep, items, host = [], [], []
     total = 0
-    for v in versions:
+    for v, sp in zip(versions, spans):
         pic = sim.picture(v, sim.random_blocks(v, rng)).numpy()
         # AWGN at 20 dB SNR against the picture's own variance
         grid = (pic + rng.standard_normal(pic.shape) * np.sqrt(pic.var() / 100.0)).astype(np.float32)
+        host.append(dict(grid=grid, version=v, span=sp))
         d = lizard_amd.DeviceArray.of(grid)
         keep.append(d)
         items.append(dict(grid=d))
@@ -289,7 +290,10 @@ def bench_fft_demap(args):
     b.close()
     for d in keep:
         d.free()
-    return {"workload": "%d picture grids, n drawn from {384, 768, 1152, 1536} (%s), top formats, sender picture + "
+    cpu = None if args.no_cpu else cpu_fft_demap(host, args.cpu_seconds)
+    return {"cpu_baseline": cpu,
+            "graph_over_cpu": round(len(ns) / (ev * 1e-3) / cpu["value"], 1) if cpu and cpu["value"] > 0 else None,
+            "workload": "%d picture grids, n drawn from {384, 768, 1152, 1536} (%s), top formats, sender picture + "
                         "AWGN 20 dB; erase, detrend, 2D FFT, pilot shift, soft demap" %
                         (len(ns), ", ".join("%d x %d" % (int((ns == n).sum()), n) for n in sorted(table))),
             "graph_ms_per_batch": round(ev, 3), "frames_per_s": round(len(ns) / wall, 1),
@@ -299,6 +303,74 @@ def bench_fft_demap(args):
             "algorithmic_bytes_per_batch": int(total),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+
+
+def cpu_fft_demap(grids, seconds):
+    """The reference's finish path on the same picture grids (x86-64 build; erase, detrend, FFT, pilot shift, soft
+    values, then its LDPC, which at 20 dB converges in one iteration), one format instance per size and thread; grids
+    re-laid into its strip layout beforehand (picture.to_strips), copied in per call (the finish transforms in
+    place)."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+    if not os.path.exists(ref_so):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import picture as oracle_picture
+    lib = C.CDLL(ref_so)
+    lib.ref_format_new.restype = C.c_void_p
+    lib.ref_format_new.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.ref_format_free.argtypes = [C.c_void_p]
+    fn = lib.ref_finish_grid
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p] * 12
+    strips = []
+    for g in grids:
+        info = lizard_amd.format_info(g["version"])
+        strips.append(dict(s=oracle_picture.to_strips(g["grid"], info["n"]), n=info["n"], subch=info["subch"],
+                           span=g["span"], blocks=info["blocks"]))
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = max(1, min(cores, os.cpu_count() or 1, 16))
+    lock = threading.Lock()
+    state = {"next": 0, "frames": 0, "good": 0, "blocks": 0}
+    deadline = [0.0]
+
+    def worker():
+        forms = {}
+        work = np.empty(max(g["s"].size for g in strips), np.float32)
+        out = np.zeros(124 * 476, np.uint8)
+        ok, its, be = np.zeros(124, np.uint8), np.zeros(124, np.int8), np.zeros(124, np.float32)
+        while True:
+            with lock:
+                if time.perf_counter() > deadline[0]:
+                    break
+                g = strips[state["next"] % len(strips)]
+                state["next"] += 1
+            key = (g["n"], g["subch"], g["span"])
+            if key not in forms:
+                forms[key] = lib.ref_format_new(*key)
+            w = work[:g["s"].size]
+            w[:] = g["s"]
+            good = fn(forms[key], w.ctypes.data, out.ctypes.data, ok.ctypes.data, its.ctypes.data, be.ctypes.data,
+                      None, None, None, None, None, None)
+            with lock:
+                state["frames"] += 1
+                state["good"] += good
+                state["blocks"] += g["blocks"]
+        for f in forms.values():
+            lib.ref_format_free(f)
+
+    t0 = time.perf_counter()
+    deadline[0] = t0 + seconds
+    th = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    return {"value": round(state["frames"] / wall, 2), "unit": "frames/s", "cores": cores, "kind": "reference",
+            "blocks_verified": round(state["good"] / max(state["blocks"], 1), 4),
+            "sample": "%d grids (the same %d cycled), %.1f s wall on %d threads; liblizard lizard_finish on supplied "
+                      "grids (x86-64 SIMD build), which also runs its LDPC (one iteration at 20 dB)" %
+                      (state["frames"], len(strips), wall, cores)}
 
 
 def refine_frames(count, seed):
