@@ -83,6 +83,7 @@ def lib():
     sig("lizx_finish_grid", _I, _P, _I, _I, _P, _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_node_lattice", _I, _I, _P)
     sig("lizx_frame_slots", _I, _I, _P, _I, _P)
+    sig("lizx_modulate", _I, _P, _I, _I, _P, _I, _P)
     sig("lizx_format_coefs", _I, _I, _P, _P)
     sig("lizx_format_blocks", _I, _I, _P, _P, _P)
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
@@ -357,6 +358,16 @@ class Decoder:
         be = np.zeros(B, np.float32)
         good = check(lib().lizx_finish_llr(self.h, version, _p(llr), _p(est), _p(blocks), _p(ok), _p(its), _p(be)))
         return {"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be}
+
+    def modulate(self, version, blocks, cnt=0, out=None):
+        """The sender's pictures on the GPU (lizx_modulate): blocks (frames x blocks x 476 bytes, or one frame's
+        blocks x 476) -> a DeviceArray of frames x n x n float32 pictures (or into `out`, a DeviceArray that large)."""
+        info = format_info(version)
+        blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, info["blocks"] * LIZ_BLOCK)
+        frames, n = blocks.shape[0], info["n"]
+        d = out if out is not None else DeviceArray(frames * n * n * 4)
+        check(lib().lizx_modulate(self.h, version, frames, _p(blocks), cnt, d.ptr))
+        return d
 
     def warp(self, version, span, img, H, nodes):
         img = np.ascontiguousarray(img, np.uint8)

@@ -120,6 +120,9 @@ struct liz_decoder {
   int held_version = 0, held_span = 0;
   lizx_registration held_reg{};
   std::vector<float> held_nodes;
+  // the sender's pictures
+  DevBuf mod_bits, mod_G;
+  std::vector<uint8_t> mod_slots;
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
       if (p->version == version && p->span == span) return *p;
@@ -286,37 +289,43 @@ LIZ_API int lizx_format_info(int version, int *info) {
   });
 }
 
+// The sender's bit side (lizard_phy.c:1087-1150): every block's CRC-32C, LDPC codeword, bit map and pilot tail, the
+// padding past the last block, whitened; one byte (0/1) a frame slot.
+static void frame_slots_of(const Format *f, const uint8_t *blocks, int cnt, uint8_t *slots) {
+  std::memcpy(slots, f->white.data(), f->white.size());  // the padding: the whitening alone
+  std::vector<uint8_t> data(4096), cw(MAX_N);
+  for (int b = 0; b < f->blocks; b++) {
+    const Code &c = code(f->blk_code[b]);
+    const uint8_t *src = blocks + static_cast<size_t>(b) * BLOCK_BYTES;
+    std::fill(data.begin(), data.end(), 0);
+    for (int i = 0; i < 8 * BLOCK_BYTES; i++) data[i] = (src[i >> 3] >> (7 - (i & 7))) & 1;
+    const uint32_t crc = crc32c(src, BLOCK_BYTES);
+    for (int i = 0; i < 32; i++) data[8 * BLOCK_BYTES + i] = (crc >> (31 - i)) & 1;
+    encode_word(c, data.data(), cw.data());
+    // slot i: codeword bit at the bit map's position, or past nt the pilot bit (bit 0 of the count on even blocks,
+    // bit 1 on odd ones), whitened (SPEC 7.3)
+    const auto &s2p = f->slot_to_post[f->blk_code[b]];
+    const uint8_t q = static_cast<uint8_t>((cnt >> (b & 1)) & 1);
+    const int s0 = f->blk_slot[b], nb = f->blk_nslots[b];
+    for (int i = 0; i < nb; i++) {
+      uint8_t bit = q;
+      if (i < c.nt) {
+        // the map holds the posterior index of the codeword bit; back to the codeword index
+        const int pi = s2p[i];
+        const int v = pi < c.k ? pi : c.k + ((pi - c.k) % Z) * c.mb + (pi - c.k) / Z;
+        bit = cw[v];
+      }
+      slots[s0 + i] = static_cast<uint8_t>(bit ^ f->white[s0 + i]);
+    }
+  }
+}
+
 LIZ_API int lizx_frame_slots(int version, const uint8_t *blocks, int cnt, uint8_t *slots) {
   return guard([&] {
     need(blocks && slots && cnt >= 0 && cnt <= 3, "blocks, slots and a count 0 to 3");
     const Format *f = format_for(version);
     if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
-    std::memcpy(slots, f->white.data(), f->white.size());  // the padding: the whitening alone
-    std::vector<uint8_t> data(4096), cw(MAX_N);
-    for (int b = 0; b < f->blocks; b++) {
-      const Code &c = code(f->blk_code[b]);
-      const uint8_t *src = blocks + static_cast<size_t>(b) * BLOCK_BYTES;
-      std::fill(data.begin(), data.end(), 0);
-      for (int i = 0; i < 8 * BLOCK_BYTES; i++) data[i] = (src[i >> 3] >> (7 - (i & 7))) & 1;
-      const uint32_t crc = crc32c(src, BLOCK_BYTES);
-      for (int i = 0; i < 32; i++) data[8 * BLOCK_BYTES + i] = (crc >> (31 - i)) & 1;
-      encode_word(c, data.data(), cw.data());
-      // slot i: codeword bit at the bit map's position, or past nt the pilot bit (bit 0 of the count on even blocks,
-      // bit 1 on odd ones), whitened (SPEC 7.3)
-      const auto &s2p = f->slot_to_post[f->blk_code[b]];
-      const uint8_t q = static_cast<uint8_t>((cnt >> (b & 1)) & 1);
-      const int s0 = f->blk_slot[b], nb = f->blk_nslots[b];
-      for (int i = 0; i < nb; i++) {
-        uint8_t bit = q;
-        if (i < c.nt) {
-          // the map holds the posterior index of the codeword bit; back to the codeword index
-          const int pi = s2p[i];
-          const int v = pi < c.k ? pi : c.k + ((pi - c.k) % Z) * c.mb + (pi - c.k) / Z;
-          bit = cw[v];
-        }
-        slots[s0 + i] = static_cast<uint8_t>(bit ^ f->white[s0 + i]);
-      }
-    }
+    frame_slots_of(f, blocks, cnt, slots);
     return 0;
   });
 }
@@ -715,6 +724,29 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
   const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr);
   if (soft) copy_soft(d, p, soft, dbg);
   return good;
+}
+
+LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t *blocks, int cnt, float *d_pictures) {
+  return guard([&] {
+    need(d && blocks && d_pictures && frames > 0 && frames <= 4096 && cnt >= 0 && cnt <= 3,
+         "a decoder, 1 to 4096 frames of blocks, a pilot count 0 to 3 and device memory for the pictures");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    // any ring's plan of the format serves: the picture depends on n and the coefficient order only
+    const FramePlan &p = d->plan(version, f->n <= 384 ? 128 : f->n <= 768 ? 256 : 512);
+    const size_t S2 = f->white.size(), fb = static_cast<size_t>(f->blocks) * BLOCK_BYTES;
+    d->mod_slots.resize(S2 * frames);
+    for (int fr = 0; fr < frames; fr++) frame_slots_of(f, blocks + fb * fr, cnt, d->mod_slots.data() + S2 * fr);
+    d->mod_bits.reserve(d->mod_slots.size());
+    d->mod_G.reserve(static_cast<size_t>(frames) * p.E * p.n * sizeof(float2));
+    hip_check(hipMemcpyAsync(d->mod_bits.p, d->mod_slots.data(), d->mod_slots.size(), hipMemcpyHostToDevice, e.stream),
+              "frame slots");
+    launch_modulate(p, d->mod_bits.as<uint8_t>(), d->mod_G.as<float2>(), d_pictures, frames, e.stream);
+    e.sync();
+    return 0;
+  });
 }
 
 LIZ_API int lizx_node_lattice(int span, float *pos) {

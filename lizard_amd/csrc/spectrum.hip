@@ -1086,6 +1086,112 @@ void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, floa
   hip_check(hipGetLastError(), "warp launch");
 }
 
+// ------------------------------------------------------------------------------------------------ the sender's picture
+// Frame slots to the n x n picture (research/SPEC.md 6.5-6.7; the picture liblizard/src/lizard_phy.c:1087-1200
+// resamples into its drive): QPSK (+-1 +- i) / sqrt 2 at each coefficient (u, v), the Hermitian spectrum, the
+// unscaled inverse DFT exp(+2 pi i (u x + v y) / n) as conj(DFT(conj)) through the same wave FFT, clipped at
+// L = 2 sqrt(640 subch) into [0, 1]. Two passes, the receive path's in reverse order:
+//   k_mod_cols  one column u < E a wave: the column's coefficients (and, for u = 0, their Hermitian partners) set in
+//               the wave's LDS, the inverse DFT along v, G[frame][u][y] column-major
+//   k_mod_rows  two rows a wave: each row's spectrum G[u][y] (u < E) with conj G[u][y] at n - u, the two rows as one
+//               complex sequence, the inverse DFT along u, row y the real part and row y + 1 the imaginary
+
+// Frame slot pair (2 ci, 2 ci + 1) as its QPSK point.
+__device__ __forceinline__ float2 qpsk(const uint8_t *bits, int ci) {
+  constexpr float a = 0.70710678118654752f;
+  return make_float2(bits[2 * ci] ? -a : a, bits[2 * ci + 1] ? -a : a);
+}
+
+template <int N, int WV>
+__global__ __launch_bounds__(64 * WV) void k_mod_cols(PlanDev P, const uint8_t *bits_all, float2 *G) {
+  using F = WaveFft<N>;
+  constexpr int M = F::M;
+  extern __shared__ float2 zs[];  // WV waves x F::LDS (at least N points each)
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E;
+  const int u = blockIdx.x * WV + w;
+  if (u >= E) return;  // the whole wave; nothing below waits on the workgroup
+  float2 *T = zs + w * F::LDS;
+  const uint8_t *bits = bits_all + static_cast<size_t>(f) * P.subch * SUB * 2;
+#pragma unroll
+  for (int m = 0; m < M; m++) T[lane + 64 * m] = make_float2(0.0f, 0.0f);
+  wave_sync();
+  // coefficient (u, v) at row v, coefficient (-u, v) as conj at row n - v (the Hermitian partner of (-u, v) lies in
+  // column u); in column 0 every coefficient's partner (0, -v) too
+  for (int e = P.col_off[u] + lane; e < P.col_off[u + 1]; e += 64) {
+    const uint32_t q = static_cast<uint32_t>(P.col_ent[e]);
+    const float2 h = qpsk(bits, static_cast<int>(q & 0x7FFFF));
+    const int k = static_cast<int>((q >> 19) & 0x7FF);
+    T[k] = (q >> 30) ? make_float2(h.x, -h.y) : h;
+    if (u == 0 && k) T[N - k] = make_float2(h.x, -h.y);
+  }
+  wave_sync();
+  float2 v[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const float2 t = T[lane + 64 * m];
+    v[m] = make_float2(t.x, -t.y);
+  }
+  wave_sync();
+  F::run(v, T, P.tw, lane);
+  float2 *out = G + (static_cast<size_t>(f) * E + u) * N;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int y = lane + 64 * m;
+    const float2 z = T[F::pos(y)];
+    out[y] = make_float2(z.x, -z.y);
+  }
+}
+
+template <int N, int W>
+__global__ __launch_bounds__(64 * W) void k_mod_rows(PlanDev P, const float2 *G, float *pic, float L) {
+  using F = WaveFft<N>;
+  constexpr int M = F::M;
+  extern __shared__ float2 zs[];  // W waves x F::LDS
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E;
+  const int y = (blockIdx.x * W + w) * 2;
+  float2 *T = zs + w * F::LDS;
+#pragma unroll
+  for (int m = 0; m < M; m++) T[lane + 64 * m] = make_float2(0.0f, 0.0f);
+  wave_sync();
+  // W = Z_y + i Z_y+1: at u, a + i b; at n - u, conj a + i conj b
+  const float2 *col = G + static_cast<size_t>(f) * E * N + y;
+  for (int u = lane; u < E; u += 64) {
+    const float4 ab = *reinterpret_cast<const float4 *>(col + static_cast<size_t>(u) * N);
+    T[u] = make_float2(ab.x - ab.w, ab.y + ab.z);
+    if (u) T[N - u] = make_float2(ab.x + ab.w, ab.z - ab.y);
+  }
+  wave_sync();
+  float2 v[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const float2 t = T[lane + 64 * m];
+    v[m] = make_float2(t.x, -t.y);
+  }
+  wave_sync();
+  F::run(v, T, P.tw, lane);
+  const float inv2L = 0.5f / L;
+  float *row0 = pic + (static_cast<size_t>(f) * N + y) * N, *row1 = row0 + N;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int x = lane + 64 * m;
+    const float2 z = T[F::pos(x)];  // conj of the DFT: (p_y, p_y+1) = (z.x, -z.y)
+    row0[x] = 0.5f + fminf(fmaxf(z.x, -L), L) * inv2L;
+    row1[x] = 0.5f + fminf(fmaxf(-z.y, -L), L) * inv2L;
+  }
+}
+
+void launch_modulate(const FramePlan &p, const uint8_t *d_bits, float2 *d_G, float *d_pic, int frames, hipStream_t s) {
+  const float L = static_cast<float>(2.0 * std::sqrt(640.0 * p.subch));
+  with_size(p.n, [&](auto Nc) {
+    constexpr int NN = decltype(Nc)::value, W = 4;
+    hipLaunchKernelGGL((k_mod_cols<NN, W>), dim3((p.E + W - 1) / W, frames), dim3(64 * W),
+                       W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, d_bits, d_G);
+    hipLaunchKernelGGL((k_mod_rows<NN, W>), dim3(NN / (2 * W), frames), dim3(64 * W),
+                       W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, d_G, d_pic, L);
+  });
+  hip_check(hipGetLastError(), "modulator launches");
+}
+
 // Node centres every 16 modules from 3.5 (the middle of the line-and-track band) to side - 3.5 (layout.c:110-118,
 // layout.h OB_NODE_LINE, OB_THIN_PITCH, OB_MAX_NODES).
 int node_lattice(int span, float *pos) {

@@ -111,6 +111,9 @@ inline void launch_soft_warp(const FramePlan &p, FrameBufs &b, const WarpFrame *
 }
 // The warp alone: n x n picture samples a frame, row-major.
 void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, float *d_grid, hipStream_t s);
+// The sender's pictures: frame slots (0/1, subch * 640 a frame) -> n x n float pictures in [0, 1], row-major;
+// d_G a work area of E * n complex a frame.
+void launch_modulate(const FramePlan &p, const uint8_t *d_bits, float2 *d_G, float *d_pic, int frames, hipStream_t s);
 // The node lattice of a ring's layout (module coordinates; layout.c:110-118).
 int node_lattice(int span, float *pos);
 

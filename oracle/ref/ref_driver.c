@@ -123,6 +123,15 @@ int ref_encode(void *p, const uint8_t *blocks, int c, float *drive, uint8_t *slo
   return 0;
 }
 void ref_paint_grey(const void *p, const float *drive, uint8_t *grey) { lizard_paint_grey(p, drive, grey); }
+/* The encoder's picture, after ref_encode: the n x n clipped picture in [0, 1], row-major, that lizard_encode
+   resamples into its drive (its workspace's rs_pic, lizard_phy.c:1183-1200). */
+int ref_encoded_picture(const void *p, float *pic) {
+  const lizard_t *f = p;
+  const struct lizard_ws *w = f->ws;
+  if (!w || !w->rs_pic) return -1;
+  memcpy(pic, w->rs_pic, (size_t)f->n * f->n * sizeof(float));
+  return 0;
+}
 
 /* ---------------------------------------------------------------- the sampler (acquire.c) */
 /* The picture grid through a registration given as H (9 floats, h8 = 1) and the node lattice's displacements

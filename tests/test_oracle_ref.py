@@ -128,3 +128,31 @@ def test_finish_llr(version, sigma):
             assert np.array_equal(a[1][blk], b[1][blk])
     r.ref_format_free(fr)
     o.or_format_free(fo)
+
+
+@pytest.mark.parametrize("version,span", [(4, 128), (5, 128), (20, 256), (64, 512)])
+def test_modulator_matches_reference_picture(version, span):
+    """The FP64 modulator (oracle/picture.py) against the picture the reference's own encoder makes before it
+    resamples it into the drive (lizard_encode's rs_pic), both scalar and x86 builds: within float rounding."""
+    import lizard_amd as L
+    from pictures import picture as opic
+    for form in ("scalar", "x86"):
+        r = ref(form)
+        if r is None:
+            continue
+        subch = 16 * version
+        n = L.format_info(version)["n"]
+        f = r.ref_format_new(n, subch, span)
+        info = np.zeros(10, np.int32)
+        r.ref_format_info(f, ptr(info))
+        rng = np.random.default_rng(version)
+        blocks = rng.integers(0, 256, int(info[2]) * 476).astype(np.uint8)
+        u, v = L.format_coefs(version)
+        for cnt in (0, 3):
+            drive = np.zeros(int(info[8]) ** 2, np.float32)
+            assert r.ref_encode(f, ptr(blocks), cnt, ptr(drive), None) == 0
+            pic = np.zeros(n * n, np.float32)
+            assert r.ref_encoded_picture(f, ptr(pic)) == 0
+            want = opic.modulate(n, subch, L.frame_slots(version, blocks, cnt).astype(np.float64), u, v)
+            assert np.abs(pic.reshape(n, n) - want).max() < 1e-6
+        r.ref_format_free(f)
