@@ -20,7 +20,13 @@ cores. Two more workloads ride along as sub-objects:
                LDPC launched once a codeword; frames/s and payload MB/s; the reference decoder on the host cores.
   "fft_demap"  64 picture grids with n drawn from {384, 768, 1152, 1536} (each at its top format), the sender's
                picture plus AWGN at 20 dB SNR: erase, detrend, 2D FFT, pilot shift, soft demap as one HIP graph;
-               algorithmic HBM GB/s against the 8 TB/s peak.
+               algorithmic HBM GB/s against the 8 TB/s peak; the reference's finish path on the same grids.
+  "refine"     256 captured frames of LIZARD-320 whose homography corners are up to 1.5 px off: block-fail before and
+               after 10 Gauss-Newton steps over the homography (the hand-written adjoint of the warp, FP64 8x8 solve).
+  "small"      one n = 384 LIZARD-80 frame in a 1280x720 capture through liz_decode, against the reference's decoder
+               on one core.
+  "sender"     the sender's pictures for the FFT+demap mix on the GPU (lizx_modulate): HBM roofline of its two passes,
+               the reference's encoder on the host cores beside it.
 """
 import argparse
 import ctypes as C
@@ -373,6 +379,110 @@ def cpu_fft_demap(grids, seconds):
                       (state["frames"], len(strips), wall, cores)}
 
 
+def bench_sender(args, dec):
+    """The sender's pictures on the GPU (lizx_modulate) for the FFT+demap leg's mix of sizes: the bit side (CRC,
+    LDPC encoding, bit map, whitening), QPSK, the Hermitian spectrum's inverse 2D DFT and the clip, all on the GPU
+    after the blocks' upload; HBM roofline over the GPU time; the reference's own encoder (lizard_encode) on the host
+    cores."""
+    rng = np.random.default_rng(78)
+    table = {384: 5, 768: 20, 1152: 45, 1536: 64}
+    ns = rng.choice(sorted(table), size=args.fft_frames)
+    groups = {}
+    for n in ns:
+        groups.setdefault(table[int(n)], 0)
+        groups[table[int(n)]] += 1
+    work, total = [], 0
+    for v, k in sorted(groups.items()):
+        info = lizard_amd.format_info(v)
+        u, vv = lizard_amd.format_coefs(v)
+        E = int(max(np.abs(u).max(), vv.max())) + 1
+        n = info["n"]
+        blocks = rng.integers(0, 256, (k, info["blocks"] * 476)).astype(np.uint8)
+        out = lizard_amd.DeviceArray(k * n * n * 4)
+        work.append((v, blocks, out))
+        # blocks in, slots out and back in, the column pass's E n complex out and back in, the picture out
+        total += k * (info["blocks"] * 476 + 2 * info["subch"] * 640 + 2 * E * n * 8 + n * n * 4)
+    for v, blocks, out in work:
+        dec.modulate(v, blocks, out=out)
+    runs = max(1, args.pipe_runs)
+    t0 = time.perf_counter()
+    kms = 0.0
+    for _ in range(runs):
+        for v, blocks, out in work:
+            kms += dec.modulate(v, blocks, out=out, timed=True)[1]
+    wall = (time.perf_counter() - t0) / runs
+    kms /= runs
+    for _, _, out in work:
+        out.free()
+    gbs = total / (kms * 1e-3) / 1e9
+    cpu = None if args.no_cpu else cpu_sender(work, args.cpu_seconds)
+    return {"workload": "%d sender pictures, n drawn from {384, 768, 1152, 1536} (%s), top formats, random blocks" %
+                        (len(ns), ", ".join("%d x %d" % (int((ns == n).sum()), n) for n in sorted(table))),
+            "gpu_ms_per_batch": round(kms, 3), "gpu_frames_per_s": round(len(ns) / (kms * 1e-3), 1),
+            "frames_per_s_end_to_end": round(len(ns) / wall, 1),
+            "algorithmic_bytes_per_batch": int(total),
+            "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(gbs / HBM_PEAK_GBS, 4)},
+            "cpu_baseline": cpu}
+
+
+def cpu_sender(work, seconds):
+    """The reference's encoder (lizard_encode, x86-64 build) on the same blocks, one format instance per size and
+    thread: its bit side, inverse FFT and picture, and also the Lanczos-3 resample into its drive."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+    if not os.path.exists(ref_so):
+        return None
+    lib = C.CDLL(ref_so)
+    lib.ref_format_new.restype = C.c_void_p
+    lib.ref_format_new.argtypes = [C.c_int, C.c_int, C.c_int]
+    lib.ref_format_free.argtypes = [C.c_void_p]
+    lib.ref_format_info.argtypes = [C.c_void_p, C.c_void_p]
+    fn = lib.ref_encode
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    span = {5: 128, 20: 256, 45: 512, 64: 512}
+    items = [(v, blocks[i]) for v, blocks, _ in work for i in range(blocks.shape[0])]
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+    cores = max(1, min(cores, os.cpu_count() or 1, 16))
+    lock = threading.Lock()
+    state = {"next": 0, "frames": 0}
+    deadline = [0.0]
+
+    def worker():
+        forms = {}
+        while True:
+            with lock:
+                if time.perf_counter() > deadline[0]:
+                    break
+                v, bl = items[state["next"] % len(items)]
+                state["next"] += 1
+            if v not in forms:
+                info = lizard_amd.format_info(v)
+                f = lib.ref_format_new(info["n"], info["subch"], span[v])
+                fi = np.zeros(10, np.int32)
+                lib.ref_format_info(f, fi.ctypes.data)
+                forms[v] = (f, np.zeros(int(fi[8]) ** 2, np.float32))
+            f, drive = forms[v]
+            fn(f, bl.ctypes.data, 0, drive.ctypes.data, None)
+            with lock:
+                state["frames"] += 1
+        for f, _ in forms.values():
+            lib.ref_format_free(f)
+
+    t0 = time.perf_counter()
+    deadline[0] = t0 + seconds
+    th = [threading.Thread(target=worker) for _ in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    wall = time.perf_counter() - t0
+    return {"value": round(state["frames"] / wall, 2), "unit": "frames/s", "cores": cores, "kind": "reference",
+            "sample": "%d frames (the same %d cycled), %.1f s wall on %d threads; liblizard lizard_encode (x86-64 SIMD "
+                      "build): bit side, inverse FFT, clipped picture, and the Lanczos-3 resample into its drive" %
+                      (state["frames"], len(items), wall, cores)}
+
+
 def refine_frames(count, seed):
     """count captures of LIZARD-320 (n = 768) in the 128 ring through random cameras, each with a registration whose
     homography corners are up to 1.5 px off (the lens term known)."""
@@ -582,7 +692,7 @@ def main():
     ap.add_argument("--fft-frames", type=int, default=64)
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--refine-frames", type=int, default=256)
-    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small"), default=None,
+    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small", "sender"), default=None,
                     help="run one workload alone (for per-kernel profiles); the JSON line then carries only it")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -593,10 +703,10 @@ def main():
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
 
-    if args.only in ("pipeline", "fft_demap", "refine", "small"):
+    if args.only in ("pipeline", "fft_demap", "refine", "small", "sender"):
         out = (bench_pipeline(args, pipeline_frames(args.pipe_frames, 4242)) if args.only == "pipeline"
                else bench_fft_demap(args) if args.only == "fft_demap" else bench_refine(args) if args.only == "refine"
-               else bench_small(args))
+               else bench_sender(args, lizard_amd.Decoder(0)) if args.only == "sender" else bench_small(args))
         print(json.dumps({"only": args.only, args.only: out}))
         return
     if args.only == "ldpc":
@@ -693,9 +803,12 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(batches, args.cpu_seconds)
 
-    pipe = fft = refine = small = None
+    pipe = fft = refine = small = sender = None
     if not args.no_pipeline:
         small = bench_small(args)
+        sdec = lizard_amd.Decoder(0)
+        sender = bench_sender(args, sdec)
+        sdec.close()
         refine = bench_refine(args)
         caps = pipeline_frames(args.pipe_frames, 4242)
         pipe = bench_pipeline(args, caps)
@@ -735,6 +848,7 @@ def main():
         "fft_demap": fft,
         "refine": refine,
         "small": small,
+        "sender": sender,
     }
     print(json.dumps(line))
 

@@ -99,12 +99,19 @@ LIZ_API int lizx_frame_slots(int version, const uint8_t *blocks, int cnt, uint8_
 LIZ_API int lizx_format_blocks(int version, int *rate, int *slot0, int *nslots);
 /* The coefficient order (research/SPEC.md 6.3) of LIZARD-16V: u and v of each of its subch * 320 coefficients. */
 LIZ_API int lizx_format_coefs(int version, int16_t *u, int16_t *v);
+/* lizx_frame_slots on the GPU for frames x (blocks x 476 bytes) in device memory -> frames x subch x 640 slot bytes
+   (0/1) in device memory; bit for bit lizx_frame_slots'. */
+LIZ_API int lizx_frame_slots_device(liz_decoder *d, int version, int frames, const uint8_t *d_blocks, int cnt,
+                                    uint8_t *d_slots);
 /* The sender's pictures on the GPU: frames x (blocks x 476 bytes) (host, frame after frame) with pilot count cnt ->
    frames n x n float pictures in [0, 1], row-major, into device memory d_pictures (frame f at f n n floats). The
    bit side is lizx_frame_slots'; the picture is research/SPEC.md 6.5-6.7's (QPSK at each coefficient, the
    Hermitian spectrum's unscaled inverse DFT, clipped at 2 sqrt(640 subch)), the one lizard_encode resamples into its
-   drive (liblizard/src/lizard_phy.c:1087-1200). Returns when the pictures are written. */
-LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t *blocks, int cnt, float *d_pictures);
+   drive (liblizard/src/lizard_phy.c:1087-1200); the bit side (CRC, LDPC, bit map, whitening) on the GPU too.
+   Returns when the pictures are written; *kernel_ms (optional) receives the GPU time after the blocks' upload (HIP
+   events). */
+LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t *blocks, int cnt, float *d_pictures,
+                          float *kernel_ms);
 
 /* lizard_finish_ext with ext_llr (liblizard/src/lizard_phy.c:1566, :1753-1871): the frame's soft values made
    elsewhere: llr = 16 * version * 640 int8 in frame slot order (whitened), est = one information estimate per

@@ -83,7 +83,8 @@ def lib():
     sig("lizx_finish_grid", _I, _P, _I, _I, _P, _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_node_lattice", _I, _I, _P)
     sig("lizx_frame_slots", _I, _I, _P, _I, _P)
-    sig("lizx_modulate", _I, _P, _I, _I, _P, _I, _P)
+    sig("lizx_modulate", _I, _P, _I, _I, _P, _I, _P, _P)
+    sig("lizx_frame_slots_device", _I, _P, _I, _I, _P, _I, _P)
     sig("lizx_format_coefs", _I, _I, _P, _P)
     sig("lizx_format_blocks", _I, _I, _P, _P, _P)
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
@@ -359,15 +360,21 @@ class Decoder:
         good = check(lib().lizx_finish_llr(self.h, version, _p(llr), _p(est), _p(blocks), _p(ok), _p(its), _p(be)))
         return {"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be}
 
-    def modulate(self, version, blocks, cnt=0, out=None):
+    def modulate(self, version, blocks, cnt=0, out=None, timed=False):
         """The sender's pictures on the GPU (lizx_modulate): blocks (frames x blocks x 476 bytes, or one frame's
-        blocks x 476) -> a DeviceArray of frames x n x n float32 pictures (or into `out`, a DeviceArray that large)."""
+        blocks x 476) -> a DeviceArray of frames x n x n float32 pictures (or into `out`, a DeviceArray that large);
+        with timed, (pictures, the GPU passes' ms)."""
         info = format_info(version)
         blocks = np.ascontiguousarray(blocks, np.uint8).reshape(-1, info["blocks"] * LIZ_BLOCK)
         frames, n = blocks.shape[0], info["n"]
         d = out if out is not None else DeviceArray(frames * n * n * 4)
-        check(lib().lizx_modulate(self.h, version, frames, _p(blocks), cnt, d.ptr))
-        return d
+        ms = C.c_float(0.0)
+        check(lib().lizx_modulate(self.h, version, frames, _p(blocks), cnt, d.ptr, C.byref(ms)))
+        return (d, ms.value) if timed else d
+
+    def frame_slots_device(self, version, d_blocks, frames, cnt, d_slots):
+        """lizx_frame_slots_device: device blocks (frames x blocks x 476) -> device slots (frames x subch x 640)."""
+        check(lib().lizx_frame_slots_device(self.h, version, frames, d_blocks.ptr, cnt, d_slots.ptr))
 
     def warp(self, version, span, img, H, nodes):
         img = np.ascontiguousarray(img, np.uint8)

@@ -121,8 +121,7 @@ struct liz_decoder {
   lizx_registration held_reg{};
   std::vector<float> held_nodes;
   // the sender's pictures
-  DevBuf mod_bits, mod_G;
-  std::vector<uint8_t> mod_slots;
+  DevBuf mod_blocks, mod_bits, mod_G;
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
       if (p->version == version && p->span == span) return *p;
@@ -726,7 +725,32 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
   return good;
 }
 
-LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t *blocks, int cnt, float *d_pictures) {
+// The device bit side writes the padding after the last block: the format's blocks must tile its slots from 0.
+static void need_tiled(const Format *f) {
+  for (int b = 0; b < f->blocks; b++)
+    if (f->blk_slot[b] != (b ? f->blk_slot[b - 1] + f->blk_nslots[b - 1] : 0))
+      throw Error(LIZ_E_INTERNAL, "the format's blocks do not tile its slots");
+}
+
+LIZ_API int lizx_frame_slots_device(liz_decoder *d, int version, int frames, const uint8_t *d_blocks, int cnt,
+                                    uint8_t *d_slots) {
+  return guard([&] {
+    need(d && d_blocks && d_slots && frames > 0 && frames <= 4096 && cnt >= 0 && cnt <= 3,
+         "a decoder, 1 to 4096 frames of blocks in device memory, a pilot count 0 to 3 and device memory for the slots");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    need_tiled(f);
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, f->n <= 384 ? 128 : f->n <= 768 ? 256 : 512);
+    e.frame_slots(p.dev.blk, f->blocks, static_cast<int>(f->white.size()), d_blocks, cnt, d_slots, frames);
+    e.sync();
+    return 0;
+  });
+}
+
+LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t *blocks, int cnt, float *d_pictures,
+                          float *kernel_ms) {
   return guard([&] {
     need(d && blocks && d_pictures && frames > 0 && frames <= 4096 && cnt >= 0 && cnt <= 3,
          "a decoder, 1 to 4096 frames of blocks, a pilot count 0 to 3 and device memory for the pictures");
@@ -734,17 +758,21 @@ LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t
     if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
     LdpcEngine &e = *d->eng;
     hip_check(hipSetDevice(e.device), "hipSetDevice");
+    need_tiled(f);
     // any ring's plan of the format serves: the picture depends on n and the coefficient order only
     const FramePlan &p = d->plan(version, f->n <= 384 ? 128 : f->n <= 768 ? 256 : 512);
     const size_t S2 = f->white.size(), fb = static_cast<size_t>(f->blocks) * BLOCK_BYTES;
-    d->mod_slots.resize(S2 * frames);
-    for (int fr = 0; fr < frames; fr++) frame_slots_of(f, blocks + fb * fr, cnt, d->mod_slots.data() + S2 * fr);
-    d->mod_bits.reserve(d->mod_slots.size());
+    d->mod_blocks.reserve(fb * frames);
+    d->mod_bits.reserve(S2 * frames);
     d->mod_G.reserve(static_cast<size_t>(frames) * p.E * p.n * sizeof(float2));
-    hip_check(hipMemcpyAsync(d->mod_bits.p, d->mod_slots.data(), d->mod_slots.size(), hipMemcpyHostToDevice, e.stream),
-              "frame slots");
+    hip_check(hipMemcpyAsync(d->mod_blocks.p, blocks, fb * frames, hipMemcpyHostToDevice, e.stream), "blocks");
+    hip_check(hipEventRecord(e.ev0, e.stream), "hipEventRecord");
+    e.frame_slots(p.dev.blk, f->blocks, static_cast<int>(S2), d->mod_blocks.as<uint8_t>(), cnt,
+                  d->mod_bits.as<uint8_t>(), frames);
     launch_modulate(p, d->mod_bits.as<uint8_t>(), d->mod_G.as<float2>(), d_pictures, frames, e.stream);
+    hip_check(hipEventRecord(e.ev1, e.stream), "hipEventRecord");
     e.sync();
+    if (kernel_ms) hip_check(hipEventElapsedTime(kernel_ms, e.ev0, e.ev1), "hipEventElapsedTime");
     return 0;
   });
 }

@@ -280,6 +280,99 @@ __global__ __launch_bounds__(LDPC_THREADS) void k_ldpc(LdpcParams p) {
   }
 }
 
+// ---- the sender's bit side (lizard_phy.c:1087-1150): one workgroup a block
+// The block's 3808 bits MSB first and its CRC-32C's 32 (the XOR of one table word per set bit), zeros to k (the 7/8
+// code's known zeros); parity bit c (check c = MB i + r, lane i) = the XOR of every syndrome up to c (the staircase,
+// ldpc.c ldpc_encode): per lane the running XOR over its rows, then an exclusive prefix XOR across the lanes; the
+// block's slots through the bit map, the pilot bit past nt, whitened.
+template <int CI>
+__device__ __forceinline__ void encode_slots(const SlotTables &t, const uint8_t *src, uint8_t *slots, int s0, int nb,
+                                             int pilot, uint8_t *cw, uint32_t *red, int lane) {
+  using S = tm::Shape<CI>;
+  constexpr int K = S::K, MB = S::MB, NT = S::N - S::cb.np * Z;
+  static_assert(MB <= 32, "a lane's running XOR in one word");
+  uint32_t x = 0;
+  for (int j = lane; j < BLOCK_BYTES; j += LDPC_THREADS) {
+    const int b = src[j];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int bit = (b >> (7 - q)) & 1;
+      cw[8 * j + q] = static_cast<uint8_t>(bit);
+      x ^= bit ? t.crc_tab[8 * j + q] : 0u;
+    }
+  }
+  for (int v = INFO_BITS + lane; v < K; v += LDPC_THREADS) cw[v] = 0;
+  const uint32_t crc = t.crc0 ^ group_xor(x, red, lane);
+  if (lane < 32) cw[8 * BLOCK_BYTES + lane] = static_cast<uint8_t>((crc >> (31 - lane)) & 1);
+  __syncthreads();
+  int run = 0;
+  uint32_t loc = 0;
+  tm::static_for<0, MB>([&](auto r) {
+    constexpr int R = r, D0 = S::d0(R), ND = S::nd(R);
+    int syn = 0;
+    tm::static_for<0, ND>([&](auto e) { syn ^= cw[S::col(D0 + e) * Z + ((lane - S::shift(D0 + e)) & (Z - 1))]; });
+    run ^= syn;
+    loc |= static_cast<uint32_t>(run) << R;
+  });
+  int incl = run;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(incl, o, 64);
+    if ((lane & 63) >= o) incl ^= y;
+  }
+  if ((lane & 63) == 63) red[2 + (lane >> 6)] = static_cast<uint32_t>(incl);
+  __syncthreads();
+  const int pre = (incl ^ run) ^ (lane >= 64 ? static_cast<int>(red[2]) : 0);
+#pragma unroll
+  for (int r = 0; r < MB; r++) cw[K + MB * lane + r] = static_cast<uint8_t>(pre ^ ((loc >> r) & 1));
+  __syncthreads();
+  const int16_t *s2p = t.s2p[CI];
+  for (int u = lane; u < nb; u += LDPC_THREADS) {
+    int bit = pilot;
+    if (u < NT) {
+      const int pi = s2p[u];
+      bit = cw[pi < K ? pi : K + ((pi - K) % Z) * MB + (pi - K) / Z];
+    }
+    slots[s0 + u] = static_cast<uint8_t>(bit ^ t.white[s0 + u]);
+  }
+}
+
+// Grid (blocks + 1, frames): workgroup b < blocks encodes block b of frame f; workgroup blocks writes the padding
+// past the last block (the whitening alone). blk: first slot, nt, slots, code index per block.
+__global__ __launch_bounds__(LDPC_THREADS) void k_frame_slots(SlotTables t, const int4 *blk, int blocks, int total,
+                                                              const uint8_t *blocks_all, int cnt, uint8_t *slots_all) {
+  __shared__ uint8_t cw[MAX_N];
+  __shared__ uint32_t red[4];
+  const int b = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+  uint8_t *slots = slots_all + static_cast<size_t>(f) * total;
+  if (b == blocks) {
+    const int4 last = blk[blocks - 1];
+    for (int u = last.x + last.z + lane; u < total; u += LDPC_THREADS) slots[u] = t.white[u];
+    return;
+  }
+  const int4 bk = blk[b];
+  const uint8_t *src = blocks_all + (static_cast<size_t>(f) * blocks + b) * BLOCK_BYTES;
+  const int pilot = (cnt >> (b & 1)) & 1;
+  switch (bk.w) {
+    case 0: encode_slots<0>(t, src, slots, bk.x, bk.z, pilot, cw, red, lane); break;
+    case 1: encode_slots<1>(t, src, slots, bk.x, bk.z, pilot, cw, red, lane); break;
+    case 2: encode_slots<2>(t, src, slots, bk.x, bk.z, pilot, cw, red, lane); break;
+    default: encode_slots<3>(t, src, slots, bk.x, bk.z, pilot, cw, red, lane); break;
+  }
+}
+
+void LdpcEngine::frame_slots(const int4 *d_blk, int blocks, int total, const uint8_t *d_blocks, int cnt,
+                             uint8_t *d_slots, int frames) {
+  ensure_frame_tables(total);
+  SlotTables t{};
+  for (int ci = 0; ci < NCODES; ci++) t.s2p[ci] = s2p[ci].as<int16_t>();
+  t.white = white.as<uint8_t>();
+  t.crc_tab = crc_tab.as<uint32_t>();
+  t.crc0 = crc0;
+  hipLaunchKernelGGL(k_frame_slots, dim3(blocks + 1, frames), dim3(LDPC_THREADS), 0, stream, t, d_blk, blocks, total,
+                     d_blocks, cnt, d_slots);
+  hip_check(hipGetLastError(), "frame slots launch");
+}
+
 LdpcEngine::LdpcEngine(int dev) : device(dev) {
   hip_check(hipSetDevice(device), "hipSetDevice");
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");

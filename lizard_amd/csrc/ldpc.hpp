@@ -46,6 +46,14 @@ struct LdpcParams {
 };
 
 constexpr int LDPC_THREADS = 128;
+
+// The sender's bit side on the device: per-code bit maps, the whitening, the CRC table.
+struct SlotTables {
+  const int16_t *s2p[NCODES];
+  const uint8_t *white;
+  const uint32_t *crc_tab;
+  uint32_t crc0;
+};
 constexpr size_t MSG_TRACE_STRIDE = MAX_SLOTS_TOTAL * Z;  // 26880 bytes a codeword
 
 // Per-device state: the code tables in constant memory, the CRC table, frame-mode bit maps, the queue counter.
@@ -67,6 +75,10 @@ struct LdpcEngine {
   void make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count);
   void sync();
   void ensure_frame_tables(int max_slots);
+  // The frame slots (0/1 bytes, total a frame) of frames x blocks x 476 bytes on the device, enqueued on the stream;
+  // d_blk: per block first slot, nt, slots, code index (contiguous from slot 0).
+  void frame_slots(const int4 *d_blk, int blocks, int total, const uint8_t *d_blocks, int cnt, uint8_t *d_slots,
+                   int frames);
 };
 
 float copy_probe(void *dst, const void *src, size_t bytes);

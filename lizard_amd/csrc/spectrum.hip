@@ -899,7 +899,7 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
   d.npilots = static_cast<int>(p->pilot_host.size());
   std::vector<int4> blk(f->blocks);
   for (int b = 0; b < f->blocks; b++)
-    blk[b] = make_int4(f->blk_slot[b], code(f->blk_code[b]).nt, f->blk_nslots[b], 0);
+    blk[b] = make_int4(f->blk_slot[b], code(f->blk_code[b]).nt, f->blk_nslots[b], f->blk_code[b]);
   p->blk.reserve(blk.size() * sizeof(int4));
   hip_check(hipMemcpy(p->blk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice), "blocks");
   d.blk = p->blk.as<int4>();

@@ -51,7 +51,7 @@ struct PlanDev {
   const float *mcoord;                     // n: module coordinate of sample index t
   const int32_t *mcell;                    // n: node cell of sample index t (the same on both axes)
   int cell0[MAX_NODES];                    // first sample index of node cell i (cell0[nx - 1] = n)
-  const int4 *blk;                         // per block: first slot, code's sent bits nt, slots nb, unused
+  const int4 *blk;                         // per block: first slot, code's sent bits nt, slots nb, code index
   const int32_t *col_off, *col_ent;        // per column u < E: its coefficients (ci | source index << 19 | conj << 30)
 };
 

@@ -73,3 +73,23 @@ def test_modulate_argument_errors(decoder_gpu):
     with pytest.raises(L.LizError):
         decoder_gpu.modulate(5, np.zeros(B * 476, np.uint8), cnt=4, out=d)
     d.free()
+
+
+@pytest.mark.parametrize("version", [4, 5, 13, 20, 45, 64])
+def test_device_frame_slots_match_host(decoder_gpu, version):
+    """The bit side on the GPU (CRC-32C, QC-IRA encoding by a prefix XOR across lanes, bit map, pilots, whitening)
+    equals the host encoder's frame slots, themselves the oracle's and the reference encoder's (test_sim,
+    test_oracle_ref)."""
+    rng = np.random.default_rng(200 + version)
+    info = L.format_info(version)
+    frames, S2 = 3, info["subch"] * 640
+    blocks = blocks_for(version, frames, rng)
+    db = L.DeviceArray.of(blocks)
+    ds = L.DeviceArray(frames * S2)
+    for cnt in range(4):
+        decoder_gpu.frame_slots_device(version, db, frames, cnt, ds)
+        got = ds.get(np.uint8, frames * S2).reshape(frames, S2)
+        for f in range(frames):
+            assert np.array_equal(got[f], L.frame_slots(version, blocks[f], cnt)), (version, cnt, f)
+    db.free()
+    ds.free()
