@@ -99,6 +99,22 @@ LIZ_API int lizx_frame_slots(int version, const uint8_t *blocks, int cnt, uint8_
 LIZ_API int lizx_format_blocks(int version, int *rate, int *slot0, int *nslots);
 /* The coefficient order (research/SPEC.md 6.3) of LIZARD-16V: u and v of each of its subch * 320 coefficients. */
 LIZ_API int lizx_format_coefs(int version, int16_t *u, int16_t *v);
+/* A camera for synthetic captures: w x h pixels; H maps module coordinates (0 .. span + 30 on a side) to capture
+   pixels; a radial lens k1 |q|^2 q D about the frame's centre, q the offset over the farthest corner's distance D
+   (the reference's lens form, liblizard/src/acquire.c:1470-1512); Gaussian blur sigma (pixels, edges repeated);
+   AWGN sigma (grey levels) from a counter-based generator seeded by seed. */
+typedef struct {
+  int w, h;
+  double H[9];
+  double k1;
+  float blur, noise;
+  uint64_t seed;
+} lizx_camera;
+/* The channel on the GPU: picture (n x n floats in [0, 1], device, LIZARD-16V shown in the ring of span modules;
+   grey 128 outside it) -> capture (h x w uint8, device). No counterpart in the reference's decoder (a measurement
+   input maker; the model of lizard_amd/sim.py's capture). */
+LIZ_API int lizx_capture(liz_decoder *d, int version, int span, const float *d_picture, const lizx_camera *cam,
+                         uint8_t *d_img);
 /* lizx_frame_slots on the GPU for frames x (blocks x 476 bytes) in device memory -> frames x subch x 640 slot bytes
    (0/1) in device memory; bit for bit lizx_frame_slots'. */
 LIZ_API int lizx_frame_slots_device(liz_decoder *d, int version, int frames, const uint8_t *d_blocks, int cnt,

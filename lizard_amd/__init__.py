@@ -85,6 +85,7 @@ def lib():
     sig("lizx_frame_slots", _I, _I, _P, _I, _P)
     sig("lizx_modulate", _I, _P, _I, _I, _P, _I, _P, _P)
     sig("lizx_frame_slots_device", _I, _P, _I, _I, _P, _I, _P)
+    sig("lizx_capture", _I, _P, _I, _I, _P, _P, _P)
     sig("lizx_format_coefs", _I, _I, _P, _P)
     sig("lizx_format_blocks", _I, _I, _P, _P, _P)
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
@@ -229,6 +230,12 @@ def copy_probe(dst, src, nbytes):
     return ms.value
 
 
+class Camera(C.Structure):
+    """lizx_camera: w x h pixels, H (module -> pixel, 9 doubles), radial k1, blur sigma, AWGN sigma, noise seed."""
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("H", C.c_double * 9), ("k1", C.c_double), ("blur", C.c_float),
+                ("noise", C.c_float), ("seed", C.c_uint64)]
+
+
 class DeviceArray:
     """A device buffer owned through the library's allocator (no HIP runtime needed in Python)."""
 
@@ -371,6 +378,14 @@ class Decoder:
         ms = C.c_float(0.0)
         check(lib().lizx_modulate(self.h, version, frames, _p(blocks), cnt, d.ptr, C.byref(ms)))
         return (d, ms.value) if timed else d
+
+    def capture(self, version, span, d_picture, w, h, H, k1=0.0, blur=0.0, noise=0.0, seed=0, out=None):
+        """lizx_capture: a device picture through a camera -> a DeviceArray of h x w uint8 (or into `out`)."""
+        cam = Camera(int(w), int(h), (C.c_double * 9)(*np.asarray(H, np.float64).ravel()), float(k1), float(blur),
+                     float(noise), int(seed))
+        d = out if out is not None else DeviceArray(int(w) * int(h))
+        check(lib().lizx_capture(self.h, version, span, d_picture.ptr, C.byref(cam), d.ptr))
+        return d
 
     def frame_slots_device(self, version, d_blocks, frames, cnt, d_slots):
         """lizx_frame_slots_device: device blocks (frames x blocks x 476) -> device slots (frames x subch x 640)."""

@@ -10,6 +10,7 @@
 #include "common.hpp"
 #include "format.hpp"
 #include "ldpc.hpp"
+#include "channel.hpp"
 #include "register.hpp"
 #include "spectrum.hpp"
 
@@ -120,8 +121,8 @@ struct liz_decoder {
   int held_version = 0, held_span = 0;
   lizx_registration held_reg{};
   std::vector<float> held_nodes;
-  // the sender's pictures
-  DevBuf mod_blocks, mod_bits, mod_G;
+  // the sender's pictures and the camera channel
+  DevBuf mod_blocks, mod_bits, mod_G, chan_a, chan_b;
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
       if (p->version == version && p->span == span) return *p;
@@ -773,6 +774,60 @@ LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t
     hip_check(hipEventRecord(e.ev1, e.stream), "hipEventRecord");
     e.sync();
     if (kernel_ms) hip_check(hipEventElapsedTime(kernel_ms, e.ev0, e.ev1), "hipEventElapsedTime");
+    return 0;
+  });
+}
+
+LIZ_API int lizx_capture(liz_decoder *d, int version, int span, const float *d_picture, const lizx_camera *cam,
+                         uint8_t *d_img) {
+  return guard([&] {
+    need(d && d_picture && cam && d_img && cam->w > 0 && cam->h > 0 && cam->w <= 16384 && cam->h <= 16384,
+         "a decoder, a device picture, a camera of 1 to 16384 pixels a side and device memory for the capture");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    need(cam->blur >= 0 && cam->blur <= CHAN_MAX_R / 3.0f && cam->noise >= 0, "blur 0 to 5.33 px, noise >= 0");
+    const Format *f = format_for(version);
+    if (!f) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    ChannelParams c{};
+    c.W = cam->w;
+    c.Hh = cam->h;
+    c.n = f->n;
+    c.span = span;
+    const double *H = cam->H;
+    const double det = H[0] * (H[4] * H[8] - H[5] * H[7]) - H[1] * (H[3] * H[8] - H[5] * H[6]) +
+                       H[2] * (H[3] * H[7] - H[4] * H[6]);
+    need(std::fabs(det) > 0, "an invertible homography");
+    c.Hi[0] = (H[4] * H[8] - H[5] * H[7]) / det;
+    c.Hi[1] = (H[2] * H[7] - H[1] * H[8]) / det;
+    c.Hi[2] = (H[1] * H[5] - H[2] * H[4]) / det;
+    c.Hi[3] = (H[5] * H[6] - H[3] * H[8]) / det;
+    c.Hi[4] = (H[0] * H[8] - H[2] * H[6]) / det;
+    c.Hi[5] = (H[2] * H[3] - H[0] * H[5]) / det;
+    c.Hi[6] = (H[3] * H[7] - H[4] * H[6]) / det;
+    c.Hi[7] = (H[1] * H[6] - H[0] * H[7]) / det;
+    c.Hi[8] = (H[0] * H[4] - H[1] * H[3]) / det;
+    c.k1 = cam->k1;
+    c.ax = cam->w / 2.0;
+    c.ay = cam->h / 2.0;
+    c.D = std::hypot(c.ax, c.ay);
+    c.r = cam->blur > 0 ? std::max(1, static_cast<int>(std::ceil(3.0 * cam->blur))) : 0;
+    if (c.r > 0) {
+      float sum = 0.0f;
+      for (int k = -c.r; k <= c.r; k++) {
+        const float t = static_cast<float>(k) / cam->blur;
+        c.taps[k + c.r] = std::exp(-0.5f * t * t);
+        sum += c.taps[k + c.r];
+      }
+      for (int k = 0; k <= 2 * c.r; k++) c.taps[k] /= sum;
+    }
+    c.noise = cam->noise;
+    c.seed = cam->seed;
+    const size_t px = static_cast<size_t>(c.W) * c.Hh;
+    d->chan_a.reserve(px * sizeof(float));
+    d->chan_b.reserve(px * sizeof(float));
+    launch_channel(c, d_picture, d->chan_a.as<float>(), d->chan_b.as<float>(), d_img, e.stream);
+    e.sync();
     return 0;
   });
 }

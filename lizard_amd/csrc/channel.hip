@@ -1,0 +1,106 @@
+// The camera channel on the GPU, for synthetic captures: a picture rendered into a W x H uint8 frame through a radial
+// lens and a homography, blurred and noised (the model of lizard_amd/sim.py's capture; the lens form of
+// liblizard/src/acquire.c:1470-1512). Not part of the receive path: it makes that path's measurement inputs on the
+// device.
+//   k_chan_render  per capture pixel: the lens undone by fixed-point iteration, the inverse homography to module
+//                  coordinates, the picture read bilinearly (grey 128 outside it), times 255
+//   k_chan_blur_x  the Gaussian along x (edge pixels repeated)
+//   k_chan_blur_y  the Gaussian along y, AWGN from a counter-based generator, rounded half-even and clamped to uint8
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "channel.hpp"
+#include "common.hpp"
+
+namespace lz {
+namespace {
+
+__global__ __launch_bounds__(256) void k_chan_render(ChannelParams c, const float *pic, float *out) {
+  const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+  if (x >= c.W) return;
+  const double X = x + 0.5, Y = y + 0.5;
+  double px = X, py = Y;
+  for (int it = 0; it < 6; it++) {
+    const double qx = (px - c.ax) / c.D, qy = (py - c.ay) / c.D, r2 = qx * qx + qy * qy;
+    px = X - c.k1 * r2 * qx * c.D;
+    py = Y - c.k1 * r2 * qy * c.D;
+  }
+  const double w = c.Hi[6] * px + c.Hi[7] * py + c.Hi[8];
+  const double mx = (c.Hi[0] * px + c.Hi[1] * py + c.Hi[2]) / w, my = (c.Hi[3] * px + c.Hi[4] * py + c.Hi[5]) / w;
+  const int n = c.n;
+  double tx = (mx - 15.0) * (static_cast<double>(n) / c.span) - 0.5, ty = (my - 15.0) * (static_cast<double>(n) / c.span) - 0.5;
+  float v = 128.0f / 255.0f;  // outside the picture: grey 128
+  if (tx > -0.5 && tx < n - 0.5 && ty > -0.5 && ty < n - 0.5) {
+    const double lim = n - 1.000001;
+    tx = tx < 0 ? 0 : tx > lim ? lim : tx;
+    ty = ty < 0 ? 0 : ty > lim ? lim : ty;
+    const int x0 = static_cast<int>(tx), y0 = static_cast<int>(ty);
+    const int x1 = x0 + 1 < n ? x0 + 1 : n - 1, y1 = y0 + 1 < n ? y0 + 1 : n - 1;
+    const float fx = static_cast<float>(tx - x0), fy = static_cast<float>(ty - y0);
+    const float top = pic[y0 * n + x0] * (1 - fx) + pic[y0 * n + x1] * fx;
+    const float bot = pic[y1 * n + x0] * (1 - fx) + pic[y1 * n + x1] * fx;
+    v = top * (1 - fy) + bot * fy;
+  }
+  out[static_cast<size_t>(y) * c.W + x] = v * 255.0f;
+}
+
+__global__ __launch_bounds__(256) void k_chan_blur_x(ChannelParams c, const float *in, float *out) {
+  const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+  if (x >= c.W) return;
+  const float *row = in + static_cast<size_t>(y) * c.W;
+  float s = 0.0f;
+  for (int k = -c.r; k <= c.r; k++) {
+    const int xx = x + k < 0 ? 0 : x + k >= c.W ? c.W - 1 : x + k;
+    s += c.taps[k + c.r] * row[xx];
+  }
+  out[static_cast<size_t>(y) * c.W + x] = s;
+}
+
+// Two standard normals from the frame's seed and a pixel's index (splitmix64 twice, Box-Muller).
+__device__ __forceinline__ float normal_at(uint64_t seed, uint64_t i) {
+  auto mix = [](uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const uint64_t a = mix(seed + 0x9E3779B97F4A7C15ull * (2 * i + 1)), b = mix(seed + 0x9E3779B97F4A7C15ull * (2 * i + 2));
+  const double u1 = (static_cast<double>(a >> 11) + 1.0) * (1.0 / 9007199254740992.0);  // (0, 1]
+  const double u2 = static_cast<double>(b >> 11) * (1.0 / 9007199254740992.0);
+  return static_cast<float>(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+__global__ __launch_bounds__(256) void k_chan_blur_y(ChannelParams c, const float *in, uint8_t *img) {
+  const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
+  if (x >= c.W) return;
+  float s;
+  if (c.r > 0) {
+    s = 0.0f;
+    for (int k = -c.r; k <= c.r; k++) {
+      const int yy = y + k < 0 ? 0 : y + k >= c.Hh ? c.Hh - 1 : y + k;
+      s += c.taps[k + c.r] * in[static_cast<size_t>(yy) * c.W + x];
+    }
+  } else {
+    s = in[static_cast<size_t>(y) * c.W + x];
+  }
+  const uint64_t i = static_cast<uint64_t>(y) * c.W + x;
+  if (c.noise > 0) s += c.noise * normal_at(c.seed, i);
+  s = rintf(s);
+  img[i] = static_cast<uint8_t>(s < 0 ? 0 : s > 255 ? 255 : s);
+}
+
+}  // namespace
+
+void launch_channel(const ChannelParams &c, const float *d_pic, float *d_a, float *d_b, uint8_t *d_img,
+                    hipStream_t s) {
+  const dim3 g((c.W + 255) / 256, c.Hh);
+  hipLaunchKernelGGL(k_chan_render, g, dim3(256), 0, s, c, d_pic, d_a);
+  if (c.r > 0) {
+    hipLaunchKernelGGL(k_chan_blur_x, g, dim3(256), 0, s, c, d_a, d_b);
+    hipLaunchKernelGGL(k_chan_blur_y, g, dim3(256), 0, s, c, d_b, d_img);
+  } else {
+    hipLaunchKernelGGL(k_chan_blur_y, g, dim3(256), 0, s, c, d_a, d_img);
+  }
+  hip_check(hipGetLastError(), "channel launches");
+}
+
+}  // namespace lz

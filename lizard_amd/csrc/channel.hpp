@@ -1,0 +1,24 @@
+// The camera channel on the GPU (synthetic captures for measurement inputs; not part of the receive path).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lz {
+
+constexpr int CHAN_MAX_R = 16;  // the Gaussian's half-width, at most
+
+struct ChannelParams {
+  int W, Hh, n, span;
+  double Hi[9];     // capture pixel -> module coordinates (the homography's inverse)
+  double k1, ax, ay, D;  // radial lens k1 |q|^2 q D about (ax, ay), q = (p - a) / D
+  int r;            // Gaussian half-width (0: no blur)
+  float taps[2 * CHAN_MAX_R + 1];
+  float noise;      // AWGN sigma, grey levels
+  uint64_t seed;
+};
+
+// Picture (n x n floats in [0, 1], device) -> capture (W x Hh uint8, device); d_a and d_b W x Hh float work areas.
+void launch_channel(const ChannelParams &c, const float *d_pic, float *d_a, float *d_b, uint8_t *d_img,
+                    hipStream_t s);
+
+}  // namespace lz

@@ -8,7 +8,8 @@ sender   blocks -> frame slots through the library (lizx_frame_slots: CRC-32C, L
          axis at each coefficient's (u, v) (lizx_format_coefs), the Hermitian spectrum, the unscaled inverse DFT,
          clipped at 2 sigma (sigma = sqrt(640 subch)) into [0, 1].
 paint    one picture sample a paint pixel (LIZARD's rings hold 3 samples and 3 pixels a module), read bilinearly; the
-         ring outside the picture is mid-grey (the finder's marks are not part of this path).
+         ring outside the picture is grey 128 (the finder's marks are not part of this path; a whole grey level, so
+         blur and rounding leave it in place).
 channel  module coordinates -> capture pixels by a homography (rotation, scale, perspective) and a radial term
          k1 |q|^2 q about the image centre, q the offset over the farthest corner's distance (the lens form of
          liblizard/src/acquire.c:1470-1512), then a Gaussian blur and AWGN, rounded to uint8.
@@ -167,7 +168,7 @@ def capture(pic, span, cam, seed=0):
     x1, y1 = (x0 + 1).clamp(max=n - 1), (y0 + 1).clamp(max=n - 1)
     top = pic[y0, x0] * (1 - fx) + pic[y0, x1] * fx
     bot = pic[y1, x0] * (1 - fx) + pic[y1, x1] * fx
-    img = torch.where(inside, top * (1 - fy) + bot * fy, torch.full_like(top, 0.5)) * 255.0
+    img = torch.where(inside, top * (1 - fy) + bot * fy, torch.full_like(top, 128.0 / 255.0)) * 255.0
     if cam.blur > 0:
         r = max(1, int(math.ceil(3 * cam.blur)))
         k = torch.exp(-0.5 * (torch.arange(-r, r + 1, device=dev, dtype=torch.float32) / cam.blur) ** 2)
@@ -180,6 +181,17 @@ def capture(pic, span, cam, seed=0):
         g.manual_seed(int(seed))
         img = img + torch.randn(img.shape, generator=g, device=dev) * cam.noise
     img = torch.clamp(torch.round(img), 0, 255).to(torch.uint8)
+    Hn = (Hm / Hm[2, 2]).astype(np.float32).ravel()
+    return img, Hn, nodes_for(cam, span, Hm)
+
+
+def capture_gpu(dec, version, span, d_picture, cam, seed=0):
+    """capture() on the GPU (lizx_capture) from a device picture (lizard_amd.DeviceArray, n x n float32): returns
+    (img as a DeviceArray of Hh x W uint8, H as 9 float32, nodes float32 [ny, nx, 2]). The same model; the noise
+    comes from the library's counter-based generator, not torch's."""
+    Hm = cam.H(span)
+    img = dec.capture(version, span, d_picture, cam.W, cam.Hh, Hm, k1=cam.k1, blur=cam.blur, noise=cam.noise,
+                      seed=seed)
     Hn = (Hm / Hm[2, 2]).astype(np.float32).ravel()
     return img, Hn, nodes_for(cam, span, Hm)
 
