@@ -15,7 +15,7 @@ cores. Two more workloads ride along as sub-objects:
   "pipeline"   the full receive path, batched: 32 captured 1920x1080 uint8 frames, each an n = 1536 picture
                (LIZARD-1024, 124 blocks) in the 256 ring through a random homography (rotation +-10 deg, scale 0.6-0.9
                of the frame height, perspective up to 0.05), radial k1 in [-0.05, 0.05], blur sigma 0.5-1.5 px, AWGN
-               sigma 2-8 grey levels (lizard_amd.sim, rendered on the host); warp -> FFT -> demap -> LDPC -> CRC as one
+               sigma 2-8 grey levels (made on the GPU: lizx_modulate, lizx_capture); warp -> FFT -> demap -> LDPC -> CRC as one
                HIP graph, against the same launches without the graph, frame by frame, and frame by frame with the
                LDPC launched once a codeword; frames/s and payload MB/s; the reference decoder on the host cores.
   "fft_demap"  64 picture grids with n drawn from {384, 768, 1152, 1536} (each at its top format), the sender's
@@ -181,27 +181,35 @@ def cpu_baseline(batches, seconds):
 
 
 def pipeline_frames(count, seed):
-    """count captures of LIZARD-1024 in the 256 ring through random cameras (rendered on the host: the library's
-    device memory is its own, so inputs travel through its allocator)."""
+    """count captures of LIZARD-1024 in the 256 ring through random cameras, made on the GPU (the sender's picture,
+    lizx_modulate, then the camera channel, lizx_capture) and copied to the host, where the reference decoder reads
+    them and from where the batch legs upload them."""
     from lizard_amd import sim
     rng = np.random.default_rng(seed)
+    dec = lizard_amd.Decoder(0)
     caps = []
     for k in range(count):
         cam = sim.Camera.random(rng)
         blocks = sim.random_blocks(64, rng)
-        img, H, nodes = sim.capture(sim.picture(64, blocks, cnt=k % 4), 512, cam, seed=seed + k)
-        caps.append(dict(img=img.numpy(), H=H, nodes=np.ascontiguousarray(nodes), blocks=blocks, cam=cam))
+        dpic = dec.modulate(64, blocks, cnt=k % 4)
+        dimg, H, nodes = sim.capture_gpu(dec, 64, 512, dpic, cam, seed=seed + k)
+        img = dimg.get(np.uint8, cam.W * cam.Hh).reshape(cam.Hh, cam.W)
+        dimg.free()
+        dpic.free()
+        caps.append(dict(img=img, H=H, nodes=np.ascontiguousarray(nodes), blocks=blocks, cam=cam))
+    dec.close()
     return caps
 
 
 def soft_bytes(version):
     """Algorithmic HBM bytes of one frame's soft-value kernels from a grid: row pass n^2 4 read + E n 8 write, column
-    pass E n 8 read + E 2E 8 write, demap 320 subch 8 read + 640 subch write (plus sub-channel estimates)."""
+    pass E n 8 read + 320 subch 8 write (the coefficients in coefficient order), demap 320 subch 8 read + 640 subch
+    write (plus sub-channel estimates)."""
     info = lizard_amd.format_info(version)
     n, subch = info["n"], info["subch"]
     u, v = lizard_amd.format_coefs(version)
     E = int(max(np.abs(u).max(), v.max())) + 1
-    return {"rows": n * n * 4 + E * n * 8, "cols": E * n * 8 + E * 2 * E * 8,
+    return {"rows": n * n * 4 + E * n * 8, "cols": E * n * 8 + 320 * subch * 8,
             "demap": 320 * subch * 8 + 640 * subch + subch * 12, "n": n, "E": E}
 
 
@@ -484,23 +492,29 @@ def cpu_sender(work, seconds):
 
 
 def refine_frames(count, seed):
-    """count captures of LIZARD-320 (n = 768) in the 128 ring through random cameras, each with a registration whose
-    homography corners are up to 1.5 px off (the lens term known)."""
+    """count captures of LIZARD-320 (n = 768) in the 128 ring through random cameras (sender and channel on the GPU),
+    each with a registration whose homography corners are up to 1.5 px off (the lens term known)."""
     from lizard_amd import sim
     rng = np.random.default_rng(seed)
+    dec = lizard_amd.Decoder(0)
     side = 256 + 30
     cs = np.array([[0, 0], [side, 0], [side, side], [0, side]], float)
     out = []
     for k in range(count):
         cam = sim.Camera.random(rng, scale=(0.75, 0.9))
         blocks = sim.random_blocks(20, rng)
-        img, _, _ = sim.capture(sim.picture(20, blocks), 256, cam, seed=seed + k)
+        dpic = dec.modulate(20, blocks)
+        dimg, _, _ = sim.capture_gpu(dec, 20, 256, dpic, cam, seed=seed + k)
+        img = dimg.get(np.uint8, cam.W * cam.Hh).reshape(cam.Hh, cam.W)
+        dimg.free()
+        dpic.free()
         Ht = cam.H(256)
         Ht = Ht / Ht[2, 2]
         w = cs @ Ht[:, :2].T + Ht[:, 2]
         H0 = sim.homography(cs, w[:, :2] / w[:, 2:] + rng.uniform(-1.5, 1.5, (4, 2)))
         scale = np.sqrt(abs(np.linalg.det(Ht[:2, :2]))) * 256 / 768  # capture pixels a picture sample
-        out.append(dict(img=img.numpy(), H0=H0, k1=cam.k1, blocks=blocks, blur=np.sqrt(cam.blur ** 2 + 1 / 3) / scale))
+        out.append(dict(img=img, H0=H0, k1=cam.k1, blocks=blocks, blur=np.sqrt(cam.blur ** 2 + 1 / 3) / scale))
+    dec.close()
     return out
 
 
