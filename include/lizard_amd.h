@@ -167,6 +167,21 @@ typedef struct {
   const float *nodes;
 } lizx_registration;
 
+/* The format word (research/SPEC.md 5.3-5.4; liblizard/src/fmt.h, fmt.c, rs.c): the border band's RS(bytes, 3) word
+   naming the symbol's version and display rate. Host.
+   lizx_word_geometry: its word cells a side and codeword bytes in ring span (fmt.c ob_fmt_side_cells, ob_fmt_bytes).
+   lizx_word_encode: the codeword (magic, version, fps, Reed-Solomon parity; ob_fmt_encode); returns its bytes.
+   lizx_word_decode: soft values q[side * cells + i] in -0.5 .. 0.5, positive dark -> 1 with *version, *fps, or 0
+   (hard decisions, then the least confident bytes as erasures; ob_fmt_decode).
+   lizx_read_word: the same from a capture (GREY8 w x h, rows stride bytes apart) through a registration of ring span,
+   the band read against the timing track (acquire.c:4503 ob_thin_read_fmt); q (optional) receives the 4 * cells soft
+   values. 0 also where the border shows no contrast. */
+LIZ_API int lizx_word_geometry(int span, int *cells, int *bytes);
+LIZ_API int lizx_word_encode(int span, int version, int fps, uint8_t *cw);
+LIZ_API int lizx_word_decode(int span, const float *q, int *version, int *fps);
+LIZ_API int lizx_read_word(int span, const uint8_t *img, int w, int h, int stride, const lizx_registration *reg,
+                           float *q, int *version, int *fps);
+
 /* The warp alone (the reference's picture sampler, liblizard/src/acquire.c:5121-5196 ob_sample_grid): the n x n
    picture samples of LIZARD-16V in the ring of span, through reg, from a w x h GREY8 capture (rows stride bytes
    apart), into grid (row-major). The samples are the reference's scalar grid walk's, to the bit. */

@@ -86,6 +86,10 @@ def lib():
     sig("lizx_modulate", _I, _P, _I, _I, _P, _I, _P, _P)
     sig("lizx_frame_slots_device", _I, _P, _I, _I, _P, _I, _P)
     sig("lizx_capture", _I, _P, _I, _I, _P, _P, _P)
+    sig("lizx_read_word", _I, _I, _P, _I, _I, _I, _P, _P, _P, _P)
+    sig("lizx_word_geometry", _I, _I, _P, _P)
+    sig("lizx_word_encode", _I, _I, _I, _I, _P)
+    sig("lizx_word_decode", _I, _I, _P, _P, _P)
     sig("lizx_format_coefs", _I, _I, _P, _P)
     sig("lizx_format_blocks", _I, _I, _P, _P, _P)
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
@@ -330,6 +334,42 @@ class LdpcDecoder:
         check(lib().lizx_ldpc_tune(self.h, count, d_rate.ptr, d_llr.ptr, d_off.ptr, d_its.ptr, max_iter, stall_it,
                                    stall_ratio, reps, C.byref(pc)))
         return pc.value
+
+
+def word_geometry(span):
+    """lizx_word_geometry: (word cells a side, codeword bytes) in the ring of span."""
+    c, b = C.c_int(0), C.c_int(0)
+    check(lib().lizx_word_geometry(span, C.byref(c), C.byref(b)))
+    return c.value, b.value
+
+
+def word_encode(span, version, fps=0):
+    """lizx_word_encode: the format word's codeword (uint8) for the ring of span."""
+    cw = np.zeros(64, np.uint8)
+    n = check(lib().lizx_word_encode(span, version, fps, _p(cw)))
+    return cw[:n]
+
+
+def word_decode(span, q):
+    """lizx_word_decode: soft values (4 x cells) -> (version, fps), or None."""
+    q = np.ascontiguousarray(q, np.float32)
+    v, f = C.c_int(0), C.c_int(0)
+    got = check(lib().lizx_word_decode(span, _p(q), C.byref(v), C.byref(f)))
+    return (v.value, f.value) if got else None
+
+
+def read_word(span, img, H, nodes, soft=False):
+    """lizx_read_word: the format word of a capture (uint8 h x w) through a registration (H 9, nodes [ny, nx, 2]):
+    (version, fps), or None where it did not decode; with soft, also the soft values read."""
+    img = np.ascontiguousarray(img, np.uint8)
+    reg = registration(H, nodes)
+    cells, _ = word_geometry(span)
+    q = np.zeros(4 * cells, np.float32)
+    v, f = C.c_int(0), C.c_int(0)
+    got = check(lib().lizx_read_word(span, _p(img), img.shape[1], img.shape[0], img.shape[1], C.byref(reg), _p(q),
+                                     C.byref(v), C.byref(f)))
+    w = (v.value, f.value) if got else None
+    return (w, q) if soft else w
 
 
 class Decoder:

@@ -11,6 +11,7 @@
 #include "format.hpp"
 #include "ldpc.hpp"
 #include "channel.hpp"
+#include "word.hpp"
 #include "register.hpp"
 #include "spectrum.hpp"
 
@@ -829,6 +830,57 @@ LIZ_API int lizx_capture(liz_decoder *d, int version, int span, const float *d_p
     launch_channel(c, d_picture, d->chan_a.as<float>(), d->chan_b.as<float>(), d_img, e.stream);
     e.sync();
     return 0;
+  });
+}
+
+LIZ_API int lizx_word_geometry(int span, int *cells, int *bytes) {
+  return guard([&] {
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    if (cells) *cells = fmt_side_cells(span + 30);
+    if (bytes) *bytes = fmt_bytes(span + 30);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_word_encode(int span, int version, int fps, uint8_t *cw) {
+  return guard([&] {
+    need(cw && (span == 128 || span == 256 || span == 512), "a ring's span and a place for the codeword");
+    const int bytes = fmt_bytes(span + 30);
+    FmtWord fw;
+    fw.version = version;
+    fw.fps = fps;
+    need(fmt_encode(fw, cw, bytes) == 0, "a version 4 to 64 and a rate 0 to 255");
+    return bytes;
+  });
+}
+
+LIZ_API int lizx_word_decode(int span, const float *q, int *version, int *fps) {
+  return guard([&] {
+    need(q && (span == 128 || span == 256 || span == 512), "a ring's span and the soft values");
+    const int side = span + 30;
+    FmtWord fw;
+    if (fmt_decode(q, fmt_side_cells(side), fmt_bytes(side), &fw) != 0) return 0;
+    if (version) *version = fw.version;
+    if (fps) *fps = fw.fps;
+    return 1;
+  });
+}
+
+LIZ_API int lizx_read_word(int span, const uint8_t *img, int w, int h, int stride, const lizx_registration *reg,
+                           float *q, int *version, int *fps) {
+  return guard([&] {
+    need(img && reg && reg->nodes && w >= 2 && h >= 2 && stride >= w, "an image of 2 x 2 pixels or more and a registration");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    const int side = span + 30, cells = fmt_side_cells(side), bytes = fmt_bytes(side);
+    if (!cells) return 0;
+    std::vector<float> qv(4 * cells);
+    if (!read_word_soft(span, img, w, h, stride, reg->H, reg->nodes, qv.data(), cells)) return 0;
+    if (q) std::memcpy(q, qv.data(), qv.size() * sizeof(float));
+    FmtWord fw;
+    if (fmt_decode(qv.data(), cells, bytes, &fw) != 0) return 0;
+    if (version) *version = fw.version;
+    if (fps) *fps = fw.fps;
+    return 1;
   });
 }
 

@@ -3,6 +3,7 @@
  * only calls its functions, the way the reference's own test programs do (liblizard/test/phy_x86.c includes
  * lizard_phy.c and defines ob_debug_hash the same way). Used to pin the oracle and as the CPU baseline. */
 #include "lizard_phy.c"
+#include "rs.h"
 #include <stdio.h>
 
 uint32_t ob_debug_hash;
@@ -152,6 +153,42 @@ int ref_sample_grid(const void *p, const uint8_t *img, int iw, int ih, const flo
   ob_reg_free(&reg);
   return 0;
 }
+/* ---------------------------------------------------------------- the format word (fmt.c, rs.c, acquire.c) */
+void ref_rs_encode(uint8_t *cw, int k, int nroots) { rs_encode(cw, k, nroots); }
+int ref_rs_decode(uint8_t *cw, int n, int nroots, const int *eras, int neras) {
+  return neras ? rs_decode_er(cw, n, nroots, eras, neras) : rs_decode(cw, n, nroots);
+}
+int ref_fmt_encode(int version, int fps, uint8_t *cw, int bytes) { ob_fmt_t f = { version, fps }; return ob_fmt_encode(&f, cw, bytes); }
+int ref_fmt_decode(const float *q, int cells, int bytes, int *version, int *fps) {
+  ob_fmt_t f = { 0, 0 };
+  const int r = ob_fmt_decode(q, cells, bytes, &f);
+  *version = f.version; *fps = f.fps;
+  return r;
+}
+int ref_fmt_geom(int side, int *cells, int *bytes) { *cells = ob_fmt_side_cells(side, OB_THIN_RESERVE); *bytes = ob_fmt_bytes(side, OB_THIN_RESERVE); return 0; }
+/* The word read as lizard_acquire holds it (lizard_phy.c:1554-1556): the soft values (4 * cells, into q) and the
+   decode through a registration given as H and the node lattice's displacements. dims out: cells, bytes. Returns 1
+   decoded (version, fps set), 0 read but not decoded, -1 nothing read. */
+int ref_read_word(const void *p, const uint8_t *img, int iw, int ih, const float *H, const float *nodes, float *q,
+                  int *dims, int *version, int *fps) {
+  const lizard_t *f = p;
+  const struct lizard_ws *w = f->ws;
+  image_t im;
+  ob_image_init(&im, img, iw, ih, 1.0f);
+  ob_reg_t reg;
+  memcpy(reg.H.h, H, sizeof reg.H.h);
+  const int nn = f->frame.nx * f->frame.ny;
+  reg.nodes = calloc((size_t)nn, sizeof(node_t));
+  for (int i = 0; i < nn; i++) { reg.nodes[i].dx = nodes[2 * i]; reg.nodes[i].dy = nodes[2 * i + 1]; reg.nodes[i].done = 1; }
+  dims[0] = w->fmt_cells; dims[1] = w->fmt_bytes;
+  const int got = w->fmt_cells && ob_thin_read_fmt(&f->frame, &im, &reg, q, w->fmt_cells);
+  ob_fmt_t fm = { 0, 0 };
+  const int ok = got && ob_fmt_decode(q, w->fmt_cells, w->fmt_bytes, &fm) == 0;
+  *version = ok ? fm.version : 0; *fps = ok ? fm.fps : 0;
+  ob_reg_free(&reg);
+  return got ? ok : -1;
+}
+
 /* The node lattice: nx, ny, then node_x[nx], node_y[ny] in modules. */
 int ref_node_lattice(const void *p, int *dims, float *nx_out, float *ny_out) {
   const lizard_t *f = p;
