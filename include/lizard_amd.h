@@ -73,9 +73,13 @@ LIZ_API void liz_decoder_free(liz_decoder *d);
 /* Replaces lizard.h:130 (api/codec.cpp:118). */
 LIZ_API int liz_decoder_max_blocks(const liz_decoder *d);
 /* Replaces lizard.h:154 (api/codec.cpp:121-160). The finder (registration estimation) is not part of this library:
-   a frame decodes through the registration held by lizx_decoder_hold_registration, the seam the reference's own
-   split decode uses (liblizard/src/lizard_phy.h:193-199, lizard_held_reg lizard_phy.c:1899-1900). With none held it
-   returns LIZ_E_STATE. */
+   a frame decodes through the registration held by lizx_decoder_hold, the seam the reference's own split decode uses
+   (liblizard/src/lizard_phy.h:193-199, lizard_held_reg lizard_phy.c:1899-1900). With none held it returns
+   LIZ_E_STATE. The frame's own format word is read through that registration (acquire.c:4503, fmt.h:75) and the
+   version taken as liblizard/src/any.c:48-51 takes it: the word's (out->word = 1, out->fps, *held set to it), else
+   *held when it is 4..64 (out->held_used = 1), else the version the hold states (0: none); with no version, or a
+   picture larger than nmax, the frame is not decoded (returns 0, out->n = 0). out->quad: the lattice's corner nodes
+   in the image. */
 LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stride, liz_pixfmt fmt, int *held,
                        uint8_t *verified, liz_decoded *out);
 
@@ -195,8 +199,9 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
                             const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
                             lizx_soft *soft);
 
-/* Holds a registration and format for liz_decode (the state lizard_acquire leaves for lizard_finish). The nodes are
-   copied. version 4..64, span 128/256/512. */
+/* Holds a registration for liz_decode (the state lizard_acquire leaves for lizard_finish). The nodes are copied.
+   span 128/256/512. version 0: the frames' words (or *held) name the format; 4..64: the format for frames whose word
+   does not read and with none held (frames painted without the border word; no counterpart in the reference). */
 LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_registration *reg);
 
 /* ---- lizx_ldpc: the LDPC decoder alone, batched ----------------------------------------------------------------- */

@@ -593,7 +593,7 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     const int bpp = fmt == LIZ_GREY8 ? 1 : 4;
     need(stride >= w * bpp, "a stride of at least the image's width in bytes");
     if (out) *out = liz_decoded{};
-    if (!d->held_version) throw Error(LIZ_E_STATE, "no registration held: this library has no finder; hold one first");
+    if (!d->held_span) throw Error(LIZ_E_STATE, "no registration held: this library has no finder; hold one first");
     // tight luma as the reference's codec reads it (api/codec.cpp:128-140): BGRA/RGBA/RGBX weights 77/150/29
     const uint8_t *img = px;
     int istride = stride;
@@ -611,7 +611,47 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
       img = d->luma.data();
       istride = w;
     }
-    const Format *f = format_for(d->held_version);
+    const int span = d->held_span;
+    liz_decoded o{};
+    o.found = 1;
+    o.ring = span == 128 ? 0 : span == 256 ? 1 : 2;
+    // the corner nodes in the image (acquire.c:656-660 corner_coords, 5026-5033): H at the lattice's first and last
+    // node positions plus those nodes' displacements; top left, top right, bottom right, bottom left
+    {
+      float pos[MAX_NODES];
+      const int nx = node_lattice(span, pos);
+      const float *H = d->held_reg.H;
+      for (int c = 0; c < 4; c++) {
+        const int i = (c == 1 || c == 2) ? nx - 1 : 0, j = c >= 2 ? nx - 1 : 0;
+        const float mx = pos[i], my = pos[j], ww = H[6] * mx + H[7] * my + H[8];
+        o.quad[2 * c] = (H[0] * mx + H[1] * my + H[2]) / ww + d->held_nodes[2 * (j * nx + i)];
+        o.quad[2 * c + 1] = (H[3] * mx + H[4] * my + H[5]) / ww + d->held_nodes[2 * (j * nx + i) + 1];
+      }
+    }
+    // the frame's own word, read through the held registration (ob_thin_read_fmt acquire.c:4503, ob_fmt_decode
+    // fmt.h:75); the version as any.c:48-51 takes it: the word's, else the caller's held one, else (an extension) the
+    // version the hold states for frames that carry no word; with none of them the frame is not decoded
+    const int side = span + 30, cells = fmt_side_cells(side);
+    std::vector<float> q(4 * cells);
+    FmtWord fw{};
+    o.word = read_word_soft(span, img, w, h, istride, d->held_reg.H, d->held_nodes.data(), q.data(), cells) &&
+             fmt_decode(q.data(), cells, fmt_bytes(side), &fw) == 0;
+    int version = 0;
+    if (o.word) {
+      version = fw.version;
+      o.fps = fw.fps;
+      *held = version;  // api/codec.cpp:143: updated whenever a frame's own word reads
+    } else if (*held >= FMT_VERSION_MIN && *held <= FMT_VERSION_MAX) {
+      version = *held;
+      o.held_used = 1;
+    } else {
+      version = d->held_version;
+    }
+    const Format *f = version ? format_for(version) : nullptr;
+    if (!f || f->n > d->nmax) {  // not finished: no version, or a picture beyond this decoder's (any.c:52-53)
+      if (out) *out = o;
+      return 0;
+    }
     const int B = f->blocks;
     d->ok.resize(B);
     std::vector<int8_t> its(B);
@@ -620,20 +660,15 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     std::vector<float> pil(B);
     lizx_soft sp{};
     sp.blk_pilot = pil.data();
-    decode_capture(d, d->held_version, d->held_span, img, w, h, istride, &d->held_reg, all.data(), d->ok.data(),
-                   its.data(), est.data(), &sp);
+    decode_capture(d, version, span, img, w, h, istride, &d->held_reg, all.data(), d->ok.data(), its.data(), est.data(),
+                   &sp);
     int k = 0;
     const int total = std::min(B, d->top);
     for (int b = 0; b < total; b++)
       if (d->ok[b]) std::memcpy(verified + static_cast<size_t>(k++) * BLOCK_BYTES, all.data() + static_cast<size_t>(b) * BLOCK_BYTES, BLOCK_BYTES);
     if (out) {
-      liz_decoded o{};
-      o.found = 1;
-      o.ring = d->held_span == 128 ? 0 : d->held_span == 256 ? 1 : 2;
       o.n = f->n;
-      o.word = 0;
-      o.blocks = d->held_version;
-      o.held_used = 1;
+      o.blocks = version;
       o.total = B;
       o.verified = k;
       // the pilots over the even blocks and the odd (bits 0 and 1 of the picture's count): each parity's mean reading
@@ -659,7 +694,6 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
       }
       *out = o;
     }
-    if (!*held) *held = d->held_version;
     return k;
   });
 }
@@ -925,7 +959,7 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
 LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_registration *reg) {
   return guard([&] {
     need(d && reg && reg->nodes, "a decoder and a registration with nodes");
-    if (!format_for(version)) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64)");
+    if (version && !format_for(version)) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64, or 0)");
     need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
     float pos[MAX_NODES];
     const int nx = node_lattice(span, pos);

@@ -109,3 +109,55 @@ def test_liz_decode_without_registration_is_state_error():
         dec.decode(np.zeros((10, 10), np.uint8))
     assert e.value.code == lizard_amd.LIZ_E_STATE
     dec.close()
+
+
+@pytest.mark.parametrize("version,span,W,Hh,stale", [(5, 128, 1280, 720, 0), (20, 256, 1920, 1080, 64)])
+def test_liz_decode_word_names_the_format(version, span, W, Hh, stale):
+    """A hold that states no format: the frame's own painted word names it (liblizard/src/any.c:48-51), over a stale
+    *held too; *held follows the word (api/codec.cpp:143), out->fps is the word's, and every block verifies."""
+    C = Capture(version, span, W=W, Hh=Hh, scale=0.85, seed=17, blur=0.7, noise=3.0)
+    want = lizard_amd.read_word(span, C.img, C.H, C.nodes)
+    assert want is not None and want[0] == version
+    dec = lizard_amd.Decoder(1536)
+    dec.hold(0, span, C.H, C.nodes)
+    k, blocks, out, held = dec.decode(C.img, held=stale)
+    dec.close()
+    assert out.found == 1 and out.word == 1 and out.held_used == 0
+    assert out.blocks == version and held == version and out.fps == want[1]
+    assert out.ring == {128: 0, 256: 1, 512: 2}[span] and out.n == lizard_amd.format_info(version)["n"]
+    assert k == C.blocks_n == out.total == out.verified
+    assert np.array_equal(blocks.ravel(), C.blocks)
+    # the corner nodes in the image, top left, top right, bottom right, bottom left (nodes zero here: H alone)
+    H = C.H.reshape(3, 3).astype(np.float64)
+    x0, x1 = C.node_x[0], C.node_x[-1]
+    for c, (mx, my) in enumerate([(x0, x0), (x1, x0), (x1, x1), (x0, x1)]):
+        p = H @ np.array([mx, my, 1.0])
+        assert abs(out.quad[2 * c] - p[0] / p[2]) < 1e-3 and abs(out.quad[2 * c + 1] - p[1] / p[2]) < 1e-3
+
+
+def test_liz_decode_without_a_word():
+    """Frames with no word band (a grey ring): decoded at *held when it names a format (held_used), else at the version
+    the hold states, else not decoded; *held is never changed without a word (api/codec.cpp:143)."""
+    from lizard_amd import sim
+    rng = np.random.default_rng(4)
+    blocks = sim.random_blocks(5, rng)
+    img, H, nodes = sim.capture(sim.picture(5, blocks), 128, sim.Camera(1280, 720, blur=0.7, noise=3.0), seed=4)
+    img = img.numpy()
+    assert lizard_amd.read_word(128, img, H, nodes) is None
+    dec = lizard_amd.Decoder(384)
+    dec.hold(0, 128, H, nodes)
+    k, got, out, held = dec.decode(img, held=5)
+    assert out.word == 0 and out.held_used == 1 and out.blocks == 5 and held == 5 and out.fps == 0
+    assert k == out.total and np.array_equal(got.ravel(), np.asarray(blocks).ravel())
+    k, got, out, held = dec.decode(img, held=0)
+    assert k == 0 and out.found == 1 and out.n == 0 and out.blocks == 0 and out.total == 0 and held == 0
+    k, got, out, held = dec.decode(img, held=99)  # not a version: as none held
+    assert k == 0 and out.n == 0 and held == 99
+    dec.hold(5, 128, H, nodes)
+    k, got, out, held = dec.decode(img, held=0)
+    assert out.word == 0 and out.held_used == 0 and out.blocks == 5 and held == 0 and k == out.total
+    # a word-less frame at a picture beyond this decoder's nmax is not decoded
+    dec.hold(0, 128, H, nodes)
+    k, got, out, held = dec.decode(img, held=64)
+    assert k == 0 and out.n == 0 and out.held_used == 1
+    dec.close()
