@@ -181,6 +181,10 @@ typedef struct {
    the band read against the timing track (acquire.c:4503 ob_thin_read_fmt); q (optional) receives the 4 * cells soft
    values. 0 also where the border shows no contrast. */
 LIZ_API int lizx_word_geometry(int span, int *cells, int *bytes);
+/* The symbol's border as modules (liblizard/src/layout.c ob_layout_init + ob_layout_set_fmt): (span + 30)^2 kinds
+   row-major, 0 picture and guard, 1 light, 2 dark, | 8 on the word's cells; version 0: no word (its cells light).
+   Returns the count. Host only. */
+LIZ_API int lizx_border_kinds(int span, int version, int fps, uint8_t *kinds);
 LIZ_API int lizx_word_encode(int span, int version, int fps, uint8_t *cw);
 LIZ_API int lizx_word_decode(int span, const float *q, int *version, int *fps);
 LIZ_API int lizx_read_word(int span, const uint8_t *img, int w, int h, int stride, const lizx_registration *reg,

@@ -88,6 +88,7 @@ def lib():
     sig("lizx_capture", _I, _P, _I, _I, _P, _P, _P)
     sig("lizx_read_word", _I, _I, _P, _I, _I, _I, _P, _P, _P, _P)
     sig("lizx_word_geometry", _I, _I, _P, _P)
+    sig("lizx_border_kinds", _I, _I, _I, _I, _P)
     sig("lizx_word_encode", _I, _I, _I, _I, _P)
     sig("lizx_word_decode", _I, _I, _P, _P, _P)
     sig("lizx_format_coefs", _I, _I, _P, _P)
@@ -356,6 +357,15 @@ def word_decode(span, q):
     v, f = C.c_int(0), C.c_int(0)
     got = check(lib().lizx_word_decode(span, _p(q), C.byref(v), C.byref(f)))
     return (v.value, f.value) if got else None
+
+
+def border_kinds(span, version=0, fps=0):
+    """lizx_border_kinds: the symbol's border as a (span + 30)^2 uint8 module map (0 picture, 1 light, 2 dark, | 8
+    word cells)."""
+    L = span + 30
+    k = np.zeros((L, L), np.uint8)
+    check(lib().lizx_border_kinds(span, version, fps, _p(k)))
+    return k
 
 
 def read_word(span, img, H, nodes, soft=False):

@@ -876,6 +876,23 @@ LIZ_API int lizx_word_geometry(int span, int *cells, int *bytes) {
   });
 }
 
+LIZ_API int lizx_border_kinds(int span, int version, int fps, uint8_t *kinds) {
+  return guard([&] {
+    need(kinds && (span == 128 || span == 256 || span == 512), "a ring's span and a place for the map");
+    need(version == 0 || (version >= FMT_VERSION_MIN && version <= FMT_VERSION_MAX), "version 4..64, or 0 for no word");
+    need(fps >= 0 && fps <= 255, "fps 0..255");
+    uint8_t cw[FMT_BYTES_MAX];
+    if (version) {
+      FmtWord fw;
+      fw.version = version;
+      fw.fps = fps;
+      fmt_encode(fw, cw, fmt_bytes(span + 30));
+    }
+    border_kinds(span, version ? cw : nullptr, kinds);
+    return (span + 30) * (span + 30);
+  });
+}
+
 LIZ_API int lizx_word_encode(int span, int version, int fps, uint8_t *cw) {
   return guard([&] {
     need(cw && (span == 128 || span == 256 || span == 512), "a ring's span and a place for the codeword");

@@ -152,6 +152,64 @@ inline int word_at(int i) { return (band_pair(i) + BAND_RUN) * CELL; }
 inline int thin_cells(int side) { return side > 2 * RESERVE ? (side - 2 * RESERVE) / CELL : 0; }
 inline int track_cells(int side) { return (thin_cells(side) + 1) / 2; }
 
+// depths into the border (layout.h): the light module, the solid line 1..2, the light gap 3..4, the band 5..6, then
+// light to the guard interval at 12 (the picture at 15 less its 3 modules of guard); corner marks fill [0, 12)^2
+constexpr int LINE = 2, GAP = 1 + LINE, BAND = GAP + 2, THIN = BAND + 2, CORNER = 12, GUARD = 12;
+}  // namespace
+
+int thin_track(int side, int j) {
+  // a per-side hash of the Manchester pair's index, flipped in the pair's second cell (layout.c:57-62, the format's)
+  uint32_t v = static_cast<uint32_t>(side * 4099 + (j >> 1)) * 2654435761u + 40503u;
+  v ^= v >> 15;
+  v *= 2246822519u;
+  v ^= v >> 13;
+  return static_cast<int>((v ^ static_cast<uint32_t>(j)) & 1);
+}
+
+void border_kinds(int span, const uint8_t *cw, uint8_t *kinds) {
+  const int L = span + BORDER_SIDE;
+  for (int y = 0; y < L; y++)
+    for (int x = 0; x < L; x++) {
+      const int dx = x < L - 1 - x ? x : L - 1 - x, dy = y < L - 1 - y ? y : L - 1 - y, d = dx < dy ? dx : dy;
+      int k;
+      if (dx < CORNER && dy < CORNER) {
+        // the corner mark: solid from depth 1 to CORNER - 2 on both axes, the gap ring left light through it
+        const bool in = dx >= 1 && dx <= CORNER - 2 && dy >= 1 && dy <= CORNER - 2;
+        k = in && !(d >= GAP && d < BAND) ? KIND_DARK : KIND_LIGHT;
+      } else if (d >= GUARD) {
+        k = KIND_DATA;
+      } else if (d >= 1 && d <= LINE) {
+        k = KIND_DARK;
+      } else if (d >= BAND && d < THIN) {
+        // the band: even runs of BAND_RUN cells the track, odd runs the word's (light until the word is set)
+        const int along = dy <= dx ? x : y, side = dy <= dx ? (y < L / 2 ? 0 : 2) : (x < L / 2 ? 3 : 1);
+        k = KIND_LIGHT;
+        if (along >= RESERVE && along < L - RESERVE) {
+          const int cell = (along - RESERVE) / CELL, run = cell / BAND_RUN;
+          if (!(run & 1) && thin_track(side, run / 2 * BAND_RUN + cell % BAND_RUN)) k = KIND_DARK;
+        }
+      } else {
+        k = KIND_LIGHT;
+      }
+      kinds[y * L + x] = static_cast<uint8_t>(k);
+    }
+  // the word's cells, two modules along by the band's two deep, flagged (layout.c:74-89); none without a word
+  if (!cw) return;
+  const int cells = fmt_side_cells(L), bytes = fmt_bytes(L);
+  for (int side = 0; side < 4; side++)
+    for (int i = 0; i < cells; i++) {
+      const uint8_t v = static_cast<uint8_t>((fmt_bit(cw, bytes, side, i) ? KIND_DARK : KIND_LIGHT) | KIND_WORD);
+      for (int j = 0; j < CELL; j++)
+        for (int e = 0; e < 2; e++) {
+          const int t = RESERVE + word_at(i) + j, dd = BAND + e;
+          const int x = side == 0 || side == 2 ? t : side == 3 ? dd : L - 1 - dd;
+          const int y = side == 0 ? dd : side == 2 ? L - 1 - dd : t;
+          kinds[y * L + x] = v;
+        }
+    }
+}
+
+namespace {
 // module coordinates on side 0..3 (top, right, bottom, left), t along it, d deep
 inline void side_xy(int len, int side, float t, float d, float *x, float *y) {
   *x = side == 0 || side == 2 ? t : side == 3 ? d : static_cast<float>(len) - d;

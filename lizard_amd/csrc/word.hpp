@@ -30,6 +30,14 @@ int fmt_bit(const uint8_t *cw, int bytes, int edge, int cell);
 // confident bytes as erasures, two more a step. 0, or -1 where nothing decoded with the magic and a valid version.
 int fmt_decode(const float *q, int cells, int bytes, FmtWord *f);
 
+// The border's module map (liblizard/src/layout.c thin_init + ob_layout_set_fmt): (span + 30)^2 kinds, row-major,
+// KIND_DATA where the picture and its guard interval are painted, else KIND_LIGHT or KIND_DARK, | KIND_WORD on the
+// word's cells. cw: the word's bytes (fmt_bytes of the side), or null for none (its cells light, not flagged).
+constexpr int KIND_DATA = 0, KIND_LIGHT = 1, KIND_DARK = 2, KIND_WORD = 8;
+void border_kinds(int span, const uint8_t *cw, uint8_t *kinds);
+// 1 where timing-track cell j of side 0..3 (top, right, bottom, left) is dark (layout.c ob_thin_track).
+int thin_track(int side, int j);
+
 // The soft word of a capture (tight uint8 luma w x h, row stride) through a registration: H (module -> pixel, 9
 // floats) and the node lattice's displacements (nodes: nx * nx pairs, the layout of ring `span`). q: 4 * cells values.
 // false where the border shows no contrast (no band to read).

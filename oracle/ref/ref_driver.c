@@ -166,6 +166,15 @@ int ref_fmt_decode(const float *q, int cells, int bytes, int *version, int *fps)
   return r;
 }
 int ref_fmt_geom(int side, int *cells, int *bytes) { *cells = ob_fmt_side_cells(side, OB_THIN_RESERVE); *bytes = ob_fmt_bytes(side, OB_THIN_RESERVE); return 0; }
+/* The border's module map (layout.c ob_layout_init, then ob_layout_set_fmt with cw when given): side^2 kinds. */
+int ref_layout_kinds(int side, const uint8_t *cw, uint8_t *kinds) {
+  ob_layout_t L;
+  if (ob_layout_init(&L, side)) return -1;
+  if (cw) ob_layout_set_fmt(&L, cw);
+  memcpy(kinds, L.kind, (size_t)side * side);
+  ob_layout_free(&L);
+  return 0;
+}
 /* The word read as lizard_acquire holds it (lizard_phy.c:1554-1556): the soft values (4 * cells, into q) and the
    decode through a registration given as H and the node lattice's displacements. dims out: cells, bytes. Returns 1
    decoded (version, fps set), 0 read but not decoded, -1 nothing read. */

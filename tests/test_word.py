@@ -123,3 +123,26 @@ def test_read_word_matches_reference_on_painted_captures(version, span, blur, no
     assert (int(dims[0]), int(dims[1])) == (cells, nbytes)
     assert np.abs(q - q_ref).max() < 1e-5
     assert got == (int(v[0]), int(fp[0]))
+
+
+@ref_needed
+@pytest.mark.parametrize("span", SPANS)
+def test_border_kinds_match_reference_layout(span):
+    """The border's module map, word cells and all, equals the reference's layout (ob_layout_init +
+    ob_layout_set_fmt) module for module; with no word, its cells light as the reference's before the word is set."""
+    r = ref("scalar")
+    side = span + 30
+    want = np.zeros(side * side, np.uint8)
+    assert r.ref_layout_kinds(side, None, ptr(want)) == 0
+    got = kinds = L.border_kinds(span)
+    assert np.array_equal(got.ravel(), want)
+    for version, fps in ((4, 0), (37, 60), (64, 255)):
+        cw = L.word_encode(span, version, fps)
+        assert r.ref_layout_kinds(side, ptr(cw), ptr(want)) == 0
+        got = L.border_kinds(span, version, fps)
+        assert np.array_equal(got.ravel(), want), (version, fps)
+    # what the map must show whatever the reference: corner marks, the solid line, the guard, word cells flagged
+    k, S = kinds, side
+    assert k[0, :].max() == 1 and (k[1:3, 12:S - 12] == 2).all() and (k[3:5, 12:S - 12] == 1).all()
+    assert (k[12:S - 12, 12:S - 12] == 0).all() and (k[5:11, 5:11] == 2).all() and k[11, 11] == 1
+    assert ((got & 8) != 0).sum() == 4 * 2 * 2 * L.word_geometry(span)[0] and not (kinds & 8).any()
