@@ -119,6 +119,13 @@ typedef struct {
    input maker; the model of lizard_amd/sim.py's capture). */
 LIZ_API int lizx_capture(liz_decoder *d, int version, int span, const float *d_picture, const lizx_camera *cam,
                          uint8_t *d_img);
+/* The same camera on the whole symbol as a page shows it (liblizard/src/lizard_phy.c:1150-1236 and :700-716 paint):
+   the border of lizx_border_kinds with the word for (version, fps), the 7 x 7 anchors, the picture taken as periodic
+   (its 3-module guard interval the picture wrapped round its edge), 3 paint pixels a module, white round it. The
+   camera's H maps module coordinates of the symbol (0 .. span + 30) to capture pixels. No counterpart in the
+   reference's decoder (a measurement input maker). */
+LIZ_API int lizx_capture_symbol(liz_decoder *d, int version, int span, int fps, const float *d_picture,
+                                const lizx_camera *cam, uint8_t *d_img);
 /* lizx_frame_slots on the GPU for frames x (blocks x 476 bytes) in device memory -> frames x subch x 640 slot bytes
    (0/1) in device memory; bit for bit lizx_frame_slots'. */
 LIZ_API int lizx_frame_slots_device(liz_decoder *d, int version, int frames, const uint8_t *d_blocks, int cnt,

@@ -86,6 +86,7 @@ def lib():
     sig("lizx_modulate", _I, _P, _I, _I, _P, _I, _P, _P)
     sig("lizx_frame_slots_device", _I, _P, _I, _I, _P, _I, _P)
     sig("lizx_capture", _I, _P, _I, _I, _P, _P, _P)
+    sig("lizx_capture_symbol", _I, _P, _I, _I, _I, _P, _P, _P)
     sig("lizx_read_word", _I, _I, _P, _I, _I, _I, _P, _P, _P, _P)
     sig("lizx_word_geometry", _I, _I, _P, _P)
     sig("lizx_border_kinds", _I, _I, _I, _I, _P)
@@ -429,12 +430,16 @@ class Decoder:
         check(lib().lizx_modulate(self.h, version, frames, _p(blocks), cnt, d.ptr, C.byref(ms)))
         return (d, ms.value) if timed else d
 
-    def capture(self, version, span, d_picture, w, h, H, k1=0.0, blur=0.0, noise=0.0, seed=0, out=None):
-        """lizx_capture: a device picture through a camera -> a DeviceArray of h x w uint8 (or into `out`)."""
+    def capture(self, version, span, d_picture, w, h, H, k1=0.0, blur=0.0, noise=0.0, seed=0, out=None, fps=None):
+        """lizx_capture: a device picture through a camera -> a DeviceArray of h x w uint8 (or into `out`); with fps,
+        lizx_capture_symbol: the whole symbol (border, word for (version, fps), anchors, guard) round the picture."""
         cam = Camera(int(w), int(h), (C.c_double * 9)(*np.asarray(H, np.float64).ravel()), float(k1), float(blur),
                      float(noise), int(seed))
         d = out if out is not None else DeviceArray(int(w) * int(h))
-        check(lib().lizx_capture(self.h, version, span, d_picture.ptr, C.byref(cam), d.ptr))
+        if fps is None:
+            check(lib().lizx_capture(self.h, version, span, d_picture.ptr, C.byref(cam), d.ptr))
+        else:
+            check(lib().lizx_capture_symbol(self.h, version, span, int(fps), d_picture.ptr, C.byref(cam), d.ptr))
         return d
 
     def frame_slots_device(self, version, d_blocks, frames, cnt, d_slots):

@@ -123,7 +123,7 @@ struct liz_decoder {
   lizx_registration held_reg{};
   std::vector<float> held_nodes;
   // the sender's pictures and the camera channel
-  DevBuf mod_blocks, mod_bits, mod_G, chan_a, chan_b;
+  DevBuf mod_blocks, mod_bits, mod_G, chan_a, chan_b, chan_kinds;
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
       if (p->version == version && p->span == span) return *p;
@@ -813,9 +813,11 @@ LIZ_API int lizx_modulate(liz_decoder *d, int version, int frames, const uint8_t
   });
 }
 
-LIZ_API int lizx_capture(liz_decoder *d, int version, int span, const float *d_picture, const lizx_camera *cam,
-                         uint8_t *d_img) {
-  return guard([&] {
+// The channel from a device picture through a camera; with a word's fps (>= 0) the whole symbol is painted round it
+// (border, word, anchors, guard interval, white quiet zone), else the picture alone in grey 128.
+static int capture_impl(liz_decoder *d, int version, int span, int fps, const float *d_picture, const lizx_camera *cam,
+                        uint8_t *d_img) {
+  {
     need(d && d_picture && cam && d_img && cam->w > 0 && cam->h > 0 && cam->w <= 16384 && cam->h <= 16384,
          "a decoder, a device picture, a camera of 1 to 16384 pixels a side and device memory for the capture");
     need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
@@ -858,12 +860,41 @@ LIZ_API int lizx_capture(liz_decoder *d, int version, int span, const float *d_p
     }
     c.noise = cam->noise;
     c.seed = cam->seed;
+    if (fps >= 0) {
+      const int L = span + 30;
+      std::vector<uint8_t> kinds(static_cast<size_t>(L) * L);
+      uint8_t cw[FMT_BYTES_MAX];
+      FmtWord fw;
+      fw.version = version;
+      fw.fps = fps;
+      fmt_encode(fw, cw, fmt_bytes(L));
+      border_kinds(span, cw, kinds.data());
+      d->chan_kinds.reserve(kinds.size());
+      hip_check(hipMemcpyAsync(d->chan_kinds.p, kinds.data(), kinds.size(), hipMemcpyHostToDevice, e.stream),
+                "border map");
+      c.kinds = d->chan_kinds.as<uint8_t>();
+      c.anch_a = span / 64;
+      for (int i = 0; i < 7; i++) c.anch_m0[i] = anchor_m0(span, i);
+    }
     const size_t px = static_cast<size_t>(c.W) * c.Hh;
     d->chan_a.reserve(px * sizeof(float));
     d->chan_b.reserve(px * sizeof(float));
     launch_channel(c, d_picture, d->chan_a.as<float>(), d->chan_b.as<float>(), d_img, e.stream);
     e.sync();
     return 0;
+  }
+}
+
+LIZ_API int lizx_capture(liz_decoder *d, int version, int span, const float *d_picture, const lizx_camera *cam,
+                         uint8_t *d_img) {
+  return guard([&] { return capture_impl(d, version, span, -1, d_picture, cam, d_img); });
+}
+
+LIZ_API int lizx_capture_symbol(liz_decoder *d, int version, int span, int fps, const float *d_picture,
+                                const lizx_camera *cam, uint8_t *d_img) {
+  return guard([&] {
+    need(fps >= 0 && fps <= 255, "fps 0..255");
+    return capture_impl(d, version, span, fps, d_picture, cam, d_img);
   });
 }
 

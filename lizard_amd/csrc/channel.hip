@@ -3,7 +3,11 @@
 // liblizard/src/acquire.c:1470-1512). Not part of the receive path: it makes that path's measurement inputs on the
 // device.
 //   k_chan_render  per capture pixel: the lens undone by fixed-point iteration, the inverse homography to module
-//                  coordinates, the picture read bilinearly (grey 128 outside it), times 255
+//                  coordinates, the picture read bilinearly (grey 128 outside it), times 255; or, given the border's
+//                  module map, the whole symbol as a page shows it (liblizard/src/lizard_phy.c:1150-1236, :700-716):
+//                  3 paint pixels a module read bilinearly, each white outside the symbol (the quiet zone and the
+//                  page), black or white on the border's cells, the anchors' squares, else the picture taken as
+//                  periodic (so the 3-module guard interval round it is the picture wrapped round its own edge)
 //   k_chan_blur_x  the Gaussian along x (edge pixels repeated)
 //   k_chan_blur_y  the Gaussian along y, AWGN from a counter-based generator, rounded half-even and clamped to uint8
 #include <hip/hip_runtime.h>
@@ -14,6 +18,42 @@
 
 namespace lz {
 namespace {
+
+// One paint pixel (ix, iy) of the symbol, 3 a module from its outer corner, in [0, 1].
+__device__ float paint_px(const ChannelParams &c, const float *pic, int ix, int iy) {
+  const int L = c.span + 30;
+  if (ix < 0 || iy < 0 || ix >= 3 * L || iy >= 3 * L) return 1.0f;
+  const int mx = ix / 3, my = iy / 3, k = c.kinds[my * L + mx] & 3;
+  if (k) return k == 2 ? 0.0f : 1.0f;
+  // the anchors (lizard_phy.c:1218-1234): a light middle to a quarter of the half-side, a dark ring to 0.6 of it, a
+  // light rim, by the Chebyshev distance of the pixel's middle from the square's
+  int ai = -1, aj = -1;
+  for (int i = 0; i < 7; i++) {
+    if (mx >= c.anch_m0[i] && mx < c.anch_m0[i] + c.anch_a) ai = i;
+    if (my >= c.anch_m0[i] && my < c.anch_m0[i] + c.anch_a) aj = i;
+  }
+  if (ai >= 0 && aj >= 0) {
+    const float half = 1.5f * c.anch_a;
+    const float dx = fabsf(static_cast<float>(ix - 3 * c.anch_m0[ai]) + 0.5f - half);
+    const float dy = fabsf(static_cast<float>(iy - 3 * c.anch_m0[aj]) + 0.5f - half);
+    const float d = fmaxf(dx, dy) / half;
+    return d < 0.25f || d >= 0.6f ? 1.0f : 0.0f;
+  }
+  // the picture, periodic: sample coordinate of the pixel's middle (exact where n = 3 span)
+  const int n = c.n;
+  const double sc = static_cast<double>(n) / (3.0 * c.span), off = 15.0 * n / c.span + 0.5;
+  double sx = (ix + 0.5) * sc - off, sy = (iy + 0.5) * sc - off;
+  sx -= floor(sx / n) * n;
+  sy -= floor(sy / n) * n;
+  int x0 = static_cast<int>(sx), y0 = static_cast<int>(sy);
+  x0 = x0 >= n ? n - 1 : x0;
+  y0 = y0 >= n ? n - 1 : y0;
+  const int x1 = x0 + 1 == n ? 0 : x0 + 1, y1 = y0 + 1 == n ? 0 : y0 + 1;
+  const float fx = static_cast<float>(sx - x0), fy = static_cast<float>(sy - y0);
+  const float top = pic[y0 * n + x0] * (1 - fx) + pic[y0 * n + x1] * fx;
+  const float bot = pic[y1 * n + x0] * (1 - fx) + pic[y1 * n + x1] * fx;
+  return top * (1 - fy) + bot * fy;
+}
 
 __global__ __launch_bounds__(256) void k_chan_render(ChannelParams c, const float *pic, float *out) {
   const int x = blockIdx.x * 256 + threadIdx.x, y = blockIdx.y;
@@ -27,6 +67,23 @@ __global__ __launch_bounds__(256) void k_chan_render(ChannelParams c, const floa
   }
   const double w = c.Hi[6] * px + c.Hi[7] * py + c.Hi[8];
   const double mx = (c.Hi[0] * px + c.Hi[1] * py + c.Hi[2]) / w, my = (c.Hi[3] * px + c.Hi[4] * py + c.Hi[5]) / w;
+  if (c.kinds) {
+    // paint pixels' middles at k + 1/2: bilinear over the four round the point
+    const double u = 3.0 * mx - 0.5, t = 3.0 * my - 0.5;
+    const double uf = floor(u), tf = floor(t);
+    // far outside the symbol (or no finite point) every paint pixel is white; keep the indices in int range
+    const double lim = 3.0 * (c.span + 30) + 8.0;
+    if (!(uf >= -8.0 && tf >= -8.0 && uf <= lim && tf <= lim)) {
+      out[static_cast<size_t>(y) * c.W + x] = 255.0f;
+      return;
+    }
+    const int i0 = static_cast<int>(uf), j0 = static_cast<int>(tf);
+    const float fx = static_cast<float>(u - uf), fy = static_cast<float>(t - tf);
+    const float top = paint_px(c, pic, i0, j0) * (1 - fx) + paint_px(c, pic, i0 + 1, j0) * fx;
+    const float bot = paint_px(c, pic, i0, j0 + 1) * (1 - fx) + paint_px(c, pic, i0 + 1, j0 + 1) * fx;
+    out[static_cast<size_t>(y) * c.W + x] = (top * (1 - fy) + bot * fy) * 255.0f;
+    return;
+  }
   const int n = c.n;
   double tx = (mx - 15.0) * (static_cast<double>(n) / c.span) - 0.5, ty = (my - 15.0) * (static_cast<double>(n) / c.span) - 0.5;
   float v = 128.0f / 255.0f;  // outside the picture: grey 128

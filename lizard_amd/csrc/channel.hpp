@@ -15,6 +15,11 @@ struct ChannelParams {
   float taps[2 * CHAN_MAX_R + 1];
   float noise;      // AWGN sigma, grey levels
   uint64_t seed;
+  // the whole symbol (null: the picture alone, grey 128 round it): the border's module map ((span + 30)^2 kinds,
+  // device), the anchors' side and first modules; painted at 3 pixels a module, white round it
+  const uint8_t *kinds;
+  int anch_a;
+  int anch_m0[7];
 };
 
 // Picture (n x n floats in [0, 1], device) -> capture (W x Hh uint8, device); d_a and d_b W x Hh float work areas.

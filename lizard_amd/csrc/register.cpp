@@ -6,13 +6,13 @@
 
 namespace lz {
 
-namespace {
-
 // The anchors' squares (liblizard/src/acquire.c:41-45): side span / 64 modules, square i's first module.
 int anchor_m0(int span, int i) {
   const int a = span / 64;
   return static_cast<int>(std::floor(BORDER + static_cast<double>(span) * (i + 0.5) / REG_ANCH - a / 2.0 + 0.5));
 }
+
+namespace {
 
 // Cubic Hermite weights through knots t[0..n-1] at x: the value is sum w[a] v[b + a], a = 0..3; slopes from the
 // neighbouring knots, one-sided at the ends (the interpolation of acquire.c:2196-2210).

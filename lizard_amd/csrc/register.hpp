@@ -13,6 +13,9 @@ namespace lz {
 
 constexpr int REG_ANCH = 7;                                 // anchors a side (SPEC 6.13)
 constexpr int REG_PARAMS = 8 + 1 + 2 * REG_ANCH * REG_ANCH; // homography, k1, anchors' (dx, dy)
+
+// Anchor square i's first module on either axis in ring span (squares span / 64 modules a side).
+int anchor_m0(int span, int i);
 constexpr int REG_PART = 58;                                // floats of partial sums a piece of a node cell
 constexpr int REG_SPLIT = 1;                                // pieces a node cell's rows are split into
 

@@ -209,6 +209,24 @@ int ref_node_lattice(const void *p, int *dims, float *nx_out, float *ny_out) {
 
 /* lizard_finish on a capture through a given registration (what lizard_acquire would have held): the reference's
    own sampler, erase, transform, soft values, LDPC and CRC. */
+/* The reference's own finder (lizard_acquire: the thin frame's finder, the corner nodes, the anchors, the border mesh)
+   on a tight uint8 capture at gamma 1. Returns 1 registered (H 9 floats, nodes nx * ny pairs, quad 8 floats, orient),
+   else 0; the hold is let go either way. */
+int ref_acquire(const void *p, const uint8_t *img, int iw, int ih, float *H, float *nodes, float *quad, int *orient) {
+  const lizard_t *f = p;
+  ob_result_t res;
+  const int got = lizard_acquire(f, img, iw, ih, 1.0f, &res);
+  if (got) {
+    const ob_reg_t *r = lizard_held_reg(f);
+    memcpy(H, r->H.h, 9 * sizeof(float));
+    const int nn = f->frame.nx * f->frame.ny;
+    for (int i = 0; i < nn; i++) { nodes[2 * i] = r->nodes[i].dx; nodes[2 * i + 1] = r->nodes[i].dy; }
+    memcpy(quad, res.quad, sizeof res.quad);
+    *orient = res.orient;
+  }
+  lizard_release(f);
+  return got ? 1 : 0;
+}
 int ref_decode_reg(const void *p, const uint8_t *img, int iw, int ih, const float *H, const float *nodes, uint8_t *blocks,
                    uint8_t *ok, int8_t *its, float *blk_est, int8_t *llr_out, float *align) {
   const lizard_t *f = p;

@@ -95,6 +95,9 @@ def ref(form="scalar"):
     _sig(lib, "ref_fmt_decode", I32, P, I32, I32, P, P)
     _sig(lib, "ref_fmt_geom", I32, I32, P, P)
     _sig(lib, "ref_layout_kinds", I32, I32, P, P)
+    _sig(lib, "ref_acquire", I32, P, P, I32, I32, P, P, P, P)
+    _sig(lib, "ref_anchor_m0", I32, I32, I32)
+    _sig(lib, "ref_anchor_side", I32, I32)
     _sig(lib, "ref_read_word", I32, P, P, I32, I32, P, P, P, P, P, P)
     _sig(lib, "ref_sample_grid", I32, P, P, I32, I32, P, P, I32, P)
     _sig(lib, "ref_node_lattice", I32, P, P, P, P)

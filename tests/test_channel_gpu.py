@@ -77,3 +77,80 @@ def test_capture_argument_errors(decoder_gpu):
     with pytest.raises(L.LizError):
         decoder_gpu.capture(5, 128, pic, 64, 64, np.eye(3), blur=9.0)
     pic.free()
+
+
+SYMBOLS = [(5, 128), (20, 256), (64, 512), (45, 512)]
+
+
+@pytest.mark.parametrize("version,span", SYMBOLS)
+def test_symbol_paint_matches_reference_page(decoder_gpu, version, span):
+    """lizx_capture_symbol through a camera that puts each paint pixel on one capture pixel (3 a module, the 2-module
+    quiet zone) against the reference encoder's own page (lizard_encode + lizard_paint_grey), from the same picture:
+    the border, word, corner marks, anchors and quiet zone to the grey level; where the format's picture is 3 samples
+    a module (the reference then paints the samples themselves) the picture and its wrapped guard too, but for
+    rounding ties."""
+    from _libs import ptr, ref
+    r = ref("scalar")
+    info = L.format_info(version)
+    n = info["n"]
+    f = r.ref_format_new(n, info["subch"], span)
+    fi = np.zeros(10, np.int32)
+    r.ref_format_info(f, ptr(fi))
+    pxm, px = int(fi[7]), int(fi[8])
+    assert pxm == 3 and px == 3 * (span + 30)
+    blocks = np.random.default_rng(version).integers(0, 256, info["blocks"] * 476).astype(np.uint8)
+    drive = np.zeros(px * px, np.float32)
+    assert r.ref_encode(f, ptr(blocks), 0, ptr(drive), None) == 0
+    Wg = px + 4 * pxm
+    page = np.zeros(Wg * Wg, np.uint8)
+    r.ref_paint_grey(f, ptr(drive), ptr(page))
+    pic = np.zeros(n * n, np.float32)
+    assert r.ref_encoded_picture(f, ptr(pic)) == 0
+    r.ref_format_free(f)
+    page = page.reshape(Wg, Wg)
+    Hm = np.array([[3.0, 0, 6.0], [0, 3.0, 6.0], [0, 0, 1.0]])
+    nx = len(L.node_lattice(span))
+    word = L.read_word(span, page, Hm.astype(np.float32).ravel(), np.zeros((nx, nx, 2), np.float32))
+    assert word is not None and word[0] == version
+    dpic = L.DeviceArray.of(pic)
+    dimg = decoder_gpu.capture(version, span, dpic, Wg, Wg, Hm, fps=word[1])
+    img = dimg.get(np.uint8, Wg * Wg).reshape(Wg, Wg)
+    dimg.free()
+    dpic.free()
+    kinds = L.border_kinds(span, version, word[1])
+    module = np.full((Wg, Wg), 255, np.int32)  # kind per capture pixel; 255 the quiet zone
+    module[6:Wg - 6, 6:Wg - 6] = np.repeat(np.repeat(kinds, 3, 0), 3, 1)
+    fixed = module != 0
+    assert np.array_equal(img[fixed], page[fixed])
+    d = np.abs(img.astype(np.int32) - page)
+    if n == 3 * span:
+        assert d.max() <= 1 and (d > 0).mean() < 1e-4
+    # the anchors too (inside the picture's modules, exactly black or white in both)
+    a = r.ref_anchor_side(span)
+    m0 = [r.ref_anchor_m0(span, i) for i in range(7)]
+    for j in m0:
+        for i in m0:
+            ys, xs = 6 + 3 * j, 6 + 3 * i
+            assert np.array_equal(img[ys:ys + 3 * a, xs:xs + 3 * a], page[ys:ys + 3 * a, xs:xs + 3 * a])
+
+
+def test_symbol_capture_decodes_and_names_itself(decoder_gpu):
+    """A symbol painted on the GPU and seen through a lens: its word reads (version and fps), and it decodes through
+    the channel's registration to the sent blocks."""
+    version, span, fps = 20, 256, 30
+    rng = np.random.default_rng(8)
+    blocks = sim.random_blocks(version, rng)
+    dpic = decoder_gpu.modulate(version, blocks)
+    cam = sim.Camera(1920, 1080, rot_deg=8.0, scale=0.85, persp=0.03, k1=-0.03, blur=0.9, noise=4.0)
+    Hm = cam.H(span)
+    dimg = decoder_gpu.capture(version, span, dpic, cam.W, cam.Hh, Hm, k1=cam.k1, blur=cam.blur, noise=cam.noise,
+                               seed=3, fps=fps)
+    img = dimg.get(np.uint8, cam.W * cam.Hh).reshape(cam.Hh, cam.W)
+    dimg.free()
+    dpic.free()
+    H = (Hm / Hm[2, 2]).astype(np.float32).ravel()
+    nodes = sim.nodes_for(cam, span, Hm)
+    assert L.read_word(span, img, H, nodes) == (version, fps)
+    out = decoder_gpu.decode_reg(version, span, img, H, nodes, soft=False)
+    assert out["good"] == L.format_info(version)["blocks"]
+    assert np.array_equal(out["blocks"], blocks.reshape(-1, 476))
