@@ -72,10 +72,10 @@ LIZ_API liz_decoder *liz_decoder_new(int nmax);
 LIZ_API void liz_decoder_free(liz_decoder *d);
 /* Replaces lizard.h:130 (api/codec.cpp:118). */
 LIZ_API int liz_decoder_max_blocks(const liz_decoder *d);
-/* Replaces lizard.h:154 (api/codec.cpp:121-160). The finder (registration estimation) is not part of this library:
-   a frame decodes through the registration held by lizx_decoder_hold, the seam the reference's own split decode uses
-   (liblizard/src/lizard_phy.h:193-199, lizard_held_reg lizard_phy.c:1899-1900). With none held it returns
-   LIZ_E_STATE. The frame's own format word is read through that registration (acquire.c:4503, fmt.h:75) and the
+/* Replaces lizard.h:154 (api/codec.cpp:121-160). A frame decodes through the registration held by lizx_decoder_hold
+   (the seam the reference's own split decode uses, liblizard/src/lizard_phy.h:193-199, lizard_held_reg
+   lizard_phy.c:1899-1900), or, with none held, through the one lizx_find finds on it (lizard_acquire's place; not
+   found: returns 0 with out->found = 0, out->ring = -1). The frame's own format word is read through that registration (acquire.c:4503, fmt.h:75) and the
    version taken as liblizard/src/any.c:48-51 takes it: the word's (out->word = 1, out->fps, *held set to it), else
    *held when it is 4..64 (out->held_used = 1), else the version the hold states (0: none); with no version, or a
    picture larger than nmax, the frame is not decoded (returns 0, out->n = 0). out->quad: the lattice's corner nodes
@@ -210,10 +210,23 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
                             const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
                             lizx_soft *soft);
 
-/* Holds a registration for liz_decode (the state lizard_acquire leaves for lizard_finish). The nodes are copied.
+/* Holds a registration for liz_decode (the state lizard_acquire leaves for lizard_finish); reg NULL lets it go, and
+   liz_decode then finds each frame's symbol itself. The nodes are copied.
    span 128/256/512. version 0: the frames' words (or *held) name the format; 4..64: the format for frames whose word
    does not read and with none held (frames painted without the border word; no counterpart in the reference). */
 LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_registration *reg);
+
+/* The finder (no entry of its own in the reference, whose liz_decode runs lizard_acquire inside: acquire.c:4631
+   find_frame, the corner nodes, the lens fit): from a capture alone (GREY8 w x h, rows stride bytes apart) the
+   symbol's ring (*span 128/256/512) and its registration: H (9 floats, module -> pixel) and nodes (caller's room for
+   lizx_node_lattice(span)^2 pairs, 40 x 40 at most: the radial lens's field at the nodes). quad (optional, 8 floats):
+   the border line's outer corners in the capture, TL TR BR BL. info (optional, 11 floats): k1, the timing track's
+   agreement (about 1), the border's contrast (0..1), the fit's rms (modules), edge points used, then the host clock's
+   milliseconds: upload, vote + peaks, sides, first probes, rings, refinement. Returns 1 found, 0
+   not. Its own design: a Hough vote of the gradients (GPU), the solid line's two edges probed to a fraction of a
+   pixel (GPU), the homography and lens fitted to them, ring and orientation by the timing track (host). */
+LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stride, int *span, float *H, float *nodes,
+                      float *quad, float *info);
 
 /* ---- lizx_ldpc: the LDPC decoder alone, batched ----------------------------------------------------------------- */
 

@@ -100,6 +100,7 @@ def lib():
     sig("lizx_warp", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P)
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
+    sig("lizx_find", _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_new", _P, _I)
     sig("lizx_batch_free", None, _P)
     sig("lizx_batch_plan", _I, _P, _I, _P, _P, _I, _I)
@@ -458,6 +459,34 @@ class Decoder:
     def hold(self, version, span, H, nodes):
         self._held = registration(H, nodes)
         check(lib().lizx_decoder_hold(self.h, version, span, C.byref(self._held)))
+
+    def release(self):
+        """Lets the held registration go: liz_decode then finds each frame's symbol itself."""
+        check(lib().lizx_decoder_hold(self.h, 0, 0, None))
+        self._held = None
+
+    def find(self, img):
+        """lizx_find on a capture (uint8 h x w): None, or a dict of span, H (9), nodes [nx, nx, 2], quad (4, 2), k1,
+        track, contrast, rms, points, ms (the stages' host milliseconds). Rows may be padded (a view whose rows are
+        contiguous is passed as it is, with its row stride)."""
+        img = np.asarray(img, np.uint8)
+        if img.ndim != 2 or img.strides[1] != 1 or img.strides[0] < img.shape[1]:
+            img = np.ascontiguousarray(img)
+        h, w = img.shape[:2]
+        nodes = np.zeros((40, 40, 2), np.float32)
+        H = np.zeros(9, np.float32)
+        span = C.c_int(0)
+        quad = np.zeros(8, np.float32)
+        info = np.zeros(11, np.float32)
+        got = check(lib().lizx_find(self.h, _p(img), w, h, img.strides[0], C.byref(span), _p(H), _p(nodes), _p(quad),
+                                    _p(info)))
+        if not got:
+            return None
+        nx = len(node_lattice(span.value))
+        return {"span": span.value, "H": H,
+                "nodes": nodes.ravel()[:nx * nx * 2].reshape(nx, nx, 2).copy(), "quad": quad.reshape(4, 2),
+                "k1": float(info[0]), "track": float(info[1]), "contrast": float(info[2]), "rms": float(info[3]),
+                "points": int(info[4]), "ms": info[5:11].copy()}
 
     def decode(self, img, fmt=1, held=0):
         """liz_decode on a capture (through the held registration): verified blocks, count, liz_decoded."""

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -13,6 +14,7 @@
 #include "channel.hpp"
 #include "word.hpp"
 #include "register.hpp"
+#include "finder.hpp"
 #include "spectrum.hpp"
 
 #define LIZARD_AMD_VERSION "lizard-amd 0.1 (gfx950)"
@@ -124,6 +126,8 @@ struct liz_decoder {
   std::vector<float> held_nodes;
   // the sender's pictures and the camera channel
   DevBuf mod_blocks, mod_bits, mod_G, chan_a, chan_b, chan_kinds;
+  FinderBufs finder;
+  std::vector<float> found_nodes;
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
       if (p->version == version && p->span == span) return *p;
@@ -584,6 +588,9 @@ LIZ_API int liz_decoder_max_blocks(const liz_decoder *d) {
   });
 }
 
+static bool find_registration(liz_decoder *d, const uint8_t *img, int w, int h, int stride, FindResult &fr,
+                              lizx_registration &reg, std::vector<float> &nodes);
+
 LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stride, liz_pixfmt fmt, int *held,
                        uint8_t *verified, liz_decoded *out) {
   return guard([&]() -> int {
@@ -593,7 +600,6 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     const int bpp = fmt == LIZ_GREY8 ? 1 : 4;
     need(stride >= w * bpp, "a stride of at least the image's width in bytes");
     if (out) *out = liz_decoded{};
-    if (!d->held_span) throw Error(LIZ_E_STATE, "no registration held: this library has no finder; hold one first");
     // tight luma as the reference's codec reads it (api/codec.cpp:128-140): BGRA/RGBA/RGBX weights 77/150/29
     const uint8_t *img = px;
     int istride = stride;
@@ -611,8 +617,21 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
       img = d->luma.data();
       istride = w;
     }
-    const int span = d->held_span;
+    // the registration: the held one, else the finder's on this frame (lizard_acquire's place in the reference)
     liz_decoded o{};
+    o.ring = -1;
+    int span = d->held_span;
+    lizx_registration found{};
+    const lizx_registration *reg = &d->held_reg;
+    if (!span) {
+      FindResult fr;
+      if (w < 16 || h < 16 || !find_registration(d, img, w, h, istride, fr, found, d->found_nodes)) {
+        if (out) *out = o;
+        return 0;
+      }
+      span = fr.span;
+      reg = &found;
+    }
     o.found = 1;
     o.ring = span == 128 ? 0 : span == 256 ? 1 : 2;
     // the corner nodes in the image (acquire.c:656-660 corner_coords, 5026-5033): H at the lattice's first and last
@@ -620,12 +639,12 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     {
       float pos[MAX_NODES];
       const int nx = node_lattice(span, pos);
-      const float *H = d->held_reg.H;
+      const float *H = reg->H;
       for (int c = 0; c < 4; c++) {
         const int i = (c == 1 || c == 2) ? nx - 1 : 0, j = c >= 2 ? nx - 1 : 0;
         const float mx = pos[i], my = pos[j], ww = H[6] * mx + H[7] * my + H[8];
-        o.quad[2 * c] = (H[0] * mx + H[1] * my + H[2]) / ww + d->held_nodes[2 * (j * nx + i)];
-        o.quad[2 * c + 1] = (H[3] * mx + H[4] * my + H[5]) / ww + d->held_nodes[2 * (j * nx + i) + 1];
+        o.quad[2 * c] = (H[0] * mx + H[1] * my + H[2]) / ww + reg->nodes[2 * (j * nx + i)];
+        o.quad[2 * c + 1] = (H[3] * mx + H[4] * my + H[5]) / ww + reg->nodes[2 * (j * nx + i) + 1];
       }
     }
     // the frame's own word, read through the held registration (ob_thin_read_fmt acquire.c:4503, ob_fmt_decode
@@ -634,7 +653,7 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     const int side = span + 30, cells = fmt_side_cells(side);
     std::vector<float> q(4 * cells);
     FmtWord fw{};
-    o.word = read_word_soft(span, img, w, h, istride, d->held_reg.H, d->held_nodes.data(), q.data(), cells) &&
+    o.word = read_word_soft(span, img, w, h, istride, reg->H, reg->nodes, q.data(), cells) &&
              fmt_decode(q.data(), cells, fmt_bytes(side), &fw) == 0;
     int version = 0;
     if (o.word) {
@@ -644,7 +663,7 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     } else if (*held >= FMT_VERSION_MIN && *held <= FMT_VERSION_MAX) {
       version = *held;
       o.held_used = 1;
-    } else {
+    } else if (reg == &d->held_reg) {
       version = d->held_version;
     }
     const Format *f = version ? format_for(version) : nullptr;
@@ -660,8 +679,7 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
     std::vector<float> pil(B);
     lizx_soft sp{};
     sp.blk_pilot = pil.data();
-    decode_capture(d, version, span, img, w, h, istride, &d->held_reg, all.data(), d->ok.data(), its.data(), est.data(),
-                   &sp);
+    decode_capture(d, version, span, img, w, h, istride, reg, all.data(), d->ok.data(), its.data(), est.data(), &sp);
     int k = 0;
     const int total = std::min(B, d->top);
     for (int b = 0; b < total; b++)
@@ -1006,7 +1024,13 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
 
 LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_registration *reg) {
   return guard([&] {
-    need(d && reg && reg->nodes, "a decoder and a registration with nodes");
+    need(d, "a decoder");
+    if (!reg) {  // let the hold go: liz_decode finds each frame's symbol itself
+      d->held_span = d->held_version = 0;
+      d->held_nodes.clear();
+      return 0;
+    }
+    need(reg->nodes, "a registration with nodes");
     if (version && !format_for(version)) throw Error(LIZ_E_FORMAT, "no format at that version (4 to 64, or 0)");
     need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
     float pos[MAX_NODES];
@@ -1017,6 +1041,59 @@ LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_
     d->held_version = version;
     d->held_span = span;
     return 0;
+  });
+}
+
+// The finder on a capture: uploads it, searches, and writes the registration (nodes into `nodes`, nx * nx pairs).
+static bool find_registration(liz_decoder *d, const uint8_t *img, int w, int h, int stride, FindResult &fr,
+                              lizx_registration &reg, std::vector<float> &nodes) {
+  need(img && w >= 16 && h >= 16 && stride >= w, "a capture of 16 x 16 pixels or more, rows stride >= w bytes apart");
+  LdpcEngine &e = *d->eng;
+  hip_check(hipSetDevice(e.device), "hipSetDevice");
+  const auto t0 = std::chrono::steady_clock::now();
+  d->finder.img.reserve(static_cast<size_t>(w) * h);
+  if (stride == w) {
+    hip_check(hipMemcpyAsync(d->finder.img.p, img, static_cast<size_t>(w) * h, hipMemcpyHostToDevice, e.stream),
+              "capture");
+  } else {
+    hip_check(hipMemcpy2DAsync(d->finder.img.p, w, img, stride, w, h, hipMemcpyHostToDevice, e.stream), "capture");
+  }
+  hip_check(hipStreamSynchronize(e.stream), "capture upload");
+  fr.ms[0] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  find_symbol(d->finder, d->finder.img.as<uint8_t>(), img, w, h, stride, e.stream, fr);
+  if (!fr.found) return false;
+  for (int i = 0; i < 9; i++) reg.H[i] = static_cast<float>(fr.H[i]);
+  RegModel m;
+  reg_model(fr.span, reg.H, w, h, m);
+  float params[REG_PARAMS] = {};
+  for (int i = 0; i < 8; i++) params[i] = reg.H[i];
+  params[8] = static_cast<float>(fr.k1);
+  reg_nodes(m, params, nodes);
+  reg.nodes = nodes.data();
+  return true;
+}
+
+LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stride, int *span, float *H, float *nodes,
+                      float *quad, float *info) {
+  return guard([&] {
+    need(d && H && nodes, "a decoder and room for H and the nodes");
+    FindResult fr;
+    lizx_registration got{};
+    std::vector<float> nv;
+    if (!find_registration(d, img, w, h, stride, fr, got, nv)) return 0;
+    std::memcpy(H, got.H, sizeof got.H);
+    std::memcpy(nodes, nv.data(), nv.size() * sizeof(float));
+    if (span) *span = fr.span;
+    if (quad) std::memcpy(quad, fr.quad, sizeof fr.quad);
+    if (info) {
+      info[0] = static_cast<float>(fr.k1);
+      info[1] = fr.track;
+      info[2] = fr.contrast;
+      info[3] = fr.rms;
+      info[4] = static_cast<float>(fr.points);
+      for (int k = 0; k < 6; k++) info[5 + k] = fr.ms[k];
+    }
+    return 1;
   });
 }
 

@@ -1,0 +1,69 @@
+// The finder: a symbol's registration (ring, homography, radial lens, orientation) from a capture alone, the job of
+// liblizard/src/acquire.c's find_frame + refine_node + the lens fit (:4631, :1242, :1470-1512), in a design of this
+// library's own. The GPU does the pixel work:
+//   k_find_vote   per pixel a Sobel gradient, thinned to its ridge (non-maximum suppression along the gradient), and
+//                 a vote into a Hough accumulator over (gradient direction, distance from the frame's centre): the
+//                 direction keeps its sign, so a dark-to-light edge and a light-to-dark one never share a bin
+//   k_find_max    the accumulator's maximum
+//   k_find_peaks  its local maxima above a fraction of that, with a 3 x 5 neighbourhood sum as their score
+//   k_find_probe  per probe (a point and a normal) the nearest dark-to-light edge along the normal, to a fraction of a
+//                 pixel (the derivative's peak, refined by a parabola)
+// The host (find.cpp) turns peaks into the border's four sides (the solid line's outer and inner edges, a light
+// module outside it and the light gap inside), fits the homography and the lens to the probed edge points, and settles
+// ring and orientation by the timing track.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.hpp"
+
+namespace lz {
+
+constexpr int FIND_THETA = 720;       // direction bins over a full turn
+constexpr int FIND_MAX_PEAKS = 4096;  // local maxima kept
+constexpr int FIND_MAX_PROBES = 1 << 16;
+
+struct FindPeak {
+  float theta, rho, votes, score;  // gradient direction (radians), rho = n . (p - centre) in pixels
+};
+struct FindProbe {
+  float x, y, nx, ny;  // the point and the unit normal, pointing from dark to light
+};
+struct FindEdge {
+  float x, y, w, off;  // the edge's point, its gradient (grey levels a pixel; 0 none), its offset along the normal
+};
+
+void launch_find_vote(const uint8_t *img, int w, int h, int stride, float thr, const float2 *trig, int nr, float *acc,
+                      hipStream_t s);
+void launch_find_peaks(const float *acc, int nr, float frac, unsigned *d_max, FindPeak *peaks, int *count,
+                       hipStream_t s);
+void launch_find_probe(const uint8_t *img, int w, int h, int stride, const FindProbe *probes, int n, float reach,
+                       FindEdge *out, hipStream_t s);
+void launch_find_zero(float *acc, size_t count, unsigned *d_max, int *count_out, hipStream_t s);
+
+// What the finder settles on: H maps module coordinates (0 .. span + 30) to undistorted capture pixels (h8 = 1), k1
+// the radial lens about the frame's centre (q over the farthest corner's distance); quad the border line's outer
+// corners (module (1, 1), (L - 1, 1), (L - 1, L - 1), (1, L - 1)) in the capture, lens included.
+struct FindResult {
+  int found = 0, span = 0;
+  double H[9] = {};
+  double k1 = 0;
+  float quad[8] = {};
+  float track = 0, contrast = 0, rms = 0;  // the track's agreement, the border's contrast, the fit's rms (modules)
+  int points = 0;                          // edge points the final fit used
+  float ms[6] = {};  // host clock: upload, vote + peaks, sides, first probes, rings, refinement
+};
+
+struct FinderBufs {
+  DevBuf acc, trig, max, peaks, count, probes, edges, img;
+  int trig_nr = 0;
+};
+
+// The whole search on a capture already on the device (d_img, rows w bytes apart) with its host copy (img, rows
+// stride bytes apart) for the host's reads.
+void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w, int h, int stride, hipStream_t s,
+                 FindResult &r);
+
+}  // namespace lz

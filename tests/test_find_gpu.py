@@ -1,0 +1,154 @@
+"""The finder (lizx_find; liz_decode with no registration held) against the truth of the channel that made each
+capture and against the reference's own finder (lizard_acquire, compiled from its sources) on the same frames: the
+ring and orientation, the registration at the node lattice (as close to the truth as the reference's or closer),
+the lens, and the blind decode (as many blocks as the reference's finder and decoder verify); frames with no symbol
+are not found; a padded capture gives what the tight one does."""
+import numpy as np
+import pytest
+
+import lizard_amd as L
+from lizard_amd import sim
+from _libs import ptr, ref
+from pictures import Capture
+
+pytestmark = pytest.mark.gpu
+
+
+def node_err(H, nodes, Ht, nodes_t, node_x):
+    X, Y = np.meshgrid(node_x, node_x)
+    P = np.stack([X.ravel(), Y.ravel(), np.ones(X.size)])
+
+    def at(Hm, nd):
+        q = np.asarray(Hm, np.float64).reshape(3, 3) @ P
+        return q[:2] / q[2] + np.asarray(nd, np.float64).reshape(-1, 2).T
+
+    e = np.hypot(*(at(H, nodes) - at(Ht, nodes_t)))
+    return float(e.mean()), float(e.max())
+
+
+def reference_path(version, span, img):
+    """The reference's finder and decoder on a capture: (registration error inputs, verified blocks)."""
+    r = ref("x86")
+    info = L.format_info(version)
+    f = r.ref_format_new(info["n"], info["subch"], span)
+    nx = len(L.node_lattice(span))
+    H = np.zeros(9, np.float32)
+    nodes = np.zeros(2 * nx * nx, np.float32)
+    quad = np.zeros(8, np.float32)
+    o = np.zeros(1, np.int32)
+    h, w = img.shape
+    got = r.ref_acquire(f, ptr(img), w, h, ptr(H), ptr(nodes), ptr(quad), ptr(o))
+    verified = 0
+    if got:
+        B = info["blocks"]
+        blocks = np.zeros(B * 476, np.uint8)
+        ok = np.zeros(B, np.uint8)
+        its = np.zeros(B, np.int8)
+        est = np.zeros(B, np.float32)
+        r.ref_decode_reg(f, ptr(img), w, h, ptr(H), ptr(nodes), ptr(blocks), ptr(ok), ptr(its), ptr(est), None, None)
+        verified = int(ok.sum())
+    r.ref_format_free(f)
+    return (H, nodes) if got else None, verified
+
+
+PAINTED = [(5, 128, 1280, 720, 5.0), (20, 256, 1920, 1080, -8.0), (64, 512, 1920, 1080, 12.0),
+           (20, 256, 1920, 1080, 100.0), (8, 256, 1280, 720, 190.0), (45, 512, 1920, 1080, 275.0)]
+
+
+@pytest.mark.parametrize("version,span,W,Hh,rot", PAINTED)
+def test_finder_on_reference_painted_symbols(decoder_gpu, version, span, W, Hh, rot):
+    C = Capture(version, span, W=W, Hh=Hh, rot_deg=rot, scale=0.85, blur=0.8, noise=4.0, seed=version)
+    img = np.ascontiguousarray(C.img)
+    f = decoder_gpu.find(img)
+    assert f is not None and f["span"] == span
+    mean, mx = node_err(f["H"], f["nodes"], C.H, C.nodes, C.node_x)
+    assert mean < 0.1 and mx < 0.3, (mean, mx)
+    assert abs(f["k1"]) < 0.003 and f["track"] > 0.8
+    reg, ref_verified = reference_path(version, span, img)
+    assert reg is not None
+    ref_mean, _ = node_err(reg[0], reg[1], C.H, C.nodes, C.node_x)
+    assert mean <= max(ref_mean, 0.04) + 0.02, (mean, ref_mean)
+    # blind: the finder, the word, the decode
+    dec = L.Decoder(1536)
+    k, blocks, out, held = dec.decode(img)
+    dec.close()
+    assert out.found == 1 and out.word == 1 and out.blocks == version and held == version
+    assert out.ring == {128: 0, 256: 1, 512: 2}[span]
+    assert k >= ref_verified - 1, (k, ref_verified)
+    truth = decoder_gpu.decode_reg(version, span, img, C.H, C.nodes, soft=False)
+    assert k >= truth["good"] - 1
+
+
+LENS = [(64, 512, 4.0, 0.04, 0.8, 4.0, 0.85), (20, 256, 95.0, 0.03, 1.0, 5.0, 0.8),
+        (45, 512, 181.0, -0.02, 0.6, 3.0, 0.9), (64, 512, -7.0, -0.05, 1.0, 4.0, 0.8)]
+
+
+@pytest.mark.parametrize("version,span,rot,k1,blur,noise,scale", LENS)
+def test_finder_recovers_the_lens(decoder_gpu, version, span, rot, k1, blur, noise, scale):
+    """Symbols painted on the GPU behind a radial lens: k1 and the registration from the border alone."""
+    rng = np.random.default_rng(version)
+    blocks = sim.random_blocks(version, rng)
+    dpic = decoder_gpu.modulate(version, blocks)
+    cam = sim.Camera(1920, 1080, rot_deg=rot, scale=scale, persp=0.03, k1=k1, blur=blur, noise=noise)
+    Hm = cam.H(span)
+    dimg = decoder_gpu.capture(version, span, dpic, 1920, 1080, Hm, k1=k1, blur=blur, noise=noise, seed=7, fps=30)
+    img = dimg.get(np.uint8, 1920 * 1080).reshape(1080, 1920)
+    dimg.free()
+    dpic.free()
+    f = decoder_gpu.find(img)
+    assert f is not None and f["span"] == span
+    assert abs(f["k1"] - k1) < 0.006, f["k1"]
+    Ht = (Hm / Hm[2, 2]).astype(np.float32).ravel()
+    mean, mx = node_err(f["H"], f["nodes"], Ht, sim.nodes_for(cam, span, Hm), L.node_lattice(span))
+    assert mean < 0.12 and mx < 0.4, (mean, mx)
+    dec = L.Decoder(1536)
+    k, got, out, held = dec.decode(img)
+    dec.close()
+    assert out.word == 1 and out.blocks == version and out.fps == 30
+    truth = decoder_gpu.decode_reg(version, span, img, Ht, sim.nodes_for(cam, span, Hm), soft=False)
+    assert k >= truth["good"] - 2, (k, truth["good"])
+
+
+def test_finder_padded_rows_match_tight(decoder_gpu):
+    C = Capture(20, 256, W=1280, Hh=720, rot_deg=-3.0, scale=0.9, blur=0.7, noise=3.0, seed=2)
+    tight = np.ascontiguousarray(C.img)
+    padded = np.zeros((720, 1280 + 96), np.uint8)
+    padded[:, :1280] = tight
+    a = decoder_gpu.find(tight)
+    b = decoder_gpu.find(padded[:, :1280])  # rows 1376 bytes apart
+    assert a is not None and b is not None
+    assert np.array_equal(a["H"], b["H"]) and a["k1"] == b["k1"] and np.array_equal(a["nodes"], b["nodes"])
+
+
+def test_frames_without_a_symbol_are_not_found(decoder_gpu):
+    rng = np.random.default_rng(0)
+    noise = rng.integers(0, 256, (720, 1280)).astype(np.uint8)
+    assert decoder_gpu.find(noise) is None
+    flat = np.full((720, 1280), 200, np.uint8)
+    assert decoder_gpu.find(flat) is None
+    # a picture with no border round it (grey ring): nothing to find
+    blocks = sim.random_blocks(5, rng)
+    img, H, nodes = sim.capture(sim.picture(5, blocks), 128, sim.Camera(1280, 720, blur=0.7, noise=3.0), seed=1)
+    assert decoder_gpu.find(img.numpy()) is None
+    dec = L.Decoder(384)
+    k, got, out, held = dec.decode(noise)
+    assert k == 0 and out.found == 0 and out.ring == -1 and held == 0
+    k, got, out, held = dec.decode(np.zeros((8, 8), np.uint8))
+    assert k == 0 and out.found == 0
+    dec.close()
+
+
+def test_hold_then_release(decoder_gpu):
+    """A held registration is used while held (even a wrong one); released, each frame's own symbol is found."""
+    C = Capture(5, 128, seed=3, blur=0.6, noise=2.0)
+    img = np.ascontiguousarray(C.img)
+    dec = L.Decoder(384)
+    wrong = C.H.copy()
+    wrong[2] += 40.0  # shifted 40 pixels
+    dec.hold(0, 128, wrong, C.nodes)
+    k, got, out, held = dec.decode(img)
+    assert out.found == 1 and k < C.blocks_n
+    dec.release()
+    k, got, out, held = dec.decode(img)
+    dec.close()
+    assert out.found == 1 and out.word == 1 and k == C.blocks_n and np.array_equal(got.ravel(), C.blocks)

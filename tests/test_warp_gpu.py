@@ -103,11 +103,13 @@ def test_liz_decode_held(fmt):
     assert 0 <= out.pilot_sd[0] < 0.2 and 0 <= out.pilot_sd[1] < 0.2
 
 
-def test_liz_decode_without_registration_is_state_error():
+def test_liz_decode_without_registration_finds_nothing_in_a_blank_frame():
+    """No registration held: liz_decode looks for the symbol itself (lizx_find); a blank frame has none."""
     dec = lizard_amd.Decoder(384)
-    with pytest.raises(lizard_amd.LizError) as e:
-        dec.decode(np.zeros((10, 10), np.uint8))
-    assert e.value.code == lizard_amd.LIZ_E_STATE
+    k, blocks, out, held = dec.decode(np.zeros((10, 10), np.uint8))
+    assert k == 0 and out.found == 0 and out.ring == -1 and held == 0
+    k, blocks, out, held = dec.decode(np.full((200, 300), 255, np.uint8), held=7)
+    assert k == 0 and out.found == 0 and held == 7
     dec.close()
 
 
