@@ -573,6 +573,125 @@ def bench_small(args):
     return res
 
 
+def finder_frames(count, seed):
+    """The full-pipeline workload's channel (1920x1080, LIZARD-1024 in the 256 ring: rotation +-10 deg, scale 0.6-0.9
+    of the height, perspective <= 0.05, k1 +-0.05, blur 0.5-1.5 px, AWGN 2-8 grey) on the whole symbol as a page
+    shows it (border, word, anchors, guard; lizx_capture_symbol), pictures and captures made on the GPU; with each
+    capture's true registration."""
+    from lizard_amd import sim
+    rng = np.random.default_rng(seed)
+    dec = lizard_amd.Decoder(0)
+    version, span = 64, 512
+    out = []
+    for i in range(count):
+        blocks = sim.random_blocks(version, rng)
+        cam = sim.Camera(1920, 1080, rot_deg=float(rng.uniform(-10, 10)), scale=float(rng.uniform(0.6, 0.9)),
+                         persp=float(rng.uniform(0, 0.05)), k1=float(rng.uniform(-0.05, 0.05)),
+                         blur=float(rng.uniform(0.5, 1.5)), noise=float(rng.uniform(2, 8)))
+        Hm = cam.H(span)
+        dpic = dec.modulate(version, blocks)
+        dimg = dec.capture(version, span, dpic, cam.W, cam.Hh, Hm, k1=cam.k1, blur=cam.blur, noise=cam.noise, seed=i,
+                           fps=60)
+        img = dimg.get(np.uint8, cam.W * cam.Hh).reshape(cam.Hh, cam.W)
+        dimg.free()
+        dpic.free()
+        out.append(dict(img=img, H=(Hm / Hm[2, 2]).astype(np.float32).ravel(), nodes=sim.nodes_for(cam, span, Hm),
+                        k1=cam.k1, blocks=blocks))
+    dec.close()
+    return out, version, span
+
+
+def bench_finder(args):
+    """The finder (lizx_find) and the blind decode (liz_decode with nothing held: find, word, decode) on whole-symbol
+    captures, against the true registration's decode and the reference's own finder (lizard_acquire) on the host."""
+    frames, version, span = finder_frames(16, 1234)
+    node_x = lizard_amd.node_lattice(span)
+    X, Y = np.meshgrid(node_x, node_x)
+    P = np.stack([X.ravel(), Y.ravel(), np.ones(X.size)])
+
+    def place(Hm, nd):
+        q = np.asarray(Hm, np.float64).reshape(3, 3) @ P
+        return q[:2] / q[2] + np.asarray(nd, np.float64).reshape(-1, 2).T
+
+    dec = lizard_amd.Decoder(0)
+    for f in frames[:2]:
+        dec.find(f["img"])
+    found, errs, k1err, stages = 0, [], [], []
+    t0 = time.perf_counter()
+    res = [dec.find(f["img"]) for f in frames]
+    find_ms = (time.perf_counter() - t0) * 1e3 / len(frames)
+    for f, r in zip(frames, res):
+        if r is None or r["span"] != span:
+            continue
+        found += 1
+        errs.append(float(np.hypot(*(place(r["H"], r["nodes"]) - place(f["H"], f["nodes"]))).mean()))
+        k1err.append(abs(r["k1"] - f["k1"]))
+        stages.append(r["ms"])
+    # blind decode against the decode through the true registration
+    blind = lizard_amd.Decoder(0)
+    t0 = time.perf_counter()
+    kb = 0
+    for f in frames:
+        k, _, out, _ = blind.decode(f["img"])
+        kb += k
+    blind_ms = (time.perf_counter() - t0) * 1e3 / len(frames)
+    blind.close()
+    held = lizard_amd.Decoder(0)
+    t0 = time.perf_counter()
+    kt = 0
+    for f in frames:
+        held.hold(version, span, f["H"], f["nodes"])
+        k, _, _, _ = held.decode(f["img"])
+        kt += k
+    held_ms = (time.perf_counter() - t0) * 1e3 / len(frames)
+    held.close()
+    dec.close()
+    st = np.mean(np.array(stages), axis=0) if stages else np.zeros(6)
+    out = {"workload": "16 captures 1920x1080 of LIZARD-1024 (n=1536, 256 ring) painted whole (border, word, anchors, "
+                       "guard) on the GPU: rotation +-10 deg, scale 0.6-0.9, perspective <= 0.05, k1 +-0.05, blur "
+                       "0.5-1.5 px, AWGN 2-8 grey",
+           "found": found, "frames": len(frames), "find_ms_per_frame": round(find_ms, 3),
+           "find_stage_ms": {k: round(float(v), 3) for k, v in zip(("upload", "vote_peaks", "sides", "probes",
+                                                                       "rings", "refine"), st)},
+           "node_err_px_mean": round(float(np.mean(errs)), 4) if errs else None,
+           "node_err_px_worst_frame": round(float(np.max(errs)), 4) if errs else None,
+           "k1_err_max": round(float(np.max(k1err)), 5) if k1err else None,
+           "blind_decode_ms_per_frame": round(blind_ms, 3), "blind_verified": int(kb),
+           "held_truth_decode_ms_per_frame": round(held_ms, 3), "held_truth_verified": int(kt),
+           "blocks": 124 * len(frames)}
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+    if os.path.exists(ref_so) and not args.no_cpu:
+        lib = C.CDLL(ref_so)
+        lib.ref_format_new.restype = C.c_void_p
+        lib.ref_format_new.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.ref_format_free.argtypes = [C.c_void_p]
+        fn = lib.ref_acquire
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int] + [C.c_void_p] * 4
+        fmt = lib.ref_format_new(1536, 1024, span)
+        nx = len(node_x)
+        H = np.zeros(9, np.float32)
+        nodes = np.zeros(2 * nx * nx, np.float32)
+        quad = np.zeros(8, np.float32)
+        o = np.zeros(1, np.int32)
+        rfound, rerr = 0, []
+        t0 = time.perf_counter()
+        for f in frames:
+            img = f["img"]
+            if fn(fmt, img.ctypes.data, img.shape[1], img.shape[0], H.ctypes.data, nodes.ctypes.data,
+                  quad.ctypes.data, o.ctypes.data):
+                rfound += 1
+                rerr.append(float(np.hypot(*(place(H, nodes) - place(f["H"], f["nodes"]))).mean()))
+        cpu_ms = (time.perf_counter() - t0) * 1e3 / len(frames)
+        lib.ref_format_free(fmt)
+        out["cpu_baseline"] = {"value": round(cpu_ms, 3), "unit": "ms/frame", "cores": 1, "kind": "reference",
+                               "found": rfound,
+                               "node_err_px_mean": round(float(np.mean(rerr)), 4) if rerr else None,
+                               "sample": "the same 16 captures, one thread: liblizard lizard_acquire (the thin frame's "
+                                         "finder, corner nodes, anchors, border mesh; x86-64 SIMD build)"}
+    return out
+
+
 def bench_refine(args):
     """Decision-directed registration refinement: decode through the perturbed registrations (one batch), re-modulate
     the CRC-passed blocks, 10 Gauss-Newton steps a frame on the GPU, decode again."""
@@ -706,7 +825,8 @@ def main():
     ap.add_argument("--fft-frames", type=int, default=64)
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--refine-frames", type=int, default=256)
-    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small", "sender"), default=None,
+    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small", "sender", "finder"),
+                    default=None,
                     help="run one workload alone (for per-kernel profiles); the JSON line then carries only it")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -717,10 +837,11 @@ def main():
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
 
-    if args.only in ("pipeline", "fft_demap", "refine", "small", "sender"):
+    if args.only in ("pipeline", "fft_demap", "refine", "small", "sender", "finder"):
         out = (bench_pipeline(args, pipeline_frames(args.pipe_frames, 4242)) if args.only == "pipeline"
                else bench_fft_demap(args) if args.only == "fft_demap" else bench_refine(args) if args.only == "refine"
-               else bench_sender(args, lizard_amd.Decoder(0)) if args.only == "sender" else bench_small(args))
+               else bench_sender(args, lizard_amd.Decoder(0)) if args.only == "sender"
+               else bench_finder(args) if args.only == "finder" else bench_small(args))
         print(json.dumps({"only": args.only, args.only: out}))
         return
     if args.only == "ldpc":
@@ -798,6 +919,33 @@ def main():
     tops = ops / (launch_ms * 1e-3) / 1e12
     hbm_bytes = float(sum(b[1].size for b in batches) / len(batches) + count * (476 + 2))
 
+    # the PHY's own setting beside it: 30 iterations with the stall rule (9, 0.95), CRC verdicts per point
+    def step30(s):
+        d_rt, d_llr, d_off = dev[s % len(dev)]
+        dec.decode_device(count, d_rt, d_llr, d_off, None, d_its, max_iter=30, stall_it=9, stall_ratio=0.95, crc=True,
+                          d_bytes=d_bytes, d_crc_ok=d_ok)
+
+    ok30, outcome30 = [], []
+    for p in range(len(EBN0)):
+        step30(p)
+        dec.sync()
+        ok30.append(int(d_ok.get(np.uint8, count).sum()))
+        it = d_its.get(np.int8, count).astype(np.int64)
+        conv = it[it > 0]
+        outcome30.append({"its_mean_converged": round(float(conv.mean()), 3) if conv.size else None,
+                          "capped": int((it == -1).sum()), "stalled": int((it == -3).sum())})
+    for s in range(2):
+        step30(s)
+    dec.sync()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step30(s)
+    dec.sync()
+    w30 = time.perf_counter() - t0
+    stall = {"max_iter": 30, "stall_it": 9, "stall_ratio": 0.95,
+             "Gbps": round(3840.0 * count * args.steps / w30 / 1e9, 3),
+             "verified_by_point": ok30, "by_point": outcome30}
+
     plain = None
     if not args.no_plain:
         t0 = time.perf_counter()
@@ -817,9 +965,10 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(batches, args.cpu_seconds)
 
-    pipe = fft = refine = small = sender = None
+    pipe = fft = refine = small = sender = finder = None
     if not args.no_pipeline:
         small = bench_small(args)
+        finder = bench_finder(args)
         sdec = lizard_amd.Decoder(0)
         sender = bench_sender(args, sdec)
         sdec.close()
@@ -844,6 +993,7 @@ def main():
                    "codewords": count, "ebn0_db": list(EBN0), "max_iter": MAX_ITER},
         "payload_MBps": round(payload_mbps, 2),
         "autotune": {"ldpc_wg_per_cu": tuned_per_cu or "max"},
+        "stall_rule": stall,
         "plain_Gbps": round(plain, 4) if plain else None,
         "batched_over_plain": round(gbps / plain, 2) if plain else None,
         "points": points,
@@ -863,6 +1013,7 @@ def main():
         "refine": refine,
         "small": small,
         "sender": sender,
+        "finder": finder,
     }
     print(json.dumps(line))
 

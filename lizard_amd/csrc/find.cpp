@@ -164,10 +164,22 @@ struct Fit {
     std::vector<int> axis(n);
     std::vector<double> val(n);
     for (int k = 0; k < n; k++) edge_line(L, pts[k].side, pts[k].depth, &axis[k], &val[k]);
+    std::vector<double> px(n), py(n);  // undistorted places, each round starting from the last
+    for (int k = 0; k < n; k++) {
+      px[k] = pts[k].x;
+      py[k] = pts[k].y;
+    }
+    int iters = 8;
     auto place = [&]() {
       for (int k = 0; k < n; k++) {
-        double x, y;
-        lens.undistort(pts[k].x, pts[k].y, &x, &y);
+        double x = px[k], y = py[k];
+        for (int it = 0; it < iters; it++) {
+          const double qx = (x - lens.ax) / lens.D, qy = (y - lens.ay) / lens.D, r2 = qx * qx + qy * qy;
+          x = pts[k].x - lens.k1 * r2 * qx * lens.D;
+          y = pts[k].y - lens.k1 * r2 * qy * lens.D;
+        }
+        px[k] = x;
+        py[k] = y;
         const double qx = (x - lens.ax) / lens.D, qy = (y - lens.ay) / lens.D, r2 = qx * qx + qy * qy;
         ux[k] = (x - cx) / s;
         uy[k] = (y - cy) / s;
@@ -194,6 +206,7 @@ struct Fit {
       if (m < 16 || !solve8(A, b, g)) return 1e9;
     }
     double rms = 1e9;
+    iters = 3;
     for (int round = 0; round < rounds; round++) {
       if (round) place();
       double A[9][9] = {}, b[9] = {};
@@ -583,11 +596,23 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
   float bestCon = 0;
   std::vector<Pt> sub;  // every fourth point for the search
   for (size_t k = 0; k < pts.size(); k += 4) sub.push_back(pts[k]);
-  for (int ring : RINGS) {
-    const int L = ring + 30;
+  // rings in the order the line's thickness (2 modules) against the side's length (L - 4 modules between the
+  // midlines) suggests
+  int order[3] = {0, 1, 2};
+  {
+    const double side = 0.25 * (std::hypot(Cx[1] - Cx[0], Cy[1] - Cy[0]) + std::hypot(Cx[2] - Cx[1], Cy[2] - Cy[1]) +
+                                std::hypot(Cx[3] - Cx[2], Cy[3] - Cy[2]) + std::hypot(Cx[0] - Cx[3], Cy[0] - Cy[3]));
+    const double modules = thick > 0 ? 2.0 * side / thick + 4.0 : 0.0;
+    std::sort(order, order + 3, [&](int a, int c) {
+      return std::fabs(std::log((RINGS[a] + 30) / modules)) < std::fabs(std::log((RINGS[c] + 30) / modules));
+    });
+  }
+  for (int oi = 0; oi < 3; oi++) {
+    if (bestTrack >= 0.7) break;  // a clear track under the likelier ring
+    const int L = RINGS[order[oi]] + 30;
     Lens ln = lens;
     double H[9];
-    const double rms = fit.solve(sub, ln, L, 1.0, 10, H, &ln.k1, nullptr);
+    const double rms = fit.solve(sub, ln, L, 1.0, 6, H, &ln.k1, nullptr);
     if (!(rms < 0.5)) continue;
     // the four rotations about the symbol's centre: module (x, y) -> (L - y, x) a quarter turn
     double Rq[9] = {0, -1, static_cast<double>(L), 1, 0, 0, 0, 0, 1}, Rk[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -613,10 +638,10 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
   if (bestTrack < 0.4) return;
   const int L = bestL;
 
-  // refine: probes from the model at every module along both edges of every side, and the fit again, three times
+  // refine: probes from the model at every module along both edges of every side, and the fit again, twice
   lens.k1 = bestk1;
   std::memcpy(r.H, bestH, sizeof bestH);
-  for (int pass = 0; pass < 3; pass++) {
+  for (int pass = 0; pass < 2; pass++) {
     pr.clear();
     std::vector<Pt> lab;
     const double mod = std::hypot(r.H[0], r.H[3]);  // pixels a module near the origin, for the reach
@@ -645,7 +670,7 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
       if (ed[k].w > 0) pts.push_back(Pt{ed[k].x, ed[k].y, ed[k].w, lab[k].side, lab[k].depth});
     double H[9], k1 = lens.k1;
     int used = 0;
-    const double rms = fit.solve(pts, lens, L, 0.3, 8, H, &k1, &used);
+    const double rms = fit.solve(pts, lens, L, 0.3, 5, H, &k1, &used);
     if (!(rms < 0.3)) return;
     std::memcpy(r.H, H, sizeof H);
     lens.k1 = k1;
