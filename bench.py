@@ -53,6 +53,8 @@ LDS_PEAK_GBS = 256 * 128 * 2.4
 HBM_PEAK_GBS = 8000.0
 # Vector integer issue: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz, two 16-bit operations a lane (packed) a clock.
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 * 2 / 1e12
+# 32-bit integer vector operations: 256 CUs x 4 SIMDs x 16 lanes a clock at 2.4 GHz (no packed form for 32-bit ints)
+VALU32_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
 # Integer operations of the reference's scalar layer kernel per edge per iteration (liblizard/src/ldpc.c:171-191):
 # subtract, magnitude, sign, first- and second-minimum updates; magnitude select, sign, add, clamp (two).
 OPS_PER_EDGE = 10
@@ -692,6 +694,70 @@ def bench_finder(args):
     return out
 
 
+B3_OPS_PER_COMPRESSION = 7 * 8 * 14 + 8  # 7 rounds of 8 G (6 adds, 4 xors, 4 rotates), the final 8 xors
+
+
+def bench_blake3(args):
+    """The transfer's chunk check on the GPU: BLAKE3 chaining values of every 1 MiB chunk of a 1 GiB file resident in
+    HBM (lizx_b3_chunk_cvs), and the whole file's hash; the reference's BLAKE3 (xfer_chunk_cv over the vendored
+    official C with its AVX2 kernel) on the host's cores beside it."""
+    length, k = 1 << 30, 20
+    rng = np.random.default_rng(33)
+    data = rng.integers(0, 256, length, dtype=np.uint8)
+    d = lizard_amd.DeviceArray.of(data)
+    chunks = length >> k
+    dc = lizard_amd.DeviceArray(32 * chunks)
+    dec = lizard_amd.Decoder(0)
+    for _ in range(2):
+        dec.b3_chunk_cvs(d, length, k, d_cvs=dc)
+    ms = []
+    for _ in range(5):
+        _, t = dec.b3_chunk_cvs(d, length, k, d_cvs=dc, timed=True)
+        ms.append(t)
+    kms = float(np.median(ms))
+    cvs = dc.get(np.uint8, 32 * chunks).reshape(chunks, 32)
+    root, hms = dec.b3_hash_device(d, length, timed=True)
+    same_root = lizard_amd.b3_root(cvs) == root
+    dec.close()
+    dc.free()
+    d.free()
+    compressions = length // 64 + (length // 1024 - chunks)  # leaves' blocks, then parents under each chunk
+    ops = compressions * B3_OPS_PER_COMPRESSION
+    out = {"workload": "BLAKE3 chaining values of every 1 MiB chunk of a 1 GiB file resident in HBM (the transfer's "
+                       "chunk check, xfer_chunk_cv over the file), then the file's root",
+           "GBps": round(length / (kms * 1e-3) / 1e9, 2), "kernel_ms": round(kms, 4),
+           "hash_kernel_ms": round(hms, 4), "root_from_chunks_equals_hash": bool(same_root),
+           "roofline": {"bound": "valu", "achieved": round(ops / (kms * 1e-3) / 1e12, 2), "peak": VALU32_PEAK_TOPS,
+                        "unit": "TOP/s", "frac": round(ops / (kms * 1e-3) / 1e12 / VALU32_PEAK_TOPS, 4),
+                        "ops_per_compression": B3_OPS_PER_COMPRESSION, "compressions": compressions,
+                        "hbm_GBps": round(length / (kms * 1e-3) / 1e9, 1), "hbm_peak_GBps": HBM_PEAK_GBS}}
+    xfer_so = os.path.join(ROOT, "oracle", "_ref", "libref_xfer.so")
+    if os.path.exists(xfer_so) and not args.no_cpu:
+        import concurrent.futures as cf
+        lib = C.CDLL(xfer_so)
+        fn = lib.xfer_chunk_cv
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_void_p, C.c_size_t, C.c_uint32, C.c_int, C.c_void_p]
+        threads = min(16, os.cpu_count() or 1)
+        sample = min(chunks, 16 * threads)  # chunks of the same file
+        outb = np.zeros((sample, 32), np.uint8)
+
+        def work(i):
+            fn(data.ctypes.data + (i << k), 1 << k, i, k, outb[i].ctypes.data)
+
+        with cf.ThreadPoolExecutor(threads) as ex:
+            list(ex.map(work, range(threads)))
+            t0 = time.perf_counter()
+            list(ex.map(work, range(sample)))
+            wall = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(sample * (1 << k) / wall / 1e9, 3), "unit": "GB/s", "cores": threads,
+                               "kind": "reference", "same_values": bool(np.array_equal(outb, cvs[:sample])),
+                               "sample": "%d of the file's 1 MiB chunks on %d threads (ctypes releases the GIL): "
+                                         "xfer_chunk_cv over the vendored BLAKE3 C with its AVX2 kernel" % (sample,
+                                                                                                          threads)}
+    return out
+
+
 def bench_refine(args):
     """Decision-directed registration refinement: decode through the perturbed registrations (one batch), re-modulate
     the CRC-passed blocks, 10 Gauss-Newton steps a frame on the GPU, decode again."""
@@ -825,8 +891,8 @@ def main():
     ap.add_argument("--fft-frames", type=int, default=64)
     ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--refine-frames", type=int, default=256)
-    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small", "sender", "finder"),
-                    default=None,
+    ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small", "sender", "finder",
+                                       "blake3"), default=None,
                     help="run one workload alone (for per-kernel profiles); the JSON line then carries only it")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -837,11 +903,12 @@ def main():
         print("bench.py: no GPU", file=sys.stderr)
         sys.exit(1)
 
-    if args.only in ("pipeline", "fft_demap", "refine", "small", "sender", "finder"):
+    if args.only in ("pipeline", "fft_demap", "refine", "small", "sender", "finder", "blake3"):
         out = (bench_pipeline(args, pipeline_frames(args.pipe_frames, 4242)) if args.only == "pipeline"
                else bench_fft_demap(args) if args.only == "fft_demap" else bench_refine(args) if args.only == "refine"
                else bench_sender(args, lizard_amd.Decoder(0)) if args.only == "sender"
-               else bench_finder(args) if args.only == "finder" else bench_small(args))
+               else bench_finder(args) if args.only == "finder" else bench_blake3(args) if args.only == "blake3"
+               else bench_small(args))
         print(json.dumps({"only": args.only, args.only: out}))
         return
     if args.only == "ldpc":
@@ -965,10 +1032,11 @@ def main():
 
     cpu = None if args.no_cpu else cpu_baseline(batches, args.cpu_seconds)
 
-    pipe = fft = refine = small = sender = finder = None
+    pipe = fft = refine = small = sender = finder = blake3 = None
     if not args.no_pipeline:
         small = bench_small(args)
         finder = bench_finder(args)
+        blake3 = bench_blake3(args)
         sdec = lizard_amd.Decoder(0)
         sender = bench_sender(args, sdec)
         sdec.close()
@@ -1014,6 +1082,7 @@ def main():
         "small": small,
         "sender": sender,
         "finder": finder,
+        "blake3": blake3,
     }
     print(json.dumps(line))
 

@@ -228,6 +228,24 @@ LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_
 LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stride, int *span, float *H, float *nodes,
                       float *quad, float *info);
 
+/* ---- the transfer's BLAKE3 tree (a verified file's chunks and root) --------------------------------------------- */
+
+/* Every chunk's chaining value of a file in device memory (16-byte aligned): the file cut into chunks of
+   2^chunk_log2 bytes (10..24, at most 16,383 chunks, the last shorter), each a whole subtree of BLAKE3's tree at its
+   own offset; 32 bytes a chunk into d_cvs (device, 16-byte aligned). Replaces xfer_chunk_cv over every chunk
+   (liblizard/src/xfer.h:175, xfer.c:217-221). kernel_ms (optional): the launches' time on the decoder's stream.
+   Returns the chunk count. */
+LIZ_API int lizx_b3_chunk_cvs(liz_decoder *d, const uint8_t *d_file, uint64_t length, int chunk_log2, uint8_t *d_cvs,
+                              float *kernel_ms);
+/* BLAKE3 (32 bytes, what b3sum prints) of device data (16-byte aligned) on the GPU. Replaces xfer_b3_hash
+   (xfer.h:181) for data already on the device. */
+LIZ_API int lizx_b3_hash_device(liz_decoder *d, const uint8_t *d_data, uint64_t length, uint8_t *out,
+                                float *kernel_ms);
+/* The file's root from its 2..16,383 chunks' chaining values (host). Replaces xfer_root (xfer.h:169). */
+LIZ_API int lizx_b3_root(const uint8_t *cvs, uint32_t chunks, uint8_t *root);
+/* BLAKE3 of host data on the host (xfer_b3_hash, xfer.h:181). */
+LIZ_API int lizx_b3_hash(const uint8_t *data, uint64_t length, uint8_t *out);
+
 /* ---- lizx_ldpc: the LDPC decoder alone, batched ----------------------------------------------------------------- */
 
 typedef struct lizx_ldpc lizx_ldpc;

@@ -90,6 +90,10 @@ def lib():
     sig("lizx_read_word", _I, _I, _P, _I, _I, _I, _P, _P, _P, _P)
     sig("lizx_word_geometry", _I, _I, _P, _P)
     sig("lizx_border_kinds", _I, _I, _I, _I, _P)
+    sig("lizx_b3_chunk_cvs", _I, _P, _P, C.c_uint64, _I, _P, _P)
+    sig("lizx_b3_hash_device", _I, _P, _P, C.c_uint64, _P, _P)
+    sig("lizx_b3_root", _I, _P, C.c_uint32, _P)
+    sig("lizx_b3_hash", _I, _P, C.c_uint64, _P)
     sig("lizx_word_encode", _I, _I, _I, _I, _P)
     sig("lizx_word_decode", _I, _I, _P, _P, _P)
     sig("lizx_format_coefs", _I, _I, _P, _P)
@@ -361,6 +365,23 @@ def word_decode(span, q):
     return (v.value, f.value) if got else None
 
 
+def b3_hash(data):
+    """lizx_b3_hash: BLAKE3 of host bytes (32 bytes)."""
+    data = np.ascontiguousarray(np.frombuffer(bytes(data), np.uint8) if isinstance(data, (bytes, bytearray)) else data,
+                                np.uint8)
+    out = np.zeros(32, np.uint8)
+    check(lib().lizx_b3_hash(_p(data) if data.size else None, data.size, _p(out)))
+    return out.tobytes()
+
+
+def b3_root(cvs):
+    """lizx_b3_root: a file's root from its chunks' chaining values (chunks x 32 bytes)."""
+    cvs = np.ascontiguousarray(cvs, np.uint8).reshape(-1, 32)
+    out = np.zeros(32, np.uint8)
+    check(lib().lizx_b3_root(_p(cvs), cvs.shape[0], _p(out)))
+    return out.tobytes()
+
+
 def border_kinds(span, version=0, fps=0):
     """lizx_border_kinds: the symbol's border as a (span + 30)^2 uint8 module map (0 picture, 1 light, 2 dark, | 8
     word cells)."""
@@ -442,6 +463,26 @@ class Decoder:
         else:
             check(lib().lizx_capture_symbol(self.h, version, span, int(fps), d_picture.ptr, C.byref(cam), d.ptr))
         return d
+
+    def b3_chunk_cvs(self, d_file, length, chunk_log2, d_cvs=None, timed=False):
+        """lizx_b3_chunk_cvs on a DeviceArray file: (chaining values as chunks x 32 uint8, kernel ms if timed); with
+        d_cvs given they stay on the device (None returned for them)."""
+        chunks = (length + (1 << chunk_log2) - 1) >> chunk_log2
+        own = d_cvs is None
+        dc = DeviceArray(max(32 * chunks, 32)) if own else d_cvs
+        ms = C.c_float(0)
+        check(lib().lizx_b3_chunk_cvs(self.h, d_file.ptr, length, chunk_log2, dc.ptr, C.byref(ms)))
+        cvs = dc.get(np.uint8, 32 * chunks).reshape(chunks, 32) if own else None
+        if own:
+            dc.free()
+        return (cvs, ms.value) if timed else cvs
+
+    def b3_hash_device(self, d_data, length, timed=False):
+        """lizx_b3_hash_device: BLAKE3 of a DeviceArray's first `length` bytes (32 bytes; and kernel ms if timed)."""
+        out = np.zeros(32, np.uint8)
+        ms = C.c_float(0)
+        check(lib().lizx_b3_hash_device(self.h, d_data.ptr, length, _p(out), C.byref(ms)))
+        return (out.tobytes(), ms.value) if timed else out.tobytes()
 
     def frame_slots_device(self, version, d_blocks, frames, cnt, d_slots):
         """lizx_frame_slots_device: device blocks (frames x blocks x 476) -> device slots (frames x subch x 640)."""

@@ -15,6 +15,7 @@
 #include "word.hpp"
 #include "register.hpp"
 #include "finder.hpp"
+#include "xfer.hpp"
 #include "spectrum.hpp"
 
 #define LIZARD_AMD_VERSION "lizard-amd 0.1 (gfx950)"
@@ -127,6 +128,7 @@ struct liz_decoder {
   // the sender's pictures and the camera channel
   DevBuf mod_blocks, mod_bits, mod_G, chan_a, chan_b, chan_kinds;
   FinderBufs finder;
+  DevBuf b3_a, b3_b, b3_out;
   std::vector<float> found_nodes;
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
@@ -913,6 +915,74 @@ LIZ_API int lizx_capture_symbol(liz_decoder *d, int version, int span, int fps, 
   return guard([&] {
     need(fps >= 0 && fps <= 255, "fps 0..255");
     return capture_impl(d, version, span, fps, d_picture, cam, d_img);
+  });
+}
+
+LIZ_API int lizx_b3_chunk_cvs(liz_decoder *d, const uint8_t *d_file, uint64_t length, int chunk_log2, uint8_t *d_cvs,
+                              float *kernel_ms) {
+  return guard([&] {
+    need(d && d_file && d_cvs, "a decoder, a device file and device room for the chaining values");
+    need(chunk_log2 >= 10 && chunk_log2 <= 24 && length >= 1, "chunk_log2 10..24 and a file of a byte or more");
+    need((reinterpret_cast<uintptr_t>(d_file) & 15) == 0 && (reinterpret_cast<uintptr_t>(d_cvs) & 15) == 0,
+         "16-byte aligned device buffers");
+    const uint64_t chunks = (length + (1ull << chunk_log2) - 1) >> chunk_log2;
+    need(chunks <= 16383, "at most 16,383 chunks");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const uint64_t words = b3_scratch_words(length);
+    d->b3_a.reserve(words * 4);
+    d->b3_b.reserve(words * 4);
+    hip_check(hipEventRecord(e.ev0, e.stream), "hipEventRecord");
+    b3_chunk_cvs(d_file, length, chunk_log2, d->b3_a.as<uint32_t>(), d->b3_b.as<uint32_t>(),
+                 reinterpret_cast<uint32_t *>(d_cvs), e.stream);
+    hip_check(hipEventRecord(e.ev1, e.stream), "hipEventRecord");
+    e.sync();
+    if (kernel_ms) hip_check(hipEventElapsedTime(kernel_ms, e.ev0, e.ev1), "hipEventElapsedTime");
+    return static_cast<int>(chunks);
+  });
+}
+
+LIZ_API int lizx_b3_hash_device(liz_decoder *d, const uint8_t *d_data, uint64_t length, uint8_t *out,
+                                float *kernel_ms) {
+  return guard([&] {
+    need(d && d_data && out, "a decoder, device data and room for the hash");
+    need((reinterpret_cast<uintptr_t>(d_data) & 15) == 0, "16-byte aligned device data");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const uint64_t words = b3_scratch_words(length);
+    d->b3_a.reserve(words * 4);
+    d->b3_b.reserve(words * 4);
+    d->b3_out.reserve(32);
+    hip_check(hipEventRecord(e.ev0, e.stream), "hipEventRecord");
+    b3_hash_device(d_data, length, d->b3_a.as<uint32_t>(), d->b3_b.as<uint32_t>(), d->b3_out.as<uint32_t>(),
+                   e.stream);
+    hip_check(hipEventRecord(e.ev1, e.stream), "hipEventRecord");
+    e.sync();
+    if (kernel_ms) hip_check(hipEventElapsedTime(kernel_ms, e.ev0, e.ev1), "hipEventElapsedTime");
+    hip_check(hipMemcpy(out, d->b3_out.p, 32, hipMemcpyDeviceToHost), "hash");
+    return 0;
+  });
+}
+
+LIZ_API int lizx_b3_root(const uint8_t *cvs, uint32_t chunks, uint8_t *root) {
+  return guard([&] {
+    need(cvs && root && chunks >= 2 && chunks <= 16383, "2 to 16,383 chaining values and room for the root");
+    std::vector<uint32_t> w(static_cast<size_t>(chunks) * 8);
+    std::memcpy(w.data(), cvs, w.size() * 4);  // little-endian words
+    uint32_t r[8];
+    b3_root(w.data(), chunks, r);
+    std::memcpy(root, r, 32);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_b3_hash(const uint8_t *data, uint64_t length, uint8_t *out) {
+  return guard([&] {
+    need((data || !length) && out, "data and room for the hash");
+    uint32_t r[8];
+    b3_hash_host(data, length, r);
+    std::memcpy(out, r, 32);
+    return 0;
   });
 }
 

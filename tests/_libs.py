@@ -11,6 +11,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 REF_SCALAR_SO = os.path.join(ROOT, "oracle", "_ref", "libref_scalar.so")
 REF_X86_SO = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+REF_XFER_SO = os.path.join(ROOT, "oracle", "_ref", "libref_xfer.so")
 
 P = C.c_void_p
 I32 = C.c_int
@@ -67,11 +68,17 @@ def ref(form="scalar"):
     """The compiled reference driver, or None where oracle/_ref was not built."""
     if form in _refs:
         return _refs[form]
-    path = REF_SCALAR_SO if form == "scalar" else REF_X86_SO
+    path = REF_SCALAR_SO if form == "scalar" else REF_XFER_SO if form == "xfer" else REF_X86_SO
     if not os.path.exists(path):
         _refs[form] = None
         return None
     lib = C.CDLL(path)
+    if form == "xfer":  # the transfer's BLAKE3 tree (liblizard/src/xfer.c) over the vendored BLAKE3
+        _sig(lib, "xfer_b3_hash", None, P, C.c_size_t, P)
+        _sig(lib, "xfer_chunk_cv", I32, P, C.c_size_t, C.c_uint32, I32, P)
+        _sig(lib, "xfer_root", I32, P, C.c_uint32, P)
+        _refs[form] = lib
+        return lib
     _sig(lib, "ref_crc32c", C.c_uint32, P, I32)
     _sig(lib, "ref_ldpc_dims", I32, I32, P)
     _sig(lib, "ref_ldpc_encode", None, I32, P, P)
