@@ -44,23 +44,13 @@ double wrap_pi(double a) {
   return a;
 }
 
-// lens: p_d = p_u + k1 |q|^2 q D, q = (p_u - a) / D
+// lens: p_d = p_u + k1 |q|^2 q D, q = (p_u - a) / D (undone by fixed-point iteration where the fit needs it)
 struct Lens {
   double ax, ay, D, k1;
   void distort(double ux, double uy, double *dx, double *dy) const {
     const double qx = (ux - ax) / D, qy = (uy - ay) / D, r2 = qx * qx + qy * qy;
     *dx = ux + k1 * r2 * qx * D;
     *dy = uy + k1 * r2 * qy * D;
-  }
-  void undistort(double dx, double dy, double *ux, double *uy) const {
-    double x = dx, y = dy;
-    for (int it = 0; it < 8; it++) {
-      const double qx = (x - ax) / D, qy = (y - ay) / D, r2 = qx * qx + qy * qy;
-      x = dx - k1 * r2 * qx * D;
-      y = dy - k1 * r2 * qy * D;
-    }
-    *ux = x;
-    *uy = y;
   }
 };
 
@@ -593,7 +583,6 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
   Lens lens{cx, cy, std::hypot(cx, cy), 0.0};
   double bestH[9] = {}, bestk1 = 0, bestTrack = -1;
   int bestL = 0;
-  float bestCon = 0;
   std::vector<Pt> sub;  // every fourth point for the search
   for (size_t k = 0; k < pts.size(); k += 4) sub.push_back(pts[k]);
   // rings in the order the line's thickness (2 modules) against the side's length (L - 4 modules between the
@@ -627,7 +616,6 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
         std::memcpy(bestH, Hr, sizeof bestH);
         bestk1 = ln.k1;
         bestL = L;
-        bestCon = con;
       }
       double T[9];
       mul3(Rk, Rq, T);
@@ -689,7 +677,6 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
     r.quad[2 * c] = static_cast<float>(x);
     r.quad[2 * c + 1] = static_cast<float>(y);
   }
-  (void)bestCon;
 }
 
 }  // namespace lz

@@ -79,12 +79,15 @@ __global__ __launch_bounds__(256) void k_find_zero(float *acc, size_t count, uns
 }
 
 __global__ __launch_bounds__(256) void k_find_max(const float *acc, size_t count, unsigned *d_max) {
+  __shared__ float wmax[4];
   const size_t stride = static_cast<size_t>(gridDim.x) * 256;
   float v = 0.0f;
   for (size_t i = blockIdx.x * 256 + threadIdx.x; i < count; i += stride) v = fmaxf(v, acc[i]);
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
-  // non-negative floats order as their bit patterns
-  if ((threadIdx.x & 63) == 0) atomicMax(d_max, __float_as_uint(v));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  // one atomic a workgroup; non-negative floats order as their bit patterns
+  if (threadIdx.x == 0) atomicMax(d_max, __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]))));
 }
 
 __global__ __launch_bounds__(256) void k_find_peaks(const float *acc, int nr, float frac, const unsigned *d_max,
@@ -198,7 +201,7 @@ void launch_find_vote(const uint8_t *img, int w, int h, int stride, float thr, c
 void launch_find_peaks(const float *acc, int nr, float frac, unsigned *d_max, FindPeak *peaks, int *count,
                        hipStream_t s) {
   const size_t cells = static_cast<size_t>(FIND_THETA) * nr;
-  hipLaunchKernelGGL(k_find_max, dim3(1024), dim3(256), 0, s, acc, cells, d_max);
+  hipLaunchKernelGGL(k_find_max, dim3(256), dim3(256), 0, s, acc, cells, d_max);
   hipLaunchKernelGGL(k_find_peaks, dim3(static_cast<unsigned>((cells + 255) / 256)), dim3(256), 0, s, acc, nr, frac,
                      d_max, peaks, count);
 }
