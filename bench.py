@@ -685,7 +685,26 @@ def bench_finder(args):
                 rfound += 1
                 rerr.append(float(np.hypot(*(place(H, nodes) - place(f["H"], f["nodes"]))).mean()))
         cpu_ms = (time.perf_counter() - t0) * 1e3 / len(frames)
+        # the reference's whole blind decode of a frame: its finder, then lizard_finish through what it found
+        dr = lib.ref_decode_reg
+        dr.restype = C.c_int
+        dr.argtypes = [C.c_void_p] * 2 + [C.c_int] * 2 + [C.c_void_p] * 8
+        blk = np.zeros(124 * 476, np.uint8)
+        ok, its, be = np.zeros(124, np.uint8), np.zeros(124, np.int8), np.zeros(124, np.float32)
+        rver = 0
+        t0 = time.perf_counter()
+        for f in frames:
+            img = f["img"]
+            if fn(fmt, img.ctypes.data, img.shape[1], img.shape[0], H.ctypes.data, nodes.ctypes.data,
+                  quad.ctypes.data, o.ctypes.data):
+                rver += dr(fmt, img.ctypes.data, img.shape[1], img.shape[0], H.ctypes.data, nodes.ctypes.data,
+                           blk.ctypes.data, ok.ctypes.data, its.ctypes.data, be.ctypes.data, None, None)
+        blind_cpu_ms = (time.perf_counter() - t0) * 1e3 / len(frames)
         lib.ref_format_free(fmt)
+        out["reference_blind_decode"] = {"ms_per_frame": round(blind_cpu_ms, 3), "verified": int(rver), "cores": 1,
+                                         "what": "lizard_acquire then lizard_finish through its registration, one "
+                                                 "thread (x86-64 SIMD build)"}
+        out["blind_decode_over_reference"] = round(blind_cpu_ms / blind_ms, 2)
         out["cpu_baseline"] = {"value": round(cpu_ms, 3), "unit": "ms/frame", "cores": 1, "kind": "reference",
                                "found": rfound,
                                "node_err_px_mean": round(float(np.mean(rerr)), 4) if rerr else None,

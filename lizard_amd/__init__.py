@@ -511,7 +511,9 @@ class Decoder:
         track, contrast, rms, points, ms (the stages' host milliseconds). Rows may be padded (a view whose rows are
         contiguous is passed as it is, with its row stride)."""
         img = np.asarray(img, np.uint8)
-        if img.ndim != 2 or img.strides[1] != 1 or img.strides[0] < img.shape[1]:
+        if img.ndim != 2:
+            raise ValueError("find takes a GREY8 image (h x w)")
+        if img.strides[1] != 1 or img.strides[0] < img.shape[1]:
             img = np.ascontiguousarray(img)
         h, w = img.shape[:2]
         nodes = np.zeros((40, 40, 2), np.float32)

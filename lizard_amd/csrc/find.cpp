@@ -396,7 +396,8 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
       tr[2 * t + 1] = static_cast<float>(std::sin(2 * PI * t / FIND_THETA));
     }
     b.trig.reserve(tr.size() * sizeof(float));
-    hip_check(hipMemcpyAsync(b.trig.p, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice, s), "trig");
+    hip_check(hipStreamSynchronize(s), "finder stream");
+    hip_check(hipMemcpy(b.trig.p, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice), "trig");
     b.trig_nr = 1;
   }
   launch_find_zero(b.acc.as<float>(), cells, b.max.as<unsigned>(), b.count.as<int>(), s);
