@@ -58,6 +58,36 @@ typedef struct {
   float pilot_sd[2];
 } liz_decoded;
 
+/* lizard.h:64-70 and :87-98, field for field: a format (blocks: its size 4..64, the version its word names; ring 0..2
+   or LIZ_RING_DEFAULT; fps 1..255; codes 1 or 2 symbols side by side) and what it paints. */
+enum { LIZ_RINGS = 3, LIZ_RING_DEFAULT = -1, LIZ_GAP_MODULES = 12 };
+typedef struct {
+  int blocks, ring, fps, codes;
+} liz_format;
+typedef struct {
+  int n, span, pxm, side, width, height, gap, frame_blocks;
+} liz_geometry;
+
+/* Replace lizard.h:102-112 (api/format.cpp:28-87): the cells a side of a ring, a format's geometry, the display room a
+   size needs, the blocks a size carries, the largest size that fits a room. Host only. */
+LIZ_API int liz_ring_cells(int ring);
+LIZ_API int liz_geometry_of(const liz_format *format, liz_geometry *out);
+LIZ_API double liz_room_for(int blocks, int ring);
+LIZ_API int liz_blocks_for(int blocks);
+LIZ_API int liz_pick(double w, double h, int codes, int ring, int top_blocks);
+
+/* Replace lizard.h:116-123 (api/codec.cpp:47-99): the per-frame encoder on the GPU. paint: blocks (frame_blocks x 476
+   bytes), picture (its count mod 4 the pilots'), out width x height pixels of fmt, rows stride bytes apart, grey on
+   white margins: the bit side, the picture's inverse 2D DFT, the Lanczos-3 resample of the periodic picture into the
+   page with its guard interval, the anchors, the border and its word, the quiet zone (lizard_phy.c:970-1236, :672-716),
+   each code's symbol in its place with the gap after it. */
+typedef struct liz_encoder liz_encoder;
+LIZ_API liz_encoder *liz_encoder_new(const liz_format *format);
+LIZ_API void liz_encoder_free(liz_encoder *e);
+LIZ_API int liz_encoder_geometry(const liz_encoder *e, liz_geometry *out);
+LIZ_API int liz_encoder_paint(liz_encoder *e, const uint8_t *blocks, uint32_t picture, uint8_t *out, int stride,
+                              liz_pixfmt fmt);
+
 typedef struct liz_decoder liz_decoder;
 
 /* Replaces lizard.h:17 liz_version. */

@@ -39,6 +39,15 @@ class _Decoded(C.Structure):
                 ("pilot_sd", C.c_float * 2)]
 
 
+class Format(C.Structure):
+    """liz_format: blocks (the size, 4..64), ring (0..2, or -1 the default), fps (1..255), codes (1 or 2)."""
+    _fields_ = [("blocks", C.c_int), ("ring", C.c_int), ("fps", C.c_int), ("codes", C.c_int)]
+
+
+class Geometry(C.Structure):
+    _fields_ = [(k, C.c_int) for k in ("n", "span", "pxm", "side", "width", "height", "gap", "frame_blocks")]
+
+
 class Registration(C.Structure):
     _fields_ = [("H", C.c_float * 9), ("nodes", C.c_void_p)]
 
@@ -90,6 +99,15 @@ def lib():
     sig("lizx_read_word", _I, _I, _P, _I, _I, _I, _P, _P, _P, _P)
     sig("lizx_word_geometry", _I, _I, _P, _P)
     sig("lizx_border_kinds", _I, _I, _I, _I, _P)
+    sig("liz_ring_cells", _I, _I)
+    sig("liz_geometry_of", _I, C.POINTER(Format), C.POINTER(Geometry))
+    sig("liz_room_for", C.c_double, _I, _I)
+    sig("liz_blocks_for", _I, _I)
+    sig("liz_pick", _I, C.c_double, C.c_double, _I, _I, _I)
+    sig("liz_encoder_new", _P, C.POINTER(Format))
+    sig("liz_encoder_free", None, _P)
+    sig("liz_encoder_geometry", _I, _P, C.POINTER(Geometry))
+    sig("liz_encoder_paint", _I, _P, _P, C.c_uint32, _P, _I, _I)
     sig("lizx_b3_chunk_cvs", _I, _P, _P, C.c_uint64, _I, _P, _P)
     sig("lizx_b3_hash_device", _I, _P, _P, C.c_uint64, _P, _P)
     sig("lizx_b3_root", _I, _P, C.c_uint32, _P)
@@ -363,6 +381,58 @@ def word_decode(span, q):
     v, f = C.c_int(0), C.c_int(0)
     got = check(lib().lizx_word_decode(span, _p(q), C.byref(v), C.byref(f)))
     return (v.value, f.value) if got else None
+
+
+def ring_cells(ring):
+    return check(lib().liz_ring_cells(ring))
+
+
+def geometry_of(blocks, ring=-1, fps=60, codes=1):
+    """liz_geometry_of as a dict."""
+    g = Geometry()
+    check(lib().liz_geometry_of(C.byref(Format(blocks, ring, fps, codes)), C.byref(g)))
+    return {k: getattr(g, k) for k, _ in Geometry._fields_}
+
+
+def room_for(blocks, ring=-1):
+    v = lib().liz_room_for(blocks, ring)
+    if v < 0:
+        check(int(v))
+    return v
+
+
+def blocks_for(blocks):
+    return lib().liz_blocks_for(blocks)
+
+
+def pick(w, h, codes=1, ring=-1, top_blocks=64):
+    return check(lib().liz_pick(float(w), float(h), codes, ring, top_blocks))
+
+
+class Encoder:
+    """liblizard's per-frame encoder (liz_encoder_*), painting on the GPU."""
+
+    def __init__(self, blocks, ring=-1, fps=60, codes=1):
+        self.h = lib().liz_encoder_new(C.byref(Format(blocks, ring, fps, codes)))
+        if not self.h:
+            check(-1)
+        g = Geometry()
+        check(lib().liz_encoder_geometry(self.h, C.byref(g)))
+        self.geometry = {k: getattr(g, k) for k, _ in Geometry._fields_}
+
+    def paint(self, blocks, picture=0, fmt=1):
+        """One frame: blocks (frame_blocks x 476 bytes) -> the page, height x width (x 4 for the 4-byte formats)."""
+        g = self.geometry
+        blocks = np.ascontiguousarray(blocks, np.uint8)
+        bpp = 1 if fmt == 1 else 4
+        out = np.zeros((g["height"], g["width"] * bpp), np.uint8)
+        check(lib().liz_encoder_paint(self.h, _p(blocks), picture, _p(out), out.strides[0], fmt))
+        return out if bpp == 1 else out.reshape(g["height"], g["width"], 4)
+
+    def close(self):
+        if self.h:
+            lib().liz_encoder_free(self.h)
+            self.h = None
 
 
 def b3_hash(data):

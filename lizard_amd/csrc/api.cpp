@@ -986,6 +986,229 @@ LIZ_API int lizx_b3_hash(const uint8_t *data, uint64_t length, uint8_t *out) {
   });
 }
 
+// ---- the format's arithmetic and the per-frame encoder (lizard.h sections 1 and 2) ----
+
+namespace {
+constexpr int RING_CELLS[3] = {64, 128, 256};  // liblizard/src/lizard_phy.h LIZARD_RING
+constexpr int RING_DEFAULT = 1;
+
+int ring_index(int ring) {
+  if (ring == LIZ_RING_DEFAULT) return RING_DEFAULT;
+  need(ring >= 0 && ring < LIZ_RINGS, "ring is 0 to 2 or LIZ_RING_DEFAULT");
+  return ring;
+}
+
+int pxm_for(int n, int span) { return static_cast<int>(std::ceil(static_cast<float>(n) / span - 1e-4f)); }
+
+liz_geometry geometry_of(const liz_format &fm) {
+  need(fm.blocks >= LIZ_MIN_BLOCKS && fm.blocks <= LIZ_MAX_BLOCKS, "blocks is 4 to 64 (LIZARD-64 to LIZARD-1024)");
+  need(fm.fps >= 1 && fm.fps <= 255, "fps is 1 to 255");
+  need(fm.codes == 1 || fm.codes == 2, "codes is 1 or 2");
+  const int r = ring_index(fm.ring);
+  const Format *f = format_for(fm.blocks);
+  if (!f) throw Error(LIZ_E_FORMAT, "no format of that size");
+  liz_geometry g{};
+  g.n = f->n;
+  g.span = 2 * RING_CELLS[r];
+  g.pxm = pxm_for(g.n, g.span);
+  g.side = (g.span + 30 + 4) * g.pxm;
+  g.gap = fm.codes > 1 ? LIZ_GAP_MODULES * g.pxm : 0;
+  g.width = fm.codes * g.side + (fm.codes - 1) * g.gap;
+  g.height = g.side;
+  g.frame_blocks = fm.codes * f->blocks;
+  return g;
+}
+
+// The room a symbol of this size needs on a display for its outer coefficients to keep 2.7 pixels a cycle: its width
+// over the picture's times the top ring's radius in cycles.
+double room_for(int blocks, int r) {
+  const double radius = std::sqrt(2.0 * 320 * 16 * blocks / 3.14159265358979323846);
+  return static_cast<double>(2 * RING_CELLS[r] + 30 + 4) / (2 * RING_CELLS[r]) * 2.7 * radius;
+}
+
+double lanczos3(double x) {
+  x = std::fabs(x);
+  if (x >= 3) return 0;
+  const double px = 3.14159265358979323846 * x;
+  return 3 * std::sin(px) * std::sin(px / 3) / (px * px);
+}
+
+// The resample's taps (lizard_phy.c:975-1000): output pixel x of the picture and its 3-module guard at K pixels a
+// module reads picture samples i0 .. i0 + 5 (taken as periodic), the Lanczos-3 weights normalised to sum to one,
+// rounded to float; a pixel that falls on a sample takes it alone.
+void lanczos_taps(int n, int span, int K, std::vector<int32_t> &i0, std::vector<float> &w) {
+  const int cp = 3, q = (span + 2 * cp) * K;
+  const double sc = static_cast<double>(n) / span;
+  i0.resize(q);
+  w.assign(static_cast<size_t>(6) * q, 0.0f);
+  for (int x = 0; x < q; x++) {
+    const double u = (x + 0.5) * sc / K - cp * sc - 0.5, r = std::floor(u + 0.5);
+    if (std::fabs(u - r) < 1e-6) {
+      i0[x] = static_cast<int32_t>(r) - 2;
+      w[6 * x + 2] = 1.0f;
+      continue;
+    }
+    const int first = static_cast<int>(std::floor(u)) - 2;
+    double tw[6], sum = 0;
+    for (int t = 0; t < 6; t++) {
+      tw[t] = lanczos3(u - (first + t));
+      sum += tw[t];
+    }
+    i0[x] = first;
+    for (int t = 0; t < 6; t++) w[6 * x + t] = static_cast<float>(tw[t] / sum);
+  }
+}
+}  // namespace
+
+struct liz_encoder {
+  liz_format fmt{};
+  liz_geometry g{};
+  liz_decoder *d = nullptr;
+  DevBuf pic, page, kinds, i0, w;
+  std::vector<uint8_t> host;
+  PageParams pp{};
+  ~liz_encoder() {
+    if (d) liz_decoder_free(d);
+  }
+};
+
+LIZ_API int liz_ring_cells(int ring) {
+  return guard([&] { return RING_CELLS[ring_index(ring)]; });
+}
+
+LIZ_API int liz_geometry_of(const liz_format *format, liz_geometry *out) {
+  return guard([&] {
+    need(format && out, "a format and a place for its geometry");
+    *out = geometry_of(*format);
+    return 0;
+  });
+}
+
+LIZ_API double liz_room_for(int blocks, int ring) {
+  double v = 0;
+  const int rc = guard([&] {
+    need(blocks >= LIZ_MIN_BLOCKS && blocks <= LIZ_MAX_BLOCKS, "blocks is 4 to 64");
+    v = room_for(blocks, ring_index(ring));
+    return 0;
+  });
+  return rc < 0 ? rc : v;
+}
+
+LIZ_API int liz_blocks_for(int blocks) {
+  if (blocks < LIZ_MIN_BLOCKS || blocks > LIZ_MAX_BLOCKS) return 0;
+  const Format *f = format_for(blocks);
+  return f ? f->blocks : 0;
+}
+
+LIZ_API int liz_pick(double w, double h, int codes, int ring, int top_blocks) {
+  return guard([&] {
+    need(codes == 1 || codes == 2, "codes is 1 or 2");
+    need(top_blocks >= LIZ_MIN_BLOCKS && top_blocks <= LIZ_MAX_BLOCKS, "top_blocks is 4 to 64");
+    const int r = ring_index(ring);
+    const double modules = 2 * RING_CELLS[r] + 30 + 4;
+    const double gap = codes > 1 ? (codes - 1) * static_cast<double>(LIZ_GAP_MODULES) / modules : 0;
+    const double room = std::max(64.0, std::min(w / (codes + gap), h));
+    int best = LIZ_MIN_BLOCKS;
+    for (int b = LIZ_MIN_BLOCKS; b <= top_blocks; b++)
+      if (room_for(b, r) <= room) best = b;
+    return best;
+  });
+}
+
+LIZ_API liz_encoder *liz_encoder_new(const liz_format *format) {
+  return guard_new<liz_encoder>([&] {
+    need(format, "a format");
+    auto e = std::make_unique<liz_encoder>();
+    e->fmt = *format;
+    e->g = geometry_of(*format);
+    e->d = liz_decoder_new(0);
+    if (!e->d) throw Error(LIZ_E_UNSUPPORTED, liz_last_error());
+    const liz_geometry &g = e->g;
+    const int L = g.span + 30;
+    std::vector<uint8_t> kinds(static_cast<size_t>(L) * L);
+    uint8_t cw[FMT_BYTES_MAX];
+    FmtWord fw;
+    fw.version = format->blocks;
+    fw.fps = format->fps;
+    fmt_encode(fw, cw, fmt_bytes(L));
+    border_kinds(g.span, cw, kinds.data());
+    std::vector<int32_t> i0;
+    std::vector<float> w;
+    lanczos_taps(g.n, g.span, g.pxm, i0, w);
+    hip_check(hipSetDevice(e->d->eng->device), "hipSetDevice");
+    e->kinds.reserve(kinds.size());
+    e->i0.reserve(i0.size() * sizeof(int32_t));
+    e->w.reserve(w.size() * sizeof(float));
+    e->pic.reserve(static_cast<size_t>(g.n) * g.n * sizeof(float));
+    e->page.reserve(static_cast<size_t>(g.side) * g.side);
+    hip_check(hipMemcpy(e->kinds.p, kinds.data(), kinds.size(), hipMemcpyHostToDevice), "border map");
+    hip_check(hipMemcpy(e->i0.p, i0.data(), i0.size() * sizeof(int32_t), hipMemcpyHostToDevice), "taps");
+    hip_check(hipMemcpy(e->w.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice), "weights");
+    PageParams &pp = e->pp;
+    pp.n = g.n;
+    pp.span = g.span;
+    pp.pxm = g.pxm;
+    pp.q = static_cast<int>(i0.size());
+    pp.kinds = e->kinds.as<uint8_t>();
+    pp.i0 = e->i0.as<int32_t>();
+    pp.w = e->w.as<float>();
+    pp.anch_a = g.span / 64;
+    for (int i = 0; i < 7; i++) pp.anch_m0[i] = anchor_m0(g.span, i);
+    e->host.resize(static_cast<size_t>(g.side) * g.side);
+    return e.release();
+  });
+}
+
+LIZ_API void liz_encoder_free(liz_encoder *e) { delete e; }
+
+LIZ_API int liz_encoder_geometry(const liz_encoder *e, liz_geometry *out) {
+  return guard([&] {
+    need(e && out, "an encoder and a place for its geometry");
+    *out = e->g;
+    return 0;
+  });
+}
+
+LIZ_API int liz_encoder_paint(liz_encoder *e, const uint8_t *blocks, uint32_t picture, uint8_t *out, int stride,
+                              liz_pixfmt fmt) {
+  return guard([&] {
+    need(e && blocks && out, "an encoder, blocks and a frame to paint");
+    need(fmt == LIZ_GREY8 || fmt == LIZ_RGBX8 || fmt == LIZ_RGBA8 || fmt == LIZ_BGRA8, "a pixel format of lizard.h's");
+    const int bpp = fmt == LIZ_GREY8 ? 1 : 4;
+    const liz_geometry &g = e->g;
+    need(stride >= g.width * bpp, "a stride of at least the frame's width in bytes");
+    LdpcEngine &eng = *e->d->eng;
+    const size_t per = static_cast<size_t>(g.frame_blocks / e->fmt.codes) * BLOCK_BYTES;
+    for (int k = 0; k < e->fmt.codes; k++) {
+      // the picture on the GPU (bit side, QPSK, inverse 2D DFT, clip), then the page round it
+      const int rc = lizx_modulate(e->d, e->fmt.blocks, 1, blocks + k * per, static_cast<int>(picture & 3),
+                                   e->pic.as<float>(), nullptr);
+      if (rc < 0) throw Error(rc, liz_last_error());
+      hip_check(hipSetDevice(eng.device), "hipSetDevice");
+      launch_paint_page(e->pp, e->pic.as<float>(), e->page.as<uint8_t>(), eng.stream);
+      eng.sync();
+      hip_check(hipMemcpy(e->host.data(), e->page.p, e->host.size(), hipMemcpyDeviceToHost), "page");
+      // each symbol into its place, the gap after it its row's last pixel (the margin's colour)
+      const int x0 = k * (g.side + g.gap), fill = k + 1 < e->fmt.codes ? g.gap : 0;
+      for (int y = 0; y < g.side; y++) {
+        const uint8_t *src = e->host.data() + static_cast<size_t>(y) * g.side;
+        uint8_t *o = out + static_cast<size_t>(y) * stride + static_cast<size_t>(x0) * bpp;
+        if (bpp == 1) {
+          std::memcpy(o, src, g.side);
+          std::memset(o + g.side, src[g.side - 1], fill);
+        } else {
+          for (int x = 0; x < g.side + fill; x++) {
+            const uint8_t v = src[x < g.side ? x : g.side - 1];
+            o[4 * x] = o[4 * x + 1] = o[4 * x + 2] = v;
+            o[4 * x + 3] = 255;
+          }
+        }
+      }
+    }
+    return 0;
+  });
+}
+
 LIZ_API int lizx_word_geometry(int span, int *cells, int *bytes) {
   return guard([&] {
     need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");

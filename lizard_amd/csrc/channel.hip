@@ -145,7 +145,66 @@ __global__ __launch_bounds__(256) void k_chan_blur_y(ChannelParams c, const floa
   img[i] = static_cast<uint8_t>(s < 0 ? 0 : s > 255 ? 255 : s);
 }
 
+// One page pixel a lane: the quiet zone white; the anchors' squares (light middle to a quarter of the half-side, dark
+// to 0.6, light rim); over modules 12 .. L - 12 the resample, down then across in the reference's order of float
+// operations (each a sum of six products taken in turn, clamped to [0, 1]); elsewhere the border's kind; then
+// (int)(v 255 + 1/2) clamped.
+__global__ __launch_bounds__(256) void k_paint_page(PageParams p, const float *pic, uint8_t *page) {
+  const int L = p.span + 30, C = p.pxm, side = (L + 4) * C;
+  const int X = blockIdx.x * 256 + threadIdx.x, Y = blockIdx.y;
+  if (X >= side) return;
+  const int dx = X - 2 * C, dy = Y - 2 * C;
+  float v = 1.0f;
+  if (dx >= 0 && dy >= 0 && dx < L * C && dy < L * C) {
+    const int mx = dx / C, my = dy / C;
+    int ai = -1, aj = -1;
+    for (int i = 0; i < 7; i++) {
+      if (mx >= p.anch_m0[i] && mx < p.anch_m0[i] + p.anch_a) ai = i;
+      if (my >= p.anch_m0[i] && my < p.anch_m0[i] + p.anch_a) aj = i;
+    }
+    if (ai >= 0 && aj >= 0) {
+      const float half = 0.5f * static_cast<float>(p.anch_a * C);
+      const float ex = fabsf(static_cast<float>(dx - p.anch_m0[ai] * C) + 0.5f - half);
+      const float ey = fabsf(static_cast<float>(dy - p.anch_m0[aj] * C) + 0.5f - half);
+      const float d = fmaxf(ex, ey) / half;
+      v = d < 0.25f || d >= 0.6f ? 1.0f : 0.0f;
+    } else if (mx >= 12 && my >= 12 && mx < L - 12 && my < L - 12) {
+      const int n = p.n, x = dx - 12 * C, y = dy - 12 * C;
+      const int ix = p.i0[x], iy = p.i0[y];
+      const float *wx = p.w + 6 * x, *wy = p.w + 6 * y;
+      int rows[6];
+#pragma unroll
+      for (int t = 0; t < 6; t++) {
+        const int r = (iy + t) % n;
+        rows[t] = r < 0 ? r + n : r;
+      }
+      float acc = 0.0f;
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        int k = (ix + s) % n;
+        k = k < 0 ? k + n : k;
+        float col = wy[0] * pic[rows[0] * n + k];
+#pragma unroll
+        for (int t = 1; t < 6; t++) col = col + wy[t] * pic[rows[t] * n + k];
+        acc = s == 0 ? wx[0] * col : acc + wx[s] * col;
+      }
+      v = acc < 0.0f ? 0.0f : acc > 1.0f ? 1.0f : acc;
+    } else {
+      v = (p.kinds[my * L + mx] & 3) == 2 ? 0.0f : 1.0f;
+    }
+  }
+  const float t = v * 255.0f + 0.5f;
+  const int g = static_cast<int>(t);
+  page[static_cast<size_t>(Y) * side + X] = static_cast<uint8_t>(g < 0 ? 0 : g > 255 ? 255 : g);
+}
+
 }  // namespace
+
+void launch_paint_page(const PageParams &p, const float *d_pic, uint8_t *d_page, hipStream_t s) {
+  const int side = (p.span + 34) * p.pxm;
+  hipLaunchKernelGGL(k_paint_page, dim3((side + 255) / 256, side), dim3(256), 0, s, p, d_pic, d_page);
+  hip_check(hipGetLastError(), "page launch");
+}
 
 void launch_channel(const ChannelParams &c, const float *d_pic, float *d_a, float *d_b, uint8_t *d_img,
                     hipStream_t s) {

@@ -22,6 +22,21 @@ struct ChannelParams {
   int anch_m0[7];
 };
 
+// The sender's page (liblizard/src/lizard_phy.c:1150-1236 and lizard_paint_grey :700-716): the symbol at pxm pixels a
+// module with its quiet zone, (span + 34) pxm pixels a side. The picture and its guard interval (modules 12 ..
+// span + 18) are the picture resampled as periodic through Lanczos-3 (i0: a pixel's first tap, w: its 6 weights, the
+// same table on both axes, q = (span + 6) pxm entries), the border's modules black or white by their kinds, the
+// anchors' squares over the picture.
+struct PageParams {
+  int n, span, pxm, q;
+  const uint8_t *kinds;
+  const int32_t *i0;
+  const float *w;
+  int anch_a;
+  int anch_m0[7];
+};
+void launch_paint_page(const PageParams &p, const float *d_pic, uint8_t *d_page, hipStream_t s);
+
 // Picture (n x n floats in [0, 1], device) -> capture (W x Hh uint8, device); d_a and d_b W x Hh float work areas.
 void launch_channel(const ChannelParams &c, const float *d_pic, float *d_a, float *d_b, uint8_t *d_img,
                     hipStream_t s);

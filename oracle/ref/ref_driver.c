@@ -52,6 +52,8 @@ void *ref_format_new(int n, int subch, int span) {
   return f;
 }
 void ref_format_free(void *p) { if (p) { lizard_free(p); free(p); } }
+/* The display rate the format's word states (lizard_fmt_fps): 0, or -1 where refused. */
+int ref_set_fps(void *p, int fps) { return lizard_fmt_fps(p, fps); }
 void ref_format_info(const void *p, int *info) {
   const lizard_t *f = p;
   info[0] = f->n; info[1] = f->subch; info[2] = f->blocks; info[3] = f->tiers; info[4] = f->vblocks; info[5] = f->used;

@@ -86,6 +86,7 @@ def ref(form="scalar"):
     _sig(lib, "ref_ldpc_decode_many", I32, I32, I32, P, P, P, I32, I32, F32)
     _sig(lib, "ref_format_new", P, I32, I32, I32)
     _sig(lib, "ref_format_free", None, P)
+    _sig(lib, "ref_set_fps", I32, P, I32)
     _sig(lib, "ref_format_info", None, P, P)
     _sig(lib, "ref_format_pos", C.POINTER(C.c_int), P)
     _sig(lib, "ref_format_blocks", None, P, P, P, P)
