@@ -426,8 +426,48 @@ def bench_sender(args, dec):
         out.free()
     gbs = total / (kms * 1e-3) / 1e9
     cpu = None if args.no_cpu else cpu_sender(work, args.cpu_seconds)
+    # the drop-in: liz_encoder_paint, blocks in and the whole GREY8 page out in host memory, frame by frame
+    enc = lizard_amd.Encoder(64, 2, 60)
+    g = enc.geometry
+    eb = rng.integers(0, 256, (8, g["frame_blocks"] * 476)).astype(np.uint8)
+    enc.paint(eb[0], 0)
+    t0 = time.perf_counter()
+    reps = 24
+    for i in range(reps):
+        enc.paint(eb[i % 8], i)
+    enc_ms = (time.perf_counter() - t0) * 1e3 / reps
+    enc.close()
+    encoder = {"workload": "liz_encoder_paint, LIZARD-1024 in the 256 ring (a %d x %d GREY8 page), blocks from and the "
+                           "page to host memory, frame by frame" % (g["side"], g["side"]),
+               "ms_per_frame": round(enc_ms, 3)}
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
+    if os.path.exists(ref_so) and not args.no_cpu:
+        lib = C.CDLL(ref_so)
+        lib.ref_format_new.restype = C.c_void_p
+        lib.ref_format_new.argtypes = [C.c_int, C.c_int, C.c_int]
+        lib.ref_format_free.argtypes = [C.c_void_p]
+        lib.ref_set_fps.argtypes = [C.c_void_p, C.c_int]
+        lib.ref_encode.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        lib.ref_paint_grey.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        f = lib.ref_format_new(1536, 1024, 512)
+        lib.ref_set_fps(f, 60)
+        px = g["side"] - 4 * g["pxm"]
+        drive = np.zeros(px * px, np.float32)
+        page = np.zeros(g["side"] * g["side"], np.uint8)
+        t0 = time.perf_counter()
+        n_ref = 0
+        while time.perf_counter() - t0 < 2.0:
+            lib.ref_encode(f, eb[n_ref % 8].ctypes.data, n_ref & 3, drive.ctypes.data, None)
+            lib.ref_paint_grey(f, drive.ctypes.data, page.ctypes.data)
+            n_ref += 1
+        ref_ms = (time.perf_counter() - t0) * 1e3 / n_ref
+        lib.ref_format_free(f)
+        encoder["cpu_baseline"] = {"value": round(ref_ms, 3), "unit": "ms/frame", "cores": 1, "kind": "reference",
+                                   "sample": "%d frames, 2 s, one thread: lizard_encode + lizard_paint_grey (x86-64 "
+                                             "SIMD build)" % n_ref}
     return {"workload": "%d sender pictures, n drawn from {384, 768, 1152, 1536} (%s), top formats, random blocks" %
                         (len(ns), ", ".join("%d x %d" % (int((ns == n).sum()), n) for n in sorted(table))),
+            "encoder": encoder,
             "gpu_ms_per_batch": round(kms, 3), "gpu_frames_per_s": round(len(ns) / (kms * 1e-3), 1),
             "frames_per_s_end_to_end": round(len(ns) / wall, 1),
             "algorithmic_bytes_per_batch": int(total),
