@@ -113,6 +113,10 @@ LIZ_API int liz_decoder_max_blocks(const liz_decoder *d);
 LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stride, liz_pixfmt fmt, int *held,
                        uint8_t *verified, liz_decoded *out);
 
+/* Replaces lizard.h:161 (api/codec.cpp:162-175, core/rx/crops.h): where a camera frame's symbols are looked for:
+   layout 1 the centre square, 2 two squares side by side along the long side; rect[k] = x, y, w, h; returns how many. */
+LIZ_API int liz_layout_rects(int w, int h, int layout, int rect[2][4]);
+
 /* ---- lizx: the path's seams ------------------------------------------------------------------------------------- */
 
 /* Per-block outcome codes, as the reference's lizard_block_stats (liblizard/src/lizard_phy.h:188-190): iterations

@@ -100,6 +100,7 @@ def lib():
     sig("lizx_word_geometry", _I, _I, _P, _P)
     sig("lizx_border_kinds", _I, _I, _I, _I, _P)
     sig("liz_ring_cells", _I, _I)
+    sig("liz_layout_rects", _I, _I, _I, _I, _P)
     sig("liz_geometry_of", _I, C.POINTER(Format), C.POINTER(Geometry))
     sig("liz_room_for", C.c_double, _I, _I)
     sig("liz_blocks_for", _I, _I)
@@ -381,6 +382,13 @@ def word_decode(span, q):
     v, f = C.c_int(0), C.c_int(0)
     got = check(lib().lizx_word_decode(span, _p(q), C.byref(v), C.byref(f)))
     return (v.value, f.value) if got else None
+
+
+def layout_rects(w, h, layout=1):
+    """liz_layout_rects: [(x, y, w, h), ...] of the frame's symbol regions."""
+    r = np.zeros((2, 4), np.int32)
+    k = check(lib().liz_layout_rects(w, h, layout, _p(r)))
+    return [tuple(int(v) for v in r[i]) for i in range(k)]
 
 
 def ring_cells(ring):

@@ -1115,6 +1115,24 @@ LIZ_API int liz_pick(double w, double h, int codes, int ring, int top_blocks) {
   });
 }
 
+LIZ_API int liz_layout_rects(int w, int h, int layout, int rect[2][4]) {
+  return guard([&] {
+    need(rect && w > 0 && h > 0, "a frame and a place for its regions");
+    need(layout == 1 || layout == 2, "layout is 1 (the centre square) or 2 (2:1)");
+    // the centre square, or two squares side by side along the long side (core/rx/crops.h cropsOf)
+    const bool two = layout == 2, wide = w >= h;
+    const int lo = std::min(w, h), hi = std::max(w, h), count = two ? 2 : 1;
+    const int side = two ? std::min(lo, hi / 2) : lo;
+    for (int k = 0; k < count; k++) {
+      const int along = (hi - count * side) / 2 + k * side, across = (lo - side) / 2;
+      rect[k][0] = wide ? along : across;
+      rect[k][1] = wide ? across : along;
+      rect[k][2] = rect[k][3] = side;
+    }
+    return count;
+  });
+}
+
 LIZ_API liz_encoder *liz_encoder_new(const liz_format *format) {
   return guard_new<liz_encoder>([&] {
     need(format, "a format");

@@ -132,3 +132,42 @@ def test_encoder_two_codes_and_rgba():
     assert np.array_equal(rgba[:, :, 0], page) and np.array_equal(rgba[:, :, 2], page) and (rgba[:, :, 3] == 255).all()
     with pytest.raises(L.LizError):
         L.Encoder(8, 0, 24, codes=3)
+
+
+def test_layout_rects():
+    assert L.layout_rects(1920, 1080) == [(420, 0, 1080, 1080)]
+    assert L.layout_rects(1080, 1920) == [(0, 420, 1080, 1080)]
+    assert L.layout_rects(1920, 1080, 2) == [(0, 60, 960, 960), (960, 60, 960, 960)]
+    assert L.layout_rects(2000, 800, 2) == [(200, 0, 800, 800), (1000, 0, 800, 800)]
+    assert L.layout_rects(801, 2001, 2) == [(0, 199, 801, 801), (0, 1000, 801, 801)]  # (2001 - 1602) // 2
+    assert L.layout_rects(7, 7) == [(0, 0, 7, 7)]
+    for bad in ((0, 10, 1), (10, 10, 3)):
+        with pytest.raises(L.LizError):
+            L.layout_rects(*bad)
+
+
+@pytest.mark.gpu
+def test_two_codes_through_crops_and_blind_decode():
+    """A sender's two codes (liz_encoder_paint, codes 2) shown in a wide frame, cut by liz_layout_rects and each crop
+    decoded with nothing held (liz_decode finds, reads the word, decodes): every block of both codes, in order."""
+    g = L.geometry_of(8, 0, 30, codes=2)
+    blocks = np.random.default_rng(6).integers(0, 256, g["frame_blocks"] * 476).astype(np.uint8)
+    enc = L.Encoder(8, 0, 30, codes=2)
+    page = enc.paint(blocks, 0)
+    enc.close()
+    H, W = 2 * g["side"] // 2 + 200, 2 * g["side"] + 400
+    frame = np.full((H, W), 255, np.uint8)
+    y0, x0 = (H - g["height"]) // 2, (W - g["width"]) // 2
+    frame[y0:y0 + g["height"], x0:x0 + g["width"]] = page
+    rects = L.layout_rects(W, H, 2)
+    assert len(rects) == 2
+    dec = L.Decoder(1536)
+    got = []
+    for x, y, w, h in rects:
+        k, b, out, held = dec.decode(frame[y:y + h, x:x + w])
+        assert out.found == 1 and out.word == 1 and out.blocks == 8 and out.fps == 30
+        got.append(b)
+    dec.close()
+    half = g["frame_blocks"] // 2
+    assert np.array_equal(np.concatenate(got).ravel(), blocks)
+    assert all(len(b) == half for b in got)
