@@ -113,6 +113,12 @@ LIZ_API int liz_decoder_max_blocks(const liz_decoder *d);
 LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stride, liz_pixfmt fmt, int *held,
                        uint8_t *verified, liz_decoded *out);
 
+/* Replaces lizard.h:48 (api/api.cpp:37): 1, the codec's vector path (here the GPU's) compiled in. */
+LIZ_API int liz_simd(void);
+/* Replaces lizard.h:164 (api/codec.cpp:177-179, src/shake.c): the test stream's payload for an id, SHAKE256 (FIPS 202)
+   of its four little-endian bytes, LIZ_PAYLOAD bytes. */
+LIZ_API void liz_stream_fill(uint32_t id, uint8_t *payload);
+
 /* Replaces lizard.h:161 (api/codec.cpp:162-175, core/rx/crops.h): where a camera frame's symbols are looked for:
    layout 1 the centre square, 2 two squares side by side along the long side; rect[k] = x, y, w, h; returns how many. */
 LIZ_API int liz_layout_rects(int w, int h, int layout, int rect[2][4]);

@@ -16,6 +16,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "lizard_amd.h")
 LIZ_OK, LIZ_E_ARG, LIZ_E_NOMEM, LIZ_E_IO, LIZ_E_UNSUPPORTED, LIZ_E_FORMAT, LIZ_E_TOOBIG, LIZ_E_STATE, LIZ_E_INTERNAL = (
     0, -1, -2, -3, -4, -5, -6, -7, -8)
 LIZ_BLOCK = 476
+LIZ_PAYLOAD = 472
 LDPC_CRC, LDPC_TRACE, LDPC_PLAIN = 1, 2, 4
 SRC_CAPTURE, SRC_GRID = 0, 1
 STAGE_SOFT, STAGE_BLOCKS = 0, 1
@@ -100,6 +101,8 @@ def lib():
     sig("lizx_word_geometry", _I, _I, _P, _P)
     sig("lizx_border_kinds", _I, _I, _I, _I, _P)
     sig("liz_ring_cells", _I, _I)
+    sig("liz_simd", _I)
+    sig("liz_stream_fill", None, C.c_uint32, _P)
     sig("liz_layout_rects", _I, _I, _I, _I, _P)
     sig("liz_geometry_of", _I, C.POINTER(Format), C.POINTER(Geometry))
     sig("liz_room_for", C.c_double, _I, _I)
@@ -382,6 +385,13 @@ def word_decode(span, q):
     v, f = C.c_int(0), C.c_int(0)
     got = check(lib().lizx_word_decode(span, _p(q), C.byref(v), C.byref(f)))
     return (v.value, f.value) if got else None
+
+
+def stream_fill(block_id):
+    """liz_stream_fill: the test stream's 472-byte payload for an id."""
+    out = np.zeros(LIZ_PAYLOAD, np.uint8)
+    lib().liz_stream_fill(block_id & 0xFFFFFFFF, _p(out))
+    return out
 
 
 def layout_rects(w, h, layout=1):

@@ -16,6 +16,7 @@
 #include "register.hpp"
 #include "finder.hpp"
 #include "xfer.hpp"
+#include "shake.hpp"
 #include "spectrum.hpp"
 
 #define LIZARD_AMD_VERSION "lizard-amd 0.1 (gfx950)"
@@ -1113,6 +1114,12 @@ LIZ_API int liz_pick(double w, double h, int codes, int ring, int top_blocks) {
       if (room_for(b, r) <= room) best = b;
     return best;
   });
+}
+
+LIZ_API int liz_simd(void) { return 1; }  // the codec's vector path, here the GPU's, is always the one that runs
+
+LIZ_API void liz_stream_fill(uint32_t id, uint8_t *payload) {
+  if (payload) stream_fill(id, payload, LIZ_PAYLOAD);
 }
 
 LIZ_API int liz_layout_rects(int w, int h, int layout, int rect[2][4]) {

@@ -77,6 +77,7 @@ def ref(form="scalar"):
         _sig(lib, "xfer_b3_hash", None, P, C.c_size_t, P)
         _sig(lib, "xfer_chunk_cv", I32, P, C.c_size_t, C.c_uint32, I32, P)
         _sig(lib, "xfer_root", I32, P, C.c_uint32, P)
+        _sig(lib, "stream_fill", None, C.c_uint32, P, C.c_uint32)
         _refs[form] = lib
         return lib
     _sig(lib, "ref_crc32c", C.c_uint32, P, I32)

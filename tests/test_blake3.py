@@ -107,3 +107,19 @@ def test_gpu_argument_errors(decoder_gpu):
     with pytest.raises(L.LizError):
         decoder_gpu.b3_chunk_cvs(d, 0, 10)
     d.free()
+
+
+def test_stream_fill_is_shake256_of_the_id():
+    """The test stream's payload (liz_stream_fill) against Python's own SHAKE256 and the reference's stream_fill
+    (liblizard/src/shake.c, compiled in oracle/_ref), over awkward ids."""
+    import hashlib
+    r = ref("xfer")
+    ids = [0, 1, 255, 256, 0xFFFF, 0x10000, 0x7FFFFFFF, 0xFFFFFFFF, 0xFFFC0000, 12345678] + list(range(1000, 1100))
+    for i in ids:
+        got = L.stream_fill(i)
+        assert got.tobytes() == hashlib.shake_256(int(i).to_bytes(4, "little")).digest(472), i
+        if r is not None:
+            want = np.zeros(472, np.uint8)
+            r.stream_fill(i, ptr(want), 472)
+            assert np.array_equal(got, want), i
+    assert L.lib().liz_simd() == 1
