@@ -688,6 +688,28 @@ def bench_finder(args):
     held_ms = (time.perf_counter() - t0) * 1e3 / len(frames)
     held.close()
     dec.close()
+    # the same blind decodes on host threads, one decoder (stream, buffers) a thread: the finder's geometry is host work
+    import concurrent.futures as cf
+    nthreads = min(16, os.cpu_count() or 1)
+    decs = [lizard_amd.Decoder(0) for _ in range(nthreads)]
+    rounds = 4
+    jobs = [(i % nthreads, frames[i % len(frames)]) for i in range(rounds * len(frames))]
+    locks = [threading.Lock() for _ in range(nthreads)]
+
+    def blind_job(j):
+        k, f = j
+        with locks[k]:
+            return decs[k].decode(f["img"])[0]
+
+    with cf.ThreadPoolExecutor(nthreads) as ex:
+        list(ex.map(blind_job, jobs[:nthreads]))
+        t0 = time.perf_counter()
+        kv = sum(ex.map(blind_job, [(i % nthreads, jobs[i][1]) for i in range(len(jobs))]))
+        tw = time.perf_counter() - t0
+    for d in decs:
+        d.close()
+    threaded = {"threads": nthreads, "frames_per_s": round(len(jobs) / tw, 1), "verified": int(kv),
+                "frames": len(jobs)}
     st = np.mean(np.array(stages), axis=0) if stages else np.zeros(6)
     out = {"workload": "16 captures 1920x1080 of LIZARD-1024 (n=1536, 256 ring) painted whole (border, word, anchors, "
                        "guard) on the GPU: rotation +-10 deg, scale 0.6-0.9, perspective <= 0.05, k1 +-0.05, blur "
@@ -700,6 +722,7 @@ def bench_finder(args):
            "k1_err_max": round(float(np.max(k1err)), 5) if k1err else None,
            "blind_decode_ms_per_frame": round(blind_ms, 3), "blind_verified": int(kb),
            "held_truth_decode_ms_per_frame": round(held_ms, 3), "held_truth_verified": int(kt),
+           "blind_decode_threaded": threaded,
            "blocks": 124 * len(frames)}
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
     if os.path.exists(ref_so) and not args.no_cpu:
@@ -744,6 +767,31 @@ def bench_finder(args):
         out["reference_blind_decode"] = {"ms_per_frame": round(blind_cpu_ms, 3), "verified": int(rver), "cores": 1,
                                          "what": "lizard_acquire then lizard_finish through its registration, one "
                                                  "thread (x86-64 SIMD build)"}
+
+        # and on the same host threads, one format instance (its own workspace) a thread
+        def ref_job(k):
+            fm = lib.ref_format_new(1536, 1024, span)
+            Hk, nk, qk, ok_ = (np.zeros(9, np.float32), np.zeros(2 * nx * nx, np.float32), np.zeros(8, np.float32),
+                               np.zeros(1, np.int32))
+            bk, okk, itk, bek = (np.zeros(124 * 476, np.uint8), np.zeros(124, np.uint8), np.zeros(124, np.int8),
+                                 np.zeros(124, np.float32))
+            done = 0
+            for i in range(k, len(jobs), nthreads):
+                img = jobs[i][1]["img"]
+                if fn(fm, img.ctypes.data, img.shape[1], img.shape[0], Hk.ctypes.data, nk.ctypes.data,
+                      qk.ctypes.data, ok_.ctypes.data):
+                    dr(fm, img.ctypes.data, img.shape[1], img.shape[0], Hk.ctypes.data, nk.ctypes.data,
+                       bk.ctypes.data, okk.ctypes.data, itk.ctypes.data, bek.ctypes.data, None, None)
+                done += 1
+            lib.ref_format_free(fm)
+            return done
+
+        with cf.ThreadPoolExecutor(nthreads) as ex:
+            t0 = time.perf_counter()
+            nref = sum(ex.map(ref_job, range(nthreads)))
+            tr = time.perf_counter() - t0
+        out["reference_blind_decode"]["threaded_frames_per_s"] = round(nref / tr, 1)
+        out["reference_blind_decode"]["threads"] = nthreads
         out["blind_decode_over_reference"] = round(blind_cpu_ms / blind_ms, 2)
         out["cpu_baseline"] = {"value": round(cpu_ms, 3), "unit": "ms/frame", "cores": 1, "kind": "reference",
                                "found": rfound,
