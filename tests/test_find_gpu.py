@@ -152,3 +152,40 @@ def test_hold_then_release(decoder_gpu):
     k, got, out, held = dec.decode(img)
     dec.close()
     assert out.found == 1 and out.word == 1 and k == C.blocks_n and np.array_equal(got.ravel(), C.blocks)
+
+
+def test_finder_against_reference_finder_on_random_hard_captures(decoder_gpu):
+    """24 GPU-painted captures over the whole range (any rotation, scale 0.6-0.9 of the height, perspective up to
+    0.05, k1 +-0.05, blur 0.5-1.5 px, AWGN 2-8 grey; three formats and rings): this finder finds at least as many as
+    the reference's own finder and registers them at least as closely on the frames both find."""
+    rng = np.random.default_rng(2024)
+    r = ref("x86")
+    mine = theirs = 0
+    err_mine, err_ref = [], []
+    for i in range(24):
+        version, span = [(5, 128), (20, 256), (64, 512)][i % 3]
+        blocks = sim.random_blocks(version, rng)
+        cam = sim.Camera(1920, 1080, rot_deg=float(rng.uniform(0, 360)), scale=float(rng.uniform(0.6, 0.9)),
+                         persp=float(rng.uniform(0, 0.05)), k1=float(rng.uniform(-0.05, 0.05)),
+                         blur=float(rng.uniform(0.5, 1.5)), noise=float(rng.uniform(2, 8)))
+        Hm = cam.H(span)
+        dpic = decoder_gpu.modulate(version, blocks)
+        dimg = decoder_gpu.capture(version, span, dpic, 1920, 1080, Hm, k1=cam.k1, blur=cam.blur, noise=cam.noise,
+                                   seed=i, fps=30)
+        img = dimg.get(np.uint8, 1920 * 1080).reshape(1080, 1920)
+        dimg.free()
+        dpic.free()
+        Ht = (Hm / Hm[2, 2]).astype(np.float32).ravel()
+        nt = sim.nodes_for(cam, span, Hm)
+        node_x = L.node_lattice(span)
+        f = decoder_gpu.find(img)
+        ok_mine = f is not None and f["span"] == span
+        reg, _ = reference_path(version, span, img)
+        ok_ref = reg is not None
+        mine += ok_mine
+        theirs += ok_ref
+        if ok_mine and ok_ref:
+            err_mine.append(node_err(f["H"], f["nodes"], Ht, nt, node_x)[0])
+            err_ref.append(node_err(reg[0], reg[1], Ht, nt, node_x)[0])
+    assert mine >= theirs and mine >= 22, (mine, theirs)
+    assert np.median(err_mine) <= np.median(err_ref), (np.median(err_mine), np.median(err_ref))
