@@ -285,6 +285,13 @@ class DeviceArray:
         check(lib().lizx_dev_upload(d.ptr, _p(arr), arr.nbytes))
         return d
 
+    def put(self, arr):
+        """Copies a host array into the start of this device array."""
+        arr = np.ascontiguousarray(arr)
+        if arr.nbytes > self.nbytes:
+            raise ValueError("the array is larger than the device array")
+        check(lib().lizx_dev_upload(self.ptr, _p(arr), arr.nbytes))
+
     def get(self, dtype, count):
         out = np.empty(count, dtype)
         check(lib().lizx_dev_download(_p(out), self.ptr, out.nbytes))
