@@ -268,6 +268,29 @@ LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_
 LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stride, int *span, float *H, float *nodes,
                       float *quad, float *info);
 
+/* ---- replays: the .lizard file (research/SPEC.md 15) ---------------------------------------------------------- */
+
+/* A replay read a frame at a time (liblizard/core/rx/lizfile.h Reader): NULL with liz_last_error where the file is not
+   a LIZARD replay of version 1 (a ZIP of stored entries, its comment "LIZARD replay 1", meta.json with frames and
+   sizes, an NNNN.zst entry a frame; every entry's CRC-32 checked). frame: the frame's w h luma, rows w apart (the
+   median predictor's residuals, one zstd frame, undone; zstd is the system's libzstd). meta: 0 meta.json, 1
+   stats.jsonl ("" none), 2 the layout ("1:1" or "2:1"). */
+typedef struct lizx_replay lizx_replay;
+LIZ_API lizx_replay *lizx_replay_open(const char *path);
+LIZ_API void lizx_replay_close(lizx_replay *r);
+LIZ_API int lizx_replay_frames(const lizx_replay *r);
+LIZ_API int lizx_replay_frame_info(const lizx_replay *r, int i, int *w, int *h, double *ms);
+LIZ_API int lizx_replay_frame(lizx_replay *r, int i, uint8_t *luma);
+LIZ_API const char *lizx_replay_meta(const lizx_replay *r, int which);
+/* A replay written as it goes (lizfile.h Writer): frames in order, then finish writes stats.jsonl (NULL or "" none)
+   and the caller's meta.json; written through path.part, renamed over path at finish. finish returns the file's
+   bytes. */
+typedef struct lizx_replay_writer lizx_replay_writer;
+LIZ_API lizx_replay_writer *lizx_replay_create(const char *path);
+LIZ_API int lizx_replay_add(lizx_replay_writer *w, const uint8_t *luma, int width, int height, int stride);
+LIZ_API int64_t lizx_replay_finish(lizx_replay_writer *w, const char *meta, const char *stats);
+LIZ_API void lizx_replay_writer_free(lizx_replay_writer *w);
+
 /* ---- the transfer's BLAKE3 tree (a verified file's chunks and root) --------------------------------------------- */
 
 /* Every chunk's chaining value of a file in device memory (16-byte aligned): the file cut into chunks of

@@ -101,6 +101,16 @@ def lib():
     sig("lizx_word_geometry", _I, _I, _P, _P)
     sig("lizx_border_kinds", _I, _I, _I, _I, _P)
     sig("liz_ring_cells", _I, _I)
+    sig("lizx_replay_open", _P, C.c_char_p)
+    sig("lizx_replay_close", None, _P)
+    sig("lizx_replay_frames", _I, _P)
+    sig("lizx_replay_frame_info", _I, _P, _I, _P, _P, _P)
+    sig("lizx_replay_frame", _I, _P, _I, _P)
+    sig("lizx_replay_meta", C.c_char_p, _P, _I)
+    sig("lizx_replay_create", _P, C.c_char_p)
+    sig("lizx_replay_add", _I, _P, _P, _I, _I, _I)
+    sig("lizx_replay_finish", C.c_int64, _P, C.c_char_p, C.c_char_p)
+    sig("lizx_replay_writer_free", None, _P)
     sig("liz_simd", _I)
     sig("liz_stream_fill", None, C.c_uint32, _P)
     sig("liz_layout_rects", _I, _I, _I, _I, _P)
@@ -392,6 +402,64 @@ def word_decode(span, q):
     v, f = C.c_int(0), C.c_int(0)
     got = check(lib().lizx_word_decode(span, _p(q), C.byref(v), C.byref(f)))
     return (v.value, f.value) if got else None
+
+
+class Replay:
+    """A .lizard replay read a frame at a time (lizx_replay_*)."""
+
+    def __init__(self, path):
+        self.h = lib().lizx_replay_open(os.fsencode(path))
+        if not self.h:
+            check(-1)
+
+    def __len__(self):
+        return check(lib().lizx_replay_frames(self.h))
+
+    def info(self, i):
+        w, h, ms = C.c_int(0), C.c_int(0), C.c_double(0)
+        check(lib().lizx_replay_frame_info(self.h, i, C.byref(w), C.byref(h), C.byref(ms)))
+        return w.value, h.value, ms.value
+
+    def frame(self, i):
+        w, h, _ = self.info(i)
+        out = np.zeros((h, w), np.uint8)
+        check(lib().lizx_replay_frame(self.h, i, _p(out)))
+        return out
+
+    def meta(self, which=0):
+        return lib().lizx_replay_meta(self.h, which).decode()
+
+    def close(self):
+        if self.h:
+            lib().lizx_replay_close(self.h)
+            self.h = None
+
+
+class ReplayWriter:
+    """A .lizard replay written as it goes (lizx_replay_create / add / finish)."""
+
+    def __init__(self, path):
+        self.h = lib().lizx_replay_create(os.fsencode(path))
+        if not self.h:
+            check(-1)
+
+    def add(self, luma):
+        luma = np.asarray(luma, np.uint8)
+        if luma.ndim != 2 or luma.strides[1] != 1:
+            luma = np.ascontiguousarray(luma)
+        return check(lib().lizx_replay_add(self.h, _p(luma), luma.shape[1], luma.shape[0], luma.strides[0]))
+
+    def finish(self, meta, stats=""):
+        n = lib().lizx_replay_finish(self.h, meta.encode(), stats.encode())
+        check(int(n) if n < 0 else 0)
+        lib().lizx_replay_writer_free(self.h)
+        self.h = None
+        return int(n)
+
+    def close(self):
+        if self.h:
+            lib().lizx_replay_writer_free(self.h)
+            self.h = None
 
 
 def stream_fill(block_id):

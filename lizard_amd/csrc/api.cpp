@@ -17,6 +17,7 @@
 #include "finder.hpp"
 #include "xfer.hpp"
 #include "shake.hpp"
+#include "replay.hpp"
 #include "spectrum.hpp"
 
 #define LIZARD_AMD_VERSION "lizard-amd 0.1 (gfx950)"
@@ -1233,6 +1234,87 @@ LIZ_API int liz_encoder_paint(liz_encoder *e, const uint8_t *blocks, uint32_t pi
     return 0;
   });
 }
+
+// ---- replays ----
+
+struct lizx_replay {
+  std::unique_ptr<ReplayReader> r;
+};
+struct lizx_replay_writer {
+  std::unique_ptr<ReplayWriter> w;
+};
+
+LIZ_API lizx_replay *lizx_replay_open(const char *path) {
+  return guard_new<lizx_replay>([&] {
+    need(path, "a path");
+    auto r = std::make_unique<lizx_replay>();
+    r->r = std::make_unique<ReplayReader>(path);
+    return r.release();
+  });
+}
+
+LIZ_API void lizx_replay_close(lizx_replay *r) { delete r; }
+
+LIZ_API int lizx_replay_frames(const lizx_replay *r) {
+  return guard([&] {
+    need(r, "a replay");
+    return static_cast<int>(r->r->frames().size());
+  });
+}
+
+LIZ_API int lizx_replay_frame_info(const lizx_replay *r, int i, int *w, int *h, double *ms) {
+  return guard([&] {
+    need(r && i >= 0 && static_cast<size_t>(i) < r->r->frames().size(), "a replay and one of its frames");
+    const ReplayReader::Frame &f = r->r->frames()[i];
+    if (w) *w = static_cast<int>(f.w);
+    if (h) *h = static_cast<int>(f.h);
+    if (ms) *ms = f.ms;
+    return 0;
+  });
+}
+
+LIZ_API int lizx_replay_frame(lizx_replay *r, int i, uint8_t *luma) {
+  return guard([&] {
+    need(r && luma && i >= 0, "a replay, a frame index and room for its luma");
+    r->r->frame(static_cast<size_t>(i), luma);
+    return 0;
+  });
+}
+
+LIZ_API const char *lizx_replay_meta(const lizx_replay *r, int which) {
+  if (!r) return nullptr;
+  return which == 0 ? r->r->meta().c_str() : which == 1 ? r->r->stats().c_str() : r->r->layout().c_str();
+}
+
+LIZ_API lizx_replay_writer *lizx_replay_create(const char *path) {
+  return guard_new<lizx_replay_writer>([&] {
+    need(path, "a path");
+    auto w = std::make_unique<lizx_replay_writer>();
+    w->w = std::make_unique<ReplayWriter>(path);
+    return w.release();
+  });
+}
+
+LIZ_API int lizx_replay_add(lizx_replay_writer *w, const uint8_t *luma, int width, int height, int stride) {
+  return guard([&] {
+    need(w && luma && width > 0 && height > 0 && width <= 16384 && height <= 16384 && stride >= width,
+         "a writer and a luma frame of 1 to 16384 pixels a side, rows stride >= w bytes apart");
+    w->w->frame(luma, static_cast<uint32_t>(width), static_cast<uint32_t>(height), static_cast<uint32_t>(stride));
+    return static_cast<int>(w->w->frames());
+  });
+}
+
+LIZ_API int64_t lizx_replay_finish(lizx_replay_writer *w, const char *meta, const char *stats) {
+  int64_t bytes = -1;
+  const int rc = guard([&] {
+    need(w && meta, "a writer and its meta.json");
+    bytes = w->w->finish(meta, stats ? stats : "");
+    return 0;
+  });
+  return rc < 0 ? rc : bytes;
+}
+
+LIZ_API void lizx_replay_writer_free(lizx_replay_writer *w) { delete w; }
 
 LIZ_API int lizx_word_geometry(int span, int *cells, int *bytes) {
   return guard([&] {
