@@ -11,7 +11,8 @@ the GPU. Inputs are resident in HBM before the timed region.
 Prints one JSON line: LDPC Gbit/s (3840 x codewords / s) as the value, payload MB/s (472 x verified blocks / s),
 the plain per-codeword path's rate beside the batched one, the decode kernel's roofline (integer operations against
 the vector issue peak, with the LDS and HBM figures beside it) and the reference CPU decoder timed on this machine's
-cores. Two more workloads ride along as sub-objects:
+cores, and the PHY's own setting beside it ("stall_rule": 30 iterations with the stall rule 9 / 0.95). More
+workloads ride along as sub-objects:
   "pipeline"   the full receive path, batched: 32 captured 1920x1080 uint8 frames, each an n = 1536 picture
                (LIZARD-1024, 124 blocks) in the 256 ring through a random homography (rotation +-10 deg, scale 0.6-0.9
                of the frame height, perspective up to 0.05), radial k1 in [-0.05, 0.05], blur sigma 0.5-1.5 px, AWGN
@@ -26,7 +27,14 @@ cores. Two more workloads ride along as sub-objects:
   "small"      one n = 384 LIZARD-80 frame in a 1280x720 capture through liz_decode, against the reference's decoder
                on one core.
   "sender"     the sender's pictures for the FFT+demap mix on the GPU (lizx_modulate): HBM roofline of its two passes,
-               the reference's encoder on the host cores beside it.
+               the reference's encoder on the host cores beside it; and liz_encoder_paint's whole page (LIZARD-1024)
+               against the reference's encode and paint on one core.
+  "finder"     16 whole-symbol captures (the pipeline's channel, painted with border and word): lizx_find's hits,
+               registration error against the truth and time, the blind liz_decode, the same on 16 host threads, and
+               the blind receiver at rate (finder on threads, then one batch graph); the reference's finder and its
+               whole blind decode beside them.
+  "blake3"     the transfer's chunk check: BLAKE3 chaining values of every 1 MiB chunk of a 1 GiB file in HBM, its
+               32-bit vector-issue roofline, the reference's BLAKE3 (AVX2) on the host's threads beside it.
 """
 import argparse
 import ctypes as C
