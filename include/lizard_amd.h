@@ -267,6 +267,12 @@ LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_
    pixel (GPU), the homography and lens fitted to them, ring and orientation by the timing track (host). */
 LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stride, int *span, float *H, float *nodes,
                       float *quad, float *info);
+/* The finder on 1..256 frames of one size at once (GREY8, rows stride bytes apart): each frame uploaded and searched on
+   one of the handle's host threads (up to 16, made on the first call), a stream each; frame f's result is the one
+   lizx_find gives it. No counterpart in the reference, whose finder takes a frame a call. Out per frame: spans[f] (0 not found), found[f], H + 9 f, nodes + 3200 f (room for 40 x 40
+   pairs each). Returns the frames found. */
+LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *imgs, int w, int h, int stride,
+                            int *spans, float *H, float *nodes, int *found);
 
 /* ---- replays: the .lizard file (research/SPEC.md 15) ---------------------------------------------------------- */
 

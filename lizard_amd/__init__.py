@@ -137,6 +137,7 @@ def lib():
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
     sig("lizx_find", _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P)
+    sig("lizx_find_batch", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P)
     sig("lizx_batch_new", _P, _I)
     sig("lizx_batch_free", None, _P)
     sig("lizx_batch_plan", _I, _P, _I, _P, _P, _I, _I)
@@ -646,6 +647,27 @@ class Decoder:
         ms = C.c_float(0)
         check(lib().lizx_b3_hash_device(self.h, d_data.ptr, length, _p(out), C.byref(ms)))
         return (out.tobytes(), ms.value) if timed else out.tobytes()
+
+    def find_batch(self, imgs):
+        """lizx_find_batch on same-size GREY8 frames: a list of None or dicts (span, H, nodes) as find gives."""
+        imgs = [np.ascontiguousarray(i, np.uint8) for i in imgs]
+        F = len(imgs)
+        h, w = imgs[0].shape
+        assert all(i.shape == (h, w) for i in imgs)
+        ptrs = (C.c_void_p * F)(*[i.ctypes.data for i in imgs])
+        spans, found = np.zeros(F, np.int32), np.zeros(F, np.int32)
+        H = np.zeros((F, 9), np.float32)
+        nodes = np.zeros((F, 40 * 40 * 2), np.float32)
+        check(lib().lizx_find_batch(self.h, F, ptrs, w, h, w, _p(spans), _p(H), _p(nodes), _p(found)))
+        out = []
+        for f in range(F):
+            if not found[f]:
+                out.append(None)
+                continue
+            nx = len(node_lattice(int(spans[f])))
+            out.append({"span": int(spans[f]), "H": H[f].copy(),
+                        "nodes": nodes[f, :nx * nx * 2].reshape(nx, nx, 2).copy()})
+        return out
 
     def frame_slots_device(self, version, d_blocks, frames, cnt, d_slots):
         """lizx_frame_slots_device: device blocks (frames x blocks x 476) -> device slots (frames x subch x 640)."""

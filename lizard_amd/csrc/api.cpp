@@ -1497,6 +1497,41 @@ LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stri
   });
 }
 
+LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *imgs, int w, int h, int stride,
+                            int *spans, float *H, float *nodes, int *found) {
+  return guard([&] {
+    need(d && imgs && spans && H && nodes && found, "a decoder, the frames and room for what is found");
+    need(frames >= 1 && frames <= 256, "1 to 256 frames");
+    need(w >= 16 && h >= 16 && w <= 16384 && h <= 16384 && stride >= w, "frames of 16 to 16384 pixels a side");
+    for (int f = 0; f < frames; f++) need(imgs[f], "every frame's pixels");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const size_t fb = static_cast<size_t>(w) * h;
+    d->finder.img.reserve(fb * frames);
+    std::vector<FindResult> res(frames);
+    find_symbols(d->finder, frames, d->finder.img.as<uint8_t>(), imgs, w, h, stride, e.stream, res.data());
+    int hits = 0;
+    for (int f = 0; f < frames; f++) {
+      const FindResult &fr = res[f];
+      found[f] = fr.found;
+      spans[f] = fr.found ? fr.span : 0;
+      if (!fr.found) continue;
+      hits++;
+      float *Hf = H + 9 * f;
+      for (int i = 0; i < 9; i++) Hf[i] = static_cast<float>(fr.H[i]);
+      RegModel m;
+      reg_model(fr.span, Hf, w, h, m);
+      float params[REG_PARAMS] = {};
+      for (int i = 0; i < 8; i++) params[i] = Hf[i];
+      params[8] = static_cast<float>(fr.k1);
+      std::vector<float> nv;
+      reg_nodes(m, params, nv);
+      std::memcpy(nodes + static_cast<size_t>(f) * MAX_NODES * MAX_NODES * 2, nv.data(), nv.size() * sizeof(float));
+    }
+    return hits;
+  });
+}
+
 // The registration (H from params with h8 = 1, nodes from k1 and the anchors) of a refinement call, staged with its
 // capture and reference picture.
 static const WarpFrame *stage_refinement(liz_decoder *d, const FramePlan &p, const uint8_t *img, int w, int h, int stride,

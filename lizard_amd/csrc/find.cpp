@@ -14,7 +14,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <exception>
+#include <functional>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 #include "finder.hpp"
@@ -371,51 +376,108 @@ void robust_edge(std::vector<Pt> &pts, double ox, double oy, double tx, double t
 
 }  // namespace
 
-void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w, int h, int stride, hipStream_t s,
-                 FindResult &r) {
-  const int d_stride = w;  // the device copy's rows are tight
-  const float upload = r.ms[0];
-  r = FindResult{};
-  r.ms[0] = upload;
-  auto t0 = std::chrono::steady_clock::now();
-  auto lap = [&](int k) {
-    const auto t1 = std::chrono::steady_clock::now();
-    r.ms[k] = std::chrono::duration<float, std::milli>(t1 - t0).count();
-    t0 = t1;
-  };
-  const int R = static_cast<int>(std::ceil(0.5 * std::hypot(w, h))) + 2, nr = 2 * R + 1;
-  const size_t cells = static_cast<size_t>(FIND_THETA) * nr;
-  b.acc.reserve(cells * sizeof(float));
-  b.max.reserve(256);
-  b.count.reserve(256);
-  b.peaks.reserve(FIND_MAX_PEAKS * sizeof(FindPeak));
-  if (b.trig_nr != 1) {
-    std::vector<float> tr(2 * FIND_THETA);
-    for (int t = 0; t < FIND_THETA; t++) {
-      tr[2 * t] = static_cast<float>(std::cos(2 * PI * t / FIND_THETA));
-      tr[2 * t + 1] = static_cast<float>(std::sin(2 * PI * t / FIND_THETA));
-    }
-    b.trig.reserve(tr.size() * sizeof(float));
-    hip_check(hipStreamSynchronize(s), "finder stream");
-    hip_check(hipMemcpy(b.trig.p, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice), "trig");
-    b.trig_nr = 1;
-  }
-  launch_find_zero(b.acc.as<float>(), cells, b.max.as<unsigned>(), b.count.as<int>(), s);
-  launch_find_vote(d_img, w, h, d_stride, 12.0f, b.trig.as<float2>(), nr, b.acc.as<float>(), s);
-  launch_find_peaks(b.acc.as<float>(), nr, 0.08f, b.max.as<unsigned>(), b.peaks.as<FindPeak>(), b.count.as<int>(), s);
-  hip_check(hipGetLastError(), "finder launches");
-  int np = 0;
-  hip_check(hipMemcpyAsync(&np, b.count.p, sizeof(int), hipMemcpyDeviceToHost, s), "peak count");
-  hip_check(hipStreamSynchronize(s), "finder peaks");
-  np = std::min(np, FIND_MAX_PEAKS);
-  std::vector<FindPeak> pk(np);
-  if (np) hip_check(hipMemcpy(pk.data(), b.peaks.p, np * sizeof(FindPeak), hipMemcpyDeviceToHost), "peaks");
-  lap(1);
-  std::sort(pk.begin(), pk.end(), [](const FindPeak &a, const FindPeak &c) { return a.score > c.score; });
-  if (pk.size() > 96) pk.resize(96);
+// ---- the search as stages over one frame's state, so that many frames can share each stage's launches ----
 
-  // sides: an outer edge (gradient out) and an inner edge (gradient in) half a turn apart, a positive thickness apart
-  const double cx = 0.5 * w, cy = 0.5 * h, dth = 2.5 * 2 * PI / FIND_THETA;
+struct FinderPool {
+  std::vector<std::thread> threads;
+  std::mutex m;
+  std::condition_variable cv, done_cv;
+  std::function<void(int, int)> task;  // (item, thread)
+  int count = 0, next = 0, done = 0;
+  uint64_t gen = 0;
+  bool stop = false;
+  explicit FinderPool(int n) {
+    for (int t = 0; t < n; t++)
+      threads.emplace_back([this, t] {
+        uint64_t seen = 0;
+        for (;;) {
+          std::unique_lock<std::mutex> lk(m);
+          cv.wait(lk, [&] { return stop || gen != seen; });
+          if (stop) return;
+          seen = gen;
+          while (next < count) {
+            const int i = next++;
+            lk.unlock();
+            task(i, t);
+            lk.lock();
+            if (++done == count) done_cv.notify_all();
+          }
+        }
+      });
+  }
+  ~FinderPool() {
+    {
+      std::lock_guard<std::mutex> lk(m);
+      stop = true;
+    }
+    cv.notify_all();
+    for (auto &t : threads) t.join();
+  }
+  // fn(0 .. n - 1, thread) on the pool's threads; returns when every call has
+  void run(int n, const std::function<void(int, int)> &fn) {
+    if (n <= 0) return;
+    std::unique_lock<std::mutex> lk(m);
+    task = fn;
+    count = n;
+    next = 0;
+    done = 0;
+    gen++;
+    cv.notify_all();
+    done_cv.wait(lk, [&] { return done == count; });
+  }
+};
+
+namespace {
+
+constexpr int ENDS[4][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}};  // a side's corners (top TL-TR, right TR-BR, ...)
+
+struct FindJob {
+  const uint8_t *img = nullptr;  // host copy
+  int w = 0, h = 0, stride = 0, frame = 0;
+  double cx = 0, cy = 0;
+  std::vector<FindPeak> pk;
+  Side S4[4];
+  double Cx[4] = {}, Cy[4] = {}, thick = 0;
+  std::vector<FindProbe> pr;  // this stage's probes
+  std::vector<int> lab1;      // straight-edge probes' labels: side * 2 + edge
+  std::vector<Pt> lab2;       // model probes' labels: side and depth
+  std::vector<Pt> pts;
+  Fit fit{};
+  Lens lens{};
+  int L = 0;
+  bool dead = false;
+  FindResult *r = nullptr;
+};
+
+bool meet_sides(const Side &p, const Side &q, double cx, double cy, double *x, double *y) {
+  const double det = p.nx * q.ny - p.ny * q.nx;
+  if (std::fabs(det) < 1e-6) return false;
+  const double a = 0.5 * (p.ro + p.ri), c = 0.5 * (q.ro + q.ri);
+  *x = cx + (a * q.ny - c * p.ny) / det;
+  *y = cy + (p.nx * c - q.nx * a) / det;
+  return true;
+}
+
+void corners(FindJob &j) {
+  meet_sides(j.S4[0], j.S4[3], j.cx, j.cy, &j.Cx[0], &j.Cy[0]);
+  meet_sides(j.S4[0], j.S4[1], j.cx, j.cy, &j.Cx[1], &j.Cy[1]);
+  meet_sides(j.S4[2], j.S4[1], j.cx, j.cy, &j.Cx[2], &j.Cy[2]);
+  meet_sides(j.S4[2], j.S4[3], j.cx, j.cy, &j.Cx[3], &j.Cy[3]);
+}
+
+// sides: an outer edge (gradient out) and an inner edge (gradient in) half a turn apart, a positive thickness apart;
+// four of them a quarter turn apart (within 30 degrees) round a convex quad that holds its centre, best by votes
+void stage_sides(FindJob &j) {
+  std::vector<FindPeak> &pk = j.pk;
+  // by score, ties by votes, direction, distance: an order that does not depend on the peaks' arrival
+  std::sort(pk.begin(), pk.end(), [](const FindPeak &a, const FindPeak &c) {
+    if (a.score != c.score) return a.score > c.score;
+    if (a.votes != c.votes) return a.votes > c.votes;
+    if (a.theta != c.theta) return a.theta < c.theta;
+    return a.rho < c.rho;
+  });
+  if (pk.size() > 96) pk.resize(96);
+  const double cx = j.cx, cy = j.cy, dth = 2.5 * 2 * PI / FIND_THETA;
   std::vector<Side> sides;
   for (size_t a = 0; a < pk.size(); a++)
     for (size_t c = 0; c < pk.size(); c++) {
@@ -432,19 +494,18 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
       sd.score = pk[a].score + pk[c].score;
       sides.push_back(sd);
     }
-  std::sort(sides.begin(), sides.end(), [](const Side &a, const Side &c) { return a.score > c.score; });
+  std::sort(sides.begin(), sides.end(), [](const Side &a, const Side &c) {
+    if (a.score != c.score) return a.score > c.score;
+    if (a.th != c.th) return a.th < c.th;
+    if (a.ro != c.ro) return a.ro < c.ro;
+    return a.ri < c.ri;
+  });
   if (sides.size() > 40) sides.resize(40);
-  if (sides.size() < 4) return;
-  // four sides a quarter turn apart (within 30 degrees) round a convex quad that holds the others' midlines' corners
+  if (sides.size() < 4) {
+    j.dead = true;
+    return;
+  }
   auto mid = [](const Side &sd) { return 0.5 * (sd.ro + sd.ri); };
-  auto meet = [&](const Side &p, const Side &q, double *x, double *y) {
-    const double det = p.nx * q.ny - p.ny * q.nx;
-    if (std::fabs(det) < 1e-6) return false;
-    const double a = mid(p), c = mid(q);
-    *x = cx + (a * q.ny - c * p.ny) / det;
-    *y = cy + (p.nx * c - q.nx * a) / det;
-    return true;
-  };
   double best = -1;
   int pick[4] = {0, 0, 0, 0};
   const int m = static_cast<int>(sides.size());
@@ -467,7 +528,7 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
           }
           if (!ok || tmax > 3.0 * tmin) continue;
           double qx[4], qy[4];
-          for (int k = 0; k < 4 && ok; k++) ok = meet(sides[id[k]], sides[id[(k + 1) & 3]], &qx[k], &qy[k]);
+          for (int k = 0; k < 4 && ok; k++) ok = meet_sides(sides[id[k]], sides[id[(k + 1) & 3]], cx, cy, &qx[k], &qy[k]);
           if (!ok) continue;
           const double mx = 0.25 * (qx[0] + qx[1] + qx[2] + qx[3]), my = 0.25 * (qy[0] + qy[1] + qy[2] + qy[3]);
           double smin = 1e9;
@@ -480,119 +541,102 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
           best = sc;
           std::memcpy(pick, id, sizeof pick);
         }
-  lap(2);
-  if (best < 0) return;
+  if (best < 0) {
+    j.dead = true;
+    return;
+  }
   // labels: the side whose outward normal is nearest straight up is side 0 (top), then clockwise
-  Side S4[4];
-  {
-    int top = 0;
-    double bd = 1e9;
-    for (int k = 0; k < 4; k++) {
-      const double d = std::fabs(wrap_pi(sides[pick[k]].th + PI / 2));
-      if (d < bd) {
-        bd = d;
-        top = k;
-      }
-    }
-    for (int k = 0; k < 4; k++) S4[k] = sides[pick[(top + k) & 3]];
-  }
-  // corners of the midlines, TL = top & left, TR = top & right, BR, BL
-  double Cx[4], Cy[4];
-  meet(S4[0], S4[3], &Cx[0], &Cy[0]);
-  meet(S4[0], S4[1], &Cx[1], &Cy[1]);
-  meet(S4[2], S4[1], &Cx[2], &Cy[2]);
-  meet(S4[2], S4[3], &Cx[3], &Cy[3]);
-
-  b.probes.reserve(FIND_MAX_PROBES * sizeof(FindProbe));
-  b.edges.reserve(FIND_MAX_PROBES * sizeof(FindEdge));
-  std::vector<FindProbe> pr;
-  std::vector<FindEdge> ed;
-  auto run_probes = [&](float reach) {
-    const int n = std::min(static_cast<int>(pr.size()), FIND_MAX_PROBES);
-    if (n <= 0) return;
-    hip_check(hipMemcpyAsync(b.probes.p, pr.data(), n * sizeof(FindProbe), hipMemcpyHostToDevice, s), "probes");
-    launch_find_probe(d_img, w, h, d_stride, b.probes.as<FindProbe>(), n, reach, b.edges.as<FindEdge>(), s);
-    hip_check(hipGetLastError(), "probe launch");
-    ed.resize(n);
-    hip_check(hipMemcpyAsync(ed.data(), b.edges.p, n * sizeof(FindEdge), hipMemcpyDeviceToHost, s), "edges");
-    hip_check(hipStreamSynchronize(s), "probe sync");
-  };
-
-  // pass 1: probes along each side's straight edges between the corners, a robust quadratic through each edge
-  std::vector<Pt> pts;
-  const int ends[4][2] = {{0, 1}, {1, 2}, {3, 2}, {0, 3}};  // the side's corners (top TL-TR, right TR-BR, ...)
-  double thick = 0;
-  for (int sd = 0; sd < 4; sd++) thick += 0.25 * (S4[sd].ro - S4[sd].ri);
-  for (int pass = 0; pass < 2; pass++) {
-    pr.clear();
-    std::vector<int> lab;
-    for (int sd = 0; sd < 4; sd++) {
-      const Side &q = S4[sd];
-      const double x0 = Cx[ends[sd][0]], y0 = Cy[ends[sd][0]], x1 = Cx[ends[sd][1]], y1 = Cy[ends[sd][1]];
-      const double len = std::hypot(x1 - x0, y1 - y0), tx = (x1 - x0) / len, ty = (y1 - y0) / len;
-      const double margin = std::max(4.0 * thick, 8.0);
-      const int cnt = std::max(0, std::min(4000, static_cast<int>((len - 2 * margin) / 1.5)));
-      for (int e = 0; e < 2; e++) {
-        const double half = 0.5 * (q.ro - q.ri) * (e == 0 ? 1 : -1);
-        for (int k = 0; k < cnt; k++) {
-          const double t = margin + (len - 2 * margin) * (k + 0.5) / cnt;
-          FindProbe p;
-          p.x = static_cast<float>(x0 + t * tx + half * q.nx);
-          p.y = static_cast<float>(y0 + t * ty + half * q.ny);
-          p.nx = static_cast<float>(e == 0 ? q.nx : -q.nx);
-          p.ny = static_cast<float>(e == 0 ? q.ny : -q.ny);
-          pr.push_back(p);
-          lab.push_back(sd * 2 + e);
-        }
-      }
-    }
-    run_probes(static_cast<float>(std::min(6.0, std::max(1.0, 0.75 * thick))));
-    // per edge, the robust quadratic; the side's straight line moved to the edges' mean for the next pass
-    pts.clear();
-    for (int sd = 0; sd < 4; sd++)
-      for (int e = 0; e < 2; e++) {
-        std::vector<Pt> ep;
-        for (size_t k = 0; k < ed.size(); k++)
-          if (lab[k] == sd * 2 + e && ed[k].w > 0) ep.push_back(Pt{ed[k].x, ed[k].y, ed[k].w, sd, e ? 3 : 1});
-        const double x0 = Cx[ends[sd][0]], y0 = Cy[ends[sd][0]], x1 = Cx[ends[sd][1]], y1 = Cy[ends[sd][1]];
-        const double len = std::hypot(x1 - x0, y1 - y0);
-        robust_edge(ep, x0, y0, (x1 - x0) / len, (y1 - y0) / len);
-        if (pass == 0 && !ep.empty()) {
-          double so = 0, sw = 0;
-          for (const Pt &p : ep) {
-            so += p.w * (S4[sd].nx * (p.x - cx) + S4[sd].ny * (p.y - cy));
-            sw += p.w;
-          }
-          if (e == 0) S4[sd].ro = so / sw;
-          else S4[sd].ri = so / sw;
-        }
-        pts.insert(pts.end(), ep.begin(), ep.end());
-      }
-    if (pass == 0) {
-      meet(S4[0], S4[3], &Cx[0], &Cy[0]);
-      meet(S4[0], S4[1], &Cx[1], &Cy[1]);
-      meet(S4[2], S4[1], &Cx[2], &Cy[2]);
-      meet(S4[2], S4[3], &Cx[3], &Cy[3]);
+  int top = 0;
+  double bd = 1e9;
+  for (int k = 0; k < 4; k++) {
+    const double d = std::fabs(wrap_pi(sides[pick[k]].th + PI / 2));
+    if (d < bd) {
+      bd = d;
+      top = k;
     }
   }
-  lap(3);
-  if (pts.size() < 64) return;
+  for (int k = 0; k < 4; k++) j.S4[k] = sides[pick[(top + k) & 3]];
+  corners(j);  // the midlines' corners: TL = top & left, TR = top & right, BR, BL
+  j.thick = 0;
+  for (int sd = 0; sd < 4; sd++) j.thick += 0.25 * (j.S4[sd].ro - j.S4[sd].ri);
+}
 
-  // the fit for each ring, the lens by golden section; then the rotation by the track
-  const Img im{img, w, h, stride};
-  Fit fit{cx, cy, 0.5 * std::max(w, h)};
-  Lens lens{cx, cy, std::hypot(cx, cy), 0.0};
+// probes along each side's straight edges between the corners
+void edge_probes(FindJob &j) {
+  j.pr.clear();
+  j.lab1.clear();
+  const float reach = static_cast<float>(std::min(6.0, std::max(1.0, 0.75 * j.thick)));
+  for (int sd = 0; sd < 4; sd++) {
+    const Side &q = j.S4[sd];
+    const double x0 = j.Cx[ENDS[sd][0]], y0 = j.Cy[ENDS[sd][0]], x1 = j.Cx[ENDS[sd][1]], y1 = j.Cy[ENDS[sd][1]];
+    const double len = std::hypot(x1 - x0, y1 - y0), tx = (x1 - x0) / len, ty = (y1 - y0) / len;
+    const double margin = std::max(4.0 * j.thick, 8.0);
+    const int cnt = std::max(0, std::min(4000, static_cast<int>((len - 2 * margin) / 1.5)));
+    for (int e = 0; e < 2; e++) {
+      const double half = 0.5 * (q.ro - q.ri) * (e == 0 ? 1 : -1);
+      for (int k = 0; k < cnt; k++) {
+        const double t = margin + (len - 2 * margin) * (k + 0.5) / cnt;
+        FindProbe p;
+        p.x = static_cast<float>(x0 + t * tx + half * q.nx);
+        p.y = static_cast<float>(y0 + t * ty + half * q.ny);
+        p.nx = static_cast<float>(e == 0 ? q.nx : -q.nx);
+        p.ny = static_cast<float>(e == 0 ? q.ny : -q.ny);
+        p.reach = reach;
+        p.frame = j.frame;
+        j.pr.push_back(p);
+        j.lab1.push_back(sd * 2 + e);
+      }
+    }
+  }
+}
+
+// per edge, a robust quadratic through its points; after the first pass the sides' straight lines move to their
+// edges' means and the corners with them
+void edge_fits(FindJob &j, const FindEdge *ed, int pass) {
+  j.pts.clear();
+  const int n = static_cast<int>(j.lab1.size());
+  for (int sd = 0; sd < 4; sd++)
+    for (int e = 0; e < 2; e++) {
+      std::vector<Pt> ep;
+      for (int k = 0; k < n; k++)
+        if (j.lab1[k] == sd * 2 + e && ed[k].w > 0) ep.push_back(Pt{ed[k].x, ed[k].y, ed[k].w, sd, e ? 3 : 1});
+      const double x0 = j.Cx[ENDS[sd][0]], y0 = j.Cy[ENDS[sd][0]], x1 = j.Cx[ENDS[sd][1]], y1 = j.Cy[ENDS[sd][1]];
+      const double len = std::hypot(x1 - x0, y1 - y0);
+      robust_edge(ep, x0, y0, (x1 - x0) / len, (y1 - y0) / len);
+      if (pass == 0 && !ep.empty()) {
+        double so = 0, sw = 0;
+        for (const Pt &p : ep) {
+          so += p.w * (j.S4[sd].nx * (p.x - j.cx) + j.S4[sd].ny * (p.y - j.cy));
+          sw += p.w;
+        }
+        if (e == 0) j.S4[sd].ro = so / sw;
+        else j.S4[sd].ri = so / sw;
+      }
+      j.pts.insert(j.pts.end(), ep.begin(), ep.end());
+    }
+  if (pass == 0) corners(j);
+}
+
+// the fit for each ring (in the order the line's thickness, 2 modules, against the side's length, L - 4 modules
+// between the midlines, suggests), the lens within it; the rotation by the track
+void stage_rings(FindJob &j) {
+  if (j.pts.size() < 64) {
+    j.dead = true;
+    return;
+  }
+  const Img im{j.img, j.w, j.h, j.stride};
+  j.fit = Fit{j.cx, j.cy, 0.5 * std::max(j.w, j.h)};
+  j.lens = Lens{j.cx, j.cy, std::hypot(j.cx, j.cy), 0.0};
   double bestH[9] = {}, bestk1 = 0, bestTrack = -1;
   int bestL = 0;
   std::vector<Pt> sub;  // every fourth point for the search
-  for (size_t k = 0; k < pts.size(); k += 4) sub.push_back(pts[k]);
-  // rings in the order the line's thickness (2 modules) against the side's length (L - 4 modules between the
-  // midlines) suggests
+  for (size_t k = 0; k < j.pts.size(); k += 4) sub.push_back(j.pts[k]);
   int order[3] = {0, 1, 2};
   {
+    const double *Cx = j.Cx, *Cy = j.Cy;
     const double side = 0.25 * (std::hypot(Cx[1] - Cx[0], Cy[1] - Cy[0]) + std::hypot(Cx[2] - Cx[1], Cy[2] - Cy[1]) +
                                 std::hypot(Cx[3] - Cx[2], Cy[3] - Cy[2]) + std::hypot(Cx[0] - Cx[3], Cy[0] - Cy[3]));
-    const double modules = thick > 0 ? 2.0 * side / thick + 4.0 : 0.0;
+    const double modules = j.thick > 0 ? 2.0 * side / j.thick + 4.0 : 0.0;
     std::sort(order, order + 3, [&](int a, int c) {
       return std::fabs(std::log((RINGS[a] + 30) / modules)) < std::fabs(std::log((RINGS[c] + 30) / modules));
     });
@@ -600,9 +644,9 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
   for (int oi = 0; oi < 3; oi++) {
     if (bestTrack >= 0.7) break;  // a clear track under the likelier ring
     const int L = RINGS[order[oi]] + 30;
-    Lens ln = lens;
+    Lens ln = j.lens;
     double H[9];
-    const double rms = fit.solve(sub, ln, L, 1.0, 6, H, &ln.k1, nullptr);
+    const double rms = j.fit.solve(sub, ln, L, 1.0, 6, H, &ln.k1, nullptr);
     if (!(rms < 0.5)) continue;
     // the four rotations about the symbol's centre: module (x, y) -> (L - y, x) a quarter turn
     double Rq[9] = {0, -1, static_cast<double>(L), 1, 0, 0, 0, 0, 1}, Rk[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
@@ -623,61 +667,259 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
       std::memcpy(Rk, T, sizeof T);
     }
   }
-  lap(4);
-  if (bestTrack < 0.4) return;
-  const int L = bestL;
-
-  // refine: probes from the model at every module along both edges of every side, and the fit again, twice
-  lens.k1 = bestk1;
-  std::memcpy(r.H, bestH, sizeof bestH);
-  for (int pass = 0; pass < 2; pass++) {
-    pr.clear();
-    std::vector<Pt> lab;
-    const double mod = std::hypot(r.H[0], r.H[3]);  // pixels a module near the origin, for the reach
-    for (int sd = 0; sd < 4; sd++)
-      for (int e = 0; e < 2; e++) {
-        const int depth = e ? 3 : 1;
-        for (double t = 3.0; t <= L - 3.0; t += 1.0) {
-          double mx, my, x, y, ax, ay, bx, by;
-          side_xy(L, sd, t, depth, &mx, &my);
-          map_pt(r.H, lens, mx, my, &x, &y);
-          // rising intensity: outward at depth 1 (light module outside), inward at depth 3 (the gap)
-          side_xy(L, sd, t, depth - 0.25, &mx, &my);
-          map_pt(r.H, lens, mx, my, &ax, &ay);
-          side_xy(L, sd, t, depth + 0.25, &mx, &my);
-          map_pt(r.H, lens, mx, my, &bx, &by);
-          const double nx = e ? bx - ax : ax - bx, ny = e ? by - ay : ay - by, nn = std::hypot(nx, ny);
-          if (!(nn > 0)) continue;
-          pr.push_back(FindProbe{static_cast<float>(x), static_cast<float>(y), static_cast<float>(nx / nn),
-                                 static_cast<float>(ny / nn)});
-          lab.push_back(Pt{0, 0, 0, sd, depth});
-        }
-      }
-    run_probes(static_cast<float>(std::min(3.0, std::max(0.75, (pass ? 0.6 : 0.9) * mod))));
-    pts.clear();
-    for (size_t k = 0; k < ed.size(); k++)
-      if (ed[k].w > 0) pts.push_back(Pt{ed[k].x, ed[k].y, ed[k].w, lab[k].side, lab[k].depth});
-    double H[9], k1 = lens.k1;
-    int used = 0;
-    const double rms = fit.solve(pts, lens, L, 0.3, 5, H, &k1, &used);
-    if (!(rms < 0.3)) return;
-    std::memcpy(r.H, H, sizeof H);
-    lens.k1 = k1;
-    r.k1 = k1;
-    r.rms = static_cast<float>(rms);
-    r.points = used;
+  if (bestTrack < 0.4) {
+    j.dead = true;
+    return;
   }
-  lap(5);
+  j.L = bestL;
+  j.lens.k1 = bestk1;
+  std::memcpy(j.r->H, bestH, sizeof bestH);
+}
+
+// probes from the model at every module along both edges of every side
+void model_probes(FindJob &j, int pass) {
+  j.pr.clear();
+  j.lab2.clear();
+  const double *H = j.r->H;
+  const int L = j.L;
+  const double mod = std::hypot(H[0], H[3]);  // pixels a module near the origin, for the reach
+  const float reach = static_cast<float>(std::min(3.0, std::max(0.75, (pass ? 0.6 : 0.9) * mod)));
+  for (int sd = 0; sd < 4; sd++)
+    for (int e = 0; e < 2; e++) {
+      const int depth = e ? 3 : 1;
+      for (double t = 3.0; t <= L - 3.0; t += 1.0) {
+        double mx, my, x, y, ax, ay, bx, by;
+        side_xy(L, sd, t, depth, &mx, &my);
+        map_pt(H, j.lens, mx, my, &x, &y);
+        // rising intensity: outward at depth 1 (light module outside), inward at depth 3 (the gap)
+        side_xy(L, sd, t, depth - 0.25, &mx, &my);
+        map_pt(H, j.lens, mx, my, &ax, &ay);
+        side_xy(L, sd, t, depth + 0.25, &mx, &my);
+        map_pt(H, j.lens, mx, my, &bx, &by);
+        const double nx = e ? bx - ax : ax - bx, ny = e ? by - ay : ay - by, nn = std::hypot(nx, ny);
+        if (!(nn > 0)) continue;
+        j.pr.push_back(FindProbe{static_cast<float>(x), static_cast<float>(y), static_cast<float>(nx / nn),
+                                 static_cast<float>(ny / nn), reach, j.frame});
+        j.lab2.push_back(Pt{0, 0, 0, sd, depth});
+      }
+    }
+}
+
+// the homography and lens again through the model's probes
+void model_fit(FindJob &j, const FindEdge *ed) {
+  j.pts.clear();
+  const int n = static_cast<int>(j.lab2.size());
+  for (int k = 0; k < n; k++)
+    if (ed[k].w > 0) j.pts.push_back(Pt{ed[k].x, ed[k].y, ed[k].w, j.lab2[k].side, j.lab2[k].depth});
+  double H[9], k1 = j.lens.k1;
+  int used = 0;
+  const double rms = j.fit.solve(j.pts, j.lens, j.L, 0.3, 5, H, &k1, &used);
+  if (!(rms < 0.3)) {
+    j.dead = true;
+    return;
+  }
+  FindResult &r = *j.r;
+  std::memcpy(r.H, H, sizeof H);
+  j.lens.k1 = k1;
+  r.k1 = k1;
+  r.rms = static_cast<float>(rms);
+  r.points = used;
+}
+
+void stage_finish(FindJob &j) {
+  FindResult &r = *j.r;
+  const int L = j.L;
   r.span = L - 30;
   r.found = 1;
-  r.track = static_cast<float>(track_score(im, r.H, lens, L, &r.contrast));
+  r.track = static_cast<float>(track_score(Img{j.img, j.w, j.h, j.stride}, r.H, j.lens, L, &r.contrast));
   const double cm[4][2] = {{1, 1}, {L - 1.0, 1}, {L - 1.0, L - 1.0}, {1, L - 1.0}};
   for (int c = 0; c < 4; c++) {
     double x, y;
-    map_pt(r.H, lens, cm[c][0], cm[c][1], &x, &y);
+    map_pt(r.H, j.lens, cm[c][0], cm[c][1], &x, &y);
     r.quad[2 * c] = static_cast<float>(x);
     r.quad[2 * c + 1] = static_cast<float>(y);
   }
+}
+
+// The Hough stage's buffers for F frames and the trig table.
+int hough_setup(FinderBufs &b, int F, int w, int h, hipStream_t s, size_t *cells_out) {
+  const int R = static_cast<int>(std::ceil(0.5 * std::hypot(w, h))) + 2, nr = 2 * R + 1;
+  const size_t cells = static_cast<size_t>(FIND_THETA) * nr;
+  b.acc.reserve(static_cast<size_t>(F) * cells * sizeof(float));
+  b.max.reserve(static_cast<size_t>(F) * sizeof(unsigned) + 256);
+  b.count.reserve(static_cast<size_t>(F) * sizeof(int) + 256);
+  b.peaks.reserve(static_cast<size_t>(F) * FIND_MAX_PEAKS * sizeof(FindPeak));
+  if (b.trig_nr != 1) {
+    std::vector<float> tr(2 * FIND_THETA);
+    for (int t = 0; t < FIND_THETA; t++) {
+      tr[2 * t] = static_cast<float>(std::cos(2 * PI * t / FIND_THETA));
+      tr[2 * t + 1] = static_cast<float>(std::sin(2 * PI * t / FIND_THETA));
+    }
+    b.trig.reserve(tr.size() * sizeof(float));
+    hip_check(hipStreamSynchronize(s), "finder stream");
+    hip_check(hipMemcpy(b.trig.p, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice), "trig");
+    b.trig_nr = 1;
+  }
+  *cells_out = cells;
+  return nr;
+}
+
+// A frame's probes on its lane: one launch, its edges back into `ed`.
+void run_probes(FinderLane &ln, const uint8_t *const *d_tab, int w, int h, const FindJob &j, std::vector<FindEdge> &ed) {
+  const size_t n = j.pr.size();
+  ed.resize(n);
+  if (!n) return;
+  ln.h_probes.reserve(n * sizeof(FindProbe));
+  ln.h_edges.reserve(n * sizeof(FindEdge));
+  ln.probes.reserve(n * sizeof(FindProbe));
+  ln.edges.reserve(n * sizeof(FindEdge));
+  std::copy(j.pr.begin(), j.pr.end(), ln.h_probes.as<FindProbe>());
+  hip_check(hipMemcpyAsync(ln.probes.p, ln.h_probes.p, n * sizeof(FindProbe), hipMemcpyHostToDevice, ln.s), "probes");
+  launch_find_probe(d_tab, w, h, w, ln.probes.as<FindProbe>(), static_cast<int>(n), ln.edges.as<FindEdge>(), ln.s);
+  hip_check(hipGetLastError(), "probe launch");
+  hip_check(hipMemcpyAsync(ln.h_edges.p, ln.edges.p, n * sizeof(FindEdge), hipMemcpyDeviceToHost, ln.s), "edges");
+  hip_check(hipStreamSynchronize(ln.s), "probe sync");
+  std::memcpy(ed.data(), ln.h_edges.p, n * sizeof(FindEdge));
+}
+
+// A frame from its peaks to its registration; the stage times (host clock) into its result.
+void run_frame(FindJob &j, FinderLane &ln, const uint8_t *const *d_tab) {
+  auto t0 = std::chrono::steady_clock::now();
+  auto lap = [&](int k) {
+    const auto t1 = std::chrono::steady_clock::now();
+    j.r->ms[k] = std::chrono::duration<float, std::milli>(t1 - t0).count();
+    t0 = t1;
+  };
+  std::vector<FindEdge> ed;
+  stage_sides(j);
+  lap(2);
+  for (int pass = 0; pass < 2 && !j.dead; pass++) {
+    edge_probes(j);
+    run_probes(ln, d_tab, j.w, j.h, j, ed);
+    edge_fits(j, ed.data(), pass);
+  }
+  lap(3);
+  if (!j.dead) stage_rings(j);
+  lap(4);
+  for (int pass = 0; pass < 2 && !j.dead; pass++) {
+    model_probes(j, pass);
+    run_probes(ln, d_tab, j.w, j.h, j, ed);
+    model_fit(j, ed.data());
+  }
+  lap(5);
+  if (!j.dead) stage_finish(j);
+}
+
+}  // namespace
+
+namespace {
+
+// A frame's vote and peaks on its lane, then the rest of its search.
+void search_frame(FinderBufs &b, FinderLane &ln, FindJob &j, const uint8_t *d_img, const uint8_t *const *d_tab, int nr,
+                  size_t cells) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int f = j.frame;
+  float *acc = b.acc.as<float>() + f * cells;
+  launch_find_zero(acc, cells, b.max.as<unsigned>() + f, b.count.as<int>() + f, ln.s);
+  launch_find_vote(d_img, j.w, j.h, j.w, 12.0f, b.trig.as<float2>(), nr, acc, ln.s);
+  launch_find_peaks(acc, nr, 0.08f, b.max.as<unsigned>() + f, b.peaks.as<FindPeak>() + f * FIND_MAX_PEAKS,
+                    b.count.as<int>() + f, ln.s);
+  hip_check(hipGetLastError(), "finder launches");
+  ln.h_peaks.reserve(FIND_MAX_PEAKS * sizeof(FindPeak) + sizeof(int));
+  FindPeak *hp = ln.h_peaks.as<FindPeak>();
+  int *hn = reinterpret_cast<int *>(hp + FIND_MAX_PEAKS);
+  hip_check(hipMemcpyAsync(hn, b.count.as<int>() + f, sizeof(int), hipMemcpyDeviceToHost, ln.s), "peak count");
+  hip_check(hipMemcpyAsync(hp, b.peaks.as<FindPeak>() + f * FIND_MAX_PEAKS, FIND_MAX_PEAKS * sizeof(FindPeak),
+                           hipMemcpyDeviceToHost, ln.s), "peaks");
+  hip_check(hipStreamSynchronize(ln.s), "finder peaks");
+  j.pk.assign(hp, hp + std::min(*hn, FIND_MAX_PEAKS));
+  j.r->ms[1] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  run_frame(j, ln, d_tab);
+}
+
+// The search of frames whose device images are in place (upload false) or are uploaded here on their lanes.
+void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
+            hipStream_t s, FindResult *res, bool upload) {
+  const float up = res[0].ms[0];
+  for (int f = 0; f < F; f++) res[f] = FindResult{};
+  res[0].ms[0] = up;
+  size_t cells = 0;
+  const int nr = hough_setup(b, F, w, h, s, &cells);
+  const size_t frame_bytes = static_cast<size_t>(w) * h;
+  std::vector<const uint8_t *> tab(F);
+  for (int f = 0; f < F; f++) tab[f] = d_imgs + f * frame_bytes;
+  b.imgtab.reserve(F * sizeof(void *));
+  hip_check(hipMemcpyAsync(b.imgtab.p, tab.data(), F * sizeof(void *), hipMemcpyHostToDevice, s), "image table");
+  hip_check(hipStreamSynchronize(s), "image table");
+  const uint8_t *const *d_tab = b.imgtab.as<const uint8_t *>();
+  std::vector<FindJob> jobs(F);
+  for (int f = 0; f < F; f++) {
+    FindJob &j = jobs[f];
+    j.img = imgs[f];
+    j.w = w;
+    j.h = h;
+    j.stride = stride;
+    j.frame = f;
+    j.cx = 0.5 * w;
+    j.cy = 0.5 * h;
+    j.r = &res[f];
+  }
+  auto one = [&](int f, FinderLane &ln) {
+    uint8_t *dst = d_imgs + f * frame_bytes;
+    if (upload) {
+      const auto t0 = std::chrono::steady_clock::now();
+      if (stride == w) {
+        hip_check(hipMemcpyAsync(dst, imgs[f], frame_bytes, hipMemcpyHostToDevice, ln.s), "capture");
+      } else {
+        hip_check(hipMemcpy2DAsync(dst, w, imgs[f], stride, w, h, hipMemcpyHostToDevice, ln.s), "capture");
+      }
+      hip_check(hipStreamSynchronize(ln.s), "capture");
+      res[f].ms[0] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    search_frame(b, ln, jobs[f], dst, d_tab, nr, cells);
+  };
+  if (F == 1) {
+    b.main.s = s;
+    one(0, b.main);
+    return;
+  }
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (!b.pool) {
+    const int nt = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    b.pool = std::make_shared<FinderPool>(nt);
+    for (int t = 0; t < nt; t++) {
+      auto ln = std::make_unique<FinderLane>();
+      hip_check(hipStreamCreateWithFlags(&ln->s, hipStreamNonBlocking), "finder lane stream");
+      ln->own = true;
+      b.lanes.push_back(std::move(ln));
+    }
+  }
+  std::mutex em;
+  std::exception_ptr err;  // the first frame's failure, raised here on the caller's thread
+  b.pool->run(F, [&](int f, int t) {
+    try {
+      hip_check(hipSetDevice(dev), "hipSetDevice");
+      one(f, *b.lanes[t]);
+    } catch (...) {
+      std::lock_guard<std::mutex> lk(em);
+      if (!err) err = std::current_exception();
+    }
+  });
+  if (err) std::rethrow_exception(err);
+}
+
+}  // namespace
+
+void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
+                  hipStream_t s, FindResult *res) {
+  search(b, F, d_imgs, imgs, w, h, stride, s, res, true);
+}
+
+void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w, int h, int stride, hipStream_t s,
+                 FindResult &r) {
+  const uint8_t *imgs[1] = {img};
+  search(b, 1, const_cast<uint8_t *>(d_img), imgs, w, h, stride, s, &r, false);
 }
 
 }  // namespace lz

@@ -135,11 +135,13 @@ __device__ __forceinline__ float grey_at(const uint8_t *img, int w, int h, int s
 
 // Per probe: the profile along the normal every quarter pixel over +-reach, its central differences, and the rising
 // edge (a local maximum of the difference at 0.3 of the window's largest or more) nearest the probe's point.
-__global__ __launch_bounds__(256) void k_find_probe(const uint8_t *img, int w, int h, int stride,
-                                                    const FindProbe *probes, int n, float reach, FindEdge *out) {
+__global__ __launch_bounds__(256) void k_find_probe(const uint8_t *const *imgs, int w, int h, int stride,
+                                                    const FindProbe *probes, int n, FindEdge *out) {
   const int k = blockIdx.x * 256 + threadIdx.x;
   if (k >= n) return;
   const FindProbe P = probes[k];
+  const uint8_t *img = imgs[P.frame];
+  const float reach = fminf(fmaxf(P.reach, 0.25f), 12.0f);
   const float step = 0.25f;
   const int ns = 2 * static_cast<int>(reach / step) + 1;
   auto val = [&](int i) {
@@ -206,9 +208,9 @@ void launch_find_peaks(const float *acc, int nr, float frac, unsigned *d_max, Fi
                      d_max, peaks, count);
 }
 
-void launch_find_probe(const uint8_t *img, int w, int h, int stride, const FindProbe *probes, int n, float reach,
+void launch_find_probe(const uint8_t *const *imgs, int w, int h, int stride, const FindProbe *probes, int n,
                        FindEdge *out, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_find_probe, dim3((n + 255) / 256), dim3(256), 0, s, img, w, h, stride, probes, n, reach, out);
+  if (n > 0) hipLaunchKernelGGL(k_find_probe, dim3((n + 255) / 256), dim3(256), 0, s, imgs, w, h, stride, probes, n, out);
 }
 
 }  // namespace lz

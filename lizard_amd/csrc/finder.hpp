@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <vector>
 
 #include "common.hpp"
@@ -30,6 +31,8 @@ struct FindPeak {
 };
 struct FindProbe {
   float x, y, nx, ny;  // the point and the unit normal, pointing from dark to light
+  float reach;         // how far along the normal to look, either way (pixels)
+  int frame;           // which frame of a batch (its image in the launch's table)
 };
 struct FindEdge {
   float x, y, w, off;  // the edge's point, its gradient (grey levels a pixel; 0 none), its offset along the normal
@@ -39,7 +42,7 @@ void launch_find_vote(const uint8_t *img, int w, int h, int stride, float thr, c
                       hipStream_t s);
 void launch_find_peaks(const float *acc, int nr, float frac, unsigned *d_max, FindPeak *peaks, int *count,
                        hipStream_t s);
-void launch_find_probe(const uint8_t *img, int w, int h, int stride, const FindProbe *probes, int n, float reach,
+void launch_find_probe(const uint8_t *const *imgs, int w, int h, int stride, const FindProbe *probes, int n,
                        FindEdge *out, hipStream_t s);
 void launch_find_zero(float *acc, size_t count, unsigned *d_max, int *count_out, hipStream_t s);
 
@@ -56,14 +59,62 @@ struct FindResult {
   float ms[6] = {};  // host clock: upload, vote + peaks, sides, first probes, rings, refinement
 };
 
+struct FinderPool;  // host threads for a batch's per-frame geometry
+
+// Pinned host memory (hipHostMalloc) that grows as asked, for the copies between the stages.
+struct HostBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  HostBuf() = default;
+  HostBuf(const HostBuf &) = delete;
+  HostBuf &operator=(const HostBuf &) = delete;
+  ~HostBuf() {
+    if (p) (void)hipHostFree(p);
+  }
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (p) hip_check(hipHostFree(p), "hipHostFree");
+    p = nullptr;
+    bytes = 0;
+    hip_check(hipHostMalloc(&p, n, hipHostMallocDefault), "hipHostMalloc");
+    bytes = n;
+  }
+  template <class T>
+  T *as() const {
+    return static_cast<T *>(p);
+  }
+};
+
+// One host thread's probe buffers and stream, so that a frame's stages run without waiting on the other frames.
+struct FinderLane {
+  hipStream_t s = nullptr;
+  bool own = false;  // the stream was made for this lane
+  DevBuf probes, edges;
+  HostBuf h_probes, h_edges, h_peaks;
+  FinderLane() = default;
+  FinderLane(const FinderLane &) = delete;
+  FinderLane &operator=(const FinderLane &) = delete;
+  ~FinderLane() {
+    if (own && s) (void)hipStreamDestroy(s);
+  }
+};
+
 struct FinderBufs {
-  DevBuf acc, trig, max, peaks, count, probes, edges, img;
+  DevBuf acc, trig, max, peaks, count, img, imgtab;
+  FinderLane main;  // the caller's stream, for a single frame
   int trig_nr = 0;
+  std::shared_ptr<FinderPool> pool;
+  std::vector<std::unique_ptr<FinderLane>> lanes;  // one a pool thread
 };
 
 // The whole search on a capture already on the device (d_img, rows w bytes apart) with its host copy (img, rows
 // stride bytes apart) for the host's reads.
 void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w, int h, int stride, hipStream_t s,
                  FindResult &r);
+// The same for F frames of one size at once (d_imgs: room for F device images, rows w apart, each w h bytes apart in
+// one buffer; imgs: the host frames, rows stride apart), each frame uploaded into its place and searched on a host
+// thread of its own with a stream of its own. Each frame's result is the one find_symbol gives it.
+void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
+                  hipStream_t s, FindResult *r);
 
 }  // namespace lz

@@ -189,3 +189,33 @@ def test_finder_against_reference_finder_on_random_hard_captures(decoder_gpu):
             err_ref.append(node_err(reg[0], reg[1], Ht, nt, node_x)[0])
     assert mine >= theirs and mine >= 22, (mine, theirs)
     assert np.median(err_mine) <= np.median(err_ref), (np.median(err_mine), np.median(err_ref))
+
+
+def test_find_batch_equals_single(decoder_gpu):
+    """lizx_find_batch over mixed frames (three rings, rotations, lenses, a frame with no symbol) gives each frame what
+    lizx_find gives it alone, bit for bit: the stages are the same code, their launches only shared."""
+    rng = np.random.default_rng(77)
+    imgs = []
+    for i, (version, span) in enumerate([(5, 128), (20, 256), (64, 512), (45, 512), (8, 256), (20, 256)]):
+        blocks = sim.random_blocks(version, rng)
+        cam = sim.Camera(1920, 1080, rot_deg=float(rng.uniform(0, 360)), scale=float(rng.uniform(0.65, 0.9)),
+                         persp=float(rng.uniform(0, 0.04)), k1=float(rng.uniform(-0.04, 0.04)),
+                         blur=float(rng.uniform(0.6, 1.2)), noise=float(rng.uniform(2, 6)))
+        dpic = decoder_gpu.modulate(version, blocks)
+        dimg = decoder_gpu.capture(version, span, dpic, 1920, 1080, cam.H(span), k1=cam.k1, blur=cam.blur,
+                                   noise=cam.noise, seed=i, fps=30)
+        imgs.append(dimg.get(np.uint8, 1920 * 1080).reshape(1080, 1920))
+        dimg.free()
+        dpic.free()
+    imgs.insert(3, rng.integers(0, 256, (1080, 1920)).astype(np.uint8))  # nothing to find
+    imgs.append(np.full((1080, 1920), 255, np.uint8))
+    batch = decoder_gpu.find_batch(imgs)
+    assert len(batch) == len(imgs)
+    for i, img in enumerate(imgs):
+        one = decoder_gpu.find(img)
+        if one is None:
+            assert batch[i] is None, i
+            continue
+        assert batch[i] is not None and batch[i]["span"] == one["span"], i
+        assert np.array_equal(batch[i]["H"], one["H"]) and np.array_equal(batch[i]["nodes"], one["nodes"]), i
+    assert sum(b is not None for b in batch) == 6
