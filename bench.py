@@ -1075,8 +1075,8 @@ def main():
                     help="run one workload alone (for per-kernel profiles); the JSON line then carries only it")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the path shards by frame/codeword with no exchange (see "
-              "DESIGN.md), and no figure at several GPUs is produced", file=sys.stderr)
+        print("bench.py: measures one GPU only; the path shards by frame and codeword with no exchange between "
+              "GPUs, and no figure at several GPUs is produced", file=sys.stderr)
         sys.exit(2)
     if lizard_amd.device_count() < 1:
         print("bench.py: no GPU", file=sys.stderr)
