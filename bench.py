@@ -741,11 +741,9 @@ def bench_finder(args):
 
     def blind_batch(native):
         with cf.ThreadPoolExecutor(nthreads) as ex:
-            if native:  # one lizx_find_batch over the frames, then the words on the threads
-                regs = finder.find_batch(bimgs)
-                words = list(ex.map(lambda i: None if regs[i] is None else lizard_amd.read_word(
-                    regs[i]["span"], bimgs[i], regs[i]["H"], regs[i]["nodes"]), range(F)))
-                found = [None if regs[i] is None else (regs[i], words[i]) for i in range(F)]
+            if native:  # one lizx_find_batch over the frames, each frame's word read by the thread that found it
+                regs = finder.find_batch(bimgs, words=True)
+                found = [None if r is None or r["version"] is None else (r, (r["version"],)) for r in regs]
             else:  # lizx_find and the word on the threads, a decoder each
                 found = list(ex.map(find_word, range(F)))
         items = []
@@ -772,8 +770,8 @@ def bench_finder(args):
     blind_pipe["frames_per_s"] = max(blind_pipe["find_on_threads"]["frames_per_s"],
                                      blind_pipe["find_batch"]["frames_per_s"])
     blind_pipe["what"] = ("the frames' registrations found (find_on_threads: lizx_find and lizx_read_word on host "
-                          "threads, a decoder each; find_batch: one lizx_find_batch, its frames on the library's own "
-                          "threads and streams, then lizx_read_word on host threads), then one batch graph "
+                          "threads, a decoder each; find_batch: one lizx_find_batch, its frames and their words on the "
+                          "library's own threads and streams), then one batch graph "
                           "(lizx_batch) through them")
     batch.close()
     finder.close()

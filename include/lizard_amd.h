@@ -269,10 +269,12 @@ LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stri
                       float *quad, float *info);
 /* The finder on 1..256 frames of one size at once (GREY8, rows stride bytes apart): each frame uploaded and searched on
    one of the handle's host threads (up to 16, made on the first call), a stream each; frame f's result is the one
-   lizx_find gives it. No counterpart in the reference, whose finder takes a frame a call. Out per frame: spans[f] (0 not found), found[f], H + 9 f, nodes + 3200 f (room for 40 x 40
-   pairs each). Returns the frames found. */
+   lizx_find gives it. No counterpart in the reference, whose finder takes a frame a call. Out per frame: spans[f] (0
+   not found), found[f], H + 9 f, nodes + 3200 f (room for 40 x 40 pairs each), and where versions is not NULL the
+   version its format word names through that registration (lizx_read_word's), -1 where it has none or was not
+   found. Returns the frames found. */
 LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *imgs, int w, int h, int stride,
-                            int *spans, float *H, float *nodes, int *found);
+                            int *spans, float *H, float *nodes, int *found, int *versions);
 
 /* ---- replays: the .lizard file (research/SPEC.md 15) ---------------------------------------------------------- */
 

@@ -137,7 +137,7 @@ def lib():
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
     sig("lizx_find", _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P)
-    sig("lizx_find_batch", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P)
+    sig("lizx_find_batch", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_new", _P, _I)
     sig("lizx_batch_free", None, _P)
     sig("lizx_batch_plan", _I, _P, _I, _P, _P, _I, _I)
@@ -648,8 +648,9 @@ class Decoder:
         check(lib().lizx_b3_hash_device(self.h, d_data.ptr, length, _p(out), C.byref(ms)))
         return (out.tobytes(), ms.value) if timed else out.tobytes()
 
-    def find_batch(self, imgs):
-        """lizx_find_batch on same-size GREY8 frames: a list of None or dicts (span, H, nodes) as find gives."""
+    def find_batch(self, imgs, words=False):
+        """lizx_find_batch on same-size GREY8 frames: a list of None or dicts (span, H, nodes) as find gives; with
+        words, each dict also carries "version", what the frame's format word names (None where it has none)."""
         imgs = [np.ascontiguousarray(i, np.uint8) for i in imgs]
         F = len(imgs)
         h, w = imgs[0].shape
@@ -658,7 +659,9 @@ class Decoder:
         spans, found = np.zeros(F, np.int32), np.zeros(F, np.int32)
         H = np.zeros((F, 9), np.float32)
         nodes = np.zeros((F, 40 * 40 * 2), np.float32)
-        check(lib().lizx_find_batch(self.h, F, ptrs, w, h, w, _p(spans), _p(H), _p(nodes), _p(found)))
+        versions = np.zeros(F, np.int32)
+        check(lib().lizx_find_batch(self.h, F, ptrs, w, h, w, _p(spans), _p(H), _p(nodes), _p(found),
+                                    _p(versions) if words else None))
         out = []
         for f in range(F):
             if not found[f]:
@@ -667,6 +670,8 @@ class Decoder:
             nx = len(node_lattice(int(spans[f])))
             out.append({"span": int(spans[f]), "H": H[f].copy(),
                         "nodes": nodes[f, :nx * nx * 2].reshape(nx, nx, 2).copy()})
+            if words:
+                out[-1]["version"] = int(versions[f]) if versions[f] >= 0 else None
         return out
 
     def frame_slots_device(self, version, d_blocks, frames, cnt, d_slots):

@@ -1498,7 +1498,7 @@ LIZ_API int lizx_find(liz_decoder *d, const uint8_t *img, int w, int h, int stri
 }
 
 LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *imgs, int w, int h, int stride,
-                            int *spans, float *H, float *nodes, int *found) {
+                            int *spans, float *H, float *nodes, int *found, int *versions) {
   return guard([&] {
     need(d && imgs && spans && H && nodes && found, "a decoder, the frames and room for what is found");
     need(frames >= 1 && frames <= 256, "1 to 256 frames");
@@ -1509,14 +1509,13 @@ LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *im
     const size_t fb = static_cast<size_t>(w) * h;
     d->finder.img.reserve(fb * frames);
     std::vector<FindResult> res(frames);
-    find_symbols(d->finder, frames, d->finder.img.as<uint8_t>(), imgs, w, h, stride, e.stream, res.data());
-    int hits = 0;
-    for (int f = 0; f < frames; f++) {
+    // each frame's registration (and word) on the thread that found it, while the other frames are still searched
+    auto done = [&](int f) {
       const FindResult &fr = res[f];
       found[f] = fr.found;
       spans[f] = fr.found ? fr.span : 0;
-      if (!fr.found) continue;
-      hits++;
+      if (versions) versions[f] = -1;
+      if (!fr.found) return;
       float *Hf = H + 9 * f;
       for (int i = 0; i < 9; i++) Hf[i] = static_cast<float>(fr.H[i]);
       RegModel m;
@@ -1526,8 +1525,19 @@ LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *im
       params[8] = static_cast<float>(fr.k1);
       std::vector<float> nv;
       reg_nodes(m, params, nv);
-      std::memcpy(nodes + static_cast<size_t>(f) * MAX_NODES * MAX_NODES * 2, nv.data(), nv.size() * sizeof(float));
-    }
+      float *nf = nodes + static_cast<size_t>(f) * MAX_NODES * MAX_NODES * 2;
+      std::memcpy(nf, nv.data(), nv.size() * sizeof(float));
+      if (!versions) return;
+      const int side = fr.span + 30, cells = fmt_side_cells(side), bytes = fmt_bytes(side);
+      std::vector<float> qv(4 * static_cast<size_t>(cells));
+      FmtWord fw;
+      if (cells && read_word_soft(fr.span, imgs[f], w, h, stride, Hf, nf, qv.data(), cells) &&
+          fmt_decode(qv.data(), cells, bytes, &fw) == 0)
+        versions[f] = fw.version;
+    };
+    find_symbols(d->finder, frames, d->finder.img.as<uint8_t>(), imgs, w, h, stride, e.stream, res.data(), done);
+    int hits = 0;
+    for (int f = 0; f < frames; f++) hits += res[f].found ? 1 : 0;
     return hits;
   });
 }

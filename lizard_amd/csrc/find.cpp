@@ -31,6 +31,7 @@ namespace {
 constexpr double PI = 3.14159265358979323846;
 constexpr int RINGS[3] = {128, 256, 512};
 constexpr int BAND_CELL = 2, RUN = 4, RESV = 15;
+constexpr int EDGE_PROBES = 4000;  // the most probes along one edge of a side
 
 struct Side {
   double th, nx, ny;  // outward normal
@@ -571,7 +572,7 @@ void edge_probes(FindJob &j) {
     const double x0 = j.Cx[ENDS[sd][0]], y0 = j.Cy[ENDS[sd][0]], x1 = j.Cx[ENDS[sd][1]], y1 = j.Cy[ENDS[sd][1]];
     const double len = std::hypot(x1 - x0, y1 - y0), tx = (x1 - x0) / len, ty = (y1 - y0) / len;
     const double margin = std::max(4.0 * j.thick, 8.0);
-    const int cnt = std::max(0, std::min(4000, static_cast<int>((len - 2 * margin) / 1.5)));
+    const int cnt = std::max(0, std::min(EDGE_PROBES, static_cast<int>((len - 2 * margin) / 1.5)));
     for (int e = 0; e < 2; e++) {
       const double half = 0.5 * (q.ro - q.ri) * (e == 0 ? 1 : -1);
       for (int k = 0; k < cnt; k++) {
@@ -814,6 +815,18 @@ void run_frame(FindJob &j, FinderLane &ln, const uint8_t *const *d_tab) {
 
 namespace {
 
+// The most probes one frame's stage makes: both edges of four sides (the model's probes are fewer).
+constexpr size_t MAX_FRAME_PROBES = static_cast<size_t>(EDGE_PROBES) * 2 * 4;
+
+// A lane's buffers at their largest at once, so that no later call reallocates (and so synchronises) them.
+void lane_reserve(FinderLane &ln) {
+  ln.probes.reserve(MAX_FRAME_PROBES * sizeof(FindProbe));
+  ln.edges.reserve(MAX_FRAME_PROBES * sizeof(FindEdge));
+  ln.h_probes.reserve(MAX_FRAME_PROBES * sizeof(FindProbe));
+  ln.h_edges.reserve(MAX_FRAME_PROBES * sizeof(FindEdge));
+  ln.h_peaks.reserve(FIND_MAX_PEAKS * sizeof(FindPeak) + sizeof(int));
+}
+
 // A frame's vote and peaks on its lane, then the rest of its search.
 void search_frame(FinderBufs &b, FinderLane &ln, FindJob &j, const uint8_t *d_img, const uint8_t *const *d_tab, int nr,
                   size_t cells) {
@@ -839,7 +852,7 @@ void search_frame(FinderBufs &b, FinderLane &ln, FindJob &j, const uint8_t *d_im
 
 // The search of frames whose device images are in place (upload false) or are uploaded here on their lanes.
 void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
-            hipStream_t s, FindResult *res, bool upload) {
+            hipStream_t s, FindResult *res, bool upload, const std::function<void(int)> &done) {
   const float up = res[0].ms[0];
   for (int f = 0; f < F; f++) res[f] = FindResult{};
   res[0].ms[0] = up;
@@ -877,9 +890,11 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
       res[f].ms[0] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     search_frame(b, ln, jobs[f], dst, d_tab, nr, cells);
+    if (done) done(f);
   };
   if (F == 1) {
     b.main.s = s;
+    lane_reserve(b.main);
     one(0, b.main);
     return;
   }
@@ -892,6 +907,7 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
       auto ln = std::make_unique<FinderLane>();
       hip_check(hipStreamCreateWithFlags(&ln->s, hipStreamNonBlocking), "finder lane stream");
       ln->own = true;
+      lane_reserve(*ln);
       b.lanes.push_back(std::move(ln));
     }
   }
@@ -912,14 +928,14 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
 }  // namespace
 
 void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
-                  hipStream_t s, FindResult *res) {
-  search(b, F, d_imgs, imgs, w, h, stride, s, res, true);
+                  hipStream_t s, FindResult *res, const std::function<void(int)> &done) {
+  search(b, F, d_imgs, imgs, w, h, stride, s, res, true, done);
 }
 
 void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w, int h, int stride, hipStream_t s,
                  FindResult &r) {
   const uint8_t *imgs[1] = {img};
-  search(b, 1, const_cast<uint8_t *>(d_img), imgs, w, h, stride, s, &r, false);
+  search(b, 1, const_cast<uint8_t *>(d_img), imgs, w, h, stride, s, &r, false, {});
 }
 
 }  // namespace lz

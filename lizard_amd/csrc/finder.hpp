@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -113,8 +114,9 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
                  FindResult &r);
 // The same for F frames of one size at once (d_imgs: room for F device images, rows w apart, each w h bytes apart in
 // one buffer; imgs: the host frames, rows stride apart), each frame uploaded into its place and searched on a host
-// thread of its own with a stream of its own. Each frame's result is the one find_symbol gives it.
+// thread of its own with a stream of its own. Each frame's result is the one find_symbol gives it. done (may be empty)
+// is called with each frame's index on the thread that searched it, as soon as its result is in place.
 void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
-                  hipStream_t s, FindResult *r);
+                  hipStream_t s, FindResult *r, const std::function<void(int)> &done);
 
 }  // namespace lz

@@ -3,6 +3,8 @@ capture and against the reference's own finder (lizard_acquire, compiled from it
 ring and orientation, the registration at the node lattice (as close to the truth as the reference's or closer),
 the lens, and the blind decode (as many blocks as the reference's finder and decoder verify); frames with no symbol
 are not found; a padded capture gives what the tight one does."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -209,7 +211,7 @@ def test_find_batch_equals_single(decoder_gpu):
         dpic.free()
     imgs.insert(3, rng.integers(0, 256, (1080, 1920)).astype(np.uint8))  # nothing to find
     imgs.append(np.full((1080, 1920), 255, np.uint8))
-    batch = decoder_gpu.find_batch(imgs)
+    batch = decoder_gpu.find_batch(imgs, words=True)
     assert len(batch) == len(imgs)
     for i, img in enumerate(imgs):
         one = decoder_gpu.find(img)
@@ -218,4 +220,44 @@ def test_find_batch_equals_single(decoder_gpu):
             continue
         assert batch[i] is not None and batch[i]["span"] == one["span"], i
         assert np.array_equal(batch[i]["H"], one["H"]) and np.array_equal(batch[i]["nodes"], one["nodes"]), i
+        # the word read on the finder's thread is the one lizx_read_word reads through the same registration
+        word = L.read_word(one["span"], img, one["H"], one["nodes"])
+        assert batch[i]["version"] == (None if word is None else word[0]), i
     assert sum(b is not None for b in batch) == 6
+
+
+def test_find_batch_padded_rows_and_reuse(decoder_gpu):
+    """lizx_find_batch through the C ABI on frames whose rows are padded (stride > w) gives what the tight frames give,
+    and a smaller batch after a larger one on the same handle gives each frame the same again."""
+    rng = np.random.default_rng(91)
+    imgs = []
+    for i, (version, span) in enumerate([(20, 256), (5, 128), (64, 512)]):
+        blocks = sim.random_blocks(version, rng)
+        cam = sim.Camera(1280, 720, rot_deg=float(rng.uniform(-20, 20)), scale=float(rng.uniform(0.7, 0.9)),
+                         persp=0.02, k1=0.01, blur=0.8, noise=3.0)
+        dpic = decoder_gpu.modulate(version, blocks)
+        dimg = decoder_gpu.capture(version, span, dpic, 1280, 720, cam.H(span), k1=cam.k1, blur=cam.blur,
+                                   noise=cam.noise, seed=40 + i, fps=30)
+        imgs.append(dimg.get(np.uint8, 1280 * 720).reshape(720, 1280))
+        dimg.free()
+        dpic.free()
+    tight = decoder_gpu.find_batch(imgs)
+    assert all(t is not None for t in tight)
+    pad = 1280 + 96
+    bufs = [np.full((720, pad), 17, np.uint8) for _ in imgs]
+    for b, im in zip(bufs, imgs):
+        b[:, :1280] = im
+    F = len(imgs)
+    ptrs = (C.c_void_p * F)(*[b.ctypes.data for b in bufs])
+    spans, found = np.zeros(F, np.int32), np.zeros(F, np.int32)
+    H, nodes = np.zeros((F, 9), np.float32), np.zeros((F, 3200), np.float32)
+    got = L.lib().lizx_find_batch(decoder_gpu.h, F, ptrs, 1280, 720, pad, L._p(spans),
+                                           L._p(H), L._p(nodes), L._p(found), None)
+    assert got == F
+    for f in range(F):
+        nx = tight[f]["nodes"].shape[0]
+        assert spans[f] == tight[f]["span"] and np.array_equal(H[f], tight[f]["H"])
+        assert np.array_equal(nodes[f, :nx * nx * 2].reshape(nx, nx, 2), tight[f]["nodes"])
+    again = decoder_gpu.find_batch(imgs[1:])
+    for f, a in enumerate(again):
+        assert a is not None and np.array_equal(a["H"], tight[f + 1]["H"]), f

@@ -81,3 +81,17 @@ def test_no_gpu_is_an_error_not_a_fallback():
         lizard_amd.LdpcDecoder()
     with pytest.raises(lizard_amd.LizError):
         lizard_amd.Decoder()
+
+
+def test_finder_entries_check_their_arguments():
+    """The finder's entries refuse a missing decoder or output before any device work (no GPU needed)."""
+    L = lizard_amd.lib()
+    img = np.zeros((64, 64), np.uint8)
+    H, nodes = np.zeros(9, np.float32), np.zeros(3200, np.float32)
+    assert L.lizx_find(None, lizard_amd._p(img), 64, 64, 64, None, lizard_amd._p(H), lizard_amd._p(nodes), None,
+                       None) == lizard_amd.LIZ_E_ARG
+    ptrs = (C.c_void_p * 1)(img.ctypes.data)
+    spans, found = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    assert L.lizx_find_batch(None, 1, ptrs, 64, 64, 64, lizard_amd._p(spans), lizard_amd._p(H), lizard_amd._p(nodes),
+                             lizard_amd._p(found), None) == lizard_amd.LIZ_E_ARG
+    assert L.liz_last_error_code() == lizard_amd.LIZ_E_ARG
