@@ -261,3 +261,30 @@ def test_find_batch_padded_rows_and_reuse(decoder_gpu):
     again = decoder_gpu.find_batch(imgs[1:])
     for f, a in enumerate(again):
         assert a is not None and np.array_equal(a["H"], tight[f + 1]["H"]), f
+
+
+def test_find_batch_at_its_largest(decoder_gpu):
+    """256 frames (the entry's most) in one call give each frame what lizx_find gives it; 257 are refused before any
+    device work."""
+    rng = np.random.default_rng(123)
+    base = []
+    for i, (version, span) in enumerate([(5, 128), (20, 256), (8, 256), (5, 128)]):
+        blocks = sim.random_blocks(version, rng)
+        cam = sim.Camera(960, 720, rot_deg=float(rng.uniform(0, 360)), scale=float(rng.uniform(0.75, 0.9)),
+                         persp=0.02, k1=0.0, blur=0.8, noise=3.0)
+        dpic = decoder_gpu.modulate(version, blocks)
+        dimg = decoder_gpu.capture(version, span, dpic, 960, 720, cam.H(span), k1=cam.k1, blur=cam.blur,
+                                   noise=cam.noise, seed=60 + i, fps=30)
+        base.append(dimg.get(np.uint8, 960 * 720).reshape(720, 960))
+        dimg.free()
+        dpic.free()
+    singles = [decoder_gpu.find(b) for b in base]
+    assert all(s is not None for s in singles)
+    imgs = [base[f % 4] for f in range(256)]
+    batch = decoder_gpu.find_batch(imgs)
+    for f, got in enumerate(batch):
+        one = singles[f % 4]
+        assert got is not None and got["span"] == one["span"], f
+        assert np.array_equal(got["H"], one["H"]) and np.array_equal(got["nodes"], one["nodes"]), f
+    with pytest.raises(L.LizError):
+        decoder_gpu.find_batch(imgs + imgs[:1])
