@@ -11,8 +11,8 @@ the GPU. Inputs are resident in HBM before the timed region.
 Prints one JSON line: LDPC Gbit/s (3840 x codewords / s) as the value, payload MB/s (472 x verified blocks / s),
 the plain per-codeword path's rate beside the batched one, the decode kernel's roofline (integer operations against
 the vector issue peak, with the LDS and HBM figures beside it) and the reference CPU decoder timed on this machine's
-cores, and the PHY's own setting beside it ("stall_rule": 30 iterations with the stall rule 9 / 0.95). More
-workloads ride along as sub-objects:
+cores, the PHY's own setting beside it ("stall_rule": 30 iterations with the stall rule 9 / 0.95), and the same
+decode from host buffers ("host_buffers": the PCIe copies included). More workloads ride along as sub-objects:
   "pipeline"   the full receive path, batched: 32 captured 1920x1080 uint8 frames, each an n = 1536 picture
                (LIZARD-1024, 124 blocks) in the 256 ring through a random homography (rotation +-10 deg, scale 0.6-0.9
                of the frame height, perspective up to 0.05), radial k1 in [-0.05, 0.05], blur sigma 0.5-1.5 px, AWGN
@@ -1158,6 +1158,17 @@ def main():
              "Gbps": round(3840.0 * count * args.steps / w30 / 1e9, 3),
              "verified_by_point": ok30, "by_point": outcome30}
 
+    # the same decode from host buffers (lizx_ldpc_decode: the LLRs in and the decisions out over PCIe each call),
+    # before the timed region (whose launches are the last of the run), one call a point; never the headline value
+    dec.decode(batches[0][0], batches[0][1], batches[0][2], max_iter=MAX_ITER, stall_it=0)
+    t0 = time.perf_counter()
+    for p in range(len(EBN0)):
+        dec.decode(batches[p][0], batches[p][1], batches[p][2], max_iter=MAX_ITER, stall_it=0)
+    wh = time.perf_counter() - t0
+    host_buffers = {"Gbps": round(3840.0 * count * len(EBN0) / wh / 1e9, 3),
+                    "ms_per_step": round(wh / len(EBN0) * 1e3, 4),
+                    "what": "lizx_ldpc_decode with host buffers: each call copies the LLRs in and the decisions out"}
+
     # the autotuner picks the persistent grid's size on the first Eb/N0 point's batch (the bits do not depend on it)
     tuned_per_cu = dec.tune(count, dev[0][0], dev[0][1], dev[0][2], d_its, max_iter=MAX_ITER, stall_it=0, reps=3)
     for s in range(args.warmup):
@@ -1172,6 +1183,7 @@ def main():
     codewords = count * args.steps
     verified = sum(verified_all[s % len(EBN0)] for s in range(args.steps))
     gbps = 3840.0 * codewords / wall / 1e9
+
     payload_mbps = 472.0 * verified / wall / 1e6
 
     # rooflines of the decode kernel over the iterations actually run, per launch: integer operations (the bound:
@@ -1240,6 +1252,7 @@ def main():
         "payload_MBps": round(payload_mbps, 2),
         "autotune": {"ldpc_wg_per_cu": tuned_per_cu or "max"},
         "stall_rule": stall,
+        "host_buffers": host_buffers,
         "plain_Gbps": round(plain, 4) if plain else None,
         "batched_over_plain": round(gbps / plain, 2) if plain else None,
         "points": points,
