@@ -288,3 +288,37 @@ def test_find_batch_at_its_largest(decoder_gpu):
         assert np.array_equal(got["H"], one["H"]) and np.array_equal(got["nodes"], one["nodes"]), f
     with pytest.raises(L.LizError):
         decoder_gpu.find_batch(imgs + imgs[:1])
+
+
+def test_find_batch_from_two_threads_at_once():
+    """Two handles' batches searched at the same time from two caller threads (each handle with its own threads and
+    streams) give what each gives alone."""
+    import concurrent.futures as cf
+    rng = np.random.default_rng(131)
+    d0, d1 = L.Decoder(0), L.Decoder(0)
+    try:
+        imgs = []
+        for i, (version, span) in enumerate([(5, 128), (20, 256), (8, 256), (45, 512)]):
+            blocks = sim.random_blocks(version, rng)
+            cam = sim.Camera(1280, 720, rot_deg=float(rng.uniform(0, 360)), scale=float(rng.uniform(0.75, 0.9)),
+                             persp=0.02, k1=0.01, blur=0.8, noise=3.0)
+            dpic = d0.modulate(version, blocks)
+            dimg = d0.capture(version, span, dpic, 1280, 720, cam.H(span), k1=cam.k1, blur=cam.blur, noise=cam.noise,
+                              seed=80 + i, fps=30)
+            imgs.append(dimg.get(np.uint8, 1280 * 720).reshape(720, 1280))
+            dimg.free()
+            dpic.free()
+        alone = d0.find_batch(imgs * 4)
+        with cf.ThreadPoolExecutor(2) as ex:
+            a, b = ex.submit(d0.find_batch, imgs * 4), ex.submit(d1.find_batch, imgs * 4)
+            got = [a.result(), b.result()]
+        for g in got:
+            for f, (x, y) in enumerate(zip(g, alone)):
+                assert (x is None) == (y is None), f
+                if x is not None:
+                    assert x["span"] == y["span"] and np.array_equal(x["H"], y["H"]), f
+                    assert np.array_equal(x["nodes"], y["nodes"]), f
+        assert sum(x is not None for x in alone) == 16
+    finally:
+        d0.close()
+        d1.close()
