@@ -360,6 +360,11 @@ LIZ_API int lizx_ldpc_sync(lizx_ldpc *h);
 LIZ_API int lizx_ldpc_tune(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr, const int64_t *d_off,
                            int8_t *d_its, int max_iter, int stall_it, float stall_ratio, int reps, int *per_cu);
 LIZ_API float lizx_ldpc_last_ms(lizx_ldpc *h);
+/* Sets the persistent grid's size for this handle's later decodes without timing anything: per_cu workgroups a CU
+   (0: as many as fit; above that, as many as fit). The grid is min(count, CUs x per_cu) workgroups, each taking
+   codewords off the queue until it is empty, so with per_cu = 1 or 2 every workgroup decodes several codewords of
+   a large batch. The bits never depend on it. Returns the workgroups a CU now used. */
+LIZ_API int lizx_ldpc_shape(lizx_ldpc *h, int per_cu);
 
 /* ---- Registration refinement (no counterpart in the reference, whose registration is estimated on the CPU;
    liblizard/src/acquire.c:1470-1512 lens model, :2196-2352 anchors' mesh). A registration's parameters, params[107]:

@@ -156,6 +156,7 @@ def lib():
     sig("lizx_ldpc_sync", _I, _P)
     sig("lizx_ldpc_tune", _I, _P, _I, _P, _P, _P, _P, _I, _I, _F, _I, _P)
     sig("lizx_ldpc_last_ms", _F, _P)
+    sig("lizx_ldpc_shape", _I, _P, _I)
     sig("lizx_ldpc_encode", _I, _I, _I, _P, _P)
     sig("lizx_crc32c", C.c_uint32, _P, _SZ)
     sig("lizx_copy_probe", _I, _P, _P, _SZ, C.POINTER(C.c_float))
@@ -374,6 +375,12 @@ class LdpcDecoder:
     def sync(self):
         check(lib().lizx_ldpc_sync(self.h))
         return float(lib().lizx_ldpc_last_ms(self.h))
+
+    def shape(self, per_cu):
+        """Sets the persistent grid's workgroups a CU (lizx_ldpc_shape; 0: as many as fit); returns the count used."""
+        rc = lib().lizx_ldpc_shape(self.h, per_cu)
+        check(rc)
+        return rc
 
     def tune(self, count, d_rate, d_llr, d_off, d_its, max_iter=20, stall_it=0, stall_ratio=0.95, reps=3):
         """The persistent grid's autotuner (lizx_ldpc_tune): workgroups a CU kept (0: as many as fit)."""

@@ -480,6 +480,14 @@ LIZ_API int lizx_ldpc_tune(lizx_ldpc *h, int count, const uint8_t *d_rate, const
   });
 }
 
+LIZ_API int lizx_ldpc_shape(lizx_ldpc *h, int per_cu) {
+  return guard([&] {
+    need(h && per_cu >= 0, "a decoder and a count of workgroups a CU");
+    h->per_cu = per_cu >= h->eng->wg_per_cu ? 0 : per_cu;
+    return h->per_cu ? h->per_cu : h->eng->wg_per_cu;
+  });
+}
+
 LIZ_API int lizx_ldpc_sync(lizx_ldpc *h) {
   return guard([&] {
     need(h, "a decoder");

@@ -134,3 +134,71 @@ class Capture:
         self.nodes = np.zeros((nx, nx, 2), np.float32)
         if node_fn is not None:
             self.nodes = node_fn(self.node_x).astype(np.float32)
+
+
+def format_tables(version):
+    """The oracle's tables of a format: n, subch, blocks, (u, v) of each coefficient, the whitening of every slot and
+    the pilot coefficients (the tails of the 7/8 and 3/4 blocks, SPEC 7.3)."""
+    o = oracle()
+    subch = 16 * version
+    n = o.or_n_for(subch)
+    f = o.or_format_new(n, subch)
+    info = np.zeros(6, np.int32)
+    o.or_format_info(f, ptr(info))
+    B = int(info[2])
+    S = subch * 320
+    pos = np.ctypeslib.as_array(o.or_format_pos(f), (S,)).copy()
+    rate, s0, ns = np.zeros(B, np.int32), np.zeros(B, np.int32), np.zeros(B, np.int32)
+    o.or_format_blocks(f, ptr(rate), ptr(s0), ptr(ns))
+    o.or_format_free(f)
+    white = np.zeros(2 * S, np.uint8)
+    o.or_whiten(ptr(white), 2 * S)
+    nt = {6: 4480, 4: 5120, 3: 5760, 2: 7680}
+    pil = [s0[b] // 2 + s for b in range(B) for s in range(nt[int(rate[b])] // 2, ns[b] // 2)]
+    u, v = picture.coef_uv(pos, n)
+    return dict(n=n, subch=subch, blocks=B, u=u, v=v, white=white, pilots=np.array(pil, np.int64))
+
+
+def soft64(grid, version, span):
+    """The FP64 restatement's finish of a row-major n x n grid (oracle/picture.py finish64)."""
+    t = format_tables(version)
+    return picture.finish64(grid, t["n"], span, t["u"], t["v"], t["white"], t["pilots"])
+
+
+def assert_ties(a, b, soft, tol=1e-4):
+    """Two int8 soft-value arrays may differ only by one, and only where the FP64 value 8 clamp(k y, +-10) lies
+    within tol (relative, at least 1 absolute unit of tol) of a half-step: a rounding tie that float rounding on
+    either side decides. Returns the fraction of slots that differ."""
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    d = a != b
+    if not d.any():
+        return 0.0
+    assert np.abs(a - b).max() <= 1
+    x = 8.0 * np.clip(np.asarray(soft)[d], -10.0, 10.0)
+    dist = np.abs(x - (np.floor(x) + 0.5))
+    bad = dist > tol * np.maximum(1.0, np.abs(x))
+    assert not bad.any(), "%d of %d differing soft values are not rounding ties (max distance %.3g)" % (
+        int(bad.sum()), int(d.sum()), float(dist.max()))
+    return float(d.mean())
+
+
+def ref_finish_soft(r, version, span, llr, est):
+    """The compiled reference's decline gate, LDPC and CRC (lizard_finish_ext with ext_llr, ext_est,
+    liblizard/src/lizard_phy.c:1566) on given soft values and sub-channel estimates: good, ok, its, blocks, blk_est."""
+    t = format_tables(version)
+    B = t["blocks"]
+    f = r.ref_format_new(t["n"], t["subch"], span)
+    bl, ok, its, be = np.zeros(B * 476, np.uint8), np.zeros(B, np.uint8), np.zeros(B, np.int8), np.zeros(B, np.float32)
+    good = r.ref_finish_llr(f, ptr(np.ascontiguousarray(llr, np.int8)), ptr(np.ascontiguousarray(est, np.float32)),
+                            ptr(bl), ptr(ok), ptr(its), ptr(be))
+    r.ref_format_free(f)
+    return dict(good=good, ok=ok, its=its, blocks=bl.reshape(B, 476), blk_est=be)
+
+
+def assert_finish_equal(out, want):
+    """The GPU's verdicts, iterations and verified bytes equal a reference finish of the same soft values."""
+    assert np.array_equal(out["its"], want["its"]), np.nonzero(out["its"] != want["its"])
+    assert np.array_equal(out["ok"].astype(bool), want["ok"].astype(bool))
+    v = want["ok"].astype(bool)
+    assert np.array_equal(np.asarray(out["blocks"]).reshape(-1, 476)[v], want["blocks"][v])

@@ -8,6 +8,7 @@ import pytest
 import lizard_amd as L
 from lizard_amd import sim
 from _libs import ptr, ref
+from pictures import assert_finish_equal, assert_ties, ref_finish_soft, soft64
 
 pytestmark = pytest.mark.gpu
 
@@ -101,6 +102,10 @@ def test_batch_verified_bytes_are_the_sent_blocks(caps):
             r.ref_format_free(f)
             assert np.array_equal(got["ok"][at:at + B], ok)
             assert np.array_equal(got["its"][at:at + B], its)
+            # and the reference's integer path on the batch's own soft values, whatever the ties
+            s = b.soft(len([x for x in caps[:caps.index(c)]]), c["version"])
+            assert_finish_equal({k: got[k][at:at + B] for k in ("ok", "its", "blocks")},
+                                ref_finish_soft(r, c["version"], c["span"], s["llr"], s["sub_est"]))
             at += B
     b.close()
 
@@ -198,4 +203,61 @@ def test_launch_shapes_leave_the_bits_alone(caps):
     same(b.blocks_out(), want)
     with pytest.raises(L.LizError):
         b.shape([3, 4, 4, 0])
+    b.close()
+
+
+# LIZARD-1024 (n = 1536, 124 blocks) in the 512 ring on 1920x1080 captures: the full-pipeline workload's format
+BIG = [(64, 512, dict(W=1920, Hh=1080, rot_deg=rot, scale=sc, persp=pe, k1=k1, blur=bl, noise=nz))
+       for rot, sc, pe, k1, bl, nz in ((3.0, 0.97, 0.01, 0.0, 0.5, 2.0), (-6.0, 0.92, 0.03, 0.03, 0.6, 3.0),
+                                       (9.0, 0.9, 0.02, -0.04, 0.7, 4.0), (-2.0, 0.95, 0.05, 0.02, 0.5, 6.0),
+                                       (5.0, 0.88, 0.0, -0.01, 0.8, 2.0))]
+
+
+def test_lizard1024_batch_beyond_the_grid_equals_reference():
+    """Five LIZARD-1024 captures (620 blocks) through the batch graph with the LDPC grid forced to one workgroup a
+    CU (256 on an MI355X), so workgroups take blocks of several frames in turn. Each frame: the reference's decline
+    gate, LDPC and CRC on the batch's own soft values give the batch's verdicts, iterations and bytes; the soft values
+    equal the reference's own decode of the same capture and registration (ref_decode_reg) but at rounding ties of
+    the FP64 finish; verified blocks are the blocks sent; graph, frame-by-frame and graph again agree."""
+    r = ref("scalar")
+    if r is None:
+        pytest.skip("oracle/_ref not built")
+    caps = make_captures(BIG, 77)
+    b = L.Batch().plan([c["version"] for c in caps], [c["span"] for c in caps])
+    assert b.blocks == 5 * 124
+    b.shape([4, 4, 4, 1])
+    frames, keep = device_frames(caps)
+    outs = []
+    for mode in (L.RUN_GRAPH, L.RUN_FRAMES, L.RUN_GRAPH):
+        b.run(frames, mode)
+        b.sync()
+        outs.append(b.blocks_out())
+    for o in outs[1:]:
+        same(o, outs[0])
+    got = outs[0]
+    sent = np.concatenate([c["blocks"].reshape(-1, 476) for c in caps])
+    v = got["ok"].astype(bool)
+    assert v.sum() > 0 and np.array_equal(got["blocks"][v], sent[v])
+    for i, c in enumerate(caps):
+        at = slice(124 * i, 124 * (i + 1))
+        mine = {k: got[k][at] for k in ("ok", "its", "blocks")}
+        s = b.soft(i, c["version"])
+        assert_finish_equal(mine, ref_finish_soft(r, 64, 512, s["llr"], s["sub_est"]))
+        info = L.format_info(64)
+        f = r.ref_format_new(info["n"], info["subch"], 512)
+        B = 124
+        out, ok = np.zeros(B * 476, np.uint8), np.zeros(B, np.uint8)
+        its, be = np.zeros(B, np.int8), np.zeros(B, np.float32)
+        llr, al = np.zeros(info["subch"] * 640, np.int8), np.zeros(2, np.float32)
+        nodes = np.ascontiguousarray(c["nodes"], np.float32)
+        r.ref_decode_reg(f, ptr(c["img"]), c["img"].shape[1], c["img"].shape[0], ptr(c["H"]), ptr(nodes), ptr(out),
+                         ptr(ok), ptr(its), ptr(be), ptr(llr), ptr(al))
+        grid = np.zeros((info["n"], info["n"]), np.float32)
+        r.ref_sample_grid(f, ptr(c["img"]), c["img"].shape[1], c["img"].shape[0], ptr(c["H"]), ptr(nodes), 0,
+                          ptr(grid))
+        r.ref_format_free(f)
+        tie = assert_ties(s["llr"], llr, soft64(grid, 64, 512)["soft"])
+        assert tie < 1e-3, tie
+        if tie == 0:
+            assert np.array_equal(mine["its"], its) and np.array_equal(mine["ok"], ok)
     b.close()

@@ -79,6 +79,22 @@ def test_finder_on_reference_painted_symbols(decoder_gpu, version, span, W, Hh, 
     assert k >= ref_verified - 1, (k, ref_verified)
     truth = decoder_gpu.decode_reg(version, span, img, C.H, C.nodes, soft=False)
     assert k >= truth["good"] - 1
+    assert sent_in_order(blocks, k, C.blocks)
+
+
+def sent_in_order(got, k, sent):
+    """The k verified blocks liz_decode packed (in block order, api/codec.cpp:143-147) are blocks that were sent, in
+    the order they were sent: every verified byte is a sent byte."""
+    sent = np.asarray(sent, np.uint8).reshape(-1, 476)
+    got = np.asarray(got, np.uint8).reshape(-1, 476)[:k]
+    j = 0
+    for blk in got:
+        while j < len(sent) and not np.array_equal(sent[j], blk):
+            j += 1
+        if j == len(sent):
+            return False
+        j += 1
+    return True
 
 
 LENS = [(64, 512, 4.0, 0.04, 0.8, 4.0, 0.85), (20, 256, 95.0, 0.03, 1.0, 5.0, 0.8),
@@ -109,6 +125,7 @@ def test_finder_recovers_the_lens(decoder_gpu, version, span, rot, k1, blur, noi
     assert out.word == 1 and out.blocks == version and out.fps == 30
     truth = decoder_gpu.decode_reg(version, span, img, Ht, sim.nodes_for(cam, span, Hm), soft=False)
     assert k >= truth["good"] - 2, (k, truth["good"])
+    assert sent_in_order(got, k, blocks)
 
 
 def test_finder_padded_rows_match_tight(decoder_gpu):

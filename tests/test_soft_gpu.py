@@ -1,15 +1,15 @@
 """The GPU's picture-to-soft-value stages (lizx_finish_grid: anchor erase, detrend, 2D FFT, pilot shift, soft demap,
 then LDPC and CRC) against the FP64 restatement (oracle/picture.py) and the compiled reference (oracle/_ref).
 Tolerances (north star): coefficients within 1e-5 relative L2 of FP64; LLR floats within 1e-4 relative; int8 soft
-values may differ by one at rounding ties (a small fraction); every block's verdict, iterations and bytes equal the
-reference's where the soft values agree."""
+values may differ by one, and only at rounding ties (the FP64 value within 1e-4 of a half-step); the reference's
+decline gate, LDPC and CRC on the GPU's own soft values give the GPU's verdicts, iterations and bytes on every frame."""
 import numpy as np
 
 import lizard_amd as L
 import pytest
 
 from _libs import ptr, ref
-from pictures import Frame, picture
+from pictures import Frame, assert_finish_equal, assert_ties, picture, ref_finish_soft
 
 pytestmark = pytest.mark.gpu
 
@@ -29,11 +29,15 @@ def test_finish_grid(decoder_gpu, version, span, shift, snr):
     assert rel < 1e-5, rel
     assert abs(out["align"][0] - g64["align"][0]) < 1e-3 and abs(out["align"][1] - g64["align"][1]) < 1e-3
     assert np.allclose(out["sub_est"], g64["est"], rtol=1e-4, atol=1e-7)
-    dl = np.abs(out["llr"].astype(int) - g64["llr"].astype(int))
-    assert dl.max() <= 1 and (dl > 0).mean() < 1e-3, ((dl > 0).mean(), dl.max())
+    # int8 soft values: equal to FP64's but at rounding ties (the FP64 value within 1e-4 of a half-step)
+    tie64 = assert_ties(out["llr"], g64["llr"], g64["soft"])
+    assert tie64 < 1e-3, tie64
     r = ref("scalar")
     if r is None:
         return
+    # the integer path on the GPU's own soft values and estimates: the reference's decline gate, LDPC and CRC give
+    # the same verdicts, iterations and bytes, whatever the ties
+    assert_finish_equal(out, ref_finish_soft(r, version, span, out["llr"], out["sub_est"]))
     B, S = F.blocks_n, F.subch * 320
     fr = r.ref_format_new(F.n, F.subch, span)
     strip = picture.to_strips(F.grid, F.n)
@@ -51,13 +55,13 @@ def test_finish_grid(decoder_gpu, version, span, shift, snr):
     assert np.allclose(gp[v], bp[v], rtol=1e-4, atol=1e-4), np.abs(gp[v] - bp[v]).max()
     cr = coef[0::2] + 1j * coef[1::2]
     assert np.linalg.norm(c - cr) / np.linalg.norm(cr) < 1e-5
-    dr = np.abs(out["llr"].astype(int) - llr.astype(int))
-    assert dr.max() <= 1 and (dr > 0).mean() < 1e-3
-    if not dr.any():
-        # identical soft values: the integer path after them is bit-exact
+    assert np.allclose(out["sub_est"], est, rtol=1e-4, atol=1e-7)
+    # against the reference's own soft values: ties only, and where none differs the same verdicts
+    tie_ref = assert_ties(out["llr"], llr, g64["soft"])
+    assert tie_ref < 1e-3, tie_ref
+    if tie_ref == 0:
         assert out["good"] == good
         assert np.array_equal(out["its"], its) and np.array_equal(out["ok"], ok)
-    assert np.array_equal(out["ok"].astype(bool) & ok.astype(bool), ok.astype(bool) & out["ok"].astype(bool))
     for b in range(B):
         if out["ok"][b]:
             assert np.array_equal(out["blocks"][b], F.blocks.reshape(B, 476)[b])
