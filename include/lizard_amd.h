@@ -197,6 +197,8 @@ typedef struct {
                        pilots' turn (lizard_debug's coefficients, lizard_phy.c:1719) */
   float align[2];   /* out: the pilots' grid shift in samples along u and v, or 0, 0 (lizard_align) */
   float *blk_pilot; /* per block: the pilot tail's mean of +-y / rms (lizard_phy.c:1795-1809), NaN where none */
+  float *soft;      /* 16 * version * 640 floats: each slot's k y before the clamp to +-10 and the rounding of 8 k y
+                       (lizard_phy.c:1764-1768), frame slot order; to tell rounding ties from real differences */
 } lizx_soft;
 
 /* lizard_finish with the caller's grid (liblizard/src/lizard_phy.c:1566 with grid, ext_llr NULL; lizard_phy.h:

@@ -61,7 +61,7 @@ class Frame(C.Structure):
 
 class Soft(C.Structure):
     _fields_ = [("llr", C.c_void_p), ("sub_est", C.c_void_p), ("sub_rms", C.c_void_p), ("coef", C.c_void_p),
-                ("align", C.c_float * 2), ("blk_pilot", C.c_void_p)]
+                ("align", C.c_float * 2), ("blk_pilot", C.c_void_p), ("soft", C.c_void_p)]
 
 
 _lib = None
@@ -752,8 +752,9 @@ class Decoder:
             out["sub_rms"] = np.zeros(info["subch"], np.float32)
             out["coef"] = np.zeros(2 * S, np.float32)
             out["blk_pilot"] = np.zeros(B, np.float32)
+            out["soft"] = np.zeros(2 * S, np.float32)
             sp = Soft(_p(out["llr"]), _p(out["sub_est"]), _p(out["sub_rms"]), _p(out["coef"]), (C.c_float * 2)(0, 0),
-                      _p(out["blk_pilot"]))
+                      _p(out["blk_pilot"]), _p(out["soft"]))
         reg = registration(H, nodes)
         good = check(lib().lizx_decode_reg(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
                                            C.byref(reg), _p(blocks), _p(ok), _p(its), _p(be),
@@ -801,8 +802,9 @@ class Decoder:
             out["sub_rms"] = np.zeros(info["subch"], np.float32)
             out["coef"] = np.zeros(2 * S, np.float32)
             out["blk_pilot"] = np.zeros(B, np.float32)
+            out["soft"] = np.zeros(2 * S, np.float32)
             sp = Soft(_p(out["llr"]), _p(out["sub_est"]), _p(out["sub_rms"]), _p(out["coef"]), (C.c_float * 2)(0, 0),
-                      _p(out["blk_pilot"]))
+                      _p(out["blk_pilot"]), _p(out["soft"]))
         good = check(lib().lizx_finish_grid(self.h, version, span, _p(grid), _p(blocks), _p(ok), _p(its), _p(be),
                                             C.byref(sp) if sp is not None else None))
         out.update({"good": good, "blocks": blocks.reshape(B, LIZ_BLOCK), "ok": ok, "its": its, "est": be})

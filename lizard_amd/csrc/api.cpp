@@ -766,6 +766,9 @@ static void copy_soft(liz_decoder *d, const FramePlan &p, lizx_soft *soft, bool 
     hip_check(hipMemcpy(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
   if (soft->coef && dbg)
     hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
+  if (soft->soft && dbg)
+    hip_check(hipMemcpy(soft->soft, d->fb.coef_dbg.as<float>() + 2 * S, S * 2 * sizeof(float), hipMemcpyDeviceToHost),
+              "soft floats");
   hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
 }
 
@@ -780,7 +783,7 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
   hip_check(hipSetDevice(e.device), "hipSetDevice");
   const FramePlan &p = d->plan(version, span);
   const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg);
-  const bool dbg = soft && soft->coef;
+  const bool dbg = soft && (soft->coef || soft->soft);
   d->fb.reserve(p, 1, dbg);
   SoftBufs v = d->fb.view(dbg);
   v.blk_pilot = pilot_area(d, f);
@@ -1695,7 +1698,7 @@ LIZ_API int lizx_finish_grid(liz_decoder *d, int version, int span, const float 
     LdpcEngine &e = *d->eng;
     hip_check(hipSetDevice(e.device), "hipSetDevice");
     const FramePlan &p = d->plan(version, span);
-    const bool dbg = soft && soft->coef;
+    const bool dbg = soft && (soft->coef || soft->soft);
     const size_t n = p.n;
     d->fb.reserve(p, 1, dbg);
     d->grid.reserve(n * n * sizeof(float));

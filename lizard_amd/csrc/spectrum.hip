@@ -691,7 +691,7 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_al
     int u, v;
     coef_uv(P, ci, &u, &v);
     float2 c = coef_at(P, C, pr, ci, u, v);
-    if (dbg_all) reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * P.subch * SUB + ci] = c;
+    if (dbg_all) reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * 2 * P.subch * SUB + ci] = c;
     if (dx != 0 || dy != 0) c = turn(P, c, u, v, dx, dy);
     y[q] = c;
     const double e = static_cast<double>(c.x) * c.x + static_cast<double>(c.y) * c.y;
@@ -716,6 +716,9 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_al
 #pragma unroll
   for (int q = 0; q < PER; q++) {
     float lr = __fmul_rn(k, y[q].x), li = __fmul_rn(k, y[q].y);
+    if (dbg_all)
+      reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * 2 * P.subch * SUB + P.subch * SUB + j * SUB + lane +
+                                          64 * q] = make_float2(lr, li);
     lr = lr > 10.f ? 10.f : (lr < -10.f ? -10.f : lr);
     li = li > 10.f ? 10.f : (li < -10.f ? -10.f : li);
     const int at = 2 * (lane + 64 * q);
@@ -987,7 +990,7 @@ void FrameBufs::reserve(const FramePlan &p, int frames, bool debug) {
   llr.reserve(frames * 2 * S);
   est.reserve(frames * p.subch * sizeof(double));
   rms.reserve(frames * p.subch * sizeof(float));
-  if (debug) coef_dbg.reserve(frames * S * sizeof(float2));
+  if (debug) coef_dbg.reserve(frames * 2 * S * sizeof(float2));  // the coefficients, then the soft floats
   pilot_r.reserve(frames * p.dev.blocks * sizeof(float));
 }
 
@@ -1019,7 +1022,7 @@ SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
   v.llr = llr + F * 2 * S;
   v.est = est + F * p.subch;
   v.rms = rms + F * p.subch;
-  v.coef_dbg = coef_dbg ? coef_dbg + F * 2 * S : nullptr;
+  v.coef_dbg = coef_dbg ? coef_dbg + F * 4 * S : nullptr;
   v.blk_pilot = blk_pilot ? blk_pilot + F * p.dev.blocks : nullptr;
   return v;
 }

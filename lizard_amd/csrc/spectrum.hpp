@@ -76,7 +76,7 @@ struct SoftBufs {
   int8_t *llr = nullptr;
   double *est = nullptr;
   float *rms = nullptr;
-  float *coef_dbg = nullptr;  // null: no debug copy of the coefficients
+  float *coef_dbg = nullptr;  // null: no debug copy of the coefficients (then the soft floats, 2S floats each)
   float *blk_pilot = nullptr; // null: no pilot readings
   SoftBufs at(const FramePlan &p, int f) const;  // frame f's slice
 };

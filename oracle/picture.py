@@ -6,7 +6,8 @@ The picture side of the receive path restated in FP64 numpy, as the checker for 
                 inverse DFT, clipped at 2 sigma, sigma = sqrt(640 subch).
   erase_ranges  the samples erased round the anchors (SPEC 6.13; lizard_phy.c:402-407, acquire.c:41-45)
   finish64      erase fill, detrend, forward DFT, pilot shift and soft values in FP64 (lizard_phy.c:1584-1772,
-                :280-329, :1322-1332; SPEC 6.9, 6.10, 7.3, 8.5)
+                :280-329, :1322-1332; SPEC 6.9, 6.10, 7.3, 8.5); the pilot turn through the reference's table index
+                (lizard_phy.c:214-218), the angle exact
 Pinned by: numpy's FFT (a library DFT) for the transforms; the closed forms of SPEC 6.10 (the detrend basis is
 orthogonal, so a picture that is exactly the shading surface detrends to zero) and 6.13; a known grid shift
 recovered by the pilot fit; and the compiled reference (oracle/_ref) on the same pictures (tests/test_soft_*.py).
@@ -63,9 +64,22 @@ def axis_info(g):
     return np.where(g > 0, (1 - 2.0 ** (-0.3073 * (2 * np.sqrt(np.maximum(g, 0))) ** 1.7870)) ** 1.1064, 0.0)
 
 
-def finish64(grid, n, span, u, v, white, pilots):
+ALIGN_TURNS = 4096
+
+
+def turn_index(u, v, n, dx, dy):
+    """The pilot turn's table index of each coefficient: round-half-even((u dx + v dy) 4096 / n) in float32, in the
+    reference's order of operations, modulo 4096 (liblizard/src/lizard_phy.c:215, ALIGN_TURNS :206)."""
+    f = np.float32
+    a = (np.asarray(u).astype(f) * f(dx) + np.asarray(v).astype(f) * f(dy)) * (f(ALIGN_TURNS) / f(n))
+    return np.rint(a).astype(np.int64) & (ALIGN_TURNS - 1)
+
+
+def finish64(grid, n, span, u, v, white, pilots, align=None):
     """grid: n x n (row y, column x). Returns dict: coef (pos order, detrended, before the turn), align (dx, dy),
-    soft (k y per slot, float64, before quantisation), llr (int8), est, rms."""
+    soft (k y per slot, float64, before quantisation), llr (int8), est, rms. The turn by the pilots' shift goes through
+    the reference's 4096-step table index (lizard_phy.c:214-218; the angle itself exact); align, where given, is the
+    shift the soft values are turned by (the fit's own result is still returned as align)."""
     g = np.array(grid, np.float64)
     c, q = basis(n)
     lo, hi = erase_ranges(n, span)
@@ -123,8 +137,9 @@ def finish64(grid, n, span, u, v, white, pilots):
             dx = dy = 0.0
     else:
         dx = dy = 0.0
-    if dx or dy:
-        y = y * np.exp(-1j * TWO_PI * (u * dx + v * dy) / n)
+    tx, ty = (dx, dy) if align is None else (float(align[0]), float(align[1]))
+    if tx or ty:
+        y = y * np.exp(-1j * TWO_PI * turn_index(u, v, n, tx, ty) / ALIGN_TURNS)
     # soft values per sub-channel (lizard_phy.c:1322-1332, :1756-1772)
     S = len(u)
     ys = y.reshape(-1, 320)

@@ -54,8 +54,8 @@ class Frame:
                 pil.append(first + s)
         self.pilots = np.array(pil, np.int64)
 
-    def fp64(self):
-        return picture.finish64(self.grid, self.n, self.span, self.u, self.v, self.white, self.pilots)
+    def fp64(self, align=None):
+        return picture.finish64(self.grid, self.n, self.span, self.u, self.v, self.white, self.pilots, align)
 
 
 def lattice(span):
@@ -165,22 +165,90 @@ def soft64(grid, version, span):
     return picture.finish64(grid, t["n"], span, t["u"], t["v"], t["white"], t["pilots"])
 
 
-def assert_ties(a, b, soft, tol=1e-4):
-    """Two int8 soft-value arrays may differ only by one, and only where the FP64 value 8 clamp(k y, +-10) lies
-    within tol (relative, at least 1 absolute unit of tol) of a half-step: a rounding tie that float rounding on
-    either side decides. Returns the fraction of slots that differ."""
+def turned(version, coef, align):
+    """Coefficients (re, im interleaved, coefficient order) turned by a pilot shift through the table index the
+    float path takes for it (oracle/picture.py turn_index), the angle exact."""
+    t = format_tables(version)
+    y = np.asarray(coef, np.float64)[0::2] + 1j * np.asarray(coef, np.float64)[1::2]
+    if align[0] or align[1]:
+        y = y * np.exp(-2j * np.pi * picture.turn_index(t["u"], t["v"], t["n"], *align) / picture.ALIGN_TURNS)
+    out = np.empty(2 * y.size)
+    out[0::2], out[1::2] = y.real, y.imag
+    return out
+
+
+def float_soft(yt):
+    """A float path's soft values 8 clamp(k y, +-10) from its turned coefficients (re, im interleaved, float32 values):
+    each sub-channel's k from the moments of its 320 coefficients in double, as the reference's scalar path forms it
+    (liblizard/src/lizard_phy.c:1322-1332, :1760-1768), rounded to float, then k y and 8 (.) in float."""
+    f32 = np.float32
+    y = np.asarray(yt, np.float64).reshape(-1, 640)
+    e = y[:, 0::2] ** 2 + y[:, 1::2] ** 2
+    m2, m4 = e.sum(1) / 320, (e * e).sum(1) / 320
+    a2 = np.sqrt(np.maximum(2 * m2 * m2 - m4, 0))
+    nv = np.maximum(m2 - a2, 0.02 * m2)
+    k = (0.7 * 2 * np.sqrt(2 * a2) / nv).astype(f32)
+    ky = (k[:, None] * np.asarray(yt, f32).reshape(-1, 640)).ravel()
+    return (f32(8.0) * np.clip(ky, f32(-10.0), f32(10.0))).astype(np.float64)
+
+
+def assert_ties(a, b, soft_a, yt_b, spread=1e-2):
+    """Two int8 soft-value arrays a (this build's; soft_a its floats k y before the clamp) and b (the reference's)
+    may differ by one, and only at rounding ties: where they differ, this build's 8 k y and the reference's own
+    (float_soft of its turned coefficients yt_b, which must round to b) lie on either side of the half-step between a
+    and b, within `spread` of each other (relative, at least absolute). Returns (fraction of slots that differ, the
+    largest relative distance between the two floats there)."""
     a = np.asarray(a, np.int64)
     b = np.asarray(b, np.int64)
     d = a != b
     if not d.any():
-        return 0.0
+        return 0.0, 0.0
     assert np.abs(a - b).max() <= 1
-    x = 8.0 * np.clip(np.asarray(soft)[d], -10.0, 10.0)
-    dist = np.abs(x - (np.floor(x) + 0.5))
-    bad = dist > tol * np.maximum(1.0, np.abs(x))
-    assert not bad.any(), "%d of %d differing soft values are not rounding ties (max distance %.3g)" % (
-        int(bad.sum()), int(d.sum()), float(dist.max()))
-    return float(d.mean())
+    f32 = np.float32
+    idx = np.nonzero(d)[0]
+    xa = (f32(8.0) * np.clip(np.asarray(soft_a, f32), f32(-10.0), f32(10.0))).astype(np.float64)[idx]
+    xb = float_soft(yt_b)[idx]
+    assert np.array_equal(np.rint(xa).astype(np.int64), a[idx]), "this build's floats do not round to its values"
+    assert np.array_equal(np.rint(xb).astype(np.int64), b[idx]), "the reference's floats are not reproduced"
+    h = (a[idx] + b[idx]) / 2.0
+    rel = np.abs(xa - xb) / np.maximum(1.0, np.abs(xa))
+    bad = ~(((xa - h) * (xb - h) <= 0) & (rel <= spread))
+    assert not bad.any(), ("%d of %d differing soft values are not rounding ties: slots %s, 8 k y here %s, there %s"
+                           % (int(bad.sum()), int(d.sum()), idx[bad][:8].tolist(), xa[bad][:8].tolist(),
+                              xb[bad][:8].tolist()))
+    return float(d.mean()), float(rel.max())
+
+
+def ref_grid_finish(r, version, span, grid):
+    """The compiled reference's finish of a row-major grid (ref_finish_grid, lizard_finish with the grid): its soft
+    values, estimates, pilot shift, coefficients (before the turn), verdicts."""
+    t = format_tables(version)
+    B, S, n = t["blocks"], t["subch"] * 320, t["n"]
+    f = r.ref_format_new(n, t["subch"], span)
+    strip = picture.to_strips(grid, n)
+    bl, ok, its, be, bp = (np.zeros(B * 476, np.uint8), np.zeros(B, np.uint8), np.zeros(B, np.int8),
+                           np.zeros(B, np.float32), np.zeros(B, np.float32))
+    llr, est, rms, al, coef = (np.zeros(2 * S, np.int8), np.zeros(t["subch"]), np.zeros(t["subch"], np.float32),
+                               np.zeros(2, np.float32), np.zeros(2 * S, np.float32))
+    good = r.ref_finish_grid(f, ptr(strip), ptr(bl), ptr(ok), ptr(its), ptr(be), ptr(bp), ptr(llr), ptr(est),
+                             ptr(rms), ptr(al), ptr(coef))
+    r.ref_format_free(f)
+    return dict(good=good, blocks=bl.reshape(B, 476), ok=ok, its=its, blk_est=be, blk_pilot=bp, llr=llr, sub_est=est,
+                sub_rms=rms, align=(float(al[0]), float(al[1])), coef=coef)
+
+
+def ties_against_reference(version, out, ref_out):
+    """assert_ties of this build's soft values against the reference's (its coefficients turned by its own pilot
+    shift); also the two paths' float soft values within 1e-4 relative L2 over the frame. Returns the fraction of
+    slots whose int8 values differ."""
+    xb = float_soft(turned(version, ref_out["coef"], ref_out["align"]))
+    assert np.array_equal(np.rint(xb).astype(np.int64), np.asarray(ref_out["llr"], np.int64)), "reference floats"
+    f32 = np.float32
+    xa = (f32(8.0) * np.clip(np.asarray(out["soft"], f32), f32(-10.0), f32(10.0))).astype(np.float64)
+    rel = np.linalg.norm(xa - xb) / np.linalg.norm(xb)
+    assert rel < 1e-4, rel
+    frac, _ = assert_ties(out["llr"], ref_out["llr"], out["soft"], turned(version, ref_out["coef"], ref_out["align"]))
+    return frac
 
 
 def ref_finish_soft(r, version, span, llr, est):

@@ -8,7 +8,7 @@ import pytest
 import lizard_amd as L
 from lizard_amd import sim
 from _libs import ptr, ref
-from pictures import assert_finish_equal, assert_ties, ref_finish_soft, soft64
+from pictures import assert_finish_equal, ref_finish_soft, ref_grid_finish, ties_against_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -91,7 +91,7 @@ def test_batch_verified_bytes_are_the_sent_blocks(caps):
     r = ref("scalar")
     if r is not None:  # the reference's own decoder on the same captures and registrations: the same verdicts
         at = 0
-        for c in caps:
+        for i, c in enumerate(caps):
             info = L.format_info(c["version"])
             B = info["blocks"]
             f = r.ref_format_new(info["n"], info["subch"], c["span"])
@@ -103,7 +103,7 @@ def test_batch_verified_bytes_are_the_sent_blocks(caps):
             assert np.array_equal(got["ok"][at:at + B], ok)
             assert np.array_equal(got["its"][at:at + B], its)
             # and the reference's integer path on the batch's own soft values, whatever the ties
-            s = b.soft(len([x for x in caps[:caps.index(c)]]), c["version"])
+            s = b.soft(i, c["version"])
             assert_finish_equal({k: got[k][at:at + B] for k in ("ok", "its", "blocks")},
                                 ref_finish_soft(r, c["version"], c["span"], s["llr"], s["sub_est"]))
             at += B
@@ -213,12 +213,13 @@ BIG = [(64, 512, dict(W=1920, Hh=1080, rot_deg=rot, scale=sc, persp=pe, k1=k1, b
                                        (5.0, 0.88, 0.0, -0.01, 0.8, 2.0))]
 
 
-def test_lizard1024_batch_beyond_the_grid_equals_reference():
+def test_lizard1024_batch_beyond_the_grid_equals_reference(decoder_gpu):
     """Five LIZARD-1024 captures (620 blocks) through the batch graph with the LDPC grid forced to one workgroup a
     CU (256 on an MI355X), so workgroups take blocks of several frames in turn. Each frame: the reference's decline
     gate, LDPC and CRC on the batch's own soft values give the batch's verdicts, iterations and bytes; the soft values
     equal the reference's own decode of the same capture and registration (ref_decode_reg) but at rounding ties of
-    the FP64 finish; verified blocks are the blocks sent; graph, frame-by-frame and graph again agree."""
+    the GPU's floats (the single-frame path's, equal to the batch's); verified blocks are the blocks sent; graph,
+    frame-by-frame and graph again agree."""
     r = ref("scalar")
     if r is None:
         pytest.skip("oracle/_ref not built")
@@ -253,10 +254,14 @@ def test_lizard1024_batch_beyond_the_grid_equals_reference():
         r.ref_decode_reg(f, ptr(c["img"]), c["img"].shape[1], c["img"].shape[0], ptr(c["H"]), ptr(nodes), ptr(out),
                          ptr(ok), ptr(its), ptr(be), ptr(llr), ptr(al))
         grid = np.zeros((info["n"], info["n"]), np.float32)
-        r.ref_sample_grid(f, ptr(c["img"]), c["img"].shape[1], c["img"].shape[0], ptr(c["H"]), ptr(nodes), 0,
+        r.ref_sample_grid(f, ptr(c["img"]), c["img"].shape[1], c["img"].shape[0], ptr(c["H"]), ptr(nodes), 1,
                           ptr(grid))
         r.ref_format_free(f)
-        tie = assert_ties(s["llr"], llr, soft64(grid, 64, 512)["soft"])
+        rg = ref_grid_finish(r, 64, 512, grid)
+        assert np.array_equal(rg["llr"], llr)
+        one = decoder_gpu.decode_reg(64, 512, c["img"], c["H"], nodes)
+        assert np.array_equal(one["llr"], s["llr"])
+        tie = ties_against_reference(64, one, rg)
         assert tie < 1e-3, tie
         if tie == 0:
             assert np.array_equal(mine["its"], its) and np.array_equal(mine["ok"], ok)
