@@ -355,6 +355,16 @@ LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const
 LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr,
                                     const int64_t *d_off, uint8_t *d_bits, int8_t *d_its, int max_iter, int stall_it,
                                     float stall_ratio, int flags, uint8_t *d_bytes, uint8_t *d_crc_ok);
+/* No counterpart in the reference: for callers that cannot allocate device memory (code under HIP graph capture,
+   tight serving loops, harnesses that forbid hipMalloc). lizx_ldpc_decode_device with the same arguments, results
+   and stream, but its task list and queue counter live in a fixed work area the library holds on each GPU for the
+   process's life (16384 codewords, 512 KB); it calls no hipMalloc, hipFree or allocating routine. count above 16384
+   is refused (LIZ_E_ARG). Every call on one GPU shares that GPU's area: calls that overlap on one GPU (from several
+   handles or threads, or enqueued on streams that run at once) race and must be serialised by the caller. */
+LIZ_API int lizx_ldpc_decode_device_noalloc(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr,
+                                            const int64_t *d_off, uint8_t *d_bits, int8_t *d_its, int max_iter,
+                                            int stall_it, float stall_ratio, int flags, uint8_t *d_bytes,
+                                            uint8_t *d_crc_ok);
 LIZ_API int lizx_ldpc_sync(lizx_ldpc *h);
 /* The autotuner of the persistent grid: times these device-resident codewords (reps runs a candidate) at every grid
    size from as many workgroups a CU as fit down to 2, keeps the fastest for this handle's later decodes (the bits do

@@ -157,6 +157,7 @@ def lib():
     sig("lizx_ldpc_tune", _I, _P, _I, _P, _P, _P, _P, _I, _I, _F, _I, _P)
     sig("lizx_ldpc_last_ms", _F, _P)
     sig("lizx_ldpc_shape", _I, _P, _I)
+    sig("lizx_ldpc_decode_device_noalloc", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P)
     sig("lizx_ldpc_encode", _I, _I, _I, _P, _P)
     sig("lizx_crc32c", C.c_uint32, _P, _SZ)
     sig("lizx_copy_probe", _I, _P, _P, _SZ, C.POINTER(C.c_float))
@@ -371,6 +372,15 @@ class LdpcDecoder:
         check(lib().lizx_ldpc_decode_device(self.h, count, ptr(d_rate), ptr(d_llr), ptr(d_off), ptr(d_bits),
                                             ptr(d_its), max_iter, stall_it, stall_ratio, flags, ptr(d_bytes),
                                             ptr(d_crc_ok)))
+
+    def decode_device_noalloc(self, count, d_rate, d_llr, d_off, d_bits, d_its, max_iter=20, stall_it=0,
+                              stall_ratio=0.95, crc=False, plain=False, d_bytes=None, d_crc_ok=None):
+        """lizx_ldpc_decode_device_noalloc: the device decode with its scratch in the library's fixed work area."""
+        flags = (LDPC_CRC if crc else 0) | (LDPC_PLAIN if plain else 0)
+        ptr = lambda d: d.ptr if d is not None else None  # noqa: E731
+        check(lib().lizx_ldpc_decode_device_noalloc(self.h, count, ptr(d_rate), ptr(d_llr), ptr(d_off), ptr(d_bits),
+                                                    ptr(d_its), max_iter, stall_it, stall_ratio, flags, ptr(d_bytes),
+                                                    ptr(d_crc_ok)))
 
     def sync(self):
         check(lib().lizx_ldpc_sync(self.h))

@@ -451,6 +451,44 @@ LIZ_API int lizx_ldpc_decode_device(lizx_ldpc *h, int count, const uint8_t *d_ra
   });
 }
 
+// lizx_ldpc_decode_device with its task list and queue counter in the library's fixed per-device work area: nothing
+// allocated, freed or grown. The kernels are the same; only where their scratch lives differs.
+LIZ_API int lizx_ldpc_decode_device_noalloc(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr,
+                                            const int64_t *d_off, uint8_t *d_bits, int8_t *d_its, int max_iter,
+                                            int stall_it, float stall_ratio, int flags, uint8_t *d_bytes,
+                                            uint8_t *d_crc_ok) {
+  return guard([&] {
+    need(h && count >= 0 && d_rate && d_llr && d_off && d_its, "a decoder and device arrays");
+    if (count > SCRATCH_TASKS)
+      throw Error(LIZ_E_ARG, "more codewords than the fixed work area holds (16384 a call)");
+    check_params(max_iter, stall_it, stall_ratio);
+    need(!(flags & LIZX_LDPC_CRC) || (d_bytes && d_crc_ok), "CRC outputs");
+    need(!(flags & LIZX_LDPC_TRACE), "traces are a host-call option");
+    if (!count) return 0;
+    hip_check(hipSetDevice(h->eng->device), "hipSetDevice");
+    const LdpcScratch sc = ldpc_scratch(h->eng->device);  // looked up when the handle was made: no call here
+    h->eng->make_tasks(sc.tasks, d_rate, d_off, count, sc.queue);
+    LdpcParams p{};
+    p.tasks = sc.tasks;
+    p.count = count;
+    p.queue = sc.queue;
+    p.llr = d_llr;
+    p.bits = d_bits;
+    p.its = d_its;
+    p.bytes = d_bytes;
+    p.crc_ok = d_crc_ok;
+    p.crc_tab = h->eng->crc_tab.as<uint32_t>();
+    p.crc0 = h->eng->crc0;
+    p.max_iter = max_iter;
+    p.stall_it = stall_it;
+    p.stall_ratio = stall_ratio;
+    p.mode = MODE_CODEWORD;
+    p.flags = flags & LIZX_LDPC_CRC;
+    h->eng->launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true, true, h->per_cu);
+    return 0;
+  });
+}
+
 LIZ_API int lizx_ldpc_tune(lizx_ldpc *h, int count, const uint8_t *d_rate, const int8_t *d_llr, const int64_t *d_off,
                            int8_t *d_its, int max_iter, int stall_it, float stall_ratio, int reps, int *per_cu) {
   return guard([&] {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "ldpc.hpp"
 #include "ldpc_tmpl.hpp"
@@ -373,8 +374,35 @@ void LdpcEngine::frame_slots(const int4 *d_blk, int blocks, int total, const uin
   hip_check(hipGetLastError(), "frame slots launch");
 }
 
+// The fixed work area: task descriptors then the queue counter, in device memory that the code object holds for the
+// process's life (one copy per GPU, at its own address on each).
+constexpr size_t SCRATCH_BYTES = SCRATCH_TASKS * sizeof(LdpcTask) + 256;
+static __device__ __align__(256) uint8_t scratch[SCRATCH_BYTES];
+
+namespace {
+std::mutex scratch_mu;
+void *scratch_at[64];  // per device: the array's device address, looked up with that device current
+}  // namespace
+
+LdpcScratch ldpc_scratch(int dev) {
+  if (dev < 0 || dev >= 64) throw Error(LIZ_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (!scratch_at[dev]) {
+    int cur = 0;
+    hip_check(hipGetDevice(&cur), "hipGetDevice");
+    if (cur != dev) throw Error(LIZ_E_INTERNAL, "the scratch area is looked up with its device current");
+    void *ptr = nullptr;
+    hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress (LDPC work area)");
+    scratch_at[dev] = ptr;
+  }
+  uint8_t *base = static_cast<uint8_t *>(scratch_at[dev]);
+  return LdpcScratch{reinterpret_cast<LdpcTask *>(base),
+                     reinterpret_cast<int *>(base + SCRATCH_TASKS * sizeof(LdpcTask))};
+}
+
 LdpcEngine::LdpcEngine(int dev) : device(dev) {
   hip_check(hipSetDevice(device), "hipSetDevice");
+  (void)ldpc_scratch(device);  // the fixed work area's address, once per device, with the device current
   hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
   hip_check(hipEventCreate(&ev0), "hipEventCreate");
   hip_check(hipEventCreate(&ev1), "hipEventCreate");
@@ -409,8 +437,9 @@ void LdpcEngine::ensure_frame_tables(int max_slots) {
   }
 }
 
-void LdpcEngine::make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count) {
-  hipLaunchKernelGGL(k_make_tasks, dim3(1), dim3(1024), 0, stream, d_tasks, d_rate, d_off, count, queue.as<int>());
+void LdpcEngine::make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count, int *d_queue) {
+  hipLaunchKernelGGL(k_make_tasks, dim3(1), dim3(1024), 0, stream, d_tasks, d_rate, d_off, count,
+                     d_queue ? d_queue : queue.as<int>());
   hip_check(hipGetLastError(), "task launch");
 }
 
@@ -431,7 +460,7 @@ void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed, bool queue
       hipLaunchKernelGGL(k_ldpc, dim3(1), dim3(LDPC_THREADS), 0, stream, q);
     }
   } else {
-    p.queue = queue.as<int>();
+    p.queue = p0.queue ? p0.queue : queue.as<int>();
     if (!queue_zeroed) hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, stream, p.queue, 1);
     const int grid = std::min(p.count, cus * (per_cu > 0 ? std::min(per_cu, wg_per_cu) : wg_per_cu));
     hipLaunchKernelGGL(k_ldpc, dim3(grid), dim3(LDPC_THREADS), 0, stream, p);

@@ -71,8 +71,10 @@ struct LdpcEngine {
   // Enqueue a decode of count tasks already on the device. Plain: one launch per task. Timed: HIP events bracket the
   // launches from the first since the last sync() to the last.
   // per_cu > 0 caps the persistent grid at that many workgroups a CU (a launch shape: the bits do not change).
+  // A non-null p.queue is the queue counter to use (zeroed by make_tasks); null: the engine's own.
   void launch(const LdpcParams &p, bool plain, bool timed, bool queue_zeroed = false, int per_cu = 0);
-  void make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count);
+  // Writes the task list and zeroes the queue counter (d_queue, or null for the engine's own).
+  void make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count, int *d_queue = nullptr);
   void sync();
   void ensure_frame_tables(int max_slots);
   // The frame slots (0/1 bytes, total a frame) of frames x blocks x 476 bytes on the device, enqueued on the stream;
@@ -82,5 +84,16 @@ struct LdpcEngine {
 };
 
 float copy_probe(void *dst, const void *src, size_t bytes);
+
+// The device-resident work area of the decode that allocates nothing (lizx_ldpc_decode_device_fixed): one per GPU
+// for the process's life, shared by every call on that GPU. SCRATCH_TASKS task descriptors, then the queue counter.
+constexpr int SCRATCH_TASKS = 16384;
+struct LdpcScratch {
+  LdpcTask *tasks;
+  int *queue;
+};
+// This device's work area (its address looked up once per device, with that device current); throws if the lookup
+// fails.
+LdpcScratch ldpc_scratch(int device);
 
 }  // namespace lz
