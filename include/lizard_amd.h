@@ -252,6 +252,11 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
                             const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
                             lizx_soft *soft);
 
+/* Whether the decoder replays its single-frame decodes (liz_decode with a registration held or found,
+   lizx_decode_reg) as one HIP graph of the frame's kernels, captured once per format and work areas (on, 1, the
+   default), or launches them one by one (0). Results are the same bits either way. No counterpart in the reference
+   (liblizard's decoder runs on the CPU, api/codec.cpp:121-160). */
+LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on);
 /* Holds a registration for liz_decode (the state lizard_acquire leaves for lizard_finish); reg NULL lets it go, and
    liz_decode then finds each frame's symbol itself. The nodes are copied.
    span 128/256/512. version 0: the frames' words (or *held) name the format; 4..64: the format for frames whose word

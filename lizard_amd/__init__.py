@@ -136,6 +136,7 @@ def lib():
     sig("lizx_warp", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P)
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
+    sig("lizx_decoder_graphs", _I, _P, _I)
     sig("lizx_find", _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_find_batch", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_new", _P, _I)
@@ -703,6 +704,10 @@ class Decoder:
         check(lib().lizx_warp(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
                               C.byref(reg), _p(grid)))
         return grid
+
+    def graphs(self, on):
+        """Single-frame decodes replayed as one captured HIP graph (lizx_decoder_graphs; on by default)."""
+        check(lib().lizx_decoder_graphs(self.h, 1 if on else 0))
 
     def hold(self, version, span, H, nodes):
         self._held = registration(H, nodes)

@@ -117,6 +117,7 @@ struct liz_decoder {
   std::vector<std::unique_ptr<FramePlan>> plans;  // one per (version, span) met
   FrameBufs fb;
   DevBuf img, nodes, wframe;
+  PinnedBuf img_host;  // the capture's box staged in pinned memory, then one contiguous copy
   WarpFrame wframe_host{};
   std::vector<uint8_t> luma;
   // registration refinement
@@ -132,6 +133,21 @@ struct liz_decoder {
   FinderBufs finder;
   DevBuf b3_a, b3_b, b3_out;
   std::vector<float> found_nodes;
+  // the single-frame decode's kernels as one HIP graph, kept for the format and work areas it was captured with
+  bool use_graphs = true;
+  struct FrameGraph {
+    int version = 0, span = 0;
+    bool dbg = false;
+    const void *keys[8] = {};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } fg;
+  void drop_frame_graph() {
+    if (fg.exec) (void)hipGraphExecDestroy(fg.exec);
+    if (fg.graph) (void)hipGraphDestroy(fg.graph);
+    fg = FrameGraph{};
+  }
+  ~liz_decoder() { drop_frame_graph(); }
   const FramePlan &plan(int version, int span) {
     for (auto &p : plans)
       if (p->version == version && p->span == span) return *p;
@@ -141,8 +157,9 @@ struct liz_decoder {
 };
 
 // The decline gate, LDPC and CRC of one frame's blocks from soft values and estimates already on the device
-// (lizard_phy.c:1775-1870), enqueued on the decoder's stream.
-static void finish_blocks(liz_decoder *d, const Format *f, const int8_t *d_llr, const double *d_est) {
+// (lizard_phy.c:1775-1870): the tables and the frame's task list made ready (host copies), and the launch's
+// parameters.
+static LdpcParams finish_params(liz_decoder *d, const Format *f, const int8_t *d_llr, const double *d_est) {
   LdpcEngine &e = *d->eng;
   e.ensure_frame_tables(static_cast<int>(f->white.size()));
   const int B = f->blocks;
@@ -176,7 +193,12 @@ static void finish_blocks(liz_decoder *d, const Format *f, const int8_t *d_llr, 
   p.stall_ratio = 0.95f;
   p.mode = MODE_FRAME;
   p.flags = 1;
-  e.launch(p, false, true);
+  return p;
+}
+
+// The same, enqueued on the decoder's stream.
+static void finish_blocks(liz_decoder *d, const Format *f, const int8_t *d_llr, const double *d_est) {
+  d->eng->launch(finish_params(d, f, d_llr, d_est), false, true);
 }
 
 // The frame's results area's pilot readings (where the picture stages write them).
@@ -631,6 +653,15 @@ LIZ_API liz_decoder *liz_decoder_new(int nmax) {
 
 LIZ_API void liz_decoder_free(liz_decoder *d) { delete d; }
 
+LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on) {
+  return guard([&] {
+    need(d, "a decoder");
+    d->use_graphs = on != 0;
+    if (!on) d->drop_frame_graph();
+    return 0;
+  });
+}
+
 LIZ_API int liz_decoder_max_blocks(const liz_decoder *d) {
   return guard([&] {
     need(d, "a decoder");
@@ -766,26 +797,73 @@ LIZ_API int liz_decode(liz_decoder *d, const uint8_t *px, int w, int h, int stri
   });
 }
 
-// A capture and its registration onto the device as a WarpFrame (one frame), on the decoder's stream. Every entry
-// that stages a capture synchronises the stream before it returns, so the caller's buffers are read in time.
+// The pixels of a w x h capture the warp through this registration can read, as [x0, x1) x [y0, y1): the picture's
+// sample square widened by a module each way, through H (its corners bound a projective image while the denominator
+// stays positive at all four), widened by the largest node displacement and by the bilinear read's next pixel and two
+// more, clamped to the frame. The whole frame where the denominator does not stay positive.
+static void read_box(const FramePlan &p, const lizx_registration *reg, int w, int h, int box[4]) {
+  box[0] = 0, box[1] = 0, box[2] = w, box[3] = h;
+  const double lo = 14.0, hi = 16.0 + p.span;  // module coordinates: the sampled square is 15 .. 15 + span
+  const float *H = reg->H;
+  double xmin = 1e300, xmax = -1e300, ymin = 1e300, ymax = -1e300;
+  for (int c = 0; c < 4; c++) {
+    const double mx = (c & 1) ? hi : lo, my = (c & 2) ? hi : lo;
+    const double ww = static_cast<double>(H[6]) * mx + static_cast<double>(H[7]) * my + H[8];
+    if (!(ww > 1e-6)) return;
+    const double x = (static_cast<double>(H[0]) * mx + static_cast<double>(H[1]) * my + H[2]) / ww;
+    const double y = (static_cast<double>(H[3]) * mx + static_cast<double>(H[4]) * my + H[5]) / ww;
+    if (!(std::fabs(x) < 1e7 && std::fabs(y) < 1e7)) return;
+    xmin = std::min(xmin, x), xmax = std::max(xmax, x), ymin = std::min(ymin, y), ymax = std::max(ymax, y);
+  }
+  double dmx = 0, dmy = 0;
+  const int nn = p.dev.nx * p.dev.ny;
+  for (int i = 0; i < nn; i++) {
+    const double ax = std::fabs(static_cast<double>(reg->nodes[2 * i])), ay = std::fabs(static_cast<double>(reg->nodes[2 * i + 1]));
+    if (!(ax < 1e6 && ay < 1e6)) return;  // NaN or wild nodes: the whole frame
+    dmx = std::max(dmx, ax), dmy = std::max(dmy, ay);
+  }
+  const auto clampi = [](double v, int a, int b) { return v < a ? a : v > b ? b : static_cast<int>(v); };
+  box[0] = clampi(std::floor(xmin - dmx - 0.5) - 2, 0, w);
+  box[1] = clampi(std::floor(ymin - dmy - 0.5) - 2, 0, h);
+  box[2] = clampi(std::floor(xmax + dmx - 0.5) + 4, 0, w);
+  box[3] = clampi(std::floor(ymax + dmy - 0.5) + 4, 0, h);
+  if (box[2] - box[0] < 2 || box[3] - box[1] < 2) box[0] = 0, box[1] = 0, box[2] = w, box[3] = h;
+}
+
+// A capture and its registration onto the device as a WarpFrame (one frame), on the decoder's stream: only the
+// pixels the warp can reach through this registration when crop (read_box), else the whole frame (the refinement
+// moves the registration on the device). Every entry that stages a capture synchronises the stream before it
+// returns, so the caller's buffers are read in time.
 static const WarpFrame *stage_capture(liz_decoder *d, const FramePlan &p, const uint8_t *img, int w, int h, int stride,
-                                      const lizx_registration *reg) {
+                                      const lizx_registration *reg, bool crop = false) {
   need(img && reg && reg->nodes, "a capture and a registration with nodes");
   need(w >= 2 && h >= 2 && stride >= w, "a capture of at least 2 x 2 pixels, rows stride >= w bytes apart");
   LdpcEngine &e = *d->eng;
-  const size_t npix = static_cast<size_t>(w) * h, nn = static_cast<size_t>(p.dev.nx) * p.dev.ny;
-  d->img.reserve(npix);
-  if (stride == w) {
-    hip_check(hipMemcpyAsync(d->img.p, img, npix, hipMemcpyHostToDevice, e.stream), "capture");
+  const size_t nn = static_cast<size_t>(p.dev.nx) * p.dev.ny;
+  int box[4] = {0, 0, w, h};
+  if (crop) read_box(p, reg, w, h, box);
+  const int bw = box[2] - box[0], bh = box[3] - box[1];
+  d->img.reserve(static_cast<size_t>(w) * h);  // the whole frame's room, so a later larger box never reallocates
+  const uint8_t *src = img + static_cast<size_t>(box[1]) * stride + box[0];
+  const size_t bytes = static_cast<size_t>(bw) * bh;
+  if (bw == stride) {
+    hip_check(hipMemcpyAsync(d->img.p, src, bytes, hipMemcpyHostToDevice, e.stream), "capture");
   } else {
-    hip_check(hipMemcpy2DAsync(d->img.p, w, img, stride, w, h, hipMemcpyHostToDevice, e.stream), "capture");
+    // the box's rows gathered into pinned memory by the host (a strided copy from pageable memory is slow), then one
+    // copy; the pinned area is not reused before the stream has read it (every entry syncs before it returns)
+    d->img_host.reserve(static_cast<size_t>(w) * h);
+    uint8_t *stage = static_cast<uint8_t *>(d->img_host.p);
+    for (int y = 0; y < bh; y++) std::memcpy(stage + static_cast<size_t>(y) * bw, src + static_cast<size_t>(y) * stride, bw);
+    hip_check(hipMemcpyAsync(d->img.p, stage, bytes, hipMemcpyHostToDevice, e.stream), "capture");
   }
   d->nodes.reserve(nn * sizeof(float2));
   hip_check(hipMemcpyAsync(d->nodes.p, reg->nodes, nn * sizeof(float2), hipMemcpyHostToDevice, e.stream), "nodes");
   WarpFrame wf{};
   for (int k = 0; k < 9; k++) wf.H[k] = reg->H[k];
   wf.nodes = d->nodes.as<float2>();
-  wf.img = d->img.as<uint8_t>();
+  // pixel (x, y) at img + y bw + x: the stored box's first pixel is (box[0], box[1])
+  wf.img = d->img.as<uint8_t>() - (static_cast<ptrdiff_t>(box[1]) * bw + box[0]);
+  wf.stride = bw;
   wf.w = w;
   wf.h = h;
   d->wframe.reserve(sizeof(WarpFrame));
@@ -820,13 +898,43 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
   LdpcEngine &e = *d->eng;
   hip_check(hipSetDevice(e.device), "hipSetDevice");
   const FramePlan &p = d->plan(version, span);
-  const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg);
+  const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg, true);
   const bool dbg = soft && (soft->coef || soft->soft);
   d->fb.reserve(p, 1, dbg);
   SoftBufs v = d->fb.view(dbg);
   v.blk_pilot = pilot_area(d, f);
-  launch_soft_warp(p, v, wf, 1, e.stream);
-  finish_blocks(d, f, v.llr, v.est);
+  const LdpcParams lp = finish_params(d, f, v.llr, v.est);
+  if (!d->use_graphs) {
+    launch_soft_warp(p, v, wf, 1, e.stream);
+    e.launch(lp, false, true);
+  } else {
+    // the frame's kernels (picture stages, queue reset, LDPC) as one graph: captured once for this format and these
+    // work areas (the capture and registration sit at fixed device addresses the kernels read), then replayed
+    auto &g = d->fg;
+    const void *keys[8] = {v.X, v.C, v.llr, v.coef_dbg, d->res.p, d->tasks.p, wf, e.white.p};
+    bool same = g.exec && g.version == version && g.span == span && g.dbg == dbg;
+    for (int k = 0; same && k < 8; k++) same = g.keys[k] == keys[k];
+    if (!same) {
+      d->drop_frame_graph();
+      hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      try {
+        launch_soft_warp(p, v, wf, 1, e.stream);
+        e.launch(lp, false, false);
+      } catch (...) {
+        hipGraph_t gg = nullptr;
+        (void)hipStreamEndCapture(e.stream, &gg);
+        if (gg) (void)hipGraphDestroy(gg);
+        throw;
+      }
+      hip_check(hipStreamEndCapture(e.stream, &g.graph), "hipStreamEndCapture");
+      hip_check(hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+      g.version = version;
+      g.span = span;
+      g.dbg = dbg;
+      for (int k = 0; k < 8; k++) g.keys[k] = keys[k];
+    }
+    hip_check(hipGraphLaunch(g.exec, e.stream), "hipGraphLaunch");
+  }
   const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr);
   if (soft) copy_soft(d, p, soft, dbg);
   return good;
@@ -1452,7 +1560,7 @@ LIZ_API int lizx_warp(liz_decoder *d, int version, int span, const uint8_t *img,
     LdpcEngine &e = *d->eng;
     hip_check(hipSetDevice(e.device), "hipSetDevice");
     const FramePlan &p = d->plan(version, span);
-    const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg);
+    const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, reg, true);
     const size_t n = p.n;
     d->grid.reserve(n * n * sizeof(float));
     launch_warp(p, wf, 1, d->grid.as<float>(), e.stream);

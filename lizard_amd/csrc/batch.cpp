@@ -297,6 +297,7 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
         w.img = in.img;
         w.w = in.w;
         w.h = in.h;
+        w.stride = in.w;
         b->host_frames[at] = w;
       }
     }

@@ -1,6 +1,6 @@
-// Layered min-sum LDPC + CRC-32C on gfx950: one codeword per wave (64 lanes, each owning two check positions of the
-// current block row), the codeword's int16 posteriors in LDS and its int8 messages in VGPRs, the code's tables
-// compiled in (ldpc_tmpl.hpp). A persistent grid pulls codewords off an atomic queue ordered by code, so codewords of
+// Layered min-sum LDPC + CRC-32C on gfx950: one codeword per 128-lane workgroup (two waves; lane i owns check position
+// i of every block row, a workgroup barrier between block rows), the codeword's int16 posteriors in LDS and its int8
+// messages in VGPRs, the code's tables compiled in (ldpc_tmpl.hpp). A persistent grid pulls codewords off an atomic queue ordered by code, so codewords of
 // four rates with uneven iteration counts share one launch; the plain path launches one workgroup per codeword.
 //
 // Follows liblizard/src/ldpc.c:1080-1172 (ldpc_decode_stall) and, for frame mode, liblizard/src/lizard_phy.c:1775-1870

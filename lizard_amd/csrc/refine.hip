@@ -75,11 +75,11 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
       const int xi = static_cast<int>(xs), yi = static_cast<int>(ys);
       fxs[j] = xs - xi;
       fys[j] = ys - yi;
-      const uint8_t *p = F.img + static_cast<size_t>(yi) * F.w + xi;
+      const uint8_t *p = F.img + static_cast<ptrdiff_t>(yi) * F.stride + xi;
       pa[j] = p[0];
       pb[j] = p[1];
-      pc[j] = p[F.w];
-      pd[j] = p[F.w + 1];
+      pc[j] = p[F.stride];
+      pd[j] = p[F.stride + 1];
       rs[j] = ref[static_cast<size_t>(y) * n + x];
       mxs[j] = mx;
       mys[j] = my;

@@ -253,17 +253,42 @@ __device__ __forceinline__ float hole_value(const PlanDev &P, const double *beta
   return static_cast<float>(v);
 }
 
-template <int NT>
-__device__ double block_sum(double v, double *red, int tid) {
-  red[tid] = v;
+// K sums over the workgroup at once, each by the same fixed tree (level o = NT/2, ..., 1 adds element t + o to element
+// t, t < o), so the sums are run-to-run identical: the levels across waves through LDS, the last six within wave 0 by
+// shuffles. Every thread gets the sums. red holds K * NT doubles.
+template <int NT, int K>
+__device__ void block_sums(double (&v)[K], double *red, int tid) {
+  static_assert(NT >= 64 && (NT & (NT - 1)) == 0, "a power-of-two workgroup of whole waves");
+#pragma unroll
+  for (int k = 0; k < K; k++) red[k * NT + tid] = v[k];
   __syncthreads();
-  for (int o = NT / 2; o > 0; o >>= 1) {
-    if (tid < o) red[tid] += red[tid + o];
+  for (int o = NT / 2; o >= 64; o >>= 1) {
+    if (tid < o) {
+#pragma unroll
+      for (int k = 0; k < K; k++) red[k * NT + tid] += red[k * NT + tid + o];
+    }
     __syncthreads();
   }
-  const double r = red[0];
+  if (tid < 64) {
+    double x[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) x[k] = red[k * NT + tid];
+    for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        const double y = __shfl_down(x[k], o, 64);
+        if (tid < o) x[k] += y;
+      }
+    }
+    if (tid == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) red[k * NT] = x[k];
+    }
+  }
   __syncthreads();
-  return r;
+#pragma unroll
+  for (int k = 0; k < K; k++) v[k] = red[k * NT];
+  __syncthreads();
 }
 
 // ------------------------------------------------------------------------------------------------ the warp
@@ -285,8 +310,8 @@ __device__ __forceinline__ float sample(const WarpFrame &F, float x, float y) {
   if (y < 0) y = 0; else if (y > ym) y = ym;
   const int xi = x >= 0 ? static_cast<int>(x) : 0, yi = y >= 0 ? static_cast<int>(y) : 0;
   const float fx = __fsub_rn(x, static_cast<float>(xi)), fy = __fsub_rn(y, static_cast<float>(yi));
-  const uint8_t *p = F.img + static_cast<size_t>(yi) * F.w + xi;
-  const float a = grey(p[0]), b = grey(p[1]), c = grey(p[F.w]), d = grey(p[F.w + 1]);
+  const uint8_t *p = F.img + static_cast<ptrdiff_t>(yi) * F.stride + xi;
+  const float a = grey(p[0]), b = grey(p[1]), c = grey(p[F.stride]), d = grey(p[F.stride + 1]);
   const float top = __fadd_rn(a, __fmul_rn(__fsub_rn(b, a), fx)), bot = __fadd_rn(c, __fmul_rn(__fsub_rn(d, c), fx));
   return __fadd_rn(__fmul_rn(top, __fsub_rn(1.0f, fy)), __fmul_rn(bot, fy));
 }
@@ -577,7 +602,7 @@ __device__ __forceinline__ void coef_uv(const PlanDev &P, int i, int *u, int *v)
 // The projections from the row sums, then the pilots' grid shift (lizard_phy.c:280-329, :1697, :1739).
 __global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, const float2 *C_all, double *proj_out,
                                                float *align_out) {
-  __shared__ double red[256];
+  __shared__ double red[6 * 256];
   __shared__ double s_d[2];
   __shared__ int s_go;
   const int tid = threadIdx.x, f = blockIdx.x, n = P.n;
@@ -592,11 +617,13 @@ __global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, 
     a4 += rs * P.qx[y];
   }
   Proj pr;
-  pr.px = block_sum<256>(a0, red, tid) / (n * P.sx2);
-  pr.py = block_sum<256>(a1, red, tid) / (n * P.sx2);
-  pr.pxy = block_sum<256>(a2, red, tid) / (P.sx2 * P.sx2);
-  pr.pxx = block_sum<256>(a3, red, tid) / (n * P.sq2);
-  pr.pyy = block_sum<256>(a4, red, tid) / (n * P.sq2);
+  double sums[5] = {a0, a1, a2, a3, a4};
+  block_sums<256, 5>(sums, red, tid);
+  pr.px = sums[0] / (n * P.sx2);
+  pr.py = sums[1] / (n * P.sx2);
+  pr.pxy = sums[2] / (P.sx2 * P.sx2);
+  pr.pxx = sums[3] / (n * P.sq2);
+  pr.pyy = sums[4] / (n * P.sq2);
   if (tid == 0) {
     double *o = proj_out + f * 5;
     o[0] = pr.px;
@@ -630,12 +657,14 @@ __global__ __launch_bounds__(256) void k_align(PlanDev P, const double *rowsum, 
       q += cv * zi;
       yy += zi * zi;
     }
-    A = block_sum<256>(a, red, tid);
-    B = block_sum<256>(b, red, tid);
-    C = block_sum<256>(c, red, tid);
-    Pq = block_sum<256>(p, red, tid);
-    Q = block_sum<256>(q, red, tid);
-    YY = block_sum<256>(yy, red, tid);
+    double six[6] = {a, b, c, p, q, yy};
+    block_sums<256, 6>(six, red, tid);
+    A = six[0];
+    B = six[1];
+    C = six[2];
+    Pq = six[3];
+    Q = six[4];
+    YY = six[5];
     if (tid == 0) {
       const double det = A * C - B * B;
       if (P.npilots < 8 || !(det > 0)) {

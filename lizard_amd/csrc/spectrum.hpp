@@ -24,11 +24,15 @@ constexpr int FIT_STRIDE = FIT_BETA + 8;              // the frame's shading bet
 
 // One frame's registration and capture for the warp: H maps module coordinates to capture pixels (h8 = 1), nodes
 // the node lattice's displacements in pixels (ny rows of nx), img a tight uint8 luma capture of w x h.
+// A capture and its registration as the warp reads them. w, h: the whole frame (the read's clamp); pixel (x, y) at
+// img + y stride + x, where only the pixels the warp can reach need be stored (img then points before the stored
+// rectangle by its offset).
 struct WarpFrame {
   float H[9];
   const float2 *nodes;
   const uint8_t *img;
   int w, h;
+  int stride;
 };
 
 // Device-side view of a plan (by value into kernels).

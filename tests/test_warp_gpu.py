@@ -172,3 +172,33 @@ def test_liz_decode_without_a_word():
     k, got, out, held = dec.decode(img, held=64)
     assert k == 0 and out.n == 0 and out.held_used == 1
     dec.close()
+
+
+def test_single_frame_graph_replays_equal_eager(decoder_gpu):
+    """The single-frame decode's kernels replayed as a captured graph (the default) give the launch-by-launch
+    decode's bits on every replay, with other eager work of the library between replays, across two formats (each
+    change of format or work area captures anew) and back."""
+    caps = [Capture(5, 128, seed=31, blur=0.7, noise=3.0), Capture(5, 128, seed=32, blur=0.9, noise=6.0),
+            Capture(20, 256, seed=33, W=1280, Hh=720, scale=0.9, blur=0.7, noise=4.0)]
+    dec = lizard_amd.Decoder(1536)
+    try:
+        dec.graphs(False)
+        want = [dec.decode_reg(c.version, c.span, c.img, c.H, c.nodes, soft=False) for c in caps]
+        dec.graphs(True)
+        for rnd in range(3):
+            for i in (0, 1, 2, 0):
+                c = caps[i]
+                got = dec.decode_reg(c.version, c.span, c.img, c.H, c.nodes, soft=False)
+                for k in ("its", "ok", "est"):
+                    assert np.array_equal(got[k], want[i][k]), (rnd, i, k)
+                v = want[i]["ok"].astype(bool)
+                assert np.array_equal(got["blocks"][v], want[i]["blocks"][v])
+                # eager work of the library between replays: a grid decode with its own launches
+                decoder_gpu.decode_reg(caps[1].version, caps[1].span, caps[1].img, caps[1].H, caps[1].nodes)
+        # liz_decode through a hold takes the same graph path
+        dec.hold(5, 128, caps[0].H, caps[0].nodes)
+        k, blocks, out, held = dec.decode(caps[0].img)
+        assert k == int(want[0]["ok"].sum())
+        assert np.array_equal(blocks, want[0]["blocks"][want[0]["ok"].astype(bool)])
+    finally:
+        dec.close()
