@@ -390,6 +390,19 @@ LIZ_API int lizx_ldpc_shape(lizx_ldpc *h, int per_cu);
    after row. The node lattice's displacements are k1 times the lens field at each node's place under a reference
    homography H0 (9 floats) plus the anchors' cubic Hermite mesh (interior nodes; 0 at the lattice's ends). */
 #define LIZX_REG_PARAMS 107
+/* The node lattice's interior from its border as the reference builds it (liblizard/src/acquire.c:1531-1605
+   border_fill_axis): the lens model of the border (the first-order change of H, an affine map, a shift and two
+   perspective terms, beside K = 0 to 3 radial terms k |q|^(2k) q about the camera's axis, q over the farthest frame
+   corner's distance; acquire.c:1470-1512) fitted by least squares to the corners and the border nodes use[] trusts
+   (null: all), K chosen by AICc and at most 1 where the axis lies inside the lattice's quad; then every interior node is
+   the model plus the Coons patch of what the model leaves on the border. nodes: nx * nx (dx, dy) pixels, host, the
+   border read and the interior written; axis: 2 floats (null or NaN: the frame's centre). model (optional, 15
+   doubles) receives K, the axis x, y, R and the 11 coefficients (zero past 8 + K). Returns K (-1: no model fitted). */
+LIZ_API int lizx_reg_border_fill(int span, const float *H, int w, int h, const float *axis, float *nodes,
+                                 const uint8_t *use, double *model);
+/* A lens model's displacement (model as lizx_reg_border_fill gives it) at every node's place under H (lensm_at,
+   acquire.c:1615-1620); returns nx. */
+LIZ_API int lizx_reg_lens_field(int span, const float *H, const double *model, float *nodes);
 /* The nodes (ny * nx (dx, dy) pairs, host) of these parameters in a w x h frame; returns nx. */
 LIZ_API int lizx_reg_nodes(int span, const float *H0, int w, int h, const float *params, float *nodes);
 /* The refinement loss sum (w - alpha r - beta)^2 over the picture's samples, w the capture warped through params, r

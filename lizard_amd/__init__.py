@@ -158,6 +158,8 @@ def lib():
     sig("lizx_ldpc_tune", _I, _P, _I, _P, _P, _P, _P, _I, _I, _F, _I, _P)
     sig("lizx_ldpc_last_ms", _F, _P)
     sig("lizx_ldpc_shape", _I, _P, _I)
+    sig("lizx_reg_border_fill", _I, _I, _P, _I, _I, _P, _P, _P, _P)
+    sig("lizx_reg_lens_field", _I, _I, _P, _P, _P)
     sig("lizx_ldpc_decode_device_noalloc", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P)
     sig("lizx_ldpc_encode", _I, _I, _I, _P, _P)
     sig("lizx_crc32c", C.c_uint32, _P, _SZ)
@@ -240,6 +242,26 @@ def reg_nodes(span, H0, w, h, params):
     out = np.zeros((nx, nx, 2), np.float32)
     check(lib().lizx_reg_nodes(span, _p(np.ascontiguousarray(H0, np.float32).ravel()), w, h,
                                _p(np.ascontiguousarray(params, np.float32)), _p(out)))
+    return out
+
+
+def border_fill(span, H, w, h, nodes, use=None, axis=None):
+    """The interior of the node lattice from its border (lizx_reg_border_fill): (nodes, K, model[15])."""
+    nodes = np.array(nodes, np.float32, copy=True)
+    model = np.zeros(15, np.float64)
+    H = np.ascontiguousarray(H, np.float32)
+    u = None if use is None else np.ascontiguousarray(use, np.uint8)
+    ax = None if axis is None else np.ascontiguousarray(axis, np.float32)
+    K = check(lib().lizx_reg_border_fill(span, _p(H), w, h, _p(ax), _p(nodes), _p(u), _p(model)))
+    return nodes, K, model
+
+
+def lens_field(span, H, model):
+    """A lens model's displacement at every node (lizx_reg_lens_field)."""
+    nx = len(node_lattice(span))
+    out = np.zeros((nx, nx, 2), np.float32)
+    check(lib().lizx_reg_lens_field(span, _p(np.ascontiguousarray(H, np.float32)),
+                                    _p(np.ascontiguousarray(model, np.float64)), _p(out)))
     return out
 
 

@@ -1722,6 +1722,54 @@ static const WarpFrame *stage_refinement(liz_decoder *d, const FramePlan &p, con
   return wf;
 }
 
+LIZ_API int lizx_reg_border_fill(int span, const float *H, int w, int h, const float *axis, float *nodes,
+                                 const uint8_t *use, double *model) {
+  return guard([&] {
+    need(H && nodes && w > 0 && h > 0, "a homography, nodes and a frame size");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    LensModel m;
+    const float nan = __builtin_nanf("");
+    const int K = border_fill(span, H, w, h, axis ? axis[0] : nan, axis ? axis[1] : nan, nodes, use, &m);
+    if (model) {
+      model[0] = m.K;
+      model[1] = m.ax;
+      model[2] = m.ay;
+      model[3] = m.R;
+      for (int a = 0; a < LENS_P; a++) model[4 + a] = m.t[a];
+    }
+    return K;
+  });
+}
+
+LIZ_API int lizx_reg_lens_field(int span, const float *H, const double *model, float *nodes) {
+  return guard([&] {
+    need(H && model && nodes, "a homography, a lens model and a place for the nodes");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    LensModel m;
+    m.K = static_cast<int>(model[0]);
+    need(m.K >= -1 && m.K <= LENS_TERMS, "a lens model of 0 to 3 radial terms (or -1: none)");
+    m.ax = model[1];
+    m.ay = model[2];
+    m.R = model[3];
+    need(m.R > 0, "a positive radius");
+    for (int a = 0; a < LENS_P; a++) m.t[a] = model[4 + a];
+    float pos[MAX_NODES];
+    const int nx = node_lattice(span, pos);
+    for (int j = 0; j < nx; j++)
+      for (int i = 0; i < nx; i++) {
+        float *o = nodes + 2 * (j * nx + i);
+        if (m.K < 0) {
+          o[0] = o[1] = 0.f;
+          continue;
+        }
+        const float mx = pos[i], my = pos[j], ww = H[6] * mx + H[7] * my + 1;
+        const float u = (H[0] * mx + H[1] * my + H[2]) / ww, v = (H[3] * mx + H[4] * my + H[5]) / ww;
+        lens_at(m, u, v, &o[0], &o[1]);
+      }
+    return nx;
+  });
+}
+
 LIZ_API int lizx_reg_nodes(int span, const float *H0, int w, int h, const float *params, float *nodes) {
   return guard([&] {
     need(H0 && params && nodes && w > 0 && h > 0, "a reference homography, parameters, a frame size and an output");

@@ -3,6 +3,7 @@
 #include "register.hpp"
 
 #include <cmath>
+#include <utility>
 
 namespace lz {
 
@@ -44,6 +45,193 @@ void herm4(const double *t, int n, double x, int *b, double w[4]) {
 }
 
 }  // namespace
+
+// ---- the border's lens model and the Coons fill (acquire.c:1470-1605), in the reference's order of float and double
+// operations so the nodes are its to the bit
+
+namespace {
+
+// H applied to a module point in float, the reference's order (internal.h:107-112).
+void project_f(const float *h, float x, float y, float *u, float *v) {
+  const float w = h[6] * x + h[7] * y + 1;
+  *u = (h[0] * x + h[1] * y + h[2]) / w;
+  *v = (h[3] * x + h[4] * y + h[5]) / w;
+}
+
+// The model's rows at q = (x, y): x, y, the shift and the two perspective terms on each axis, then K radial terms.
+void lens_rows(double x, double y, int K, double *rx, double *ry) {
+  rx[0] = x, rx[1] = y, rx[2] = 0, rx[3] = 0, rx[4] = 1, rx[5] = 0, rx[6] = -x * x, rx[7] = -x * y;
+  ry[0] = 0, ry[1] = 0, ry[2] = x, ry[3] = y, ry[4] = 0, ry[5] = 1, ry[6] = -x * y, ry[7] = -y * y;
+  const double s = x * x + y * y;
+  double sk = s;
+  for (int k = 0; k < K; k++, sk *= s) {
+    rx[8 + k] = sk * x;
+    ry[8 + k] = sk * y;
+  }
+}
+
+// Gauss-Jordan with partial pivots on the first n rows and columns of [A | b] (b in column LENS_P).
+bool solve_lens(double A[LENS_P][LENS_P + 1], int n) {
+  for (int c = 0; c < n; c++) {
+    int p = c;
+    for (int r = c + 1; r < n; r++)
+      if (std::fabs(A[r][c]) > std::fabs(A[p][c])) p = r;
+    if (std::fabs(A[p][c]) < 1e-14) return false;
+    for (int k = 0; k <= LENS_P; k++) std::swap(A[c][k], A[p][k]);
+    for (int r = 0; r < n; r++) {
+      if (r == c) continue;
+      const double f = A[r][c] / A[c][c];
+      for (int k = c; k <= LENS_P; k++) A[r][k] -= f * A[c][k];
+    }
+  }
+  return true;
+}
+
+// The least-squares model with K radial terms over the nodes in use; coefficients in t.
+bool lens_fit(int K, int nn, const std::vector<int> &use, const std::vector<float> &qx, const std::vector<float> &qy,
+              const float *nodes, double *t) {
+  const int p = 8 + K;
+  double A[LENS_P][LENS_P + 1] = {}, rx[LENS_P], ry[LENS_P];
+  int n = 0;
+  for (int m = 0; m < nn; m++) {
+    if (!use[m]) continue;
+    n++;
+    lens_rows(qx[m], qy[m], K, rx, ry);
+    for (int a = 0; a < p; a++) {
+      for (int b = 0; b < p; b++) A[a][b] += rx[a] * rx[b] + ry[a] * ry[b];
+      A[a][LENS_P] += rx[a] * nodes[2 * m] + ry[a] * nodes[2 * m + 1];
+    }
+  }
+  for (int a = 0; a < p; a++) A[a][a] += 1e-12;
+  if (2 * n <= p || !solve_lens(A, p)) return false;
+  for (int a = 0; a < p; a++) t[a] = A[a][LENS_P] / A[a][a];
+  return true;
+}
+
+}  // namespace
+
+void lens_axis(int w, int h, float axis_x, float axis_y, LensModel &m) {
+  float ax = axis_x, ay = axis_y;
+  if (!(ax == ax) || !(ay == ay)) {
+    ax = w * 0.5f;
+    ay = h * 0.5f;
+  }
+  m.ax = ax;
+  m.ay = ay;
+  m.R = 0;
+  for (int c = 0; c < 4; c++) {
+    const double d = std::hypot((c & 1 ? w : 0) - static_cast<double>(ax), (c & 2 ? h : 0) - static_cast<double>(ay));
+    if (d > m.R) m.R = d;
+  }
+  if (!(m.R > 0)) m.R = 1;
+}
+
+void lens_at(const LensModel &m, double u, double v, float *dx, float *dy) {
+  double rx[LENS_P], ry[LENS_P], fx = 0, fy = 0;
+  lens_rows((u - m.ax) / m.R, (v - m.ay) / m.R, m.K, rx, ry);
+  for (int a = 0; a < 8 + m.K; a++) {
+    fx += m.t[a] * rx[a];
+    fy += m.t[a] * ry[a];
+  }
+  *dx = static_cast<float>(fx);
+  *dy = static_cast<float>(fy);
+}
+
+int border_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use_in,
+                LensModel *out) {
+  float pos[MAX_NODES];
+  const int nx = node_lattice(span, pos), ny = nx, nn = nx * ny;
+  LensModel lm;
+  lens_axis(w, h, axis_x, axis_y, lm);
+  const float ax = static_cast<float>(lm.ax), ay = static_cast<float>(lm.ay);
+  const double R = lm.R;
+  std::vector<float> qx(nn), qy(nn);
+  std::vector<int> use(nn);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      float u, v;
+      project_f(H, pos[i], pos[j], &u, &v);
+      const int k = j * nx + i;
+      qx[k] = static_cast<float>((u - ax) / R);
+      qy[k] = static_cast<float>((v - ay) / R);
+      const bool ex = i == 0 || i == nx - 1, ey = j == 0 || j == ny - 1;
+      const bool corner = ex && ey, mark = (ex || ey) && !corner;  // the marked nodes: the border's, not its corners
+      use[k] = corner || (mark && (!use_in || use_in[k]));
+    }
+  // the axis inside the lattice's quad (on one side of all four edges, corners in ring order): one radial term at most
+  const int cn[4] = {0, nx - 1, nn - 1, nn - nx};
+  int npos = 0, nneg = 0;
+  for (int c = 0; c < 4; c++) {
+    const int a = cn[c], b = cn[(c + 1) & 3];
+    const double cr = static_cast<double>(qx[a]) * qy[b] - static_cast<double>(qy[a]) * qx[b];
+    npos += cr > 0;
+    nneg += cr < 0;
+  }
+  const int kmax = (npos == 4 || nneg == 4) ? 1 : LENS_TERMS;
+  // K by AICc over the used nodes' 2n residuals
+  double rx[LENS_P], ry[LENS_P];
+  int K = -1;
+  double best = INFINITY;
+  for (int k = 0; k <= kmax; k++) {
+    const int p = 8 + k;
+    double t[LENS_P] = {}, rss = 0;
+    int n = 0;
+    if (!lens_fit(k, nn, use, qx, qy, nodes, t)) continue;
+    for (int m = 0; m < nn; m++) {
+      if (!use[m]) continue;
+      n += 2;
+      lens_rows(qx[m], qy[m], k, rx, ry);
+      double fx = 0, fy = 0;
+      for (int a = 0; a < p; a++) {
+        fx += t[a] * rx[a];
+        fy += t[a] * ry[a];
+      }
+      rss += (nodes[2 * m] - fx) * (nodes[2 * m] - fx) + (nodes[2 * m + 1] - fy) * (nodes[2 * m + 1] - fy);
+    }
+    if (n - p - 1 <= 0) continue;
+    const double aicc = n * std::log(rss / n + 1e-30) + 2.0 * p + 2.0 * p * (p + 1) / (n - p - 1);
+    if (aicc < best) {
+      best = aicc;
+      K = k;
+    }
+  }
+  double th[LENS_P] = {};
+  if (K >= 0 && !lens_fit(K, nn, use, qx, qy, nodes, th)) K = -1;
+  std::vector<float> mx(nn), my(nn);
+  for (int k = 0; k < nn; k++) {
+    double X = 0, Y = 0;
+    if (K >= 0) {
+      lens_rows(qx[k], qy[k], K, rx, ry);
+      for (int a = 0; a < 8 + K; a++) {
+        X += th[a] * rx[a];
+        Y += th[a] * ry[a];
+      }
+    }
+    mx[k] = static_cast<float>(X);
+    my[k] = static_cast<float>(Y);
+  }
+  // the Coons patch of the border's residuals from the model, carried to every interior node, in float
+  const auto ex = [&](int i, int j) { return nodes[2 * (j * nx + i)] - mx[j * nx + i]; };
+  const auto ey = [&](int i, int j) { return nodes[2 * (j * nx + i) + 1] - my[j * nx + i]; };
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const float u = (pos[i] - pos[0]) / (pos[nx - 1] - pos[0]), v = (pos[j] - pos[0]) / (pos[ny - 1] - pos[0]);
+      const float fx = (1 - u) * ex(0, j) + u * ex(nx - 1, j) + (1 - v) * ex(i, 0) + v * ex(i, ny - 1) -
+                       ((1 - u) * (1 - v) * ex(0, 0) + u * (1 - v) * ex(nx - 1, 0) + (1 - u) * v * ex(0, ny - 1) +
+                        u * v * ex(nx - 1, ny - 1));
+      const float fy = (1 - u) * ey(0, j) + u * ey(nx - 1, j) + (1 - v) * ey(i, 0) + v * ey(i, ny - 1) -
+                       ((1 - u) * (1 - v) * ey(0, 0) + u * (1 - v) * ey(nx - 1, 0) + (1 - u) * v * ey(0, ny - 1) +
+                        u * v * ey(nx - 1, ny - 1));
+      nodes[2 * (j * nx + i)] = mx[j * nx + i] + fx;
+      nodes[2 * (j * nx + i) + 1] = my[j * nx + i] + fy;
+    }
+  if (out) {
+    lm.K = K;
+    for (int a = 0; a < LENS_P; a++) lm.t[a] = K >= 0 && a < 8 + K ? th[a] : 0.0;
+    *out = lm;
+  }
+  return K;
+}
 
 void reg_model(int span, const float *H0, int w, int h, RegModel &m) {
   float pos[MAX_NODES];

@@ -32,6 +32,28 @@ void reg_nodes(const RegModel &m, const float *params, std::vector<float> &nodes
 // grad_k1 and grad_anchors (98) from the gradient with respect to the nodes (nx * nx * 2).
 void reg_chain(const RegModel &m, const double *node_grad, double *grad);
 
+// The lens model of the border (liblizard/src/acquire.c:1470-1528): a node's displacement from H as the first-order
+// change of the homography (x, y on both axes, the shift, the two perspective terms -(g . q) q) plus K (0..3) radial
+// terms k |q|^(2k) q, q = (the node's place under H - the axis) / R, R the farthest frame corner's distance from the
+// axis; the axis is the frame's centre unless given.
+constexpr int LENS_TERMS = 3, LENS_P = 8 + LENS_TERMS;
+struct LensModel {
+  int K = -1;            // radial terms; -1: no model (H alone)
+  double t[LENS_P] = {};  // coefficients: 8 affine and perspective, then K radial
+  double ax = 0, ay = 0, R = 1;
+};
+// The axis and R for a w x h frame (axis x, y; NaN: the centre).
+void lens_axis(int w, int h, float axis_x, float axis_y, LensModel &m);
+// The model's displacement at image point (u, v).
+void lens_at(const LensModel &m, double u, double v, float *dx, float *dy);
+// The interior of the node lattice from its border (acquire.c:1531-1605): the lens model fitted by least squares to
+// the border nodes in use (the corners and the marked nodes use[] allows; all where use is null), K by AICc (at most 1
+// where the axis lies inside the lattice's quad), then every interior node the model plus the Coons patch of what the
+// model leaves on the border. nodes: nx * nx (dx, dy), the border read, the interior written. Returns K (-1: none fitted,
+// H alone and the patch).
+int border_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use,
+                LensModel *out);
+
 // Device state of a refinement: the homography and the intensity model w = alpha r + beta.
 struct RegState {
   float H[9];

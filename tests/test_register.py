@@ -7,6 +7,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
+import lizard_amd
 import lizard_amd as L
 from lizard_amd import sim
 from _libs import ptr, ref
@@ -112,3 +113,97 @@ def test_oracle_gradient_matches_finite_differences(small_case):
         e = np.zeros(R.NPARAM); e[k] = steps[k]
         fd = (f(c["p"] + e) - f(c["p"] - e)) / (2 * steps[k])
         assert abs(fd - g[k]) <= 2e-3 * max(abs(g[k]), 1e-3 * np.abs(g).max()) + 1e-6, (k, fd, g[k])
+
+
+def _border_case(span, W, Hh, rot, scale, persp, cx, cy, seed, K_true, noise):
+    """A homography and a border field: the lens model's own terms (K_true radial ones) about the frame's centre plus
+    a smooth residual and noise, on every node (the fill reads the border only); every fifth marked node untrusted."""
+    from lizard_amd import sim
+    cam = sim.Camera(W, Hh, rot_deg=rot, scale=scale, persp=persp)
+    cam.cx, cam.cy = cx, cy
+    Hm = cam.H(span)
+    H = (Hm / Hm[2, 2]).astype(np.float32).ravel()
+    pos = lizard_amd.node_lattice(span)
+    nx = len(pos)
+    rng = np.random.default_rng(seed)
+    X, Y = np.meshgrid(pos, pos, indexing="xy")
+    w = H[6] * X + H[7] * Y + 1
+    u = (H[0] * X + H[1] * Y + H[2]) / w
+    v = (H[3] * X + H[4] * Y + H[5]) / w
+    R = np.hypot(W / 2, Hh / 2)
+    qx, qy = (u - W / 2) / R, (v - Hh / 2) / R
+    s = qx * qx + qy * qy
+    t = rng.normal(0, 0.5, 8)
+    dx = t[0] * qx + t[1] * qy + t[4] - t[6] * qx * qx - t[7] * qx * qy
+    dy = t[2] * qx + t[3] * qy + t[5] - t[6] * qx * qy - t[7] * qy * qy
+    sk = s.copy()
+    for k in range(K_true):
+        c = rng.normal(0, 3.0)
+        dx, dy = dx + c * sk * qx, dy + c * sk * qy
+        sk = sk * s
+    dx = dx + 0.3 * np.sin(X / 40.0) + rng.normal(0, noise, X.shape)
+    dy = dy + 0.3 * np.cos(Y / 35.0) + rng.normal(0, noise, X.shape)
+    nodes = np.stack([dx, dy], -1).astype(np.float32)
+    use = np.ones((nx, nx), np.uint8)
+    use.ravel()[::5] = 0
+    return H, nodes, use
+
+
+BORDERS = [(128, 1280, 720, 5.0, 0.8, 0.02, 0, 0, 1, 1, 0.02), (256, 1920, 1080, -8.0, 0.85, 0.03, 0, 0, 2, 2, 0.05),
+           (512, 1920, 1080, 3.0, 0.9, 0.01, 0, 0, 3, 3, 0.02), (256, 1920, 1080, 12.0, 0.6, 0.04, 500, 250, 4, 3, 0.01),
+           (512, 1920, 1080, -2.0, 0.7, 0.0, -600, 0, 5, 2, 0.1), (128, 1280, 720, 0.0, 0.5, 0.0, 300, 180, 6, 0, 0.0)]
+
+
+@pytest.mark.parametrize("span,W,Hh,rot,scale,persp,cx,cy,seed,K,noise", BORDERS)
+def test_border_fill_equals_reference(span, W, Hh, rot, scale, persp, cx, cy, seed, K, noise):
+    """lizx_reg_border_fill (the lens model of the border, K by AICc, and the Coons patch) gives the reference's
+    border_fill_axis nodes to the bit, about the frame's centre and about given axes inside and outside the quad;
+    the lens field of the model it returns is the reference's lensm_at at every node."""
+    r = ref("scalar")
+    if r is None:
+        pytest.skip("oracle/_ref not built")
+    H, nodes, use = _border_case(span, W, Hh, rot, scale, persp, cx, cy, seed, K, noise)
+    nx = nodes.shape[0]
+    version = {128: 5, 256: 20, 512: 64}[span]
+    info = lizard_amd.format_info(version)
+    f = r.ref_format_new(info["n"], info["subch"], span)
+    for axis in (None, (W * 0.3, Hh * 0.6), (W * 1.4, -Hh * 0.2)):
+        want = np.ascontiguousarray(nodes.copy())
+        ax, ay = (float("nan"), float("nan")) if axis is None else axis
+        r.ref_border_fill_axis(f, ptr(H), W, Hh, ax, ay, ptr(want), ptr(use))
+        got, Kgot, model = lizard_amd.border_fill(span, H, W, Hh, nodes, use=use, axis=axis)
+        assert np.array_equal(got, want), (axis, np.abs(got - want).max())
+        assert -1 <= Kgot <= 3
+        # the border itself is left as it was
+        border = np.ones((nx, nx), bool)
+        border[1:-1, 1:-1] = False
+        assert np.array_equal(got[border], nodes[border])
+        field = lizard_amd.lens_field(span, H, model)
+        pos = lizard_amd.node_lattice(span)
+        t = np.ascontiguousarray(model[4:], np.float64)
+        for j in (0, nx // 2, nx - 1):
+            for i in (0, nx // 3, nx - 1):
+                w = H[6] * pos[i] + H[7] * pos[j] + np.float32(1)
+                u = np.float32(H[0] * pos[i] + H[1] * pos[j] + H[2]) / w
+                v = np.float32(H[3] * pos[i] + H[4] * pos[j] + H[5]) / w
+                dx, dy = C.c_float(), C.c_float()
+                r.ref_lensm_at(int(model[0]), ptr(t), model[1], model[2], model[3], float(u), float(v), C.byref(dx),
+                               C.byref(dy))
+                if Kgot >= 0:
+                    assert abs(field[j, i, 0] - dx.value) <= 1e-5 * max(1.0, abs(dx.value))
+                    assert abs(field[j, i, 1] - dy.value) <= 1e-5 * max(1.0, abs(dy.value))
+    r.ref_format_free(f)
+
+
+def test_border_fill_reproduces_a_lens_field():
+    """Border nodes that are exactly a lens model's field (no residual): the interior is that field, to the float
+    rounding of the border's values carried through the least squares (the border spans only the outer radii, so the
+    fit extrapolates inward: a hundredth of a pixel), since the Coons patch carries nothing inward."""
+    H, nodes, use = _border_case(256, 1920, 1080, 4.0, 0.8, 0.02, 0, 0, 9, 2, 0.0)
+    model = np.zeros(15)
+    model[0], model[1], model[2], model[3] = 2, 960.0, 540.0, np.hypot(960.0, 540.0)
+    model[4:14] = [0.3, -0.2, 0.1, 0.4, 0.5, -0.7, 0.05, -0.03, 2.0, -1.0]
+    field = lizard_amd.lens_field(256, H, model)
+    got, K, fitted = lizard_amd.border_fill(256, H, 1920, 1080, field)
+    assert K >= 1
+    assert np.abs(got - field).max() < 0.02
