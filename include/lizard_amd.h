@@ -400,6 +400,16 @@ LIZ_API int lizx_ldpc_shape(lizx_ldpc *h, int per_cu);
    doubles) receives K, the axis x, y, R and the 11 coefficients (zero past 8 + K). Returns K (-1: no model fitted). */
 LIZ_API int lizx_reg_border_fill(int span, const float *H, int w, int h, const float *axis, float *nodes,
                                  const uint8_t *use, double *model);
+/* The node lattice from its border and the 7 x 7 anchors as the reference builds it (acquire.c:2219-2358
+   border_fill_anchors): the lens model refitted over the border nodes in use and the anchors found, robustly (border
+   nodes further from it than the larger of mpx / 2 and three robust spreads are dropped and take the model's
+   displacement, three rounds), the model plus the Coons patch of what it leaves on the border at every interior node,
+   then the cubic Hermite mesh (acquire.c:2196-2210) of what the anchors still see, each reading shrunk by the share of
+   it that is signal, the ends' knots 0. anchors: 49 x 5 floats, row after row: the centre's module X, Y, the
+   displacement from H dx, dy in pixels, 1 if found (else 0). mpx: pixels a module. Otherwise as lizx_reg_border_fill;
+   returns the model's K. */
+LIZ_API int lizx_reg_anchor_fill(int span, const float *H, int w, int h, const float *axis, float *nodes,
+                                 const uint8_t *use, const float *anchors, float mpx, double *model);
 /* A lens model's displacement (model as lizx_reg_border_fill gives it) at every node's place under H (lensm_at,
    acquire.c:1615-1620); returns nx. */
 LIZ_API int lizx_reg_lens_field(int span, const float *H, const double *model, float *nodes);

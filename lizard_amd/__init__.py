@@ -160,6 +160,7 @@ def lib():
     sig("lizx_ldpc_shape", _I, _P, _I)
     sig("lizx_reg_border_fill", _I, _I, _P, _I, _I, _P, _P, _P, _P)
     sig("lizx_reg_lens_field", _I, _I, _P, _P, _P)
+    sig("lizx_reg_anchor_fill", _I, _I, _P, _I, _I, _P, _P, _P, _P, _F, _P)
     sig("lizx_ldpc_decode_device_noalloc", _I, _P, _I, _P, _P, _P, _P, _P, _I, _I, _F, _I, _P, _P)
     sig("lizx_ldpc_encode", _I, _I, _I, _P, _P)
     sig("lizx_crc32c", C.c_uint32, _P, _SZ)
@@ -253,6 +254,19 @@ def border_fill(span, H, w, h, nodes, use=None, axis=None):
     u = None if use is None else np.ascontiguousarray(use, np.uint8)
     ax = None if axis is None else np.ascontiguousarray(axis, np.float32)
     K = check(lib().lizx_reg_border_fill(span, _p(H), w, h, _p(ax), _p(nodes), _p(u), _p(model)))
+    return nodes, K, model
+
+
+def anchor_fill(span, H, w, h, nodes, anchors, mpx, use=None, axis=None):
+    """The node lattice from its border and the anchors (lizx_reg_anchor_fill): (nodes, K, model[15]); anchors
+    49 x 5 (X, Y, dx, dy, found)."""
+    nodes = np.array(nodes, np.float32, copy=True)
+    model = np.zeros(15, np.float64)
+    H = np.ascontiguousarray(H, np.float32)
+    u = None if use is None else np.ascontiguousarray(use, np.uint8)
+    ax = None if axis is None else np.ascontiguousarray(axis, np.float32)
+    an = np.ascontiguousarray(anchors, np.float32)
+    K = check(lib().lizx_reg_anchor_fill(span, _p(H), w, h, _p(ax), _p(nodes), _p(u), _p(an), mpx, _p(model)))
     return nodes, K, model
 
 

@@ -1741,6 +1741,30 @@ LIZ_API int lizx_reg_border_fill(int span, const float *H, int w, int h, const f
   });
 }
 
+LIZ_API int lizx_reg_anchor_fill(int span, const float *H, int w, int h, const float *axis, float *nodes,
+                                 const uint8_t *use, const float *anchors, float mpx, double *model) {
+  return guard([&] {
+    need(H && nodes && anchors && w > 0 && h > 0, "a homography, nodes, anchors and a frame size");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    need(mpx > 0, "a positive module size in pixels");
+    AnchorReading A[REG_ANCH * REG_ANCH];
+    for (int k = 0; k < REG_ANCH * REG_ANCH; k++)
+      A[k] = AnchorReading{anchors[5 * k], anchors[5 * k + 1], anchors[5 * k + 2], anchors[5 * k + 3],
+                           anchors[5 * k + 4] != 0};
+    LensModel m;
+    const float nan = __builtin_nanf("");
+    const int K = anchor_fill(span, H, w, h, axis ? axis[0] : nan, axis ? axis[1] : nan, nodes, use, A, mpx, &m);
+    if (model) {
+      model[0] = m.K;
+      model[1] = m.ax;
+      model[2] = m.ay;
+      model[3] = m.R;
+      for (int a = 0; a < LENS_P; a++) model[4 + a] = m.t[a];
+    }
+    return K;
+  });
+}
+
 LIZ_API int lizx_reg_lens_field(int span, const float *H, const double *model, float *nodes) {
   return guard([&] {
     need(H && model && nodes, "a homography, a lens model and a place for the nodes");

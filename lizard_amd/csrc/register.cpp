@@ -2,7 +2,10 @@
 // double, computed once per (ring, frame size, reference homography).
 #include "register.hpp"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
+#include <cstring>
 #include <utility>
 
 namespace lz {
@@ -231,6 +234,303 @@ int border_fill(int span, const float *H, int w, int h, float axis_x, float axis
     *out = lm;
   }
   return K;
+}
+
+namespace {
+
+// The mesh cell a module coordinate falls in along one axis: the last k of 0 .. nn - 2 with k = 0 or m >= node[k].
+int node_cell(const float *node, int nn, float m) {
+  int lo = 0, hi = nn - 2;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (m >= node[mid]) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// A module point through H and the node mesh (acquire.c:5095-5103), float, the reference's order.
+void map_point(const float *pos, int nx, const float *H, const float *nodes, float mx, float my, float *pu, float *pv) {
+  const int j = node_cell(pos, nx, my), i = node_cell(pos, nx, mx);
+  const float v = (my - pos[j]) / (pos[j + 1] - pos[j]), u = (mx - pos[i]) / (pos[i + 1] - pos[i]);
+  const float *n00 = nodes + 2 * (j * nx + i), *n10 = n00 + 2, *n01 = n00 + 2 * nx, *n11 = n01 + 2;
+  const float dx = (n00[0] * (1 - u) + n10[0] * u) * (1 - v) + (n01[0] * (1 - u) + n11[0] * u) * v;
+  const float dy = (n00[1] * (1 - u) + n10[1] * u) * (1 - v) + (n01[1] * (1 - u) + n11[1] * u) * v;
+  project_f(H, mx, my, pu, pv);
+  *pu += dx;
+  *pv += dy;
+}
+
+// The lens model over points, K = 0..3 by AICc; robust where isb is given (border points further than the larger of
+// tau_floor and three robust spreads of the border's residuals dropped and the model refitted, three rounds).
+bool lensm_fit(int np, const std::vector<double> &qx, const std::vector<double> &qy, const std::vector<double> &dx,
+               const std::vector<double> &dy, const std::vector<uint8_t> *isb, std::vector<uint8_t> &drop,
+               double tau_floor, LensModel &m) {
+  std::vector<double> res(isb ? np : 0);
+  bool ok = false;
+  for (int round = 0; round < 4; round++) {
+    double best = INFINITY, tb[LENS_P] = {};
+    int K = -1;
+    // the normal equations at every term at once (a model of fewer terms takes their leading block), the upper
+    // triangle summed and copied down
+    double A3[LENS_P][LENS_P + 1] = {}, rx[LENS_P], ry[LENS_P];
+    int n = 0;
+    for (int i = 0; i < np; i++) {
+      if (drop[i]) continue;
+      n++;
+      lens_rows(qx[i], qy[i], LENS_TERMS, rx, ry);
+      for (int a = 0; a < LENS_P; a++) {
+        for (int b = a; b < LENS_P; b++) A3[a][b] += rx[a] * rx[b] + ry[a] * ry[b];
+        A3[a][LENS_P] += rx[a] * dx[i] + ry[a] * dy[i];
+      }
+    }
+    for (int a = 1; a < LENS_P; a++)
+      for (int b = 0; b < a; b++) A3[a][b] = A3[b][a];
+    for (int k = 0; k <= LENS_TERMS; k++) {
+      const int p = 8 + k;
+      double A[LENS_P][LENS_P + 1];
+      for (int a = 0; a < LENS_P; a++)
+        for (int b = 0; b <= LENS_P; b++) A[a][b] = A3[a][b];
+      for (int a = 0; a < p; a++) A[a][a] += 1e-12;
+      if (2 * n - p - 1 <= 0 || !solve_lens(A, p)) continue;
+      double t[LENS_P] = {}, rss = 0;
+      for (int a = 0; a < p; a++) t[a] = A[a][LENS_P] / A[a][a];
+      for (int i = 0; i < np; i++) {
+        if (drop[i]) continue;
+        lens_rows(qx[i], qy[i], k, rx, ry);
+        double fx = 0, fy = 0;
+        for (int a = 0; a < p; a++) {
+          fx += t[a] * rx[a];
+          fy += t[a] * ry[a];
+        }
+        rss += (dx[i] - fx) * (dx[i] - fx) + (dy[i] - fy) * (dy[i] - fy);
+      }
+      const double nn = 2.0 * n, aicc = nn * std::log(rss / nn + 1e-30) + 2.0 * p + 2.0 * p * (p + 1) / (nn - p - 1);
+      if (aicc < best) {
+        best = aicc;
+        K = k;
+        for (int a = 0; a < LENS_P; a++) tb[a] = t[a];
+      }
+    }
+    if (K < 0) break;
+    m.K = K;
+    for (int a = 0; a < LENS_P; a++) m.t[a] = tb[a];
+    ok = true;
+    if (!isb || round == 3) break;
+    int nb = 0, changed = 0;
+    for (int i = 0; i < np; i++) {
+      if (!(*isb)[i]) continue;
+      float fx, fy;
+      lens_at(m, qx[i] * m.R + m.ax, qy[i] * m.R + m.ay, &fx, &fy);
+      res[nb++] = std::hypot(dx[i] - fx, dy[i] - fy);
+    }
+    if (!nb) break;
+    std::vector<double> srt(res.begin(), res.begin() + nb);
+    std::sort(srt.begin(), srt.end());
+    const double tau = std::fmax(tau_floor, 3 * 1.4826 * srt[nb / 2]);
+    for (int i = 0, b = 0; i < np; i++) {
+      if (!(*isb)[i]) continue;
+      const uint8_t bad = res[b++] > tau;
+      if (bad != drop[i]) {
+        drop[i] = bad;
+        changed++;
+      }
+    }
+    if (!changed) break;
+  }
+  return ok;
+}
+
+}  // namespace
+
+int anchor_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use_in,
+                const AnchorReading *A, float mpx, LensModel *out) {
+  constexpr int NA = REG_ANCH * REG_ANCH;
+  float pos[MAX_NODES];
+  const int nx = node_lattice(span, pos), ny = nx, nn = nx * ny;
+  std::vector<double> qx, qy, dx, dy;
+  std::vector<uint8_t> isb, drop;
+  std::vector<int> at;
+  LensModel lm;
+  lm.K = 0;  // the zero model (H alone) until a fit replaces it
+  lens_axis(w, h, axis_x, axis_y, lm);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      if (i && j && i < nx - 1 && j < ny - 1) continue;
+      const int k = j * nx + i;
+      const bool corner = (i == 0 || i == nx - 1) && (j == 0 || j == ny - 1);
+      if (!(corner || !use_in || use_in[k])) continue;
+      float u, v;
+      project_f(H, pos[i], pos[j], &u, &v);
+      qx.push_back((u - lm.ax) / lm.R);
+      qy.push_back((v - lm.ay) / lm.R);
+      dx.push_back(nodes[2 * k]);
+      dy.push_back(nodes[2 * k + 1]);
+      isb.push_back(1);
+      at.push_back(k);
+    }
+  for (int k = 0; k < NA; k++) {
+    if (!A[k].ok) continue;
+    float u, v;
+    project_f(H, A[k].X, A[k].Y, &u, &v);
+    qx.push_back((u - lm.ax) / lm.R);
+    qy.push_back((v - lm.ay) / lm.R);
+    dx.push_back(A[k].dx);
+    dy.push_back(A[k].dy);
+    isb.push_back(0);
+    at.push_back(-1 - k);
+  }
+  const int np = static_cast<int>(qx.size());
+  drop.assign(np, 0);
+  lensm_fit(np, qx, qy, dx, dy, &isb, drop, 0.5 * mpx, lm);
+  std::vector<float> mx(nn), my(nn);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      float u, v;
+      project_f(H, pos[i], pos[j], &u, &v);
+      lens_at(lm, u, v, &mx[j * nx + i], &my[j * nx + i]);
+    }
+  // a border node the anchors outvote reads the model's displacement, so the patch leaves nothing there
+  for (int p = 0; p < np; p++)
+    if (isb[p] && drop[p]) {
+      nodes[2 * at[p]] = mx[at[p]];
+      nodes[2 * at[p] + 1] = my[at[p]];
+    }
+  const auto ex = [&](int i, int j) { return nodes[2 * (j * nx + i)] - mx[j * nx + i]; };
+  const auto ey = [&](int i, int j) { return nodes[2 * (j * nx + i) + 1] - my[j * nx + i]; };
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      const float u = (pos[i] - pos[0]) / (pos[nx - 1] - pos[0]), v = (pos[j] - pos[0]) / (pos[ny - 1] - pos[0]);
+      const float fx = (1 - u) * ex(0, j) + u * ex(nx - 1, j) + (1 - v) * ex(i, 0) + v * ex(i, ny - 1) -
+                       ((1 - u) * (1 - v) * ex(0, 0) + u * (1 - v) * ex(nx - 1, 0) + (1 - u) * v * ex(0, ny - 1) +
+                        u * v * ex(nx - 1, ny - 1));
+      const float fy = (1 - u) * ey(0, j) + u * ey(nx - 1, j) + (1 - v) * ey(i, 0) + v * ey(i, ny - 1) -
+                       ((1 - u) * (1 - v) * ey(0, 0) + u * (1 - v) * ey(nx - 1, 0) + (1 - u) * v * ey(0, ny - 1) +
+                        u * v * ey(nx - 1, ny - 1));
+      nodes[2 * (j * nx + i)] = mx[j * nx + i] + fx;
+      nodes[2 * (j * nx + i) + 1] = my[j * nx + i] + fy;
+    }
+  // what the anchors still see through that mesh, as the sampler blends it
+  const double X0 = pos[0], SX = pos[nx - 1] - X0;
+  std::vector<std::array<double, 2>> tv;
+  std::vector<int> gi;
+  for (int p = 0; p < np; p++) {
+    if (isb[p]) continue;
+    const AnchorReading &an = A[-1 - at[p]];
+    float u, v, hu, hv;
+    map_point(pos, nx, H, nodes, an.X, an.Y, &u, &v);
+    project_f(H, an.X, an.Y, &hu, &hv);
+    tv.push_back({static_cast<double>(an.dx - (u - hu)), static_cast<double>(an.dy - (v - hv))});
+    gi.push_back(-1 - at[p]);
+  }
+  const int m = static_cast<int>(tv.size());
+  // each reading shrunk by the share of it that is signal: the noise from each reading against the mean of its found
+  // grid neighbours (robust: 1.4826 MAD over both components), the readings' power per component, max(0, 1 - s2 / P)
+  {
+    double g[NA][2] = {}, dd[2 * NA], pw = 0;
+    uint8_t has[NA] = {};
+    int na2 = 0, nd = 0;
+    for (int q = 0; q < m; q++)
+      if (gi[q] >= 0) {
+        has[gi[q]] = 1;
+        g[gi[q]][0] = tv[q][0];
+        g[gi[q]][1] = tv[q][1];
+        pw += tv[q][0] * tv[q][0] + tv[q][1] * tv[q][1];
+        na2++;
+      }
+    for (int k = 0; k < NA; k++) {
+      if (!has[k]) continue;
+      const int i = k % REG_ANCH, j = k / REG_ANCH;
+      const int nb[4] = {i > 0 ? k - 1 : -1, i < REG_ANCH - 1 ? k + 1 : -1, j > 0 ? k - REG_ANCH : -1,
+                         j < REG_ANCH - 1 ? k + REG_ANCH : -1};
+      double sx = 0, sy = 0;
+      int c = 0;
+      for (int e = 0; e < 4; e++)
+        if (nb[e] >= 0 && has[nb[e]]) {
+          sx += g[nb[e]][0];
+          sy += g[nb[e]][1];
+          c++;
+        }
+      if (c < 2) continue;
+      const double f = 1 / std::sqrt(1 + 1.0 / c);
+      dd[nd++] = std::fabs(g[k][0] - sx / c) * f;
+      dd[nd++] = std::fabs(g[k][1] - sy / c) * f;
+    }
+    if (nd >= 16 && na2 > 0) {
+      std::sort(dd, dd + nd);
+      const double sg = 1.4826 * dd[nd / 2], s2 = sg * sg, P = pw / (2.0 * na2),
+                   kf = P > 0 ? std::fmax(0, 1 - s2 / P) : 0;
+      for (int q = 0; q < m; q++)
+        if (gi[q] >= 0) {
+          tv[q][0] *= kf;
+          tv[q][1] *= kf;
+        }
+    }
+  }
+  // the knots on both axes: the lattice's ends (0) and the anchors' centres; an inner knot its anchor's shrunk
+  // reading, a missing one the mean of its found 4-neighbours (rounds until none is left, at most 6; none near: 0)
+  constexpr int KN = REG_ANCH + 2;
+  double kx[KN], kv[KN][KN][2] = {};
+  uint8_t kn[KN][KN] = {};
+  const double ha = 0.5 * (span / 64);
+  kx[0] = 0;
+  kx[KN - 1] = 1;
+  for (int i = 0; i < REG_ANCH; i++) kx[i + 1] = (anchor_m0(span, i) + ha - X0) / SX;
+  for (int q = 0; q < m; q++)
+    if (gi[q] >= 0) {
+      const int i = gi[q] % REG_ANCH + 1, j = gi[q] / REG_ANCH + 1;
+      kv[j][i][0] = tv[q][0];
+      kv[j][i][1] = tv[q][1];
+      kn[j][i] = 1;
+    }
+  for (int round = 0; round < 6; round++) {
+    uint8_t now[KN][KN];
+    std::memcpy(now, kn, sizeof now);
+    int left = 0;
+    for (int j = 1; j < KN - 1; j++)
+      for (int i = 1; i < KN - 1; i++) {
+        if (kn[j][i]) continue;
+        const int nb[4][2] = {{i - 1, j}, {i + 1, j}, {i, j - 1}, {i, j + 1}};
+        double s0 = 0, s1 = 0;
+        int c = 0;
+        for (int e = 0; e < 4; e++) {
+          const int a = nb[e][0], b = nb[e][1];
+          if (a < 1 || a > KN - 2 || b < 1 || b > KN - 2 || !kn[b][a]) continue;
+          s0 += kv[b][a][0];
+          s1 += kv[b][a][1];
+          c++;
+        }
+        if (c) {
+          kv[j][i][0] = s0 / c;
+          kv[j][i][1] = s1 / c;
+          now[j][i] = 1;
+        } else {
+          left++;
+        }
+      }
+    std::memcpy(kn, now, sizeof kn);
+    if (!left) break;
+  }
+  // each axis a cubic Hermite through the knots: a node is a 4 x 4 blend of knot values
+  for (int j = 1; j < ny - 1; j++)
+    for (int i = 1; i < nx - 1; i++) {
+      int bx, by;
+      double wx[4], wy[4];
+      herm4(kx, KN, (pos[i] - X0) / SX, &bx, wx);
+      herm4(kx, KN, (pos[j] - X0) / SX, &by, wy);
+      double d0 = 0, d1 = 0;
+      for (int b = 0; b < 4; b++)
+        for (int a = 0; a < 4; a++) {
+          const int qxi = bx + a, qyi = by + b;
+          if (qxi < 0 || qxi >= KN || qyi < 0 || qyi >= KN || (wx[a] == 0 && wy[b] == 0)) continue;
+          d0 += wy[b] * wx[a] * kv[qyi][qxi][0];
+          d1 += wy[b] * wx[a] * kv[qyi][qxi][1];
+        }
+      nodes[2 * (j * nx + i)] += static_cast<float>(d0);
+      nodes[2 * (j * nx + i) + 1] += static_cast<float>(d1);
+    }
+  if (out) *out = lm;
+  return lm.K;
 }
 
 void reg_model(int span, const float *H0, int w, int h, RegModel &m) {

@@ -53,6 +53,19 @@ void lens_at(const LensModel &m, double u, double v, float *dx, float *dy);
 // H alone and the patch).
 int border_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use,
                 LensModel *out);
+// An anchor's reading: its centre (X, Y) in modules, its displacement from H (dx, dy) in pixels, found or not.
+struct AnchorReading {
+  float X, Y, dx, dy;
+  int ok;
+};
+// The interior from the border and the anchors (acquire.c:2219-2358): the lens model refitted over the border nodes in
+// use and the anchors found, robustly (border nodes further from it than the larger of tau_floor = mpx / 2 and three
+// robust spreads are dropped and take the model's displacement; three rounds), the model at every node plus the Coons
+// patch of what it leaves on the border, then a cubic Hermite mesh of what the anchors still see through that (each
+// reading shrunk by the share of it that is signal; knots 0 at the lattice's ends, a missing anchor's knot the mean
+// of its found neighbours). Returns the model's K.
+int anchor_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use,
+                const AnchorReading *anchors, float mpx, LensModel *out);
 
 // Device state of a refinement: the homography and the intensity model w = alpha r + beta.
 struct RegState {

@@ -59,3 +59,38 @@ void ref_lensm_at(int K, const double *t, double ax, double ay, double R, double
   m.R = R;
   lensm_at(&m, u, v, dx, dy);
 }
+
+/* border_fill_anchors (acquire.c:2219-2358): as ref_border_fill_axis, with the anchors' readings (49 x 5 floats: X, Y,
+   dx, dy, found) and pixels a module mpx. */
+void ref_border_fill_anchors(const void *p, const float *H, int iw, int ih, float ax, float ay, float *nodes,
+                             const unsigned char *use, const float *anchors, float mpx) {
+  const lizard_t *f = p;
+  const ob_layout_t *L = &f->frame;
+  const int nn = L->nx * L->ny;
+  node_t *nd = calloc((size_t)nn, sizeof(node_t));
+  for (int k = 0; k < nn; k++) {
+    nd[k].dx = nodes[2 * k];
+    nd[k].dy = nodes[2 * k + 1];
+    nd[k].done = 1;
+    nd[k].score = use && !use[k] ? 0.0f : 1.0f;
+  }
+  anchor_t A[OB_ANCH_K * OB_ANCH_K];
+  memset(A, 0, sizeof A);
+  for (int k = 0; k < OB_ANCH_K * OB_ANCH_K; k++) {
+    A[k].X = anchors[5 * k];
+    A[k].Y = anchors[5 * k + 1];
+    A[k].dx = anchors[5 * k + 2];
+    A[k].dy = anchors[5 * k + 3];
+    A[k].ok = anchors[5 * k + 4] != 0;
+  }
+  homo_t Hh;
+  memcpy(Hh.h, H, sizeof Hh.h);
+  lizard_set_axis(ax, ay);
+  border_fill_anchors(L, &Hh, iw, ih, nd, A, mpx);
+  lizard_set_axis(NAN, NAN);
+  for (int k = 0; k < nn; k++) {
+    nodes[2 * k] = nd[k].dx;
+    nodes[2 * k + 1] = nd[k].dy;
+  }
+  free(nd);
+}

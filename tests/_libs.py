@@ -113,6 +113,7 @@ def ref(form="scalar"):
     _sig(lib, "ref_decode_reg", I32, P, P, I32, I32, P, P, P, P, P, P, P, P)
     _sig(lib, "ref_border_fill_axis", None, P, P, I32, I32, F32, F32, P, P)
     _sig(lib, "ref_lensm_at", None, I32, P, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, P, P)
+    _sig(lib, "ref_border_fill_anchors", None, P, P, I32, I32, F32, F32, P, P, P, F32)
     _refs[form] = lib
     return lib
 

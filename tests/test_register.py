@@ -207,3 +207,45 @@ def test_border_fill_reproduces_a_lens_field():
     got, K, fitted = lizard_amd.border_fill(256, H, 1920, 1080, field)
     assert K >= 1
     assert np.abs(got - field).max() < 0.02
+
+
+def _anchors(span, H, rng, field_fn, found_frac=0.85, noise=0.1):
+    """The 7 x 7 anchors' readings: centres in modules, displacement from H = field at the anchor's place plus noise,
+    found with probability found_frac."""
+    ha = 0.5 * (span // 64)
+    out = np.zeros((49, 5), np.float32)
+    for k in range(49):
+        i, j = k % 7, k // 7
+        X = R.anchor_m0(span, i) + ha
+        Y = R.anchor_m0(span, j) + ha
+        fx, fy = field_fn(X, Y)
+        out[k] = (X, Y, fx + rng.normal(0, noise), fy + rng.normal(0, noise), float(rng.random() < found_frac))
+    return out
+
+
+@pytest.mark.parametrize("span,W,Hh,rot,scale,persp,cx,cy,seed,K,noise", BORDERS[:5])
+def test_anchor_fill_equals_reference(span, W, Hh, rot, scale, persp, cx, cy, seed, K, noise):
+    """lizx_reg_anchor_fill (the robust lens refit over border and anchors, the Coons patch, the shrunk anchor
+    residuals' Hermite mesh) gives the reference's border_fill_anchors nodes to the bit; an outlying border node is
+    dropped there and here alike; with no anchor found the nodes are the border fill's but for the model's refit."""
+    r = ref("scalar")
+    if r is None:
+        pytest.skip("oracle/_ref not built")
+    H, nodes, use = _border_case(span, W, Hh, rot, scale, persp, cx, cy, seed, K, noise)
+    nx = nodes.shape[0]
+    nodes[0, nx // 2] += 6.0  # an outlier on the top side
+    rng = np.random.default_rng(seed + 100)
+    field = lambda X, Y: (0.8 * np.sin(X / 60.0), -0.6 * np.cos(Y / 50.0))  # noqa: E731
+    version = {128: 5, 256: 20, 512: 64}[span]
+    info = L.format_info(version)
+    f = r.ref_format_new(info["n"], info["subch"], span)
+    mpx = float(np.hypot(H[0], H[3]))
+    for found, axis in ((0.85, None), (1.0, (W * 0.3, Hh * 0.6)), (0.3, (W * 1.4, -Hh * 0.2)), (0.0, None)):
+        an = _anchors(span, H, rng, field, found)
+        want = np.ascontiguousarray(nodes.copy())
+        ax, ay = (float("nan"), float("nan")) if axis is None else axis
+        r.ref_border_fill_anchors(f, ptr(H), W, Hh, ax, ay, ptr(want), ptr(use), ptr(an), mpx)
+        got, Kgot, model = L.anchor_fill(span, H, W, Hh, nodes, an, mpx, use=use, axis=axis)
+        assert np.array_equal(got, want), (found, axis, np.abs(got - want).max())
+        assert 0 <= Kgot <= 3
+    r.ref_format_free(f)
