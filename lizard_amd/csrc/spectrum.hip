@@ -23,12 +23,17 @@
 namespace lz {
 
 // ------------------------------------------------------------------------------------------------ FFT in LDS
+// The transforms' products and sums may fuse (fused multiply-adds): the FFT is held to FP64 within a tolerance, and
+// every path uses the same transform, so this changes no comparison between the build's own paths. Everywhere else
+// (the warp, the detrend, the turn, the soft scale) every operation rounds on its own, in the reference's order.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+#pragma clang fp contract(fast)
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 template <int R>
 __device__ __forceinline__ void dft(float2 *v) {
+#pragma clang fp contract(fast)
   if constexpr (R == 2) {
     const float2 a = v[0], b = v[1];
     v[0] = make_float2(a.x + b.x, a.y + b.y);
@@ -103,6 +108,7 @@ constexpr double angle_of(int e, int m) {
 
 template <int M, int E>
 __device__ __forceinline__ float2 twiddle(float2 x) {
+#pragma clang fp contract(fast)
   constexpr int r = ((E % M) + M) % M;
   if constexpr (r == 0) {
     return x;
@@ -337,11 +343,11 @@ __device__ __forceinline__ WarpRow warp_row(const PlanDev &P, const WarpFrame &F
 // homography H plus the node lattice's bilinear displacement (acquire.c:5095-5103), read from the capture. The float
 // operations are the reference's scalar grid walk's, in its order (acquire.c:5121-5196 without OB_SIMD), so the
 // samples are that walk's to the bit.
-__device__ __forceinline__ float warp_at(const PlanDev &P, const WarpFrame &F, const WarpRow &r, int x) {
-  const float mx = P.mcoord[x];
-  const int i = P.mcell[x], nx = P.nx, j = r.j;
+// The part of a warp sample that depends on its row and its node cell i alone: the node displacement's slope and
+// intercept in mx on either axis, (ex, cx, ey, cy), in the walk's float order.
+__device__ __forceinline__ float4 cell_terms(const PlanDev &P, const WarpFrame &F, const WarpRow &r, int i) {
+  const int nx = P.nx, j = r.j;
   const float v = r.v, omv = r.omv;
-  const float *H = F.H;
   const float2 n00 = F.nodes[j * nx + i], n10 = F.nodes[j * nx + i + 1];
   const float2 n01 = F.nodes[(j + 1) * nx + i], n11 = F.nodes[(j + 1) * nx + i + 1];
   const float inv = P.node_inv[i];
@@ -349,10 +355,22 @@ __device__ __forceinline__ float warp_at(const PlanDev &P, const WarpFrame &F, c
   const float cx = __fsub_rn(__fadd_rn(__fmul_rn(n00.x, omv), __fmul_rn(n01.x, v)), __fmul_rn(ex, P.node_x[i]));
   const float ey = __fmul_rn(__fadd_rn(__fmul_rn(__fsub_rn(n10.y, n00.y), omv), __fmul_rn(__fsub_rn(n11.y, n01.y), v)), inv);
   const float cy = __fsub_rn(__fadd_rn(__fmul_rn(n00.y, omv), __fmul_rn(n01.y, v)), __fmul_rn(ey, P.node_x[i]));
+  return make_float4(ex, cx, ey, cy);
+}
+
+// The capture's point for sample x of a row, its cell terms given.
+__device__ __forceinline__ float2 warp_point(const PlanDev &P, const WarpFrame &F, const WarpRow &r, int x, float4 ct) {
+  const float mx = P.mcoord[x];
+  const float *H = F.H;
   const float rw = __fdiv_rn(1.0f, __fadd_rn(__fmul_rn(H[6], mx), r.kw));
   const float pu = __fmul_rn(__fadd_rn(__fmul_rn(H[0], mx), r.ku), rw), pv = __fmul_rn(__fadd_rn(__fmul_rn(H[3], mx), r.kv), rw);
-  const float dx = __fadd_rn(cx, __fmul_rn(ex, mx)), dy = __fadd_rn(cy, __fmul_rn(ey, mx));
-  return sample(F, __fadd_rn(pu, dx), __fadd_rn(pv, dy));
+  const float dx = __fadd_rn(ct.y, __fmul_rn(ct.x, mx)), dy = __fadd_rn(ct.w, __fmul_rn(ct.z, mx));
+  return make_float2(__fadd_rn(pu, dx), __fadd_rn(pv, dy));
+}
+
+__device__ __forceinline__ float warp_at(const PlanDev &P, const WarpFrame &F, const WarpRow &r, int x) {
+  const float2 q = warp_point(P, F, r, x, cell_terms(P, F, r, P.mcell[x]));
+  return sample(F, q.x, q.y);
 }
 
 __device__ __forceinline__ float warp_sample(const PlanDev &P, const WarpFrame &F, int x, int y) {
@@ -478,19 +496,39 @@ __global__ __launch_bounds__(64 * W) void k_rows(PlanDev P, Src src, const doubl
   constexpr int M = F::M;
   extern __shared__ float2 zs[];  // W waves x F::LDS
   __shared__ double bsh[6];
+  constexpr bool WARP = std::is_same<Src, WarpSource>::value;
+  // per wave, its two rows' node-cell terms (a cell's share of every sample in it)
+  __shared__ float4 cterm[WARP ? W : 1][2][WARP ? MAX_NODES : 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y;
   const int y = xcd_block(blockIdx.x, gridDim.x) * (2 * W) + 2 * w;
   if (tid < 6) bsh[tid] = fitpart[static_cast<size_t>(f) * FIT_STRIDE + FIT_BETA + tid];
   __syncthreads();
   const bool yin0 = in_square(P, y), yin1 = in_square(P, y + 1);
   const auto row0 = src.row(P, f, y), row1 = src.row(P, f, y + 1);
+  if constexpr (WARP) {
+    for (int i = lane; i < P.nx - 1; i += 64) {
+      cterm[w][0][i] = cell_terms(P, row0.F, row0.r, i);
+      cterm[w][1][i] = cell_terms(P, row1.F, row1.r, i);
+    }
+    wave_sync();
+  }
   float2 v[M];
   // each lane's share of the row sums in float (as the reference's float chains), the lanes' in double
   float s0f = 0, x0f = 0, q0f = 0, s1f = 0, x1f = 0, q1f = 0;
 #pragma unroll
   for (int m = 0; m < M; m++) {
     const int x = lane + 64 * m;
-    float a = src.at(P, row0, x), b = src.at(P, row1, x);
+    float a, b;
+    if constexpr (WARP) {
+      const int ci = P.mcell[x];
+      const float2 q0 = warp_point(P, row0.F, row0.r, x, cterm[w][0][ci]);
+      const float2 q1 = warp_point(P, row1.F, row1.r, x, cterm[w][1][ci]);
+      a = sample(row0.F, q0.x, q0.y);
+      b = sample(row1.F, q1.x, q1.y);
+    } else {
+      a = src.at(P, row0, x);
+      b = src.at(P, row1, x);
+    }
     if ((yin0 || yin1) && in_square(P, x)) {
       if (yin0) a = hole_value(P, bsh, x, y);
       if (yin1) b = hole_value(P, bsh, x, y + 1);
