@@ -145,14 +145,16 @@ LIZ_API int lizx_format_blocks(int version, int *rate, int *slot0, int *nslots);
 LIZ_API int lizx_format_coefs(int version, int16_t *u, int16_t *v);
 /* A camera for synthetic captures: w x h pixels; H maps module coordinates (0 .. span + 30 on a side) to capture
    pixels; a radial lens k1 |q|^2 q D about the frame's centre, q the offset over the farthest corner's distance D
-   (the reference's lens form, liblizard/src/acquire.c:1470-1512); Gaussian blur sigma (pixels, edges repeated);
-   AWGN sigma (grey levels) from a counter-based generator seeded by seed. */
+   (the reference's lens form, liblizard/src/acquire.c:1470-1512) and a second radial term k2 |q|^4 q D (the
+   reference's second lens term, which a single k1 does not hold); Gaussian blur sigma (pixels, edges repeated); AWGN
+   sigma (grey levels) from a counter-based generator seeded by seed. */
 typedef struct {
   int w, h;
   double H[9];
   double k1;
   float blur, noise;
   uint64_t seed;
+  double k2;
 } lizx_camera;
 /* The channel on the GPU: picture (n x n floats in [0, 1], device, LIZARD-16V shown in the ring of span modules;
    grey 128 outside it) -> capture (h x w uint8, device). No counterpart in the reference's decoder (a measurement
@@ -257,6 +259,30 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
    default), or launches them one by one (0). Results are the same bits either way. No counterpart in the reference
    (liblizard's decoder runs on the CPU, api/codec.cpp:121-160). */
 LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on);
+/* The finder's registration from the anchors outward (default on; off: H and the nodes of the finder's radial lens
+   alone). On: after the border's fit the 7 x 7 anchors are searched and registered (liblizard/src/acquire.c:2128-2191
+   anchor_register, as lizx_anchor_register) and the lattice's interior is made from the border nodes and the anchors
+   (acquire.c:2219-2358, as lizx_reg_anchor_fill), where 12 or more anchors are found. Returns 0. */
+LIZ_API int lizx_finder_anchors(liz_decoder *d, int on);
+/* The anchors' search on the GPU (liblizard/src/acquire.c:1876-2036 anchor_find, its scalar form, to the bit): every
+   anchor of a ring of span modules searched on a capture (host, w x h grey, rows stride bytes apart, read at gamma 1)
+   through H (module -> pixel, h8 = 1) and optionally a lens model about it (15 doubles as lizx_reg_border_fill gives
+   it; null: H alone). anchors: 49 x 8 floats, row after row: the centre X, Y (modules), dx, dy (where it was found
+   less where H puts it, pixels), the correlation's peak, where it was found u, v (pixels), 1 if found (else 0).
+   Returns the anchors found. */
+LIZ_API int lizx_anchor_find(liz_decoder *d, int span, const uint8_t *img, int w, int h, int stride, const float *H,
+                             const double *model, float *anchors);
+/* The registration from the anchors outward (liblizard/src/acquire.c:2128-2191 anchor_register): the anchors
+   searched through H, then, round after round (at most 6, while more are found), those still missing through the
+   homography and the lens model the found ones say (each round's readings held to the robust rule); none where fewer
+   than 12 are found. A corner of quad (the lattice's corners in pixels, 8 floats, top left and clockwise; in and out)
+   further than 3 modules (mpx pixels each) from where the anchors' map puts it is moved to where the lens model over
+   the anchors and the kept corners puts it and H (in and out) made again from the four: the reference searches such
+   a corner again with its node template first (acquire.c:1242), which this entry does not carry; moved (optional)
+   receives how many. anchors as lizx_anchor_find's, about the final H; model (15 doubles) the corners' and anchors'
+   lens model (acquire.c:2038-2057). Returns 1 where that model fits, else 0. */
+LIZ_API int lizx_anchor_register(liz_decoder *d, int span, const uint8_t *img, int w, int h, int stride, float *H,
+                                 float mpx, float *quad, float *anchors, double *model, int *moved);
 /* Holds a registration for liz_decode (the state lizard_acquire leaves for lizard_finish); reg NULL lets it go, and
    liz_decode then finds each frame's symbol itself. The nodes are copied.
    span 128/256/512. version 0: the frames' words (or *held) name the format; 4..64: the format for frames whose word

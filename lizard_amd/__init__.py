@@ -137,6 +137,9 @@ def lib():
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
     sig("lizx_decoder_graphs", _I, _P, _I)
+    sig("lizx_finder_anchors", _I, _P, _I)
+    sig("lizx_anchor_find", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P)
+    sig("lizx_anchor_register", _I, _P, _I, _P, _I, _I, _I, _P, _F, _P, _P, _P, _P)
     sig("lizx_find", _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_find_batch", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_new", _P, _I)
@@ -314,9 +317,10 @@ def copy_probe(dst, src, nbytes):
 
 
 class Camera(C.Structure):
-    """lizx_camera: w x h pixels, H (module -> pixel, 9 doubles), radial k1, blur sigma, AWGN sigma, noise seed."""
+    """lizx_camera: w x h pixels, H (module -> pixel, 9 doubles), radial k1, blur sigma, AWGN sigma, noise seed,
+    the second radial term k2."""
     _fields_ = [("w", C.c_int), ("h", C.c_int), ("H", C.c_double * 9), ("k1", C.c_double), ("blur", C.c_float),
-                ("noise", C.c_float), ("seed", C.c_uint64)]
+                ("noise", C.c_float), ("seed", C.c_uint64), ("k2", C.c_double)]
 
 
 class DeviceArray:
@@ -670,11 +674,12 @@ class Decoder:
         check(lib().lizx_modulate(self.h, version, frames, _p(blocks), cnt, d.ptr, C.byref(ms)))
         return (d, ms.value) if timed else d
 
-    def capture(self, version, span, d_picture, w, h, H, k1=0.0, blur=0.0, noise=0.0, seed=0, out=None, fps=None):
+    def capture(self, version, span, d_picture, w, h, H, k1=0.0, blur=0.0, noise=0.0, seed=0, out=None, fps=None,
+                k2=0.0):
         """lizx_capture: a device picture through a camera -> a DeviceArray of h x w uint8 (or into `out`); with fps,
         lizx_capture_symbol: the whole symbol (border, word for (version, fps), anchors, guard) round the picture."""
         cam = Camera(int(w), int(h), (C.c_double * 9)(*np.asarray(H, np.float64).ravel()), float(k1), float(blur),
-                     float(noise), int(seed))
+                     float(noise), int(seed), float(k2))
         d = out if out is not None else DeviceArray(int(w) * int(h))
         if fps is None:
             check(lib().lizx_capture(self.h, version, span, d_picture.ptr, C.byref(cam), d.ptr))
@@ -744,6 +749,34 @@ class Decoder:
     def graphs(self, on):
         """Single-frame decodes replayed as one captured HIP graph (lizx_decoder_graphs; on by default)."""
         check(lib().lizx_decoder_graphs(self.h, 1 if on else 0))
+
+    def anchors(self, on):
+        """The finder's registration from the anchors outward (lizx_finder_anchors; on by default)."""
+        check(lib().lizx_finder_anchors(self.h, 1 if on else 0))
+
+    def anchor_find(self, span, img, H, model=None):
+        """The anchors' search on the GPU (lizx_anchor_find): (found, records [49, 8]: X, Y, dx, dy, peak, u, v, ok)."""
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape
+        H = np.ascontiguousarray(H, np.float32).ravel()
+        m = None if model is None else np.ascontiguousarray(model, np.float64)
+        rec = np.zeros((49, 8), np.float32)
+        found = check(lib().lizx_anchor_find(self.h, span, _p(img), w, h, w, _p(H), _p(m), _p(rec)))
+        return found, rec
+
+    def anchor_register(self, span, img, H, mpx, quad):
+        """The registration from the anchors outward (lizx_anchor_register): a dict of have, H (9), quad (8), records
+        [49, 8], model (15), moved."""
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape
+        H = np.array(H, np.float32).ravel()
+        quad = np.array(quad, np.float32).ravel()
+        rec = np.zeros((49, 8), np.float32)
+        model = np.zeros(15, np.float64)
+        moved = C.c_int(0)
+        have = check(lib().lizx_anchor_register(self.h, span, _p(img), w, h, w, _p(H), float(mpx), _p(quad), _p(rec),
+                                                _p(model), C.byref(moved)))
+        return {"have": have, "H": H, "quad": quad, "records": rec, "model": model, "moved": moved.value}
 
     def hold(self, version, span, H, nodes):
         self._held = registration(H, nodes)

@@ -12,7 +12,7 @@ OUT = os.path.join(HERE, "liblizard_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 SOURCES = ["format.cpp", "api.cpp", "batch.cpp", "register.cpp", "word.cpp", "find.cpp", "shake.cpp", "replay.cpp",
-           "ldpc.hip", "spectrum.hip", "refine.hip", "channel.hip", "finder.hip", "xfer.hip"]
+           "ldpc.hip", "spectrum.hip", "refine.hip", "channel.hip", "finder.hip", "xfer.hip", "anchor.hip"]
 # -packed-fp32-ops: no packed FP32 instructions are generated (v_pk_*_f32).
 # -ffp-contract=off: every float multiply and add rounds on its own, as in every reference build
 # (liblizard/build.sh:15-17); the warp, detrend, pilot turn and soft scale reproduce the reference's float order.
@@ -35,7 +35,7 @@ def build(jobs=4):
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + [
-        "-Wl,--version-script=" + os.path.join(CSRC, "exports.map")]
+        "-Wl,--version-script=" + os.path.join(CSRC, "exports.map"), "-Wl,-z,defs"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s%s" % (r.stdout, r.stderr))

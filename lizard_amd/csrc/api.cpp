@@ -135,6 +135,7 @@ struct liz_decoder {
   std::vector<float> found_nodes;
   // the single-frame decode's kernels as one HIP graph, kept for the format and work areas it was captured with
   bool use_graphs = true;
+  bool use_anchors = true;  // the finder registers from the anchors outward
   struct FrameGraph {
     int version = 0, span = 0;
     bool dbg = false;
@@ -662,6 +663,93 @@ LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on) {
   });
 }
 
+static int lane_anchor_find(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H,
+                            const LensModel *lm, AnchorHit *A);
+
+LIZ_API int lizx_finder_anchors(liz_decoder *d, int on) {
+  return guard([&] {
+    need(d, "a decoder");
+    d->use_anchors = on != 0;
+    return 0;
+  });
+}
+
+// A capture's upload into the finder's device image (rows w bytes apart).
+static const uint8_t *finder_upload(liz_decoder *d, const uint8_t *img, int w, int h, int stride) {
+  need(img && w >= 16 && h >= 16 && w <= 16384 && h <= 16384 && stride >= w,
+       "a capture of 16 to 16384 pixels a side, rows stride >= w bytes apart");
+  LdpcEngine &e = *d->eng;
+  hip_check(hipSetDevice(e.device), "hipSetDevice");
+  d->finder.img.reserve(static_cast<size_t>(w) * h);
+  hip_check(hipMemcpy2DAsync(d->finder.img.p, w, img, stride, w, h, hipMemcpyHostToDevice, e.stream), "capture");
+  d->finder.main.s = e.stream;
+  d->finder.main.anch.reserve(ANCH_AREA);
+  d->finder.main.h_anch.reserve(ANCH_AREA);
+  return d->finder.img.as<uint8_t>();
+}
+
+static void model_of(const double *model, LensModel &m) {
+  m.K = static_cast<int>(model[0]);
+  need(m.K >= 0 && m.K <= LENS_TERMS, "a lens model of 0 to 3 radial terms");
+  m.ax = model[1];
+  m.ay = model[2];
+  m.R = model[3];
+  for (int a = 0; a < LENS_P; a++) m.t[a] = model[4 + a];
+}
+
+static void model_to(const LensModel &m, double *model) {
+  model[0] = m.K;
+  model[1] = m.ax;
+  model[2] = m.ay;
+  model[3] = m.R;
+  for (int a = 0; a < LENS_P; a++) model[4 + a] = m.t[a];
+}
+
+static void records_to(const AnchorHit *A, float *out) {
+  for (int k = 0; k < ANCH_N; k++) {
+    float *o = out + 8 * k;
+    o[0] = A[k].X, o[1] = A[k].Y, o[2] = A[k].dx, o[3] = A[k].dy;
+    o[4] = A[k].peak, o[5] = A[k].u, o[6] = A[k].v, o[7] = static_cast<float>(A[k].ok);
+  }
+}
+
+LIZ_API int lizx_anchor_find(liz_decoder *d, int span, const uint8_t *img, int w, int h, int stride, const float *H,
+                             const double *model, float *anchors) {
+  return guard([&] {
+    need(d && H && anchors, "a decoder, a homography and room for the records");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    const uint8_t *d_img = finder_upload(d, img, w, h, stride);
+    LensModel m;
+    if (model) model_of(model, m);
+    AnchorHit A[ANCH_N] = {};
+    const int found = lane_anchor_find(d->finder.main, span, d_img, w, h, H, model ? &m : nullptr, A);
+    records_to(A, anchors);
+    return found;
+  });
+}
+
+LIZ_API int lizx_anchor_register(liz_decoder *d, int span, const uint8_t *img, int w, int h, int stride, float *H,
+                                 float mpx, float *quad, float *anchors, double *model, int *moved) {
+  return guard([&] {
+    need(d && H && quad && anchors && model, "a decoder, a homography, the corners and room for the results");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    need(mpx > 0, "a positive module size in pixels");
+    const uint8_t *d_img = finder_upload(d, img, w, h, stride);
+    AnchorHit A[ANCH_N] = {};
+    LensModel m;
+    m.K = 0;
+    const AnchorFind find = [&](const float *Hs, const LensModel *lm, AnchorHit *an) {
+      return lane_anchor_find(d->finder.main, span, d_img, w, h, Hs, lm, an);
+    };
+    int mv = 0;
+    const int have = anchor_register(span, H, w, h, mpx, quad, find, A, &m, &mv);
+    records_to(A, anchors);
+    model_to(m, model);
+    if (moved) *moved = mv;
+    return have;
+  });
+}
+
 LIZ_API int liz_decoder_max_blocks(const liz_decoder *d) {
   return guard([&] {
     need(d, "a decoder");
@@ -1024,6 +1112,7 @@ static int capture_impl(liz_decoder *d, int version, int span, int fps, const fl
     c.Hi[7] = (H[1] * H[6] - H[0] * H[7]) / det;
     c.Hi[8] = (H[0] * H[4] - H[1] * H[3]) / det;
     c.k1 = cam->k1;
+    c.k2 = cam->k2;
     c.ax = cam->w / 2.0;
     c.ay = cam->h / 2.0;
     c.D = std::hypot(c.ax, c.ay);
@@ -1601,6 +1690,108 @@ LIZ_API int lizx_decoder_hold(liz_decoder *d, int version, int span, const lizx_
   });
 }
 
+// The anchors' search on a lane's stream for anchor_register: every anchor of A not yet found searched through H (and
+// the lens model, when given) on the device capture; returns how many are found in all.
+static int lane_anchor_find(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H,
+                            const LensModel *lm, AnchorHit *A) {
+  AnchorJob &J = *ln.h_anch.as<AnchorJob>();
+  AnchorHit *hh = reinterpret_cast<AnchorHit *>(ln.h_anch.as<char>() + ANCH_JOB_BYTES);
+  J = AnchorJob{};
+  J.img = d_img;
+  J.w = w;
+  J.h = h;
+  J.span = span;
+  anchor_pattern(span, J);
+  for (int i = 0; i < ANCH_K; i++) J.m0[i] = static_cast<float>(anchor_m0(span, i));
+  std::memcpy(J.H, H, sizeof J.H);
+  J.lens_on = lm ? 1 : 0;
+  if (lm) {
+    J.K = lm->K;
+    for (int a = 0; a < LENS_P; a++) J.t[a] = lm->t[a];
+    J.ax = lm->ax;
+    J.ay = lm->ay;
+    J.R = lm->R;
+  }
+  for (int k = 0; k < ANCH_N; k++)
+    if (A[k].ok) J.skip |= 1ull << k;
+  AnchorJob *dj = ln.anch.as<AnchorJob>();
+  AnchorHit *dh = reinterpret_cast<AnchorHit *>(ln.anch.as<char>() + ANCH_JOB_BYTES);
+  hip_check(hipMemcpyAsync(dj, &J, sizeof J, hipMemcpyHostToDevice, ln.s), "anchor job");
+  launch_anchor_find(dj, 1, dh, ln.s);
+  hip_check(hipGetLastError(), "anchor search");
+  hip_check(hipMemcpyAsync(hh, dh, ANCH_N * sizeof(AnchorHit), hipMemcpyDeviceToHost, ln.s), "anchor records");
+  hip_check(hipStreamSynchronize(ln.s), "anchor search");
+  int found = 0;
+  for (int k = 0; k < ANCH_N; k++) {
+    if (!((J.skip >> k) & 1ull)) A[k] = hh[k];
+    found += A[k].ok ? 1 : 0;
+  }
+  return found;
+}
+
+// The lattice's corners in pixels through H and the nodes, top left and clockwise, and pixels a module along the
+// quad's sides over the lattice's (liblizard/src/acquire.c:5009-5013).
+static float lattice_quad(int span, const float *H, const float *nodes, float *quad) {
+  float pos[MAX_NODES], src[8];
+  const int nx = node_lattice(span, pos);
+  for (int c = 0; c < 4; c++) {
+    const int i = (c == 1 || c == 2) ? nx - 1 : 0, j = c >= 2 ? nx - 1 : 0;
+    src[2 * c] = pos[i];
+    src[2 * c + 1] = pos[j];
+    float u, v;
+    project_f(H, pos[i], pos[j], &u, &v);
+    quad[2 * c] = u + nodes[2 * (j * nx + i)];
+    quad[2 * c + 1] = v + nodes[2 * (j * nx + i) + 1];
+  }
+  float mpx = 0;
+  for (int k = 0; k < 4; k++) {
+    const int k1 = (k + 1) & 3;
+    mpx += std::hypot(quad[2 * k1] - quad[2 * k], quad[2 * k1 + 1] - quad[2 * k + 1]) /
+           std::hypot(src[2 * k1] - src[2 * k], src[2 * k1 + 1] - src[2 * k + 1]) / 4;
+  }
+  return mpx;
+}
+
+// The finder's registration (H, and the nodes of its radial lens) taken on from the anchors: anchor_register through
+// H, then the interior from the border nodes and the anchors (anchor_fill). H and nodes are replaced where 12 or more
+// anchors are found and their lens model fits; returns the anchors found then (else 0, the registration unchanged).
+static int anchor_stage(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, float *H, float *nodes) {
+  float quad[8], Hn[9];
+  const float mpx = lattice_quad(span, H, nodes, quad);
+  if (!(mpx > 0)) return 0;
+  std::memcpy(Hn, H, sizeof Hn);
+  AnchorHit A[ANCH_N];
+  LensModel lm;
+  int moved = 0;
+  const AnchorFind find = [&](const float *Hs, const LensModel *m, AnchorHit *an) {
+    return lane_anchor_find(ln, span, d_img, w, h, Hs, m, an);
+  };
+  if (!anchor_register(span, Hn, w, h, mpx, quad, find, A, &lm, &moved)) return 0;
+  float pos[MAX_NODES];
+  const int nx = node_lattice(span, pos);
+  if (moved) {  // the border nodes about the new H: the same pixels
+    for (int j = 0; j < nx; j++)
+      for (int i = 0; i < nx; i++) {
+        if (i && j && i < nx - 1 && j < nx - 1) continue;
+        float u0, v0, u1, v1;
+        project_f(H, pos[i], pos[j], &u0, &v0);
+        project_f(Hn, pos[i], pos[j], &u1, &v1);
+        nodes[2 * (j * nx + i)] += u0 - u1;
+        nodes[2 * (j * nx + i) + 1] += v0 - v1;
+      }
+  }
+  AnchorReading R[ANCH_N];
+  int found = 0;
+  for (int k = 0; k < ANCH_N; k++) {
+    R[k] = AnchorReading{A[k].X, A[k].Y, A[k].dx, A[k].dy, A[k].ok};
+    found += A[k].ok ? 1 : 0;
+  }
+  const float nan = __builtin_nanf("");
+  anchor_fill(span, Hn, w, h, nan, nan, nodes, nullptr, R, mpx, nullptr);
+  std::memcpy(H, Hn, sizeof Hn);
+  return found;
+}
+
 // The finder on a capture: uploads it, searches, and writes the registration (nodes into `nodes`, nx * nx pairs).
 static bool find_registration(liz_decoder *d, const uint8_t *img, int w, int h, int stride, FindResult &fr,
                               lizx_registration &reg, std::vector<float> &nodes) {
@@ -1626,6 +1817,7 @@ static bool find_registration(liz_decoder *d, const uint8_t *img, int w, int h, 
   for (int i = 0; i < 8; i++) params[i] = reg.H[i];
   params[8] = static_cast<float>(fr.k1);
   reg_nodes(m, params, nodes);
+  if (d->use_anchors) anchor_stage(d->finder.main, fr.span, d->finder.img.as<uint8_t>(), w, h, reg.H, nodes.data());
   reg.nodes = nodes.data();
   return true;
 }
@@ -1667,7 +1859,7 @@ LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *im
     d->finder.img.reserve(fb * frames);
     std::vector<FindResult> res(frames);
     // each frame's registration (and word) on the thread that found it, while the other frames are still searched
-    auto done = [&](int f) {
+    auto done = [&](int f, FinderLane &ln) {
       const FindResult &fr = res[f];
       found[f] = fr.found;
       spans[f] = fr.found ? fr.span : 0;
@@ -1682,6 +1874,7 @@ LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *im
       params[8] = static_cast<float>(fr.k1);
       std::vector<float> nv;
       reg_nodes(m, params, nv);
+      if (d->use_anchors) anchor_stage(ln, fr.span, d->finder.img.as<uint8_t>() + f * fb, w, h, Hf, nv.data());
       float *nf = nodes + static_cast<size_t>(f) * MAX_NODES * MAX_NODES * 2;
       std::memcpy(nf, nv.data(), nv.size() * sizeof(float));
       if (!versions) return;

@@ -1,5 +1,5 @@
 // The camera channel on the GPU, for synthetic captures: a picture rendered into a W x H uint8 frame through a radial
-// lens and a homography, blurred and noised (the model of lizard_amd/sim.py's capture; the lens form of
+// lens (two radial terms) and a homography, blurred and noised (the model of lizard_amd/sim.py's capture; the lens form of
 // liblizard/src/acquire.c:1470-1512). Not part of the receive path: it makes that path's measurement inputs on the
 // device.
 //   k_chan_render  per capture pixel: the lens undone by fixed-point iteration, the inverse homography to module
@@ -62,8 +62,9 @@ __global__ __launch_bounds__(256) void k_chan_render(ChannelParams c, const floa
   double px = X, py = Y;
   for (int it = 0; it < 6; it++) {
     const double qx = (px - c.ax) / c.D, qy = (py - c.ay) / c.D, r2 = qx * qx + qy * qy;
-    px = X - c.k1 * r2 * qx * c.D;
-    py = Y - c.k1 * r2 * qy * c.D;
+    const double f = c.k1 * r2 + c.k2 * r2 * r2;
+    px = X - f * qx * c.D;
+    py = Y - f * qy * c.D;
   }
   const double w = c.Hi[6] * px + c.Hi[7] * py + c.Hi[8];
   const double mx = (c.Hi[0] * px + c.Hi[1] * py + c.Hi[2]) / w, my = (c.Hi[3] * px + c.Hi[4] * py + c.Hi[5]) / w;

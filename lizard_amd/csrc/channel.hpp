@@ -10,7 +10,8 @@ constexpr int CHAN_MAX_R = 16;  // the Gaussian's half-width, at most
 struct ChannelParams {
   int W, Hh, n, span;
   double Hi[9];     // capture pixel -> module coordinates (the homography's inverse)
-  double k1, ax, ay, D;  // radial lens k1 |q|^2 q D about (ax, ay), q = (p - a) / D
+  double k1, ax, ay, D;  // radial lens (k1 |q|^2 + k2 |q|^4) q D about (ax, ay), q = (p - a) / D
+  double k2;
   int r;            // Gaussian half-width (0: no blur)
   float taps[2 * CHAN_MAX_R + 1];
   float noise;      // AWGN sigma, grey levels

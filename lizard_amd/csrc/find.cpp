@@ -22,6 +22,7 @@
 #include <thread>
 #include <vector>
 
+#include "anchor.hpp"
 #include "finder.hpp"
 #include "word.hpp"
 
@@ -825,6 +826,8 @@ void lane_reserve(FinderLane &ln) {
   ln.h_probes.reserve(MAX_FRAME_PROBES * sizeof(FindProbe));
   ln.h_edges.reserve(MAX_FRAME_PROBES * sizeof(FindEdge));
   ln.h_peaks.reserve(FIND_MAX_PEAKS * sizeof(FindPeak) + sizeof(int));
+  ln.anch.reserve(ANCH_AREA);
+  ln.h_anch.reserve(ANCH_AREA);
 }
 
 // A frame's vote and peaks on its lane, then the rest of its search.
@@ -852,7 +855,7 @@ void search_frame(FinderBufs &b, FinderLane &ln, FindJob &j, const uint8_t *d_im
 
 // The search of frames whose device images are in place (upload false) or are uploaded here on their lanes.
 void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
-            hipStream_t s, FindResult *res, bool upload, const std::function<void(int)> &done) {
+            hipStream_t s, FindResult *res, bool upload, const std::function<void(int, FinderLane &)> &done) {
   const float up = res[0].ms[0];
   for (int f = 0; f < F; f++) res[f] = FindResult{};
   res[0].ms[0] = up;
@@ -890,7 +893,7 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
       res[f].ms[0] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     search_frame(b, ln, jobs[f], dst, d_tab, nr, cells);
-    if (done) done(f);
+    if (done) done(f, ln);
   };
   if (F == 1) {
     b.main.s = s;
@@ -928,7 +931,7 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
 }  // namespace
 
 void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
-                  hipStream_t s, FindResult *res, const std::function<void(int)> &done) {
+                  hipStream_t s, FindResult *res, const std::function<void(int, FinderLane &)> &done) {
   search(b, F, d_imgs, imgs, w, h, stride, s, res, true, done);
 }
 

@@ -90,8 +90,8 @@ struct HostBuf {
 struct FinderLane {
   hipStream_t s = nullptr;
   bool own = false;  // the stream was made for this lane
-  DevBuf probes, edges;
-  HostBuf h_probes, h_edges, h_peaks;
+  DevBuf probes, edges, anch;         // anch: the anchors' search job and records
+  HostBuf h_probes, h_edges, h_peaks, h_anch;
   FinderLane() = default;
   FinderLane(const FinderLane &) = delete;
   FinderLane &operator=(const FinderLane &) = delete;
@@ -115,8 +115,9 @@ void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w,
 // The same for F frames of one size at once (d_imgs: room for F device images, rows w apart, each w h bytes apart in
 // one buffer; imgs: the host frames, rows stride apart), each frame uploaded into its place and searched on a host
 // thread of its own with a stream of its own. Each frame's result is the one find_symbol gives it. done (may be empty)
-// is called with each frame's index on the thread that searched it, as soon as its result is in place.
+// is called with each frame's index and the lane that searched it on that lane's thread, as soon as its result is in
+// place.
 void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
-                  hipStream_t s, FindResult *r, const std::function<void(int)> &done);
+                  hipStream_t s, FindResult *r, const std::function<void(int, FinderLane &)> &done);
 
 }  // namespace lz

@@ -2,6 +2,8 @@
 // double, computed once per (ring, frame size, reference homography).
 #include "register.hpp"
 
+#include <functional>
+
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -53,25 +55,6 @@ void herm4(const double *t, int n, double x, int *b, double w[4]) {
 // operations so the nodes are its to the bit
 
 namespace {
-
-// H applied to a module point in float, the reference's order (internal.h:107-112).
-void project_f(const float *h, float x, float y, float *u, float *v) {
-  const float w = h[6] * x + h[7] * y + 1;
-  *u = (h[0] * x + h[1] * y + h[2]) / w;
-  *v = (h[3] * x + h[4] * y + h[5]) / w;
-}
-
-// The model's rows at q = (x, y): x, y, the shift and the two perspective terms on each axis, then K radial terms.
-void lens_rows(double x, double y, int K, double *rx, double *ry) {
-  rx[0] = x, rx[1] = y, rx[2] = 0, rx[3] = 0, rx[4] = 1, rx[5] = 0, rx[6] = -x * x, rx[7] = -x * y;
-  ry[0] = 0, ry[1] = 0, ry[2] = x, ry[3] = y, ry[4] = 0, ry[5] = 1, ry[6] = -x * y, ry[7] = -y * y;
-  const double s = x * x + y * y;
-  double sk = s;
-  for (int k = 0; k < K; k++, sk *= s) {
-    rx[8 + k] = sk * x;
-    ry[8 + k] = sk * y;
-  }
-}
 
 // Gauss-Jordan with partial pivots on the first n rows and columns of [A | b] (b in column LENS_P).
 bool solve_lens(double A[LENS_P][LENS_P + 1], int n) {
@@ -130,14 +113,7 @@ void lens_axis(int w, int h, float axis_x, float axis_y, LensModel &m) {
 }
 
 void lens_at(const LensModel &m, double u, double v, float *dx, float *dy) {
-  double rx[LENS_P], ry[LENS_P], fx = 0, fy = 0;
-  lens_rows((u - m.ax) / m.R, (v - m.ay) / m.R, m.K, rx, ry);
-  for (int a = 0; a < 8 + m.K; a++) {
-    fx += m.t[a] * rx[a];
-    fy += m.t[a] * ry[a];
-  }
-  *dx = static_cast<float>(fx);
-  *dy = static_cast<float>(fy);
+  lens_disp(m.K, m.t, m.ax, m.ay, m.R, u, v, dx, dy);
 }
 
 int border_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use_in,
@@ -531,6 +507,259 @@ int anchor_fill(int span, const float *H, int w, int h, float axis_x, float axis
     }
   if (out) *out = lm;
   return lm.K;
+}
+
+// ---- the registration from the anchors outward (acquire.c:2038-2191), the same float and double operations
+
+namespace {
+
+constexpr int ANCH_ROUNDS = 6, ANCH_CORNER = 3, ANCH_MIN = 12;  // acquire.c:1716, internal.h:186
+
+// Gauss-Jordan with partial pivots on [A | b], 8 unknowns (acquire.c:610-640, its scalar form).
+bool solve8(double A[8][9]) {
+  for (int c = 0; c < 8; c++) {
+    int p = c;
+    for (int r = c + 1; r < 8; r++)
+      if (std::fabs(A[r][c]) > std::fabs(A[p][c])) p = r;
+    if (std::fabs(A[p][c]) < 1e-12) return false;
+    for (int k = 0; k < 9; k++) std::swap(A[c][k], A[p][k]);
+    for (int r = 0; r < 8; r++) {
+      if (r == c) continue;
+      const double f = A[r][c] / A[c][c];
+      for (int k = c; k < 9; k++) A[r][k] -= f * A[c][k];
+    }
+  }
+  return true;
+}
+
+// The homography through four point pairs, module -> pixel, h8 = 1 (acquire.c:643-654).
+bool homography4(const float *src, const float *dst, float *H) {
+  double A[8][9];
+  for (int i = 0; i < 4; i++) {
+    const double x = src[2 * i], y = src[2 * i + 1], u = dst[2 * i], v = dst[2 * i + 1];
+    const double r0[9] = {x, y, 1, 0, 0, 0, -u * x, -u * y, u}, r1[9] = {0, 0, 0, x, y, 1, -v * x, -v * y, v};
+    std::memcpy(A[2 * i], r0, sizeof r0);
+    std::memcpy(A[2 * i + 1], r1, sizeof r1);
+  }
+  if (!solve8(A)) return false;
+  for (int i = 0; i < 8; i++) H[i] = static_cast<float>(A[i][8] / A[i][i]);
+  H[8] = 1;
+  return true;
+}
+
+// A homography through the anchors found, least squares with h8 = 1 on coordinates centred and scaled; false where
+// fewer than eight are found or they span fewer than three rows or columns (acquire.c:2061-2096).
+bool anchor_homo(const AnchorHit *A, float *H) {
+  int n = 0, rows = 0, cols = 0, rm = 0, cm = 0;
+  double mx = 0, my = 0, mu = 0, mv = 0;
+  for (int k = 0; k < ANCH_N; k++)
+    if (A[k].ok) {
+      n++;
+      mx += A[k].X;
+      my += A[k].Y;
+      mu += A[k].u;
+      mv += A[k].v;
+      rm |= 1 << (k / ANCH_K);
+      cm |= 1 << (k % ANCH_K);
+    }
+  for (int b = 0; b < ANCH_K; b++) {
+    rows += (rm >> b) & 1;
+    cols += (cm >> b) & 1;
+  }
+  if (n < 8 || rows < 3 || cols < 3) return false;
+  mx /= n;
+  my /= n;
+  mu /= n;
+  mv /= n;
+  double sx = 0, su = 0;
+  for (int k = 0; k < ANCH_N; k++)
+    if (A[k].ok) {
+      sx += std::hypot(A[k].X - mx, A[k].Y - my);
+      su += std::hypot(A[k].u - mu, A[k].v - mv);
+    }
+  sx = sx / n / 1.4142135623730951;
+  su = su / n / 1.4142135623730951;
+  if (!(sx > 0) || !(su > 0)) return false;
+  double M[LENS_P][LENS_P + 1] = {};
+  for (int k = 0; k < ANCH_N; k++) {
+    if (!A[k].ok) continue;
+    const double x = (A[k].X - mx) / sx, y = (A[k].Y - my) / sx, u = (A[k].u - mu) / su, v = (A[k].v - mv) / su;
+    const double r0[8] = {x, y, 1, 0, 0, 0, -u * x, -u * y}, r1[8] = {0, 0, 0, x, y, 1, -v * x, -v * y};
+    for (int a = 0; a < 8; a++) {
+      for (int b = 0; b < 8; b++) M[a][b] += r0[a] * r0[b] + r1[a] * r1[b];
+      M[a][LENS_P] += r0[a] * u + r1[a] * v;
+    }
+  }
+  if (!solve_lens(M, 8)) return false;
+  double g[9];
+  for (int a = 0; a < 8; a++) g[a] = M[a][LENS_P] / M[a][a];
+  g[8] = 1;
+  // the scalings undone: H = Tu^-1 G Tx
+  const double tx[9] = {1 / sx, 0, -mx / sx, 0, 1 / sx, -my / sx, 0, 0, 1}, ti[9] = {su, 0, mu, 0, su, mv, 0, 0, 1};
+  double gt[9], hh[9];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      gt[3 * r + c] = 0;
+      for (int k = 0; k < 3; k++) gt[3 * r + c] += g[3 * r + k] * tx[3 * k + c];
+    }
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      hh[3 * r + c] = 0;
+      for (int k = 0; k < 3; k++) hh[3 * r + c] += ti[3 * r + k] * gt[3 * k + c];
+    }
+  if (!(std::fabs(hh[8]) > 1e-12)) return false;
+  for (int a = 0; a < 9; a++) H[a] = static_cast<float>(hh[a] / hh[8]);
+  return true;
+}
+
+// The lens model the anchors alone say about H, held to the border's robust rule (a reading further from it than the
+// larger of mpx / 2 and three robust spreads dropped, and no longer found); false where it does not fit
+// (acquire.c:2099-2117).
+bool anchor_model_free(const float *H, int w, int h, AnchorHit *A, LensModel &m, float mpx) {
+  std::vector<double> qx, qy, dx, dy;
+  std::vector<int> at;
+  lens_axis(w, h, NAN, NAN, m);
+  for (int k = 0; k < ANCH_N; k++) {
+    if (!A[k].ok) continue;
+    float hu, hv;
+    project_f(H, A[k].X, A[k].Y, &hu, &hv);
+    A[k].dx = A[k].u - hu;
+    A[k].dy = A[k].v - hv;
+    qx.push_back((hu - m.ax) / m.R);
+    qy.push_back((hv - m.ay) / m.R);
+    dx.push_back(A[k].dx);
+    dy.push_back(A[k].dy);
+    at.push_back(k);
+  }
+  const int np = static_cast<int>(qx.size());
+  std::vector<uint8_t> isb(np, 1), drop(np, 0);
+  if (np < 4 || !lensm_fit(np, qx, qy, dx, dy, &isb, drop, 0.5 * mpx, m)) return false;
+  for (int i = 0; i < np; i++)
+    if (drop[i]) A[at[i]].ok = 0;
+  return true;
+}
+
+// The lattice's corner c (0 top left, then clockwise) in modules.
+void corner_node(const float *pos, int nx, int c, float *x, float *y) {
+  *x = pos[(c == 1 || c == 2) ? nx - 1 : 0];
+  *y = pos[c >= 2 ? nx - 1 : 0];
+}
+
+// The lens model the anchors and the four corners (displacement 0: H passes through them) say about H
+// (acquire.c:2038-2057).
+bool anchor_model(const float *pos, int nx, const float *H, int w, int h, const AnchorHit *A, LensModel &m) {
+  std::vector<double> qx, qy, dx, dy;
+  lens_axis(w, h, NAN, NAN, m);
+  for (int c = 0; c < 4; c++) {
+    float x, y, u, v;
+    corner_node(pos, nx, c, &x, &y);
+    project_f(H, x, y, &u, &v);
+    qx.push_back((u - m.ax) / m.R);
+    qy.push_back((v - m.ay) / m.R);
+    dx.push_back(0);
+    dy.push_back(0);
+  }
+  for (int k = 0; k < ANCH_N; k++) {
+    if (!A[k].ok) continue;
+    float u, v;
+    project_f(H, A[k].X, A[k].Y, &u, &v);
+    qx.push_back((u - m.ax) / m.R);
+    qy.push_back((v - m.ay) / m.R);
+    dx.push_back(A[k].dx);
+    dy.push_back(A[k].dy);
+  }
+  const int np = static_cast<int>(qx.size());
+  std::vector<uint8_t> drop(np, 0);
+  return lensm_fit(np, qx, qy, dx, dy, nullptr, drop, 0, m);
+}
+
+}  // namespace
+
+int anchor_register(int span, float *H, int w, int h, float mpx, float *quad, const AnchorFind &find, AnchorHit *A,
+                    LensModel *out, int *moved) {
+  float pos[MAX_NODES];
+  const int nx = node_lattice(span, pos);
+  float Ha[9];
+  std::memcpy(Ha, H, sizeof Ha);
+  LensModel mf;
+  bool have = false;
+  int nf = 0;
+  if (moved) *moved = 0;
+  for (int k = 0; k < ANCH_N; k++) A[k].ok = 0;
+  for (int round = 0; round < ANCH_ROUNDS; round++) {
+    const int now = find(Ha, have ? &mf : nullptr, A);
+    if (now == nf || now < 4) break;
+    nf = now;
+    float Hn[9];
+    if (anchor_homo(A, Hn)) std::memcpy(Ha, Hn, sizeof Ha);
+    if (!(have = anchor_model_free(Ha, w, h, A, mf, mpx))) break;
+    nf = 0;
+    for (int k = 0; k < ANCH_N; k++) nf += A[k].ok;
+  }
+  if (!have || nf < ANCH_MIN) return 0;
+  // the anchors and the corners about the corners' H; a corner further than ANCH_CORNER modules from where the
+  // anchors' own map (Ha and mf) puts it is left out of the fit
+  std::vector<double> qx, qy, dx, dy;
+  LensModel mc;
+  lens_axis(w, h, NAN, NAN, mc);
+  for (int c = 0; c < 4; c++) {
+    float x, y, u, v;
+    corner_node(pos, nx, c, &x, &y);
+    project_f(H, x, y, &u, &v);
+    qx.push_back((u - mc.ax) / mc.R);
+    qy.push_back((v - mc.ay) / mc.R);
+    dx.push_back(0);
+    dy.push_back(0);
+  }
+  for (int k = 0; k < ANCH_N; k++) {
+    if (!A[k].ok) continue;
+    float hu, hv;
+    project_f(H, A[k].X, A[k].Y, &hu, &hv);
+    qx.push_back((hu - mc.ax) / mc.R);
+    qy.push_back((hv - mc.ay) / mc.R);
+    dx.push_back(A[k].u - hu);
+    dy.push_back(A[k].v - hv);
+  }
+  const int np = static_cast<int>(qx.size());
+  std::vector<uint8_t> drop(np, 0);
+  for (int c = 0; c < 4; c++) {
+    float x, y, pu, pv, mu, mv;
+    corner_node(pos, nx, c, &x, &y);
+    project_f(Ha, x, y, &pu, &pv);
+    lens_at(mf, pu, pv, &mu, &mv);
+    drop[c] = std::hypot(pu + mu - quad[2 * c], pv + mv - quad[2 * c + 1]) > ANCH_CORNER * mpx;
+  }
+  const bool fit = lensm_fit(np, qx, qy, dx, dy, nullptr, drop, 0, mc);
+  // a corner the anchors disagree with goes where the model over the anchors and the kept corners puts it, and H is
+  // made again from the four
+  int nm = 0;
+  for (int c = 0; fit && c < 4; c++) {
+    if (!drop[c]) continue;
+    float x, y, hu, hv, mu, mv;
+    corner_node(pos, nx, c, &x, &y);
+    project_f(H, x, y, &hu, &hv);
+    lens_at(mc, hu, hv, &mu, &mv);
+    quad[2 * c] = hu + mu;
+    quad[2 * c + 1] = hv + mv;
+    nm++;
+  }
+  if (moved) *moved = nm;
+  if (nm) {
+    float src[8], Hc[9];
+    for (int c = 0; c < 4; c++) corner_node(pos, nx, c, &src[2 * c], &src[2 * c + 1]);
+    if (homography4(src, quad, Hc)) std::memcpy(H, Hc, sizeof Hc);
+  }
+  for (int k = 0; k < ANCH_N; k++)
+    if (A[k].ok) {
+      float hu, hv;
+      project_f(H, A[k].X, A[k].Y, &hu, &hv);
+      A[k].dx = A[k].u - hu;
+      A[k].dy = A[k].v - hv;
+    }
+  LensModel m;
+  if (!anchor_model(pos, nx, H, w, h, A, m)) return 0;
+  if (out) *out = m;
+  return 1;
 }
 
 void reg_model(int span, const float *H0, int w, int h, RegModel &m) {

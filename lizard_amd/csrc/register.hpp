@@ -5,8 +5,11 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <functional>
 #include <vector>
 
+#include "anchor.hpp"
+#include "lens.hpp"
 #include "spectrum.hpp"
 
 namespace lz {
@@ -36,7 +39,6 @@ void reg_chain(const RegModel &m, const double *node_grad, double *grad);
 // change of the homography (x, y on both axes, the shift, the two perspective terms -(g . q) q) plus K (0..3) radial
 // terms k |q|^(2k) q, q = (the node's place under H - the axis) / R, R the farthest frame corner's distance from the
 // axis; the axis is the frame's centre unless given.
-constexpr int LENS_TERMS = 3, LENS_P = 8 + LENS_TERMS;
 struct LensModel {
   int K = -1;            // radial terms; -1: no model (H alone)
   double t[LENS_P] = {};  // coefficients: 8 affine and perspective, then K radial
@@ -66,6 +68,21 @@ struct AnchorReading {
 // of its found neighbours). Returns the model's K.
 int anchor_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use,
                 const AnchorReading *anchors, float mpx, LensModel *out);
+
+// The anchors' search through H and, when given, a lens model about it: every anchor of A not yet found searched
+// (anchor.hpp); returns how many are found in all.
+using AnchorFind = std::function<int(const float *H, const LensModel *lm, AnchorHit *A)>;
+// The registration from the anchors outward (acquire.c:2128-2191): the anchors searched through H, then, round after
+// round (at most 6, while more are found), those still missing through the homography the found ones say and the lens
+// model they say about it, each round's readings held to the robust rule (acquire.c:2099-2117); none where fewer than
+// 12 are found. Then each corner of quad (the lattice's corners in pixels, top left and clockwise) further than 3
+// modules (mpx pixels each) from where the anchors' map puts it is moved to where the lens model over the anchors and
+// the kept corners puts it, and H made again from the four corners (the reference searches such a corner again with
+// its node template first and keeps a match that scores 0.35 or more); moved receives how many. Last, the anchors'
+// displacements about H and the lens model of the corners and the anchors (acquire.c:2038-2057) in out. Returns 1
+// with that model, 0 without.
+int anchor_register(int span, float *H, int w, int h, float mpx, float *quad, const AnchorFind &find, AnchorHit *A,
+                    LensModel *out, int *moved);
 
 // Device state of a refinement: the homography and the intensity model w = alpha r + beta.
 struct RegState {

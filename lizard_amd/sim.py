@@ -87,13 +87,14 @@ def homography(src, dst):
 
 class Camera:
     """A frame's geometry and noise: W x Hh pixels; the symbol rotated rot_deg, its side scale x Hh pixels, corners
-    pulled by persp (fractions of Hh), its centre shifted by (cx, cy) pixels; radial k1; blur sigma (pixels); AWGN
-    sigma (grey levels)."""
+    pulled by persp (fractions of Hh), its centre shifted by (cx, cy) pixels; radial k1 and k2; blur sigma (pixels);
+    AWGN sigma (grey levels)."""
 
     def __init__(self, W=1280, Hh=720, rot_deg=5.0, scale=0.8, persp=0.02, k1=0.0, blur=0.8, noise=4.0, cx=0.0,
-                 cy=0.0):
+                 cy=0.0, k2=0.0):
         self.W, self.Hh, self.rot_deg, self.scale, self.persp = W, Hh, rot_deg, scale, persp
         self.k1, self.blur, self.noise, self.cx, self.cy = k1, blur, noise, cx, cy
+        self.k2 = k2
 
     @classmethod
     def random(cls, rng, W=1920, Hh=1080, rot=10.0, scale=(0.6, 0.9), persp=0.05, k1=0.05, blur=(0.5, 1.5),
@@ -120,11 +121,12 @@ class Camera:
         return a, math.hypot(self.W / 2.0, self.Hh / 2.0)
 
     def distort(self, p):
-        """Undistorted pixel points (..., 2) -> where the lens puts them: p + k1 |q|^2 q D, q = (p - a) / D."""
+        """Undistorted pixel points (..., 2) -> where the lens puts them: p + (k1 |q|^2 + k2 |q|^4) q D,
+        q = (p - a) / D."""
         a, D = self.axis()
         q = (p - a) / D
         r2 = np.sum(q * q, -1, keepdims=True)
-        return p + self.k1 * r2 * q * D
+        return p + (self.k1 * r2 + self.k2 * r2 * r2) * q * D
 
 
 def nodes_for(cam, span, Hm):
@@ -152,8 +154,9 @@ def capture(pic, span, cam, seed=0):
     for _ in range(6):
         qx, qy = (px - a[0]) / D, (py - a[1]) / D
         r2 = qx * qx + qy * qy
-        px = X - cam.k1 * r2 * qx * D
-        py = Y - cam.k1 * r2 * qy * D
+        f = cam.k1 * r2 + cam.k2 * r2 * r2
+        px = X - f * qx * D
+        py = Y - f * qy * D
     w = Hi[2, 0] * px + Hi[2, 1] * py + Hi[2, 2]
     mx = (Hi[0, 0] * px + Hi[0, 1] * py + Hi[0, 2]) / w
     my = (Hi[1, 0] * px + Hi[1, 1] * py + Hi[1, 2]) / w
@@ -191,7 +194,7 @@ def capture_gpu(dec, version, span, d_picture, cam, seed=0):
     comes from the library's counter-based generator, not torch's."""
     Hm = cam.H(span)
     img = dec.capture(version, span, d_picture, cam.W, cam.Hh, Hm, k1=cam.k1, blur=cam.blur, noise=cam.noise,
-                      seed=seed)
+                      seed=seed, k2=cam.k2)
     Hn = (Hm / Hm[2, 2]).astype(np.float32).ravel()
     return img, Hn, nodes_for(cam, span, Hm)
 

@@ -94,3 +94,65 @@ void ref_border_fill_anchors(const void *p, const float *H, int iw, int ih, floa
   }
   free(nd);
 }
+
+/* An anchor record as 8 floats: X, Y, dx, dy, peak, u, v, found. */
+static void anchors_out(const anchor_t *A, float *out) {
+  for (int k = 0; k < OB_ANCH_K * OB_ANCH_K; k++) {
+    float *o = out + 8 * k;
+    o[0] = A[k].X; o[1] = A[k].Y; o[2] = A[k].dx; o[3] = A[k].dy;
+    o[4] = A[k].peak; o[5] = A[k].u; o[6] = A[k].v; o[7] = (float)A[k].ok;
+  }
+}
+/* A lens model as 15 doubles: K, ax, ay, R, the 11 coefficients. */
+static void model_out(const lensm_t *m, double *out) {
+  out[0] = m->K; out[1] = m->ax; out[2] = m->ay; out[3] = m->R;
+  for (int a = 0; a < LENS_P; a++) out[4 + a] = a < 8 + m->K ? m->t[a] : 0;
+}
+static void model_in(const double *in, lensm_t *m) {
+  memset(m, 0, sizeof *m);
+  m->K = (int)in[0]; m->ax = in[1]; m->ay = in[2]; m->R = in[3];
+  for (int a = 0; a < LENS_P; a++) m->t[a] = in[4 + a];
+}
+
+/* anchor_find (acquire.c:1876-2036) on a tight iw x ih grey capture at gamma 1, every anchor searched through H and
+   the lens model (15 doubles as model_out writes them; NULL: H alone); records into anchors (49 x 8). */
+int ref_anchor_find(const void *p, const unsigned char *img, int iw, int ih, const float *H, const double *model,
+                    float *anchors) {
+  const lizard_t *f = p;
+  image_t im;
+  ob_image_init(&im, img, iw, ih, 1.0f);
+  homo_t Hh;
+  memcpy(Hh.h, H, sizeof Hh.h);
+  lensm_t lm;
+  if (model) model_in(model, &lm);
+  anchor_t A[OB_ANCH_K * OB_ANCH_K];
+  memset(A, 0, sizeof A);
+  const int found = anchor_find(&f->frame, &im, &Hh, model ? &lm : 0, A);
+  anchors_out(A, anchors);
+  return found;
+}
+
+/* anchor_register (acquire.c:2128-2191) as acquire_core calls it, with the capture unblurred for a corner's search
+   again: H (in and out), the lattice's corners in pixels quad (in and out), pixels a module mpx; the anchors' records
+   and the corners' and anchors' lens model out. Returns the reference's result. */
+int ref_anchor_register(const void *p, const unsigned char *img, int iw, int ih, float *H, float mpx, float *quad,
+                        float *anchors, double *model) {
+  const lizard_t *f = p;
+  const ob_layout_t *L = &f->frame;
+  image_t im;
+  ob_image_init(&im, img, iw, ih, 1.0f);
+  homo_t Hh;
+  memcpy(Hh.h, H, sizeof Hh.h);
+  node_t *nodes = calloc((size_t)L->nx * L->ny, sizeof(node_t));
+  for (int c = 0; c < 4; c++) nodes[((c >= 2) ? L->ny - 1 : 0) * L->nx + ((c == 1 || c == 2) ? L->nx - 1 : 0)].done = 1;
+  anchor_t A[OB_ANCH_K * OB_ANCH_K];
+  memset(A, 0, sizeof A);
+  lensm_t m;
+  memset(&m, 0, sizeof m);
+  const int have = anchor_register(L, &im, &im, &Hh, iw, ih, mpx, quad, nodes, A, &m);
+  free(nodes);
+  memcpy(H, Hh.h, sizeof Hh.h);
+  anchors_out(A, anchors);
+  model_out(&m, model);
+  return have;
+}
