@@ -259,11 +259,14 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
    default), or launches them one by one (0). Results are the same bits either way. No counterpart in the reference
    (liblizard's decoder runs on the CPU, api/codec.cpp:121-160). */
 LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on);
-/* The finder's registration from the anchors outward (default on; off: H and the nodes of the finder's radial lens
-   alone). On: after the border's fit the 7 x 7 anchors are searched and registered (liblizard/src/acquire.c:2128-2191
-   anchor_register, as lizx_anchor_register) and the lattice's interior is made from the border nodes and the anchors
-   (acquire.c:2219-2358, as lizx_reg_anchor_fill), where 12 or more anchors are found. Returns 0. */
-LIZ_API int lizx_finder_anchors(liz_decoder *d, int on);
+/* The finder's registration from the anchors outward. mode 0: H and the nodes of the finder's radial lens alone.
+   Otherwise, after the border's fit, the 7 x 7 anchors are searched and registered (liblizard/src/acquire.c:2128-2191
+   anchor_register, as lizx_anchor_register) and, where 12 or more are found, the lattice's interior is made from the
+   border nodes and the anchors (acquire.c:2219-2358, as lizx_reg_anchor_fill): mode 1 (the default) the finder's own
+   border nodes (its fit through thousands of edge points), all trusted; mode 2 the reference's chain
+   (acquire.c:5042-5063): H through the lattice's corners, the border's marked nodes refined by their templates from
+   the anchors' lens model (as lizx_border_refine), those scoring 0.35 or more trusted. Returns 0. */
+LIZ_API int lizx_finder_anchors(liz_decoder *d, int mode);
 /* The anchors' search on the GPU (liblizard/src/acquire.c:1876-2036 anchor_find, its scalar form, to the bit): every
    anchor of a ring of span modules searched on a capture (host, w x h grey, rows stride bytes apart, read at gamma 1)
    through H (module -> pixel, h8 = 1) and optionally a lens model about it (15 doubles as lizx_reg_border_fill gives
@@ -283,6 +286,15 @@ LIZ_API int lizx_anchor_find(liz_decoder *d, int span, const uint8_t *img, int w
    lens model (acquire.c:2038-2057). Returns 1 where that model fits, else 0. */
 LIZ_API int lizx_anchor_register(liz_decoder *d, int span, const uint8_t *img, int w, int h, int stride, float *H,
                                  float mpx, float *quad, float *anchors, double *model, int *moved);
+/* The border's marked nodes refined on the GPU (liblizard/src/acquire.c:1242-1468 refine_node, its scalar form, to the
+   bit, in acquire_core's order, :5046-5058): each non-corner border node of the lattice searched with its template of
+   the border's own cells (the picture and the format word left out; 13 x 5 modules along a side) from the mean of its
+   measured neighbours less the lens model (15 doubles; null: none) plus the model at the node, on a capture (host,
+   rows stride bytes apart) whose border band reads under a box of radius mpx / 2 pixels where that is 1 or more
+   (acquire.c:4922-4960 border_blur); the corners held at 0. nodes: nx * nx x 4 floats (dx, dy pixels, score, done).
+   Returns nx. */
+LIZ_API int lizx_border_refine(liz_decoder *d, int span, const uint8_t *img, int w, int h, int stride, const float *H,
+                               float mpx, const double *model, float *nodes);
 /* Holds a registration for liz_decode (the state lizard_acquire leaves for lizard_finish); reg NULL lets it go, and
    liz_decode then finds each frame's symbol itself. The nodes are copied.
    span 128/256/512. version 0: the frames' words (or *held) name the format; 4..64: the format for frames whose word

@@ -140,6 +140,7 @@ def lib():
     sig("lizx_finder_anchors", _I, _P, _I)
     sig("lizx_anchor_find", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P)
     sig("lizx_anchor_register", _I, _P, _I, _P, _I, _I, _I, _P, _F, _P, _P, _P, _P)
+    sig("lizx_border_refine", _I, _P, _I, _P, _I, _I, _I, _P, _F, _P, _P)
     sig("lizx_find", _I, _P, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_find_batch", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_new", _P, _I)
@@ -750,9 +751,20 @@ class Decoder:
         """Single-frame decodes replayed as one captured HIP graph (lizx_decoder_graphs; on by default)."""
         check(lib().lizx_decoder_graphs(self.h, 1 if on else 0))
 
-    def anchors(self, on):
-        """The finder's registration from the anchors outward (lizx_finder_anchors; on by default)."""
-        check(lib().lizx_finder_anchors(self.h, 1 if on else 0))
+    def anchors(self, mode):
+        """The finder's registration from the anchors outward (lizx_finder_anchors): 0 off, 1 (the default) over the
+        finder's own border nodes, 2 with the border's nodes refined by their templates (the reference's chain)."""
+        check(lib().lizx_finder_anchors(self.h, int(mode)))
+
+    def border_refine(self, span, img, H, mpx, model=None):
+        """The border's marked nodes refined on the GPU (lizx_border_refine): nodes [nx, nx, 4] (dx, dy, score, done)."""
+        img = np.ascontiguousarray(img, np.uint8)
+        h, w = img.shape
+        H = np.ascontiguousarray(H, np.float32).ravel()
+        m = None if model is None else np.ascontiguousarray(model, np.float64)
+        out = np.zeros(40 * 40 * 4, np.float32)
+        nx = check(lib().lizx_border_refine(self.h, span, _p(img), w, h, w, _p(H), float(mpx), _p(m), _p(out)))
+        return out[:nx * nx * 4].reshape(nx, nx, 4)
 
     def anchor_find(self, span, img, H, model=None):
         """The anchors' search on the GPU (lizx_anchor_find): (found, records [49, 8]: X, Y, dx, dy, peak, u, v, ok)."""

@@ -23,22 +23,6 @@ constexpr int NT = 256;
 constexpr int W1MAX = ANCH_WMAX + 1;
 constexpr float ANCH_PEAK = 0.6f;  // acquire.c:1717
 
-// A point of the capture read bilinearly at gamma 1 (internal.h:11-23: the half pixel, the clamp to the frame, the
-// blend).
-__device__ float sample_px(const uint8_t *px, int w, int h, const float *lut, float x, float y) {
-  x -= 0.5f;
-  y -= 0.5f;
-  if (x < 0) x = 0;
-  else if (x > w - 1.001f) x = w - 1.001f;
-  if (y < 0) y = 0;
-  else if (y > h - 1.001f) y = h - 1.001f;
-  const int xi = x >= 0 ? static_cast<int>(x) : 0, yi = y >= 0 ? static_cast<int>(y) : 0;
-  const float fx = x - xi, fy = y - yi;
-  const uint8_t *p = px + static_cast<size_t>(yi) * w + xi;
-  const float a = lut[p[0]], b = lut[p[1]], c = lut[p[w]], d = lut[p[w + 1]];
-  return (a + (b - a) * fx) * (1 - fy) + (c + (d - c) * fx) * fy;
-}
-
 // A module point through H and, where the job has one, the lens model (acquire.c:1699-1702).
 __device__ void anchor_map(const AnchorJob &j, float X, float Y, float *u, float *v) {
   project_f(j.H, X, Y, u, v);

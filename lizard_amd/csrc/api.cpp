@@ -14,6 +14,7 @@
 #include "channel.hpp"
 #include "word.hpp"
 #include "register.hpp"
+#include "border.hpp"
 #include "finder.hpp"
 #include "xfer.hpp"
 #include "shake.hpp"
@@ -135,7 +136,7 @@ struct liz_decoder {
   std::vector<float> found_nodes;
   // the single-frame decode's kernels as one HIP graph, kept for the format and work areas it was captured with
   bool use_graphs = true;
-  bool use_anchors = true;  // the finder registers from the anchors outward
+  int anchor_mode = 1;  // the finder registers from the anchors outward (lizx_finder_anchors)
   struct FrameGraph {
     int version = 0, span = 0;
     bool dbg = false;
@@ -665,11 +666,14 @@ LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on) {
 
 static int lane_anchor_find(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H,
                             const LensModel *lm, AnchorHit *A);
+static void border_refine(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H, float mpx,
+                          const LensModel *lm, std::vector<NodeState> &st);
 
 LIZ_API int lizx_finder_anchors(liz_decoder *d, int on) {
   return guard([&] {
     need(d, "a decoder");
-    d->use_anchors = on != 0;
+    need(on >= 0 && on <= 2, "0 (off), 1 (anchors over the finder's border) or 2 (anchors and the border refined)");
+    d->anchor_mode = on;
     return 0;
   });
 }
@@ -747,6 +751,28 @@ LIZ_API int lizx_anchor_register(liz_decoder *d, int span, const uint8_t *img, i
     model_to(m, model);
     if (moved) *moved = mv;
     return have;
+  });
+}
+
+LIZ_API int lizx_border_refine(liz_decoder *d, int span, const uint8_t *img, int w, int h, int stride, const float *H,
+                               float mpx, const double *model, float *nodes) {
+  return guard([&] {
+    need(d && H && nodes, "a decoder, a homography and room for the nodes");
+    need(span == 128 || span == 256 || span == 512, "span is a ring's: 128, 256 or 512 modules");
+    need(mpx > 0, "a positive module size in pixels");
+    const uint8_t *d_img = finder_upload(d, img, w, h, stride);
+    LensModel m;
+    if (model) model_of(model, m);
+    std::vector<NodeState> st;
+    border_refine(d->finder.main, span, d_img, w, h, H, mpx, model ? &m : nullptr, st);
+    for (size_t k = 0; k < st.size(); k++) {
+      nodes[4 * k] = st[k].dx;
+      nodes[4 * k + 1] = st[k].dy;
+      nodes[4 * k + 2] = st[k].score;
+      nodes[4 * k + 3] = static_cast<float>(st[k].done);
+    }
+    float pos[MAX_NODES];
+    return node_lattice(span, pos);
   });
 }
 
@@ -1752,14 +1778,97 @@ static float lattice_quad(int span, const float *H, const float *nodes, float *q
   return mpx;
 }
 
+// The border's marked nodes refined on the GPU through H and, when given, the anchors' lens model, in the reference's
+// order (border.hpp); st: the lattice's nx * nx nodes (the corners done at 0, as the reference starts them).
+static void border_refine(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H, float mpx,
+                          const LensModel *lm, std::vector<NodeState> &st) {
+  BorderBufs &B = ln.border;
+  float pos[MAX_NODES];
+  const int nx = node_lattice(span, pos), nn = nx * nx, Lw = span + 2 * BORDER;
+  if (B.span != span) {
+    std::vector<short> order, level;
+    B.levels = border_schedule(span, order, level);
+    B.count = static_cast<int>(order.size());
+    std::vector<uint8_t> kinds(static_cast<size_t>(Lw) * Lw), cw(static_cast<size_t>(std::max(fmt_bytes(Lw), 1)), 0);
+    border_kinds(span, cw.data(), kinds.data());  // the word's cells flagged (their bits are no template's)
+    B.kinds.reserve(kinds.size());
+    B.sched.reserve((order.size() + level.size()) * sizeof(short));
+    hip_check(hipMemcpy(B.kinds.p, kinds.data(), kinds.size(), hipMemcpyHostToDevice), "border layout");
+    hip_check(hipMemcpy(B.sched.p, order.data(), order.size() * sizeof(short), hipMemcpyHostToDevice), "border order");
+    hip_check(hipMemcpy(B.sched.as<short>() + order.size(), level.data(), level.size() * sizeof(short),
+                        hipMemcpyHostToDevice), "border levels");
+    B.span = span;
+  }
+  const uint8_t *rim = d_img;
+  if (mpx / 2 >= 1) {  // the templates read the border band under a box a module wide (acquire.c:5014-5019)
+    const int r = static_cast<int>(mpx / 2);
+    int rects[16];
+    border_band(span, H, w, h, r, rects);
+    B.rim.reserve(static_cast<size_t>(w) * h);
+    launch_border_blur(d_img, B.rim.as<uint8_t>(), w, h, r, rects, ln.s);
+    hip_check(hipGetLastError(), "border blur");
+    rim = B.rim.as<uint8_t>();
+  }
+  const size_t job_bytes = (sizeof(BorderJob) + 255) / 256 * 256, node_bytes = nn * sizeof(NodeState);
+  B.job.reserve(job_bytes);
+  B.nodes.reserve(node_bytes);
+  B.h.reserve(job_bytes + node_bytes);
+  BorderJob &J = *B.h.as<BorderJob>();
+  NodeState *hn = reinterpret_cast<NodeState *>(B.h.as<char>() + job_bytes);
+  J = BorderJob{};
+  J.img = rim;
+  J.kinds = B.kinds.as<uint8_t>();
+  J.order = B.sched.as<short>();
+  J.level = B.sched.as<short>() + B.count;
+  J.nodes = B.nodes.as<NodeState>();
+  J.w = w;
+  J.h = h;
+  J.span = span;
+  J.nx = nx;
+  J.Lw = Lw;
+  J.levels = B.levels;
+  std::memcpy(J.H, H, sizeof J.H);
+  std::memcpy(J.pos, pos, sizeof J.pos);
+  J.lens_on = lm ? 1 : 0;
+  if (lm) {
+    J.K = lm->K;
+    for (int a = 0; a < LENS_P; a++) J.t[a] = lm->t[a];
+    J.ax = lm->ax;
+    J.ay = lm->ay;
+    J.R = lm->R;
+  }
+  for (int k = 0; k < nn; k++) hn[k] = NodeState{0, 0, 0, 0};
+  for (int c = 0; c < 4; c++) hn[(c >= 2 ? nx - 1 : 0) * nx + ((c == 1 || c == 2) ? nx - 1 : 0)].done = 1;
+  hip_check(hipMemcpyAsync(B.job.p, &J, sizeof J, hipMemcpyHostToDevice, ln.s), "border job");
+  hip_check(hipMemcpyAsync(B.nodes.p, hn, node_bytes, hipMemcpyHostToDevice, ln.s), "border nodes");
+  launch_border_nodes(B.job.as<BorderJob>(), 1, ln.s);
+  hip_check(hipGetLastError(), "border nodes");
+  hip_check(hipMemcpyAsync(hn, B.nodes.p, node_bytes, hipMemcpyDeviceToHost, ln.s), "border nodes");
+  hip_check(hipStreamSynchronize(ln.s), "border nodes");
+  st.assign(hn, hn + nn);
+}
+
 // The finder's registration (H, and the nodes of its radial lens) taken on from the anchors: anchor_register through
-// H, then the interior from the border nodes and the anchors (anchor_fill). H and nodes are replaced where 12 or more
-// anchors are found and their lens model fits; returns the anchors found then (else 0, the registration unchanged).
-static int anchor_stage(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, float *H, float *nodes) {
+// H; then (mode 2, the reference's chain) the border's marked nodes refined from the anchors' lens model and the
+// interior made from the trusted ones (score 0.35 or more) and the anchors, or (mode 1) the interior made from the
+// finder's own border nodes and the anchors. H and nodes are replaced where 12 or more anchors are found and their lens
+// model fits; returns the anchors found then (else 0, the registration unchanged).
+static int anchor_stage(FinderLane &ln, int mode, int span, const uint8_t *d_img, int w, int h, float *H,
+                        float *nodes) {
   float quad[8], Hn[9];
   const float mpx = lattice_quad(span, H, nodes, quad);
   if (!(mpx > 0)) return 0;
   std::memcpy(Hn, H, sizeof Hn);
+  float pos[MAX_NODES];
+  const int nx = node_lattice(span, pos), nn = nx * nx;
+  if (mode == 2) {  // H through the lattice's corners, as the reference's chain holds them (displacement 0 there)
+    float src[8];
+    for (int c = 0; c < 4; c++) {
+      src[2 * c] = pos[(c == 1 || c == 2) ? nx - 1 : 0];
+      src[2 * c + 1] = pos[c >= 2 ? nx - 1 : 0];
+    }
+    if (!homography4(src, quad, Hn)) return 0;
+  }
   AnchorHit A[ANCH_N];
   LensModel lm;
   int moved = 0;
@@ -1767,9 +1876,17 @@ static int anchor_stage(FinderLane &ln, int span, const uint8_t *d_img, int w, i
     return lane_anchor_find(ln, span, d_img, w, h, Hs, m, an);
   };
   if (!anchor_register(span, Hn, w, h, mpx, quad, find, A, &lm, &moved)) return 0;
-  float pos[MAX_NODES];
-  const int nx = node_lattice(span, pos);
-  if (moved) {  // the border nodes about the new H: the same pixels
+  std::vector<uint8_t> use;
+  if (mode == 2) {
+    std::vector<NodeState> st;
+    border_refine(ln, span, d_img, w, h, Hn, mpx, &lm, st);
+    use.assign(nn, 0);
+    for (int k = 0; k < nn; k++) {
+      nodes[2 * k] = st[k].dx;
+      nodes[2 * k + 1] = st[k].dy;
+      use[k] = st[k].done && st[k].score >= 0.35f;
+    }
+  } else if (moved) {  // the border nodes about the new H: the same pixels
     for (int j = 0; j < nx; j++)
       for (int i = 0; i < nx; i++) {
         if (i && j && i < nx - 1 && j < nx - 1) continue;
@@ -1787,7 +1904,7 @@ static int anchor_stage(FinderLane &ln, int span, const uint8_t *d_img, int w, i
     found += A[k].ok ? 1 : 0;
   }
   const float nan = __builtin_nanf("");
-  anchor_fill(span, Hn, w, h, nan, nan, nodes, nullptr, R, mpx, nullptr);
+  anchor_fill(span, Hn, w, h, nan, nan, nodes, use.empty() ? nullptr : use.data(), R, mpx, nullptr);
   std::memcpy(H, Hn, sizeof Hn);
   return found;
 }
@@ -1817,7 +1934,8 @@ static bool find_registration(liz_decoder *d, const uint8_t *img, int w, int h, 
   for (int i = 0; i < 8; i++) params[i] = reg.H[i];
   params[8] = static_cast<float>(fr.k1);
   reg_nodes(m, params, nodes);
-  if (d->use_anchors) anchor_stage(d->finder.main, fr.span, d->finder.img.as<uint8_t>(), w, h, reg.H, nodes.data());
+  if (d->anchor_mode)
+    anchor_stage(d->finder.main, d->anchor_mode, fr.span, d->finder.img.as<uint8_t>(), w, h, reg.H, nodes.data());
   reg.nodes = nodes.data();
   return true;
 }
@@ -1874,7 +1992,8 @@ LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *im
       params[8] = static_cast<float>(fr.k1);
       std::vector<float> nv;
       reg_nodes(m, params, nv);
-      if (d->use_anchors) anchor_stage(ln, fr.span, d->finder.img.as<uint8_t>() + f * fb, w, h, Hf, nv.data());
+      if (d->anchor_mode)
+        anchor_stage(ln, d->anchor_mode, fr.span, d->finder.img.as<uint8_t>() + f * fb, w, h, Hf, nv.data());
       float *nf = nodes + static_cast<size_t>(f) * MAX_NODES * MAX_NODES * 2;
       std::memcpy(nf, nv.data(), nv.size() * sizeof(float));
       if (!versions) return;

@@ -86,12 +86,21 @@ struct HostBuf {
   }
 };
 
+// A lane's buffers for the border's nodes (border.hpp): the ring's layout and order (kept per ring), the job and the
+// nodes, the capture under the border's blur.
+struct BorderBufs {
+  int span = 0, levels = 0, count = 0;
+  DevBuf kinds, sched, job, nodes, rim;
+  HostBuf h;
+};
+
 // One host thread's probe buffers and stream, so that a frame's stages run without waiting on the other frames.
 struct FinderLane {
   hipStream_t s = nullptr;
   bool own = false;  // the stream was made for this lane
   DevBuf probes, edges, anch;         // anch: the anchors' search job and records
   HostBuf h_probes, h_edges, h_peaks, h_anch;
+  BorderBufs border;
   FinderLane() = default;
   FinderLane(const FinderLane &) = delete;
   FinderLane &operator=(const FinderLane &) = delete;

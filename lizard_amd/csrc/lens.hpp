@@ -1,8 +1,11 @@
-// The projective map and the lens model's rows, for host and device alike, in the reference's order of float and
-// double operations (liblizard/src/internal.h:107-112 project, liblizard/src/acquire.c:1506-1512 lens_rows,
-// :1615-1620 lensm_at), so both sides give the reference's values to the bit (every unit is built without contraction).
+// The projective map, the lens model's rows and the capture's bilinear read, for host and device alike, in the
+// reference's order of float and double operations (liblizard/src/internal.h:107-112 project, :11-23 sample,
+// liblizard/src/acquire.c:1506-1512 lens_rows, :1615-1620 lensm_at), so both sides give the reference's values to the
+// bit (every unit is built without contraction).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
 
 namespace lz {
 
@@ -38,6 +41,22 @@ __host__ __device__ inline void lens_disp(int K, const double *t, double ax, dou
   }
   *dx = static_cast<float>(fx);
   *dy = static_cast<float>(fy);
+}
+
+// A point of a grey capture (rows w bytes apart) read bilinearly through lut (gamma 1: i / 255): the half pixel, the
+// clamp to the frame, the blend.
+__host__ __device__ inline float sample_px(const uint8_t *px, int w, int h, const float *lut, float x, float y) {
+  x -= 0.5f;
+  y -= 0.5f;
+  if (x < 0) x = 0;
+  else if (x > w - 1.001f) x = w - 1.001f;
+  if (y < 0) y = 0;
+  else if (y > h - 1.001f) y = h - 1.001f;
+  const int xi = x >= 0 ? static_cast<int>(x) : 0, yi = y >= 0 ? static_cast<int>(y) : 0;
+  const float fx = x - xi, fy = y - yi;
+  const uint8_t *p = px + static_cast<size_t>(yi) * w + xi;
+  const float a = lut[p[0]], b = lut[p[1]], c = lut[p[w]], d = lut[p[w + 1]];
+  return (a + (b - a) * fx) * (1 - fy) + (c + (d - c) * fx) * fy;
 }
 
 }  // namespace lz

@@ -532,7 +532,8 @@ bool solve8(double A[8][9]) {
   return true;
 }
 
-// The homography through four point pairs, module -> pixel, h8 = 1 (acquire.c:643-654).
+}  // namespace
+
 bool homography4(const float *src, const float *dst, float *H) {
   double A[8][9];
   for (int i = 0; i < 4; i++) {
@@ -546,6 +547,8 @@ bool homography4(const float *src, const float *dst, float *H) {
   H[8] = 1;
   return true;
 }
+
+namespace {
 
 // A homography through the anchors found, least squares with h8 = 1 on coordinates centred and scaled; false where
 // fewer than eight are found or they span fewer than three rows or columns (acquire.c:2061-2096).

@@ -69,6 +69,9 @@ struct AnchorReading {
 int anchor_fill(int span, const float *H, int w, int h, float axis_x, float axis_y, float *nodes, const uint8_t *use,
                 const AnchorReading *anchors, float mpx, LensModel *out);
 
+// The homography through four point pairs (src modules -> dst pixels), h8 = 1 (acquire.c:643-654, its scalar form).
+bool homography4(const float *src, const float *dst, float *H);
+
 // The anchors' search through H and, when given, a lens model about it: every anchor of A not yet found searched
 // (anchor.hpp); returns how many are found in all.
 using AnchorFind = std::function<int(const float *H, const LensModel *lm, AnchorHit *A)>;

@@ -156,3 +156,37 @@ int ref_anchor_register(const void *p, const unsigned char *img, int iw, int ih,
   model_out(&m, model);
   return have;
 }
+
+/* acquire_core's border (acquire.c:5014-5058): the capture's border band blurred where mpx / 2 >= 1 (border_blur),
+   then every marked node refined ring by ring outward from the corners, rows within a ring (refine_node), through H
+   and the lens model (NULL: none), the corners held done at 0. nodes: nx * ny x 4 floats (dx, dy, score, done). */
+int ref_refine_border(const void *p, const unsigned char *img, int iw, int ih, const float *H, float mpx,
+                      const double *model, float *out) {
+  const lizard_t *f = p;
+  const ob_layout_t *L = &f->frame;
+  image_t im;
+  ob_image_init(&im, img, iw, ih, 1.0f);
+  image_t rim = im;
+  homo_t Hh;
+  memcpy(Hh.h, H, sizeof Hh.h);
+  uint8_t *blurred = NULL;
+  if (mpx / 2 >= 1 && (blurred = border_blur(L, &Hh, img, iw, ih, (int)(mpx / 2), NULL))) rim.px = blurred;
+  lensm_t lm;
+  if (model) model_in(model, &lm);
+  const int nx = L->nx, ny = L->ny;
+  node_t *nodes = calloc((size_t)nx * ny, sizeof(node_t));
+  for (int c = 0; c < 4; c++) nodes[((c >= 2) ? ny - 1 : 0) * nx + ((c == 1 || c == 2) ? nx - 1 : 0)].done = 1;
+  for (int ring = 0; ring <= nx + ny; ring++)
+    for (int j = 0; j < ny; j++)
+      for (int i = 0; i < nx; i++) {
+        const int ri = (i < nx - 1 - i ? i : nx - 1 - i) + (j < ny - 1 - j ? j : ny - 1 - j);
+        if (ri != ring || !L->node_mark[j * nx + i] || nodes[j * nx + i].done) continue;
+        refine_node(L, &rim, &Hh, i, j, nodes, model ? &lm : 0);
+      }
+  for (int k = 0; k < nx * ny; k++) {
+    out[4 * k] = nodes[k].dx; out[4 * k + 1] = nodes[k].dy; out[4 * k + 2] = nodes[k].score; out[4 * k + 3] = (float)nodes[k].done;
+  }
+  free(nodes);
+  free(blurred);
+  return nx;
+}
