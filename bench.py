@@ -23,7 +23,8 @@ decode from host buffers ("host_buffers": the PCIe copies included). More worklo
                picture plus AWGN at 20 dB SNR: erase, detrend, 2D FFT, pilot shift, soft demap as one HIP graph;
                algorithmic HBM GB/s against the 8 TB/s peak; the reference's finish path on the same grids.
   "refine"     256 captured frames of LIZARD-320 whose homography corners are up to 1.5 px off: block-fail before and
-               after 10 Gauss-Newton steps over the homography (the hand-written adjoint of the warp, FP64 8x8 solve).
+               after 10 Gauss-Newton steps over the homography (the hand-written adjoint of the warp, FP64 8x8 solve);
+               and the finder's own anchor registration of the same captures, before and after the same steps.
   "small"      one n = 384 LIZARD-80 frame in a 1280x720 capture through liz_decode, against the reference's decoder
                on one core.
   "sender"     the sender's pictures for the FFT+demap mix on the GPU (lizx_modulate): HBM roofline of its two passes,
@@ -575,8 +576,9 @@ def cpu_sender(work, seconds):
 
 
 def refine_frames(count, seed):
-    """count captures of LIZARD-320 (n = 768) in the 128 ring through random cameras (sender and channel on the GPU),
-    each with a registration whose homography corners are up to 1.5 px off (the lens term known)."""
+    """count captures of LIZARD-320 (n = 768) in the 128 ring, the whole symbol as a page shows it (border, word,
+    anchors) through random cameras (sender and channel on the GPU), each with a registration whose homography corners
+    are up to 1.5 px off (the lens term known)."""
     from lizard_amd import sim
     rng = np.random.default_rng(seed)
     dec = lizard_amd.Decoder(0)
@@ -587,7 +589,8 @@ def refine_frames(count, seed):
         cam = sim.Camera.random(rng, scale=(0.75, 0.9))
         blocks = sim.random_blocks(20, rng)
         dpic = dec.modulate(20, blocks)
-        dimg, _, _ = sim.capture_gpu(dec, 20, 256, dpic, cam, seed=seed + k)
+        dimg = dec.capture(20, 256, dpic, cam.W, cam.Hh, cam.H(256), k1=cam.k1, blur=cam.blur, noise=cam.noise,
+                           seed=seed + k, fps=30)
         img = dimg.get(np.uint8, cam.W * cam.Hh).reshape(cam.Hh, cam.W)
         dimg.free()
         dpic.free()
@@ -980,11 +983,15 @@ def bench_refine(args):
     dec = lizard_amd.Decoder(0)
     keep = []
 
-    def decode(Hs):
+    def decode(Hs, node_list=None):
         items = []
-        for fr, H in zip(frames, Hs):
-            p = lizard_amd.reg_params(fr["H0"], fr["k1"])
-            nodes = lizard_amd.reg_nodes(256, fr["H0"].astype(np.float32), fr["img"].shape[1], fr["img"].shape[0], p)
+        for k, (fr, H) in enumerate(zip(frames, Hs)):
+            if node_list is None:
+                p = lizard_amd.reg_params(fr["H0"], fr["k1"])
+                nodes = lizard_amd.reg_nodes(256, fr["H0"].astype(np.float32), fr["img"].shape[1], fr["img"].shape[0],
+                                             p)
+            else:
+                nodes = node_list[k]
             d_img = fr.get("d_img") or D.of(fr["img"])
             fr["d_img"] = d_img
             d_nodes = D.of(nodes)
@@ -1017,6 +1024,40 @@ def bench_refine(args):
     right0 = (before["ok"] > 0) & np.all(before["blocks"] == sent, axis=1)
     right1 = (after["ok"] > 0) & np.all(after["blocks"] == sent, axis=1)
     tr = np.array(traces)
+    # the plain anchor registration (the finder on each capture, from the anchors outward: over its own border nodes,
+    # and the reference's chain with the border refined by its templates) and the same Gauss-Newton steps from it, the
+    # nodes held, on the re-modulation of what it verified
+    anchor_legs = {}
+    for mode, name in ((1, "anchor_registration"), (2, "anchor_registration_reference_chain")):
+        dec.anchors(mode)
+        Ha, Na, lost = [], [], 0
+        for fr in frames:
+            f = dec.find(fr["img"])
+            if f is None or f["span"] != 256:
+                lost += 1
+                Ha.append(fr["H0"])
+                Na.append(np.zeros((len(lizard_amd.node_lattice(256)),) * 2 + (2,), np.float32))
+                continue
+            Ha.append(np.asarray(f["H"], np.float64).reshape(3, 3))
+            Na.append(f["nodes"])
+        anch = decode(Ha, Na)
+        ok2 = anch["ok"].reshape(F, B).astype(bool)
+        Hr = []
+        for k, fr in enumerate(frames):
+            ref_pic = sim.remodulate(20, fr["blocks"], ok2[k], 0, fr["blur"])
+            H1, _ = dec.reg_refine_nodes(20, 256, fr["img"], Ha[k], Na[k], ref_pic, steps=10)
+            Hr.append(H1.reshape(3, 3).astype(np.float64))
+        anch_ref = decode(Hr, Na)
+        right2 = (anch["ok"] > 0) & np.all(anch["blocks"] == sent, axis=1)
+        right3 = (anch_ref["ok"] > 0) & np.all(anch_ref["blocks"] == sent, axis=1)
+        anchor_legs[name] = {"block_fail": round(1.0 - float(right2.mean()), 5),
+                             "block_fail_after_refinement": round(1.0 - float(right3.mean()), 5),
+                             "frames_not_found": lost}
+    dec.anchors(1)
+    anchor_legs["note"] = ("the finder's own registration of each capture (border fit, 7 x 7 anchors registered "
+                           "outward, interior filled from the border and the anchors; the reference chain's border "
+                           "from the nodes' templates), then 10 Gauss-Newton steps over H from it with the nodes "
+                           "held, on the re-modulation of the blocks it verified")
     dec.close()
     for d in keep:
         d.free()
@@ -1032,6 +1073,7 @@ def bench_refine(args):
             "frames_improved": int(np.sum(right1.reshape(F, B).sum(1) > right0.reshape(F, B).sum(1))),
             "frames_worse": int(np.sum(right1.reshape(F, B).sum(1) < right0.reshape(F, B).sum(1))),
             "loss_ratio_median": round(float(np.median(tr[:, -1] / tr[:, 0])), 4),
+            "from_anchor_registration": anchor_legs,
             "refine_ms_per_frame": round(refine_s * 1e3 / F, 3),
             "generation_s": round(gen_s, 1)}
 

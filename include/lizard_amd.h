@@ -465,6 +465,10 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
    steps + 1 floats) receives the loss before each step and after the last. */
 LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
                             const float *H0, float *params, const float *ref, int steps, float *trace);
+/* The same Gauss-Newton steps over H (9 floats, h8 = 1, in and out) with a node lattice held as it is (host, nx * nx
+   (dx, dy) pixels: a finder's or the anchors' registration), so a registration of any form is refined. */
+LIZ_API int lizx_reg_refine_nodes(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                                  float *H, const float *nodes, const float *ref, int steps, float *trace);
 
 /* ---- Batches: many frames of mixed formats through one launch sequence (liblizard/include/lizard.h:285-287
    liz_receiver_batch_cap, liblizard/core/rx/receiver.h:77-95 Receiver::push: the reference batches whole frames per

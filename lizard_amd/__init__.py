@@ -133,6 +133,7 @@ def lib():
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
     sig("lizx_reg_loss_grad", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _F, _F, _P, _P)
     sig("lizx_reg_refine", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
+    sig("lizx_reg_refine_nodes", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_warp", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P)
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
@@ -880,6 +881,18 @@ class Decoder:
                                     _p(np.ascontiguousarray(H0, np.float32).ravel()), _p(p),
                                     _p(np.ascontiguousarray(ref, np.float32)), steps, _p(trace)))
         return p, trace
+
+    def reg_refine_nodes(self, version, span, img, H, nodes, ref, steps=10):
+        """Gauss-Newton refinement of H with the node lattice held (lizx_reg_refine_nodes): the new H (9) and the loss
+        trace."""
+        img = np.ascontiguousarray(img, np.uint8)
+        Hn = np.array(H, np.float32).ravel()
+        Hn /= Hn[8]
+        trace = np.zeros(steps + 1, np.float32)
+        check(lib().lizx_reg_refine_nodes(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
+                                          _p(Hn), _p(np.ascontiguousarray(nodes, np.float32)),
+                                          _p(np.ascontiguousarray(ref, np.float32)), steps, _p(trace)))
+        return Hn, trace
 
     def finish_grid(self, version, span, grid, soft=True):
         """lizard_finish with a supplied n x n grid (row y, column x) of LIZARD-16V in the ring of this span."""

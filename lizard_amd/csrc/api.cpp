@@ -1974,7 +1974,7 @@ LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *im
     LdpcEngine &e = *d->eng;
     hip_check(hipSetDevice(e.device), "hipSetDevice");
     const size_t fb = static_cast<size_t>(w) * h;
-    d->finder.img.reserve(fb * frames);
+    d->finder.img.reserve(fb * finder_slots(d->finder, frames));
     std::vector<FindResult> res(frames);
     // each frame's registration (and word) on the thread that found it, while the other frames are still searched
     auto done = [&](int f, FinderLane &ln) {
@@ -1993,7 +1993,7 @@ LIZ_API int lizx_find_batch(liz_decoder *d, int frames, const uint8_t *const *im
       std::vector<float> nv;
       reg_nodes(m, params, nv);
       if (d->anchor_mode)
-        anchor_stage(ln, d->anchor_mode, fr.span, d->finder.img.as<uint8_t>() + f * fb, w, h, Hf, nv.data());
+        anchor_stage(ln, d->anchor_mode, fr.span, ln.d_img, w, h, Hf, nv.data());
       float *nf = nodes + static_cast<size_t>(f) * MAX_NODES * MAX_NODES * 2;
       std::memcpy(nf, nv.data(), nv.size() * sizeof(float));
       if (!versions) return;
@@ -2163,6 +2163,31 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
   });
 }
 
+// steps Gauss-Newton steps over H (h0..h7 in and out) on a staged refinement's warp; trace as lizx_reg_refine's.
+static void gauss_newton(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, float *H8, int steps, float *trace) {
+  LdpcEngine &e = *d->eng;
+  RegState &st = d->reg_state_host;
+  for (int k = 0; k < 8; k++) st.H[k] = H8[k];
+  st.H[8] = 1.0f;
+  st.alpha = 1.0f;
+  st.beta = 0.0f;
+  hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
+  d->reg_trace.reserve((steps + 2) * sizeof(float));
+  // pass 0 fits the intensity model only; passes 1 .. steps move H; the last pass measures the result
+  for (int s = 0; s <= steps + 1; s++) {
+    launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
+    launch_gn_step(p, d->reg_part.as<float>(), d->reg_sum.as<double>(), d->reg_state.as<RegState>(), 1e-6f,
+                   s >= 1 && s <= steps, d->reg_trace.as<float>(), s, e.stream);
+  }
+  hip_check(hipMemcpyAsync(&st, d->reg_state.p, sizeof st, hipMemcpyDeviceToHost, e.stream), "state");
+  std::vector<float> tr(steps + 2);
+  hip_check(hipMemcpyAsync(tr.data(), d->reg_trace.p, tr.size() * sizeof(float), hipMemcpyDeviceToHost, e.stream),
+            "trace");
+  hip_check(hipStreamSynchronize(e.stream), "refinement");
+  for (int k = 0; k < 8; k++) H8[k] = st.H[k];
+  if (trace) std::copy(tr.begin() + 1, tr.end(), trace);  // the loss before each step and after the last
+}
+
 LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
                             const float *H0, float *params, const float *ref, int steps, float *trace) {
   return guard([&] {
@@ -2172,26 +2197,33 @@ LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t
     const FramePlan &p = d->plan(version, span);
     RegModel model;
     const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
-    RegState &st = d->reg_state_host;
-    for (int k = 0; k < 8; k++) st.H[k] = params[k];
-    st.H[8] = 1.0f;
-    st.alpha = 1.0f;
-    st.beta = 0.0f;
-    hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
-    d->reg_trace.reserve((steps + 2) * sizeof(float));
-    // pass 0 fits the intensity model only; passes 1 .. steps move H; the last pass measures the result
-    for (int s = 0; s <= steps + 1; s++) {
-      launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
-      launch_gn_step(p, d->reg_part.as<float>(), d->reg_sum.as<double>(), d->reg_state.as<RegState>(), 1e-6f,
-                     s >= 1 && s <= steps, d->reg_trace.as<float>(), s, e.stream);
-    }
-    hip_check(hipMemcpyAsync(&st, d->reg_state.p, sizeof st, hipMemcpyDeviceToHost, e.stream), "state");
-    std::vector<float> tr(steps + 2);
-    hip_check(hipMemcpyAsync(tr.data(), d->reg_trace.p, tr.size() * sizeof(float), hipMemcpyDeviceToHost, e.stream),
-              "trace");
-    hip_check(hipStreamSynchronize(e.stream), "refinement");
-    for (int k = 0; k < 8; k++) params[k] = st.H[k];
-    if (trace) std::copy(tr.begin() + 1, tr.end(), trace);  // the loss before each step and after the last
+    gauss_newton(d, p, wf, params, steps, trace);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_reg_refine_nodes(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                                  float *H, const float *nodes, const float *ref, int steps, float *trace) {
+  return guard([&] {
+    need(d && H && nodes && ref && steps >= 0 && steps <= 100,
+         "a decoder, a homography, nodes, a reference picture and 0 to 100 steps");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    need(std::fabs(H[8] - 1.0f) < 1e-6f, "a homography with h8 = 1");
+    lizx_registration reg{};
+    std::memcpy(reg.H, H, sizeof reg.H);
+    reg.nodes = nodes;
+    const WarpFrame *wf = stage_capture(d, p, img, w, h, stride, &reg);
+    const size_t nn = static_cast<size_t>(p.n) * p.n;
+    const size_t cells = static_cast<size_t>(p.dev.nx - 1) * (p.dev.nx - 1) * REG_SPLIT;
+    d->reg_ref.reserve(nn * sizeof(float));
+    hip_check(hipMemcpyAsync(d->reg_ref.p, ref, nn * sizeof(float), hipMemcpyHostToDevice, e.stream),
+              "reference picture");
+    d->reg_part.reserve(cells * REG_PART * sizeof(float));
+    d->reg_sum.reserve(REG_PART * sizeof(double));
+    d->reg_state.reserve(sizeof(RegState));
+    gauss_newton(d, p, wf, H, steps, trace);
     return 0;
   });
 }

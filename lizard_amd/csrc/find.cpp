@@ -853,19 +853,36 @@ void search_frame(FinderBufs &b, FinderLane &ln, FindJob &j, const uint8_t *d_im
   run_frame(j, ln, d_tab);
 }
 
-// The search of frames whose device images are in place (upload false) or are uploaded here on their lanes.
+// The pool's threads and their lanes, made once.
+void ensure_pool(FinderBufs &b) {
+  if (b.pool) return;
+  const int nt = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+  b.pool = std::make_shared<FinderPool>(nt);
+  for (int t = 0; t < nt; t++) {
+    auto ln = std::make_unique<FinderLane>();
+    hip_check(hipStreamCreateWithFlags(&ln->s, hipStreamNonBlocking), "finder lane stream");
+    ln->own = true;
+    lane_reserve(*ln);
+    b.lanes.push_back(std::move(ln));
+  }
+}
+
+// The search of frames whose device images are in place (upload false: one frame, at d_imgs) or are uploaded here,
+// each into its lane's slot of d_imgs, on their lanes. The Hough stage's buffers are a lane's too: the frames a lane
+// searches one after another reuse them.
 void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
             hipStream_t s, FindResult *res, bool upload, const std::function<void(int, FinderLane &)> &done) {
   const float up = res[0].ms[0];
   for (int f = 0; f < F; f++) res[f] = FindResult{};
   res[0].ms[0] = up;
+  const int slots = finder_slots(b, F);
   size_t cells = 0;
-  const int nr = hough_setup(b, F, w, h, s, &cells);
+  const int nr = hough_setup(b, slots, w, h, s, &cells);
   const size_t frame_bytes = static_cast<size_t>(w) * h;
-  std::vector<const uint8_t *> tab(F);
-  for (int f = 0; f < F; f++) tab[f] = d_imgs + f * frame_bytes;
-  b.imgtab.reserve(F * sizeof(void *));
-  hip_check(hipMemcpyAsync(b.imgtab.p, tab.data(), F * sizeof(void *), hipMemcpyHostToDevice, s), "image table");
+  std::vector<const uint8_t *> tab(slots);
+  for (int t = 0; t < slots; t++) tab[t] = d_imgs + t * frame_bytes;
+  b.imgtab.reserve(slots * sizeof(void *));
+  hip_check(hipMemcpyAsync(b.imgtab.p, tab.data(), slots * sizeof(void *), hipMemcpyHostToDevice, s), "image table");
   hip_check(hipStreamSynchronize(s), "image table");
   const uint8_t *const *d_tab = b.imgtab.as<const uint8_t *>();
   std::vector<FindJob> jobs(F);
@@ -875,13 +892,14 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
     j.w = w;
     j.h = h;
     j.stride = stride;
-    j.frame = f;
     j.cx = 0.5 * w;
     j.cy = 0.5 * h;
     j.r = &res[f];
   }
-  auto one = [&](int f, FinderLane &ln) {
-    uint8_t *dst = d_imgs + f * frame_bytes;
+  auto one = [&](int f, FinderLane &ln, int slot) {
+    uint8_t *dst = d_imgs + slot * frame_bytes;
+    jobs[f].frame = slot;  // the lane's Hough buffers, image and probes' image index
+    ln.d_img = dst;
     if (upload) {
       const auto t0 = std::chrono::steady_clock::now();
       if (stride == w) {
@@ -898,28 +916,17 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
   if (F == 1) {
     b.main.s = s;
     lane_reserve(b.main);
-    one(0, b.main);
+    one(0, b.main, 0);
     return;
   }
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
-  if (!b.pool) {
-    const int nt = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
-    b.pool = std::make_shared<FinderPool>(nt);
-    for (int t = 0; t < nt; t++) {
-      auto ln = std::make_unique<FinderLane>();
-      hip_check(hipStreamCreateWithFlags(&ln->s, hipStreamNonBlocking), "finder lane stream");
-      ln->own = true;
-      lane_reserve(*ln);
-      b.lanes.push_back(std::move(ln));
-    }
-  }
   std::mutex em;
   std::exception_ptr err;  // the first frame's failure, raised here on the caller's thread
   b.pool->run(F, [&](int f, int t) {
     try {
       hip_check(hipSetDevice(dev), "hipSetDevice");
-      one(f, *b.lanes[t]);
+      one(f, *b.lanes[t], t);
     } catch (...) {
       std::lock_guard<std::mutex> lk(em);
       if (!err) err = std::current_exception();
@@ -929,6 +936,12 @@ void search(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, i
 }
 
 }  // namespace
+
+int finder_slots(FinderBufs &b, int F) {
+  if (F <= 1) return 1;
+  ensure_pool(b);
+  return static_cast<int>(b.lanes.size());
+}
 
 void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,
                   hipStream_t s, FindResult *res, const std::function<void(int, FinderLane &)> &done) {

@@ -98,6 +98,7 @@ struct BorderBufs {
 struct FinderLane {
   hipStream_t s = nullptr;
   bool own = false;  // the stream was made for this lane
+  const uint8_t *d_img = nullptr;     // the device image of the frame the lane searches now
   DevBuf probes, edges, anch;         // anch: the anchors' search job and records
   HostBuf h_probes, h_edges, h_peaks, h_anch;
   BorderBufs border;
@@ -121,9 +122,11 @@ struct FinderBufs {
 // stride bytes apart) for the host's reads.
 void find_symbol(FinderBufs &b, const uint8_t *d_img, const uint8_t *img, int w, int h, int stride, hipStream_t s,
                  FindResult &r);
-// The same for F frames of one size at once (d_imgs: room for F device images, rows w apart, each w h bytes apart in
-// one buffer; imgs: the host frames, rows stride apart), each frame uploaded into its place and searched on a host
-// thread of its own with a stream of its own. Each frame's result is the one find_symbol gives it. done (may be empty)
+// The device images find_symbols needs room for with F frames: one a lane of the pool (made here if not yet), or 1.
+int finder_slots(FinderBufs &b, int F);
+// The same for F frames of one size at once (d_imgs: room for finder_slots(F) device images, rows w apart, each w h
+// bytes apart in one buffer; imgs: the host frames, rows stride apart), each frame uploaded into its lane's slot and
+// searched on a host thread of its own with a stream of its own. Each frame's result is the one find_symbol gives it. done (may be empty)
 // is called with each frame's index and the lane that searched it on that lane's thread, as soon as its result is in
 // place.
 void find_symbols(FinderBufs &b, int F, uint8_t *d_imgs, const uint8_t *const *imgs, int w, int h, int stride,

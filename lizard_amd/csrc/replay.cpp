@@ -304,25 +304,40 @@ ReplayReader::ReplayReader(const std::string &path) {
   JsonParser jp{meta_.data(), meta_.data() + meta_.size()};
   const Json m = jp.value();
   if (m.kind != Json::OBJ) throw Error(LIZ_E_FORMAT, "meta.json is not an object");
-  const Json *fr = m.get("frames");
-  if (!fr || fr->kind != Json::NUM || fr->num < 0 || fr->num > 65535 || fr->num != std::floor(fr->num))
-    throw Error(LIZ_E_FORMAT, "meta.json has no frame count");
-  const size_t nf = static_cast<size_t>(fr->num);
-  const Json *W = m.get("w"), *H = m.get("h"), *sizes = m.get("sizes"), *ms = m.get("ms"), *lay = m.get("layout");
+  // the count and sizes as the reference's reader takes them (liblizard/core/rx/lizfile.cpp:200-212 framesOf): the
+  // count from "frames", else the number of "sizes"; a frame's size from "sizes", else "w" and "h"; each a whole
+  // number from 1 to 16384 (checked before any conversion)
+  const auto whole = [](const Json *j, double lo, double hi, double *out) {
+    if (!j || j->kind != Json::NUM || !(j->num >= lo && j->num <= hi) || j->num != std::floor(j->num)) return false;
+    *out = j->num;
+    return true;
+  };
+  const Json *fr = m.get("frames"), *W = m.get("w"), *H = m.get("h"), *sizes = m.get("sizes"), *ms = m.get("ms"),
+             *lay = m.get("layout");
+  const bool have_sizes = sizes && sizes->kind == Json::ARR;
+  double nfd = 0;
+  if (fr) {
+    if (!whole(fr, 0, 65535, &nfd)) throw Error(LIZ_E_FORMAT, "meta.json's frame count is not a count of 0 to 65535");
+  } else if (have_sizes) {
+    nfd = static_cast<double>(sizes->items.size());
+    if (nfd > 65535) throw Error(LIZ_E_FORMAT, "meta.json lists more than 65535 frames");
+  } else {
+    throw Error(LIZ_E_FORMAT, "meta.json has no frame count and no sizes");
+  }
+  const size_t nf = static_cast<size_t>(nfd);
   if (lay && lay->kind == Json::STR) layout_ = lay->str;
   for (size_t i = 0; i < nf; i++) {
     Frame fi;
-    const Json *sz = sizes && sizes->kind == Json::ARR && i < sizes->items.size() ? &sizes->items[i] : nullptr;
-    if (sz && sz->kind == Json::ARR && sz->items.size() == 2) {
-      fi.w = static_cast<uint32_t>(sz->items[0].num);
-      fi.h = static_cast<uint32_t>(sz->items[1].num);
-    } else if (W && H && W->kind == Json::NUM && H->kind == Json::NUM) {
-      fi.w = static_cast<uint32_t>(W->num);
-      fi.h = static_cast<uint32_t>(H->num);
-    } else {
-      throw Error(LIZ_E_FORMAT, "meta.json gives no size for frame " + std::to_string(i));
-    }
-    if (fi.w < 1 || fi.h < 1 || fi.w > 16384 || fi.h > 16384) throw Error(LIZ_E_FORMAT, "a frame size out of range");
+    const Json *sz = have_sizes && i < sizes->items.size() ? &sizes->items[i] : nullptr;
+    double fw = 0, fh = 0;
+    bool got = false;
+    if (sz && sz->kind == Json::ARR && sz->items.size() == 2)
+      got = whole(&sz->items[0], 1, 16384, &fw) && whole(&sz->items[1], 1, 16384, &fh);
+    else
+      got = whole(W, 1, 16384, &fw) && whole(H, 1, 16384, &fh);
+    if (!got) throw Error(LIZ_E_FORMAT, "meta.json gives frame " + std::to_string(i) + " no size of 1 to 16384");
+    fi.w = static_cast<uint32_t>(fw);
+    fi.h = static_cast<uint32_t>(fh);
     fi.ms = ms && ms->kind == Json::ARR && i < ms->items.size() && ms->items[i].kind == Json::NUM ? ms->items[i].num
                                                                                                    : std::nan("");
     const Entry *e = find(frame_name(i));

@@ -136,6 +136,33 @@ def test_refusals(tmp_path):
         L.Replay(str(tmp_path / "missing.lizard"))
 
 
+def test_frame_count_from_sizes_and_sizes_range_checked(tmp_path):
+    """As the reference's reader takes meta.json (liblizard/core/rx/lizfile.cpp:200-212 framesOf): with no "frames" the
+    count is the number of "sizes"; a size that is negative, fractional, zero or past 16384 is refused (it is checked
+    before any conversion)."""
+    rng = np.random.default_rng(4)
+    frames = [rng.integers(0, 256, (20, 30)).astype(np.uint8), rng.integers(0, 256, (11, 17)).astype(np.uint8)]
+
+    def made(name, meta):
+        p = str(tmp_path / name)
+        w = L.ReplayWriter(p)
+        for f in frames:
+            w.add(f)
+        w.finish(json.dumps(meta))
+        return p
+
+    r = L.Replay(made("nocount.lizard", {"sizes": [[30, 20], [17, 11]]}))
+    assert len(r) == 2 and r.info(1)[:2] == (17, 11)
+    assert np.array_equal(r.frame(1), frames[1])
+    r.close()
+    for k, meta in enumerate([{"sizes": [[30, 20], [-17, 11]]}, {"sizes": [[30, 20], [17.5, 11]]},
+                              {"frames": 2, "w": 30, "h": 0}, {"frames": 2, "w": 4294967326.0, "h": 20},
+                              {"frames": -1, "w": 30, "h": 20}, {"frames": 1.5, "w": 30, "h": 20},
+                              {"w": 30, "h": 20}]):
+        with pytest.raises(L.LizError):
+            L.Replay(made("bad%d.lizard" % k, meta))
+
+
 @pytest.mark.gpu
 def test_replay_of_painted_captures_decodes_blind(tmp_path, decoder_gpu):
     """Captures of symbols painted on the GPU (lizx_capture_symbol) packed into a replay, read back frame by frame and
