@@ -120,6 +120,22 @@ def measured_traffic():
     return None
 
 
+def profile_values(name):
+    """The "key: value" lines of a committed profile summary (profiles/name) as floats; {} where it is absent."""
+    out = {}
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                try:
+                    out[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    except OSError:
+        pass
+    return out
+
+
 def measured_issue():
     """k_ldpc's vector-instruction issue over the timed launches from the committed rocprofv3 SQ counter summary:
     {"valu_insts_per_codeword_iteration", "valu_busy"}; None where that summary is absent."""
@@ -315,6 +331,22 @@ def bench_pipeline(args, caps):
                                "ldpc_wg_per_cu": shape[3] or "max"},
                      "default_shape_gpu_ms_per_batch": round(default_ms, 3)},
     }
+    # the picture stages' HBM traffic and issue at the default shape, from the committed counter summary
+    pc = profile_values("pipeline_counters_32x1536.txt")
+    if pc:
+        stages = {}
+        for key, kernel in (("row_pass", "k_rows<WarpSource, 1536, 4>"), ("column_pass", "k_cols<1536, 4>"),
+                            ("demap", "k_demap<4>")):
+            t, us = pc.get(key + "_traffic_bytes_per_launch"), pc.get(key + "_avg_us")
+            if t and us:
+                gbs = t / (us * 1e-6) / 1e9
+                stages[key] = {"kernel": kernel, "traffic": int(t), "avg_us": us, "achieved": round(gbs, 1),
+                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        if "row_pass_valu_busy" in pc:
+            stages["row_pass"]["valu_busy"] = pc["row_pass_valu_busy"]
+        stages["from"] = ("profiles/pipeline_counters_32x1536.txt: rocprofv3 --kernel-trace and --pmc runs of this leg "
+                          "(FETCH_SIZE, WRITE_SIZE copy-calibrated; SQ counters), copied, not measured by this run")
+        out["stage_rooflines"] = stages
     b.close()
     for d in keep:
         d.free()
@@ -362,7 +394,11 @@ def bench_fft_demap(args):
             "frame_by_frame_ms_per_batch": round(fev, 3),
             "algorithmic_bytes_per_batch": int(total),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": round(gbs / HBM_PEAK_GBS, 4)}}
+                         "frac": round(gbs / HBM_PEAK_GBS, 4),
+                         "traffic": profile_values("fft_demap_traffic_64_mixed.txt").get("traffic_bytes_per_batch"),
+                         "traffic_from": "profiles/fft_demap_traffic_64_mixed.txt: rocprofv3 --pmc runs of this leg "
+                                         "(FETCH_SIZE, WRITE_SIZE, copy-calibrated), copied, not measured by this "
+                                         "run"}}
 
 
 def cpu_fft_demap(grids, seconds):
@@ -1180,6 +1216,8 @@ def main():
     ap.add_argument("--pipe-modes", default="graph,stream,frames,plain",
                     help="pipeline run modes to time (graph always): graph, stream, frames, plain")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuners (default launch shapes)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="with --only: a 1 GiB device copy after the leg, whose counters calibrate a --pmc run")
     ap.add_argument("--op-batches", type=int, default=8,
                     help="4096-codeword batches at the block-fail < 1e-4 operating point")
     ap.add_argument("--only", choices=("ldpc", "pipeline", "fft_demap", "refine", "small", "sender", "finder",
@@ -1200,6 +1238,11 @@ def main():
                else bench_sender(args, lizard_amd.Decoder(0)) if args.only == "sender"
                else bench_finder(args) if args.only == "finder" else bench_blake3(args) if args.only == "blake3"
                else bench_small(args))
+        if args.copy_probe:
+            nb = 1 << 30
+            a, b = lizard_amd.DeviceArray(nb), lizard_amd.DeviceArray(nb)
+            out["copy_probe_ms"] = round(lizard_amd.copy_probe(b, a, nb), 4)
+            a.free(), b.free()
         print(json.dumps({"only": args.only, args.only: out}))
         return
     if args.only == "ldpc":
