@@ -259,6 +259,11 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
    default), or launches them one by one (0). Results are the same bits either way. No counterpart in the reference
    (liblizard's decoder runs on the CPU, api/codec.cpp:121-160). */
 LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on);
+/* The hang bound (liblizard/core/wg/wg.h:48-55 HANG_MS, 4000 ms; LIZ_HANG_MS in the environment sets it at first
+   use, as the reference's does): a handle's wait for its own GPU work that lasts longer takes the work as hung, fails
+   with LIZ_E_STATE, and the handle refuses every later call (free it and make another, as the reference drops a lost
+   device). 0: no bound. Process-wide. Returns 0. */
+LIZ_API int lizx_hang_bound(double ms);
 /* The finder's registration from the anchors outward. mode 0: H and the nodes of the finder's radial lens alone.
    Otherwise, after the border's fit, the 7 x 7 anchors are searched and registered (liblizard/src/acquire.c:2128-2191
    anchor_register, as lizx_anchor_register) and, where 12 or more are found, the lattice's interior is made from the

@@ -10,7 +10,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblizard_amd.so")
+# LIZARD_AMD_LIB: another build of the same library (the host-sanitized one, lizard_amd/_san/)
+LIB_PATH = os.environ.get("LIZARD_AMD_LIB") or os.path.join(HERE, "liblizard_amd.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "lizard_amd.h")
 
 LIZ_OK, LIZ_E_ARG, LIZ_E_NOMEM, LIZ_E_IO, LIZ_E_UNSUPPORTED, LIZ_E_FORMAT, LIZ_E_TOOBIG, LIZ_E_STATE, LIZ_E_INTERNAL = (
@@ -139,6 +140,7 @@ def lib():
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
     sig("lizx_decoder_graphs", _I, _P, _I)
     sig("lizx_finder_anchors", _I, _P, _I)
+    sig("lizx_hang_bound", _I, C.c_double)
     sig("lizx_anchor_find", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P)
     sig("lizx_anchor_register", _I, _P, _I, _P, _I, _I, _I, _P, _F, _P, _P, _P, _P)
     sig("lizx_border_refine", _I, _P, _I, _P, _I, _I, _I, _P, _F, _P, _P)
@@ -260,6 +262,11 @@ def border_fill(span, H, w, h, nodes, use=None, axis=None):
     ax = None if axis is None else np.ascontiguousarray(axis, np.float32)
     K = check(lib().lizx_reg_border_fill(span, _p(H), w, h, _p(ax), _p(nodes), _p(u), _p(model)))
     return nodes, K, model
+
+
+def hang_bound(ms):
+    """The hang bound for every handle's waits on its GPU work (lizx_hang_bound; 0: none)."""
+    check(lib().lizx_hang_bound(float(ms)))
 
 
 def anchor_fill(span, H, w, h, nodes, anchors, mpx, use=None, axis=None):

@@ -2,6 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -39,6 +42,40 @@ int fail(int code, const std::string &message) {
 void clear_error() {
   last_error[0] = '\0';
   last_code = LIZ_OK;
+}
+
+static std::atomic<double> g_hang_ms{-1.0};
+
+double hang_bound_ms() {
+  double ms = g_hang_ms.load();
+  if (ms < 0) {
+    const char *e = std::getenv("LIZ_HANG_MS");
+    ms = e ? std::max(0.0, std::atof(e)) : 4000.0;
+    g_hang_ms.store(ms);
+  }
+  return ms;
+}
+
+void set_hang_bound_ms(double ms) { g_hang_ms.store(ms); }
+
+void wait_bounded(hipStream_t s, const char *what) {
+  const double bound = hang_bound_ms();
+  if (bound <= 0) {
+    hip_check(hipStreamSynchronize(s), what);
+    return;
+  }
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int spins = 0;; spins++) {
+    const hipError_t q = hipStreamQuery(s);
+    if (q == hipSuccess) return;
+    if (q != hipErrorNotReady) hip_check(q, what);
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (ms > bound)
+      throw Error(LIZ_E_STATE, std::string(what) + ": not done within the hang bound of " + std::to_string(bound) +
+                                   " ms, taken as hung; the handle refuses further work");
+    if (spins < 2000) std::this_thread::yield();
+    else std::this_thread::sleep_for(std::chrono::microseconds(50));
+  }
 }
 
 int current_device(int want) {
@@ -668,6 +705,14 @@ static int lane_anchor_find(FinderLane &ln, int span, const uint8_t *d_img, int 
                             const LensModel *lm, AnchorHit *A);
 static void border_refine(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H, float mpx,
                           const LensModel *lm, std::vector<NodeState> &st);
+
+LIZ_API int lizx_hang_bound(double ms) {
+  return guard([&] {
+    need(ms >= 0 && ms <= 3.6e6, "a hang bound of 0 (none) to 3600000 ms");
+    set_hang_bound_ms(ms);
+    return 0;
+  });
+}
 
 LIZ_API int lizx_finder_anchors(liz_decoder *d, int on) {
   return guard([&] {
@@ -1923,7 +1968,7 @@ static bool find_registration(liz_decoder *d, const uint8_t *img, int w, int h, 
   } else {
     hip_check(hipMemcpy2DAsync(d->finder.img.p, w, img, stride, w, h, hipMemcpyHostToDevice, e.stream), "capture");
   }
-  hip_check(hipStreamSynchronize(e.stream), "capture upload");
+  e.wait("capture upload");
   fr.ms[0] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   find_symbol(d->finder, d->finder.img.as<uint8_t>(), img, w, h, stride, e.stream, fr);
   if (!fr.found) return false;
@@ -2140,7 +2185,7 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
     d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_SPLIT * REG_PART);
     hip_check(hipMemcpyAsync(d->reg_part_host.data(), d->reg_part.p, d->reg_part_host.size() * sizeof(float),
                              hipMemcpyDeviceToHost, e.stream), "partial sums");
-    hip_check(hipStreamSynchronize(e.stream), "refinement pass");
+    e.wait("refinement pass");
     // the pieces' sums in FP64, in piece order; the nodes' gradient from the corners of the cells about them
     double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     std::vector<double> G(static_cast<size_t>(nx) * nx * 2, 0.0);
@@ -2183,7 +2228,7 @@ static void gauss_newton(liz_decoder *d, const FramePlan &p, const WarpFrame *wf
   std::vector<float> tr(steps + 2);
   hip_check(hipMemcpyAsync(tr.data(), d->reg_trace.p, tr.size() * sizeof(float), hipMemcpyDeviceToHost, e.stream),
             "trace");
-  hip_check(hipStreamSynchronize(e.stream), "refinement");
+  e.wait("refinement");
   for (int k = 0; k < 8; k++) H8[k] = st.H[k];
   if (trace) std::copy(tr.begin() + 1, tr.end(), trace);  // the loss before each step and after the last
 }

@@ -168,7 +168,7 @@ LIZ_API int lizx_batch_plan(lizx_batch *b, int frames, const int *version, const
     need(source == LIZX_SRC_CAPTURE || source == LIZX_SRC_GRID, "a source (LIZX_SRC_CAPTURE or LIZX_SRC_GRID)");
     need(stages == LIZX_STAGE_SOFT || stages == LIZX_STAGE_BLOCKS, "stages (LIZX_STAGE_SOFT or LIZX_STAGE_BLOCKS)");
     hip_check(hipSetDevice(b->eng->device), "hipSetDevice");
-    hip_check(hipStreamSynchronize(b->eng->stream), "the previous runs");
+    b->eng->wait("the previous runs");
     b->drop_graph();
     // the batch is unplanned until this call succeeds (frames = 0 rejects every run)
     b->frames = 0;
@@ -309,7 +309,7 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
                               std::memcmp(b->sent_frames.data(), b->host_frames.data(),
                                           b->frames * sizeof(WarpFrame)) == 0);
     if (!same) {
-      hip_check(hipStreamSynchronize(e.stream), "the previous runs");
+      e.wait("the previous runs");
       if (grid) {
         hip_check(hipMemcpy(b->grids.p, b->host_grids.data(), b->frames * sizeof(float *), hipMemcpyHostToDevice),
                   "grids");
@@ -344,7 +344,7 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
     } else {
       for (int f = 0; f < b->frames; f++) {
         b->enqueue_frame(f, mode == LIZX_RUN_PLAIN);
-        hip_check(hipStreamSynchronize(e.stream), "a frame");  // a decoder called once a frame returns here
+        e.wait("a frame");  // a decoder called once a frame returns here
       }
     }
     hip_check(hipEventRecord(e.ev1, e.stream), "hipEventRecord");
@@ -443,7 +443,7 @@ LIZ_API int lizx_batch_blocks(lizx_batch *b, uint8_t *blocks, uint8_t *ok, int8_
     need(b && b->frames > 0 && b->stages == LIZX_STAGE_BLOCKS, "a batch planned with LIZX_STAGE_BLOCKS");
     need(blocks && ok && its && blk_est, "output arrays");
     hip_check(hipSetDevice(b->eng->device), "hipSetDevice");
-    hip_check(hipStreamSynchronize(b->eng->stream), "the runs");
+    b->eng->wait("the runs");
     const int B = b->blocks;
     hip_check(hipMemcpy(its, b->its.p, B, hipMemcpyDeviceToHost), "iterations");
     hip_check(hipMemcpy(blk_est, b->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost), "estimates");
@@ -467,7 +467,7 @@ LIZ_API int lizx_batch_soft(lizx_batch *b, int frame, int8_t *llr, double *sub_e
   return guard([&] {
     need(b && frame >= 0 && frame < b->frames, "a batch and a frame of it");
     hip_check(hipSetDevice(b->eng->device), "hipSetDevice");
-    hip_check(hipStreamSynchronize(b->eng->stream), "the runs");
+    b->eng->wait("the runs");
     const Group &g = *b->groups[b->group_of[frame]];
     const FramePlan &p = *g.plan;
     const SoftBufs v = g.view.at(p, b->slot_of[frame]);

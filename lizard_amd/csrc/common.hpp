@@ -25,6 +25,13 @@ inline void need(bool ok, const char *what) {
   if (!ok) throw Error(LIZ_E_ARG, what);
 }
 
+// The hang bound (liblizard/core/wg/wg.h:48-55 HANG_MS): work not done this many ms after a wait began is taken as
+// hung. LIZ_HANG_MS in the environment sets it (the reference's variable), lizx_hang_bound afterwards; 0: none.
+double hang_bound_ms();
+void set_hang_bound_ms(double ms);
+// Waits for stream s as hipStreamSynchronize does, within the hang bound: LIZ_E_STATE "... taken as hung" past it.
+void wait_bounded(hipStream_t s, const char *what);
+
 inline void hip_check(hipError_t e, const char *what) {
   if (e == hipSuccess) return;
   const int code = e == hipErrorOutOfMemory ? LIZ_E_NOMEM

@@ -470,8 +470,18 @@ void LdpcEngine::launch(const LdpcParams &p0, bool plain, bool timed, bool queue
 }
 
 // Waits for the stream; the time from the first timed launch since the last sync to the last one.
+void LdpcEngine::wait(const char *what) {
+  if (lost) throw Error(LIZ_E_STATE, "this handle's earlier work was taken as hung; free it and make another");
+  try {
+    wait_bounded(stream, what);
+  } catch (const Error &e) {
+    if (e.code == LIZ_E_STATE) lost = true;
+    throw;
+  }
+}
+
 void LdpcEngine::sync() {
-  hip_check(hipStreamSynchronize(stream), "decode");
+  wait("decode");
   if (armed) {
     float ms = 0;
     hip_check(hipEventElapsedTime(&ms, ev0, ev1), "hipEventElapsedTime");

@@ -66,6 +66,7 @@ struct LdpcEngine {
   int cus = 0, wg_per_cu = 0;
   float last_ms = 0;
   bool armed = false;
+  bool lost = false;  // a wait on the stream passed the hang bound: the handle refuses further work
   explicit LdpcEngine(int device);
   ~LdpcEngine();
   // Enqueue a decode of count tasks already on the device. Plain: one launch per task. Timed: HIP events bracket the
@@ -76,6 +77,8 @@ struct LdpcEngine {
   // Writes the task list and zeroes the queue counter (d_queue, or null for the engine's own).
   void make_tasks(LdpcTask *d_tasks, const uint8_t *d_rate, const int64_t *d_off, int count, int *d_queue = nullptr);
   void sync();
+  // Waits for the stream within the hang bound (wait_bounded); past it the engine is lost and every later wait throws.
+  void wait(const char *what);
   void ensure_frame_tables(int max_slots);
   // The frame slots (0/1 bytes, total a frame) of frames x blocks x 476 bytes on the device, enqueued on the stream;
   // d_blk: per block first slot, nt, slots, code index (contiguous from slot 0).
@@ -85,7 +88,7 @@ struct LdpcEngine {
 
 float copy_probe(void *dst, const void *src, size_t bytes);
 
-// The device-resident work area of the decode that allocates nothing (lizx_ldpc_decode_device_fixed): one per GPU
+// The device-resident work area of the decode that allocates nothing (lizx_ldpc_decode_device_noalloc): one per GPU
 // for the process's life, shared by every call on that GPU. SCRATCH_TASKS task descriptors, then the queue counter.
 constexpr int SCRATCH_TASKS = 16384;
 struct LdpcScratch {
