@@ -9,6 +9,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -76,6 +77,17 @@ void wait_bounded(hipStream_t s, const char *what) {
     if (spins < 2000) std::this_thread::yield();
     else std::this_thread::sleep_for(std::chrono::microseconds(50));
   }
+}
+
+hipStream_t aux_stream() {
+  static std::mutex m;
+  static hipStream_t streams[64] = {};
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  need(dev >= 0 && dev < 64, "a device index under 64");
+  std::lock_guard<std::mutex> lk(m);
+  if (!streams[dev]) hip_check(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking), "hipStreamCreate");
+  return streams[dev];
 }
 
 int current_device(int want) {
@@ -210,7 +222,7 @@ static LdpcParams finish_params(liz_decoder *d, const Format *f, const int8_t *d
     std::stable_sort(d->host_tasks.begin(), d->host_tasks.end(),
                      [](const LdpcTask &a, const LdpcTask &b) { return a.code > b.code; });
     d->tasks.reserve(B * sizeof(LdpcTask));
-    hip_check(hipMemcpy(d->tasks.p, d->host_tasks.data(), B * sizeof(LdpcTask), hipMemcpyHostToDevice), "tasks");
+    copy_wait(d->tasks.p, d->host_tasks.data(), B * sizeof(LdpcTask), hipMemcpyHostToDevice, e.stream, "tasks");
     d->tasks_version = f->version;
   }
   uint8_t *r = d->res.as<uint8_t>();
@@ -333,13 +345,13 @@ LIZ_API void lizx_dev_free(void *p) {
 }
 LIZ_API int lizx_dev_upload(void *dst, const void *src, size_t bytes) {
   return guard([&] {
-    hip_check(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "upload");
+    copy_wait(dst, src, bytes, hipMemcpyHostToDevice, aux_stream(), "upload");
     return 0;
   });
 }
 LIZ_API int lizx_dev_download(void *dst, const void *src, size_t bytes) {
   return guard([&] {
-    hip_check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "download");
+    copy_wait(dst, src, bytes, hipMemcpyDeviceToHost, aux_stream(), "download");
     return 0;
   });
 }
@@ -652,23 +664,23 @@ LIZ_API int lizx_ldpc_decode(lizx_ldpc *h, int count, const uint8_t *rate, const
     p.flags = (crc ? 1 : 0) | (trace ? 2 : 0);
     e.launch(p, (flags & LIZX_LDPC_PLAIN) != 0, true);
     e.sync();
-    hip_check(hipMemcpy(its, h->its.p, count, hipMemcpyDeviceToHost), "iterations");
+    copy_wait(its, h->its.p, count, hipMemcpyDeviceToHost, h->eng->stream, "iterations");
     if (bits) {
       // only the codewords' own ranges: the caller's buffer between them is left as it was
       std::vector<uint8_t> all(span);
-      hip_check(hipMemcpy(all.data(), h->bits.p, span, hipMemcpyDeviceToHost), "bits");
+      copy_wait(all.data(), h->bits.p, span, hipMemcpyDeviceToHost, h->eng->stream, "bits");
       for (int i = 0; i < count; i++) std::memcpy(bits + off[i], all.data() + off[i], len[i]);
     }
     if (crc) {
-      hip_check(hipMemcpy(bytes, h->bytes.p, static_cast<size_t>(count) * BLOCK_BYTES, hipMemcpyDeviceToHost), "bytes");
-      hip_check(hipMemcpy(crc_ok, h->crc_ok.p, count, hipMemcpyDeviceToHost), "CRC verdicts");
+      copy_wait(bytes, h->bytes.p, static_cast<size_t>(count) * BLOCK_BYTES, hipMemcpyDeviceToHost, h->eng->stream, "bytes");
+      copy_wait(crc_ok, h->crc_ok.p, count, hipMemcpyDeviceToHost, h->eng->stream, "CRC verdicts");
     }
     if (trace) {
       std::vector<int16_t> all(span);
-      hip_check(hipMemcpy(all.data(), h->post.p, span * sizeof(int16_t), hipMemcpyDeviceToHost), "posteriors");
+      copy_wait(all.data(), h->post.p, span * sizeof(int16_t), hipMemcpyDeviceToHost, h->eng->stream, "posteriors");
       for (int i = 0; i < count; i++)
         std::memcpy(post + off[i], all.data() + off[i], len[i] * sizeof(int16_t));
-      hip_check(hipMemcpy(msg, h->msg.p, static_cast<size_t>(count) * MSG_TRACE_STRIDE, hipMemcpyDeviceToHost),
+      copy_wait(msg, h->msg.p, static_cast<size_t>(count) * MSG_TRACE_STRIDE, hipMemcpyDeviceToHost, h->eng->stream,
                 "messages");
     }
     int cleared = 0;
@@ -1034,17 +1046,17 @@ static const WarpFrame *stage_capture(liz_decoder *d, const FramePlan &p, const 
 // The soft values, estimates, rms, coefficients and pilot shift of the last frame to the caller's lizx_soft.
 static void copy_soft(liz_decoder *d, const FramePlan &p, lizx_soft *soft, bool dbg) {
   const size_t S = static_cast<size_t>(p.subch) * SUB;
-  if (soft->llr) hip_check(hipMemcpy(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost), "soft values");
+  if (soft->llr) copy_wait(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost, d->eng->stream, "soft values");
   if (soft->sub_est)
-    hip_check(hipMemcpy(soft->sub_est, d->fb.est.p, p.subch * sizeof(double), hipMemcpyDeviceToHost), "estimates");
+    copy_wait(soft->sub_est, d->fb.est.p, p.subch * sizeof(double), hipMemcpyDeviceToHost, d->eng->stream, "estimates");
   if (soft->sub_rms)
-    hip_check(hipMemcpy(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
+    copy_wait(soft->sub_rms, d->fb.rms.p, p.subch * sizeof(float), hipMemcpyDeviceToHost, d->eng->stream, "rms");
   if (soft->coef && dbg)
-    hip_check(hipMemcpy(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost), "coefficients");
+    copy_wait(soft->coef, d->fb.coef_dbg.p, S * 2 * sizeof(float), hipMemcpyDeviceToHost, d->eng->stream, "coefficients");
   if (soft->soft && dbg)
-    hip_check(hipMemcpy(soft->soft, d->fb.coef_dbg.as<float>() + 2 * S, S * 2 * sizeof(float), hipMemcpyDeviceToHost),
+    copy_wait(soft->soft, d->fb.coef_dbg.as<float>() + 2 * S, S * 2 * sizeof(float), hipMemcpyDeviceToHost, d->eng->stream,
               "soft floats");
-  hip_check(hipMemcpy(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
+  copy_wait(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost, d->eng->stream, "shift");
 }
 
 static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
@@ -1075,7 +1087,7 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
     for (int k = 0; same && k < 8; k++) same = g.keys[k] == keys[k];
     if (!same) {
       d->drop_frame_graph();
-      hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+      hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
       try {
         launch_soft_warp(p, v, wf, 1, e.stream);
         e.launch(lp, false, false);
@@ -1278,7 +1290,7 @@ LIZ_API int lizx_b3_hash_device(liz_decoder *d, const uint8_t *d_data, uint64_t 
     hip_check(hipEventRecord(e.ev1, e.stream), "hipEventRecord");
     e.sync();
     if (kernel_ms) hip_check(hipEventElapsedTime(kernel_ms, e.ev0, e.ev1), "hipEventElapsedTime");
-    hip_check(hipMemcpy(out, d->b3_out.p, 32, hipMemcpyDeviceToHost), "hash");
+    copy_wait(out, d->b3_out.p, 32, hipMemcpyDeviceToHost, e.stream, "hash");
     return 0;
   });
 }
@@ -1484,9 +1496,9 @@ LIZ_API liz_encoder *liz_encoder_new(const liz_format *format) {
     e->w.reserve(w.size() * sizeof(float));
     e->pic.reserve(static_cast<size_t>(g.n) * g.n * sizeof(float));
     e->page.reserve(static_cast<size_t>(g.side) * g.side);
-    hip_check(hipMemcpy(e->kinds.p, kinds.data(), kinds.size(), hipMemcpyHostToDevice), "border map");
-    hip_check(hipMemcpy(e->i0.p, i0.data(), i0.size() * sizeof(int32_t), hipMemcpyHostToDevice), "taps");
-    hip_check(hipMemcpy(e->w.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice), "weights");
+    copy_wait(e->kinds.p, kinds.data(), kinds.size(), hipMemcpyHostToDevice, e->d->eng->stream, "border map");
+    copy_wait(e->i0.p, i0.data(), i0.size() * sizeof(int32_t), hipMemcpyHostToDevice, e->d->eng->stream, "taps");
+    copy_wait(e->w.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice, e->d->eng->stream, "weights");
     PageParams &pp = e->pp;
     pp.n = g.n;
     pp.span = g.span;
@@ -1530,7 +1542,7 @@ LIZ_API int liz_encoder_paint(liz_encoder *e, const uint8_t *blocks, uint32_t pi
       hip_check(hipSetDevice(eng.device), "hipSetDevice");
       launch_paint_page(e->pp, e->pic.as<float>(), e->page.as<uint8_t>(), eng.stream);
       eng.sync();
-      hip_check(hipMemcpy(e->host.data(), e->page.p, e->host.size(), hipMemcpyDeviceToHost), "page");
+      copy_wait(e->host.data(), e->page.p, e->host.size(), hipMemcpyDeviceToHost, eng.stream, "page");
       // each symbol into its place, the gap after it its row's last pixel (the margin's colour)
       const int x0 = k * (g.side + g.gap), fill = k + 1 < e->fmt.codes ? g.gap : 0;
       for (int y = 0; y < g.side; y++) {
@@ -1725,7 +1737,7 @@ LIZ_API int lizx_warp(liz_decoder *d, int version, int span, const uint8_t *img,
     d->grid.reserve(n * n * sizeof(float));
     launch_warp(p, wf, 1, d->grid.as<float>(), e.stream);
     e.sync();
-    hip_check(hipMemcpy(grid, d->grid.p, n * n * sizeof(float), hipMemcpyDeviceToHost), "grid");
+    copy_wait(grid, d->grid.p, n * n * sizeof(float), hipMemcpyDeviceToHost, e.stream, "grid");
     return 0;
   });
 }
@@ -1838,10 +1850,10 @@ static void border_refine(FinderLane &ln, int span, const uint8_t *d_img, int w,
     border_kinds(span, cw.data(), kinds.data());  // the word's cells flagged (their bits are no template's)
     B.kinds.reserve(kinds.size());
     B.sched.reserve((order.size() + level.size()) * sizeof(short));
-    hip_check(hipMemcpy(B.kinds.p, kinds.data(), kinds.size(), hipMemcpyHostToDevice), "border layout");
-    hip_check(hipMemcpy(B.sched.p, order.data(), order.size() * sizeof(short), hipMemcpyHostToDevice), "border order");
-    hip_check(hipMemcpy(B.sched.as<short>() + order.size(), level.data(), level.size() * sizeof(short),
-                        hipMemcpyHostToDevice), "border levels");
+    copy_wait(B.kinds.p, kinds.data(), kinds.size(), hipMemcpyHostToDevice, ln.s, "border layout");
+    copy_wait(B.sched.p, order.data(), order.size() * sizeof(short), hipMemcpyHostToDevice, ln.s, "border order");
+    copy_wait(B.sched.as<short>() + order.size(), level.data(), level.size() * sizeof(short),
+                        hipMemcpyHostToDevice, ln.s, "border levels");
     B.span = span;
   }
   const uint8_t *rim = d_img;

@@ -250,8 +250,8 @@ LIZ_API int lizx_batch_plan(lizx_batch *b, int frames, const int *version, const
       const size_t tb = all.size() * sizeof(LdpcTask);
       b->tasks.reserve(tb);
       b->frame_tasks.reserve(tb);
-      hip_check(hipMemcpy(b->tasks.p, all.data(), tb, hipMemcpyHostToDevice), "tasks");
-      hip_check(hipMemcpy(b->frame_tasks.p, per.data(), tb, hipMemcpyHostToDevice), "tasks");
+      copy_wait(b->tasks.p, all.data(), tb, hipMemcpyHostToDevice, b->eng->stream, "tasks");
+      copy_wait(b->frame_tasks.p, per.data(), tb, hipMemcpyHostToDevice, b->eng->stream, "tasks");
       b->its.reserve(b->blocks);
       b->blk_est.reserve(b->blocks * sizeof(float));
       b->bytes.reserve(static_cast<size_t>(b->blocks) * BLOCK_BYTES);
@@ -311,12 +311,12 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
     if (!same) {
       e.wait("the previous runs");
       if (grid) {
-        hip_check(hipMemcpy(b->grids.p, b->host_grids.data(), b->frames * sizeof(float *), hipMemcpyHostToDevice),
+        copy_wait(b->grids.p, b->host_grids.data(), b->frames * sizeof(float *), hipMemcpyHostToDevice, b->eng->stream,
                   "grids");
         b->sent_grids = b->host_grids;
       } else {
-        hip_check(hipMemcpy(b->wframes.p, b->host_frames.data(), b->frames * sizeof(WarpFrame),
-                            hipMemcpyHostToDevice), "frames");
+        copy_wait(b->wframes.p, b->host_frames.data(), b->frames * sizeof(WarpFrame), hipMemcpyHostToDevice,
+                  b->eng->stream, "frames");
         b->sent_frames = b->host_frames;
       }
     }
@@ -326,7 +326,7 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
     }
     if (mode == LIZX_RUN_GRAPH) {
       if (!b->exec) {
-        hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
         try {
           b->enqueue_batch(false);
         } catch (...) {
@@ -445,11 +445,11 @@ LIZ_API int lizx_batch_blocks(lizx_batch *b, uint8_t *blocks, uint8_t *ok, int8_
     hip_check(hipSetDevice(b->eng->device), "hipSetDevice");
     b->eng->wait("the runs");
     const int B = b->blocks;
-    hip_check(hipMemcpy(its, b->its.p, B, hipMemcpyDeviceToHost), "iterations");
-    hip_check(hipMemcpy(blk_est, b->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost), "estimates");
-    hip_check(hipMemcpy(ok, b->crc_ok.p, B, hipMemcpyDeviceToHost), "verdicts");
+    copy_wait(its, b->its.p, B, hipMemcpyDeviceToHost, b->eng->stream, "iterations");
+    copy_wait(blk_est, b->blk_est.p, B * sizeof(float), hipMemcpyDeviceToHost, b->eng->stream, "estimates");
+    copy_wait(ok, b->crc_ok.p, B, hipMemcpyDeviceToHost, b->eng->stream, "verdicts");
     b->out_bytes.resize(static_cast<size_t>(B) * BLOCK_BYTES);
-    hip_check(hipMemcpy(b->out_bytes.data(), b->bytes.p, b->out_bytes.size(), hipMemcpyDeviceToHost), "blocks");
+    copy_wait(b->out_bytes.data(), b->bytes.p, b->out_bytes.size(), hipMemcpyDeviceToHost, b->eng->stream, "blocks");
     int good = 0;
     for (int k = 0; k < B; k++) {
       if (its[k] == -2) ok[k] = 0;
@@ -472,10 +472,10 @@ LIZ_API int lizx_batch_soft(lizx_batch *b, int frame, int8_t *llr, double *sub_e
     const FramePlan &p = *g.plan;
     const SoftBufs v = g.view.at(p, b->slot_of[frame]);
     const size_t S = static_cast<size_t>(p.subch) * SUB;
-    if (llr) hip_check(hipMemcpy(llr, v.llr, 2 * S, hipMemcpyDeviceToHost), "soft values");
-    if (sub_est) hip_check(hipMemcpy(sub_est, v.est, p.subch * sizeof(double), hipMemcpyDeviceToHost), "estimates");
-    if (sub_rms) hip_check(hipMemcpy(sub_rms, v.rms, p.subch * sizeof(float), hipMemcpyDeviceToHost), "rms");
-    if (align) hip_check(hipMemcpy(align, v.align, 2 * sizeof(float), hipMemcpyDeviceToHost), "shift");
+    if (llr) copy_wait(llr, v.llr, 2 * S, hipMemcpyDeviceToHost, b->eng->stream, "soft values");
+    if (sub_est) copy_wait(sub_est, v.est, p.subch * sizeof(double), hipMemcpyDeviceToHost, b->eng->stream, "estimates");
+    if (sub_rms) copy_wait(sub_rms, v.rms, p.subch * sizeof(float), hipMemcpyDeviceToHost, b->eng->stream, "rms");
+    if (align) copy_wait(align, v.align, 2 * sizeof(float), hipMemcpyDeviceToHost, b->eng->stream, "shift");
     return 0;
   });
 }

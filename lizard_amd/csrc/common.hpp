@@ -66,6 +66,16 @@ T *guard_new(F &&f) {
   return out;
 }
 
+// A copy ordered on stream s and waited for. The library makes no legacy-stream operation (hipMemcpy and its like):
+// the HIP runtime fails those, and invalidates the capture, while any stream of the process captures a graph, and a
+// decoder on another thread may be capturing its frame graph.
+inline void copy_wait(void *dst, const void *src, size_t n, hipMemcpyKind kind, hipStream_t s, const char *what) {
+  hip_check(hipMemcpyAsync(dst, src, n, kind, s), what);
+  hip_check(hipStreamSynchronize(s), what);
+}
+// A non-blocking stream of the current device for the copies that belong to no handle's stream (plan tables).
+hipStream_t aux_stream();
+
 // A device buffer owned by RAII; grows, never shrinks.
 struct DevBuf {
   void *p = nullptr;

@@ -759,7 +759,7 @@ int hough_setup(FinderBufs &b, int F, int w, int h, hipStream_t s, size_t *cells
     }
     b.trig.reserve(tr.size() * sizeof(float));
     hip_check(hipStreamSynchronize(s), "finder stream");
-    hip_check(hipMemcpy(b.trig.p, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice), "trig");
+    copy_wait(b.trig.p, tr.data(), tr.size() * sizeof(float), hipMemcpyHostToDevice, s, "trig");
     b.trig_nr = 1;
   }
   *cells_out = cells;

@@ -409,7 +409,7 @@ LdpcEngine::LdpcEngine(int dev) : device(dev) {
   queue.reserve(sizeof(int) * 4);
   const auto &tab = crc_bit_table(&crc0);
   crc_tab.reserve(tab.size() * sizeof(uint32_t));
-  hip_check(hipMemcpy(crc_tab.p, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice), "CRC table");
+  copy_wait(crc_tab.p, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream, "CRC table");
   hipDeviceProp_t prop;
   hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
   cus = prop.multiProcessorCount;
@@ -429,11 +429,11 @@ void LdpcEngine::ensure_frame_tables(int max_slots) {
   const Format *top = format_for(64);
   if (!top) throw Error(LIZ_E_INTERNAL, "the top format could not be built");
   white.reserve(top->white.size());
-  hip_check(hipMemcpy(white.p, top->white.data(), top->white.size(), hipMemcpyHostToDevice), "whitening");
+  copy_wait(white.p, top->white.data(), top->white.size(), hipMemcpyHostToDevice, stream, "whitening");
   for (int ci = 0; ci < NCODES; ci++) {
     const auto &m = top->slot_to_post[ci];
     s2p[ci].reserve(m.size() * sizeof(int16_t));
-    hip_check(hipMemcpy(s2p[ci].p, m.data(), m.size() * sizeof(int16_t), hipMemcpyHostToDevice), "bit map");
+    copy_wait(s2p[ci].p, m.data(), m.size() * sizeof(int16_t), hipMemcpyHostToDevice, stream, "bit map");
   }
 }
 

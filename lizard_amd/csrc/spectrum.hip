@@ -950,9 +950,9 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
     }
     if (S >= (1 << 19) || n > 2048) throw Error(LIZ_E_INTERNAL, "coefficient table packing");
     p->cols.reserve((off.size() + ent.size()) * sizeof(int32_t));
-    hip_check(hipMemcpy(p->cols.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice), "columns");
-    hip_check(hipMemcpy(static_cast<int32_t *>(p->cols.p) + off.size(), ent.data(), ent.size() * sizeof(int32_t),
-                        hipMemcpyHostToDevice), "columns");
+    copy_wait(p->cols.p, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, aux_stream(), "columns");
+    copy_wait(static_cast<int32_t *>(p->cols.p) + off.size(), ent.data(), ent.size() * sizeof(int32_t),
+                        hipMemcpyHostToDevice, aux_stream(), "columns");
     d.col_off = p->cols.as<int32_t>();
     d.col_ent = p->cols.as<int32_t>() + off.size();
   }
@@ -971,7 +971,7 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
   for (int b = 0; b < f->blocks; b++)
     blk[b] = make_int4(f->blk_slot[b], code(f->blk_code[b]).nt, f->blk_nslots[b], f->blk_code[b]);
   p->blk.reserve(blk.size() * sizeof(int4));
-  hip_check(hipMemcpy(p->blk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice), "blocks");
+  copy_wait(p->blk.p, blk.data(), blk.size() * sizeof(int4), hipMemcpyHostToDevice, aux_stream(), "blocks");
   d.blk = p->blk.as<int4>();
   // the warp's sample grid: sample t at module g0 + t step (lizard_phy.c:1592; SPEC 6.9), its node cell found as the
   // reference's walk finds it (acquire.c:5111-5119)
@@ -1011,26 +1011,26 @@ std::unique_ptr<FramePlan> make_plan(int version, int span) {
   }
   // upload
   p->tables.reserve(tab.size() * sizeof(float));
-  hip_check(hipMemcpy(p->tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice), "plan tables");
+  copy_wait(p->tables.p, tab.data(), tab.size() * sizeof(float), hipMemcpyHostToDevice, aux_stream(), "plan tables");
   p->coef.reserve(coef.size() * sizeof(int32_t));
-  hip_check(hipMemcpy(p->coef.p, coef.data(), coef.size() * sizeof(int32_t), hipMemcpyHostToDevice), "coefficients");
+  copy_wait(p->coef.p, coef.data(), coef.size() * sizeof(int32_t), hipMemcpyHostToDevice, aux_stream(), "coefficients");
   p->pilot.reserve(std::max<size_t>(1, p->pilot_host.size()) * sizeof(int32_t));
   if (!p->pilot_host.empty())
-    hip_check(hipMemcpy(p->pilot.p, p->pilot_host.data(), p->pilot_host.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+    copy_wait(p->pilot.p, p->pilot_host.data(), p->pilot_host.size() * sizeof(int32_t), hipMemcpyHostToDevice, aux_stream(),
               "pilots");
   p->tw.reserve(n * sizeof(float2));
-  hip_check(hipMemcpy(p->tw.p, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice), "twiddles");
+  copy_wait(p->tw.p, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice, aux_stream(), "twiddles");
   p->rot.reserve(ALIGN_TURNS * sizeof(float2));
-  hip_check(hipMemcpy(p->rot.p, rot.data(), ALIGN_TURNS * sizeof(float2), hipMemcpyHostToDevice), "turn table");
+  copy_wait(p->rot.p, rot.data(), ALIGN_TURNS * sizeof(float2), hipMemcpyHostToDevice, aux_stream(), "turn table");
   p->mtab.reserve(n * (sizeof(float) + sizeof(int32_t)));
-  hip_check(hipMemcpy(p->mtab.p, mc.data(), n * sizeof(float), hipMemcpyHostToDevice), "module coordinates");
-  hip_check(hipMemcpy(static_cast<char *>(p->mtab.p) + n * sizeof(float), cell.data(), n * sizeof(int32_t),
-                      hipMemcpyHostToDevice),
+  copy_wait(p->mtab.p, mc.data(), n * sizeof(float), hipMemcpyHostToDevice, aux_stream(), "module coordinates");
+  copy_wait(static_cast<char *>(p->mtab.p) + n * sizeof(float), cell.data(), n * sizeof(int32_t),
+                      hipMemcpyHostToDevice, aux_stream(),
             "node cells");
   d.mcoord = p->mtab.as<float>();
   d.mcell = reinterpret_cast<const int32_t *>(static_cast<char *>(p->mtab.p) + n * sizeof(float));
   p->white.reserve(f->white.size());
-  hip_check(hipMemcpy(p->white.p, f->white.data(), f->white.size(), hipMemcpyHostToDevice), "whitening");
+  copy_wait(p->white.p, f->white.data(), f->white.size(), hipMemcpyHostToDevice, aux_stream(), "whitening");
   const float *tb = p->tables.as<float>();
   d.cx = tb;
   d.qx = tb + n;

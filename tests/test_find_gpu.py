@@ -339,3 +339,43 @@ def test_find_batch_from_two_threads_at_once():
     finally:
         d0.close()
         d1.close()
+
+
+def test_blind_decodes_on_many_threads_at_once():
+    """liz_decode on bare frames from 12 caller threads at once, a decoder each, every decoder's first frame of each
+    format capturing its frame graph while the others capture, replay and wait: each frame's verified blocks are the
+    ones a single decoder gives it alone, in the same order."""
+    import concurrent.futures as cf
+    rng = np.random.default_rng(141)
+    d0 = L.Decoder(0)
+    frames = []
+    try:
+        for i, (version, span) in enumerate([(5, 128), (20, 256), (45, 512), (8, 256)]):
+            blocks = sim.random_blocks(version, rng)
+            cam = sim.Camera(1280, 720, rot_deg=float(rng.uniform(-10, 10)), scale=float(rng.uniform(0.8, 0.9)),
+                             persp=0.02, k1=0.01, blur=0.8, noise=3.0)
+            dpic = d0.modulate(version, blocks)
+            dimg = d0.capture(version, span, dpic, 1280, 720, cam.H(span), k1=cam.k1, blur=cam.blur, noise=cam.noise,
+                              seed=90 + i, fps=30)
+            frames.append(dimg.get(np.uint8, 1280 * 720).reshape(720, 1280))
+            dimg.free()
+            dpic.free()
+        alone = [d0.decode(f) for f in frames]
+    finally:
+        d0.close()
+    decs = [L.Decoder(0) for _ in range(12)]
+    try:
+        def job(t):
+            return [decs[t].decode(frames[(t + r) % len(frames)]) for r in range(2 * len(frames))]
+
+        with cf.ThreadPoolExecutor(12) as ex:
+            outs = list(ex.map(job, range(12)))
+        for t, out in enumerate(outs):
+            for r, (k, got, o, _) in enumerate(out):
+                ka, ga, oa, _ = alone[(t + r) % len(frames)]
+                assert k == ka and o.blocks == oa.blocks, (t, r, k, ka)
+                assert np.array_equal(np.asarray(got).ravel()[:k * 476], np.asarray(ga).ravel()[:ka * 476]), (t, r)
+        assert all(a[0] > 0 for a in alone)
+    finally:
+        for d in decs:
+            d.close()
