@@ -1060,6 +1060,41 @@ def bench_refine(args):
     right0 = (before["ok"] > 0) & np.all(before["blocks"] == sent, axis=1)
     right1 = (after["ok"] > 0) & np.all(after["blocks"] == sent, axis=1)
     tr = np.array(traces)
+    # the lens unknown: the same perturbed homographies with no lens in the registration (k1 = 0), then the steps over
+    # H alone and over H and k1 together (12 each), on the re-modulation of what the lens-less registration verified
+    def nodes_k1(fr, k1):
+        p = lizard_amd.reg_params(fr["H0"], k1)
+        return lizard_amd.reg_nodes(256, fr["H0"].astype(np.float32), fr["img"].shape[1], fr["img"].shape[0], p)
+
+    nl_nodes = [nodes_k1(fr, 0.0) for fr in frames]
+    nl0 = decode([fr["H0"] for fr in frames], nl_nodes)
+
+    def fail(out):
+        return round(1.0 - float(((out["ok"] > 0) & np.all(out["blocks"] == sent, axis=1)).mean()), 5)
+
+    lens_unknown = {"block_fail_before": fail(nl0)}
+    for free_k1, name in ((False, "H"), (True, "H_and_k1")):
+        # decision-directed rounds: each refines from where the last left off, on the re-modulation of what the last
+        # decode verified
+        ps = [lizard_amd.reg_params(fr["H0"], 0.0) for fr in frames]
+        out, path = nl0, []
+        for rnd in range(3):
+            ok = out["ok"].reshape(F, B).astype(bool)
+            for k, fr in enumerate(frames):
+                ref_pic = sim.remodulate(20, fr["blocks"], ok[k], 0, fr["blur"])
+                ps[k], _ = dec.reg_refine(20, 256, fr["img"], fr["H0"], ps[k], ref_pic, steps=8, k1=free_k1)
+            out = decode([np.append(p[:8], 1.0).reshape(3, 3) for p in ps],
+                         [nodes_k1(fr, float(p[8])) for fr, p in zip(frames, ps)])
+            path.append(fail(out))
+        lens_unknown["block_fail_after_" + name] = path
+        if free_k1:
+            err = [abs(float(p[8]) - fr["k1"]) for fr, p in zip(frames, ps)]
+            lens_unknown["k1_abs_err_median"] = round(float(np.median(err)), 5)
+            lens_unknown["k1_abs_err_max"] = round(float(np.max(err)), 5)
+    lens_unknown["what"] = ("the registration with no lens (k1 = 0; the cameras' k1 in [-0.05, 0.05]) and the corners "
+                            "1.5 px off, then three decision-directed rounds (re-modulate what the last decode "
+                            "verified, 8 Gauss-Newton steps, decode) over H alone (lizx_reg_refine) or over H and k1 "
+                            "together (lizx_reg_refine_k1); block-fail after each round")
     # the plain anchor registration (the finder on each capture, from the anchors outward: over its own border nodes,
     # and the reference's chain with the border refined by its templates) and the same Gauss-Newton steps from it, the
     # nodes held, on the re-modulation of what it verified
@@ -1110,6 +1145,7 @@ def bench_refine(args):
             "frames_worse": int(np.sum(right1.reshape(F, B).sum(1) < right0.reshape(F, B).sum(1))),
             "loss_ratio_median": round(float(np.median(tr[:, -1] / tr[:, 0])), 4),
             "from_anchor_registration": anchor_legs,
+            "lens_unknown": lens_unknown,
             "refine_ms_per_frame": round(refine_s * 1e3 / F, 3),
             "generation_s": round(gen_s, 1)}
 

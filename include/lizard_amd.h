@@ -470,6 +470,10 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
    steps + 1 floats) receives the loss before each step and after the last. */
 LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
                             const float *H0, float *params, const float *ref, int steps, float *trace);
+/* The same steps over h0..h7 and k1 together (the anchors held): the 9 x 9 normal equations, k1's column the capture's
+   gradient along the lens field's blend at each sample; params' h0..h7 and k1 updated in place. */
+LIZ_API int lizx_reg_refine_k1(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                               const float *H0, float *params, const float *ref, int steps, float *trace);
 /* The same Gauss-Newton steps over H (9 floats, h8 = 1, in and out) with a node lattice held as it is (host, nx * nx
    (dx, dy) pixels: a finder's or the anchors' registration), so a registration of any form is refined. */
 LIZ_API int lizx_reg_refine_nodes(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,

@@ -134,6 +134,7 @@ def lib():
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
     sig("lizx_reg_loss_grad", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _F, _F, _P, _P)
     sig("lizx_reg_refine", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
+    sig("lizx_reg_refine_k1", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_reg_refine_nodes", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_warp", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P)
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
@@ -879,14 +880,16 @@ class Decoder:
                                        _p(grad)))
         return loss.value, grad
 
-    def reg_refine(self, version, span, img, H0, params, ref, steps=10):
-        """Gauss-Newton refinement of h0..h7 (lizx_reg_refine): the new parameters and the loss trace."""
+    def reg_refine(self, version, span, img, H0, params, ref, steps=10, k1=False):
+        """Gauss-Newton refinement of h0..h7 (lizx_reg_refine), or of h0..h7 and k1 with k1=True
+        (lizx_reg_refine_k1): the new parameters and the loss trace."""
         img = np.ascontiguousarray(img, np.uint8)
         p = np.array(params, np.float32)
         trace = np.zeros(steps + 1, np.float32)
-        check(lib().lizx_reg_refine(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
-                                    _p(np.ascontiguousarray(H0, np.float32).ravel()), _p(p),
-                                    _p(np.ascontiguousarray(ref, np.float32)), steps, _p(trace)))
+        fn = lib().lizx_reg_refine_k1 if k1 else lib().lizx_reg_refine
+        check(fn(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
+                 _p(np.ascontiguousarray(H0, np.float32).ravel()), _p(p), _p(np.ascontiguousarray(ref, np.float32)),
+                 steps, _p(trace)))
         return p, trace
 
     def reg_refine_nodes(self, version, span, img, H, nodes, ref, steps=10):

@@ -171,7 +171,7 @@ struct liz_decoder {
   WarpFrame wframe_host{};
   std::vector<uint8_t> luma;
   // registration refinement
-  DevBuf reg_ref, reg_part, reg_sum, reg_state, reg_trace;
+  DevBuf reg_ref, reg_part, reg_sum, reg_state, reg_trace, reg_lens;
   std::vector<float> reg_nodes, reg_part_host;
   RegState reg_state_host{};
   // the held registration for liz_decode
@@ -2191,8 +2191,10 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
     st.H[8] = 1.0f;
     st.alpha = alpha;
     st.beta = beta;
+    st.dk1 = 0.0f;
     hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
-    launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
+    launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), nullptr, d->reg_part.as<float>(),
+                     e.stream);
     const int nx = p.dev.nx, c = nx - 1;
     d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_SPLIT * REG_PART);
     hip_check(hipMemcpyAsync(d->reg_part_host.data(), d->reg_part.p, d->reg_part_host.size() * sizeof(float),
@@ -2209,8 +2211,8 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
         for (int k = 0; k < 8; k++) gH[k] += q[1 + k];
         const int corner[4] = {cj * nx + ci, cj * nx + ci + 1, (cj + 1) * nx + ci, (cj + 1) * nx + ci + 1};
         for (int k = 0; k < 4; k++) {
-          G[2 * corner[k]] += 2.0 * q[50 + 2 * k];
-          G[2 * corner[k] + 1] += 2.0 * q[51 + 2 * k];
+          G[2 * corner[k]] += 2.0 * q[REG_NODE0 + 2 * k];
+          G[2 * corner[k] + 1] += 2.0 * q[REG_NODE0 + 1 + 2 * k];
         }
       }
     *loss = L;
@@ -2220,21 +2222,26 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
   });
 }
 
-// steps Gauss-Newton steps over H (h0..h7 in and out) on a staged refinement's warp; trace as lizx_reg_refine's.
-static void gauss_newton(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, float *H8, int steps, float *trace) {
+// steps Gauss-Newton steps over H (h0..h7 in and out) and, with a lens field (d_lens, the unit field at the nodes),
+// k1 too (its change from the staged nodes' into *dk1) on a staged refinement's warp; trace as lizx_reg_refine's.
+static void gauss_newton(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, float *H8, int steps, float *trace,
+                         const float2 *d_lens = nullptr, float *dk1 = nullptr) {
   LdpcEngine &e = *d->eng;
   RegState &st = d->reg_state_host;
   for (int k = 0; k < 8; k++) st.H[k] = H8[k];
   st.H[8] = 1.0f;
   st.alpha = 1.0f;
   st.beta = 0.0f;
+  st.dk1 = 0.0f;
   hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
   d->reg_trace.reserve((steps + 2) * sizeof(float));
-  // pass 0 fits the intensity model only; passes 1 .. steps move H; the last pass measures the result
+  const int nparam = d_lens ? 9 : 8;
+  // pass 0 fits the intensity model only; passes 1 .. steps move H (and k1); the last pass measures the result
   for (int s = 0; s <= steps + 1; s++) {
-    launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d->reg_part.as<float>(), e.stream);
+    launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d_lens, d->reg_part.as<float>(),
+                     e.stream);
     launch_gn_step(p, d->reg_part.as<float>(), d->reg_sum.as<double>(), d->reg_state.as<RegState>(), 1e-6f,
-                   s >= 1 && s <= steps, d->reg_trace.as<float>(), s, e.stream);
+                   s >= 1 && s <= steps, nparam, d->reg_trace.as<float>(), s, e.stream);
   }
   hip_check(hipMemcpyAsync(&st, d->reg_state.p, sizeof st, hipMemcpyDeviceToHost, e.stream), "state");
   std::vector<float> tr(steps + 2);
@@ -2242,6 +2249,7 @@ static void gauss_newton(liz_decoder *d, const FramePlan &p, const WarpFrame *wf
             "trace");
   e.wait("refinement");
   for (int k = 0; k < 8; k++) H8[k] = st.H[k];
+  if (dk1) *dk1 = st.dk1;
   if (trace) std::copy(tr.begin() + 1, tr.end(), trace);  // the loss before each step and after the last
 }
 
@@ -2255,6 +2263,28 @@ LIZ_API int lizx_reg_refine(liz_decoder *d, int version, int span, const uint8_t
     RegModel model;
     const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
     gauss_newton(d, p, wf, params, steps, trace);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_reg_refine_k1(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
+                               const float *H0, float *params, const float *ref, int steps, float *trace) {
+  return guard([&] {
+    need(d && steps >= 0 && steps <= 100, "a decoder and 0 to 100 steps");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    RegModel model;
+    const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
+    // the unit lens field at the nodes (the nodes' derivative in k1), as the warp's float pairs
+    std::vector<float> lf(model.lens.size());
+    for (size_t k = 0; k < lf.size(); k++) lf[k] = static_cast<float>(model.lens[k]);
+    d->reg_lens.reserve(lf.size() * sizeof(float));
+    hip_check(hipMemcpyAsync(d->reg_lens.p, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice, e.stream),
+              "lens field");
+    float dk1 = 0;
+    gauss_newton(d, p, wf, params, steps, trace, d->reg_lens.as<float2>(), &dk1);
+    params[8] += dk1;
     return 0;
   });
 }

@@ -1,12 +1,12 @@
 // Registration refinement on gfx950: the warp's adjoint and Gauss-Newton over the homography.
 //   k_reg_accum  one workgroup a node cell (or a share of its rows): every picture sample of the cell warped through H and the nodes' bilinear
 //                blend, read bilinearly from the capture with its gradient, the residual against the reference
-//                picture, and the cell's sums: e^2, J^T e and J^T J over the 8 homography parameters, the intensity
-//                statistics, and e * (gx, gy) spread to the cell's four nodes (the gradient with respect to the nodes,
-//                which the host carries on to k1 and the anchors)
+//                picture, and the cell's sums: e^2, J^T e and J^T J over the 8 homography parameters and k1 (the
+//                lens field's blend at the sample), the intensity statistics, and e * (gx, gy) spread to the cell's
+//                four nodes (the gradient with respect to the nodes, which the host carries on to k1 and the anchors)
 //   k_gn_sum     one workgroup a sum: the cells' partial sums in FP64 in a fixed order
-//   k_gn_solve   one lane: the least-squares intensity fit, the damped 8 x 8 normal equations solved in FP64
-//                (Jacobi-scaled Cholesky, in registers), H moved on the device
+//   k_gn_solve   one lane: the least-squares intensity fit, the damped 8 x 8 (or 9 x 9, with k1) normal equations
+//                solved in FP64 (Jacobi-scaled Cholesky, in registers), H (and k1) moved on the device
 // The loop of passes runs on the device with no host round trip; partial sums are fixed trees (no atomics), so a
 // pass is bit-identical run to run.
 #include <hip/hip_runtime.h>
@@ -17,10 +17,11 @@ namespace lz {
 
 namespace {
 
-constexpr int JTJ0 = 9, STAT0 = 45, NODE0 = 50;
+constexpr int NG = REG_GN, JTJ0 = 1 + NG, STAT0 = JTJ0 + NG * (NG + 1) / 2, NODE0 = STAT0 + 5;
+static_assert(NODE0 + 8 == REG_PART, "the partial sums' layout");
 
 __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *frame, const RegState *st,
-                                                   const float *ref, float *part) {
+                                                   const float *ref, const float2 *lens, float *part) {
   __shared__ float red[4][REG_PART];
   const int ci = blockIdx.x, cj = blockIdx.y / REG_SPLIT, piece = blockIdx.y % REG_SPLIT, tid = threadIdx.x;
   const int n = P.n, nx = P.nx;
@@ -30,9 +31,15 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
   float H[9];
 #pragma unroll
   for (int k = 0; k < 9; k++) H[k] = st->H[k];
-  const float alpha = st->alpha, beta = st->beta;
-  const float2 n00 = F.nodes[cj * nx + ci], n10 = F.nodes[cj * nx + ci + 1];
-  const float2 n01 = F.nodes[(cj + 1) * nx + ci], n11 = F.nodes[(cj + 1) * nx + ci + 1];
+  const float alpha = st->alpha, beta = st->beta, dk1 = lens ? st->dk1 : 0.0f;
+  const int q00 = cj * nx + ci, q10 = q00 + 1, q01 = q00 + nx, q11 = q01 + 1;
+  const float2 z2 = make_float2(0.f, 0.f);
+  const float2 l00 = lens ? lens[q00] : z2, l10 = lens ? lens[q10] : z2, l01 = lens ? lens[q01] : z2,
+               l11 = lens ? lens[q11] : z2;
+  const float2 n00 = make_float2(F.nodes[q00].x + dk1 * l00.x, F.nodes[q00].y + dk1 * l00.y);
+  const float2 n10 = make_float2(F.nodes[q10].x + dk1 * l10.x, F.nodes[q10].y + dk1 * l10.y);
+  const float2 n01 = make_float2(F.nodes[q01].x + dk1 * l01.x, F.nodes[q01].y + dk1 * l01.y);
+  const float2 n11 = make_float2(F.nodes[q11].x + dk1 * l11.x, F.nodes[q11].y + dk1 * l11.y);
   const float xa = P.node_x[ci], xr = 1.0f / (P.node_x[ci + 1] - xa);
   const float yn = P.node_y[cj], yr = 1.0f / (P.node_y[cj + 1] - yn);
   const float xm = static_cast<float>(F.w) - 1.001f, ym = static_cast<float>(F.h) - 1.001f;
@@ -97,7 +104,7 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
       const float e = val - alpha * r - beta;
       // d(pixel)/d(h): x = A / W, y = B / W with W = h6 mx + h7 my + 1
       const float gp = -(gx * A + gy * B) * rW * rW;
-      float J[8];
+      float J[NG];
       J[0] = gx * mx * rW;
       J[1] = gx * my * rW;
       J[2] = gx * rW;
@@ -106,15 +113,18 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
       J[5] = gy * rW;
       J[6] = gp * mx;
       J[7] = gp * my;
+      // d(pixel)/d(k1): the unit lens field's blend at the sample
+      J[8] = gx * (w00[j] * l00.x + w10[j] * l10.x + w01[j] * l01.x + w11[j] * l11.x) +
+             gy * (w00[j] * l00.y + w10[j] * l10.y + w01[j] * l01.y + w11[j] * l11.y);
       acc[0] += e * e;
 #pragma unroll
-      for (int k = 0; k < 8; k++) acc[1 + k] += e * J[k];
+      for (int k = 0; k < NG; k++) acc[1 + k] += e * J[k];
       {
         int q = JTJ0;
 #pragma unroll
-        for (int k = 0; k < 8; k++)
+        for (int k = 0; k < NG; k++)
 #pragma unroll
-          for (int l = k; l < 8; l++) acc[q++] += J[k] * J[l];
+          for (int l = k; l < NG; l++) acc[q++] += J[k] * J[l];
       }
       acc[STAT0] += val;
       acc[STAT0 + 1] += r;
@@ -163,9 +173,69 @@ __global__ __launch_bounds__(256) void k_gn_sum(const float *part, int cells, do
 
 // The intensity fit and, when update, the Gauss-Newton step: the damped normal equations, Jacobi-scaled (the
 // parameters' units span nine orders of magnitude, pixels against perspective terms), solved by Cholesky in FP64 with
-// every index a constant (the matrix stays in registers). One lane does it; the matrix is 8 x 8.
+// every index a constant (the matrix stays in registers). One lane does it; the matrix is NP x NP (8: H; 9: H, k1).
+template <int NP>
+__device__ void gn_update(const double *tot, RegState *st, float damping) {
+  double A[NP][NP], b[NP], sc[NP];
+  constexpr int ROW = NG;  // J^T J's stored rows hold NG entries less the row index
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+    const double d = tot[JTJ0 + k * ROW - k * (k - 1) / 2];  // J^T J (k, k) in the upper triangle's row order
+    sc[k] = d > 0 ? 1.0 / sqrt(d * (1.0 + damping)) : 1.0;
+  }
+#pragma unroll
+  for (int k = 0; k < NP; k++) {
+#pragma unroll
+    for (int l = k; l < NP; l++) {
+      const double v = tot[JTJ0 + k * ROW - k * (k - 1) / 2 + (l - k)] * (k == l ? 1.0 + damping : 1.0);
+      A[k][l] = A[l][k] = v * sc[k] * sc[l];
+    }
+    b[k] = -tot[1 + k] * sc[k];
+  }
+  bool ok = true;
+  // A = L L^T, L in the lower triangle of A
+#pragma unroll
+  for (int j = 0; j < NP; j++) {
+    double d = A[j][j];
+#pragma unroll
+    for (int k = 0; k < j; k++) d -= A[j][k] * A[j][k];
+    ok = ok && d > 1e-300;
+    const double ljj = sqrt(d > 1e-300 ? d : 1e-300);
+    A[j][j] = ljj;
+#pragma unroll
+    for (int i = j + 1; i < NP; i++) {
+      double t = A[i][j];
+#pragma unroll
+      for (int k = 0; k < j; k++) t -= A[i][k] * A[j][k];
+      A[i][j] = t / ljj;
+    }
+  }
+  double x[NP];
+#pragma unroll
+  for (int i = 0; i < NP; i++) {
+    double t = b[i];
+#pragma unroll
+    for (int k = 0; k < i; k++) t -= A[i][k] * x[k];
+    x[i] = t / A[i][i];
+  }
+#pragma unroll
+  for (int i = NP - 1; i >= 0; i--) {
+    double t = x[i];
+#pragma unroll
+    for (int k = i + 1; k < NP; k++) t -= A[k][i] * x[k];
+    x[i] = t / A[i][i];
+  }
+#pragma unroll
+  for (int k = 0; k < NP; k++) ok = ok && x[k] == x[k];
+  if (ok) {
+#pragma unroll
+    for (int k = 0; k < 8; k++) st->H[k] = static_cast<float>(st->H[k] + x[k] * sc[k]);
+    if constexpr (NP > 8) st->dk1 = static_cast<float>(st->dk1 + x[8] * sc[8]);
+  }
+}
+
 __global__ __launch_bounds__(64) void k_gn_solve(const double *tot, RegState *st, float damping, int update,
-                                                 float *trace, int step) {
+                                                 int nparam, float *trace, int step) {
   if (threadIdx.x != 0) return;
   if (trace) trace[step] = static_cast<float>(tot[0]);
   const double sw = tot[STAT0], sr = tot[STAT0 + 1], swr = tot[STAT0 + 2], srr = tot[STAT0 + 3], cnt = tot[STAT0 + 4];
@@ -173,60 +243,8 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double *tot, RegState *st
   const double al = den > 0 ? (cnt * swr - sw * sr) / den : 1.0;
   const double be = cnt > 0 ? (sw - al * sr) / cnt : 0.0;
   if (update) {
-    double A[8][8], b[8], sc[8];
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      const double d = tot[JTJ0 + k * 8 - k * (k - 1) / 2];  // J^T J (k, k) in the upper triangle's row order
-      sc[k] = d > 0 ? 1.0 / sqrt(d * (1.0 + damping)) : 1.0;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-#pragma unroll
-      for (int l = k; l < 8; l++) {
-        const double v = tot[JTJ0 + k * 8 - k * (k - 1) / 2 + (l - k)] * (k == l ? 1.0 + damping : 1.0);
-        A[k][l] = A[l][k] = v * sc[k] * sc[l];
-      }
-      b[k] = -tot[1 + k] * sc[k];
-    }
-    bool ok = true;
-    // A = L L^T, L in the lower triangle of A
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      double d = A[j][j];
-#pragma unroll
-      for (int k = 0; k < j; k++) d -= A[j][k] * A[j][k];
-      ok = ok && d > 1e-300;
-      const double ljj = sqrt(d > 1e-300 ? d : 1e-300);
-      A[j][j] = ljj;
-#pragma unroll
-      for (int i = j + 1; i < 8; i++) {
-        double t = A[i][j];
-#pragma unroll
-        for (int k = 0; k < j; k++) t -= A[i][k] * A[j][k];
-        A[i][j] = t / ljj;
-      }
-    }
-    double x[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) {
-      double t = b[i];
-#pragma unroll
-      for (int k = 0; k < i; k++) t -= A[i][k] * x[k];
-      x[i] = t / A[i][i];
-    }
-#pragma unroll
-    for (int i = 7; i >= 0; i--) {
-      double t = x[i];
-#pragma unroll
-      for (int k = i + 1; k < 8; k++) t -= A[k][i] * x[k];
-      x[i] = t / A[i][i];
-    }
-#pragma unroll
-    for (int k = 0; k < 8; k++) ok = ok && x[k] == x[k];
-    if (ok) {
-#pragma unroll
-      for (int k = 0; k < 8; k++) st->H[k] = static_cast<float>(st->H[k] + x[k] * sc[k]);
-    }
+    if (nparam > 8) gn_update<9>(tot, st, damping);
+    else gn_update<8>(tot, st, damping);
   }
   st->alpha = static_cast<float>(al);
   st->beta = static_cast<float>(be);
@@ -235,17 +253,18 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double *tot, RegState *st
 }  // namespace
 
 void launch_reg_accum(const FramePlan &p, const WarpFrame *d_frame, const RegState *d_state, const float *d_ref,
-                      float *d_part, hipStream_t s) {
+                      const float2 *d_lens, float *d_part, hipStream_t s) {
   const int c = p.dev.nx - 1;
-  hipLaunchKernelGGL(k_reg_accum, dim3(c, c * REG_SPLIT), dim3(256), 0, s, p.dev, d_frame, d_state, d_ref, d_part);
+  hipLaunchKernelGGL(k_reg_accum, dim3(c, c * REG_SPLIT), dim3(256), 0, s, p.dev, d_frame, d_state, d_ref, d_lens,
+                     d_part);
   hip_check(hipGetLastError(), "refinement launch");
 }
 
 void launch_gn_step(const FramePlan &p, const float *d_part, double *d_sum, RegState *d_state, float damping, int update,
-                    float *d_trace, int step, hipStream_t s) {
+                    int nparam, float *d_trace, int step, hipStream_t s) {
   const int c = p.dev.nx - 1;
   hipLaunchKernelGGL(k_gn_sum, dim3(REG_PART), dim3(256), 0, s, d_part, c * c * REG_SPLIT, d_sum);
-  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, d_sum, d_state, damping, update, d_trace, step);
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, d_sum, d_state, damping, update, nparam, d_trace, step);
   hip_check(hipGetLastError(), "Gauss-Newton launch");
 }
 

@@ -19,7 +19,9 @@ constexpr int REG_PARAMS = 8 + 1 + 2 * REG_ANCH * REG_ANCH; // homography, k1, a
 
 // Anchor square i's first module on either axis in ring span (squares span / 64 modules a side).
 int anchor_m0(int span, int i);
-constexpr int REG_PART = 58;                                // floats of partial sums a piece of a node cell
+constexpr int REG_GN = 9;                                   // Gauss-Newton parameters: h0..h7 and k1
+constexpr int REG_PART = 1 + REG_GN + REG_GN * (REG_GN + 1) / 2 + 5 + 8;  // floats of partial sums a piece of a cell
+constexpr int REG_NODE0 = REG_PART - 8;                     // where a piece's node gradient (4 corners x 2) starts
 constexpr int REG_SPLIT = 1;                                // pieces a node cell's rows are split into
 
 // The node lattice's displacements as a linear function of k1 and the anchors for one ring, frame size and reference
@@ -87,21 +89,26 @@ using AnchorFind = std::function<int(const float *H, const LensModel *lm, Anchor
 int anchor_register(int span, float *H, int w, int h, float mpx, float *quad, const AnchorFind &find, AnchorHit *A,
                     LensModel *out, int *moved);
 
-// Device state of a refinement: the homography and the intensity model w = alpha r + beta.
+// Device state of a refinement: the homography, the intensity model w = alpha r + beta, and the change of k1 from
+// the staged nodes' (dk1: the nodes are theirs plus dk1 times the unit lens field).
 struct RegState {
   float H[9];
   float alpha, beta;
-  float pad[5];
+  float dk1;
+  float pad[4];
 };
 
-// Per piece of a node cell (grid (nx - 1) x (nx - 1) REG_SPLIT, pieces (cj REG_SPLIT + piece) (nx - 1) + ci): sums of e^2, J^T e (8), J^T J (36, upper triangle), the intensity
-// statistics (sum w, sum r, sum w r, sum r r, count) and e (gx, gy) spread to the cell's four corners (8), e = w -
-// alpha r - beta the residual of the warped capture w against the reference picture r.
+// Per piece of a node cell (grid (nx - 1) x (nx - 1) REG_SPLIT, pieces (cj REG_SPLIT + piece) (nx - 1) + ci): sums of
+// e^2, J^T e (9), J^T J (45, upper triangle) over h0..h7 and k1, the intensity statistics (sum w, sum r, sum w r,
+// sum r r, count) and e (gx, gy) spread to the cell's four corners (8), e = w - alpha r - beta the residual of the
+// warped capture w against the reference picture r. d_lens (nx * nx, device; null: k1's column 0 and dk1 unused) is the
+// unit lens field at the nodes.
 void launch_reg_accum(const FramePlan &p, const WarpFrame *d_frame, const RegState *d_state, const float *d_ref,
-                      float *d_part, hipStream_t s);
-// Sums the cells' parts in FP64 and, when update, solves the damped 8 x 8 normal equations in FP64 and moves H; sets
-// alpha and beta to the least-squares intensity fit of this pass; trace[step] = the pass's loss.
+                      const float2 *d_lens, float *d_part, hipStream_t s);
+// Sums the cells' parts in FP64 and, when update, solves the damped normal equations over the first nparam (8: H; 9:
+// H and k1) parameters in FP64 and moves H (and dk1); sets alpha and beta to the least-squares intensity fit of this
+// pass; trace[step] = the pass's loss.
 void launch_gn_step(const FramePlan &p, const float *d_part, double *d_sum, RegState *d_state, float damping, int update,
-                    float *d_trace, int step, hipStream_t s);
+                    int nparam, float *d_trace, int step, hipStream_t s);
 
 }  // namespace lz

@@ -198,17 +198,23 @@ def loss_grad(img, n, span, H0, params, ref, alpha=1.0, beta=0.0, want_gn=False)
            "stats": (float(val.sum()), float(np.sum(ref)), float(np.sum(val * ref)), float(np.sum(ref * ref)),
                      float(val.size))}
     if want_gn:
-        out["JtJ"] = np.einsum("yxk,yxl->kl", J, J)
-        out["Jte"] = np.einsum("yx,yxk->k", e, J)
+        # k1's column: the capture's gradient along the lens field's blend at each sample
+        Ls = (w00[..., None] * L[CJ, CI] + w10[..., None] * L[CJ, CI + 1] + w01[..., None] * L[CJ + 1, CI]
+              + w11[..., None] * L[CJ + 1, CI + 1])
+        J9 = np.concatenate([J, (gx * Ls[..., 0] + gy * Ls[..., 1])[..., None]], -1)
+        out["JtJ"] = np.einsum("yxk,yxl->kl", J9, J9)
+        out["Jte"] = np.einsum("yx,yxk->k", e, J9)
     return out
 
 
-def gn_step(img, n, span, H0, params, ref, alpha, beta, damping=1e-6):
-    """One Gauss-Newton step over the 8 homography parameters (k1 and anchors held): the normal equations
-    (J^T J + damping diag) d = -J^T e solved in FP64."""
+def gn_step(img, n, span, H0, params, ref, alpha, beta, damping=1e-6, k1=False):
+    """One Gauss-Newton step over the 8 homography parameters (k1 too with k1=True; the anchors held): the normal
+    equations (J^T J + damping diag) d = -J^T e solved in FP64."""
     r = loss_grad(img, n, span, H0, params, ref, alpha, beta, want_gn=True)
-    A = r["JtJ"] + damping * np.diag(np.diag(r["JtJ"]))
-    d = np.linalg.solve(A, -r["Jte"])
+    m = 9 if k1 else 8
+    JtJ, Jte = r["JtJ"][:m, :m], r["Jte"][:m]
+    A = JtJ + damping * np.diag(np.diag(JtJ))
+    d = np.linalg.solve(A, -Jte)
     p = np.array(params, np.float64)
-    p[:8] += d
+    p[:m] += d
     return p, r

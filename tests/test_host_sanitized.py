@@ -24,6 +24,9 @@ def test_host_entries_clean_under_sanitizers(seed):
     rt = runtime()
     if not os.path.exists(SAN) or rt is None:
         pytest.skip("the host-sanitized build (python -m lizard_amd.build --sanitize) or its runtime is absent")
+    product = os.path.join(ROOT, "lizard_amd", "liblizard_amd.so")
+    if os.path.exists(product) and os.path.getmtime(SAN) < os.path.getmtime(product):
+        pytest.skip("the host-sanitized build is older than the product library (build both: __graft_entry__.build)")
     env = dict(os.environ, LD_PRELOAD=rt, LIZARD_AMD_LIB=SAN, ASAN_OPTIONS="detect_leaks=0:halt_on_error=1",
                UBSAN_OPTIONS="halt_on_error=1:print_stacktrace=1")
     r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "sanitized_host_calls.py"), str(seed)], env=env,
