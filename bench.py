@@ -283,7 +283,9 @@ def bench_pipeline(args, caps):
     b = lizard_amd.Batch(0).plan([64] * F, [512] * F)
     # the default launch shape's graph time, then the autotuner's pick (the bits are the same whatever the shape)
     _, default_ms = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
-    shape = b.tune(frames, reps=3) if not args.no_tune else [0, 0, 0, 0]
+    # shapes and the FFTs' radix plans (a radix plan reorders the FFT's float operations; all modes below run the
+    # one kept, so their bits are compared as before)
+    shape = b.tune_plans(frames, reps=3) if not args.no_tune else [0, 0, 0, 0, 8, 8]
     res = {}
     outs = {}
     modes = args.pipe_modes.split(",")
@@ -328,7 +330,8 @@ def bench_pipeline(args, caps):
         "graph_over_per_codeword_launches": (round(res["graph"]["frames_per_s"] / res["plain"]["frames_per_s"], 2)
                                              if "plain" in res else None),
         "autotune": {"shape": {"rows_waves": shape[0], "cols_waves": shape[1], "demap_waves": shape[2],
-                               "ldpc_wg_per_cu": shape[3] or "max"},
+                               "ldpc_wg_per_cu": shape[3] or "max", "rows_radix": "%d x %d" % (shape[4], 64 // shape[4]),
+                               "cols_radix": "%d x %d" % (shape[5], 64 // shape[5])},
                      "default_shape_gpu_ms_per_batch": round(default_ms, 3)},
     }
     # the picture stages' HBM traffic and issue at the default shape, from the committed counter summary
@@ -375,7 +378,14 @@ def bench_fft_demap(args):
     frames = lizard_amd.make_frames(items)
     b = lizard_amd.Batch(0).plan(versions, spans, source=lizard_amd.SRC_GRID, stages=lizard_amd.STAGE_SOFT)
     _, default_ms = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
-    shape = b.tune(frames, reps=3)
+    # the tuner over shapes and radix plans; the time of each radix plan alone (default shapes) beside it
+    plan_ms = {}
+    for r1, r2 in ((8, 8), (4, 4), (16, 16), (4, 16), (16, 4)):
+        b.shape([4, 4, 4, 0, r1, r2])
+        plan_ms["rows %d x %d, cols %d x %d" % (r1, 64 // r1, r2, 64 // r2)] = round(
+            run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)[1], 4)
+    b.shape([4, 4, 4, 0, 8, 8])
+    shape = b.tune_plans(frames, reps=3)
     wall, ev = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
     fwall, fev = run_batch(b, frames, lizard_amd.RUN_FRAMES, max(1, args.pipe_runs // 2))
     gbs = total / (ev * 1e-3) / 1e9
@@ -389,8 +399,11 @@ def bench_fft_demap(args):
                         "AWGN 20 dB; erase, detrend, 2D FFT, pilot shift, soft demap" %
                         (len(ns), ", ".join("%d x %d" % (int((ns == n).sum()), n) for n in sorted(table))),
             "graph_ms_per_batch": round(ev, 3), "frames_per_s": round(len(ns) / wall, 1),
-            "autotune": {"shape": {"rows_waves": shape[0], "cols_waves": shape[1], "demap_waves": shape[2]},
-                         "default_shape_graph_ms_per_batch": round(default_ms, 3)},
+            "autotune": {"shape": {"rows_waves": shape[0], "cols_waves": shape[1], "demap_waves": shape[2],
+                                   "rows_radix": "%d x %d" % (shape[4], 64 // shape[4]),
+                                   "cols_radix": "%d x %d" % (shape[5], 64 // shape[5])},
+                         "default_shape_graph_ms_per_batch": round(default_ms, 3),
+                         "graph_ms_by_radix_plan": plan_ms},
             "frame_by_frame_ms_per_batch": round(fev, 3),
             "algorithmic_bytes_per_batch": int(total),
             "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

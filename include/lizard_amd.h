@@ -516,6 +516,15 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode);
 LIZ_API int lizx_batch_tune(lizx_batch *b, const lizx_frame *frames, int reps, int *choice);
 /* Sets a launch shape (4 ints, as lizx_batch_tune reports). */
 LIZ_API int lizx_batch_shape(lizx_batch *b, const int *choice);
+/* The autotuner over launch shapes and radix plans: lizx_batch_tune's search, then the radix plan of the row and of
+   the column FFT (the 64-point stage across a wave's lanes as R1 x 64/R1, R1 = 4, 8 or 16; 8 by default) by the same
+   coordinate descent. A radix plan, unlike a shape, orders the FFT's float operations differently: every plan is
+   within float rounding of the others and within 1e-5 relative L2 of FP64, so soft values may differ between plans
+   at rounding ties. choice (optional, 6 ints) receives lizx_batch_tune's 4, then the row and column radix plans.
+   No counterpart in the reference (its FFT plans are fixed per size, liblizard/src/fft.c:38-64). */
+LIZ_API int lizx_batch_tune_plans(lizx_batch *b, const lizx_frame *frames, int reps, int *choice);
+/* Sets a launch shape and the radix plans (6 ints, as lizx_batch_tune_plans reports). */
+LIZ_API int lizx_batch_shape_plans(lizx_batch *b, const int *choice);
 /* Waits for the runs; *ms (optional) receives the GPU time from the first run's start since the last sync to the last
    run's end. */
 LIZ_API int lizx_batch_sync(lizx_batch *b, float *ms);

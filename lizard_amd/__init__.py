@@ -155,6 +155,8 @@ def lib():
     sig("lizx_batch_sync", _I, _P, C.POINTER(C.c_float))
     sig("lizx_batch_tune", _I, _P, _P, _I, _P)
     sig("lizx_batch_shape", _I, _P, _P)
+    sig("lizx_batch_tune_plans", _I, _P, _P, _I, _P)
+    sig("lizx_batch_shape_plans", _I, _P, _P)
     sig("lizx_batch_blocks", _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_soft", _I, _P, _I, _P, _P, _P, _P)
     sig("lizx_ldpc_dims", _I, _I, _P)
@@ -977,7 +979,19 @@ class Batch:
         return tuple(int(x) for x in ch)
 
     def shape(self, choice):
-        check(lib().lizx_batch_shape(self.h, _p(np.ascontiguousarray(choice, np.int32))))
+        """Sets the launch shape (4 ints) or, with 6, the shape and the row and column FFTs' radix plans."""
+        c = np.ascontiguousarray(choice, np.int32)
+        if c.size == 6:
+            check(lib().lizx_batch_shape_plans(self.h, _p(c)))
+        else:
+            check(lib().lizx_batch_shape(self.h, _p(c)))
+
+    def tune_plans(self, frames, reps=3):
+        """The autotuner over shapes and radix plans (lizx_batch_tune_plans): (rows, cols, demap waves, LDPC per CU,
+        row radix, column radix)."""
+        ch = np.zeros(6, np.int32)
+        check(lib().lizx_batch_tune_plans(self.h, frames, reps, _p(ch)))
+        return tuple(int(x) for x in ch)
 
     def blocks_out(self):
         B = self.blocks

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <initializer_list>
 #include <memory>
 #include <vector>
 
@@ -352,64 +353,87 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
   });
 }
 
-// The autotuner: coordinate descent over the launch shapes, each candidate timed on the batch's own frames (the
-// runs' results are the same bits whatever the shape); the best shape is kept and the graph re-captured with it.
+// The autotuner: coordinate descent over the launch shapes (and, with plans, the FFTs' radix plans), each candidate
+// timed on the batch's own frames; the best is kept and the graph re-captured with it. Shapes leave the results'
+// bits unchanged; a radix plan orders the FFT's float operations differently.
+static void tune(lizx_batch *b, const lizx_frame *frames, int reps, bool plans, int *choice) {
+  LdpcEngine &e = *b->eng;
+  hip_check(hipSetDevice(e.device), "hipSetDevice");
+  auto time_it = [&]() {
+    // one run to settle, then reps timed ones, on the stream (no graph: the shape changes between candidates)
+    if (lizx_batch_run(b, frames, LIZX_RUN_STREAM)) throw Error(liz_last_error_code(), liz_last_error());
+    e.sync();
+    for (int r = 0; r < reps; r++)
+      if (lizx_batch_run(b, frames, LIZX_RUN_STREAM)) throw Error(liz_last_error_code(), liz_last_error());
+    e.sync();
+    return e.last_ms;
+  };
+  auto descend = [&](int *knob, std::initializer_list<int> values) {
+    int best = *knob;
+    float best_ms = time_it();
+    for (int v : values) {
+      if (v == best) continue;
+      const int keep = *knob;
+      *knob = v;
+      const float ms = time_it();
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = v;
+      }
+      *knob = keep;
+    }
+    *knob = best;
+  };
+  descend(&b->shape.rows, {1, 2, 4});
+  descend(&b->shape.cols, {1, 2, 4});
+  descend(&b->shape.demap, {1, 2, 4});
+  if (plans) {
+    descend(&b->shape.rows_radix, {4, 8, 16});
+    descend(&b->shape.cols_radix, {4, 8, 16});
+  }
+  if (b->stages == LIZX_STAGE_BLOCKS) {
+    int *pc = &b->ldpc_per_cu;
+    if (e.wg_per_cu >= 8) descend(pc, {0, 2, 4, 6, 8});
+    else if (e.wg_per_cu >= 6) descend(pc, {0, 2, 4, 6});
+    else if (e.wg_per_cu >= 4) descend(pc, {0, 2, 4});
+    else descend(pc, {0, 2});
+  }
+  b->drop_graph();
+  if (choice) {
+    choice[0] = b->shape.rows;
+    choice[1] = b->shape.cols;
+    choice[2] = b->shape.demap;
+    choice[3] = b->ldpc_per_cu;
+    if (plans) {
+      choice[4] = b->shape.rows_radix;
+      choice[5] = b->shape.cols_radix;
+    }
+  }
+}
+
 LIZ_API int lizx_batch_tune(lizx_batch *b, const lizx_frame *frames, int reps, int *choice) {
   return guard([&] {
     need(b && b->frames > 0 && frames && reps > 0 && reps <= 100, "a planned batch, its frames and 1 to 100 repeats");
-    LdpcEngine &e = *b->eng;
-    hip_check(hipSetDevice(e.device), "hipSetDevice");
-    auto time_it = [&]() {
-      // one run to settle, then reps timed ones, on the stream (no graph: the shape changes between candidates)
-      if (lizx_batch_run(b, frames, LIZX_RUN_STREAM)) throw Error(liz_last_error_code(), liz_last_error());
-      e.sync();
-      for (int r = 0; r < reps; r++)
-        if (lizx_batch_run(b, frames, LIZX_RUN_STREAM)) throw Error(liz_last_error_code(), liz_last_error());
-      e.sync();
-      return e.last_ms;
-    };
-    const int waves[3] = {1, 2, 4};
-    int *knobs[3] = {&b->shape.rows, &b->shape.cols, &b->shape.demap};
-    for (int k = 0; k < 3; k++) {
-      int best = *knobs[k];
-      float best_ms = time_it();
-      for (int w : waves) {
-        if (w == best) continue;
-        const int keep = *knobs[k];
-        *knobs[k] = w;
-        const float ms = time_it();
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = w;
-        }
-        *knobs[k] = keep;
-      }
-      *knobs[k] = best;
-    }
-    if (b->stages == LIZX_STAGE_BLOCKS) {
-      const int per_cu[5] = {0, 2, 4, 6, 8};
-      int best = b->ldpc_per_cu;
-      float best_ms = time_it();
-      for (int c : per_cu) {
-        if (c == best || (c > 0 && c > e.wg_per_cu)) continue;
-        const int keep = b->ldpc_per_cu;
-        b->ldpc_per_cu = c;
-        const float ms = time_it();
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = c;
-        }
-        b->ldpc_per_cu = keep;
-      }
-      b->ldpc_per_cu = best;
-    }
-    b->drop_graph();
-    if (choice) {
-      choice[0] = b->shape.rows;
-      choice[1] = b->shape.cols;
-      choice[2] = b->shape.demap;
-      choice[3] = b->ldpc_per_cu;
-    }
+    tune(b, frames, reps, false, choice);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_batch_tune_plans(lizx_batch *b, const lizx_frame *frames, int reps, int *choice) {
+  return guard([&] {
+    need(b && b->frames > 0 && frames && reps > 0 && reps <= 100, "a planned batch, its frames and 1 to 100 repeats");
+    tune(b, frames, reps, true, choice);
+    return 0;
+  });
+}
+
+LIZ_API int lizx_batch_shape_plans(lizx_batch *b, const int *choice) {
+  return guard([&] {
+    need(b && choice, "a batch and a shape");
+    for (int k = 4; k < 6; k++) need(choice[k] == 4 || choice[k] == 8 || choice[k] == 16, "radix plans 4, 8, 16");
+    if (lizx_batch_shape(b, choice)) throw Error(liz_last_error_code(), liz_last_error());
+    b->shape.rows_radix = choice[4];
+    b->shape.cols_radix = choice[5];
     return 0;
   });
 }

@@ -161,21 +161,24 @@ __device__ __forceinline__ void wave_sync() {
 
 // One 64-lane wave transforms one complex sequence of N = 64 M points (unscaled forward DFT): lane l holds
 // x[l + 64 m], m < M, in registers; an M-point DFT in registers, the twiddles w_N^(l k1), then the 64-point DFTs
-// across lanes as two radix-8 passes through the wave's own LDS area S (64 MP points, MP odd so that a lane stride
-// of MP spreads over the banks). Z(k) ends at S[pos(k)]. No workgroup barrier: waves run independently.
-template <int N>
+// across lanes as two passes through the wave's own LDS area S (64 MP points, MP odd so that a lane stride of MP
+// spreads over the banks): 64 = R1 x R2, R2 = 64 / R1 (the radix plan: 8 x 8, 4 x 16 or 16 x 4). Z(k) ends at
+// S[pos(k)]. No workgroup barrier: waves run independently.
+template <int N, int R1 = 8>
 struct WaveFft {
   static constexpr int M = N / 64;
+  static constexpr int R2 = 64 / R1;
   static constexpr int MP = M % 2 == 0 ? M + 1 : M;
   static constexpr int LDS = 64 * MP;               // float2 a wave
-  static constexpr int TASKS = 8 * M;               // radix-8 butterflies a pass
-  static constexpr int ROUNDS = (TASKS + 63) / 64;
+  static constexpr int TASKS1 = R1 * M, TASKS2 = R2 * M;  // R2-point DFTs in pass one, R1-point in pass two
+  static constexpr int ROUNDS1 = (TASKS1 + 63) / 64, ROUNDS2 = (TASKS2 + 63) / 64;
   static_assert(N % 64 == 0 && M >= 1 && M <= 24, "picture sizes 64 M, M <= 24");
+  static_assert(R1 == 4 || R1 == 8 || R1 == 16, "radix plans 4 x 16, 8 x 8, 16 x 4");
 
-  // k = k1 + M (c + 8 d) sits at position (d + 8 c) MP + k1
+  // k = k1 + M (c + R2 d) sits at position (d + R1 c) MP + k1
   __device__ static __forceinline__ int pos(int k) {
     const int k1 = k % M, k2 = k / M;
-    return ((k2 >> 3) + 8 * (k2 & 7)) * MP + k1;
+    return ((k2 / R2) + R1 * (k2 % R2)) * MP + k1;
   }
 
   __device__ static __forceinline__ void run(float2 (&v)[M], float2 *S, const float2 *tw, int lane) {
@@ -183,33 +186,33 @@ struct WaveFft {
 #pragma unroll
     for (int k1 = 0; k1 < M; k1++) S[lane * MP + k1] = k1 ? cmul(v[k1], tw[lane * k1]) : v[k1];
     wave_sync();
-    // 64 = 8 x 8 on each k1: n2 = a + 8 b. Pass one: 8 points over b, twiddle w_64^(a c), in place at a + 8 c.
+    // 64 = R1 x R2 on each k1: n2 = a + R1 b. Pass one: R2 points over b, twiddle w_64^(a c), in place at a + R1 c.
 #pragma unroll
-    for (int r = 0; r < ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS1; r++) {
       const int t = lane + 64 * r;
-      if (TASKS % 64 == 0 || t < TASKS) {
+      if (TASKS1 % 64 == 0 || t < TASKS1) {
         const int k1 = t % M, a = t / M;
-        float2 u[8];
+        float2 u[R2];
 #pragma unroll
-        for (int b = 0; b < 8; b++) u[b] = S[(a + 8 * b) * MP + k1];
-        dft_small<8>(u);
+        for (int b = 0; b < R2; b++) u[b] = S[(a + R1 * b) * MP + k1];
+        dft_small<R2>(u);
 #pragma unroll
-        for (int c = 0; c < 8; c++) S[(a + 8 * c) * MP + k1] = c ? cmul(u[c], tw[a * c * M]) : u[c];
+        for (int c = 0; c < R2; c++) S[(a + R1 * c) * MP + k1] = c ? cmul(u[c], tw[a * c * M]) : u[c];
       }
     }
     wave_sync();
-    // pass two: 8 points over a, output d in place at d + 8 c
+    // pass two: R1 points over a, output d in place at d + R1 c
 #pragma unroll
-    for (int r = 0; r < ROUNDS; r++) {
+    for (int r = 0; r < ROUNDS2; r++) {
       const int t = lane + 64 * r;
-      if (TASKS % 64 == 0 || t < TASKS) {
+      if (TASKS2 % 64 == 0 || t < TASKS2) {
         const int k1 = t % M, c = t / M;
-        float2 u[8];
+        float2 u[R1];
 #pragma unroll
-        for (int a = 0; a < 8; a++) u[a] = S[(a + 8 * c) * MP + k1];
-        dft_small<8>(u);
+        for (int a = 0; a < R1; a++) u[a] = S[(a + R1 * c) * MP + k1];
+        dft_small<R1>(u);
 #pragma unroll
-        for (int d = 0; d < 8; d++) S[(d + 8 * c) * MP + k1] = u[d];
+        for (int d = 0; d < R1; d++) S[(d + R1 * c) * MP + k1] = u[d];
       }
     }
     wave_sync();
@@ -490,9 +493,9 @@ __device__ __forceinline__ int xcd_block(int bx, int nb) { return nb % 8 ? bx : 
 // Rows y and y + 1 of wave w: load with the anchors' samples erased, the detrend's row sums (s, s c(x), s q(x)) in
 // double, one wave FFT of row y + i row y + 1, the two real rows' transforms separated (Z(u) and conj Z(n - u)) for
 // u < E and written column-major, X[frame][u][y].
-template <class Src, int N, int W>
+template <class Src, int N, int W, int R1>
 __global__ __launch_bounds__(64 * W) void k_rows(PlanDev P, Src src, const double *fitpart, float2 *X, double *rowsum) {
-  using F = WaveFft<N>;
+  using F = WaveFft<N, R1>;
   constexpr int M = F::M;
   extern __shared__ float2 zs[];  // W waves x F::LDS
   __shared__ double bsh[6];
@@ -572,9 +575,9 @@ __global__ __launch_bounds__(64 * W) void k_rows(PlanDev P, Src src, const doubl
 
 // Column u (one wave): the FFT along y, then every coefficient on this column straight into coefficient order:
 // coefficient (u, v) is Y(u, v), coefficient (-u, v) is conj Y(u, n - v) (the picture is real).
-template <int N, int WV>
+template <int N, int WV, int R1>
 __global__ __launch_bounds__(64 * WV) void k_cols(PlanDev P, const float2 *X, float2 *C) {
-  using F = WaveFft<N>;
+  using F = WaveFft<N, R1>;
   constexpr int M = F::M;
   extern __shared__ float2 zs[];  // WV waves x F::LDS
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E;
@@ -1105,6 +1108,17 @@ void with_waves(int w, F &&f) {
   }
 }
 
+// The wave FFT's radix plan for its 64-point stage (R1 x 64 / R1).
+template <class F>
+void with_radix(int r, F &&f) {
+  switch (r) {
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 8: f(std::integral_constant<int, 8>{}); break;
+    case 16: f(std::integral_constant<int, 16>{}); break;
+    default: throw Error(LIZ_E_ARG, "radix plan: 4, 8 or 16");
+  }
+}
+
 template <class Src>
 static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s,
                             const LaunchShape &sh) {
@@ -1115,13 +1129,19 @@ static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int 
     constexpr int NN = decltype(N)::value;
     with_waves(sh.rows, [&](auto Wr) {
       constexpr int W = decltype(Wr)::value;
-      hipLaunchKernelGGL((k_rows<Src, NN, W>), dim3(NN / (2 * W), frames), dim3(64 * W),
-                         W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, src, b.beta, b.X, b.rowsum);
+      with_radix(sh.rows_radix, [&](auto Rr) {
+        constexpr int R = decltype(Rr)::value;
+        hipLaunchKernelGGL((k_rows<Src, NN, W, R>), dim3(NN / (2 * W), frames), dim3(64 * W),
+                           W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, src, b.beta, b.X, b.rowsum);
+      });
     });
     with_waves(sh.cols, [&](auto Wc) {
       constexpr int W = decltype(Wc)::value;
-      hipLaunchKernelGGL((k_cols<NN, W>), dim3((E + W - 1) / W, frames), dim3(64 * W),
-                         W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, b.X, b.C);
+      with_radix(sh.cols_radix, [&](auto Rc) {
+        constexpr int R = decltype(Rc)::value;
+        hipLaunchKernelGGL((k_cols<NN, W, R>), dim3((E + W - 1) / W, frames), dim3(64 * W),
+                           W * WaveFft<NN>::LDS * sizeof(float2), s, p.dev, b.X, b.C);
+      });
     });
   });
   hipLaunchKernelGGL(k_align, dim3(frames), dim3(256), 0, s, p.dev, b.rowsum, b.C, b.proj, b.align);

@@ -93,9 +93,12 @@ struct FrameBufs {
 };
 
 // Waves a workgroup of the row FFT, the column FFT and the demap (1, 2 or 4): launch shapes that leave every
-// result's bits unchanged, so an autotuner may pick them by timing.
+// result's bits unchanged, so an autotuner may pick them by timing. And the radix plan of each FFT's 64-point stage
+// across the lanes (R1 x 64 / R1, R1 = 4, 8 or 16): a different order of float operations, every plan within float
+// rounding of the others and of FP64.
 struct LaunchShape {
   int rows = 4, cols = 4, demap = 4;
+  int rows_radix = 8, cols_radix = 8;
 };
 
 // Enqueues grid (row-major n x n float per frame, frames back to back) -> soft values, estimates, pilot shift.

@@ -8,7 +8,7 @@ import pytest
 import lizard_amd as L
 from lizard_amd import sim
 from _libs import ptr, ref
-from pictures import assert_finish_equal, ref_finish_soft, ref_grid_finish, ties_against_reference
+from pictures import Frame, assert_finish_equal, ref_finish_soft, ref_grid_finish, ties_against_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -203,6 +203,47 @@ def test_launch_shapes_leave_the_bits_alone(caps):
     same(b.blocks_out(), want)
     with pytest.raises(L.LizError):
         b.shape([3, 4, 4, 0])
+    b.close()
+
+
+@pytest.mark.parametrize("plan", [(4, 4), (16, 16), (4, 16), (16, 8), (8, 4)])
+def test_radix_plans_hold_to_fp64(plan):
+    """Each radix plan of the row and column FFTs (the 64-point stage across the lanes as R1 x 64 / R1) on grids of
+    every size class: estimates within 1e-4 of FP64, int8 soft values one step from FP64's on a small fraction of
+    slots at most, the pilot shift within 1e-3 of FP64's."""
+    cases = [(5, 128, 1), (20, 256, 2), (45, 512, 3), (64, 512, 4)]  # n = 384, 768, 1152, 1536
+    fr = [Frame(v, s, seed=40 + k, snr_db=20.0, shift=(0.1, -0.05)) for v, s, k in cases]
+    keep = [L.DeviceArray.of(F.grid) for F in fr]
+    frames = L.make_frames([dict(grid=d) for d in keep])
+    b = L.Batch().plan([c[0] for c in cases], [c[1] for c in cases], source=L.SRC_GRID, stages=L.STAGE_SOFT)
+    b.shape([4, 4, 4, 0, plan[0], plan[1]])
+    for mode in (L.RUN_GRAPH, L.RUN_FRAMES):
+        b.run(frames, mode)
+        b.sync()
+        for i, F in enumerate(fr):
+            got = b.soft(i, cases[i][0])
+            g64 = F.fp64()
+            assert np.allclose(got["sub_est"], g64["est"], rtol=1e-4, atol=1e-7)
+            assert abs(got["align"][0] - g64["align"][0]) < 1e-3 and abs(got["align"][1] - g64["align"][1]) < 1e-3
+            dl = np.abs(got["llr"].astype(int) - g64["llr"].astype(int))
+            assert dl.max() <= 1 and (dl > 0).mean() < 1e-3, (plan, i, (dl > 0).mean(), dl.max())
+    with pytest.raises(L.LizError):
+        b.shape([4, 4, 4, 0, 2, 8])
+    b.close()
+
+
+def test_plan_tuner_keeps_a_decoding_plan(caps):
+    """lizx_batch_tune_plans picks shapes and radix plans by timing; the blocks it then verifies are the sent ones."""
+    b = L.Batch().plan([c["version"] for c in caps], [c["span"] for c in caps])
+    frames, keep = device_frames(caps)
+    choice = b.tune_plans(frames, reps=2)
+    assert choice[0] in (1, 2, 4) and choice[4] in (4, 8, 16) and choice[5] in (4, 8, 16)
+    b.run(frames, L.RUN_GRAPH)
+    b.sync()
+    got = b.blocks_out()
+    sent = np.concatenate([c["blocks"].reshape(-1, 476) for c in caps])
+    v = got["ok"].astype(bool)
+    assert v.sum() > 0 and np.array_equal(got["blocks"][v], sent[v])
     b.close()
 
 
