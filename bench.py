@@ -392,6 +392,7 @@ def bench_fft_demap(args):
     b.close()
     for d in keep:
         d.free()
+    otf = bench_otf(args, rng, table)
     cpu = None if args.no_cpu else cpu_fft_demap(host, args.cpu_seconds)
     return {"cpu_baseline": cpu,
             "graph_over_cpu": round(len(ns) / (ev * 1e-3) / cpu["value"], 1) if cpu and cpu["value"] > 0 else None,
@@ -411,7 +412,64 @@ def bench_fft_demap(args):
                          "traffic": profile_values("fft_demap_traffic_64_mixed.txt").get("traffic_bytes_per_batch"),
                          "traffic_from": "profiles/fft_demap_traffic_64_mixed.txt: rocprofv3 --pmc runs of this leg "
                                          "(FETCH_SIZE, WRITE_SIZE, copy-calibrated), copied, not measured by this "
-                                         "run"}}
+                                         "run"},
+            "otf_equalised": otf}
+
+
+def bench_otf(args, rng, table, frames_n=16):
+    """The FFT + demap with equalisation by each frame's known blur (lizx_batch_otf): frames_n grids of the same
+    sizes blurred by a Gaussian of sigma in [0.5, 1.0] samples (a periodic blur, through numpy's FFT) plus AWGN at 20
+    dB, and the same at sigma 1.0-1.6 and 14 dB; the graph's time with the equalisation against without, and the
+    blocks each way verifies (lizx_finish_grid's LDPC and CRC through lizx_decoder_otf)."""
+    out = {}
+    for name, lo, hi, snr in (("sigma 0.5-1.0, 20 dB", 0.5, 1.0, 20.0), ("sigma 1.0-1.6, 14 dB", 1.0, 1.6, 14.0)):
+        out[name] = bench_otf_case(args, rng, table, frames_n, lo, hi, snr)
+    return out
+
+
+def bench_otf_case(args, rng, table, frames_n, lo, hi, snr):
+    from lizard_amd import sim
+    ns = rng.choice(sorted(table), size=frames_n)
+    sig = rng.uniform(lo, hi, frames_n)
+    grids, vs, sps, blocks_sent = [], [], [], []
+    for n, s_ in zip(ns, sig):
+        v, sp = table[int(n)]
+        blk = sim.random_blocks(v, rng)
+        pic = sim.picture(v, blk).numpy().astype(np.float64)
+        f = np.fft.fftfreq(int(n)) * int(n)
+        H = np.exp(-2.0 * np.pi ** 2 * s_ * s_ * (f[:, None] ** 2 + f[None, :] ** 2) / (int(n) ** 2))
+        pic = np.real(np.fft.ifft2(np.fft.fft2(pic) * H))
+        grids.append((pic + rng.standard_normal(pic.shape) * np.sqrt(pic.var() / 10 ** (snr / 10))).astype(np.float32))
+        vs.append(v)
+        sps.append(sp)
+        blocks_sent.append(np.asarray(blk, np.uint8).reshape(-1, 476))
+    keep = [lizard_amd.DeviceArray.of(g) for g in grids]
+    frames = lizard_amd.make_frames([dict(grid=d) for d in keep])
+    b = lizard_amd.Batch(0).plan(vs, sps, source=lizard_amd.SRC_GRID, stages=lizard_amd.STAGE_SOFT)
+    _, plain_ms = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
+    b.otf(sig.astype(np.float32))
+    _, eq_ms = run_batch(b, frames, lizard_amd.RUN_GRAPH, args.pipe_runs)
+    b.close()
+    dec = lizard_amd.Decoder(0)
+    verified = {}
+    right = True
+    for name, on in (("unequalised", False), ("equalised", True)):
+        good = 0
+        for g, v, sp, s_, sent in zip(grids, vs, sps, sig, blocks_sent):
+            dec.otf(float(s_) if on else 0.0)
+            out = dec.finish_grid(v, sp, g, soft=False)
+            ok = out["ok"].astype(bool)
+            right &= bool(np.array_equal(out["blocks"][ok], sent[ok]))
+            good += int(ok.sum())
+        verified[name] = good
+    dec.close()
+    for d in keep:
+        d.free()
+    return {"workload": "%d grids of the mix's sizes blurred by a Gaussian of sigma %.1f-%.1f samples, AWGN %g dB" %
+                        (frames_n, lo, hi, snr),
+            "graph_ms_per_batch": {"unequalised": round(plain_ms, 4), "equalised": round(eq_ms, 4)},
+            "blocks": int(sum(x.shape[0] for x in blocks_sent)), "verified": verified,
+            "verified_bytes_are_sent": right}
 
 
 def cpu_fft_demap(grids, seconds):

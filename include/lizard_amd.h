@@ -254,6 +254,9 @@ LIZ_API int lizx_decode_reg(liz_decoder *d, int version, int span, const uint8_t
                             const lizx_registration *reg, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
                             lizx_soft *soft);
 
+/* The single-frame paths' equalisation by a known Gaussian blur of sigma picture samples (0 to 8; 0: none, the
+   default and the reference's demap), as lizx_batch_otf, for the decoder's following decodes. */
+LIZ_API int lizx_decoder_otf(liz_decoder *d, float sigma);
 /* Whether the decoder replays its single-frame decodes (liz_decode with a registration held or found,
    lizx_decode_reg) as one HIP graph of the frame's kernels, captured once per format and work areas (on, 1, the
    default), or launches them one by one (0). Results are the same bits either way. No counterpart in the reference
@@ -525,6 +528,12 @@ LIZ_API int lizx_batch_shape(lizx_batch *b, const int *choice);
 LIZ_API int lizx_batch_tune_plans(lizx_batch *b, const lizx_frame *frames, int reps, int *choice);
 /* Sets a launch shape and the radix plans (6 ints, as lizx_batch_tune_plans reports). */
 LIZ_API int lizx_batch_shape_plans(lizx_batch *b, const int *choice);
+/* Equalisation by each frame's known Gaussian blur: sigma (frames floats, caller order; 0 to 8 picture samples, 0
+   none; null: none) for the following runs. The demap then divides each coefficient (u, v), after the pilot turn, by
+   the blur's OTF exp(-2 pi^2 sigma^2 (u^2 + v^2) / n^2) before the sub-channel statistics and soft values. No
+   counterpart in the reference, whose demap does not equalise (its sub-channel scale absorbs the blur,
+   research/SPEC.md 6.10); with no blur set the results are the reference's. */
+LIZ_API int lizx_batch_otf(lizx_batch *b, const float *sigma);
 /* Waits for the runs; *ms (optional) receives the GPU time from the first run's start since the last sync to the last
    run's end. */
 LIZ_API int lizx_batch_sync(lizx_batch *b, float *ms);

@@ -156,6 +156,8 @@ def lib():
     sig("lizx_batch_tune", _I, _P, _P, _I, _P)
     sig("lizx_batch_shape", _I, _P, _P)
     sig("lizx_batch_tune_plans", _I, _P, _P, _I, _P)
+    sig("lizx_batch_otf", _I, _P, _P)
+    sig("lizx_decoder_otf", _I, _P, C.c_float)
     sig("lizx_batch_shape_plans", _I, _P, _P)
     sig("lizx_batch_blocks", _I, _P, _P, _P, _P, _P)
     sig("lizx_batch_soft", _I, _P, _I, _P, _P, _P, _P)
@@ -906,6 +908,10 @@ class Decoder:
                                           _p(np.ascontiguousarray(ref, np.float32)), steps, _p(trace)))
         return Hn, trace
 
+    def otf(self, sigma):
+        """lizx_decoder_otf: equalise the following decodes by a known Gaussian blur of sigma picture samples (0: none)."""
+        check(lib().lizx_decoder_otf(self.h, float(sigma)))
+
     def finish_grid(self, version, span, grid, soft=True):
         """lizard_finish with a supplied n x n grid (row y, column x) of LIZARD-16V in the ring of this span."""
         info = format_info(version)
@@ -985,6 +991,13 @@ class Batch:
             check(lib().lizx_batch_shape_plans(self.h, _p(c)))
         else:
             check(lib().lizx_batch_shape(self.h, _p(c)))
+
+    def otf(self, sigma):
+        """lizx_batch_otf: each frame's known blur (picture samples, 0 none) equalised by the demap; None: none."""
+        if sigma is None:
+            check(lib().lizx_batch_otf(self.h, None))
+        else:
+            check(lib().lizx_batch_otf(self.h, _p(np.ascontiguousarray(sigma, np.float32))))
 
     def tune_plans(self, frames, reps=3):
         """The autotuner over shapes and radix plans (lizx_batch_tune_plans): (rows, cols, demap waves, LDPC per CU,

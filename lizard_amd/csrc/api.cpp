@@ -189,7 +189,7 @@ struct liz_decoder {
   struct FrameGraph {
     int version = 0, span = 0;
     bool dbg = false;
-    const void *keys[8] = {};
+    const void *keys[9] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
   } fg;
@@ -713,6 +713,15 @@ LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on) {
   });
 }
 
+LIZ_API int lizx_decoder_otf(liz_decoder *d, float sigma) {
+  return guard([&] {
+    need(d, "a decoder");
+    hip_check(hipSetDevice(d->eng->device), "hipSetDevice");
+    d->fb.set_otf(&sigma, 1, d->eng->stream);
+    return 0;
+  });
+}
+
 static int lane_anchor_find(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H,
                             const LensModel *lm, AnchorHit *A);
 static void border_refine(FinderLane &ln, int span, const uint8_t *d_img, int w, int h, const float *H, float mpx,
@@ -1082,9 +1091,9 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
     // the frame's kernels (picture stages, queue reset, LDPC) as one graph: captured once for this format and these
     // work areas (the capture and registration sit at fixed device addresses the kernels read), then replayed
     auto &g = d->fg;
-    const void *keys[8] = {v.X, v.C, v.llr, v.coef_dbg, d->res.p, d->tasks.p, wf, e.white.p};
+    const void *keys[9] = {v.X, v.C, v.llr, v.coef_dbg, d->res.p, d->tasks.p, wf, e.white.p, v.otf};
     bool same = g.exec && g.version == version && g.span == span && g.dbg == dbg;
-    for (int k = 0; same && k < 8; k++) same = g.keys[k] == keys[k];
+    for (int k = 0; same && k < 9; k++) same = g.keys[k] == keys[k];
     if (!same) {
       d->drop_frame_graph();
       hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
@@ -1102,7 +1111,7 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
       g.version = version;
       g.span = span;
       g.dbg = dbg;
-      for (int k = 0; k < 8; k++) g.keys[k] = keys[k];
+      for (int k = 0; k < 9; k++) g.keys[k] = keys[k];
     }
     hip_check(hipGraphLaunch(g.exec, e.stream), "hipGraphLaunch");
   }

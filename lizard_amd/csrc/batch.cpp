@@ -427,6 +427,21 @@ LIZ_API int lizx_batch_tune_plans(lizx_batch *b, const lizx_frame *frames, int r
   });
 }
 
+LIZ_API int lizx_batch_otf(lizx_batch *b, const float *sigma) {
+  return guard([&] {
+    need(b && b->frames > 0, "a planned batch");
+    hip_check(hipSetDevice(b->eng->device), "hipSetDevice");
+    for (auto &g : b->groups) {
+      std::vector<float> sg(g->frames.size(), 0.0f);
+      for (size_t i = 0; sigma && i < sg.size(); i++) sg[i] = sigma[g->frames[i]];
+      g->fb.set_otf(sg.data(), static_cast<int>(sg.size()), b->eng->stream);
+      g->view.otf = g->fb.has_otf ? g->fb.otf.as<float>() : nullptr;
+    }
+    b->drop_graph();  // the kernels' arguments change
+    return 0;
+  });
+}
+
 LIZ_API int lizx_batch_shape_plans(lizx_batch *b, const int *choice) {
   return guard([&] {
     need(b && choice, "a batch and a shape");

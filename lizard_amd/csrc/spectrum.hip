@@ -743,12 +743,26 @@ __device__ __forceinline__ double axis_info(double g) {
 // Sub-channel j of frame f (lizard_phy.c:1756-1772, :1322-1332; SPEC 8.5), one wave a sub-channel (WV a
 // workgroup): each lane's five coefficients gathered, detrended and turned, the moments m2, m4 summed in double
 // across the wave by shuffles (every lane ends with the totals, so the scale needs no broadcast and no barrier).
+// Equalisation by a known Gaussian blur of sigma picture samples (an extension; the reference does not equalise, its
+// per-sub-channel scale absorbing the blur, research/SPEC.md 6.10): coefficient (u, v) divided by the blur's OTF
+// exp(-2 pi^2 sigma^2 (u^2 + v^2) / n^2), after the pilot turn. otf_rate is 2 pi^2 sigma^2 / n^2 for frame f (0: none).
+__device__ __forceinline__ float otf_rate(const PlanDev &P, const float *otf, int f) {
+  const float sg = otf ? otf[f] : 0.0f;
+  return sg > 0 ? static_cast<float>(2.0 * kPi * kPi) * sg * sg / (static_cast<float>(P.n) * static_cast<float>(P.n))
+                : 0.0f;
+}
+__device__ __forceinline__ float2 equalise(float2 c, float gk, int u, int v) {
+  const float g = expf(gk * static_cast<float>(u * u + v * v));
+  return make_float2(c.x * g, c.y * g);
+}
+
 template <int WV>
 __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_all, const double *proj,
                                                    const float *align, int8_t *llr_all, double *est_all, float *rms_all,
-                                                   float *dbg_all) {
+                                                   float *dbg_all, const float *otf) {
   const int lane = threadIdx.x & 63, j = blockIdx.x * WV + (threadIdx.x >> 6), f = blockIdx.y;
   if (j >= P.subch) return;  // the whole wave
+  const float gk = otf_rate(P, otf, f);
   const float2 *C = C_all + static_cast<size_t>(f) * P.subch * SUB;
   const Proj pr{proj[f * 5], proj[f * 5 + 1], proj[f * 5 + 2], proj[f * 5 + 3], proj[f * 5 + 4]};
   const float dx = align[2 * f], dy = align[2 * f + 1];
@@ -763,6 +777,7 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_al
     float2 c = coef_at(P, C, pr, ci, u, v);
     if (dbg_all) reinterpret_cast<float2 *>(dbg_all)[static_cast<size_t>(f) * 2 * P.subch * SUB + ci] = c;
     if (dx != 0 || dy != 0) c = turn(P, c, u, v, dx, dy);
+    if (gk > 0) c = equalise(c, gk, u, v);
     y[q] = c;
     const double e = static_cast<double>(c.x) * c.x + static_cast<double>(c.y) * c.y;
     s2 += e;
@@ -801,8 +816,10 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_al
 // against the whitening's sign in units of its sub-channel's rms axis value, averaged over the tail; NaN where the
 // block has no tail or no sub-channel of it read anything. One wave a block (tails are 16 or 32 slots).
 __global__ __launch_bounds__(64) void k_pilot_read(PlanDev P, const float2 *C_all, const double *proj,
-                                                   const float *align, const float *rms_all, float *out) {
+                                                   const float *align, const float *rms_all, float *out,
+                                                   const float *otf) {
   const int b = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+  const float gk = otf_rate(P, otf, f);
   const int4 bk = P.blk[b];
   const int s0 = bk.x, nt = bk.y, nb = bk.z;
   const float2 *C = C_all + static_cast<size_t>(f) * P.subch * SUB;
@@ -818,6 +835,7 @@ __global__ __launch_bounds__(64) void k_pilot_read(PlanDev P, const float2 *C_al
     coef_uv(P, gs >> 1, &u, &v);
     float2 y = coef_at(P, C, pr, gs >> 1, u, v);
     if (dx != 0 || dy != 0) y = turn(P, y, u, v, dx, dy);
+    if (gk > 0) y = equalise(y, gk, u, v);
     const float a = (i & 1) ? y.y : y.x;
     cs += (P.white[gs] ? -1.0 : 1.0) * a / rms;
     got++;
@@ -1077,7 +1095,20 @@ SoftBufs FrameBufs::view(bool debug) {
   v.rms = rms.as<float>();
   v.coef_dbg = debug ? coef_dbg.as<float>() : nullptr;
   v.blk_pilot = pilot_r.p ? pilot_r.as<float>() : nullptr;
+  v.otf = has_otf ? otf.as<float>() : nullptr;
   return v;
+}
+
+void FrameBufs::set_otf(const float *sigma, int frames, hipStream_t s) {
+  bool any = false;
+  for (int f = 0; sigma && f < frames; f++) {
+    if (!(sigma[f] >= 0.0f && sigma[f] <= 8.0f)) throw Error(LIZ_E_ARG, "a blur sigma of 0 to 8 picture samples");
+    any |= sigma[f] > 0.0f;
+  }
+  has_otf = any;
+  if (!any) return;
+  otf.reserve(static_cast<size_t>(frames) * sizeof(float));
+  copy_wait(otf.p, sigma, static_cast<size_t>(frames) * sizeof(float), hipMemcpyHostToDevice, s, "blur sigma");
 }
 
 SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
@@ -1094,6 +1125,7 @@ SoftBufs SoftBufs::at(const FramePlan &p, int f) const {
   v.rms = rms + F * p.subch;
   v.coef_dbg = coef_dbg ? coef_dbg + F * 4 * S : nullptr;
   v.blk_pilot = blk_pilot ? blk_pilot + F * p.dev.blocks : nullptr;
+  v.otf = otf ? otf + F : nullptr;
   return v;
 }
 
@@ -1148,11 +1180,11 @@ static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int 
   with_waves(sh.demap, [&](auto Wd) {
     constexpr int W = decltype(Wd)::value;
     hipLaunchKernelGGL(k_demap<W>, dim3((p.subch + W - 1) / W, frames), dim3(64 * W), 0, s, p.dev, b.C, b.proj,
-                       b.align, b.llr, b.est, b.rms, b.coef_dbg);
+                       b.align, b.llr, b.est, b.rms, b.coef_dbg, b.otf);
   });
   if (b.blk_pilot)
     hipLaunchKernelGGL(k_pilot_read, dim3(p.dev.blocks, frames), dim3(64), 0, s, p.dev, b.C, b.proj, b.align, b.rms,
-                       b.blk_pilot);
+                       b.blk_pilot, b.otf);
   hip_check(hipGetLastError(), "soft-value launches");
 }
 

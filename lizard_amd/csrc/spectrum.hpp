@@ -82,14 +82,19 @@ struct SoftBufs {
   float *rms = nullptr;
   float *coef_dbg = nullptr;  // null: no debug copy of the coefficients (then the soft floats, 2S floats each)
   float *blk_pilot = nullptr; // null: no pilot readings
+  const float *otf = nullptr;  // null: no equalisation; else a blur sigma (picture samples) a frame, 0 for none
   SoftBufs at(const FramePlan &p, int f) const;  // frame f's slice
 };
 
 // Per-frame device work areas for one decode.
 struct FrameBufs {
-  DevBuf grid, X, C, rowsum, beta, proj, align, llr, est, rms, coef_dbg, pilot_r;
+  DevBuf grid, X, C, rowsum, beta, proj, align, llr, est, rms, coef_dbg, pilot_r, otf;
+  bool has_otf = false;
   void reserve(const FramePlan &p, int frames, bool debug);
   SoftBufs view(bool debug);
+  // The known blur of each frame (sigma in picture samples, 0 to 8; 0 none) that the demap equalises; all zero or
+  // null: none (the reference's demap). Copied on stream s.
+  void set_otf(const float *sigma, int frames, hipStream_t s);
 };
 
 // Waves a workgroup of the row FFT, the column FFT and the demap (1, 2 or 4): launch shapes that leave every

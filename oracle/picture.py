@@ -7,7 +7,9 @@ The picture side of the receive path restated in FP64 numpy, as the checker for 
   erase_ranges  the samples erased round the anchors (SPEC 6.13; lizard_phy.c:402-407, acquire.c:41-45)
   finish64      erase fill, detrend, forward DFT, pilot shift and soft values in FP64 (lizard_phy.c:1584-1772,
                 :280-329, :1322-1332; SPEC 6.9, 6.10, 7.3, 8.5); the pilot turn through the reference's table index
-                (lizard_phy.c:214-218), the angle exact
+                (lizard_phy.c:214-218), the angle exact; optionally (an extension the reference lacks) each
+                coefficient divided, after the turn, by a known Gaussian blur's OTF exp(-2 pi^2 s^2 (u^2 + v^2) / n^2)
+  blur          a picture blurred by a Gaussian of s samples as a periodic picture (its spectrum times the OTF)
 Pinned by: numpy's FFT (a library DFT) for the transforms; the closed forms of SPEC 6.10 (the detrend basis is
 orthogonal, so a picture that is exactly the shading surface detrends to zero) and 6.13; a known grid shift
 recovered by the pilot fit; and the compiled reference (oracle/_ref) on the same pictures (tests/test_soft_*.py).
@@ -75,7 +77,16 @@ def turn_index(u, v, n, dx, dy):
     return np.rint(a).astype(np.int64) & (ALIGN_TURNS - 1)
 
 
-def finish64(grid, n, span, u, v, white, pilots, align=None):
+def blur(pic, s):
+    """The periodic picture blurred by a Gaussian of s samples: every DFT coefficient (u, v) times
+    exp(-2 pi^2 s^2 (u^2 + v^2) / n^2), u and v the signed frequencies."""
+    n = pic.shape[0]
+    f = np.fft.fftfreq(n) * n
+    otf = np.exp(-2.0 * np.pi ** 2 * s * s * (f[:, None] ** 2 + f[None, :] ** 2) / (n * n))
+    return np.real(np.fft.ifft2(np.fft.fft2(pic) * otf))
+
+
+def finish64(grid, n, span, u, v, white, pilots, align=None, otf_sigma=0.0):
     """grid: n x n (row y, column x). Returns dict: coef (pos order, detrended, before the turn), align (dx, dy),
     soft (k y per slot, float64, before quantisation), llr (int8), est, rms. The turn by the pilots' shift goes through
     the reference's 4096-step table index (lizard_phy.c:214-218; the angle itself exact); align, where given, is the
@@ -140,6 +151,8 @@ def finish64(grid, n, span, u, v, white, pilots, align=None):
     tx, ty = (dx, dy) if align is None else (float(align[0]), float(align[1]))
     if tx or ty:
         y = y * np.exp(-1j * TWO_PI * turn_index(u, v, n, tx, ty) / ALIGN_TURNS)
+    if otf_sigma > 0:
+        y = y * np.exp(2.0 * np.pi ** 2 * otf_sigma ** 2 * (u.astype(float) ** 2 + v.astype(float) ** 2) / (n * n))
     # soft values per sub-channel (lizard_phy.c:1322-1332, :1756-1772)
     S = len(u)
     ys = y.reshape(-1, 320)

@@ -13,7 +13,8 @@ from _libs import oracle, ptr  # noqa: E402
 
 
 class Frame:
-    def __init__(self, version, span=256, seed=0, snr_db=20.0, shift=(0.0, 0.0), ramp=0.05, junk=True, cnt=0):
+    def __init__(self, version, span=256, seed=0, snr_db=20.0, shift=(0.0, 0.0), ramp=0.05, junk=True, cnt=0,
+                 blur=0.0):
         o = oracle()
         self.version, self.span, self.subch = version, span, 16 * version
         self.n = n = o.or_n_for(self.subch)
@@ -35,6 +36,9 @@ class Frame:
         o.or_whiten(ptr(self.white), 2 * S)
         o.or_format_free(f)
         pic = picture.modulate(n, self.subch, self.slots, self.u, self.v, shift)
+        self.blur = blur
+        if blur > 0:
+            pic = picture.blur(pic, blur)
         yy, xx = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
         pic = pic + ramp * ((xx - n / 2) / n + 0.5 * ((yy - n / 2) / n) ** 2)
         if junk:
@@ -54,8 +58,8 @@ class Frame:
                 pil.append(first + s)
         self.pilots = np.array(pil, np.int64)
 
-    def fp64(self, align=None):
-        return picture.finish64(self.grid, self.n, self.span, self.u, self.v, self.white, self.pilots, align)
+    def fp64(self, align=None, otf_sigma=0.0):
+        return picture.finish64(self.grid, self.n, self.span, self.u, self.v, self.white, self.pilots, align, otf_sigma)
 
 
 def lattice(span):

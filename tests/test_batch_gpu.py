@@ -307,3 +307,33 @@ def test_lizard1024_batch_beyond_the_grid_equals_reference(decoder_gpu):
         if tie == 0:
             assert np.array_equal(mine["its"], its) and np.array_equal(mine["ok"], ok)
     b.close()
+
+
+def test_batch_otf_equals_single_frames():
+    """lizx_batch_otf: per-frame blurs equalised in a batch give the single-frame path's soft values under the same
+    blur (lizx_decoder_otf), frame by frame; frames with no blur stay unequalised."""
+    cases = [(5, 128, 0.6), (20, 256, 0.0), (45, 512, 0.5)]
+    fr = [Frame(v, s, seed=70 + k, snr_db=22.0, blur=bl) for k, (v, s, bl) in enumerate(cases)]
+    keep = [L.DeviceArray.of(F.grid) for F in fr]
+    frames = L.make_frames([dict(grid=d) for d in keep])
+    b = L.Batch().plan([c[0] for c in cases], [c[1] for c in cases], source=L.SRC_GRID, stages=L.STAGE_SOFT)
+    b.otf([c[2] for c in cases])
+    dec = L.Decoder(0)
+    try:
+        for mode in (L.RUN_GRAPH, L.RUN_FRAMES):
+            b.run(frames, mode)
+            b.sync()
+            for i, (v, s, bl) in enumerate(cases):
+                dec.otf(bl)
+                want = dec.finish_grid(v, s, fr[i].grid)
+                got = b.soft(i, v)
+                assert np.array_equal(got["llr"], want["llr"]) and np.array_equal(got["sub_est"], want["sub_est"])
+        b.otf(None)
+        b.run(frames, L.RUN_GRAPH)
+        b.sync()
+        dec.otf(0.0)
+        want = dec.finish_grid(cases[0][0], cases[0][1], fr[0].grid)
+        assert np.array_equal(b.soft(0, cases[0][0])["llr"], want["llr"])
+    finally:
+        dec.close()
+        b.close()

@@ -93,3 +93,36 @@ def test_clean_frame_all_verify(decoder_gpu):
     out = decoder_gpu.finish_grid(20, 256, F.grid)
     assert out["good"] == F.blocks_n
     assert np.array_equal(out["blocks"].ravel(), F.blocks)
+
+
+@pytest.mark.parametrize("version,span,blur", [(5, 128, 0.6), (20, 256, 0.8), (45, 512, 0.5), (64, 512, 0.7)])
+def test_otf_equalised_demap_holds_to_fp64(version, span, blur):
+    """Equalisation by a known Gaussian blur (an extension; lizx_decoder_otf): on a picture blurred by that Gaussian,
+    the estimates within 1e-4 of FP64 equalising the same way (oracle/picture.py finish64 with otf_sigma), the float
+    soft values within 1e-4 relative L2 of FP64's, int8 soft values one step from FP64's on a small fraction of
+    slots; verified blocks are the sent ones; with the blur set back to 0 the decoder's results are the unequalised
+    path's."""
+    F = Frame(version, span, seed=3 * version + span, snr_db=24, shift=(0.1, -0.1), blur=blur)
+    dec = L.Decoder(0)
+    try:
+        plain = dec.finish_grid(version, span, F.grid)
+        dec.otf(blur)
+        out = dec.finish_grid(version, span, F.grid)
+        g64 = F.fp64(align=out["align"], otf_sigma=blur)
+        assert np.allclose(out["sub_est"], g64["est"], rtol=1e-4, atol=1e-7)
+        rel_soft = np.linalg.norm(out["soft"] - g64["soft"]) / np.linalg.norm(g64["soft"])
+        assert rel_soft < 1e-4, rel_soft
+        dl = np.abs(out["llr"].astype(int) - g64["llr"].astype(int))
+        assert dl.max() <= 1 and (dl > 0).mean() < 1e-3, ((dl > 0).mean(), dl.max())
+        # the coefficients themselves are not equalised (the debug copy is taken before the turn)
+        assert np.array_equal(out["coef"], plain["coef"])
+        sent = F.blocks.reshape(F.blocks_n, 476)
+        v = out["ok"].astype(bool)
+        assert v.sum() > 0 and np.array_equal(out["blocks"][v], sent[v])
+        dec.otf(0.0)
+        again = dec.finish_grid(version, span, F.grid)
+        assert np.array_equal(again["llr"], plain["llr"]) and np.array_equal(again["ok"], plain["ok"])
+        with pytest.raises(L.LizError):
+            dec.otf(-1.0)
+    finally:
+        dec.close()
