@@ -468,6 +468,18 @@ LIZ_API int lizx_reg_nodes(int span, const float *H0, int w, int h, const float 
 LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
                                const float *H0, const float *params, const float *ref, float alpha, float beta,
                                double *loss, double *grad);
+/* The loss on the coefficients and its hand-written adjoint through the FFT and the warp (an extension; no
+   counterpart in the reference): with e = w - alpha r - beta (w the capture warped through H and the model's nodes,
+   r the reference picture, n x n, as lizx_reg_loss_grad) and E its unscaled 2D DFT, loss = (2 / n^2) times the sum of
+   |E(u, v)|^2 over the format's coefficients where mask (one byte a coefficient, coefficient order, subch * 320) is
+   set: a decision-directed loss restricted to the coefficients whose values are known (the verified blocks' and the
+   padding's), so that blocks not yet decoded add nothing to it. Over every coefficient it is lizx_reg_loss_grad's
+   loss (Parseval). grad (107 doubles) as lizx_reg_loss_grad's: the inverse transform of the masked coefficients is
+   the loss's derivative with respect to the warped picture, carried through the warp's adjoint to H, k1 and the
+   anchors. */
+LIZ_API int lizx_reg_loss_grad_spectral(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h,
+                                        int stride, const float *H0, const float *params, const float *ref,
+                                        const uint8_t *mask, float alpha, float beta, double *loss, double *grad);
 /* steps Gauss-Newton steps over h0..h7 on the GPU (k1 and the anchors held; alpha and beta refitted each pass; the
    damped 8 x 8 normal equations solved in FP64 on the device); params' h0..h7 updated in place; trace (optional,
    steps + 1 floats) receives the loss before each step and after the last. */

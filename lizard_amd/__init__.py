@@ -133,6 +133,7 @@ def lib():
     sig("lizx_format_blocks", _I, _I, _P, _P, _P)
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
     sig("lizx_reg_loss_grad", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _F, _F, _P, _P)
+    sig("lizx_reg_loss_grad_spectral", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _P, _F, _F, _P, _P)
     sig("lizx_reg_refine", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_reg_refine_k1", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_reg_refine_nodes", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
@@ -882,6 +883,19 @@ class Decoder:
                                        _p(np.ascontiguousarray(params, np.float32)),
                                        _p(np.ascontiguousarray(ref, np.float32)), alpha, beta, C.byref(loss),
                                        _p(grad)))
+        return loss.value, grad
+
+    def reg_loss_grad_spectral(self, version, span, img, H0, params, ref, mask, alpha=1.0, beta=0.0):
+        """The loss on the masked coefficients and its gradient through the FFT and the warp
+        (lizx_reg_loss_grad_spectral); mask: one byte a coefficient, coefficient order."""
+        img = np.ascontiguousarray(img, np.uint8)
+        loss, grad = C.c_double(0), np.zeros(REG_PARAMS, np.float64)
+        check(lib().lizx_reg_loss_grad_spectral(self.h, version, span, _p(img), img.shape[1], img.shape[0],
+                                                img.shape[1], _p(np.ascontiguousarray(H0, np.float32).ravel()),
+                                                _p(np.ascontiguousarray(params, np.float32)),
+                                                _p(np.ascontiguousarray(ref, np.float32)),
+                                                _p(np.ascontiguousarray(mask, np.uint8)), alpha, beta, C.byref(loss),
+                                                _p(grad)))
         return loss.value, grad
 
     def reg_refine(self, version, span, img, H0, params, ref, steps=10, k1=False):

@@ -172,6 +172,8 @@ struct liz_decoder {
   std::vector<uint8_t> luma;
   // registration refinement
   DevBuf reg_ref, reg_part, reg_sum, reg_state, reg_trace, reg_lens;
+  DevBuf spec_w, spec_X, spec_C, spec_G, spec_dl, spec_part, spec_mask;  // the spectral loss's work areas
+  std::vector<double> spec_part_host;
   std::vector<float> reg_nodes, reg_part_host;
   RegState reg_state_host{};
   // the held registration for liz_decode
@@ -2185,6 +2187,45 @@ LIZ_API int lizx_reg_nodes(int span, const float *H0, int w, int h, const float 
   });
 }
 
+// One pass of the warp's adjoint on a staged refinement (with d_dl: e = d_dl / 2, a loss's derivative with respect
+// to the warped picture): the pieces' sums in FP64, in piece order, the sum of e^2 into *sq and the gradient over
+// the 107 parameters (the nodes' gradient from the corners of the cells about them, then on to k1 and the anchors).
+static void adjoint_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, const RegModel &model,
+                         const float *params, float alpha, float beta, const float *d_dl, double *sq, double *grad) {
+  LdpcEngine &e = *d->eng;
+  RegState &st = d->reg_state_host;
+  for (int k = 0; k < 8; k++) st.H[k] = params[k];
+  st.H[8] = 1.0f;
+  st.alpha = alpha;
+  st.beta = beta;
+  st.dk1 = 0.0f;
+  hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
+  launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), nullptr, d->reg_part.as<float>(),
+                   e.stream, d_dl);
+  const int nx = p.dev.nx, c = nx - 1;
+  d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_SPLIT * REG_PART);
+  hip_check(hipMemcpyAsync(d->reg_part_host.data(), d->reg_part.p, d->reg_part_host.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, e.stream), "partial sums");
+  e.wait("refinement pass");
+  double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  std::vector<double> G(static_cast<size_t>(nx) * nx * 2, 0.0);
+  for (int row = 0; row < c * REG_SPLIT; row++)
+    for (int ci = 0; ci < c; ci++) {
+      const int cj = row / REG_SPLIT;
+      const float *q = &d->reg_part_host[(static_cast<size_t>(row) * c + ci) * REG_PART];
+      L += q[0];
+      for (int k = 0; k < 8; k++) gH[k] += q[1 + k];
+      const int corner[4] = {cj * nx + ci, cj * nx + ci + 1, (cj + 1) * nx + ci, (cj + 1) * nx + ci + 1};
+      for (int k = 0; k < 4; k++) {
+        G[2 * corner[k]] += 2.0 * q[REG_NODE0 + 2 * k];
+        G[2 * corner[k] + 1] += 2.0 * q[REG_NODE0 + 1 + 2 * k];
+      }
+    }
+  *sq = L;
+  for (int k = 0; k < 8; k++) grad[k] = 2.0 * gH[k];
+  reg_chain(model, G.data(), grad);
+}
+
 LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
                                const float *H0, const float *params, const float *ref, float alpha, float beta,
                                double *loss, double *grad) {
@@ -2195,38 +2236,52 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
     const FramePlan &p = d->plan(version, span);
     RegModel model;
     const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
-    RegState &st = d->reg_state_host;
-    for (int k = 0; k < 8; k++) st.H[k] = params[k];
-    st.H[8] = 1.0f;
-    st.alpha = alpha;
-    st.beta = beta;
-    st.dk1 = 0.0f;
-    hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
-    launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), nullptr, d->reg_part.as<float>(),
-                     e.stream);
-    const int nx = p.dev.nx, c = nx - 1;
-    d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_SPLIT * REG_PART);
-    hip_check(hipMemcpyAsync(d->reg_part_host.data(), d->reg_part.p, d->reg_part_host.size() * sizeof(float),
-                             hipMemcpyDeviceToHost, e.stream), "partial sums");
-    e.wait("refinement pass");
-    // the pieces' sums in FP64, in piece order; the nodes' gradient from the corners of the cells about them
-    double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    std::vector<double> G(static_cast<size_t>(nx) * nx * 2, 0.0);
-    for (int row = 0; row < c * REG_SPLIT; row++)
-      for (int ci = 0; ci < c; ci++) {
-        const int cj = row / REG_SPLIT;
-        const float *q = &d->reg_part_host[(static_cast<size_t>(row) * c + ci) * REG_PART];
-        L += q[0];
-        for (int k = 0; k < 8; k++) gH[k] += q[1 + k];
-        const int corner[4] = {cj * nx + ci, cj * nx + ci + 1, (cj + 1) * nx + ci, (cj + 1) * nx + ci + 1};
-        for (int k = 0; k < 4; k++) {
-          G[2 * corner[k]] += 2.0 * q[REG_NODE0 + 2 * k];
-          G[2 * corner[k] + 1] += 2.0 * q[REG_NODE0 + 1 + 2 * k];
-        }
-      }
-    *loss = L;
-    for (int k = 0; k < 8; k++) grad[k] = 2.0 * gH[k];
-    reg_chain(model, G.data(), grad);
+    adjoint_pass(d, p, wf, model, params, alpha, beta, nullptr, loss, grad);
+    return 0;
+  });
+}
+
+// The spectral loss on a staged refinement: the warped picture, its residual's masked coefficients and their
+// inverse transform (launch_spectral_adjoint), then the warp's adjoint of that derivative.
+static double spectral_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, const RegModel &model,
+                            const float *params, const uint8_t *mask, float alpha, float beta, double *grad) {
+  LdpcEngine &e = *d->eng;
+  const size_t nn = static_cast<size_t>(p.n) * p.n, S = static_cast<size_t>(p.subch) * SUB,
+               X = static_cast<size_t>(p.E) * p.n;
+  d->spec_w.reserve(nn * sizeof(float));
+  d->spec_dl.reserve(nn * sizeof(float));
+  d->spec_X.reserve(X * sizeof(float2));
+  d->spec_G.reserve(X * sizeof(float2));
+  d->spec_C.reserve(S * sizeof(float2));
+  d->spec_mask.reserve(S);
+  d->spec_part.reserve(((S + 255) / 256) * sizeof(double));
+  hip_check(hipMemcpyAsync(d->spec_mask.p, mask, S, hipMemcpyHostToDevice, e.stream), "coefficient mask");
+  launch_warp(p, wf, 1, d->spec_w.as<float>(), e.stream);
+  const int parts = launch_spectral_adjoint(p, d->spec_w.as<float>(), d->reg_ref.as<float>(), alpha, beta,
+                                            d->spec_mask.as<uint8_t>(), d->spec_X.as<float2>(),
+                                            d->spec_C.as<float2>(), d->spec_part.as<double>(),
+                                            d->spec_G.as<float2>(), d->spec_dl.as<float>(), e.stream);
+  d->spec_part_host.resize(parts);
+  hip_check(hipMemcpyAsync(d->spec_part_host.data(), d->spec_part.p, parts * sizeof(double), hipMemcpyDeviceToHost,
+                           e.stream), "loss parts");
+  double sq = 0;
+  adjoint_pass(d, p, wf, model, params, alpha, beta, d->spec_dl.as<float>(), &sq, grad);  // waits for the stream
+  double L = 0;
+  for (int k = 0; k < parts; k++) L += d->spec_part_host[k];
+  return 2.0 * L / (static_cast<double>(p.n) * p.n);
+}
+
+LIZ_API int lizx_reg_loss_grad_spectral(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h,
+                                        int stride, const float *H0, const float *params, const float *ref,
+                                        const uint8_t *mask, float alpha, float beta, double *loss, double *grad) {
+  return guard([&] {
+    need(d && loss && grad && mask, "a decoder, a coefficient mask and outputs");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    RegModel model;
+    const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
+    *loss = spectral_pass(d, p, wf, model, params, mask, alpha, beta, grad);
     return 0;
   });
 }

@@ -21,7 +21,8 @@ constexpr int NG = REG_GN, JTJ0 = 1 + NG, STAT0 = JTJ0 + NG * (NG + 1) / 2, NODE
 static_assert(NODE0 + 8 == REG_PART, "the partial sums' layout");
 
 __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *frame, const RegState *st,
-                                                   const float *ref, const float2 *lens, float *part) {
+                                                   const float *ref, const float2 *lens, float *part,
+                                                   const float *gsrc) {
   __shared__ float red[4][REG_PART];
   const int ci = blockIdx.x, cj = blockIdx.y / REG_SPLIT, piece = blockIdx.y % REG_SPLIT, tid = threadIdx.x;
   const int n = P.n, nx = P.nx;
@@ -53,7 +54,7 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
   constexpr int UNR = 4;
   constexpr float k255 = 1.0f / 255.0f;
   for (unsigned t0 = tid; t0 < cnt; t0 += 256 * UNR) {
-    float mxs[UNR], mys[UNR], As[UNR], Bs[UNR], rWs[UNR], fxs[UNR], fys[UNR], rs[UNR];
+    float mxs[UNR], mys[UNR], As[UNR], Bs[UNR], rWs[UNR], fxs[UNR], fys[UNR], rs[UNR], gs[UNR];
     uint8_t pa[UNR], pb[UNR], pc[UNR], pd[UNR];
     bool ok[UNR], inx[UNR], iny[UNR];
     float w00[UNR], w10[UNR], w01[UNR], w11[UNR];
@@ -88,6 +89,7 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
       pc[j] = p[F.stride];
       pd[j] = p[F.stride + 1];
       rs[j] = ref[static_cast<size_t>(y) * n + x];
+      gs[j] = gsrc ? gsrc[static_cast<size_t>(y) * n + x] : 0.0f;
       mxs[j] = mx;
       mys[j] = my;
       As[j] = A;
@@ -101,7 +103,9 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
       const float a = pa[j] * k255, b = pb[j] * k255, c = pc[j] * k255, d = pd[j] * k255;
       const float top = a + (b - a) * fx, bot = c + (d - c) * fx, val = top + (bot - top) * fy;
       const float gx = inx[j] ? (1 - fy) * (b - a) + fy * (d - c) : 0.f, gy = iny[j] ? bot - top : 0.f;
-      const float e = val - alpha * r - beta;
+      // the residual, or half the loss's derivative with respect to the sample where one is given (the spectral
+      // loss's adjoint through the FFT), so that 2 e J is that loss's gradient either way
+      const float e = gsrc ? 0.5f * gs[j] : val - alpha * r - beta;
       // d(pixel)/d(h): x = A / W, y = B / W with W = h6 mx + h7 my + 1
       const float gp = -(gx * A + gy * B) * rW * rW;
       float J[NG];
@@ -253,10 +257,10 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double *tot, RegState *st
 }  // namespace
 
 void launch_reg_accum(const FramePlan &p, const WarpFrame *d_frame, const RegState *d_state, const float *d_ref,
-                      const float2 *d_lens, float *d_part, hipStream_t s) {
+                      const float2 *d_lens, float *d_part, hipStream_t s, const float *d_dl) {
   const int c = p.dev.nx - 1;
   hipLaunchKernelGGL(k_reg_accum, dim3(c, c * REG_SPLIT), dim3(256), 0, s, p.dev, d_frame, d_state, d_ref, d_lens,
-                     d_part);
+                     d_part, d_dl);
   hip_check(hipGetLastError(), "refinement launch");
 }
 

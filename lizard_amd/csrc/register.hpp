@@ -102,9 +102,10 @@ struct RegState {
 // e^2, J^T e (9), J^T J (45, upper triangle) over h0..h7 and k1, the intensity statistics (sum w, sum r, sum w r,
 // sum r r, count) and e (gx, gy) spread to the cell's four corners (8), e = w - alpha r - beta the residual of the
 // warped capture w against the reference picture r. d_lens (nx * nx, device; null: k1's column 0 and dk1 unused) is the
-// unit lens field at the nodes.
+// unit lens field at the nodes. With d_dl (n x n, device) e is instead d_dl / 2 at each sample: a loss's derivative
+// with respect to the warped picture (launch_spectral_adjoint), so that the sums' 2 J^T e is that loss's gradient.
 void launch_reg_accum(const FramePlan &p, const WarpFrame *d_frame, const RegState *d_state, const float *d_ref,
-                      const float2 *d_lens, float *d_part, hipStream_t s);
+                      const float2 *d_lens, float *d_part, hipStream_t s, const float *d_dl = nullptr);
 // Sums the cells' parts in FP64 and, when update, solves the damped normal equations over the first nparam (8: H; 9:
 // H and k1) parameters in FP64 and moves H (and dk1); sets alpha and beta to the least-squares intensity fit of this
 // pass; trace[step] = the pass's loss.

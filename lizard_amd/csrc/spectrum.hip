@@ -1224,8 +1224,11 @@ __device__ __forceinline__ float2 qpsk(const uint8_t *bits, int ci) {
   return make_float2(bits[2 * ci] ? -a : a, bits[2 * ci + 1] ? -a : a);
 }
 
-template <int N, int WV>
-__global__ __launch_bounds__(64 * WV) void k_mod_cols(PlanDev P, const uint8_t *bits_all, float2 *G) {
+// COEF: the column's values are the coefficients coef[ci] where mask[ci] (else 0), one frame, instead of QPSK points
+// from frame slots (the spectral loss's adjoint).
+template <int N, int WV, bool COEF = false>
+__global__ __launch_bounds__(64 * WV) void k_mod_cols(PlanDev P, const uint8_t *bits_all, float2 *G,
+                                                      const float2 *coef = nullptr, const uint8_t *mask = nullptr) {
   using F = WaveFft<N>;
   constexpr int M = F::M;
   extern __shared__ float2 zs[];  // WV waves x F::LDS (at least N points each)
@@ -1241,7 +1244,13 @@ __global__ __launch_bounds__(64 * WV) void k_mod_cols(PlanDev P, const uint8_t *
   // column u); in column 0 every coefficient's partner (0, -v) too
   for (int e = P.col_off[u] + lane; e < P.col_off[u + 1]; e += 64) {
     const uint32_t q = static_cast<uint32_t>(P.col_ent[e]);
-    const float2 h = qpsk(bits, static_cast<int>(q & 0x7FFFF));
+    float2 h;
+    if constexpr (COEF) {
+      const int ci = static_cast<int>(q & 0x7FFFF);
+      h = mask[ci] ? coef[ci] : make_float2(0.0f, 0.0f);
+    } else {
+      h = qpsk(bits, static_cast<int>(q & 0x7FFFF));
+    }
     const int k = static_cast<int>((q >> 19) & 0x7FF);
     T[k] = (q >> 30) ? make_float2(h.x, -h.y) : h;
     if (u == 0 && k) T[N - k] = make_float2(h.x, -h.y);
@@ -1264,7 +1273,8 @@ __global__ __launch_bounds__(64 * WV) void k_mod_cols(PlanDev P, const uint8_t *
   }
 }
 
-template <int N, int W>
+// LIN: the rows are the transform times L, unclipped (the spectral loss's adjoint), instead of the clipped picture.
+template <int N, int W, bool LIN = false>
 __global__ __launch_bounds__(64 * W) void k_mod_rows(PlanDev P, const float2 *G, float *pic, float L) {
   using F = WaveFft<N>;
   constexpr int M = F::M;
@@ -1297,9 +1307,85 @@ __global__ __launch_bounds__(64 * W) void k_mod_rows(PlanDev P, const float2 *G,
   for (int m = 0; m < M; m++) {
     const int x = lane + 64 * m;
     const float2 z = T[F::pos(x)];  // conj of the DFT: (p_y, p_y+1) = (z.x, -z.y)
-    row0[x] = 0.5f + fminf(fmaxf(z.x, -L), L) * inv2L;
-    row1[x] = 0.5f + fminf(fmaxf(-z.y, -L), L) * inv2L;
+    if constexpr (LIN) {
+      row0[x] = L * z.x;
+      row1[x] = -L * z.y;
+    } else {
+      row0[x] = 0.5f + fminf(fmaxf(z.x, -L), L) * inv2L;
+      row1[x] = 0.5f + fminf(fmaxf(-z.y, -L), L) * inv2L;
+    }
   }
+}
+
+// ------------------------------------------------------------------------------------------------ the spectral loss
+// The refinement's loss on the format's coefficients (an extension; the reference has no refinement): with e = w -
+// alpha r - beta the warped picture's residual against the reference picture and E its unscaled 2D DFT,
+// L = (2 / n^2) sum over the masked coefficients of |E(u, v)|^2 (each listed coefficient stands for itself and its
+// Hermitian partner; over every coefficient of the spectrum this is sum e^2, by Parseval). Its derivative with respect
+// to the picture is dL/dw(x, y) = (4 / n^2) Re sum_masked E(u, v) exp(+2 pi i (u x + v y) / n) = (2 / n^2) times the
+// real picture whose Hermitian spectrum is the masked E: the sender's inverse transform (k_mod_cols, k_mod_rows) on
+// those coefficients. The forward transform is the row and column passes without the erase and the detrend.
+
+// Rows y, y + 1 of e (one frame) as one complex sequence, transformed along x, the two real rows' transforms separated
+// for u < E into X[u][y] (the column pass's input, as k_rows writes it).
+template <int N, int W>
+__global__ __launch_bounds__(64 * W) void k_res_rows(PlanDev P, const float *wpic, const float *ref, float alpha,
+                                                     float beta, float2 *X) {
+  using F = WaveFft<N>;
+  constexpr int M = F::M;
+  extern __shared__ float2 zs[];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int y = (blockIdx.x * W + w) * 2;
+  const float *a0 = wpic + static_cast<size_t>(y) * N, *a1 = a0 + N;
+  const float *r0 = ref + static_cast<size_t>(y) * N, *r1 = r0 + N;
+  float2 v[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    const int x = lane + 64 * m;
+    v[m] = make_float2(a0[x] - alpha * r0[x] - beta, a1[x] - alpha * r1[x] - beta);
+  }
+  float2 *S = zs + w * F::LDS;
+  F::run(v, S, P.tw, lane);
+  for (int u = lane; u < P.E; u += 64) {
+    const float2 zu = S[F::pos(u)], zm = S[F::pos(u ? N - u : 0)];
+    const float4 o = make_float4(0.5f * (zu.x + zm.x), 0.5f * (zu.y - zm.y), 0.5f * (zu.y + zm.y), -0.5f * (zu.x - zm.x));
+    *reinterpret_cast<float4 *>(X + static_cast<size_t>(u) * N + y) = o;
+  }
+}
+
+// Per 256 coefficients: sum of mask |C|^2 in FP64 by a fixed tree.
+__global__ __launch_bounds__(256) void k_mask_loss(int S, const float2 *C, const uint8_t *mask, double *part) {
+  __shared__ double red[256];
+  const int tid = threadIdx.x, i = blockIdx.x * 256 + tid;
+  double a = 0;
+  if (i < S && mask[i]) a = static_cast<double>(C[i].x) * C[i].x + static_cast<double>(C[i].y) * C[i].y;
+  red[tid] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o) red[tid] += red[tid + o];
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = red[0];
+}
+
+int launch_spectral_adjoint(const FramePlan &p, const float *d_w, const float *d_ref, float alpha, float beta,
+                            const uint8_t *d_mask, float2 *d_X, float2 *d_C, double *d_part, float2 *d_G, float *d_dl,
+                            hipStream_t s) {
+  const int S = p.subch * SUB, parts = (S + 255) / 256;
+  with_size(p.n, [&](auto Nc) {
+    constexpr int NN = decltype(Nc)::value, W = 4;
+    const size_t lds = W * WaveFft<NN>::LDS * sizeof(float2);
+    hipLaunchKernelGGL((k_res_rows<NN, W>), dim3(NN / (2 * W)), dim3(64 * W), lds, s, p.dev, d_w, d_ref, alpha, beta,
+                       d_X);
+    hipLaunchKernelGGL((k_cols<NN, W, 8>), dim3((p.E + W - 1) / W, 1), dim3(64 * W), lds, s, p.dev, d_X, d_C);
+    hipLaunchKernelGGL(k_mask_loss, dim3(parts), dim3(256), 0, s, S, d_C, d_mask, d_part);
+    hipLaunchKernelGGL((k_mod_cols<NN, W, true>), dim3((p.E + W - 1) / W, 1), dim3(64 * W), lds, s, p.dev, nullptr,
+                       d_G, d_C, d_mask);
+    hipLaunchKernelGGL((k_mod_rows<NN, W, true>), dim3(NN / (2 * W), 1), dim3(64 * W), lds, s, p.dev, d_G, d_dl,
+                       static_cast<float>(2.0 / (static_cast<double>(NN) * NN)));
+  });
+  hip_check(hipGetLastError(), "spectral loss launches");
+  return parts;
 }
 
 void launch_modulate(const FramePlan &p, const uint8_t *d_bits, float2 *d_G, float *d_pic, int frames, hipStream_t s) {

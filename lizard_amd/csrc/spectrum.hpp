@@ -126,6 +126,14 @@ void launch_warp(const FramePlan &p, const WarpFrame *d_frames, int frames, floa
 // The sender's pictures: frame slots (0/1, subch * 640 a frame) -> n x n float pictures in [0, 1], row-major;
 // d_G a work area of E * n complex a frame.
 void launch_modulate(const FramePlan &p, const uint8_t *d_bits, float2 *d_G, float *d_pic, int frames, hipStream_t s);
+// The refinement's spectral loss and its adjoint through the FFT, one frame: the residual e = w - alpha r - beta
+// (n x n, device) transformed (d_X: E n complex; d_C: the format's coefficients in coefficient order), per 256
+// coefficients the sum of mask |C|^2 (d_part, FP64; the loss is 2 / n^2 times their sum), and the loss's derivative
+// with respect to w (d_dl, n x n) through the inverse transform of the masked coefficients (d_G: E n complex).
+// Returns the number of parts.
+int launch_spectral_adjoint(const FramePlan &p, const float *d_w, const float *d_ref, float alpha, float beta,
+                            const uint8_t *d_mask, float2 *d_X, float2 *d_C, double *d_part, float2 *d_G, float *d_dl,
+                            hipStream_t s);
 // The node lattice of a ring's layout (module coordinates; layout.c:110-118).
 int node_lattice(int span, float *pos);
 

@@ -14,6 +14,9 @@ hand-written adjoint:
   loss        sum over samples of (w - alpha r - beta)^2, w the warped capture, r a reference picture (the blurred
               re-modulation of what is known), alpha and beta held.
   gradient    with respect to the 8 homography parameters, k1 and the 49 anchors' (dx, dy), by the chain rule in FP64.
+  spectral    the loss on the format's masked coefficients, (2 / n^2) sum |E(u, v)|^2 with E the unscaled DFT of
+              w - alpha r - beta (numpy's FFT), and its gradient: dL/dw = (4 / n^2) Re sum E exp(+2 pi i (u x + v y) / n)
+              carried through the same chain rule.
 Pinned by: finite differences of this same FP64 loss (tests/test_register.py), a capture rendered through a known
 registration that the model reproduces (the nodes of the synthetic channel equal k1 L exactly), and the mesh's
 interpolation properties (knot values reproduced at the knots, zero at the lattice's ends).
@@ -155,8 +158,9 @@ def image_sample(img, x, y):
     return val, gx, gy
 
 
-def loss_grad(img, n, span, H0, params, ref, alpha=1.0, beta=0.0, want_gn=False):
-    """Loss and its gradient (NPARAM) in FP64; with want_gn also J^T J and J^T e over the 8 homography parameters."""
+def loss_grad(img, n, span, H0, params, ref, alpha=1.0, beta=0.0, want_gn=False, spectral=None):
+    """Loss and its gradient (NPARAM) in FP64; with want_gn also J^T J and J^T e over the 8 homography parameters.
+    spectral = (u, v, mask): the loss on the masked coefficients (u, v) instead (SPEC 6.12's coefficient list)."""
     Hh, W = img.shape
     H, k1, anch = unpack(params)
     node = lattice(span)
@@ -180,6 +184,16 @@ def loss_grad(img, n, span, H0, params, ref, alpha=1.0, beta=0.0, want_gn=False)
     val, gx, gy = image_sample(img, px, py)
     e = val - alpha * np.asarray(ref, np.float64) - beta
     loss = float(np.sum(e * e))
+    if spectral is not None:
+        cu, cv, mk = spectral
+        m = np.asarray(mk).astype(bool)
+        Fe = np.fft.fft2(e)  # Fe[v, u] = sum e[y, x] exp(-2 pi i (u x + v y) / n)
+        Ek = Fe[np.asarray(cv)[m] % n, np.asarray(cu)[m] % n]
+        loss = float(2.0 / (n * n) * np.sum(np.abs(Ek) ** 2))
+        Z = np.zeros((n, n), complex)
+        np.add.at(Z, (np.asarray(cv)[m] % n, np.asarray(cu)[m] % n), Ek)
+        # (4 / n^2) Re sum E exp(+2 pi i (u x + v y) / n); ifft2 carries 1 / n^2; e below is half of dL/dw
+        e = 2.0 * np.real(np.fft.ifft2(Z))
     z = np.zeros_like(MX)
     jx = [MX / Wd, MY / Wd, 1 / Wd, z, z, z, -A * MX / Wd ** 2, -A * MY / Wd ** 2]
     jy = [z, z, z, MX / Wd, MY / Wd, 1 / Wd, -B * MX / Wd ** 2, -B * MY / Wd ** 2]

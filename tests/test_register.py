@@ -115,6 +115,38 @@ def test_oracle_gradient_matches_finite_differences(small_case):
         assert abs(fd - g[k]) <= 2e-3 * max(abs(g[k]), 1e-3 * np.abs(g).max()) + 1e-6, (k, fd, g[k])
 
 
+def test_oracle_spectral_gradient_matches_finite_differences(small_case):
+    """The loss on masked coefficients (the DFT by numpy's FFT) and its gradient through the inverse transform and the
+    warp: central differences of the same FP64 loss; and over a mask of every coefficient of a picture that carries
+    only listed coefficients, the loss is the picture loss (Parseval over each listed coefficient and its partner)."""
+    c = small_case
+    u, v = L.format_coefs(c["version"])
+    mask = (np.random.default_rng(4).random(u.size) < 0.5).astype(np.uint8)
+    sp = (u, v, mask)
+    f = lambda p: R.loss_grad(c["img"], c["n"], c["span"], c["H0"], p, c["ref"], 0.9, 0.05,  # noqa: E731
+                              spectral=sp)["loss"]
+    g = R.loss_grad(c["img"], c["n"], c["span"], c["H0"], c["p"], c["ref"], 0.9, 0.05, spectral=sp)["grad"]
+    steps = np.full(R.NPARAM, 1e-3)
+    steps[[0, 1, 3, 4]] = 1e-6
+    steps[[6, 7]] = 1e-9
+    steps[8] = 1e-5
+    for k in [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 57, 58, 105, 106]:
+        e = np.zeros(R.NPARAM); e[k] = steps[k]
+        fd = (f(c["p"] + e) - f(c["p"] - e)) / (2 * steps[k])
+        assert abs(fd - g[k]) <= 2e-3 * max(abs(g[k]), 1e-3 * np.abs(g).max()) + 1e-6, (k, fd, g[k])
+    # Parseval: a residual made of listed coefficients only (none at DC or on its own partner) is wholly seen
+    n = c["n"]
+    rng = np.random.default_rng(5)
+    Z = np.zeros((n, n), complex)
+    h = rng.normal(size=u.size) + 1j * rng.normal(size=u.size)
+    Z[v % n, u % n] = h
+    Z[(-v) % n, (-u) % n] = np.conj(h)
+    pic = np.real(np.fft.ifft2(Z)) * n  # a real picture whose spectrum is exactly the listed coefficients and partners
+    Fe = np.fft.fft2(pic)
+    spec = 2.0 / (n * n) * np.sum(np.abs(Fe[v % n, u % n]) ** 2)
+    assert abs(spec - np.sum(pic * pic)) <= 1e-9 * np.sum(pic * pic)
+
+
 def _border_case(span, W, Hh, rot, scale, persp, cx, cy, seed, K_true, noise):
     """A homography and a border field: the lens model's own terms (K_true radial ones) about the frame's centre plus
     a smooth residual and noise, on every node (the fill reads the border only); every fifth marked node untrusted."""

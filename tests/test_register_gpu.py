@@ -104,6 +104,67 @@ def test_autograd_gradcheck(decoder_gpu, case384):
     assert torch.autograd.gradcheck(RegLoss.apply, (delta,), eps=0.02, atol=0.5, rtol=0.05, nondet_tol=1e-3)
 
 
+def _half_mask(version, seed):
+    u, _ = L.format_coefs(version)
+    return (np.random.default_rng(seed).random(u.size) < 0.5).astype(np.uint8)
+
+
+@pytest.mark.parametrize("which", ["384", "768"])
+def test_spectral_adjoint_matches_fp64(decoder_gpu, case384, which):
+    """The loss on masked coefficients and its adjoint through the FFT and the warp (lizx_reg_loss_grad_spectral)
+    against the FP64 restatement (numpy's FFT, the same chain rule): the loss within 1e-4, each parameter group's
+    gradient within 2e-3 relative."""
+    c = case384 if which == "384" else make_case(20, 256, 4)
+    rng = np.random.default_rng(12)
+    p = L.reg_params(c["H0"], c["k1"], rng.normal(0, 0.3, 98))
+    mask = _half_mask(c["version"], 7)
+    u, v = L.format_coefs(c["version"])
+    loss, grad = decoder_gpu.reg_loss_grad_spectral(c["version"], c["span"], c["img"], c["H0"], p, c["ref"], mask,
+                                                    0.9, 0.04)
+    want = R.loss_grad(c["img"], c["n"], c["span"], c["H0"].astype(np.float32).astype(np.float64),
+                       p.astype(np.float64), c["ref"].astype(np.float64), 0.9, 0.04, spectral=(u, v, mask))
+    assert abs(loss - want["loss"]) <= 1e-4 * want["loss"], (loss, want["loss"])
+    g, gw = grad, want["grad"]
+    for sl in (slice(0, 2), slice(2, 3), slice(3, 5), slice(5, 6), slice(6, 8), slice(8, 9), slice(9, None)):
+        err = np.linalg.norm(g[sl] - gw[sl]) / max(np.linalg.norm(gw[sl]), 1e-30)
+        assert err < 2e-3, (sl, err)
+    # no coefficient: no loss, no gradient
+    z, gz = decoder_gpu.reg_loss_grad_spectral(c["version"], c["span"], c["img"], c["H0"], p, c["ref"],
+                                               np.zeros_like(mask), 0.9, 0.04)
+    assert z == 0 and not np.any(gz)
+
+
+def test_spectral_autograd_gradcheck(decoder_gpu, case384):
+    """torch.autograd.gradcheck of the spectral loss through the C ABI adjoint against its own finite differences."""
+    torch = pytest.importorskip("torch")
+    c = case384
+    p0 = L.reg_params(c["H0"], c["k1"], np.random.default_rng(6).normal(0, 0.3, 98)).astype(np.float64)
+    mask = _half_mask(c["version"], 8)
+    idx = np.array([0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 58, 106])
+    scale = np.array([1 / 300, 1 / 300, 1.0, 1 / 300, 1 / 300, 1.0, 3e-6, 3e-6, 1 / 300, 1.0, 1.0, 1.0])
+
+    class SpecLoss(torch.autograd.Function):
+        @staticmethod
+        def forward(ctx, delta):
+            p = p0.copy()
+            p[idx] += delta.detach().cpu().numpy() * scale
+            loss, grad = decoder_gpu.reg_loss_grad_spectral(c["version"], c["span"], c["img"], c["H0"],
+                                                            p.astype(np.float32), c["ref"], mask, 0.9, 0.04)
+            ctx.save_for_backward(torch.as_tensor(grad[idx] * scale, dtype=torch.float64))
+            return torch.tensor(loss, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, g):
+            (gr,) = ctx.saved_tensors
+            return g * gr
+
+    delta = torch.zeros(len(idx), dtype=torch.float64, requires_grad=True)
+    out = SpecLoss.apply(delta)
+    out.backward()
+    assert torch.isfinite(delta.grad).all() and delta.grad.abs().max() > 0
+    assert torch.autograd.gradcheck(SpecLoss.apply, (delta,), eps=0.02, atol=0.5, rtol=0.05, nondet_tol=1e-3)
+
+
 def test_refine_recovers_the_homography(decoder_gpu):
     c = make_case(20, 256, 6)
     p = L.reg_params(c["H0"], c["k1"])
