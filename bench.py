@@ -1144,28 +1144,39 @@ def bench_refine(args):
         return round(1.0 - float(((out["ok"] > 0) & np.all(out["blocks"] == sent, axis=1)).mean()), 5)
 
     lens_unknown = {"block_fail_before": fail(nl0)}
-    for free_k1, name in ((False, "H"), (True, "H_and_k1")):
+    spec_s = 0.0
+    for free_k1, name, spectral in ((False, "H", False), (True, "H_and_k1", False), (True, "H_and_k1_spectral", True)):
         # decision-directed rounds: each refines from where the last left off, on the re-modulation of what the last
-        # decode verified
+        # decode verified (the spectral loss: on those blocks' and the padding's coefficients alone)
         ps = [lizard_amd.reg_params(fr["H0"], 0.0) for fr in frames]
         out, path = nl0, []
         for rnd in range(3):
             ok = out["ok"].reshape(F, B).astype(bool)
             for k, fr in enumerate(frames):
                 ref_pic = sim.remodulate(20, fr["blocks"], ok[k], 0, fr["blur"])
-                ps[k], _ = dec.reg_refine(20, 256, fr["img"], fr["H0"], ps[k], ref_pic, steps=8, k1=free_k1)
+                if spectral:
+                    t1 = time.perf_counter()
+                    ps[k], _ = dec.reg_refine_spectral(20, 256, fr["img"], fr["H0"], ps[k], ref_pic,
+                                                       sim.known_mask(20, ok[k]), steps=8)
+                    spec_s += time.perf_counter() - t1
+                else:
+                    ps[k], _ = dec.reg_refine(20, 256, fr["img"], fr["H0"], ps[k], ref_pic, steps=8, k1=free_k1)
             out = decode([np.append(p[:8], 1.0).reshape(3, 3) for p in ps],
                          [nodes_k1(fr, float(p[8])) for fr, p in zip(frames, ps)])
             path.append(fail(out))
         lens_unknown["block_fail_after_" + name] = path
         if free_k1:
             err = [abs(float(p[8]) - fr["k1"]) for fr, p in zip(frames, ps)]
-            lens_unknown["k1_abs_err_median"] = round(float(np.median(err)), 5)
-            lens_unknown["k1_abs_err_max"] = round(float(np.max(err)), 5)
+            sfx = "_spectral" if spectral else ""
+            lens_unknown["k1_abs_err_median" + sfx] = round(float(np.median(err)), 5)
+            lens_unknown["k1_abs_err_max" + sfx] = round(float(np.max(err)), 5)
+    lens_unknown["spectral_ms_per_frame_round"] = round(spec_s / (3 * F) * 1e3, 3)
     lens_unknown["what"] = ("the registration with no lens (k1 = 0; the cameras' k1 in [-0.05, 0.05]) and the corners "
                             "1.5 px off, then three decision-directed rounds (re-modulate what the last decode "
                             "verified, 8 Gauss-Newton steps, decode) over H alone (lizx_reg_refine) or over H and k1 "
-                            "together (lizx_reg_refine_k1); block-fail after each round")
+                            "together (lizx_reg_refine_k1), or over H and k1 on the loss restricted to the verified "
+                            "blocks' coefficients through the FFT's adjoint (lizx_reg_refine_spectral, 8 steps with "
+                            "a line search); block-fail after each round")
     # the plain anchor registration (the finder on each capture, from the anchors outward: over its own border nodes,
     # and the reference's chain with the border refined by its templates) and the same Gauss-Newton steps from it, the
     # nodes held, on the re-modulation of what it verified

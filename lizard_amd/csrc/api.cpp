@@ -2188,10 +2188,18 @@ LIZ_API int lizx_reg_nodes(int span, const float *H0, int w, int h, const float 
 }
 
 // One pass of the warp's adjoint on a staged refinement (with d_dl: e = d_dl / 2, a loss's derivative with respect
-// to the warped picture): the pieces' sums in FP64, in piece order, the sum of e^2 into *sq and the gradient over
-// the 107 parameters (the nodes' gradient from the corners of the cells about them, then on to k1 and the anchors).
+// to the warped picture): the pieces' sums in FP64, in piece order: the sum of e^2, the gradient over the 107
+// parameters (the nodes' gradient from the corners of the cells about them, then on to k1 and the anchors), J^T J
+// over h0..h7 and k1 (k1's column 0 without d_lens) and the intensity statistics.
+struct AdjointSums {
+  double sq = 0;
+  double grad[REG_PARAMS] = {};
+  double jtj[REG_GN * REG_GN] = {};
+  double stats[5] = {};
+};
 static void adjoint_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, const RegModel &model,
-                         const float *params, float alpha, float beta, const float *d_dl, double *sq, double *grad) {
+                         const float *params, float alpha, float beta, const float *d_dl, const float2 *d_lens,
+                         AdjointSums &out) {
   LdpcEngine &e = *d->eng;
   RegState &st = d->reg_state_host;
   for (int k = 0; k < 8; k++) st.H[k] = params[k];
@@ -2200,14 +2208,15 @@ static void adjoint_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf
   st.beta = beta;
   st.dk1 = 0.0f;
   hip_check(hipMemcpyAsync(d->reg_state.p, &st, sizeof st, hipMemcpyHostToDevice, e.stream), "state");
-  launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), nullptr, d->reg_part.as<float>(),
+  launch_reg_accum(p, wf, d->reg_state.as<RegState>(), d->reg_ref.as<float>(), d_lens, d->reg_part.as<float>(),
                    e.stream, d_dl);
   const int nx = p.dev.nx, c = nx - 1;
   d->reg_part_host.resize(static_cast<size_t>(c) * c * REG_SPLIT * REG_PART);
   hip_check(hipMemcpyAsync(d->reg_part_host.data(), d->reg_part.p, d->reg_part_host.size() * sizeof(float),
                            hipMemcpyDeviceToHost, e.stream), "partial sums");
   e.wait("refinement pass");
-  double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  constexpr int JTJ0 = 1 + REG_GN, STAT0 = JTJ0 + REG_GN * (REG_GN + 1) / 2;
+  double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0}, up[REG_GN * (REG_GN + 1) / 2] = {}, stats[5] = {};
   std::vector<double> G(static_cast<size_t>(nx) * nx * 2, 0.0);
   for (int row = 0; row < c * REG_SPLIT; row++)
     for (int ci = 0; ci < c; ci++) {
@@ -2215,15 +2224,20 @@ static void adjoint_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf
       const float *q = &d->reg_part_host[(static_cast<size_t>(row) * c + ci) * REG_PART];
       L += q[0];
       for (int k = 0; k < 8; k++) gH[k] += q[1 + k];
+      for (int k = 0; k < REG_GN * (REG_GN + 1) / 2; k++) up[k] += q[JTJ0 + k];
+      for (int k = 0; k < 5; k++) stats[k] += q[STAT0 + k];
       const int corner[4] = {cj * nx + ci, cj * nx + ci + 1, (cj + 1) * nx + ci, (cj + 1) * nx + ci + 1};
       for (int k = 0; k < 4; k++) {
         G[2 * corner[k]] += 2.0 * q[REG_NODE0 + 2 * k];
         G[2 * corner[k] + 1] += 2.0 * q[REG_NODE0 + 1 + 2 * k];
       }
     }
-  *sq = L;
-  for (int k = 0; k < 8; k++) grad[k] = 2.0 * gH[k];
-  reg_chain(model, G.data(), grad);
+  out.sq = L;
+  for (int k = 0; k < 8; k++) out.grad[k] = 2.0 * gH[k];
+  reg_chain(model, G.data(), out.grad);
+  for (int k = 0, q = 0; k < REG_GN; k++)
+    for (int l = k; l < REG_GN; l++, q++) out.jtj[k * REG_GN + l] = out.jtj[l * REG_GN + k] = up[q];
+  for (int k = 0; k < 5; k++) out.stats[k] = stats[k];
 }
 
 LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
@@ -2236,15 +2250,19 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
     const FramePlan &p = d->plan(version, span);
     RegModel model;
     const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
-    adjoint_pass(d, p, wf, model, params, alpha, beta, nullptr, loss, grad);
+    AdjointSums a;
+    adjoint_pass(d, p, wf, model, params, alpha, beta, nullptr, nullptr, a);
+    *loss = a.sq;
+    std::copy(a.grad, a.grad + REG_PARAMS, grad);
     return 0;
   });
 }
 
 // The spectral loss on a staged refinement: the warped picture, its residual's masked coefficients and their
-// inverse transform (launch_spectral_adjoint), then the warp's adjoint of that derivative.
+// inverse transform (launch_spectral_adjoint), then the warp's adjoint of that derivative; the mask already on the
+// device (d->spec_mask).
 static double spectral_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, const RegModel &model,
-                            const float *params, const uint8_t *mask, float alpha, float beta, double *grad) {
+                            const float *params, float alpha, float beta, const float2 *d_lens, AdjointSums &out) {
   LdpcEngine &e = *d->eng;
   const size_t nn = static_cast<size_t>(p.n) * p.n, S = static_cast<size_t>(p.subch) * SUB,
                X = static_cast<size_t>(p.E) * p.n;
@@ -2253,9 +2271,7 @@ static double spectral_pass(liz_decoder *d, const FramePlan &p, const WarpFrame 
   d->spec_X.reserve(X * sizeof(float2));
   d->spec_G.reserve(X * sizeof(float2));
   d->spec_C.reserve(S * sizeof(float2));
-  d->spec_mask.reserve(S);
   d->spec_part.reserve(((S + 255) / 256) * sizeof(double));
-  hip_check(hipMemcpyAsync(d->spec_mask.p, mask, S, hipMemcpyHostToDevice, e.stream), "coefficient mask");
   launch_warp(p, wf, 1, d->spec_w.as<float>(), e.stream);
   const int parts = launch_spectral_adjoint(p, d->spec_w.as<float>(), d->reg_ref.as<float>(), alpha, beta,
                                             d->spec_mask.as<uint8_t>(), d->spec_X.as<float2>(),
@@ -2264,11 +2280,16 @@ static double spectral_pass(liz_decoder *d, const FramePlan &p, const WarpFrame 
   d->spec_part_host.resize(parts);
   hip_check(hipMemcpyAsync(d->spec_part_host.data(), d->spec_part.p, parts * sizeof(double), hipMemcpyDeviceToHost,
                            e.stream), "loss parts");
-  double sq = 0;
-  adjoint_pass(d, p, wf, model, params, alpha, beta, d->spec_dl.as<float>(), &sq, grad);  // waits for the stream
+  adjoint_pass(d, p, wf, model, params, alpha, beta, d->spec_dl.as<float>(), d_lens, out);  // waits for the stream
   double L = 0;
   for (int k = 0; k < parts; k++) L += d->spec_part_host[k];
   return 2.0 * L / (static_cast<double>(p.n) * p.n);
+}
+
+static void upload_mask(liz_decoder *d, const FramePlan &p, const uint8_t *mask) {
+  const size_t S = static_cast<size_t>(p.subch) * SUB;
+  d->spec_mask.reserve(S);
+  hip_check(hipMemcpyAsync(d->spec_mask.p, mask, S, hipMemcpyHostToDevice, d->eng->stream), "coefficient mask");
 }
 
 LIZ_API int lizx_reg_loss_grad_spectral(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h,
@@ -2281,7 +2302,117 @@ LIZ_API int lizx_reg_loss_grad_spectral(liz_decoder *d, int version, int span, c
     const FramePlan &p = d->plan(version, span);
     RegModel model;
     const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
-    *loss = spectral_pass(d, p, wf, model, params, mask, alpha, beta, grad);
+    upload_mask(d, p, mask);
+    AdjointSums a;
+    *loss = spectral_pass(d, p, wf, model, params, alpha, beta, nullptr, a);
+    std::copy(a.grad, a.grad + REG_PARAMS, grad);
+    return 0;
+  });
+}
+
+// The staged registration moved to new parameters: the model's nodes for them and H (the capture stays staged).
+static void restage(liz_decoder *d, const RegModel &model, const float *params) {
+  LdpcEngine &e = *d->eng;
+  reg_nodes(model, params, d->reg_nodes);
+  hip_check(hipMemcpyAsync(d->nodes.p, d->reg_nodes.data(), d->reg_nodes.size() * sizeof(float), hipMemcpyHostToDevice,
+                           e.stream), "nodes");
+  for (int k = 0; k < 8; k++) d->wframe_host.H[k] = params[k];
+  d->wframe_host.H[8] = 1.0f;
+  hip_check(hipMemcpyAsync(d->wframe.p, &d->wframe_host, sizeof(WarpFrame), hipMemcpyHostToDevice, e.stream),
+            "registration");
+}
+
+// (A + damping diag A) x = b for the 9 x 9 symmetric positive A, Jacobi-scaled, by Cholesky in FP64 (host).
+static bool solve9(const double *A0, const double *b0, double damping, double *x) {
+  constexpr int N = REG_GN;
+  double A[N][N], b[N], sc[N];
+  for (int k = 0; k < N; k++) sc[k] = A0[k * N + k] > 0 ? 1.0 / std::sqrt(A0[k * N + k] * (1.0 + damping)) : 1.0;
+  for (int k = 0; k < N; k++) {
+    for (int l = 0; l < N; l++) A[k][l] = A0[k * N + l] * (k == l ? 1.0 + damping : 1.0) * sc[k] * sc[l];
+    b[k] = b0[k] * sc[k];
+  }
+  for (int j = 0; j < N; j++) {
+    double dd = A[j][j];
+    for (int k = 0; k < j; k++) dd -= A[j][k] * A[j][k];
+    if (!(dd > 0)) return false;
+    A[j][j] = std::sqrt(dd);
+    for (int i = j + 1; i < N; i++) {
+      double v = A[i][j];
+      for (int k = 0; k < j; k++) v -= A[i][k] * A[j][k];
+      A[i][j] = v / A[j][j];
+    }
+  }
+  for (int i = 0; i < N; i++) {
+    double v = b[i];
+    for (int k = 0; k < i; k++) v -= A[i][k] * b[k];
+    b[i] = v / A[i][i];
+  }
+  for (int i = N - 1; i >= 0; i--) {
+    double v = b[i];
+    for (int k = i + 1; k < N; k++) v -= A[k][i] * x[k];
+    x[i] = v / A[i][i];
+  }
+  for (int k = 0; k < N; k++) x[k] *= sc[k];
+  return true;
+}
+
+LIZ_API int lizx_reg_refine_spectral(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h,
+                                     int stride, const float *H0, float *params, const float *ref,
+                                     const uint8_t *mask, int steps, float *trace) {
+  return guard([&] {
+    need(d && params && mask && steps >= 0 && steps <= 100, "a decoder, parameters, a mask and 0 to 100 steps");
+    LdpcEngine &e = *d->eng;
+    hip_check(hipSetDevice(e.device), "hipSetDevice");
+    const FramePlan &p = d->plan(version, span);
+    RegModel model;
+    const WarpFrame *wf = stage_refinement(d, p, img, w, h, stride, H0, params, ref, model);
+    upload_mask(d, p, mask);
+    std::vector<float> lf(model.lens.size());
+    for (size_t k = 0; k < lf.size(); k++) lf[k] = static_cast<float>(model.lens[k]);
+    d->reg_lens.reserve(lf.size() * sizeof(float));
+    hip_check(hipMemcpyAsync(d->reg_lens.p, lf.data(), lf.size() * sizeof(float), hipMemcpyHostToDevice, e.stream),
+              "lens field");
+    const float2 *d_lens = d->reg_lens.as<float2>();
+    const size_t S = static_cast<size_t>(p.subch) * SUB;
+    size_t on = 0;
+    for (size_t k = 0; k < S; k++) on += mask[k] != 0;
+    // the loss's curvature is about the masked share of the picture's: the first trial step is the spatial Gauss-
+    // Newton step over that share, halved until the loss falls
+    const double share = std::max(0.05, static_cast<double>(on) / static_cast<double>(S));
+    std::vector<float> cur(params, params + REG_PARAMS), trial(REG_PARAMS);
+    AdjointSums a, b;
+    // the intensity fit from a first pass (the statistics do not depend on alpha and beta), held after
+    (void)spectral_pass(d, p, wf, model, cur.data(), 1.0f, 0.0f, d_lens, a);
+    const double sw = a.stats[0], sr = a.stats[1], swr = a.stats[2], srr = a.stats[3], cnt = a.stats[4];
+    const double var = srr - sr * sr / std::max(cnt, 1.0);
+    const float alpha = var > 0 ? static_cast<float>((swr - sw * sr / cnt) / var) : 1.0f;
+    const float beta = cnt > 0 ? static_cast<float>((sw - alpha * sr) / cnt) : 0.0f;
+    double L = spectral_pass(d, p, wf, model, cur.data(), alpha, beta, d_lens, a);
+    if (trace) trace[0] = static_cast<float>(L);
+    int s = 0;
+    for (; s < steps; s++) {
+      double rhs[REG_GN], dx[REG_GN];
+      for (int k = 0; k < REG_GN; k++) rhs[k] = -0.5 * a.grad[k];
+      if (!solve9(a.jtj, rhs, 1e-6, dx)) break;
+      bool moved = false;
+      for (double t = 1.0 / share; t > 1.0 / (share * 64.0); t *= 0.5) {
+        trial = cur;
+        for (int k = 0; k < REG_GN; k++) trial[k] = static_cast<float>(cur[k] + t * dx[k]);
+        restage(d, model, trial.data());
+        const double Lt = spectral_pass(d, p, wf, model, trial.data(), alpha, beta, d_lens, b);
+        if (Lt < L) {
+          cur = trial;
+          a = b;
+          L = Lt;
+          moved = true;
+          break;
+        }
+      }
+      if (trace) trace[s + 1] = static_cast<float>(L);
+      if (!moved) break;
+    }
+    for (int k = s + 1; trace && k <= steps; k++) trace[k] = static_cast<float>(L);
+    std::copy(cur.begin(), cur.end(), params);
     return 0;
   });
 }

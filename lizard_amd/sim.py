@@ -74,6 +74,18 @@ def remodulate(version, blocks, known, cnt=0, blur=0.0):
     return (0.5 + p / (4.0 * math.sqrt(640.0 * subch))).to(torch.float32).numpy()
 
 
+def known_mask(version, known):
+    """The coefficients remodulate() re-modulates (the blocks known[b] and the padding past the last block), one
+    byte a coefficient in coefficient order: the mask of the spectral refinement loss."""
+    info = format_info(version)
+    keep = np.ones(info["subch"] * 320, np.uint8)
+    _, s0, ns = format_blocks(version)
+    for b in range(len(s0)):
+        if not known[b]:
+            keep[s0[b] // 2:(s0[b] + ns[b]) // 2] = 0
+    return keep
+
+
 def homography(src, dst):
     """The 3 x 3 projective map (h8 = 1) taking 4 points src to dst (float64 Gauss-Jordan on the 8 x 8 system)."""
     A, b = np.zeros((8, 8)), np.zeros(8)

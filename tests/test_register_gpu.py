@@ -165,6 +165,25 @@ def test_spectral_autograd_gradcheck(decoder_gpu, case384):
     assert torch.autograd.gradcheck(SpecLoss.apply, (delta,), eps=0.02, atol=0.5, rtol=0.05, nondet_tol=1e-3)
 
 
+def test_spectral_refine_recovers_the_registration(decoder_gpu):
+    """Refinement on the spectral loss (lizx_reg_refine_spectral) from corners 1.5 px off and no lens, over the
+    coefficients of a third of the blocks (the decoded ones in a decision-directed round; the rest unknown): the loss
+    falls step by step, the corners come within 0.3 px of the truth and k1 toward the camera's."""
+    B = L.format_info(20)["blocks"]
+    known = np.zeros(B, bool)
+    known[::3] = True
+    c = make_case(20, 256, 9, k1=0.04, known=known)
+    mask = sim.known_mask(20, known)
+    p = L.reg_params(c["H0"], 0.0)
+    e0 = corner_error(c["Ht"], p, c["span"])
+    p1, tr = decoder_gpu.reg_refine_spectral(20, 256, c["img"], c["H0"], p, c["ref"], mask, steps=12)
+    assert np.all(np.diff(tr) <= 0) and tr[-1] < 0.7 * tr[0], tr
+    e1 = corner_error(c["Ht"], p1, c["span"])
+    assert e1 < 0.3 < e0, (e0, e1)
+    assert abs(p1[8] - 0.04) < abs(0.0 - 0.04), p1[8]
+    assert np.array_equal(p1[9:], p[9:])  # the anchors held
+
+
 def test_refine_recovers_the_homography(decoder_gpu):
     c = make_case(20, 256, 6)
     p = L.reg_params(c["H0"], c["k1"])
