@@ -682,10 +682,11 @@ def cpu_sender(work, seconds):
                       (state["frames"], len(items), wall, cores)}
 
 
-def refine_frames(count, seed):
+def refine_frames(count, seed, k2=0.0):
     """count captures of LIZARD-320 (n = 768) in the 128 ring, the whole symbol as a page shows it (border, word,
     anchors) through random cameras (sender and channel on the GPU), each with a registration whose homography corners
-    are up to 1.5 px off (the lens term known)."""
+    are up to 1.5 px off (the lens term known); with k2, a second radial term drawn in [-k2, k2] (which the registration
+    model's H and k1 cannot hold)."""
     from lizard_amd import sim
     rng = np.random.default_rng(seed)
     dec = lizard_amd.Decoder(0)
@@ -694,10 +695,11 @@ def refine_frames(count, seed):
     out = []
     for k in range(count):
         cam = sim.Camera.random(rng, scale=(0.75, 0.9))
+        kk2 = float(rng.uniform(-k2, k2)) if k2 else 0.0
         blocks = sim.random_blocks(20, rng)
         dpic = dec.modulate(20, blocks)
         dimg = dec.capture(20, 256, dpic, cam.W, cam.Hh, cam.H(256), k1=cam.k1, blur=cam.blur, noise=cam.noise,
-                           seed=seed + k, fps=30)
+                           seed=seed + k, fps=30, k2=kk2)
         img = dimg.get(np.uint8, cam.W * cam.Hh).reshape(cam.Hh, cam.W)
         dimg.free()
         dpic.free()
@@ -706,9 +708,67 @@ def refine_frames(count, seed):
         w = cs @ Ht[:, :2].T + Ht[:, 2]
         H0 = sim.homography(cs, w[:, :2] / w[:, 2:] + rng.uniform(-1.5, 1.5, (4, 2)))
         scale = np.sqrt(abs(np.linalg.det(Ht[:2, :2]))) * 256 / 768  # capture pixels a picture sample
-        out.append(dict(img=img, H0=H0, k1=cam.k1, blocks=blocks, blur=np.sqrt(cam.blur ** 2 + 1 / 3) / scale))
+        out.append(dict(img=img, H0=H0, k1=cam.k1, k2=kk2, blocks=blocks, blur=np.sqrt(cam.blur ** 2 + 1 / 3) / scale))
     dec.close()
     return out
+
+
+def refine_two_terms(dec, count, B):
+    """Captures behind a lens of two radial terms (k1 in [-0.05, 0.05], k2 in [-0.06, 0.06]) registered by H (corners
+    1.5 px off) and k1 alone, then three decision-directed rounds of the spectral refinement over H and k1, and over
+    H, k1 and the anchors' offsets (which carry what k2 adds); block-fail after each round."""
+    from lizard_amd import sim
+    frames = refine_frames(count, 919, k2=0.06)
+    D = lizard_amd.DeviceArray
+    sent = np.concatenate([fr["blocks"].reshape(-1, 476) for fr in frames])
+    keep = []
+
+    def decode(ps):
+        items = []
+        for fr, p in zip(frames, ps):
+            nodes = lizard_amd.reg_nodes(256, fr["H0"].astype(np.float32), fr["img"].shape[1], fr["img"].shape[0], p)
+            d_img = fr.get("d_img") or D.of(fr["img"])
+            fr["d_img"] = d_img
+            d_nodes = D.of(nodes)
+            keep.append(d_nodes)
+            items.append(dict(img=d_img, w=fr["img"].shape[1], h=fr["img"].shape[0],
+                              H=np.append(p[:8], 1.0).reshape(3, 3), nodes=d_nodes))
+        b = lizard_amd.Batch(0).plan([20] * len(frames), [256] * len(frames))
+        b.run(lizard_amd.make_frames(items), lizard_amd.RUN_GRAPH)
+        b.sync()
+        out = b.blocks_out()
+        b.close()
+        return out
+
+    def fail(out):
+        return round(1.0 - float(((out["ok"] > 0) & np.all(out["blocks"] == sent, axis=1)).mean()), 5)
+
+    p0 = [lizard_amd.reg_params(fr["H0"], fr["k1"]) for fr in frames]
+    out0 = decode(p0)
+    res = {"frames": len(frames), "block_fail_before": fail(out0)}
+    for anchors, name in ((False, "H_and_k1"), (True, "H_k1_and_anchors")):
+        ps = [p.copy() for p in p0]
+        out, path = out0, []
+        t0 = time.perf_counter()
+        for rnd in range(3):
+            ok = out["ok"].reshape(len(frames), B).astype(bool)
+            for k, fr in enumerate(frames):
+                ref_pic = sim.remodulate(20, fr["blocks"], ok[k], 0, fr["blur"])
+                ps[k], _ = dec.reg_refine_spectral(20, 256, fr["img"], fr["H0"], ps[k], ref_pic,
+                                                   sim.known_mask(20, ok[k]), steps=8, anchors=anchors)
+            out = decode(ps)
+            path.append(fail(out))
+        res["block_fail_after_" + name] = path
+        res["ms_per_frame_round_" + name] = round((time.perf_counter() - t0) / (3 * len(frames)) * 1e3, 3)
+    for fr in frames:
+        if fr.get("d_img"):
+            fr["d_img"].free()
+    for d in keep:
+        d.free()
+    res["what"] = ("the registration knows H (corners 1.5 px off) and the camera's k1 but not its k2; decision-directed "
+                   "rounds of lizx_reg_refine_spectral (8 steps) over H and k1, or over H, k1 and the 49 anchors' "
+                   "offsets; block-fail after each round")
+    return res
 
 
 def bench_small(args):
@@ -1177,6 +1237,7 @@ def bench_refine(args):
                             "together (lizx_reg_refine_k1), or over H and k1 on the loss restricted to the verified "
                             "blocks' coefficients through the FFT's adjoint (lizx_reg_refine_spectral, 8 steps with "
                             "a line search); block-fail after each round")
+    two_terms = refine_two_terms(dec, min(F, 64), B)
     # the plain anchor registration (the finder on each capture, from the anchors outward: over its own border nodes,
     # and the reference's chain with the border refined by its templates) and the same Gauss-Newton steps from it, the
     # nodes held, on the re-modulation of what it verified
@@ -1228,6 +1289,7 @@ def bench_refine(args):
             "loss_ratio_median": round(float(np.median(tr[:, -1] / tr[:, 0])), 4),
             "from_anchor_registration": anchor_legs,
             "lens_unknown": lens_unknown,
+            "lens_two_terms": two_terms,
             "refine_ms_per_frame": round(refine_s * 1e3 / F, 3),
             "generation_s": round(gen_s, 1)}
 

@@ -480,14 +480,17 @@ LIZ_API int lizx_reg_loss_grad(liz_decoder *d, int version, int span, const uint
 LIZ_API int lizx_reg_loss_grad_spectral(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h,
                                         int stride, const float *H0, const float *params, const float *ref,
                                         const uint8_t *mask, float alpha, float beta, double *loss, double *grad);
-/* Refinement of h0..h7 and k1 (params[0..8], in and out; the anchors held) on the spectral loss of
-   lizx_reg_loss_grad_spectral, the intensity fit (alpha, beta) taken once from the first pass: each step the
-   Gauss-Newton direction of the warp's J^T J (FP64 on the host) on that loss's gradient, the step halved from the
-   masked share's full step until the loss falls (at most 6 halvings; none falling ends the steps). trace (optional,
-   steps + 1 floats): the loss before the first step and after each. No counterpart in the reference. */
+/* Refinement of h0..h7 and k1 (params[0..8], in and out) and, with anchors nonzero, the 49 anchors' offsets
+   (params[9..106]) on the spectral loss of lizx_reg_loss_grad_spectral, the intensity fit (alpha, beta) taken once
+   from the first pass. Each step: the Gauss-Newton direction of the warp's J^T J over h0..h7 and k1 (FP64 on the
+   host) on that loss's gradient, then, with anchors, the anchors' direction under the nodes' metric carried through
+   the anchors' mesh (each node's 2 x 2 block of the warp's J^T J; a 98 x 98 FP64 solve on the host); each direction's
+   step halved from the masked share's full step until the loss falls (at most 6 halvings; a step where neither falls
+   ends the refinement). trace (optional, steps + 1 floats): the loss before the first step and after each. No
+   counterpart in the reference. */
 LIZ_API int lizx_reg_refine_spectral(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h,
                                      int stride, const float *H0, float *params, const float *ref,
-                                     const uint8_t *mask, int steps, float *trace);
+                                     const uint8_t *mask, int steps, int anchors, float *trace);
 /* steps Gauss-Newton steps over h0..h7 on the GPU (k1 and the anchors held; alpha and beta refitted each pass; the
    damped 8 x 8 normal equations solved in FP64 on the device); params' h0..h7 updated in place; trace (optional,
    steps + 1 floats) receives the loss before each step and after the last. */

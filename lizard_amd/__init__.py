@@ -134,7 +134,7 @@ def lib():
     sig("lizx_reg_nodes", _I, _I, _P, _I, _I, _P, _P)
     sig("lizx_reg_loss_grad", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _F, _F, _P, _P)
     sig("lizx_reg_loss_grad_spectral", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _P, _F, _F, _P, _P)
-    sig("lizx_reg_refine_spectral", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _P, _I, _P)
+    sig("lizx_reg_refine_spectral", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _P, _I, _I, _P)
     sig("lizx_reg_refine", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_reg_refine_k1", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
     sig("lizx_reg_refine_nodes", _I, _P, _I, _I, _P, _I, _I, _I, _P, _P, _P, _I, _P)
@@ -899,16 +899,17 @@ class Decoder:
                                                 _p(grad)))
         return loss.value, grad
 
-    def reg_refine_spectral(self, version, span, img, H0, params, ref, mask, steps=10):
-        """Refinement of h0..h7 and k1 on the spectral loss (lizx_reg_refine_spectral): the new parameters and the
-        loss trace."""
+    def reg_refine_spectral(self, version, span, img, H0, params, ref, mask, steps=10, anchors=False):
+        """Refinement of h0..h7 and k1 (and with anchors the 49 anchors' offsets) on the spectral loss
+        (lizx_reg_refine_spectral): the new parameters and the loss trace."""
         img = np.ascontiguousarray(img, np.uint8)
         p = np.array(params, np.float32)
         trace = np.zeros(steps + 1, np.float32)
         check(lib().lizx_reg_refine_spectral(self.h, version, span, _p(img), img.shape[1], img.shape[0], img.shape[1],
                                              _p(np.ascontiguousarray(H0, np.float32).ravel()), _p(p),
                                              _p(np.ascontiguousarray(ref, np.float32)),
-                                             _p(np.ascontiguousarray(mask, np.uint8)), steps, _p(trace)))
+                                             _p(np.ascontiguousarray(mask, np.uint8)), steps, int(anchors),
+                                             _p(trace)))
         return p, trace
 
     def reg_refine(self, version, span, img, H0, params, ref, steps=10, k1=False):

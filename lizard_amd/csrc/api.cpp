@@ -2196,6 +2196,7 @@ struct AdjointSums {
   double grad[REG_PARAMS] = {};
   double jtj[REG_GN * REG_GN] = {};
   double stats[5] = {};
+  std::vector<double> met;  // each node's share of the warp's metric (gx^2, gx gy, gy^2 weighted), nx * nx * 3
 };
 static void adjoint_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf, const RegModel &model,
                          const float *params, float alpha, float beta, const float *d_dl, const float2 *d_lens,
@@ -2218,6 +2219,7 @@ static void adjoint_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf
   constexpr int JTJ0 = 1 + REG_GN, STAT0 = JTJ0 + REG_GN * (REG_GN + 1) / 2;
   double L = 0, gH[8] = {0, 0, 0, 0, 0, 0, 0, 0}, up[REG_GN * (REG_GN + 1) / 2] = {}, stats[5] = {};
   std::vector<double> G(static_cast<size_t>(nx) * nx * 2, 0.0);
+  out.met.assign(static_cast<size_t>(nx) * nx * 3, 0.0);
   for (int row = 0; row < c * REG_SPLIT; row++)
     for (int ci = 0; ci < c; ci++) {
       const int cj = row / REG_SPLIT;
@@ -2230,6 +2232,7 @@ static void adjoint_pass(liz_decoder *d, const FramePlan &p, const WarpFrame *wf
       for (int k = 0; k < 4; k++) {
         G[2 * corner[k]] += 2.0 * q[REG_NODE0 + 2 * k];
         G[2 * corner[k] + 1] += 2.0 * q[REG_NODE0 + 1 + 2 * k];
+        for (int m = 0; m < 3; m++) out.met[3 * corner[k] + m] += q[REG_MET0 + 3 * k + m];
       }
     }
   out.sq = L;
@@ -2322,43 +2325,70 @@ static void restage(liz_decoder *d, const RegModel &model, const float *params) 
             "registration");
 }
 
-// (A + damping diag A) x = b for the 9 x 9 symmetric positive A, Jacobi-scaled, by Cholesky in FP64 (host).
-static bool solve9(const double *A0, const double *b0, double damping, double *x) {
-  constexpr int N = REG_GN;
-  double A[N][N], b[N], sc[N];
-  for (int k = 0; k < N; k++) sc[k] = A0[k * N + k] > 0 ? 1.0 / std::sqrt(A0[k * N + k] * (1.0 + damping)) : 1.0;
-  for (int k = 0; k < N; k++) {
-    for (int l = 0; l < N; l++) A[k][l] = A0[k * N + l] * (k == l ? 1.0 + damping : 1.0) * sc[k] * sc[l];
+// (A + damping diag A) x = b for the n x n symmetric positive A (row-major), Jacobi-scaled, by Cholesky in FP64
+// (host); false where A is not positive.
+static bool solve_spd(const double *A0, const double *b0, int n, double damping, double *x) {
+  std::vector<double> A(static_cast<size_t>(n) * n), b(n), sc(n);
+  for (int k = 0; k < n; k++) {
+    const double dg = A0[static_cast<size_t>(k) * n + k];
+    sc[k] = dg > 0 ? 1.0 / std::sqrt(dg * (1.0 + damping)) : 1.0;
+  }
+  for (int k = 0; k < n; k++) {
+    for (int l = 0; l < n; l++)
+      A[static_cast<size_t>(k) * n + l] = A0[static_cast<size_t>(k) * n + l] * (k == l ? 1.0 + damping : 1.0) * sc[k] * sc[l];
     b[k] = b0[k] * sc[k];
   }
-  for (int j = 0; j < N; j++) {
-    double dd = A[j][j];
-    for (int k = 0; k < j; k++) dd -= A[j][k] * A[j][k];
+  auto at = [&](int i, int j) -> double & { return A[static_cast<size_t>(i) * n + j]; };
+  for (int j = 0; j < n; j++) {
+    double dd = at(j, j);
+    for (int k = 0; k < j; k++) dd -= at(j, k) * at(j, k);
     if (!(dd > 0)) return false;
-    A[j][j] = std::sqrt(dd);
-    for (int i = j + 1; i < N; i++) {
-      double v = A[i][j];
-      for (int k = 0; k < j; k++) v -= A[i][k] * A[j][k];
-      A[i][j] = v / A[j][j];
+    at(j, j) = std::sqrt(dd);
+    for (int i = j + 1; i < n; i++) {
+      double v = at(i, j);
+      for (int k = 0; k < j; k++) v -= at(i, k) * at(j, k);
+      at(i, j) = v / at(j, j);
     }
   }
-  for (int i = 0; i < N; i++) {
+  for (int i = 0; i < n; i++) {
     double v = b[i];
-    for (int k = 0; k < i; k++) v -= A[i][k] * b[k];
-    b[i] = v / A[i][i];
+    for (int k = 0; k < i; k++) v -= at(i, k) * b[k];
+    b[i] = v / at(i, i);
   }
-  for (int i = N - 1; i >= 0; i--) {
+  for (int i = n - 1; i >= 0; i--) {
     double v = b[i];
-    for (int k = i + 1; k < N; k++) v -= A[k][i] * x[k];
-    x[i] = v / A[i][i];
+    for (int k = i + 1; k < n; k++) v -= at(k, i) * x[k];
+    x[i] = v / at(i, i);
   }
-  for (int k = 0; k < N; k++) x[k] *= sc[k];
+  for (int k = 0; k < n; k++) x[k] *= sc[k];
   return true;
+}
+
+// The anchors' metric from the nodes' (each node's 2 x 2 block; the blocks between neighbouring nodes left out):
+// A[(a, c), (b, d)] = sum over nodes of phi_a phi_b M_node[c][d], 98 x 98.
+static void anchor_metric(const RegModel &m, const std::vector<double> &met, std::vector<double> &A) {
+  constexpr int NA = REG_ANCH * REG_ANCH, NP = 2 * NA;
+  A.assign(static_cast<size_t>(NP) * NP, 0.0);
+  for (int k = 0; k < m.nx * m.nx; k++) {
+    const double *ph = &m.phi[static_cast<size_t>(k) * NA];
+    const double mxx = met[3 * k], mxy = met[3 * k + 1], myy = met[3 * k + 2];
+    for (int a = 0; a < NA; a++) {
+      if (ph[a] == 0) continue;
+      for (int b = 0; b < NA; b++) {
+        const double w = ph[a] * ph[b];
+        if (w == 0) continue;
+        A[(2 * a) * NP + 2 * b] += w * mxx;
+        A[(2 * a) * NP + 2 * b + 1] += w * mxy;
+        A[(2 * a + 1) * NP + 2 * b] += w * mxy;
+        A[(2 * a + 1) * NP + 2 * b + 1] += w * myy;
+      }
+    }
+  }
 }
 
 LIZ_API int lizx_reg_refine_spectral(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h,
                                      int stride, const float *H0, float *params, const float *ref,
-                                     const uint8_t *mask, int steps, float *trace) {
+                                     const uint8_t *mask, int steps, int anchors, float *trace) {
   return guard([&] {
     need(d && params && mask && steps >= 0 && steps <= 100, "a decoder, parameters, a mask and 0 to 100 steps");
     LdpcEngine &e = *d->eng;
@@ -2389,24 +2419,38 @@ LIZ_API int lizx_reg_refine_spectral(liz_decoder *d, int version, int span, cons
     const float beta = cnt > 0 ? static_cast<float>((sw - alpha * sr) / cnt) : 0.0f;
     double L = spectral_pass(d, p, wf, model, cur.data(), alpha, beta, d_lens, a);
     if (trace) trace[0] = static_cast<float>(L);
-    int s = 0;
-    for (; s < steps; s++) {
-      double rhs[REG_GN], dx[REG_GN];
-      for (int k = 0; k < REG_GN; k++) rhs[k] = -0.5 * a.grad[k];
-      if (!solve9(a.jtj, rhs, 1e-6, dx)) break;
-      bool moved = false;
+    // a step along dx over parameters [first, first + n): the step halved from the masked share's full one until
+    // the loss falls (at most 6 halvings)
+    auto try_step = [&](int first, int n, const double *dx) {
       for (double t = 1.0 / share; t > 1.0 / (share * 64.0); t *= 0.5) {
         trial = cur;
-        for (int k = 0; k < REG_GN; k++) trial[k] = static_cast<float>(cur[k] + t * dx[k]);
+        for (int k = 0; k < n; k++) trial[first + k] = static_cast<float>(cur[first + k] + t * dx[k]);
         restage(d, model, trial.data());
         const double Lt = spectral_pass(d, p, wf, model, trial.data(), alpha, beta, d_lens, b);
         if (Lt < L) {
           cur = trial;
           a = b;
           L = Lt;
-          moved = true;
-          break;
+          return true;
         }
+      }
+      restage(d, model, cur.data());
+      return false;
+    };
+    constexpr int NA2 = 2 * REG_ANCH * REG_ANCH;
+    std::vector<double> AA;
+    int s = 0;
+    for (; s < steps; s++) {
+      bool moved = false;
+      double rhs[REG_GN], dx[REG_GN];
+      for (int k = 0; k < REG_GN; k++) rhs[k] = -0.5 * a.grad[k];
+      if (solve_spd(a.jtj, rhs, REG_GN, 1e-6, dx)) moved |= try_step(0, REG_GN, dx);
+      if (anchors) {
+        // the anchors by the nodes' metric through the mesh, H and k1 held
+        double ra[NA2], da[NA2];
+        for (int k = 0; k < NA2; k++) ra[k] = -0.5 * a.grad[REG_GN + k];
+        anchor_metric(model, a.met, AA);
+        if (solve_spd(AA.data(), ra, NA2, 1e-3, da)) moved |= try_step(REG_GN, NA2, da);
       }
       if (trace) trace[s + 1] = static_cast<float>(L);
       if (!moved) break;

@@ -17,8 +17,8 @@ namespace lz {
 
 namespace {
 
-constexpr int NG = REG_GN, JTJ0 = 1 + NG, STAT0 = JTJ0 + NG * (NG + 1) / 2, NODE0 = STAT0 + 5;
-static_assert(NODE0 + 8 == REG_PART, "the partial sums' layout");
+constexpr int NG = REG_GN, JTJ0 = 1 + NG, STAT0 = JTJ0 + NG * (NG + 1) / 2, NODE0 = STAT0 + 5, MET0 = NODE0 + 8;
+static_assert(NODE0 == REG_NODE0 && MET0 == REG_MET0 && MET0 + 12 == REG_PART, "the partial sums' layout");
 
 __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *frame, const RegState *st,
                                                    const float *ref, const float2 *lens, float *part,
@@ -144,6 +144,15 @@ __global__ __launch_bounds__(256) void k_reg_accum(PlanDev P, const WarpFrame *f
       acc[NODE0 + 5] += ey * w01[j];
       acc[NODE0 + 6] += ex * w11[j];
       acc[NODE0 + 7] += ey * w11[j];
+      // the corners' shares of the warp's metric (the diagonal blocks of J^T J over the node displacements)
+      const float gxx = gx * gx, gxy = gx * gy, gyy = gy * gy;
+      const float wq[4] = {w00[j] * w00[j], w10[j] * w10[j], w01[j] * w01[j], w11[j] * w11[j]};
+#pragma unroll
+      for (int a = 0; a < 4; a++) {
+        acc[MET0 + 3 * a] += wq[a] * gxx;
+        acc[MET0 + 3 * a + 1] += wq[a] * gxy;
+        acc[MET0 + 3 * a + 2] += wq[a] * gyy;
+      }
     }
   }
   const int lane = tid & 63, w = tid >> 6;

@@ -20,8 +20,9 @@ constexpr int REG_PARAMS = 8 + 1 + 2 * REG_ANCH * REG_ANCH; // homography, k1, a
 // Anchor square i's first module on either axis in ring span (squares span / 64 modules a side).
 int anchor_m0(int span, int i);
 constexpr int REG_GN = 9;                                   // Gauss-Newton parameters: h0..h7 and k1
-constexpr int REG_PART = 1 + REG_GN + REG_GN * (REG_GN + 1) / 2 + 5 + 8;  // floats of partial sums a piece of a cell
-constexpr int REG_NODE0 = REG_PART - 8;                     // where a piece's node gradient (4 corners x 2) starts
+constexpr int REG_NODE0 = 1 + REG_GN + REG_GN * (REG_GN + 1) / 2 + 5;  // a piece's node gradient (4 corners x 2)
+constexpr int REG_MET0 = REG_NODE0 + 8;                     // a piece's node metric (4 corners x gx^2, gx gy, gy^2)
+constexpr int REG_PART = REG_MET0 + 12;                     // floats of partial sums a piece of a cell
 constexpr int REG_SPLIT = 1;                                // pieces a node cell's rows are split into
 
 // The node lattice's displacements as a linear function of k1 and the anchors for one ring, frame size and reference
@@ -100,7 +101,8 @@ struct RegState {
 
 // Per piece of a node cell (grid (nx - 1) x (nx - 1) REG_SPLIT, pieces (cj REG_SPLIT + piece) (nx - 1) + ci): sums of
 // e^2, J^T e (9), J^T J (45, upper triangle) over h0..h7 and k1, the intensity statistics (sum w, sum r, sum w r,
-// sum r r, count) and e (gx, gy) spread to the cell's four corners (8), e = w - alpha r - beta the residual of the
+// sum r r, count), e (gx, gy) spread to the cell's four corners (8) and each corner's share of the warp's metric
+// (w^2 gx^2, w^2 gx gy, w^2 gy^2 with w the corner's bilinear weight; 12), e = w - alpha r - beta the residual of the
 // warped capture w against the reference picture r. d_lens (nx * nx, device; null: k1's column 0 and dk1 unused) is the
 // unit lens field at the nodes. With d_dl (n x n, device) e is instead d_dl / 2 at each sample: a loss's derivative
 // with respect to the warped picture (launch_spectral_adjoint), so that the sums' 2 J^T e is that loss's gradient.

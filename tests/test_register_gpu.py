@@ -35,9 +35,9 @@ def corner_error(Ht, params, span):
     return float(np.abs(a - b).max())
 
 
-def make_case(version, span, seed, W=1920, Hh=1080, k1=0.03, known=None):
+def make_case(version, span, seed, W=1920, Hh=1080, k1=0.03, known=None, k2=0.0):
     rng = np.random.default_rng(seed)
-    cam = sim.Camera(W, Hh, rot_deg=rng.uniform(-8, 8), scale=0.85, persp=0.03, k1=k1, blur=1.0, noise=4.0)
+    cam = sim.Camera(W, Hh, rot_deg=rng.uniform(-8, 8), scale=0.85, persp=0.03, k1=k1, blur=1.0, noise=4.0, k2=k2)
     blocks = sim.random_blocks(version, rng)
     img, Hn, nodes = sim.capture(sim.picture(version, blocks), span, cam, seed=seed)
     Ht = cam.H(span)
@@ -182,6 +182,38 @@ def test_spectral_refine_recovers_the_registration(decoder_gpu):
     assert e1 < 0.3 < e0, (e0, e1)
     assert abs(p1[8] - 0.04) < abs(0.0 - 0.04), p1[8]
     assert np.array_equal(p1[9:], p[9:])  # the anchors held
+
+
+def test_spectral_refine_moves_the_anchors_for_a_second_lens_term(decoder_gpu):
+    """Behind a lens of two radial terms (k2, which the H and k1 model cannot hold) the spectral refinement that also
+    moves the anchors (lizx_reg_refine_spectral with anchors) ends at a lower loss than the one over H and k1 alone,
+    and its node lattice is closer to the channel's (rms over the interior nodes)."""
+    B = L.format_info(20)["blocks"]
+    known = np.ones(B, bool)
+    c = make_case(20, 256, 13, k1=0.02, known=known, k2=0.05)
+    mask = sim.known_mask(20, known)
+    p = L.reg_params(c["Ht"], 0.02)
+    W, Hh = c["img"].shape[1], c["img"].shape[0]
+    res = {}
+    for anchors in (False, True):
+        p1, tr = decoder_gpu.reg_refine_spectral(20, 256, c["img"], c["Ht"], p, c["ref"], mask, steps=8,
+                                                 anchors=anchors)
+        nodes = L.reg_nodes(256, c["Ht"].astype(np.float32), W, Hh, p1).reshape(c["nodes_true"].shape)
+        Hp = np.append(p1[:8], 1.0).reshape(3, 3)
+        # each interior node's place in the capture, against the channel's
+        node = np.asarray(R.lattice(256))
+        X, Y = np.meshgrid(node, node, indexing="xy")
+        px, py = R.project(Hp, X, Y)
+        tx, ty = R.project(c["Ht"], X, Y)
+        got = np.stack([px + nodes[..., 0], py + nodes[..., 1]], -1)
+        want = np.stack([tx + c["nodes_true"][..., 0], ty + c["nodes_true"][..., 1]], -1)
+        err = np.sqrt(np.mean(np.sum((got - want)[1:-1, 1:-1] ** 2, -1)))
+        res[anchors] = (float(tr[-1]), err)
+        assert np.all(np.diff(tr) <= 0)
+        if not anchors:
+            assert np.array_equal(p1[9:], p[9:])
+    assert res[True][0] < res[False][0], res
+    assert res[True][1] < res[False][1], res
 
 
 def test_refine_recovers_the_homography(decoder_gpu):
