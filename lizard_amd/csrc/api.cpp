@@ -1085,10 +1085,11 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
   d->fb.reserve(p, 1, dbg);
   SoftBufs v = d->fb.view(dbg);
   v.blk_pilot = pilot_area(d, f);
+  v.zero_int = e.queue.as<int>();  // the LDPC queue counter, reset by the first picture kernel (no launch of its own)
   const LdpcParams lp = finish_params(d, f, v.llr, v.est);
   if (!d->use_graphs) {
     launch_soft_warp(p, v, wf, 1, e.stream);
-    e.launch(lp, false, true);
+    e.launch(lp, false, true, true);
   } else {
     // the frame's kernels (picture stages, queue reset, LDPC) as one graph: captured once for this format and these
     // work areas (the capture and registration sit at fixed device addresses the kernels read), then replayed
@@ -1101,7 +1102,7 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
       hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
       try {
         launch_soft_warp(p, v, wf, 1, e.stream);
-        e.launch(lp, false, false);
+        e.launch(lp, false, false, true);
       } catch (...) {
         hipGraph_t gg = nullptr;
         (void)hipStreamEndCapture(e.stream, &gg);

@@ -432,7 +432,8 @@ __global__ __launch_bounds__(256) void k_warp(PlanDev P, const WarpFrame *frames
 // The shading fit (lizard_phy.c:1605-1624): rhs over every eighth row and column outside the squares, beta =
 // hole_inv rhs.
 template <class Src>
-__global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *part_out) {
+__global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *part_out, int *zero_int) {
+  if (zero_int && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *zero_int = 0;
   __shared__ double red[4][6];
   const int tid = threadIdx.x, b = blockIdx.x, f = blockIdx.y, n = P.n, c = n / 8;
   double rhs[6] = {0, 0, 0, 0, 0, 0};
@@ -460,27 +461,6 @@ __global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *p
       (red[0][tid] + red[1][tid]) + (red[2][tid] + red[3][tid]);
 }
 
-// The shading beta = hole_inv rhs (lizard_phy.c:1618-1624) from the fit's parts, summed in part order; one wave a
-// frame, stored after the parts.
-__global__ __launch_bounds__(64) void k_fit_solve(PlanDev P, double *fit) {
-  __shared__ double part_sh[FIT_BETA], rhs_sh[6];
-  const int tid = threadIdx.x, f = blockIdx.x, parts = P.n / 8 / FIT_ROWS;
-  double *F = fit + static_cast<size_t>(f) * FIT_STRIDE;
-  for (int k = tid; k < parts * 6; k += 64) part_sh[k] = F[k];
-  __syncthreads();
-  if (tid < 6) {
-    double s = 0;
-    for (int b = 0; b < parts; b++) s += part_sh[b * 6 + tid];
-    rhs_sh[tid] = s;
-  }
-  __syncthreads();
-  if (tid < 6) {
-    double s = 0;
-    for (int k = 0; k < 6; k++) s += P.hole_inv[tid * 6 + k] * rhs_sh[k];
-    F[FIT_BETA + tid] = s;
-  }
-}
-
 // ------------------------------------------------------------------------------------------------ the two passes
 // Workgroups are dealt to the 8 XCDs round-robin (linear id mod 8), each XCD with its own L2. Where a grid row holds
 // a multiple of 8 workgroups, block bx takes logical block (bx mod 8) (nb / 8) + bx / 8: each XCD gets a contiguous
@@ -504,8 +484,24 @@ __global__ __launch_bounds__(64 * W) void k_rows(PlanDev P, Src src, const doubl
   __shared__ float4 cterm[WARP ? W : 1][2][WARP ? MAX_NODES : 1];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y;
   const int y = xcd_block(blockIdx.x, gridDim.x) * (2 * W) + 2 * w;
-  if (tid < 6) bsh[tid] = fitpart[static_cast<size_t>(f) * FIT_STRIDE + FIT_BETA + tid];
-  __syncthreads();
+  // the shading beta = hole_inv rhs (lizard_phy.c:1618-1624) from the fit's parts summed in part order, by each
+  // workgroup (a few dozen doubles; one launch fewer on the frame's path)
+  {
+    __shared__ double rhs_sh[6];
+    if (tid < 6) {
+      const double *Fp = fitpart + static_cast<size_t>(f) * FIT_STRIDE;
+      double s = 0;
+      for (int b = 0; b < N / 8 / FIT_ROWS; b++) s += Fp[b * 6 + tid];
+      rhs_sh[tid] = s;
+    }
+    __syncthreads();
+    if (tid < 6) {
+      double s = 0;
+      for (int k = 0; k < 6; k++) s += P.hole_inv[tid * 6 + k] * rhs_sh[k];
+      bsh[tid] = s;
+    }
+    __syncthreads();
+  }
   const bool yin0 = in_square(P, y), yin1 = in_square(P, y + 1);
   const auto row0 = src.row(P, f, y), row1 = src.row(P, f, y + 1);
   if constexpr (WARP) {
@@ -1155,8 +1151,8 @@ template <class Src>
 static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int frames, hipStream_t s,
                             const LaunchShape &sh) {
   const int n = p.n, E = p.E;
-  hipLaunchKernelGGL(k_erase_fit<Src>, dim3(n / 8 / FIT_ROWS, frames), dim3(256), 0, s, p.dev, src, b.beta);
-  hipLaunchKernelGGL(k_fit_solve, dim3(frames), dim3(64), 0, s, p.dev, b.beta);
+  hipLaunchKernelGGL(k_erase_fit<Src>, dim3(n / 8 / FIT_ROWS, frames), dim3(256), 0, s, p.dev, src, b.beta,
+                     b.zero_int);
   with_size(n, [&](auto N) {
     constexpr int NN = decltype(N)::value;
     with_waves(sh.rows, [&](auto Wr) {

@@ -83,6 +83,7 @@ struct SoftBufs {
   float *coef_dbg = nullptr;  // null: no debug copy of the coefficients (then the soft floats, 2S floats each)
   float *blk_pilot = nullptr; // null: no pilot readings
   const float *otf = nullptr;  // null: no equalisation; else a blur sigma (picture samples) a frame, 0 for none
+  int *zero_int = nullptr;     // an int the run's first kernel sets to 0 (a later launch's queue counter), or null
   SoftBufs at(const FramePlan &p, int f) const;  // frame f's slice
 };
 
