@@ -783,19 +783,40 @@ def bench_small(args):
     img = img.numpy()
     nodes = np.ascontiguousarray(nodes)
     dec = lizard_amd.Decoder(384)
+    if args.small_eager:
+        dec.graphs(False)
     dec.hold(5, 128, H, nodes)
-    for _ in range(5):
-        dec.decode(img)
-    t0 = time.perf_counter()
-    reps = 50
-    for _ in range(reps):
-        k, got, out, _ = dec.decode(img)
-    gpu_ms = (time.perf_counter() - t0) * 1e3 / reps
+    k, got, out, _ = dec.decode(img)
+
+    def steady(call, warm_s=0.3, run_s=0.5):
+        """ms a call in a tight loop once clocks have settled: warm_s of calls untimed, then run_s timed."""
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < warm_s:
+            call()
+        t0, n = time.perf_counter(), 0
+        while time.perf_counter() - t0 < run_s:
+            call()
+            n += 1
+        return (time.perf_counter() - t0) * 1e3 / n, n
+
+    # the C-ABI call with its arguments made once (as the reference's decoder is timed below), and the Python wrapper
+    ver = np.zeros(dec.max_blocks() * 476, np.uint8)
+    held, dout = C.c_int(0), lizard_amd._Decoded()
+    L = lizard_amd.lib()
+    abi_args = (dec.h, img.ctypes.data_as(C.c_void_p), 1280, 720, img.strides[0], 1, C.byref(held),
+                ver.ctypes.data_as(C.c_void_p), C.byref(dout))
+    gpu_ms, n_abi = steady(lambda: L.liz_decode(*abi_args))
+    wrap_ms, _ = steady(lambda: dec.decode(img), warm_s=0.05, run_s=0.3)
+    same = int(dout.verified) == int(k) and np.array_equal(ver[:k * 476], got.ravel())
     dec.close()
     res = {"workload": "one n=384 LIZARD-80 picture (9 blocks, 64 ring) in a 1280x720 capture: homography 5 deg, "
                        "persp 0.02, blur 0.8 px, AWGN 4 grey; liz_decode through the held registration",
            "verified": int(k), "blocks": 9, "bytes_are_sent": bool(np.array_equal(got.ravel(), blocks[:k * 476])),
-           "gpu_ms_per_frame": round(gpu_ms, 3), "pilot_r": [round(out.pilot_r[0], 3), round(out.pilot_r[1], 3)]}
+           "gpu_ms_per_frame": round(gpu_ms, 4), "wrapper_ms_per_frame": round(wrap_ms, 4),
+           "abi_equals_wrapper": bool(same), "graph": not args.small_eager,
+           "timing": "%d liz_decode calls through the C ABI in 0.5 s after 0.3 s of the same (host frame in, blocks "
+                     "out, a stream sync each); wrapper: lizard_amd.Decoder.decode, the same loop" % n_abi,
+           "pilot_r": [round(out.pilot_r[0], 3), round(out.pilot_r[1], 3)]}
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libref_x86.so")
     if os.path.exists(ref_so) and not args.no_cpu:
         lib = C.CDLL(ref_so)
@@ -1396,6 +1417,9 @@ def main():
     ap.add_argument("--pipe-modes", default="graph,stream,frames,plain",
                     help="pipeline run modes to time (graph always): graph, stream, frames, plain")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuners (default launch shapes)")
+    ap.add_argument("--small-eager", action="store_true",
+                    help="small: the decoder's kernels launched one by one instead of replayed as a graph (the same "
+                         "kernels and bits; for kernel traces)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="with --only: a 1 GiB device copy after the leg, whose counters calibrate a --pmc run")
     ap.add_argument("--op-batches", type=int, default=8,

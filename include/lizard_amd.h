@@ -262,6 +262,11 @@ LIZ_API int lizx_decoder_otf(liz_decoder *d, float sigma);
    default), or launches them one by one (0). Results are the same bits either way. No counterpart in the reference
    (liblizard's decoder runs on the CPU, api/codec.cpp:121-160). */
 LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on);
+/* Waves a workgroup (1, 2 or 4; 4 the default) of the single-frame decode's row FFT, column FFT and demap: launch
+   shapes only, the same bits at every one, so a caller may pick them by timing (a lone frame leaves most of the
+   chip idle, where smaller workgroups spread it over more CUs). No counterpart in the reference (its decoder runs
+   on the CPU). */
+LIZ_API int lizx_decoder_shape(liz_decoder *d, int rows, int cols, int demap);
 /* The hang bound (liblizard/core/wg/wg.h:48-55 HANG_MS, 4000 ms; LIZ_HANG_MS in the environment sets it at first
    use, as the reference's does): a handle's wait for its own GPU work that lasts longer takes the work as hung, fails
    with LIZ_E_STATE, and the handle refuses every later call (free it and make another, as the reference drops a lost

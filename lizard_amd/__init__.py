@@ -142,6 +142,7 @@ def lib():
     sig("lizx_decode_reg", _I, _P, _I, _I, _P, _I, _I, _I, C.POINTER(Registration), _P, _P, _P, _P, C.POINTER(Soft))
     sig("lizx_decoder_hold", _I, _P, _I, _I, C.POINTER(Registration))
     sig("lizx_decoder_graphs", _I, _P, _I)
+    sig("lizx_decoder_shape", _I, _P, _I, _I, _I)
     sig("lizx_finder_anchors", _I, _P, _I)
     sig("lizx_hang_bound", _I, C.c_double)
     sig("lizx_anchor_find", _I, _P, _I, _P, _I, _I, _I, _P, _P, _P)
@@ -765,6 +766,11 @@ class Decoder:
     def graphs(self, on):
         """Single-frame decodes replayed as one captured HIP graph (lizx_decoder_graphs; on by default)."""
         check(lib().lizx_decoder_graphs(self.h, 1 if on else 0))
+
+    def shape(self, rows, cols, demap):
+        """Waves a workgroup (1, 2 or 4) of the single-frame decode's row FFT, column FFT and demap
+        (lizx_decoder_shape; the same bits at every shape)."""
+        check(lib().lizx_decoder_shape(self.h, rows, cols, demap))
 
     def anchors(self, mode):
         """The finder's registration from the anchors outward (lizx_finder_anchors): 0 off, 1 (the default) over the

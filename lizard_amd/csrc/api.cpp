@@ -122,17 +122,19 @@ struct lizx_ldpc {
 };
 
 // A frame's per-block results in one device area (and one pinned host mirror), so they come back in one copy:
-// bytes (B x 476), iterations (B), verdicts (B), estimates (B floats), pilot readings (B floats).
+// bytes (B x 476), iterations (B), verdicts (B), estimates (B floats), pilot readings (B floats), the pilots' grid
+// shift (2 floats).
 struct Results {
   int B = 0;
-  size_t at_its = 0, at_ok = 0, at_est = 0, at_pilot = 0, total = 0;
+  size_t at_its = 0, at_ok = 0, at_est = 0, at_pilot = 0, at_align = 0, total = 0;
   void layout(int blocks) {
     B = blocks;
     at_its = static_cast<size_t>(B) * BLOCK_BYTES;
     at_ok = at_its + B;
     at_est = (at_ok + B + 3) & ~static_cast<size_t>(3);
     at_pilot = at_est + 4 * static_cast<size_t>(B);
-    total = at_pilot + 4 * static_cast<size_t>(B);
+    at_align = at_pilot + 4 * static_cast<size_t>(B);
+    total = at_align + 2 * sizeof(float);
   }
 };
 
@@ -168,6 +170,10 @@ struct liz_decoder {
   FrameBufs fb;
   DevBuf img, nodes, wframe;
   PinnedBuf img_host;  // the capture's box staged in pinned memory, then one contiguous copy
+  // a cropped capture's upload as one copy: its WarpFrame, nodes and box side by side (CAP_* offsets), staged in
+  // cap_host and copied to cap (the WarpFrame at cap's start, a fixed address the frame graph reads)
+  DevBuf cap;
+  PinnedBuf cap_host;
   WarpFrame wframe_host{};
   std::vector<uint8_t> luma;
   // registration refinement
@@ -187,6 +193,13 @@ struct liz_decoder {
   std::vector<float> found_nodes;
   // the single-frame decode's kernels as one HIP graph, kept for the format and work areas it was captured with
   bool use_graphs = true;
+  // the single-frame picture stages' waves a workgroup (the same bits at every shape): one wave a workgroup for the
+  // row pass and the demap spreads a lone frame over the most CUs (measured fastest on n = 384 frames)
+  LaunchShape shape = [] {
+    LaunchShape s;
+    s.rows = 1, s.cols = 2, s.demap = 1;
+    return s;
+  }();
   int anchor_mode = 1;  // the finder registers from the anchors outward (lizx_finder_anchors)
   struct FrameGraph {
     int version = 0, span = 0;
@@ -261,10 +274,17 @@ static float *pilot_area(liz_decoder *d, const Format *f) {
   return reinterpret_cast<float *>(d->res.as<uint8_t>() + d->lay.at_pilot);
 }
 
+// The frame's results area's grid shift (where k_align writes it, in place of the work area's).
+static float *align_area(liz_decoder *d, const Format *f) {
+  d->lay.layout(f->blocks);
+  d->res.reserve(d->lay.total);
+  return reinterpret_cast<float *>(d->res.as<uint8_t>() + d->lay.at_align);
+}
+
 // The per-block results of finish_blocks to the host in one copy (after the stream's work); returns the verified
-// count. pilot (optional) receives the pilot readings.
+// count. pilot (optional) receives the pilot readings, align (optional) the grid shift.
 static int collect_blocks(liz_decoder *d, const Format *f, uint8_t *blocks, uint8_t *ok, int8_t *its, float *blk_est,
-                          float *pilot = nullptr) {
+                          float *pilot = nullptr, float *align = nullptr) {
   LdpcEngine &e = *d->eng;
   const int B = f->blocks;
   d->res_host.reserve(d->lay.total);
@@ -275,6 +295,7 @@ static int collect_blocks(liz_decoder *d, const Format *f, uint8_t *blocks, uint
   std::memcpy(ok, h + d->lay.at_ok, B);
   std::memcpy(blk_est, h + d->lay.at_est, B * sizeof(float));
   if (pilot) std::memcpy(pilot, h + d->lay.at_pilot, B * sizeof(float));
+  if (align) std::memcpy(align, h + d->lay.at_align, 2 * sizeof(float));
   int good = 0;
   for (int b = 0; b < B; b++) {
     if (its[b] == -2) ok[b] = 0;
@@ -715,6 +736,19 @@ LIZ_API int lizx_decoder_graphs(liz_decoder *d, int on) {
   });
 }
 
+LIZ_API int lizx_decoder_shape(liz_decoder *d, int rows, int cols, int demap) {
+  return guard([&] {
+    need(d, "a decoder");
+    const auto ok = [](int w) { return w == 1 || w == 2 || w == 4; };
+    need(ok(rows) && ok(cols) && ok(demap), "waves a workgroup: 1, 2 or 4");
+    d->shape.rows = rows;
+    d->shape.cols = cols;
+    d->shape.demap = demap;
+    d->drop_frame_graph();
+    return 0;
+  });
+}
+
 LIZ_API int lizx_decoder_otf(liz_decoder *d, float sigma) {
   return guard([&] {
     need(d, "a decoder");
@@ -1012,6 +1046,10 @@ static void read_box(const FramePlan &p, const lizx_registration *reg, int w, in
   if (box[2] - box[0] < 2 || box[3] - box[1] < 2) box[0] = 0, box[1] = 0, box[2] = w, box[3] = h;
 }
 
+// The cropped upload's layout: the WarpFrame at 0, the nodes at CAP_NODES, the capture's box at CAP_IMG.
+constexpr size_t CAP_NODES = 256, CAP_IMG = CAP_NODES + ((MAX_NODES * MAX_NODES * sizeof(float2) + 255) & ~size_t{255});
+static_assert(sizeof(WarpFrame) <= CAP_NODES, "the WarpFrame fits before the nodes");
+
 // A capture and its registration onto the device as a WarpFrame (one frame), on the decoder's stream: only the
 // pixels the warp can reach through this registration when crop (read_box), else the whole frame (the refinement
 // moves the registration on the device). Every entry that stages a capture synchronises the stream before it
@@ -1025,6 +1063,30 @@ static const WarpFrame *stage_capture(liz_decoder *d, const FramePlan &p, const 
   int box[4] = {0, 0, w, h};
   if (crop) read_box(p, reg, w, h, box);
   const int bw = box[2] - box[0], bh = box[3] - box[1];
+  if (crop) {
+    // the WarpFrame, the nodes and the box's rows gathered into pinned memory by the host, then one copy
+    need(p.dev.nx <= MAX_NODES && p.dev.ny <= MAX_NODES, "a node lattice of at most 40 x 40");
+    const size_t room = CAP_IMG + static_cast<size_t>(w) * h;  // the whole frame's, so a larger box never reallocates
+    d->cap.reserve(room);
+    d->cap_host.reserve(room);
+    uint8_t *dev = d->cap.as<uint8_t>(), *stage = static_cast<uint8_t *>(d->cap_host.p);
+    WarpFrame wf{};
+    for (int k = 0; k < 9; k++) wf.H[k] = reg->H[k];
+    wf.nodes = reinterpret_cast<const float2 *>(dev + CAP_NODES);
+    // pixel (x, y) at img + y bw + x: the stored box's first pixel is (box[0], box[1])
+    wf.img = dev + CAP_IMG - (static_cast<ptrdiff_t>(box[1]) * bw + box[0]);
+    wf.stride = bw;
+    wf.w = w;
+    wf.h = h;
+    std::memcpy(stage, &wf, sizeof wf);
+    std::memcpy(stage + CAP_NODES, reg->nodes, nn * sizeof(float2));
+    const uint8_t *src = img + static_cast<size_t>(box[1]) * stride + box[0];
+    for (int y = 0; y < bh; y++)
+      std::memcpy(stage + CAP_IMG + static_cast<size_t>(y) * bw, src + static_cast<size_t>(y) * stride, bw);
+    hip_check(hipMemcpyAsync(dev, stage, CAP_IMG + static_cast<size_t>(bw) * bh, hipMemcpyHostToDevice, e.stream),
+              "capture");
+    return d->cap.as<WarpFrame>();
+  }
   d->img.reserve(static_cast<size_t>(w) * h);  // the whole frame's room, so a later larger box never reallocates
   const uint8_t *src = img + static_cast<size_t>(box[1]) * stride + box[0];
   const size_t bytes = static_cast<size_t>(bw) * bh;
@@ -1054,7 +1116,8 @@ static const WarpFrame *stage_capture(liz_decoder *d, const FramePlan &p, const 
   return d->wframe.as<WarpFrame>();
 }
 
-// The soft values, estimates, rms, coefficients and pilot shift of the last frame to the caller's lizx_soft.
+// The soft values, estimates, rms and coefficients of the last frame to the caller's lizx_soft (the pilot shift comes
+// back with the results).
 static void copy_soft(liz_decoder *d, const FramePlan &p, lizx_soft *soft, bool dbg) {
   const size_t S = static_cast<size_t>(p.subch) * SUB;
   if (soft->llr) copy_wait(soft->llr, d->fb.llr.p, 2 * S, hipMemcpyDeviceToHost, d->eng->stream, "soft values");
@@ -1067,7 +1130,6 @@ static void copy_soft(liz_decoder *d, const FramePlan &p, lizx_soft *soft, bool 
   if (soft->soft && dbg)
     copy_wait(soft->soft, d->fb.coef_dbg.as<float>() + 2 * S, S * 2 * sizeof(float), hipMemcpyDeviceToHost, d->eng->stream,
               "soft floats");
-  copy_wait(soft->align, d->fb.align.p, 2 * sizeof(float), hipMemcpyDeviceToHost, d->eng->stream, "shift");
 }
 
 static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *img, int w, int h, int stride,
@@ -1085,14 +1147,17 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
   d->fb.reserve(p, 1, dbg);
   SoftBufs v = d->fb.view(dbg);
   v.blk_pilot = pilot_area(d, f);
+  v.align = align_area(d, f);
   v.zero_int = e.queue.as<int>();  // the LDPC queue counter, reset by the first picture kernel (no launch of its own)
   const LdpcParams lp = finish_params(d, f, v.llr, v.est);
   if (!d->use_graphs) {
-    launch_soft_warp(p, v, wf, 1, e.stream);
+    launch_soft_warp(p, v, wf, 1, e.stream, d->shape);
     e.launch(lp, false, true, true);
   } else {
-    // the frame's kernels (picture stages, queue reset, LDPC) as one graph: captured once for this format and these
-    // work areas (the capture and registration sit at fixed device addresses the kernels read), then replayed
+    // the frame's kernels (picture stages, LDPC) as one graph: captured once for this format and these work areas
+    // (the capture and registration sit at fixed device addresses the kernels read), then replayed; the results'
+    // copy stays outside it (with that copy captured as a graph node the process crashed on the first launch under
+    // rocprofv3's kernel trace)
     auto &g = d->fg;
     const void *keys[9] = {v.X, v.C, v.llr, v.coef_dbg, d->res.p, d->tasks.p, wf, e.white.p, v.otf};
     bool same = g.exec && g.version == version && g.span == span && g.dbg == dbg;
@@ -1101,7 +1166,7 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
       d->drop_frame_graph();
       hip_check(hipStreamBeginCapture(e.stream, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
       try {
-        launch_soft_warp(p, v, wf, 1, e.stream);
+        launch_soft_warp(p, v, wf, 1, e.stream, d->shape);
         e.launch(lp, false, false, true);
       } catch (...) {
         hipGraph_t gg = nullptr;
@@ -1118,7 +1183,8 @@ static int decode_capture(liz_decoder *d, int version, int span, const uint8_t *
     }
     hip_check(hipGraphLaunch(g.exec, e.stream), "hipGraphLaunch");
   }
-  const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr);
+  const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr,
+                                  soft ? soft->align : nullptr);
   if (soft) copy_soft(d, p, soft, dbg);
   return good;
 }
@@ -2594,9 +2660,11 @@ LIZ_API int lizx_finish_grid(liz_decoder *d, int version, int span, const float 
     hip_check(hipMemcpyAsync(d->grid.p, grid, n * n * sizeof(float), hipMemcpyHostToDevice, e.stream), "grid");
     SoftBufs v = d->fb.view(dbg);
     v.blk_pilot = pilot_area(d, f);
+    v.align = align_area(d, f);
     launch_soft(p, v, d->grid.as<float>(), 1, e.stream);
     finish_blocks(d, f, v.llr, v.est);
-    const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr);
+    const int good = collect_blocks(d, f, blocks, ok, its, blk_est, soft ? soft->blk_pilot : nullptr,
+                                  soft ? soft->align : nullptr);
     if (soft) copy_soft(d, p, soft, dbg);
     return good;
   });

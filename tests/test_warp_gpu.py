@@ -202,3 +202,30 @@ def test_single_frame_graph_replays_equal_eager(decoder_gpu):
         assert np.array_equal(blocks, want[0]["blocks"][want[0]["ok"].astype(bool)])
     finally:
         dec.close()
+
+
+def test_single_frame_shapes_equal(decoder_gpu):
+    """Every single-frame launch shape (waves a workgroup of the row FFT, column FFT and demap; lizx_decoder_shape)
+    gives the same soft values, verdicts, iterations, estimates and bytes, graphs on and off; a shape outside 1, 2, 4
+    is refused."""
+    c = Capture(20, 256, seed=35, W=1280, Hh=720, scale=0.9, blur=0.8, noise=5.0)
+    dec = lizard_amd.Decoder(1536)
+    try:
+        want = None
+        for graphs in (True, False):
+            dec.graphs(graphs)
+            for sh in ((4, 4, 4), (1, 2, 1), (1, 1, 1), (2, 4, 2), (4, 1, 2)):
+                dec.shape(*sh)
+                got = dec.decode_reg(c.version, c.span, c.img, c.H, c.nodes, soft=True)
+                if want is None:
+                    want = got
+                    assert want["ok"].sum() > 0
+                    continue
+                for k in ("its", "ok", "est", "llr"):
+                    assert np.array_equal(got[k], want[k]), (graphs, sh, k)
+                v = want["ok"].astype(bool)
+                assert np.array_equal(got["blocks"][v], want["blocks"][v]), (graphs, sh)
+        with pytest.raises(lizard_amd.LizError):
+            dec.shape(3, 1, 1)
+    finally:
+        dec.close()
