@@ -1058,6 +1058,7 @@ static const WarpFrame *stage_capture(liz_decoder *d, const FramePlan &p, const 
                                       const lizx_registration *reg, bool crop = false) {
   need(img && reg && reg->nodes, "a capture and a registration with nodes");
   need(w >= 2 && h >= 2 && stride >= w, "a capture of at least 2 x 2 pixels, rows stride >= w bytes apart");
+  need(warp_frame_fits(h, w), "a capture under 2^31 pixels with rows under 2^24 pixels");
   LdpcEngine &e = *d->eng;
   const size_t nn = static_cast<size_t>(p.dev.nx) * p.dev.ny;
   int box[4] = {0, 0, w, h};

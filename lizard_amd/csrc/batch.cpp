@@ -292,6 +292,7 @@ LIZ_API int lizx_batch_run(lizx_batch *b, const lizx_frame *frames, int mode) {
         b->host_grids[at] = in.grid;
       } else {
         need(in.img && in.nodes && in.w >= 2 && in.h >= 2, "a capture and nodes for every frame");
+        need(warp_frame_fits(in.h, in.w), "captures under 2^31 pixels with rows under 2^24 pixels");
         WarpFrame w{};
         std::memcpy(w.H, in.H, sizeof w.H);
         w.nodes = reinterpret_cast<const float2 *>(in.nodes);

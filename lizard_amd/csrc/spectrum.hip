@@ -310,19 +310,40 @@ __device__ __forceinline__ float grey(uint8_t g) {
 }
 
 // A bilinear read of the uint8 capture at pixel coordinates (x, y) (pixel centres at half integers), clamped to the
-// frame, grey level i / 255 (liblizard/src/internal.h:11-23 at gamma 1). In bounds for any input (NaN reads pixel 0).
+// frame, grey level i / 255 (liblizard/src/internal.h:11-23 at gamma 1). In bounds for any input: NaN reads pixel 0
+// (the reference leaves NaN to C's undefined float-to-int conversion). Branch-free: the clamp as two selects, the
+// fraction x - (int) x as v_fract (exact for x >= 0, checked on the device for every float in [0, 2^31)), the
+// pixel's offset in 32 bits (the frame's rows stride < 2^24 bytes apart and stride h < 2^31, which every host entry
+// that builds a WarpFrame checks) from the frame's base held in scalar registers, read through global loads.
 __device__ __forceinline__ float sample(const WarpFrame &F, float x, float y) {
   x = __fsub_rn(x, 0.5f);
   y = __fsub_rn(y, 0.5f);
   const float xm = __fsub_rn(static_cast<float>(F.w), 1.001f), ym = __fsub_rn(static_cast<float>(F.h), 1.001f);
-  if (x < 0) x = 0; else if (x > xm) x = xm;
-  if (y < 0) y = 0; else if (y > ym) y = ym;
-  const int xi = x >= 0 ? static_cast<int>(x) : 0, yi = y >= 0 ? static_cast<int>(y) : 0;
-  const float fx = __fsub_rn(x, static_cast<float>(xi)), fy = __fsub_rn(y, static_cast<float>(yi));
-  const uint8_t *p = F.img + static_cast<ptrdiff_t>(yi) * F.stride + xi;
-  const float a = grey(p[0]), b = grey(p[1]), c = grey(p[F.stride]), d = grey(p[F.stride + 1]);
+  x = fminf(x > 0.0f ? x : 0.0f, xm);
+  y = fminf(y > 0.0f ? y : 0.0f, ym);
+  const int xi = static_cast<int>(x), yi = static_cast<int>(y);
+  const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
+  // a pixel pair (x, x + 1) as one 16-bit load at any byte address (the device's unaligned global access)
+  struct __attribute__((packed)) Pair { uint16_t v; };
+  const auto *g = (const __attribute__((address_space(1))) uint8_t *)F.img;
+  const uint32_t o = __umul24(static_cast<uint32_t>(yi), static_cast<uint32_t>(F.stride)) + static_cast<uint32_t>(xi);
+  const uint32_t t = ((const __attribute__((address_space(1))) Pair *)(g + o))->v;
+  const uint32_t u = ((const __attribute__((address_space(1))) Pair *)(g + o + static_cast<uint32_t>(F.stride)))->v;
+  const float a = grey(t & 0xff), b = grey(t >> 8), c = grey(u & 0xff), d = grey(u >> 8);
   const float top = __fadd_rn(a, __fmul_rn(__fsub_rn(b, a), fx)), bot = __fadd_rn(c, __fmul_rn(__fsub_rn(d, c), fx));
   return __fadd_rn(__fmul_rn(top, __fsub_rn(1.0f, fy)), __fmul_rn(bot, fy));
+}
+
+// 1 / d correctly rounded: v_rcp_f32 and one Newton step of fused multiply-adds equal the IEEE quotient for every
+// normal d with |d| < 2^126, both signs (checked on the device for each such float); the rest (zero, subnormal,
+// |d| >= 2^126, inf, NaN) take the IEEE division.
+__device__ __forceinline__ float recip_rn(float d) {
+  const float a = fabsf(d);
+  if (a >= 0x1p-126f && a < 0x1p126f) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return __fmaf_rn(__fmaf_rn(-d, r, 1.0f), r, r);
+  }
+  return __fdiv_rn(1.0f, d);
 }
 
 // The row part of a warp sample (everything that depends on y alone).
@@ -365,7 +386,7 @@ __device__ __forceinline__ float4 cell_terms(const PlanDev &P, const WarpFrame &
 __device__ __forceinline__ float2 warp_point(const PlanDev &P, const WarpFrame &F, const WarpRow &r, int x, float4 ct) {
   const float mx = P.mcoord[x];
   const float *H = F.H;
-  const float rw = __fdiv_rn(1.0f, __fadd_rn(__fmul_rn(H[6], mx), r.kw));
+  const float rw = recip_rn(__fadd_rn(__fmul_rn(H[6], mx), r.kw));
   const float pu = __fmul_rn(__fadd_rn(__fmul_rn(H[0], mx), r.ku), rw), pv = __fmul_rn(__fadd_rn(__fmul_rn(H[3], mx), r.kv), rw);
   const float dx = __fadd_rn(ct.y, __fmul_rn(ct.x, mx)), dy = __fadd_rn(ct.w, __fmul_rn(ct.z, mx));
   return make_float2(__fadd_rn(pu, dx), __fadd_rn(pv, dy));

@@ -34,6 +34,10 @@ struct WarpFrame {
   int w, h;
   int stride;
 };
+// The sampler's pixel offsets are 32-bit: rows under 2^24 bytes apart and stride h under 2^31 bytes.
+inline bool warp_frame_fits(int h, int stride) {
+  return stride > 0 && h > 0 && stride < (1 << 24) && static_cast<long long>(stride) * h < (1LL << 31);
+}
 
 // Device-side view of a plan (by value into kernels).
 struct PlanDev {
