@@ -18,6 +18,7 @@
 #include <mutex>
 #include <type_traits>
 
+#include "sample_math.hpp"
 #include "spectrum.hpp"
 
 namespace lz {
@@ -301,13 +302,6 @@ __device__ void block_sums(double (&v)[K], double *red, int tid) {
 }
 
 // ------------------------------------------------------------------------------------------------ the warp
-// Grey level g / 255, correctly rounded: the product by the rounded reciprocal, corrected by one fused multiply-add
-// (the residual g - 255 q is exact in it). Equal to the IEEE quotient for all 256 levels (checked exhaustively).
-__device__ __forceinline__ float grey(uint8_t g) {
-  constexpr float c = 1.0f / 255.0f;
-  const float f = static_cast<float>(g), q = __fmul_rn(f, c);
-  return __fmaf_rn(__fmaf_rn(-q, 255.0f, f), c, q);
-}
 
 // A bilinear read of the uint8 capture at pixel coordinates (x, y) (pixel centres at half integers), clamped to the
 // frame, grey level i / 255 (liblizard/src/internal.h:11-23 at gamma 1). In bounds for any input: NaN reads pixel 0
@@ -322,7 +316,7 @@ __device__ __forceinline__ float sample(const WarpFrame &F, float x, float y) {
   x = fminf(x > 0.0f ? x : 0.0f, xm);
   y = fminf(y > 0.0f ? y : 0.0f, ym);
   const int xi = static_cast<int>(x), yi = static_cast<int>(y);
-  const float fx = __builtin_amdgcn_fractf(x), fy = __builtin_amdgcn_fractf(y);
+  const float fx = frac_pos(x), fy = frac_pos(y);
   // a pixel pair (x, x + 1) as one 16-bit load at any byte address (the device's unaligned global access)
   struct __attribute__((packed)) Pair { uint16_t v; };
   const auto *g = (const __attribute__((address_space(1))) uint8_t *)F.img;
@@ -332,18 +326,6 @@ __device__ __forceinline__ float sample(const WarpFrame &F, float x, float y) {
   const float a = grey(t & 0xff), b = grey(t >> 8), c = grey(u & 0xff), d = grey(u >> 8);
   const float top = __fadd_rn(a, __fmul_rn(__fsub_rn(b, a), fx)), bot = __fadd_rn(c, __fmul_rn(__fsub_rn(d, c), fx));
   return __fadd_rn(__fmul_rn(top, __fsub_rn(1.0f, fy)), __fmul_rn(bot, fy));
-}
-
-// 1 / d correctly rounded: v_rcp_f32 and one Newton step of fused multiply-adds equal the IEEE quotient for every
-// normal d with |d| < 2^126, both signs (checked on the device for each such float); the rest (zero, subnormal,
-// |d| >= 2^126, inf, NaN) take the IEEE division.
-__device__ __forceinline__ float recip_rn(float d) {
-  const float a = fabsf(d);
-  if (a >= 0x1p-126f && a < 0x1p126f) {
-    const float r = __builtin_amdgcn_rcpf(d);
-    return __fmaf_rn(__fmaf_rn(-d, r, 1.0f), r, r);
-  }
-  return __fdiv_rn(1.0f, d);
 }
 
 // The row part of a warp sample (everything that depends on y alone).

@@ -229,3 +229,18 @@ def test_single_frame_shapes_equal(decoder_gpu):
             dec.shape(3, 1, 1)
     finally:
         dec.close()
+
+
+def test_sampler_math_equals_plain_forms():
+    """The warp sampler's cheap float forms (lizard_amd/csrc/sample_math.hpp) equal the plain operators bit for bit on
+    the device: 1/d by v_rcp and one Newton step against the IEEE quotient for all 2^32 float patterns (the forms
+    split at |d| in [2^-126, 2^126)), x - (int) x as v_fract for every float in [0, 2^31), g / 255 for all 256
+    levels."""
+    import ctypes as C
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "devcheck", "libsample_math.so")
+    assert os.path.exists(path), "build the device check first (python __graft_entry__.py build)"
+    lib = C.CDLL(path)
+    counts, first = (C.c_ulonglong * 3)(), (C.c_uint32 * 12)()
+    assert lib.lz_check_sample_math(counts, first) == 0
+    assert list(counts) == [0, 0, 0], (list(counts), ["%08x" % v for v in first])
