@@ -101,7 +101,9 @@ def test_autograd_gradcheck(decoder_gpu, case384):
     out = RegLoss.apply(delta)
     out.backward()
     assert torch.isfinite(delta.grad).all() and delta.grad.abs().max() > 0
-    assert torch.autograd.gradcheck(RegLoss.apply, (delta,), eps=0.02, atol=0.5, rtol=0.05, nondet_tol=1e-3)
+    # central differences at 0.02 units agree with the adjoint to 4e-4 of the largest component (measured on this
+    # case: at most 0.16 against components up to 393, each within 0.2 % of itself)
+    assert torch.autograd.gradcheck(RegLoss.apply, (delta,), eps=0.02, atol=0.05, rtol=0.005, nondet_tol=1e-3)
 
 
 def _half_mask(version, seed):
@@ -162,7 +164,8 @@ def test_spectral_autograd_gradcheck(decoder_gpu, case384):
     out = SpecLoss.apply(delta)
     out.backward()
     assert torch.isfinite(delta.grad).all() and delta.grad.abs().max() > 0
-    assert torch.autograd.gradcheck(SpecLoss.apply, (delta,), eps=0.02, atol=0.5, rtol=0.05, nondet_tol=1e-3)
+    # measured on this case: differences at most 0.14 against components up to 198, each within 0.25 % of itself
+    assert torch.autograd.gradcheck(SpecLoss.apply, (delta,), eps=0.02, atol=0.05, rtol=0.005, nondet_tol=1e-3)
 
 
 def test_spectral_refine_recovers_the_registration(decoder_gpu):
