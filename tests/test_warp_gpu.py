@@ -244,3 +244,41 @@ def test_sampler_math_equals_plain_forms():
     counts, first = (C.c_ulonglong * 3)(), (C.c_uint32 * 12)()
     assert lib.lz_check_sample_math(counts, first) == 0
     assert list(counts) == [0, 0, 0], (list(counts), ["%08x" % v for v in first])
+
+
+def test_decode_padded_rows_equal_tight():
+    """liz_decode and lizx_decode_reg on a capture whose rows are padded (stride > w bytes apart, the cropped upload
+    gathering the box's rows) give the tight capture's verdicts and bytes; a capture of 2^31 pixels or more is refused
+    before anything is read."""
+    import ctypes as C
+    c = Capture(5, 128, seed=36, blur=0.8, noise=4.0)
+    dec = lizard_amd.Decoder(384)
+    try:
+        want = dec.decode_reg(c.version, c.span, c.img, c.H, c.nodes, soft=False)
+        assert want["ok"].sum() > 0
+        h, w = c.img.shape
+        pad = np.full((h, w + 77), 201, np.uint8)
+        pad[:, :w] = c.img
+        L = lizard_amd.lib()
+        reg = lizard_amd.registration(c.H, c.nodes)
+        B = want["ok"].size
+        blocks, ok = np.zeros(B * 476, np.uint8), np.zeros(B, np.uint8)
+        its, be = np.zeros(B, np.int8), np.zeros(B, np.float32)
+        good = L.lizx_decode_reg(dec.h, c.version, c.span, ptr(pad), w, h, w + 77, C.byref(reg), ptr(blocks), ptr(ok),
+                                 ptr(its), ptr(be), None)
+        assert good == int(want["ok"].sum())
+        assert np.array_equal(ok, want["ok"]) and np.array_equal(its, want["its"]) and np.array_equal(be, want["est"])
+        v = ok.astype(bool)
+        assert np.array_equal(blocks.reshape(B, 476)[v], want["blocks"][v])
+        # liz_decode through a hold on the padded rows
+        dec.hold(c.version, c.span, c.H, c.nodes)
+        ver = np.zeros(dec.max_blocks() * 476, np.uint8)
+        held, out = C.c_int(0), lizard_amd._Decoded()
+        k = L.liz_decode(dec.h, ptr(pad), w, h, w + 77, 1, C.byref(held), ptr(ver), C.byref(out))
+        assert k == good and np.array_equal(ver[:k * 476].reshape(k, 476), want["blocks"][v])
+        # 65536 x 32768 pixels claimed over a small buffer: refused by its size alone
+        rc = L.lizx_decode_reg(dec.h, c.version, c.span, ptr(pad), 65536, 32768, 65536, C.byref(reg), ptr(blocks),
+                               ptr(ok), ptr(its), ptr(be), None)
+        assert rc == -1 and b"2^31" in L.liz_last_error()
+    finally:
+        dec.close()
