@@ -2,14 +2,17 @@
 // 8.5):
 //   k_erase_fit  n / 32 workgroups a frame: the shading's least-squares sums on every eighth sample outside the
 //                anchors, four fit rows a workgroup
-//   k_rows       eight rows a workgroup, two a wave packed as one complex sequence: the warp (or a supplied grid) read
-//                on load with the anchors' samples erased, the detrend row sums, the wave FFT along x, the two rows'
-//                transforms separated and pruned to u < E, written column-major
+//   k_rows       2 W rows a workgroup, two a wave packed as one complex sequence: the warp (or a supplied grid) read
+//                on load with the anchors' samples erased, the shading's beta from the fit's parts, the detrend row
+//                sums, the wave FFT along x, the two rows' transforms separated and pruned to u < E, written
+//                column-major
 //   k_cols       one column u a wave: the wave FFT along y, the column's coefficients scattered into coefficient order
 //   k_align      one workgroup a frame: the detrend projections and the pilots' two-pass least-squares grid shift
 //   k_demap      one wave a sub-channel: 320 coefficients gathered, detrended, turned, their moments, int8 LLRs
-// The wave FFT (WaveFft): n = 64 M points, M in registers a lane, an M-point DFT in registers, then two radix-8 passes
-// through the wave's own LDS; every size compiled in. Floating-point steps the reference rounds in a fixed order (the
+//   k_est        one lane a sub-channel: the estimate J from the argument the demap left (single-frame paths: in
+//                k_pilot_read, one wave a block, which also reads the block's pilots)
+// The wave FFT (WaveFft): n = 64 M points, M in registers a lane, an M-point DFT in registers, then the 64-point stage
+// as R1 x 64 / R1 (a radix plan the autotuner picks) through the wave's own LDS; every size compiled in. Floating-point steps the reference rounds in a fixed order (the
 // warp, the detrend terms, the turn, the LLR scale) use explicitly rounded operations, so no contraction into fused
 // multiply-adds changes them.
 #include <hip/hip_runtime.h>
@@ -794,7 +797,9 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_al
   const float k = static_cast<float>(0.7 * 2 * sqrt(2 * a2) / nv);
   if (lane == 0) {
     rms_all[f * P.subch + j] = m2 > 0 ? static_cast<float>(sqrt(m2 / 2)) : 0.f;
-    est_all[f * P.subch + j] = axis_info(a2 / nv);
+    // the estimate's argument a2 / nv; est_in_place turns it into J (axis_info), many sub-channels a wave, so this
+    // wave does not spend its issue on one lane's double pow chain
+    est_all[f * P.subch + j] = a2 / nv;
   }
   int8_t *o = llr_all + static_cast<size_t>(f) * P.subch * 2 * SUB + 2 * j * SUB;
 #pragma unroll
@@ -814,10 +819,22 @@ __global__ __launch_bounds__(64 * WV) void k_demap(PlanDev P, const float2 *C_al
 // Block b's pilot reading (lizard_phy.c:1795-1809; SPEC 7.3): over the slots past its codeword, the axis value
 // against the whitening's sign in units of its sub-channel's rms axis value, averaged over the tail; NaN where the
 // block has no tail or no sub-channel of it read anything. One wave a block (tails are 16 or 32 slots).
+// est[j] = axis_info(est[j]) in place: the sub-channel estimates from the arguments k_demap left (lizard_phy.c:75,
+// :1771), one lane a sub-channel.
+__device__ __forceinline__ void est_in_place(double *est, int j) { est[j] = axis_info(est[j]); }
+
+__global__ __launch_bounds__(256) void k_est(double *est, int count) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j < count) est_in_place(est, j);
+}
+
+// The pilot readings of block b of frame f; its wave also turns the frame's sub-channels b, b + blocks, ... into
+// their estimates (the single-frame path's est step, without a launch of its own).
 __global__ __launch_bounds__(64) void k_pilot_read(PlanDev P, const float2 *C_all, const double *proj,
                                                    const float *align, const float *rms_all, float *out,
-                                                   const float *otf) {
+                                                   const float *otf, double *est_all) {
   const int b = blockIdx.x, f = blockIdx.y, lane = threadIdx.x;
+  for (int j = b + P.blocks * lane; j < P.subch; j += P.blocks * 64) est_in_place(est_all + f * P.subch, j);
   const float gk = otf_rate(P, otf, f);
   const int4 bk = P.blk[b];
   const int s0 = bk.x, nt = bk.y, nb = bk.z;
@@ -1183,7 +1200,9 @@ static void launch_soft_src(const FramePlan &p, const SoftBufs &b, Src src, int 
   });
   if (b.blk_pilot)
     hipLaunchKernelGGL(k_pilot_read, dim3(p.dev.blocks, frames), dim3(64), 0, s, p.dev, b.C, b.proj, b.align, b.rms,
-                       b.blk_pilot, b.otf);
+                       b.blk_pilot, b.otf, b.est);
+  else
+    hipLaunchKernelGGL(k_est, dim3((frames * p.subch + 255) / 256), dim3(256), 0, s, b.est, frames * p.subch);
   hip_check(hipGetLastError(), "soft-value launches");
 }
 
