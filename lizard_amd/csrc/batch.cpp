@@ -115,9 +115,16 @@ struct lizx_batch {
     if (groups.size() == 1) {
       soft(*groups[0], 0, static_cast<int>(groups[0]->frames.size()), s);
     } else {
-      // fork: each format on its own stream, joined back before the LDPC (a graph captures the same dependencies)
+      // fork: each format on its own stream, joined back before the LDPC (a graph captures the same dependencies);
+      // the groups with the most work (frames x n^2) launched first, the longest branch starting soonest
+      std::vector<size_t> order(groups.size());
+      for (size_t g = 0; g < order.size(); g++) order[g] = g;
+      const auto work = [&](size_t g) {
+        return static_cast<double>(groups[g]->frames.size()) * groups[g]->plan->n * groups[g]->plan->n;
+      };
+      std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return work(a) > work(b); });
       hip_check(hipEventRecord(fork, s), "hipEventRecord");
-      for (size_t g = 0; g < groups.size(); g++) {
+      for (const size_t g : order) {
         hip_check(hipStreamWaitEvent(side[g], fork, 0), "hipStreamWaitEvent");
         soft(*groups[g], 0, static_cast<int>(groups[g]->frames.size()), side[g]);
         hip_check(hipEventRecord(join[g], side[g]), "hipEventRecord");
