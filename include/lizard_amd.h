@@ -375,6 +375,45 @@ LIZ_API int lizx_b3_root(const uint8_t *cvs, uint32_t chunks, uint8_t *root);
 /* BLAKE3 of host data on the host (xfer_b3_hash, xfer.h:181). */
 LIZ_API int lizx_b3_hash(const uint8_t *data, uint64_t length, uint8_t *out);
 
+/* The transfer's control blocks (host): the header and manifest block payloads of the transfer's wire format,
+   version 3 (liblizard/src/xfer.h:40-92), held to the same rules on writing and reading (xfer.c:11-132). The struct
+   has xfer_header_t's fields in its order (xfer.h:132-138), so a binding passes either; the returns are xfer.h's
+   (0, or -1, -2 version, -3 hash, -4 codec for a header). */
+typedef struct {
+  uint8_t version, hash, chunk_log2, codec, seed, name_len, type_len;
+  uint64_t length;
+  uint32_t chunks, sent;
+  uint8_t root[32];
+  uint8_t name[255], type[158];
+} lizx_xfer_header;
+enum { LIZX_XFER_RESERVED = -1, LIZX_XFER_DATA = 0, LIZX_XFER_HEADER = 1, LIZX_XFER_MANIFEST = 2 };
+/* ceil(length / 2^chunk_log2), 0 past 16,383 chunks or for chunk_log2 outside 10..24 (xfer_chunk_count). */
+LIZ_API uint32_t lizx_xfer_chunk_count(uint64_t length, int chunk_log2);
+/* Bytes of chunk index (the file's own), 0 past the last (xfer_chunk_len). */
+LIZ_API uint64_t lizx_xfer_chunk_len(const lizx_xfer_header *h, uint32_t index);
+/* A chunk's fountain source blocks for sent bytes, ceil(sent / 472) (xfer_blocks). */
+LIZ_API uint32_t lizx_xfer_blocks(uint32_t sent);
+/* 0 where sent and seed can be chunk index's (xfer_sent_ok). */
+LIZ_API int lizx_xfer_sent_ok(const lizx_xfer_header *h, uint32_t index, uint32_t sent, int seed);
+/* Manifest blocks of a file of chunks chunks, 0 for fewer than 2 (xfer_manifest_blocks). */
+LIZ_API int lizx_xfer_manifest_blocks(uint32_t chunks);
+/* A block id's kind: LIZX_XFER_DATA, _HEADER, _MANIFEST or _RESERVED (xfer_id_kind, xfer.h:123-126). */
+LIZ_API int lizx_xfer_id_kind(uint32_t id);
+/* A header for a file of length bytes in chunks of 2^chunk_log2 (xfer_header_init). */
+LIZ_API int lizx_xfer_header_init(lizx_xfer_header *h, uint64_t length, int chunk_log2);
+/* The 472-byte payload from h, -1 where h breaks a rule (xfer_header_write). */
+LIZ_API int lizx_xfer_header_write(const lizx_xfer_header *h, uint8_t *payload);
+/* h from a 472-byte payload (xfer_header_parse). */
+LIZ_API int lizx_xfer_header_parse(const uint8_t *payload, lizx_xfer_header *h);
+/* Manifest block m of a file of chunks chunks whose root is root (xfer_manifest_write). */
+LIZ_API int lizx_xfer_manifest_write(const uint8_t *cvs, const uint32_t *sent, const uint8_t *seeds, uint32_t chunks,
+                                     const uint8_t *root, uint32_t m, uint8_t *payload);
+/* Manifest block m's entries into cvs, sent and seeds at 12 m; nothing written on -1 (xfer_manifest_parse). */
+LIZ_API int lizx_xfer_manifest_parse(const uint8_t *payload, const lizx_xfer_header *h, uint32_t m, uint8_t *cvs,
+                                     uint32_t *sent, uint8_t *seeds);
+/* 0 where the 2..16,383 chaining values combine to root (xfer_manifest_check). */
+LIZ_API int lizx_xfer_manifest_check(const uint8_t *cvs, uint32_t chunks, const uint8_t *root);
+
 /* ---- lizx_ldpc: the LDPC decoder alone, batched ----------------------------------------------------------------- */
 
 typedef struct lizx_ldpc lizx_ldpc;

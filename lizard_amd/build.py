@@ -15,7 +15,7 @@ OBJ = os.path.join(HERE, "_obj")
 OUT = os.path.join(HERE, "liblizard_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["format.cpp", "api.cpp", "batch.cpp", "register.cpp", "word.cpp", "find.cpp", "shake.cpp", "replay.cpp",
+SOURCES = ["format.cpp", "api.cpp", "batch.cpp", "register.cpp", "word.cpp", "find.cpp", "shake.cpp", "replay.cpp", "xfer_format.cpp",
            "ldpc.hip", "spectrum.hip", "refine.hip", "channel.hip", "finder.hip", "xfer.hip", "anchor.hip", "border.hip"]
 # -packed-fp32-ops: no packed FP32 instructions are generated (v_pk_*_f32).
 # -ffp-contract=off: every float multiply and add rounds on its own, as in every reference build
