@@ -125,8 +125,30 @@ def misc(rng):
                 pass
 
 
+def xfer_blocks(rng):
+    """Random and half-valid 472-byte payloads through the transfer's header and manifest readers (untrusted input)."""
+    import ctypes as C
+    lib = L.lib()
+    h = (C.c_uint8 * 512)()  # room for lizx_xfer_header (472 bytes)
+    # outputs for the most chunks a header can name (16,383), whatever the parsed header says
+    cvs, sent, seeds = np.zeros((16383, 32), np.uint8), np.zeros(16383, np.uint32), np.zeros(16383, np.uint8)
+    n = 0
+    for _ in range(300):
+        pay = rng.integers(0, 256, 472).astype(np.uint8)
+        if rng.random() < 0.5:
+            pay[0], pay[1], pay[7] = 3, 1, 0
+            pay[469:] = 0
+        lib.lizx_xfer_header_parse(pay.ctypes.data_as(C.c_void_p), h)
+        lib.lizx_xfer_manifest_parse(pay.ctypes.data_as(C.c_void_p), h, C.c_uint32(int(rng.integers(0, 2000))),
+                                     cvs.ctypes.data_as(C.c_void_p), sent.ctypes.data_as(C.c_void_p),
+                                     seeds.ctypes.data_as(C.c_void_p))
+        n += 1
+    return n
+
+
 def main():
     rng = np.random.default_rng(int(sys.argv[1]) if len(sys.argv) > 1 else 1)
+    xfer_blocks(rng)
     with tempfile.TemporaryDirectory() as tmp:
         r = replays(rng, tmp)
     w = words(rng)
