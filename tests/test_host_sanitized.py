@@ -1,6 +1,6 @@
 """The host code under AddressSanitizer and UndefinedBehaviorSanitizer: tests/sanitized_host_calls.py (replays that are
 truncated, bit-flipped or ill-formed, random soft words, random borders and anchors through the fills, the format
-arithmetic, encoder, CRC, test stream and BLAKE3) run in a child process on the library's host-sanitized build
+arithmetic, encoder, CRC, test stream, BLAKE3, and the transfer's header and manifest readers on random blocks) run in a child process on the library's host-sanitized build
 (python -m lizard_amd.build --sanitize; lizard_amd/_san/) with the sanitizers' runtime preloaded. Any report, or a
 child that does not finish, fails the test. The GPU code is not sanitized (no such build exists for it here)."""
 import glob
