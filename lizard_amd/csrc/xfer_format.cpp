@@ -51,8 +51,10 @@ uint32_t chunk_count(uint64_t length, int log2) {
 }
 
 uint64_t chunk_len(const lizx_xfer_header &h, uint32_t index) {
-  if (index >= h.chunks) return 0;
+  // a chunk size outside the format's has no chunks (and no shift past 63 bits)
+  if (index >= h.chunks || h.chunk_log2 < LOG2_MIN || h.chunk_log2 > LOG2_MAX) return 0;
   const uint64_t size = uint64_t{1} << h.chunk_log2, at = static_cast<uint64_t>(index) << h.chunk_log2;
+  if (at >= h.length) return 0;  // chunks beyond the length (an inconsistent header)
   const uint64_t left = h.length - at;
   return left < size ? left : size;
 }
