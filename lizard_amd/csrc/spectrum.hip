@@ -474,6 +474,18 @@ __global__ __launch_bounds__(256) void k_erase_fit(PlanDev P, Src src, double *p
 // written in two. Only the assignment changes, never a result.
 __device__ __forceinline__ int xcd_block(int bx, int nb) { return nb % 8 ? bx : (bx & 7) * (nb >> 3) + (bx >> 3); }
 
+// The same for a grid of (blocks, frames) whose blocks of one frame write into the same lines of that frame's output
+// (the column pass's coefficients in coefficient order): where the frames are a multiple of 8, workgroup linear id L
+// (on XCD L mod 8) takes frame (L mod 8) + 8 (L / 8 / nb) and block (L / 8) mod nb, so each frame's blocks all run on
+// one XCD and its lines fill in that XCD's L2. A bijection of (block, frame); only the assignment changes.
+__device__ __forceinline__ void xcd_frame(int &bx, int &f) {
+  const int nb = gridDim.x, nf = gridDim.y;
+  if (nf % 8) return;
+  const int l = blockIdx.x + blockIdx.y * nb, k = l >> 3;
+  f = (l & 7) + 8 * (k / nb);
+  bx = k % nb;
+}
+
 // k_rows: W waves a workgroup, two picture rows a wave packed as one complex sequence (2 W rows a workgroup)
 
 // Rows y and y + 1 of wave w: load with the anchors' samples erased, the detrend's row sums (s, s c(x), s q(x)) in
@@ -582,8 +594,10 @@ __global__ __launch_bounds__(64 * WV) void k_cols(PlanDev P, const float2 *X, fl
   using F = WaveFft<N, R1>;
   constexpr int M = F::M;
   extern __shared__ float2 zs[];  // WV waves x F::LDS
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, f = blockIdx.y, E = P.E;
-  const int u = blockIdx.x * WV + w;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, E = P.E;
+  int bx = blockIdx.x, f = blockIdx.y;
+  xcd_frame(bx, f);
+  const int u = bx * WV + w;
   if (u >= E) return;  // the whole wave; nothing below waits on the workgroup
   const float2 *col = X + (static_cast<size_t>(f) * E + u) * N;
   float2 v[M];
