@@ -163,6 +163,33 @@ def test_grid_batch_of_mixed_sizes(decoder_gpu):
     b.close()
 
 
+def test_grid_batch_of_sixteen_one_format(decoder_gpu):
+    """Sixteen grids of one format in one batch: the column pass then runs each frame's columns on one XCD (frames a
+    multiple of 8), and every frame's soft values, estimates and rms still equal its own single-frame finish's, graph
+    and frame by frame."""
+    rng = np.random.default_rng(6)
+    versions, spans = [20] * 16, [256] * 16
+    grids, keep, items = [], [], []
+    for v in versions:
+        pic = sim.picture(v, sim.random_blocks(v, rng)).numpy()
+        g = (pic + rng.standard_normal(pic.shape) * 0.03).astype(np.float32)
+        grids.append(g)
+        d = L.DeviceArray.of(g)
+        keep.append(d)
+        items.append(dict(grid=d))
+    frames = L.make_frames(items)
+    b = L.Batch().plan(versions, spans, source=L.SRC_GRID, stages=L.STAGE_SOFT)
+    for mode in (L.RUN_GRAPH, L.RUN_FRAMES):
+        b.run(frames, mode)
+        b.sync()
+        for i in range(16):
+            want = decoder_gpu.finish_grid(20, 256, grids[i])
+            got = b.soft(i, 20)
+            for k in ("llr", "sub_est", "sub_rms"):
+                assert np.array_equal(got[k], want[k]), (mode, i, k)
+    b.close()
+
+
 def test_batch_argument_errors():
     b = L.Batch()
     with pytest.raises(L.LizError):
